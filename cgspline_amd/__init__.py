@@ -1,0 +1,241 @@
+"""cgspline_amd — MI355X-native ray–bicubic-Bézier-patch path of cgspline.
+
+The product is ``libcgs.so`` (hand-written gfx950 HIP kernels behind the C ABI
+in ``include/cgs.h``); this package is its Python binding plus the synthetic
+scene generators.  There is no CPU execution path: on a machine without a HIP
+device :class:`Context` raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _abi, scenes
+from ._abi import CgsError, EMPTY, PATCH_OVERFLOW, check, lib, ptr
+
+__all__ = ["Context", "DeviceArray", "DeviceScene", "CgsError", "EMPTY", "PATCH_OVERFLOW", "scenes", "lib"]
+
+
+class DeviceArray:
+    """A device allocation owned by a :class:`Context` (cgs_malloc)."""
+
+    def __init__(self, ctx: "Context", nbytes: int):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        check(lib().cgs_malloc(ctx.h, max(1, self.nbytes), C.byref(p)), "cgs_malloc")
+        self.dptr = p.value
+
+    @classmethod
+    def from_host(cls, ctx: "Context", a: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = cls(ctx, a.nbytes)
+        if a.nbytes:
+            check(lib().cgs_memcpy_h2d(ctx.h, d.dptr, a.ctypes.data, a.nbytes, None), "cgs_memcpy_h2d")
+        return d
+
+    def to_host(self, dtype, shape) -> np.ndarray:
+        out = np.empty(shape, dtype)
+        if out.nbytes:
+            assert out.nbytes <= self.nbytes
+            check(lib().cgs_memcpy_d2h(self.ctx.h, out.ctypes.data, self.dptr, out.nbytes, None), "cgs_memcpy_d2h")
+        return out
+
+    def free(self):
+        if self.dptr is not None and self.ctx.h is not None:
+            lib().cgs_free(self.ctx.h, self.dptr)
+        self.dptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Context:
+    """One HIP device (cgs_open)."""
+
+    def __init__(self, device: int = 0):
+        p = C.c_void_p()
+        check(lib().cgs_open(device, C.byref(p)), "cgs_open")
+        self.h = p.value
+        self.device = device
+
+    def close(self):
+        if self.h is not None:
+            lib().cgs_close(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # --- per-patch entry points -------------------------------------------------
+    def bezier_clip(self, rays: np.ndarray, patches: np.ndarray, tol: float = 2.0 ** -12) -> np.ndarray:
+        """doBezierClipping of ray r against patch r; returns (n, 4) records
+        (t, u, v as float32; patch word as uint32 view in column 3)."""
+        return self._per_patch(lib().cgs_bezier_clip, rays, patches, tol)
+
+    def subdivision(self, rays: np.ndarray, patches: np.ndarray, threshold: float = 2.0 ** -4) -> np.ndarray:
+        return self._per_patch(lib().cgs_subdivision, rays, patches, threshold)
+
+    def _per_patch(self, fn, rays, patches, param):
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        patches = np.ascontiguousarray(patches, np.float32).reshape(-1, 64)
+        n = rays.shape[0]
+        assert patches.shape[0] == n
+        dr, dp = DeviceArray.from_host(self, rays), DeviceArray.from_host(self, patches)
+        dh = DeviceArray(self, n * 16)
+        check(fn(self.h, dr.dptr, dp.dptr, n, param, dh.dptr, None))
+        check(lib().cgs_stream_sync(self.h, None))
+        return dh.to_host(np.float32, (n, 4))
+
+    # --- BVH / hash -------------------------------------------------------------
+    @staticmethod
+    def patch_elements(vertices: np.ndarray, indices: np.ndarray) -> np.ndarray:
+        v = np.ascontiguousarray(vertices, np.float32)
+        i = np.ascontiguousarray(indices, np.uint32)
+        out = np.zeros((i.shape[0], 10), np.float32)
+        check(lib().cgs_patch_elements(v.ctypes.data, i.ctypes.data, i.shape[0], out.ctypes.data))
+        return out
+
+    @staticmethod
+    def bvh_build(elements10: np.ndarray, elements_per_node: int = 1):
+        """Host BVH build; returns (nodes (N,16) u32 words, keys, indices)."""
+        e = np.ascontiguousarray(elements10, np.float32)
+        n_, k_, i_ = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        nc, ic = C.c_uint32(), C.c_uint32()
+        check(lib().cgs_bvh_build(e.ctypes.data, e.shape[0], elements_per_node, C.byref(n_), C.byref(k_),
+                                  C.byref(nc), C.byref(i_), C.byref(ic)), "cgs_bvh_build")
+        try:
+            nodes = np.ctypeslib.as_array((C.c_uint32 * (16 * nc.value)).from_address(n_.value)).reshape(-1, 16).copy()
+            keys = np.ctypeslib.as_array((C.c_uint32 * nc.value).from_address(k_.value)).copy()
+            idx = np.ctypeslib.as_array((C.c_uint32 * ic.value).from_address(i_.value)).copy()
+        finally:
+            for p in (n_, k_, i_):
+                lib().cgs_free_host(p)
+        return nodes, keys, idx
+
+    def phf_build(self, keys: np.ndarray, seed: int = 42):
+        """GPU perfect-hash build; returns (status, params dict, displacement)."""
+        keys = np.ascontiguousarray(keys, np.uint32)
+        n = keys.shape[0]
+        dk = DeviceArray.from_host(self, keys)
+        m = lib().cgs_phf_displacement_size(n)
+        dd = DeviceArray(self, 4 * m)
+        h = _abi.PHF()
+        st = lib().cgs_phf_build(self.h, dk.dptr, n, seed, dd.dptr, C.byref(h), None)
+        check(lib().cgs_stream_sync(self.h, None))
+        params = dict(A=h.A, B=h.B, mask=h.mask, table_size=h.table_size, attempts=h.attempts)
+        return st, params, dd.to_host(np.uint32, (m,))
+
+    def phf_hash(self, params: dict, displacement: np.ndarray, keys: np.ndarray) -> np.ndarray:
+        keys = np.ascontiguousarray(keys, np.uint32)
+        h = _abi.PHF(params["A"], params["B"], params["mask"], params["table_size"], 0, displacement.shape[0])
+        dk, dd = DeviceArray.from_host(self, keys), DeviceArray.from_host(self, displacement.astype(np.uint32))
+        out = DeviceArray(self, 4 * keys.shape[0])
+        check(lib().cgs_phf_hash(self.h, C.byref(h), dd.dptr, dk.dptr, keys.shape[0], out.dptr, None))
+        check(lib().cgs_stream_sync(self.h, None))
+        return out.to_host(np.uint32, keys.shape)
+
+    # --- scenes -------------------------------------------------------------------
+    def scene(self, scene: "scenes.Scene") -> "DeviceScene":
+        return DeviceScene(self, scene)
+
+    def sync(self, stream=None):
+        check(lib().cgs_stream_sync(self.h, stream), "cgs_stream_sync")
+
+
+class DeviceScene:
+    """A scene resident in HBM (cgs_scene_create)."""
+
+    def __init__(self, ctx: Context, scene: "scenes.Scene"):
+        self.ctx = ctx
+        self.scene = scene
+        keep = [np.ascontiguousarray(a) for a in (
+            scene.vertices, scene.indices, scene.surf_vbase, scene.surf_pbase, scene.obj_surface,
+            scene.obj_local2world, scene.obj_world2local, scene.obj_material, scene.lights)]
+        d = _abi.SceneDesc(keep[0].ctypes.data, keep[1].ctypes.data, keep[2].ctypes.data, keep[3].ctypes.data,
+                           scene.nsurf, keep[4].ctypes.data, keep[5].ctypes.data, keep[6].ctypes.data,
+                           keep[7].ctypes.data, scene.nobj, keep[8].ctypes.data, keep[8].shape[0],
+                           (C.c_float * 3)(*scene.background), (C.c_float * 3)(*scene.ambient), scene.hash_seed)
+        p = C.c_void_p()
+        check(lib().cgs_scene_create(ctx.h, C.byref(d), C.byref(p)), "cgs_scene_create")
+        self.h = p.value
+
+    def destroy(self):
+        if self.h is not None and self.ctx.h is not None:
+            lib().cgs_scene_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+    def bvh(self, surface: int) -> dict:
+        nc, ic, h = C.c_uint32(), C.c_uint32(), _abi.PHF()
+        check(lib().cgs_scene_bvh_info(self.h, surface, C.byref(nc), C.byref(ic), C.byref(h)))
+        nodes = np.zeros((nc.value, 16), np.uint32)
+        keys = np.zeros(nc.value, np.uint32)
+        idx = np.zeros(ic.value, np.uint32)
+        D = np.zeros(h.displacement_size, np.uint32)
+        T = np.zeros(h.table_size, np.uint32)
+        check(lib().cgs_scene_bvh_copy(self.h, surface, nodes.ctypes.data, keys.ctypes.data, idx.ctypes.data,
+                                       D.ctypes.data, T.ctypes.data))
+        return dict(nodes=nodes, keys=keys, indices=idx, A=h.A, B=h.B, mask=h.mask, table_size=h.table_size,
+                    attempts=h.attempts, displacement=D, table=T)
+
+    # device-pointer entry points (for resident buffers)
+    def trace_closest_dev(self, rays, n, hits, objects=None, stream=None):
+        check(lib().cgs_trace_closest(self.ctx.h, self.h, ptr(rays), n, ptr(hits), ptr(objects), stream),
+              "cgs_trace_closest")
+
+    def trace_any_dev(self, rays, n, occluded, stream=None):
+        check(lib().cgs_trace_any(self.ctx.h, self.h, ptr(rays), n, ptr(occluded), stream), "cgs_trace_any")
+
+    def render_dev(self, camera: "scenes.Camera", frame, color=None, counts=None, eps=1e-4, depth=0, stream=None):
+        cam = _abi.Camera((C.c_float * 16)(*camera.to_world), (C.c_float * 3)(*camera.position), camera.fovy,
+                          int(camera.flip_y))
+        op = _abi.RenderOptions(eps, depth)
+        return lib().cgs_render(self.ctx.h, self.h, C.byref(cam), C.byref(op), camera.width, camera.height,
+                                ptr(frame), ptr(color), ptr(counts), stream)
+
+    # host convenience wrappers
+    def trace_closest(self, rays: np.ndarray):
+        """Closest hits of host rays (n, 8): returns (t,u,v (n,3), patch (n,), object (n,))."""
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        n = rays.shape[0]
+        dr = DeviceArray.from_host(self.ctx, rays)
+        dh = DeviceArray(self.ctx, 16 * n)
+        do = DeviceArray(self.ctx, 4 * n)
+        self.trace_closest_dev(dr, n, dh, do)
+        self.ctx.sync()
+        h = dh.to_host(np.float32, (n, 4))
+        return h[:, :3].copy(), h.view(np.uint32)[:, 3].copy(), do.to_host(np.int32, (n,))
+
+    def trace_any(self, rays: np.ndarray) -> np.ndarray:
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        n = rays.shape[0]
+        dr = DeviceArray.from_host(self.ctx, rays)
+        do = DeviceArray(self.ctx, 4 * n)
+        self.trace_any_dev(dr, n, do)
+        self.ctx.sync()
+        return do.to_host(np.uint32, (n,))
+
+    def render(self, camera: "scenes.Camera", eps=1e-4, depth=0):
+        """Returns (color (h,w,3) f32, frame (h,w) u32, counts dict)."""
+        w, h = camera.width, camera.height
+        df = DeviceArray(self.ctx, 4 * w * h)
+        dc = DeviceArray(self.ctx, 12 * w * h)
+        dn = DeviceArray.from_host(self.ctx, np.zeros(4, np.uint64))
+        check(self.render_dev(camera, df, dc, dn, eps, depth), "cgs_render")
+        self.ctx.sync()
+        cnt = dn.to_host(np.uint64, (4,))
+        return (dc.to_host(np.float32, (h, w, 3)), df.to_host(np.uint32, (h, w)),
+                dict(primary=int(cnt[0]), shadow=int(cnt[1]), secondary=int(cnt[2]), overflow=int(cnt[3])))

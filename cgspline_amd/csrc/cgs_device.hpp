@@ -1,0 +1,675 @@
+// SPDX-License-Identifier: MIT
+//
+// Device building blocks for gfx950 (wave64): ray–bicubic-patch Bézier
+// clipping with the reference's host semantics, the hashed stackless BVH
+// traversal and the per-object ray transform.
+//
+// Numerics.  The library is compiled with -ffp-contract=off and IEEE
+// (correctly rounded) f32 division and square root, and every expression keeps
+// the reference's evaluation order, so an FP32 result equals the one the
+// reference's host path computes for the same inputs bit for bit.  Keep that
+// in mind before "simplifying" an expression here: a*b+c*d written in another
+// order, a dropped "+ 0.0f" or a reciprocal multiply in place of a division
+// changes results.
+//
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/cgs.h"
+
+#define CGS_DEV __device__ __forceinline__
+
+namespace cgs
+{
+
+constexpr uint32_t EMPTY = 0xFFFFFFFFu;
+constexpr float FLT_EPS = 1.1920928955078125e-07f; // 2^-23
+
+struct V2 { float x, y; };
+struct V3 { float x, y, z; };
+
+// glibc x86-64 fminf/fmaxf (the reference's host fmin/fmax): a NaN operand
+// yields the other operand, ties return the second operand.
+CGS_DEV float fmin_(float a, float b) { return (a < b || b != b) ? a : b; }
+CGS_DEV float fmax_(float a, float b) { return (a > b || b != b) ? a : b; }
+
+CGS_DEV V3 add3(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+CGS_DEV V3 sub3(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+CGS_DEV V3 mul3(float s, V3 v) { return {s * v.x, s * v.y, s * v.z}; }
+CGS_DEV V3 mulv3(V3 a, V3 b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+CGS_DEV V3 neg3(V3 a) { return {-a.x, -a.y, -a.z}; }
+CGS_DEV float dot3(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+CGS_DEV V3 cross3(V3 a, V3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+CGS_DEV float len3(V3 v) { return __builtin_sqrtf(v.x * v.x + v.y * v.y + v.z * v.z); }
+CGS_DEV float max3(V3 v) { return fmax_(v.x, fmax_(v.y, v.z)); }
+
+// Vector::versor (Ds): scale by the reciprocal length unless the length is
+// within machine epsilon of zero.
+CGS_DEV V3 versor3(V3 v)
+{
+    float len = len3(v);
+    if (!(__builtin_fabsf(len) <= FLT_EPS))
+    {
+        float inv = 1.0f / len;
+        v = {v.x * inv, v.y * inv, v.z * inv};
+    }
+    return v;
+}
+CGS_DEV V2 normalize2(V2 v)
+{
+    float len = __builtin_sqrtf(v.x * v.x + v.y * v.y);
+    if (!(__builtin_fabsf(len) <= FLT_EPS))
+    {
+        float inv = 1.0f / len;
+        v = {v.x * inv, v.y * inv};
+    }
+    return v;
+}
+
+// 4×4 column-major matrix times (x, y, z, w) (AlignedTypes.h:185-188).
+struct M4
+{
+    float4 c[4];
+};
+CGS_DEV float4 m4mul(const M4& m, float4 v)
+{
+    float4 r;
+    r.x = ((v.x * m.c[0].x + v.y * m.c[1].x) + v.z * m.c[2].x) + v.w * m.c[3].x;
+    r.y = ((v.x * m.c[0].y + v.y * m.c[1].y) + v.z * m.c[2].y) + v.w * m.c[3].y;
+    r.z = ((v.x * m.c[0].z + v.y * m.c[1].z) + v.z * m.c[2].z) + v.w * m.c[3].z;
+    r.w = ((v.x * m.c[0].w + v.y * m.c[1].w) + v.z * m.c[2].w) + v.w * m.c[3].w;
+    return r;
+}
+CGS_DEV M4 loadM4(const float4* __restrict__ p)
+{
+    M4 m;
+    m.c[0] = p[0]; m.c[1] = p[1]; m.c[2] = p[2]; m.c[3] = p[3];
+    return m;
+}
+// mat3(M) * v
+CGS_DEV V3 m3mul(const M4& m, V3 v)
+{
+    return {(v.x * m.c[0].x + v.y * m.c[1].x) + v.z * m.c[2].x,
+            (v.x * m.c[0].y + v.y * m.c[1].y) + v.z * m.c[2].y,
+            (v.x * m.c[0].z + v.y * m.c[1].z) + v.z * m.c[2].z};
+}
+// mat3(M).transposed() * v: columns of the transposed are the rows of M.
+CGS_DEV V3 m3tmul(const M4& m, V3 v)
+{
+    return {(v.x * m.c[0].x + v.y * m.c[0].y) + v.z * m.c[0].z,
+            (v.x * m.c[1].x + v.y * m.c[1].y) + v.z * m.c[1].z,
+            (v.x * m.c[2].x + v.y * m.c[2].y) + v.z * m.c[2].z};
+}
+CGS_DEV V3 project4(float4 p) { return {p.x / p.w, p.y / p.w, p.z / p.w}; }
+
+// ---------------------------------------------------------------------------
+// Bicubic evaluation (SplineMat.h:22-44, 101-187), power-basis matrix form
+// with every term of the constant matrices kept (so signed zeros and the
+// rounding of each partial sum match the reference).
+// ---------------------------------------------------------------------------
+
+CGS_DEV float4 basisMul(float4 v) // CUBIC_BASE * v
+{
+    float4 r;
+    r.x = ((v.x * 1.0f + v.y * -3.0f) + v.z * 3.0f) + v.w * -1.0f;
+    r.y = ((v.x * 0.0f + v.y * 3.0f) + v.z * -6.0f) + v.w * 3.0f;
+    r.z = ((v.x * 0.0f + v.y * 0.0f) + v.z * 3.0f) + v.w * -3.0f;
+    r.w = ((v.x * 0.0f + v.y * 0.0f) + v.z * 0.0f) + v.w * 1.0f;
+    return r;
+}
+CGS_DEV float4 dbasisMul(float4 v) // CUBIC_DERIVATIVE_BASE * v
+{
+    float4 r;
+    r.x = ((v.x * -3.0f + v.y * 6.0f) + v.z * -3.0f) + v.w * 0.0f;
+    r.y = ((v.x * 3.0f + v.y * -12.0f) + v.z * 9.0f) + v.w * 0.0f;
+    r.z = ((v.x * 0.0f + v.y * 6.0f) + v.z * -9.0f) + v.w * 0.0f;
+    r.w = ((v.x * 0.0f + v.y * 0.0f) + v.z * 3.0f) + v.w * 0.0f;
+    return r;
+}
+CGS_DEV float4 evalVec(float t) { float t2 = t * t; return {1.0f, t, t2, t2 * t}; }
+CGS_DEV float4 evalVecDt(float t) { return {1.0f, t, t * t, 0.0f}; }
+
+// x.x*a + x.y*b + x.z*c + x.w*d over float4 points (mat4(a,b,c,d) * x)
+CGS_DEV float4 comb4(float4 a, float4 b, float4 c, float4 d, float4 x)
+{
+    float4 r;
+    r.x = ((x.x * a.x + x.y * b.x) + x.z * c.x) + x.w * d.x;
+    r.y = ((x.x * a.y + x.y * b.y) + x.z * c.y) + x.w * d.y;
+    r.z = ((x.x * a.z + x.y * b.z) + x.z * c.z) + x.w * d.z;
+    r.w = ((x.x * a.w + x.y * b.w) + x.z * c.w) + x.w * d.w;
+    return r;
+}
+
+// mat::interpolate (SplineMat.h:125-140)
+CGS_DEV float4 interpolateMat(const float4* __restrict__ S, float u, float v)
+{
+    float4 x = basisMul(evalVec(u));
+    float4 y = basisMul(evalVec(v));
+    float4 a0 = comb4(S[0], S[1], S[2], S[3], x);
+    float4 a1 = comb4(S[4], S[5], S[6], S[7], x);
+    float4 a2 = comb4(S[8], S[9], S[10], S[11], x);
+    float4 a3 = comb4(S[12], S[13], S[14], S[15], x);
+    return comb4(a0, a1, a2, a3, y);
+}
+// mat::derivativeU / derivativeV / normal (SplineMat.h:142-187)
+CGS_DEV V3 normalMat(const float4* __restrict__ S, float u, float v)
+{
+    float4 x = dbasisMul(evalVecDt(u));
+    float4 y = basisMul(evalVec(v));
+    float4 a0 = comb4(S[0], S[1], S[2], S[3], x);
+    float4 a1 = comb4(S[4], S[5], S[6], S[7], x);
+    float4 a2 = comb4(S[8], S[9], S[10], S[11], x);
+    float4 a3 = comb4(S[12], S[13], S[14], S[15], x);
+    float4 dU = comb4(a0, a1, a2, a3, y);
+    float4 xb = basisMul(evalVec(u));
+    float4 yd = dbasisMul(evalVecDt(v));
+    float4 b0 = comb4(S[0], S[4], S[8], S[12], yd);
+    float4 b1 = comb4(S[1], S[5], S[9], S[13], yd);
+    float4 b2 = comb4(S[2], S[6], S[10], S[14], yd);
+    float4 b3 = comb4(S[3], S[7], S[11], S[15], yd);
+    float4 dV = comb4(b0, b1, b2, b3, xb);
+    return cross3({dU.x, dU.y, dU.z}, {dV.x, dV.y, dV.z});
+}
+
+// ---------------------------------------------------------------------------
+// Curve slicing (Spline.h:324-364): split1 at u, then split0 at (v-u)/(1-u).
+// Both coordinates of a 2D point are sliced with the same scalars.
+// ---------------------------------------------------------------------------
+CGS_DEV void slice1(float& c0, float& c1, float& c2, float& c3, float u, float iu, float vv, float ivv)
+{
+    float p5, p6, p8;
+    p6 = u * c1 + iu * c0;
+    p5 = u * c2 + iu * c1;
+    c2 = u * c3 + iu * c2;
+    p8 = u * p5 + iu * p6;
+    c1 = u * c2 + iu * p5;
+    c0 = u * c1 + iu * p8;
+    p6 = ivv * c2 + vv * c3;
+    p5 = ivv * c1 + vv * c2;
+    c1 = ivv * c0 + vv * c1;
+    p8 = ivv * p5 + vv * p6;
+    c2 = ivv * c1 + vv * p5;
+    c3 = ivv * c2 + vv * p8;
+}
+
+// subpatchU (rows) / subpatchV (columns) of a 2D patch held in registers.
+CGS_DEV void subpatch2(V2 (&p)[16], bool vside, float a, float b)
+{
+    float iu = 1.0f - a;
+    float vv = (b - a) / iu;
+    float ivv = 1.0f - vv;
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+    {
+        // row k (U side): [4k+0..3]; column k (V side): [k, k+4, k+8, k+12]
+        const int i0 = vside ? k : 4 * k;
+        const int st = vside ? 4 : 1;
+        slice1(p[i0].x, p[i0 + st].x, p[i0 + 2 * st].x, p[i0 + 3 * st].x, a, iu, vv, ivv);
+        slice1(p[i0].y, p[i0 + st].y, p[i0 + 2 * st].y, p[i0 + 3 * st].y, a, iu, vv, ivv);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Hull helpers (Spline_BezierClipping.cu:127-165, Spline.h:709-737)
+// ---------------------------------------------------------------------------
+CGS_DEV bool xAxisIntersection(float& lo, float& hi, V2 A, V2 B)
+{
+    float q = A.y - B.y;
+    if (__builtin_fabsf(q) > FLT_EPS)
+    {
+        float t = A.y / q;
+        if (0.0f <= t && t <= 1.0f)
+        {
+            float s = (1.0f - t) * A.x + t * B.x;
+            lo = s;
+            hi = s;
+            return true;
+        }
+    }
+    else if (__builtin_fabsf(A.y) <= FLT_EPS)
+    {
+        lo = fmin_(A.x, B.x);
+        hi = fmax_(A.x, B.x);
+        return true;
+    }
+    return false;
+}
+
+CGS_DEV float isCCW(V2 P, V2 Q) { return P.x * Q.y - P.y * Q.x; }
+
+CGS_DEV bool triangleOrigin(V2& coord, V2 P, V2 A, V2 B)
+{
+    V2 p {A.x - P.x, A.y - P.y};
+    V2 q {B.x - P.x, B.y - P.y};
+    float det = p.x * q.y - q.x * p.y;
+    if (__builtin_fabsf(det) < 0x1p-80f) return false;
+    det = 1.0f / det;
+    // m0 = (q.y, -q.x), m1 = (-p.y, p.x); dot with -P
+    float u = det * (q.y * -P.x + -q.x * -P.y);
+    if (u < 0.0f || u > 1.0f) return false;
+    float v = det * (-p.y * -P.x + p.x * -P.y);
+    if (v < 0.0f || u + v > 1.0f) return false;
+    coord = {u, v};
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// Ray frame for clipping: the patch is projected on two planes containing the
+// ray (Spline_BezierClipping.cu:57-83).  `dir` is the Ray3f direction.
+// ---------------------------------------------------------------------------
+struct ClipRay
+{
+    V3 o, d, ax0, ax1;
+};
+CGS_DEV ClipRay makeClipRay(V3 o, V3 dir)
+{
+    ClipRay r;
+    r.o = o;
+    r.d = dir;
+    V3 e {dir.x + 0.0f, dir.y + 0.0f, dir.z + 8.0f};
+    r.ax0 = versor3(cross3(dir, e));
+    r.ax1 = versor3(cross3(dir, r.ax0));
+    return r;
+}
+
+// Region stack in private (scratch) memory: per entry the 16 working points
+// and the region (min, max, size, side).  Exactly the state the reference's
+// host clipper keeps per std::vector entry.
+struct ClipStack
+{
+    V2 pts[CGS_CLIP_STACK][16];
+    float4 reg[CGS_CLIP_STACK][2];
+};
+
+// doBezierClipping with host semantics (Spline_BezierClipping.cu:46-114,
+// 174-445).  S: the 16 control points (global).  best/bu/bv: current hit,
+// updated by accepted candidates (t < best && V·d > 0).  With ANY the test
+// stops at the first accepted candidate (the caller only needs "found").
+// Returns found; sets overflow when the region stack is exhausted.
+template<bool ANY>
+CGS_DEV bool clipPatch(const float4* __restrict__ S, const ClipRay& R, float tol,
+                       float& best, float& bu, float& bv, ClipStack& stk, bool& overflow)
+{
+    V2 p[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++)
+    {
+        float4 q = S[i];
+        V3 D {q.x - R.o.x, q.y - R.o.y, q.z - R.o.z};
+        p[i] = {dot3(R.ax0, D), dot3(R.ax1, D)};
+    }
+    V2 mn {0.0f, 0.0f}, mx {1.0f, 1.0f}, sz {1.0f, 1.0f};
+    bool vside = false;
+    int sp = 0; // entries stored below the current one
+    bool found = false;
+
+    auto onHit = [&](float eu, float ev) -> bool
+    {
+        float4 X = interpolateMat(S, eu, ev);
+        V3 V = sub3(project4(X), R.o);
+        float t = len3(V);
+        if (t < best && dot3(V, R.d) > 0.0f)
+        {
+            best = t;
+            bu = eu;
+            bv = ev;
+            return true;
+        }
+        return false;
+    };
+
+    while (true)
+    {
+        bool pop = false;
+        V2 t0 {p[3].x - p[12].x, p[3].y - p[12].y};
+        V2 t1 {p[15].x - p[0].x, p[15].y - p[0].y};
+        float d0 = __builtin_sqrtf(t0.x * t0.x + t0.y * t0.y);
+        float d1 = __builtin_sqrtf(t1.x * t1.x + t1.y * t1.y);
+        if (d0 < tol && d1 < tol)
+        {
+            // approximateIntersection (Spline.h:739-765)
+            V2 c;
+            if (triangleOrigin(c, p[0], p[3], p[12]))
+                found |= onHit(mn.x + c.x * sz.x, mn.y + c.y * sz.y);
+            if (triangleOrigin(c, p[15], p[12], p[3]))
+                found |= onHit(mx.x - c.x * sz.x, mx.y - c.y * sz.y);
+            if (ANY && found) break;
+            pop = true;
+        }
+        else
+        {
+            V2 L {p[15].x - p[0].x, p[15].y - p[0].y};
+            V2 Dv {p[12].x - p[3].x, p[12].y - p[3].y};
+            if (!vside) L = {L.x + Dv.x, L.y + Dv.y};
+            else L = {L.x - Dv.x, L.y - Dv.y};
+            L = normalize2(L);
+            const V2 N {-L.y, L.x};
+            float d[16];
+#pragma unroll
+            for (int i = 0; i < 16; i++) d[i] = N.x * p[i].x + N.y * p[i].y;
+            float ytop[4], ybot[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+            {
+                float y0 = vside ? d[4 * i] : d[i];
+                ybot[i] = y0;
+                ytop[i] = y0;
+#pragma unroll
+                for (int k = 1; k < 4; k++)
+                {
+                    float y = vside ? d[4 * i + k] : d[4 * k + i];
+                    ytop[i] = fmax_(ytop[i], y);
+                    ybot[i] = fmin_(ybot[i], y);
+                }
+            }
+            float lower = 1.0f, upper = 0.0f;
+            const float o1 = 1.0f / 3.0f, o2 = 2.0f / 3.0f;
+            const V2 hull[7] {{o2, ybot[2]}, {1.0f, ybot[3]}, {1.0f, ytop[3]}, {o2, ytop[2]},
+                              {o1, ytop[1]}, {0.0f, ytop[0]}, {0.0f, ybot[0]}};
+            V2 P1 {0.0f, ybot[0]};
+            V2 P2 {o1, ybot[1]};
+#pragma unroll
+            for (int h = 0; h < 7; h++)
+            {
+                const V2 P3 = hull[h];
+                if (isCCW({P2.x - P1.x, P2.y - P1.y}, {P3.x - P1.x, P3.y - P1.y}) >= 0.0f)
+                {
+                    float lo, hi;
+                    if (xAxisIntersection(lo, hi, P1, P2))
+                    {
+                        lower = fmin_(lower, lo);
+                        upper = fmax_(upper, hi);
+                    }
+                    P1 = P2;
+                }
+                P2 = P3;
+            }
+            {
+                float lo, hi;
+                if (xAxisIntersection(lo, hi, P1, P2))
+                {
+                    lower = fmin_(lower, lo);
+                    upper = fmax_(upper, hi);
+                }
+            }
+            lower = lower * 0x0FFp-8f;
+            upper = upper + (1.0f - upper) * 0x001p-8f;
+            float delta = upper - lower;
+            if (delta < 0.0f)
+            {
+                pop = true;
+            }
+            else if (delta > 0.8f)
+            {
+                // split in half; keep the upper half, push the lower half
+                if (sp >= CGS_CLIP_STACK)
+                {
+                    overflow = true;
+                    break;
+                }
+                V2 q[16];
+#pragma unroll
+                for (int i = 0; i < 16; i++) q[i] = p[i];
+                subpatch2(q, vside, 0.0f, 0.5f);
+                subpatch2(p, vside, 0.5f, 1.0f);
+                // Spline_BezierClipping.cu:373-409: the lower half keeps the
+                // old minimum and ends at the midpoint; the upper half (kept
+                // in registers, processed first) starts at the midpoint.
+                const V2 omn = mn;
+                V2 emx;
+                if (!vside)
+                {
+                    float hs = 0.5f * sz.x;
+                    float hx = mn.x + hs;
+                    emx = {hx, mx.y};
+                    mn = {hx, mn.y};
+                    sz = {hs, sz.y};
+                }
+                else
+                {
+                    float hs = 0.5f * sz.y;
+                    float hy = mn.y + hs;
+                    emx = {mx.x, hy};
+                    mn = {mn.x, hy};
+                    sz = {sz.x, hs};
+                }
+                vside = !vside;
+#pragma unroll
+                for (int i = 0; i < 16; i++) stk.pts[sp][i] = q[i];
+                stk.reg[sp][0] = {omn.x, omn.y, emx.x, emx.y};
+                stk.reg[sp][1] = {sz.x, sz.y, vside ? 1.0f : 0.0f, 0.0f};
+                sp++;
+                continue;
+            }
+            else
+            {
+                if (!vside)
+                {
+                    float u0 = mn.x + sz.x * lower;
+                    float u1 = mx.x - sz.x * (1.0f - upper);
+                    subpatch2(p, false, lower, upper);
+                    mn.x = u0;
+                    mx.x = u1;
+                    sz.x = u1 - u0;
+                }
+                else
+                {
+                    float v0 = mn.y + sz.y * lower;
+                    float v1 = mx.y - sz.y * (1.0f - upper);
+                    subpatch2(p, true, lower, upper);
+                    mn.y = v0;
+                    mx.y = v1;
+                    sz.y = v1 - v0;
+                }
+                vside = !vside;
+                continue;
+            }
+        }
+        if (pop)
+        {
+            if (sp == 0) break;
+            sp--;
+#pragma unroll
+            for (int i = 0; i < 16; i++) p[i] = stk.pts[sp][i];
+            float4 r0 = stk.reg[sp][0], r1 = stk.reg[sp][1];
+            mn = {r0.x, r0.y};
+            mx = {r0.z, r0.w};
+            sz = {r1.x, r1.y};
+            vside = r1.z != 0.0f;
+        }
+    }
+    return found;
+}
+
+// ---------------------------------------------------------------------------
+// BVH traversal (rt/BVH.h:147-350)
+// ---------------------------------------------------------------------------
+
+// Exact 32-bit remainder by an invariant divisor (Lemire, Kaser & Kurz,
+// "Faster remainder by direct computation", 2019): M = floor((2^64-1)/d) + 1,
+// x mod d = hi64(lo64(M·x) · d).  Exact for every 32-bit x and d >= 1.
+struct FastMod
+{
+    uint64_t M;
+    uint32_t d;
+};
+CGS_DEV uint32_t fastmod(uint32_t x, const FastMod& f)
+{
+    uint64_t low = f.M * (uint64_t)x;
+    return (uint32_t)__umul64hi(low, (uint64_t)f.d);
+}
+
+// One BVH with its perfect hash, as laid out in device memory.  Node and
+// table entries are absolute offsets into the scene's node array.
+struct BVHRef
+{
+    const float4* __restrict__ nodes;    // 4 float4 (64 B) per node
+    const uint32_t* __restrict__ indices;
+    const uint32_t* __restrict__ table;
+    const uint32_t* __restrict__ displ;
+    uint32_t root;                       // absolute node offset of the root
+    uint32_t A, B, mask;
+    FastMod T;
+    uint32_t nodeCount;                  // size of `nodes`, bounds table lookups
+};
+
+struct NodeBoxes
+{
+    V3 lmin, lmax, rmin, rmax;
+    uint32_t left, right;
+};
+CGS_DEV NodeBoxes loadNode(const float4* __restrict__ nodes, uint32_t i)
+{
+    float4 a = nodes[4 * (size_t)i], b = nodes[4 * (size_t)i + 1], c = nodes[4 * (size_t)i + 2],
+           d = nodes[4 * (size_t)i + 3];
+    NodeBoxes n;
+    n.lmin = {a.x, a.y, a.z};
+    n.lmax = {a.w, b.x, b.y};
+    n.rmin = {b.z, b.w, c.x};
+    n.rmax = {c.y, c.z, c.w};
+    n.left = __float_as_uint(d.x);
+    n.right = __float_as_uint(d.y);
+    return n;
+}
+
+// _node_intersect (rt/BVH.h:147-176)
+CGS_DEV bool nodeIntersect(float& tMin, V3 bmin, V3 bmax, V3 o, V3 Dinv)
+{
+    float tmin = -__builtin_inff(), tmax = __builtin_inff();
+    const float b0[3] = {bmin.x, bmin.y, bmin.z}, b1[3] = {bmax.x, bmax.y, bmax.z};
+    const float oo[3] = {o.x, o.y, o.z}, di[3] = {Dinv.x, Dinv.y, Dinv.z};
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+    {
+        bool s = __builtin_signbit(di[i]);
+        float ta = (b0[i] - oo[i]) * di[i];
+        float tb = (b1[i] - oo[i]) * di[i];
+        float tn = s ? tb : ta;
+        float tf = s ? ta : tb;
+        if (tn > tmin) tmin = tn;
+        if (tf < tmax) tmax = tf;
+    }
+    tMin = tmin;
+    return tmin < tmax && tmax > 0.0001f;
+}
+
+CGS_DEV uint32_t hashKey(const BVHRef& b, uint32_t key)
+{
+    uint32_t x = b.A * key + b.B + b.displ[key & b.mask];
+    return fastmod(x, b.T);
+}
+
+// Stackless closest-hit traversal (rt/BVH.h:178-266).  `limit` is the
+// caller's hit.t (read through a reference so leaf hits tighten the culling);
+// `leaf(index)` tests one element and returns whether it improved the hit.
+template<class Leaf>
+CGS_DEV bool hashIntersect(const BVHRef& b, V3 o, V3 dir, const float& limit, Leaf&& leaf)
+{
+    const V3 Dinv {1.0f / dir.x, 1.0f / dir.y, 1.0f / dir.z};
+    uint32_t node = b.root, key = 1, postponed = 0;
+    bool result = false;
+    while (true)
+    {
+        NodeBoxes n = loadNode(b.nodes, node);
+        if (n.left != EMPTY)
+        {
+            float L0, R0;
+            bool iL = nodeIntersect(L0, n.lmin, n.lmax, o, Dinv) && L0 < limit;
+            bool iR = nodeIntersect(R0, n.rmin, n.rmax, o, Dinv) && R0 < limit;
+            if (iL || iR) postponed <<= 1;
+            if (iL)
+            {
+                if (iR)
+                {
+                    postponed |= 1;
+                    if (L0 < R0) { key = key << 1; node = n.left; }
+                    else { key = (key << 1) ^ 1u; node = n.right; }
+                }
+                else { key = key << 1; node = n.left; }
+                continue;
+            }
+            else if (iR)
+            {
+                key = (key << 1) ^ 1u;
+                node = n.right;
+                continue;
+            }
+        }
+        else
+        {
+            uint32_t i = n.right;
+            uint32_t e;
+            while ((e = b.indices[i]) != EMPTY)
+            {
+                result |= leaf(e);
+                i++;
+            }
+        }
+        if (!postponed) break;
+        int levels = __builtin_ctz(postponed);
+        postponed >>= levels;
+        key = (key >> levels) ^ 1u;
+        postponed ^= 1;
+        node = b.table[hashKey(b, key)];
+        if (node >= b.nodeCount) break; // a corrupt table ends the ray instead of faulting
+    }
+    return result;
+}
+
+// Stackless any-hit traversal against `tmax` (rt/BVH.h:268-350).
+template<class Leaf>
+CGS_DEV bool hashIntersectAny(const BVHRef& b, V3 o, V3 dir, float tmax, Leaf&& leaf)
+{
+    const V3 Dinv {1.0f / dir.x, 1.0f / dir.y, 1.0f / dir.z};
+    uint32_t node = b.root, key = 1, postponed = 0;
+    while (true)
+    {
+        NodeBoxes n = loadNode(b.nodes, node);
+        if (n.left != EMPTY)
+        {
+            float L0, R0;
+            bool iL = nodeIntersect(L0, n.lmin, n.lmax, o, Dinv) && L0 < tmax;
+            bool iR = nodeIntersect(R0, n.rmin, n.rmax, o, Dinv) && R0 < tmax;
+            if (iL || iR) postponed <<= 1;
+            if (iL)
+            {
+                if (iR)
+                {
+                    postponed |= 1;
+                    if (L0 < R0) { key = key << 1; node = n.left; }
+                    else { key = (key << 1) ^ 1u; node = n.right; }
+                }
+                else { key = key << 1; node = n.left; }
+                continue;
+            }
+            else if (iR)
+            {
+                key = (key << 1) ^ 1u;
+                node = n.right;
+                continue;
+            }
+        }
+        else
+        {
+            uint32_t i = n.right;
+            uint32_t e;
+            while ((e = b.indices[i]) != EMPTY)
+            {
+                if (leaf(e)) return true;
+                i++;
+            }
+        }
+        if (!postponed) break;
+        int levels = __builtin_ctz(postponed);
+        postponed >>= levels;
+        key = (key >> levels) ^ 1u;
+        postponed ^= 1;
+        node = b.table[hashKey(b, key)];
+        if (node >= b.nodeCount) break; // a corrupt table ends the ray instead of faulting
+    }
+    return false;
+}
+
+} // namespace cgs

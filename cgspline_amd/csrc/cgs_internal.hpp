@@ -1,0 +1,57 @@
+// SPDX-License-Identifier: MIT
+//
+// Declarations shared between the host library and the kernel translation
+// units (not part of the public ABI).
+//
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/cgs.h"
+
+namespace cgs
+{
+
+struct SceneDev;
+
+// One perfect-hash build (one BVH's key set).
+struct PhfJob
+{
+    uint32_t keyOff, n, M, mask;
+    uint32_t scratchOff;   // per-job scratch words: 13·M + 2·tcap
+    uint32_t tcap;         // capacity for the growing table: n + 16·step
+    uint32_t displOff;     // output displacement (M words)
+    uint32_t tableOff;     // output table (tcap words)
+    uint32_t step;         // table growth per failed function
+    uint32_t abOff;        // 16 (A,B) pairs drawn by the host
+    uint32_t nodeBase;     // value added to every table entry
+    uint32_t pad;
+};
+
+struct PhfResult
+{
+    uint32_t A, B, T, attempts, status;
+};
+
+inline uint32_t phfScratchWords(uint32_t M, uint32_t tcap) { return 13u * M + 2u * tcap; }
+
+hipError_t launchPhfBuild(const PhfJob* jobs, uint32_t njobs, const uint32_t* keys, const uint32_t* ab, uint32_t* scratch,
+                          uint32_t* displ, uint32_t* tables, PhfResult* results, hipStream_t s);
+hipError_t launchPhfHash(const cgs_phf& h, const uint32_t* D, const uint32_t* keys, uint32_t n, uint32_t* out, hipStream_t s);
+hipError_t launchPhfFill(const cgs_phf& h, const uint32_t* D, const uint32_t* keys, uint32_t n, uint32_t base,
+                         uint32_t* table, hipStream_t s);
+hipError_t launchFill(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
+
+hipError_t launchTraceClosest(const SceneDev& sc, const cgs_ray* rays, uint32_t n, cgs_hit* hits, int32_t* objects,
+                              unsigned long long* overflow, hipStream_t s);
+hipError_t launchTraceAny(const SceneDev& sc, const cgs_ray* rays, uint32_t n, uint32_t* occluded,
+                          unsigned long long* overflow, hipStream_t s);
+hipError_t launchClipBatch(const cgs_ray* rays, const float* patches, uint32_t n, float tol, cgs_hit* hits, hipStream_t s);
+hipError_t launchSubdivBatch(const cgs_ray* rays, const float* patches, uint32_t n, float threshold, cgs_hit* hits,
+                             hipStream_t s);
+hipError_t launchRender(const SceneDev& sc, const float* toWorld16, const float* pos3, float tlx, float tly, float hdx,
+                        float hdy, float eps, int depth, uint32_t w, uint32_t h, uint32_t* frame, float* color,
+                        unsigned long long* counts, hipStream_t s);
+
+} // namespace cgs

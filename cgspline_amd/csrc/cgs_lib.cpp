@@ -1,0 +1,886 @@
+// SPDX-License-Identifier: MIT
+//
+// Host side of libcgs: the C ABI (include/cgs.h), the BVH build
+// (rt/BVH.cpp) and the scene upload (MainWindow::convertScene).
+//
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <bit>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <future>
+#include <limits>
+#include <map>
+#include <mutex>
+#include <numeric>
+#include <random>
+#include <span>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "cgs_internal.hpp"
+#include "cgs_scene_dev.hpp"
+
+using namespace cgs;
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+static thread_local std::string g_lastError;
+
+static cgs_status fail(cgs_status s, const std::string& msg)
+{
+    g_lastError = msg;
+    return s;
+}
+
+#define HIP_TRY(expr)                                                                        \
+    do                                                                                       \
+    {                                                                                        \
+        hipError_t _e = (expr);                                                              \
+        if (_e != hipSuccess)                                                                \
+            return fail(CGS_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));     \
+    } while (0)
+
+struct cgs_ctx
+{
+    int device = 0;
+    hipEvent_t renderDone = nullptr;
+    bool renderLaunched = false;
+    unsigned long long* dOverflow = nullptr; // device counter of overflowed clip stacks
+    std::mutex mu;
+};
+
+// ---------------------------------------------------------------------------
+// BVH build (rt/BVH.cpp:9-108)
+// ---------------------------------------------------------------------------
+namespace
+{
+
+constexpr float kFMax = std::numeric_limits<float>::max();
+
+struct Box3
+{
+    float mn[3] {kFMax, kFMax, kFMax};
+    float mx[3] {-kFMax, -kFMax, -kFMax};
+    void inflate(const float* p)
+    {
+        for (int i = 0; i < 3; i++)
+        {
+            if (p[i] < mn[i]) mn[i] = p[i];
+            if (p[i] > mx[i]) mx[i] = p[i];
+        }
+    }
+    void inflate(const Box3& b)
+    {
+        inflate(b.mn);
+        inflate(b.mx);
+    }
+};
+
+struct BuiltBVH
+{
+    std::vector<cgs_bvh_node> nodes;
+    std::vector<uint32_t> keys;
+    std::vector<uint32_t> indices;
+};
+
+// Node and index-slot counts of a median-split subtree of c elements at the
+// given depth: they depend only on (c, depth), so a subtree's position in the
+// pre-order arrays is known before its siblings are built, and subtrees can
+// be built concurrently with the sequential build's exact output.
+struct Sizes
+{
+    uint32_t epn;
+    std::map<std::pair<uint64_t, int>, std::pair<uint64_t, uint64_t>> memo;
+    // Fills the memo for every (count, depth) reachable from the root.
+    std::pair<uint64_t, uint64_t> fill(uint64_t c, int depth)
+    {
+        if (c <= epn || depth >= 31) return {1, c + 1};
+        auto it = memo.find({c, depth});
+        if (it != memo.end()) return it->second;
+        auto l = fill(c / 2, depth + 1);
+        auto r = fill(c - c / 2, depth + 1);
+        std::pair<uint64_t, uint64_t> v {1 + l.first + r.first, l.second + r.second};
+        memo[{c, depth}] = v;
+        return v;
+    }
+    // Read-only lookup, safe from concurrent subtree builds after fill().
+    std::pair<uint64_t, uint64_t> operator()(uint64_t c, int depth) const
+    {
+        if (c <= epn || depth >= 31) return {1, c + 1};
+        return memo.at({c, depth});
+    }
+};
+
+struct Builder
+{
+    BuiltBVH& out;
+    uint32_t epn;
+    Sizes& sizes;
+    std::atomic<int>& spawned;
+
+    // BVH.cpp:71-78: a child's box is its left box, inflated by its right
+    // box when the child is an inner node.
+    static void nodeBoxOf(const cgs_bvh_node& n, Box3& b)
+    {
+        std::memcpy(b.mn, n.lmin, 12);
+        std::memcpy(b.mx, n.lmax, 12);
+        if (n.left != CGS_EMPTY)
+        {
+            b.inflate(n.rmin);
+            b.inflate(n.rmax);
+        }
+    }
+
+    // Restates BVH::split / BVH::wrap with the node written at `ni`, its
+    // index slots at `ii`.
+    void build(cgs_bvh_element* el, uint64_t count, uint32_t key, int depth, uint64_t ni, uint64_t ii)
+    {
+        cgs_bvh_node& node = out.nodes[ni];
+        out.keys[ni] = key;
+        if (count <= epn || (key & (1u << 31)))
+        {
+            Box3 box;
+            for (uint64_t i = 0; i < count; i++)
+            {
+                out.indices[ii + i] = el[i].index;
+                box.inflate(el[i].bmin);
+                box.inflate(el[i].bmax);
+            }
+            out.indices[ii + count] = CGS_EMPTY;
+            Box3 empty;
+            std::memcpy(node.lmin, box.mn, 12);
+            std::memcpy(node.lmax, box.mx, 12);
+            std::memcpy(node.rmin, empty.mn, 12);
+            std::memcpy(node.rmax, empty.mx, 12);
+            node.left = CGS_EMPTY;
+            node.right = (uint32_t)ii;
+            node.pad[0] = node.pad[1] = 0;
+            return;
+        }
+        // the split axis comes from a never-inflated box: every component of
+        // its size is equal, so the comparison chain always selects Y
+        Box3 nobox;
+        float v[3] = {nobox.mx[0] - nobox.mn[0], nobox.mx[1] - nobox.mn[1], nobox.mx[2] - nobox.mn[2]};
+        const int dim = v[2] > v[1] ? (v[0] > v[2] ? 0 : 2) : (v[0] > v[1] ? 0 : 1);
+        const uint64_t middle = count / 2;
+        std::nth_element(el, el + middle, el + count,
+                         [dim](const cgs_bvh_element& a, const cgs_bvh_element& b) {
+                             return a.centroid[dim] < b.centroid[dim];
+                         });
+        const auto ls = sizes(middle, depth + 1);
+        const uint64_t L = ni + 1, R = L + ls.first;
+        const uint64_t IL = ii, IR = ii + ls.second;
+        const uint32_t kl = key << 1, kr = (key << 1) ^ 1u;
+        if (count > (1u << 15) && spawned.fetch_add(1) < 64)
+        {
+            auto f = std::async(std::launch::async, [&] { build(el, middle, kl, depth + 1, L, IL); });
+            build(el + middle, count - middle, kr, depth + 1, R, IR);
+            f.get();
+        }
+        else
+        {
+            build(el, middle, kl, depth + 1, L, IL);
+            build(el + middle, count - middle, kr, depth + 1, R, IR);
+        }
+        Box3 lb, rb;
+        nodeBoxOf(out.nodes[L], lb);
+        nodeBoxOf(out.nodes[R], rb);
+        std::memcpy(node.lmin, lb.mn, 12);
+        std::memcpy(node.lmax, lb.mx, 12);
+        std::memcpy(node.rmin, rb.mn, 12);
+        std::memcpy(node.rmax, rb.mx, 12);
+        node.left = (uint32_t)L;
+        node.right = (uint32_t)R;
+        node.pad[0] = node.pad[1] = 0;
+    }
+};
+
+cgs_status buildBVH(std::vector<cgs_bvh_element>& el, uint32_t epn, BuiltBVH& out)
+{
+    if (el.empty()) return fail(CGS_ERR_EMPTY_SPLIT, "BVH build over an empty element range");
+    if (epn == 0) epn = 1;
+    Sizes sizes {epn, {}};
+    auto total = sizes.fill(el.size(), 0);
+    out.nodes.assign(total.first, cgs_bvh_node {});
+    out.keys.assign(total.first, 0);
+    out.indices.assign(total.second, 0);
+    std::atomic<int> spawned {0};
+    Builder b {out, epn, sizes, spawned};
+    b.build(el.data(), el.size(), 1u, 0, 0, 0);
+    return CGS_OK;
+}
+
+// BezierSurface::buildBVH element data (rt/Primitive.cu:332-353).
+void patchElements(const float* verts, const uint32_t* idx, uint32_t npatch, cgs_bvh_element* out)
+{
+    for (uint32_t i = 0; i < npatch; i++)
+    {
+        Box3 b;
+        float c[3] = {0.0f, 0.0f, 0.0f};
+        const float inv = 1.0f / 16.0f;
+        for (int j = 0; j < 16; j++)
+        {
+            const float* w = verts + 4 * (size_t)idx[16 * (size_t)i + j];
+            float p[3] = {w[0] / w[3], w[1] / w[3], w[2] / w[3]};
+            b.inflate(p);
+            for (int k = 0; k < 3; k++) c[k] = c[k] + inv * p[k];
+        }
+        std::memcpy(out[i].bmin, b.mn, 12);
+        std::memcpy(out[i].bmax, b.mx, 12);
+        std::memcpy(out[i].centroid, c, 12);
+        out[i].index = i;
+    }
+}
+
+// PerfectHashFunction's (A,B) draws (rt/PerfectHashFunction.h:45-46,
+// .cu:452-473): mt19937(seed), uniform_int_distribution<uint32_t>(0, n),
+// A = 0 replaced by 31, redrawn while gcd(n, A) > 1.  All 16 functions'
+// pairs, in order.
+std::vector<uint32_t> drawAB(uint32_t n, uint32_t seed)
+{
+    std::mt19937 mt(seed);
+    std::uniform_int_distribution<uint32_t> d(0, n);
+    std::vector<uint32_t> ab;
+    for (int fn = 0; fn < 16; fn++)
+    {
+        uint32_t A, B;
+        do
+        {
+            A = d(mt);
+            if (A == 0) A = 31;
+            B = d(mt);
+        } while (std::gcd((size_t)n, (size_t)A) > 1);
+        ab.push_back(A);
+        ab.push_back(B);
+    }
+    return ab;
+}
+
+uint32_t displSize(uint64_t n)
+{
+    int s = int(n - 1) >> 1;
+    if (s < 2) s = 2;
+    return std::bit_floor((unsigned)s);
+}
+
+uint32_t growthStep(uint64_t n) { return std::max<uint32_t>(4u, (uint32_t)(0.05f * (float)n)); }
+
+template<class F> void parallelFor(size_t n, F&& f)
+{
+    unsigned th = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    if (n < 2 || th == 1)
+    {
+        for (size_t i = 0; i < n; i++) f(i);
+        return;
+    }
+    std::atomic<size_t> next {0};
+    std::vector<std::thread> ts;
+    for (unsigned t = 0; t < th; t++)
+        ts.emplace_back([&] {
+            size_t i;
+            while ((i = next.fetch_add(1)) < n) f(i);
+        });
+    for (auto& t : ts) t.join();
+}
+
+struct DeviceBuffer
+{
+    void* p = nullptr;
+    ~DeviceBuffer()
+    {
+        if (p) (void)hipFree(p);
+    }
+    hipError_t alloc(size_t bytes)
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        return hipMalloc(&p, std::max<size_t>(bytes, 256));
+    }
+    template<class T> T* as() const { return (T*)p; }
+};
+
+// Batched PHF build: jobs over keys already on the device.
+struct PhfBatch
+{
+    std::vector<PhfJob> jobs;
+    std::vector<uint32_t> ab;
+    uint64_t scratchWords = 0, displWords = 0, tableWords = 0;
+    std::map<std::pair<uint32_t, uint32_t>, uint32_t> abCache;
+
+    // Adds a job; returns its index.
+    uint32_t add(uint32_t keyOff, uint32_t n, uint32_t seed, uint32_t nodeBase)
+    {
+        PhfJob j {};
+        j.keyOff = keyOff;
+        j.n = n;
+        j.M = displSize(n);
+        j.mask = j.M - 1;
+        j.step = growthStep(n);
+        j.tcap = n + 16 * j.step;
+        j.scratchOff = (uint32_t)scratchWords;
+        scratchWords += phfScratchWords(j.M, j.tcap);
+        j.displOff = (uint32_t)displWords;
+        displWords += j.M;
+        j.tableOff = (uint32_t)tableWords;
+        tableWords += j.tcap;
+        auto key = std::make_pair(n, seed);
+        auto it = abCache.find(key);
+        if (it == abCache.end())
+        {
+            uint32_t off = (uint32_t)(ab.size() / 2);
+            auto v = drawAB(n, seed);
+            ab.insert(ab.end(), v.begin(), v.end());
+            it = abCache.emplace(key, off).first;
+        }
+        j.abOff = it->second;
+        j.nodeBase = nodeBase;
+        jobs.push_back(j);
+        return (uint32_t)jobs.size() - 1;
+    }
+};
+
+cgs_status runPhfBatch(PhfBatch& b, const uint32_t* dKeys, uint32_t* dDispl, uint32_t* dTables,
+                       std::vector<PhfResult>& results, hipStream_t s)
+{
+    DeviceBuffer jobs, ab, scratch, res;
+    HIP_TRY(jobs.alloc(b.jobs.size() * sizeof(PhfJob)));
+    HIP_TRY(ab.alloc(b.ab.size() * 4));
+    HIP_TRY(scratch.alloc(b.scratchWords * 4));
+    HIP_TRY(res.alloc(b.jobs.size() * sizeof(PhfResult)));
+    HIP_TRY(hipMemcpyAsync(jobs.p, b.jobs.data(), b.jobs.size() * sizeof(PhfJob), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(ab.p, b.ab.data(), b.ab.size() * 4, hipMemcpyHostToDevice, s));
+    HIP_TRY(launchPhfBuild(jobs.as<PhfJob>(), (uint32_t)b.jobs.size(), dKeys, ab.as<uint32_t>(), scratch.as<uint32_t>(),
+                           dDispl, dTables, res.as<PhfResult>(), s));
+    results.resize(b.jobs.size());
+    HIP_TRY(hipMemcpyAsync(results.data(), res.p, b.jobs.size() * sizeof(PhfResult), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return CGS_OK;
+}
+
+uint64_t fastmodM(uint32_t T) { return 0xFFFFFFFFFFFFFFFFull / T + 1; }
+
+} // namespace
+
+// ---------------------------------------------------------------------------
+// scene
+// ---------------------------------------------------------------------------
+struct cgs_scene
+{
+    cgs_ctx* ctx = nullptr;
+    uint32_t nsurf = 0, nobj = 0, nlight = 0, nodeCount = 0;
+    std::vector<BuiltBVH> bvh;     // per surface, then the top level (local offsets)
+    std::vector<uint32_t> nodeBase, indexBase;
+    std::vector<DevBVH> dev;       // per surface, then the top level
+    std::vector<cgs_phf> phf;
+    DeviceBuffer patches, nodes, indices, keys, displ, tables, surfaces, objects, lights;
+    float background[3], ambient[3];
+    SceneDev view() const
+    {
+        SceneDev s {};
+        s.patches = patches.as<float4>();
+        s.nodes = nodes.as<float4>();
+        s.indices = indices.as<uint32_t>();
+        s.displ = displ.as<uint32_t>();
+        s.tables = tables.as<uint32_t>();
+        s.surfaces = surfaces.as<DevBVH>();
+        s.objects = objects.as<DevObject>();
+        s.lights = lights.as<DevLight>();
+        s.nlight = nlight;
+        s.nobj = nobj;
+        s.nodeCount = nodeCount;
+        s.top = dev.back();
+        for (int i = 0; i < 3; i++)
+        {
+            s.background[i] = background[i];
+            s.ambient[i] = ambient[i];
+        }
+        return s;
+    }
+};
+
+extern "C" {
+
+int cgs_abi_version(void) { return CGS_ABI_VERSION; }
+
+const char* cgs_status_string(int s)
+{
+    switch (s)
+    {
+    case CGS_OK: return "ok";
+    case CGS_ERR_INVALID: return "invalid argument";
+    case CGS_ERR_HIP: return "HIP runtime error";
+    case CGS_ERR_HASH_FAILED: return "hash table build failed";
+    case CGS_ERR_HASH_DUPLICATE: return "perfect hash: duplicate found";
+    case CGS_ERR_BUCKET_OVERFLOW: return "perfect hash: bucket overflow";
+    case CGS_ERR_BUSY: return "ray tracer not ready: previous render still running";
+    case CGS_ERR_EMPTY_SPLIT: return "invalid";
+    case CGS_ERR_STACK_OVERFLOW: return "clipping region stack overflow";
+    case CGS_ERR_NO_DEVICE: return "no HIP device";
+    case CGS_ERR_OUT_OF_MEMORY: return "out of memory";
+    default: return "unknown status";
+    }
+}
+
+const char* cgs_last_error(void) { return g_lastError.c_str(); }
+
+cgs_status cgs_open(int device, cgs_ctx** out)
+{
+    if (!out) return fail(CGS_ERR_INVALID, "cgs_open: null out");
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return fail(CGS_ERR_NO_DEVICE, "no HIP device");
+    if (device < 0 || device >= count) return fail(CGS_ERR_INVALID, "cgs_open: device out of range");
+    HIP_TRY(hipSetDevice(device));
+    auto* c = new cgs_ctx;
+    c->device = device;
+    hipError_t e = hipEventCreateWithFlags(&c->renderDone, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipMalloc(&c->dOverflow, 256);
+    if (e == hipSuccess) e = hipMemset(c->dOverflow, 0, 256);
+    if (e != hipSuccess)
+    {
+        delete c;
+        return fail(CGS_ERR_HIP, std::string("cgs_open: ") + hipGetErrorString(e));
+    }
+    *out = c;
+    return CGS_OK;
+}
+
+cgs_status cgs_close(cgs_ctx* ctx)
+{
+    if (!ctx) return CGS_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->renderDone) (void)hipEventSynchronize(ctx->renderDone), (void)hipEventDestroy(ctx->renderDone);
+    if (ctx->dOverflow) (void)hipFree(ctx->dOverflow);
+    delete ctx;
+    return CGS_OK;
+}
+
+cgs_status cgs_bezier_clip(cgs_ctx* ctx, const cgs_ray* rays, const float* patches, uint32_t n, float tol, cgs_hit* hits,
+                           void* stream)
+{
+    if (!ctx || (n && (!rays || !patches || !hits))) return fail(CGS_ERR_INVALID, "cgs_bezier_clip: null argument");
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(launchClipBatch(rays, patches, n, tol, hits, (hipStream_t)stream));
+    return CGS_OK;
+}
+
+cgs_status cgs_subdivision(cgs_ctx* ctx, const cgs_ray* rays, const float* patches, uint32_t n, float threshold,
+                           cgs_hit* hits, void* stream)
+{
+    if (!ctx || (n && (!rays || !patches || !hits))) return fail(CGS_ERR_INVALID, "cgs_subdivision: null argument");
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(launchSubdivBatch(rays, patches, n, threshold, hits, (hipStream_t)stream));
+    return CGS_OK;
+}
+
+cgs_status cgs_bvh_build(const cgs_bvh_element* elements, uint32_t n, uint32_t epn, cgs_bvh_node** nodes,
+                         uint32_t** keys, uint32_t* node_count, uint32_t** indices, uint32_t* index_count)
+{
+    if (!nodes || !keys || !node_count || !indices || !index_count || (n && !elements))
+        return fail(CGS_ERR_INVALID, "cgs_bvh_build: null argument");
+    std::vector<cgs_bvh_element> el(elements, elements + n);
+    BuiltBVH b;
+    cgs_status st = buildBVH(el, epn, b);
+    if (st != CGS_OK) return st;
+    *nodes = (cgs_bvh_node*)std::malloc(b.nodes.size() * sizeof(cgs_bvh_node));
+    *keys = (uint32_t*)std::malloc(b.keys.size() * 4);
+    *indices = (uint32_t*)std::malloc(b.indices.size() * 4);
+    if (!*nodes || !*keys || !*indices) return fail(CGS_ERR_OUT_OF_MEMORY, "cgs_bvh_build: malloc");
+    std::memcpy(*nodes, b.nodes.data(), b.nodes.size() * sizeof(cgs_bvh_node));
+    std::memcpy(*keys, b.keys.data(), b.keys.size() * 4);
+    std::memcpy(*indices, b.indices.data(), b.indices.size() * 4);
+    *node_count = (uint32_t)b.nodes.size();
+    *index_count = (uint32_t)b.indices.size();
+    return CGS_OK;
+}
+
+cgs_status cgs_patch_elements(const float* vertices, const uint32_t* indices, uint32_t npatch, cgs_bvh_element* elements)
+{
+    if (npatch && (!vertices || !indices || !elements)) return fail(CGS_ERR_INVALID, "cgs_patch_elements: null argument");
+    patchElements(vertices, indices, npatch, elements);
+    return CGS_OK;
+}
+
+void cgs_free_host(void* p) { std::free(p); }
+
+uint32_t cgs_phf_displacement_size(uint32_t n) { return displSize(n); }
+
+cgs_status cgs_phf_build(cgs_ctx* ctx, const uint32_t* keys, uint32_t n, uint32_t seed, uint32_t* displacement,
+                         cgs_phf* out, void* stream)
+{
+    if (!ctx || !out || !displacement || (n && !keys)) return fail(CGS_ERR_INVALID, "cgs_phf_build: null argument");
+    if (n == 0) return fail(CGS_ERR_INVALID, "cgs_phf_build: empty key set");
+    HIP_TRY(hipSetDevice(ctx->device));
+    hipStream_t s = (hipStream_t)stream;
+    PhfBatch b;
+    b.add(0, n, seed, 0);
+    DeviceBuffer tables;
+    HIP_TRY(tables.alloc(b.tableWords * 4));
+    std::vector<PhfResult> r;
+    cgs_status st = runPhfBatch(b, keys, displacement, tables.as<uint32_t>(), r, s);
+    if (st != CGS_OK) return st;
+    out->A = r[0].A;
+    out->B = r[0].B;
+    out->mask = b.jobs[0].mask;
+    out->table_size = r[0].T;
+    out->attempts = r[0].attempts;
+    out->displacement_size = b.jobs[0].M;
+    if (r[0].status != CGS_OK)
+        return fail((cgs_status)r[0].status, r[0].status == CGS_ERR_HASH_DUPLICATE ? "PerfectHashFunction: DUPLICATE FOUND!"
+                                                                                  : "PerfectHashFunction: bucket overflow");
+    return CGS_OK;
+}
+
+cgs_status cgs_phf_fill_table(cgs_ctx* ctx, const cgs_phf* h, const uint32_t* displacement, const uint32_t* keys,
+                              uint32_t n, uint32_t* table, void* stream)
+{
+    if (!ctx || !h || !displacement || !table || (n && !keys) || h->table_size == 0)
+        return fail(CGS_ERR_INVALID, "cgs_phf_fill_table: bad argument");
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(launchPhfFill(*h, displacement, keys, n, 0, table, (hipStream_t)stream));
+    return CGS_OK;
+}
+
+cgs_status cgs_phf_hash(cgs_ctx* ctx, const cgs_phf* h, const uint32_t* displacement, const uint32_t* keys, uint32_t n,
+                        uint32_t* out, void* stream)
+{
+    if (!ctx || !h || !displacement || !out || (n && !keys) || h->table_size == 0)
+        return fail(CGS_ERR_INVALID, "cgs_phf_hash: bad argument");
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(launchPhfHash(*h, displacement, keys, n, out, (hipStream_t)stream));
+    return CGS_OK;
+}
+
+cgs_status cgs_scene_create(cgs_ctx* ctx, const cgs_scene_desc* d, cgs_scene** out)
+{
+    if (!ctx || !d || !out) return fail(CGS_ERR_INVALID, "cgs_scene_create: null argument");
+    if (d->nsurf == 0 || d->nobj == 0 || !d->vertices || !d->indices || !d->surf_vbase || !d->surf_pbase ||
+        !d->obj_surface || !d->obj_local2world || !d->obj_world2local || !d->obj_material || (d->nlight && !d->lights))
+        return fail(CGS_ERR_INVALID, "cgs_scene_create: incomplete description");
+    for (uint32_t i = 0; i < d->nobj; i++)
+        if (d->obj_surface[i] >= d->nsurf) return fail(CGS_ERR_INVALID, "cgs_scene_create: object surface out of range");
+    for (uint32_t k = 0; k < d->nsurf; k++)
+        if (d->surf_pbase[k + 1] <= d->surf_pbase[k] || d->surf_vbase[k + 1] < d->surf_vbase[k])
+            return fail(CGS_ERR_EMPTY_SPLIT, "cgs_scene_create: a surface has no patch");
+    HIP_TRY(hipSetDevice(ctx->device));
+    auto sc = std::make_unique<cgs_scene>();
+    sc->ctx = ctx;
+    sc->nsurf = d->nsurf;
+    sc->nobj = d->nobj;
+    sc->nlight = d->nlight;
+    std::memcpy(sc->background, d->background, 12);
+    std::memcpy(sc->ambient, d->ambient, 12);
+    const uint32_t S = d->nsurf;
+    const uint64_t npatch = d->surf_pbase[S];
+    // validate indices
+    for (uint32_t k = 0; k < S; k++)
+    {
+        const uint32_t nv = d->surf_vbase[k + 1] - d->surf_vbase[k];
+        for (uint64_t q = 16ull * d->surf_pbase[k]; q < 16ull * d->surf_pbase[k + 1]; q++)
+            if (d->indices[q] >= nv) return fail(CGS_ERR_INVALID, "cgs_scene_create: patch index out of range");
+    }
+
+    // 1. per-surface BVHs (BezierSurface::buildBVH), in parallel over surfaces
+    sc->bvh.resize(S + 1);
+    std::vector<cgs_status> st(S, CGS_OK);
+    parallelFor(S, [&](size_t k) {
+        const uint32_t p0 = d->surf_pbase[k], p1 = d->surf_pbase[k + 1];
+        std::vector<cgs_bvh_element> el(p1 - p0);
+        patchElements(d->vertices + 4 * (size_t)d->surf_vbase[k], d->indices + 16 * (size_t)p0, p1 - p0, el.data());
+        st[k] = buildBVH(el, 1, sc->bvh[k]);
+    });
+    for (auto s : st)
+        if (s != CGS_OK) return s;
+
+    // 2. top-level BVH over object world bounds (Scene::buildBVH)
+    {
+        std::vector<cgs_bvh_element> el(d->nobj);
+        for (uint32_t i = 0; i < d->nobj; i++)
+        {
+            const uint32_t k = d->obj_surface[i];
+            const float* M = d->obj_local2world + 16 * (size_t)i;
+            const float* V = d->vertices + 4 * (size_t)d->surf_vbase[k];
+            Box3 b;
+            for (uint64_t q = 16ull * d->surf_pbase[k]; q < 16ull * d->surf_pbase[k + 1]; q++)
+            {
+                const float* v = V + 4 * (size_t)d->indices[q];
+                float p[3];
+                for (int r = 0; r < 3; r++)
+                    p[r] = ((v[0] * M[r] + v[1] * M[4 + r]) + v[2] * M[8 + r]) + v[3] * M[12 + r];
+                b.inflate(p);
+            }
+            std::memcpy(el[i].bmin, b.mn, 12);
+            std::memcpy(el[i].bmax, b.mx, 12);
+            for (int r = 0; r < 3; r++) el[i].centroid[r] = 0.5f * (b.mn[r] + b.mx[r]);
+            el[i].index = i;
+        }
+        cgs_status s = buildBVH(el, 1, sc->bvh[S]);
+        if (s != CGS_OK) return s;
+    }
+
+    // 3. layout: absolute node offsets, leaf slots and patch bases
+    uint64_t nodeTotal = 0, indexTotal = 0, keyTotal = 0;
+    sc->nodeBase.resize(S + 1);
+    sc->indexBase.resize(S + 1);
+    for (uint32_t k = 0; k <= S; k++)
+    {
+        sc->nodeBase[k] = (uint32_t)nodeTotal;
+        sc->indexBase[k] = (uint32_t)indexTotal;
+        nodeTotal += sc->bvh[k].nodes.size();
+        indexTotal += sc->bvh[k].indices.size();
+    }
+    keyTotal = nodeTotal;
+    sc->nodeCount = (uint32_t)nodeTotal;
+    if (nodeTotal >= 0xFFFFFFF0ull || indexTotal >= 0xFFFFFFF0ull)
+        return fail(CGS_ERR_INVALID, "cgs_scene_create: scene too large for 32-bit offsets");
+    std::vector<cgs_bvh_node> nodes(nodeTotal);
+    std::vector<uint32_t> indices(indexTotal), keys(keyTotal);
+    parallelFor(S + 1, [&](size_t k) {
+        const BuiltBVH& b = sc->bvh[k];
+        const uint32_t nb = sc->nodeBase[k], ib = sc->indexBase[k];
+        for (size_t i = 0; i < b.nodes.size(); i++)
+        {
+            cgs_bvh_node n = b.nodes[i];
+            if (n.left != CGS_EMPTY)
+            {
+                n.left += nb;
+                n.right += nb;
+            }
+            else
+                n.right += ib;
+            nodes[nb + i] = n;
+            keys[nb + i] = b.keys[i];
+        }
+        std::copy(b.indices.begin(), b.indices.end(), indices.begin() + ib);
+    });
+
+    // 4. gathered patches (256 B each), objects, lights
+    std::vector<float> patches(npatch * 64);
+    parallelFor(S, [&](size_t k) {
+        const float* V = d->vertices + 4 * (size_t)d->surf_vbase[k];
+        for (uint64_t p = d->surf_pbase[k]; p < d->surf_pbase[k + 1]; p++)
+            for (int j = 0; j < 16; j++)
+                std::memcpy(&patches[64 * p + 4 * j], V + 4 * (size_t)d->indices[16 * p + j], 16);
+    });
+    std::vector<DevObject> objs(d->nobj);
+    for (uint32_t i = 0; i < d->nobj; i++)
+    {
+        DevObject& o = objs[i];
+        std::memcpy(o.w2l, d->obj_world2local + 16 * (size_t)i, 64);
+        const float* m = d->obj_material + 8 * (size_t)i;
+        o.mat0 = make_float4(m[0], m[1], m[2], m[6]);
+        o.mat1 = make_float4(m[3], m[4], m[5], m[7]);
+        o.surface = d->obj_surface[i];
+        o.pad[0] = o.pad[1] = o.pad[2] = 0;
+    }
+    hipStream_t s = nullptr;
+    HIP_TRY(sc->patches.alloc(patches.size() * 4));
+    HIP_TRY(sc->nodes.alloc(nodes.size() * sizeof(cgs_bvh_node)));
+    HIP_TRY(sc->indices.alloc(indices.size() * 4));
+    HIP_TRY(sc->keys.alloc(keys.size() * 4));
+    HIP_TRY(sc->objects.alloc(objs.size() * sizeof(DevObject)));
+    HIP_TRY(sc->lights.alloc(std::max<size_t>(1, d->nlight) * sizeof(DevLight)));
+    HIP_TRY(hipMemcpyAsync(sc->patches.p, patches.data(), patches.size() * 4, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(sc->nodes.p, nodes.data(), nodes.size() * sizeof(cgs_bvh_node), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(sc->indices.p, indices.data(), indices.size() * 4, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(sc->keys.p, keys.data(), keys.size() * 4, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(sc->objects.p, objs.data(), objs.size() * sizeof(DevObject), hipMemcpyHostToDevice, s));
+    if (d->nlight)
+        HIP_TRY(hipMemcpyAsync(sc->lights.p, d->lights, d->nlight * sizeof(DevLight), hipMemcpyHostToDevice, s));
+    sc->nlight = d->nlight;
+
+    // 5. perfect hashes for every BVH in one launch (BVH::buildHashTable)
+    PhfBatch batch;
+    for (uint32_t k = 0; k <= S; k++)
+        batch.add(sc->nodeBase[k], (uint32_t)sc->bvh[k].keys.size(), d->hash_seed, sc->nodeBase[k]);
+    HIP_TRY(sc->displ.alloc(batch.displWords * 4));
+    HIP_TRY(sc->tables.alloc(batch.tableWords * 4));
+    std::vector<PhfResult> res;
+    cgs_status ps = runPhfBatch(batch, sc->keys.as<uint32_t>(), sc->displ.as<uint32_t>(), sc->tables.as<uint32_t>(), res, s);
+    if (ps != CGS_OK) return ps;
+    sc->dev.resize(S + 1);
+    sc->phf.resize(S + 1);
+    for (uint32_t k = 0; k <= S; k++)
+    {
+        const PhfJob& j = batch.jobs[k];
+        const PhfResult& r = res[k];
+        if (r.status != CGS_OK)
+            return fail(CGS_ERR_HASH_FAILED, "hash table build failed (BVH " + std::to_string(k) + ", " +
+                                                 cgs_status_string((int)r.status) + ")");
+        DevBVH& b = sc->dev[k];
+        b = DevBVH {};
+        b.root = sc->nodeBase[k];
+        b.displBase = j.displOff;
+        b.tableBase = j.tableOff;
+        b.A = r.A;
+        b.B = r.B;
+        b.mask = j.mask;
+        b.T = r.T;
+        b.patchBase = k < S ? d->surf_pbase[k] : 0;
+        b.Tm = fastmodM(r.T);
+        sc->phf[k] = cgs_phf {r.A, r.B, j.mask, r.T, r.attempts, j.M};
+    }
+    HIP_TRY(sc->surfaces.alloc(S * sizeof(DevBVH)));
+    HIP_TRY(hipMemcpyAsync(sc->surfaces.p, sc->dev.data(), S * sizeof(DevBVH), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    *out = sc.release();
+    return CGS_OK;
+}
+
+cgs_status cgs_scene_destroy(cgs_scene* scene)
+{
+    if (!scene) return CGS_OK;
+    (void)hipSetDevice(scene->ctx->device);
+    (void)hipDeviceSynchronize();
+    delete scene;
+    return CGS_OK;
+}
+
+cgs_status cgs_scene_bvh_info(cgs_scene* sc, int32_t surface, uint32_t* node_count, uint32_t* index_count, cgs_phf* hash)
+{
+    if (!sc || surface < -1 || surface >= (int32_t)sc->nsurf) return fail(CGS_ERR_INVALID, "cgs_scene_bvh_info: bad surface");
+    const uint32_t k = surface < 0 ? sc->nsurf : (uint32_t)surface;
+    if (node_count) *node_count = (uint32_t)sc->bvh[k].nodes.size();
+    if (index_count) *index_count = (uint32_t)sc->bvh[k].indices.size();
+    if (hash) *hash = sc->phf[k];
+    return CGS_OK;
+}
+
+cgs_status cgs_scene_bvh_copy(cgs_scene* sc, int32_t surface, cgs_bvh_node* nodes, uint32_t* keys, uint32_t* indices,
+                              uint32_t* displacement, uint32_t* table)
+{
+    if (!sc || surface < -1 || surface >= (int32_t)sc->nsurf) return fail(CGS_ERR_INVALID, "cgs_scene_bvh_copy: bad surface");
+    const uint32_t k = surface < 0 ? sc->nsurf : (uint32_t)surface;
+    const BuiltBVH& b = sc->bvh[k];
+    if (nodes) std::memcpy(nodes, b.nodes.data(), b.nodes.size() * sizeof(cgs_bvh_node));
+    if (keys) std::memcpy(keys, b.keys.data(), b.keys.size() * 4);
+    if (indices) std::memcpy(indices, b.indices.data(), b.indices.size() * 4);
+    HIP_TRY(hipSetDevice(sc->ctx->device));
+    if (displacement)
+        HIP_TRY(hipMemcpy(displacement, sc->displ.as<uint32_t>() + sc->dev[k].displBase, sc->phf[k].displacement_size * 4,
+                          hipMemcpyDeviceToHost));
+    if (table)
+    {
+        HIP_TRY(hipMemcpy(table, sc->tables.as<uint32_t>() + sc->dev[k].tableBase, sc->phf[k].table_size * 4,
+                          hipMemcpyDeviceToHost));
+        for (uint32_t i = 0; i < sc->phf[k].table_size; i++)
+            if (table[i] != CGS_EMPTY) table[i] -= sc->nodeBase[k];
+    }
+    return CGS_OK;
+}
+
+cgs_status cgs_trace_closest(cgs_ctx* ctx, cgs_scene* sc, const cgs_ray* rays, uint32_t n, cgs_hit* hits, int32_t* objects,
+                             void* stream)
+{
+    if (!ctx || !sc || (n && (!rays || !hits))) return fail(CGS_ERR_INVALID, "cgs_trace_closest: null argument");
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(launchTraceClosest(sc->view(), rays, n, hits, objects, ctx->dOverflow, (hipStream_t)stream));
+    return CGS_OK;
+}
+
+cgs_status cgs_trace_any(cgs_ctx* ctx, cgs_scene* sc, const cgs_ray* rays, uint32_t n, uint32_t* occluded, void* stream)
+{
+    if (!ctx || !sc || (n && (!rays || !occluded))) return fail(CGS_ERR_INVALID, "cgs_trace_any: null argument");
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(launchTraceAny(sc->view(), rays, n, occluded, ctx->dOverflow, (hipStream_t)stream));
+    return CGS_OK;
+}
+
+cgs_status cgs_render(cgs_ctx* ctx, cgs_scene* sc, const cgs_camera* cam, const cgs_render_options* op, uint32_t w,
+                      uint32_t h, uint32_t* frame, float* color, cgs_ray_counts* counts, void* stream)
+{
+    if (!ctx || !sc || !cam || !op || !frame) return fail(CGS_ERR_INVALID, "cgs_render: null argument");
+    if ((uint64_t)w * h >= (1ull << 32)) return fail(CGS_ERR_INVALID, "cgs_render: frame too large");
+    HIP_TRY(hipSetDevice(ctx->device));
+    std::lock_guard lk(ctx->mu);
+    if (ctx->renderLaunched)
+    {
+        hipError_t q = hipEventQuery(ctx->renderDone);
+        if (q == hipErrorNotReady)
+            return fail(CGS_ERR_BUSY, "CUDA ray tracer not ready: wait the previous work to finish or create a new instance");
+        HIP_TRY(q);
+    }
+    // camera setup (RayTracer.cu:221-237), on the host
+    const float halfHeight = std::tan(0.5f * cam->fovy);
+    float hdy = halfHeight / (float)h;
+    const float hdx = hdy;
+    const float tlx = -hdx * (float)(w - 1);
+    float tly = halfHeight - hdy;
+    if (cam->flip_y)
+    {
+        tly = -tly;
+        hdy = -hdy;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    HIP_TRY(launchRender(sc->view(), cam->to_world, cam->position, tlx, tly, hdx, hdy, op->eps, op->recursion_depth, w, h,
+                         frame, color, (unsigned long long*)counts, s));
+    HIP_TRY(hipEventRecord(ctx->renderDone, s));
+    ctx->renderLaunched = true;
+    return CGS_OK;
+}
+
+int cgs_render_running(cgs_ctx* ctx)
+{
+    if (!ctx || !ctx->renderLaunched) return 0;
+    return hipEventQuery(ctx->renderDone) == hipErrorNotReady ? 1 : 0;
+}
+
+cgs_status cgs_malloc(cgs_ctx* ctx, size_t bytes, void** dptr)
+{
+    if (!ctx || !dptr) return fail(CGS_ERR_INVALID, "cgs_malloc: null argument");
+    HIP_TRY(hipSetDevice(ctx->device));
+    hipError_t e = hipMalloc(dptr, std::max<size_t>(bytes, 1));
+    if (e == hipErrorOutOfMemory) return fail(CGS_ERR_OUT_OF_MEMORY, "cgs_malloc: out of memory");
+    HIP_TRY(e);
+    return CGS_OK;
+}
+
+cgs_status cgs_free(cgs_ctx* ctx, void* dptr)
+{
+    if (!ctx) return fail(CGS_ERR_INVALID, "cgs_free: null context");
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipFree(dptr));
+    return CGS_OK;
+}
+
+cgs_status cgs_memcpy_h2d(cgs_ctx* ctx, void* dst, const void* src, size_t bytes, void* stream)
+{
+    if (!ctx) return fail(CGS_ERR_INVALID, "cgs_memcpy_h2d: null context");
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, (hipStream_t)stream));
+    HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+    return CGS_OK;
+}
+
+cgs_status cgs_memcpy_d2h(cgs_ctx* ctx, void* dst, const void* src, size_t bytes, void* stream)
+{
+    if (!ctx) return fail(CGS_ERR_INVALID, "cgs_memcpy_d2h: null context");
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, (hipStream_t)stream));
+    HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+    return CGS_OK;
+}
+
+cgs_status cgs_memset(cgs_ctx* ctx, void* dst, int value, size_t bytes, void* stream)
+{
+    if (!ctx) return fail(CGS_ERR_INVALID, "cgs_memset: null context");
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipMemsetAsync(dst, value, bytes, (hipStream_t)stream));
+    return CGS_OK;
+}
+
+cgs_status cgs_stream_sync(cgs_ctx* ctx, void* stream)
+{
+    if (!ctx) return fail(CGS_ERR_INVALID, "cgs_stream_sync: null context");
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+    return CGS_OK;
+}
+
+} // extern "C"

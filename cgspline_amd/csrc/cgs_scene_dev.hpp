@@ -1,0 +1,90 @@
+// SPDX-License-Identifier: MIT
+//
+// Device-resident scene layout shared by the kernels and the host library.
+//
+// HBM layout (all arrays 256-B aligned allocations):
+//   patches  : float4[16] per patch, control points pre-gathered patch-major
+//              (256 B per patch; one patch test reads one contiguous block
+//              instead of 16 index loads + 16 gathers)
+//   nodes    : 64-B BVH nodes of every surface BVH, then the top-level BVH;
+//              child offsets, leaf `first` slots and hash-table entries are
+//              absolute (rebased at upload)
+//   indices  : leaf element lists ([id, EMPTY] per leaf), ids local to the BVH
+//   displ    : perfect-hash displacement tables, concatenated
+//   tables   : perfect-hash node tables, concatenated
+//   surfaces : one DevSurface per surface;  objects : one DevObject per object
+//
+#pragma once
+
+#include <stdint.h>
+
+#include "cgs_device.hpp"
+
+namespace cgs
+{
+
+struct DevBVH
+{
+    uint32_t root;        // absolute node offset of the root
+    uint32_t displBase;   // offset into displ
+    uint32_t tableBase;   // offset into tables
+    uint32_t A, B, mask;
+    uint32_t T;           // table size
+    uint32_t patchBase;   // first patch of the surface (surfaces only)
+    uint64_t Tm;          // fastmod multiplier for T
+    uint64_t pad;
+};
+static_assert(sizeof(DevBVH) == 48, "DevBVH layout");
+
+struct DevObject
+{
+    float4 w2l[4];        // world→local, column-major
+    float4 mat0;          // diffuse rgb, metalness
+    float4 mat1;          // specular rgb, roughness
+    uint32_t surface;
+    uint32_t pad[3];
+};
+static_assert(sizeof(DevObject) == 112, "DevObject layout");
+
+struct DevLight
+{
+    float4 a;             // color rgb, strength
+    float4 b;             // position xyz, range
+    float4 c;             // direction xyz, angle
+};
+
+struct SceneDev
+{
+    const float4* __restrict__ patches;
+    const float4* __restrict__ nodes;
+    const uint32_t* __restrict__ indices;
+    const uint32_t* __restrict__ displ;
+    const uint32_t* __restrict__ tables;
+    const DevBVH* __restrict__ surfaces;
+    const DevObject* __restrict__ objects;
+    const DevLight* __restrict__ lights;
+    uint32_t nlight;
+    uint32_t nobj;
+    uint32_t nodeCount;
+    DevBVH top;
+    float background[3];
+    float ambient[3];
+};
+
+CGS_DEV BVHRef bvhRef(const SceneDev& sc, const DevBVH& d)
+{
+    BVHRef b;
+    b.nodes = sc.nodes;
+    b.indices = sc.indices;
+    b.table = sc.tables + d.tableBase;
+    b.displ = sc.displ + d.displBase;
+    b.root = d.root;
+    b.A = d.A;
+    b.B = d.B;
+    b.mask = d.mask;
+    b.T = {d.Tm, d.T};
+    b.nodeCount = sc.nodeCount;
+    return b;
+}
+
+} // namespace cgs

@@ -1,0 +1,603 @@
+// SPDX-License-Identifier: MIT
+//
+// Ray-query and render kernels for gfx950.
+//
+// One ray per lane; 256-lane workgroups (4 waves).  A lane runs the reference's
+// complete per-ray algorithm — top-level hashed traversal, per-object local
+// ray, per-object hashed traversal, Bézier clipping at each leaf — so every
+// result equals the CPU restatement's bit for bit.
+//
+#include <hip/hip_runtime.h>
+
+#include "cgs_internal.hpp"
+#include "cgs_scene_dev.hpp"
+
+namespace cgs
+{
+
+constexpr float kClipTol = 0x1p-12f; // Spline.h:493
+
+struct LocalRay
+{
+    V3 o, d;
+    float max, dinv;
+};
+
+// RayTracer.cu:379-389: world ray -> object-local ray; t scales by |M d|.
+CGS_DEV LocalRay toLocal(const DevObject& ob, V3 o, V3 d, float tmax)
+{
+    M4 M;
+    M.c[0] = ob.w2l[0]; M.c[1] = ob.w2l[1]; M.c[2] = ob.w2l[2]; M.c[3] = ob.w2l[3];
+    LocalRay l;
+    l.o = project4(m4mul(M, make_float4(o.x, o.y, o.z, 1.0f)));
+    V3 ld = m3mul(M, d);
+    float len = len3(ld);
+    l.max = tmax * len;
+    float dinv = 1.0f / len;
+    l.d = mul3(dinv, ld);
+    l.dinv = dinv;
+    return l;
+}
+
+struct ClosestHit
+{
+    float t, u, v;
+    uint32_t patch;
+    int32_t object;
+};
+
+// Context::intersect(hit, ray) (RayTracer.cu:369-423) with
+// rt::intersect(BezierSurface, hit, ray) (Primitive.cu:355-381) at the leaves.
+CGS_DEV ClosestHit sceneClosest(const SceneDev& sc, V3 o, V3 d, float tmax, ClipStack& stk, bool& overflow)
+{
+    ClosestHit h {tmax, 0.0f, 0.0f, EMPTY, -1};
+    const BVHRef top = bvhRef(sc, sc.top);
+    hashIntersect(top, o, d, h.t, [&](uint32_t i) -> bool {
+        const DevObject& ob = sc.objects[i];
+        LocalRay lr = toLocal(ob, o, d, tmax);
+        const DevBVH surf = sc.surfaces[ob.surface];
+        const BVHRef b = bvhRef(sc, surf);
+        const ClipRay cr = makeClipRay(lr.o, versor3(lr.d));
+        float lt = lr.max, lu = 0.0f, lv = 0.0f;
+        uint32_t lp = EMPTY;
+        bool hit = hashIntersect(b, lr.o, lr.d, lt, [&](uint32_t pid) -> bool {
+            const float4* S = sc.patches + 16 * (size_t)(surf.patchBase + pid);
+            bool f = clipPatch<false>(S, cr, kClipTol, lt, lu, lv, stk, overflow);
+            if (f) lp = pid;
+            return f;
+        });
+        if (hit)
+        {
+            float t = lt * lr.dinv;
+            if (t < h.t)
+            {
+                h.t = t;
+                h.u = lu;
+                h.v = lv;
+                h.patch = lp;
+                h.object = (int32_t)i;
+                return true;
+            }
+        }
+        return false;
+    });
+    return h;
+}
+
+// Context::intersect(ray) (RayTracer.cu:425-457) with
+// rt::intersect(BezierSurface, ray) (Primitive.cu:383-401) at the leaves.
+CGS_DEV bool sceneAny(const SceneDev& sc, V3 o, V3 d, float tmax, ClipStack& stk, bool& overflow)
+{
+    const BVHRef top = bvhRef(sc, sc.top);
+    return hashIntersectAny(top, o, d, tmax, [&](uint32_t i) -> bool {
+        const DevObject& ob = sc.objects[i];
+        LocalRay lr = toLocal(ob, o, d, tmax);
+        const DevBVH surf = sc.surfaces[ob.surface];
+        const BVHRef b = bvhRef(sc, surf);
+        const ClipRay cr = makeClipRay(lr.o, versor3(lr.d));
+        return hashIntersectAny(b, lr.o, lr.d, lr.max, [&](uint32_t pid) -> bool {
+            const float4* S = sc.patches + 16 * (size_t)(surf.patchBase + pid);
+            float best = lr.max, bu, bv;
+            return clipPatch<true>(S, cr, kClipTol, best, bu, bv, stk, overflow);
+        });
+    });
+}
+
+// Wave-aggregated counter increment: one atomic per wave.
+CGS_DEV void countWave(unsigned long long* ctr, bool pred)
+{
+    unsigned long long m = __ballot(pred);
+    if (m && (threadIdx.x & 63) == (uint32_t)__builtin_ctzll(m))
+        atomicAdd(ctr, (unsigned long long)__popcll(m));
+}
+
+__global__ __launch_bounds__(256) void kTraceClosest(SceneDev sc, const cgs_ray* __restrict__ rays, uint32_t n,
+                                                     cgs_hit* __restrict__ hits, int32_t* __restrict__ objects,
+                                                     unsigned long long* __restrict__ overflowCount)
+{
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    bool overflow = false;
+    if (r < n)
+    {
+        const cgs_ray ray = rays[r];
+        ClipStack stk;
+        ClosestHit h = sceneClosest(sc, {ray.ox, ray.oy, ray.oz}, {ray.dx, ray.dy, ray.dz}, ray.tmax, stk, overflow);
+        cgs_hit out;
+        out.t = h.t;
+        out.u = h.object < 0 ? 0.0f : h.u;
+        out.v = h.object < 0 ? 0.0f : h.v;
+        out.patch = overflow ? CGS_PATCH_OVERFLOW : (h.object < 0 ? EMPTY : h.patch);
+        hits[r] = out;
+        if (objects) objects[r] = h.object;
+    }
+    countWave(overflowCount, overflow);
+}
+
+__global__ __launch_bounds__(256) void kTraceAny(SceneDev sc, const cgs_ray* __restrict__ rays, uint32_t n,
+                                                 uint32_t* __restrict__ occluded,
+                                                 unsigned long long* __restrict__ overflowCount)
+{
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    bool overflow = false;
+    if (r < n)
+    {
+        const cgs_ray ray = rays[r];
+        ClipStack stk;
+        bool occ = sceneAny(sc, {ray.ox, ray.oy, ray.oz}, {ray.dx, ray.dy, ray.dz}, ray.tmax, stk, overflow);
+        occluded[r] = overflow ? 2u : (occ ? 1u : 0u);
+    }
+    countWave(overflowCount, overflow);
+}
+
+// Batched doBezierClipping: ray r against patch r (Spline_BezierClipping.cu:46-114).
+__global__ __launch_bounds__(256) void kClipBatch(const cgs_ray* __restrict__ rays, const float4* __restrict__ patches,
+                                                  uint32_t n, float tol, cgs_hit* __restrict__ hits)
+{
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const cgs_ray ray = rays[r];
+    ClipStack stk;
+    bool overflow = false;
+    const ClipRay cr = makeClipRay({ray.ox, ray.oy, ray.oz}, versor3({ray.dx, ray.dy, ray.dz}));
+    float t = ray.tmax, u = 0.0f, v = 0.0f;
+    bool f = clipPatch<false>(patches + 16 * (size_t)r, cr, tol, t, u, v, stk, overflow);
+    hits[r] = {t, u, v, overflow ? CGS_PATCH_OVERFLOW : (f ? 1u : EMPTY)};
+}
+
+// ---------------------------------------------------------------------------
+// Subdivision intersection, host semantics (Spline_Subdivision.cu:168-288)
+// ---------------------------------------------------------------------------
+
+// Slab test over [0, +inf) (Ds Bounds3::intersect as restated).
+CGS_DEV bool boxHit(V3 bmin, V3 bmax, V3 o, V3 d)
+{
+    float tmin = 0.0f, tmax = __builtin_inff();
+    const float b0[3] = {bmin.x, bmin.y, bmin.z}, b1[3] = {bmax.x, bmax.y, bmax.z};
+    const float oo[3] = {o.x, o.y, o.z}, dd[3] = {d.x, d.y, d.z};
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+    {
+        float inv = 1.0f / dd[i];
+        float t0 = (b0[i] - oo[i]) * inv;
+        float t1 = (b1[i] - oo[i]) * inv;
+        if (inv < 0.0f)
+        {
+            float s = t0;
+            t0 = t1;
+            t1 = s;
+        }
+        if (t0 > tmin) tmin = t0;
+        if (t1 < tmax) tmax = t1;
+        if (tmin > tmax) return false;
+    }
+    return true;
+}
+
+// Möller–Trumbore (Ds triangle::intersect as restated): t in (0, +inf).
+CGS_DEV bool triHit(V3 o, V3 d, V3 p0, V3 p1, V3 p2, float& t)
+{
+    V3 e1 = sub3(p1, p0), e2 = sub3(p2, p0);
+    V3 s1 = cross3(d, e2);
+    float det = dot3(s1, e1);
+    if (__builtin_fabsf(det) < 1.17549435e-38f) return false;
+    float inv = 1.0f / det;
+    V3 s = sub3(o, p0);
+    float b1 = dot3(s, s1) * inv;
+    if (b1 < 0.0f || b1 > 1.0f) return false;
+    V3 s2 = cross3(s, e1);
+    float b2 = dot3(d, s2) * inv;
+    if (b2 < 0.0f || b1 + b2 > 1.0f) return false;
+    float tt = dot3(e2, s2) * inv;
+    if (!(tt > 0.0f)) return false;
+    t = tt;
+    return true;
+}
+
+CGS_DEV void subpatch4(float4 (&p)[16], bool vside, float a, float b)
+{
+    float iu = 1.0f - a;
+    float vv = (b - a) / iu;
+    float ivv = 1.0f - vv;
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+    {
+        const int i0 = vside ? k : 4 * k;
+        const int st = vside ? 4 : 1;
+        slice1(p[i0].x, p[i0 + st].x, p[i0 + 2 * st].x, p[i0 + 3 * st].x, a, iu, vv, ivv);
+        slice1(p[i0].y, p[i0 + st].y, p[i0 + 2 * st].y, p[i0 + 3 * st].y, a, iu, vv, ivv);
+        slice1(p[i0].z, p[i0 + st].z, p[i0 + 2 * st].z, p[i0 + 3 * st].z, a, iu, vv, ivv);
+        slice1(p[i0].w, p[i0 + st].w, p[i0 + 2 * st].w, p[i0 + 3 * st].w, a, iu, vv, ivv);
+    }
+}
+
+CGS_DEV void projBox(const float4 (&p)[16], V3& mn, V3& mx)
+{
+    const float big = 3.40282347e+38f;
+    mn = {big, big, big};
+    mx = {-big, -big, -big};
+#pragma unroll
+    for (int i = 0; i < 16; i++)
+    {
+        V3 q = project4(p[i]);
+        if (q.x < mn.x) mn.x = q.x;
+        if (q.x > mx.x) mx.x = q.x;
+        if (q.y < mn.y) mn.y = q.y;
+        if (q.y > mx.y) mx.y = q.y;
+        if (q.z < mn.z) mn.z = q.z;
+        if (q.z > mx.z) mx.z = q.z;
+    }
+}
+
+constexpr int kSubdivStack = 64;
+
+__global__ __launch_bounds__(256) void kSubdivBatch(const cgs_ray* __restrict__ rays, const float4* __restrict__ patches,
+                                                    uint32_t n, float threshold, cgs_hit* __restrict__ hits)
+{
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const cgs_ray ray = rays[r];
+    const V3 o {ray.ox, ray.oy, ray.oz};
+    const V3 d = versor3({ray.dx, ray.dy, ray.dz});
+    const float4* S = patches + 16 * (size_t)r;
+    float best = ray.tmax, bu = 0.0f, bv = 0.0f;
+    bool found = false, overflow = false;
+    float4 sub[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) sub[i] = S[i];
+    V3 bmn, bmx;
+    projBox(sub, bmn, bmx);
+    if (boxHit(bmn, bmx, o, d))
+    {
+        float4 st[kSubdivStack]; // (umin, vmin, umax, vmax); side in the sign bit of nothing: separate array
+        uint8_t side[kSubdivStack];
+        int sp = 0;
+        st[sp] = {0.0f, 0.0f, 0.5f, 1.0f}; side[sp++] = 0;
+        st[sp] = {0.5f, 0.0f, 1.0f, 1.0f}; side[sp++] = 0;
+        while (sp > 0)
+        {
+            sp--;
+            const float4 e = st[sp];
+            const bool es = side[sp] != 0;
+#pragma unroll
+            for (int i = 0; i < 16; i++) sub[i] = S[i];
+            subpatch4(sub, true, e.y, e.w);
+            subpatch4(sub, false, e.x, e.z);
+            projBox(sub, bmn, bmx);
+            if (!boxHit(bmn, bmx, o, d)) continue;
+            float x = 0.5f * (e.x + e.z);
+            float y = 0.5f * (e.y + e.w);
+            if ((e.z - e.x) < threshold && (e.w - e.y) < threshold)
+            {
+                V3 p00 = project4(sub[0]), p10 = project4(sub[3]), p01 = project4(sub[12]), p11 = project4(sub[15]);
+                float t = 0.0f, t1 = 0.0f;
+                bool b0 = triHit(o, d, p10, p01, p00, t);
+                bool b1 = triHit(o, d, p01, p10, p11, t1);
+                if (b0)
+                {
+                    if (b1 && t1 < t) t = t1;
+                }
+                else if (b1)
+                {
+                    t = t1;
+                    b0 = true;
+                }
+                if (b0 && t < best)
+                {
+                    best = t;
+                    bu = x;
+                    bv = y;
+                    found = true;
+                }
+            }
+            else
+            {
+                if (sp + 2 > kSubdivStack)
+                {
+                    overflow = true;
+                    break;
+                }
+                if (es)
+                {
+                    st[sp] = {e.x, e.y, x, e.w}; side[sp++] = !es;
+                    st[sp] = {x, e.y, e.z, e.w}; side[sp++] = !es;
+                }
+                else
+                {
+                    st[sp] = {e.x, e.y, e.z, y}; side[sp++] = !es;
+                    st[sp] = {e.x, y, e.z, e.w}; side[sp++] = !es;
+                }
+            }
+        }
+    }
+    hits[r] = {best, bu, bv, overflow ? CGS_PATCH_OVERFLOW : (found ? 1u : EMPTY)};
+}
+
+// ---------------------------------------------------------------------------
+// Renderer (RayTracer.cu:268-618, non-Monte-Carlo path)
+// ---------------------------------------------------------------------------
+
+struct CamDev
+{
+    float4 c[4];          // camera→world columns
+    float px, py, pz;     // position
+    float tlx, tly;       // top-left pixel centre
+    float hdx, hdy;       // half pixel sizes
+};
+
+struct RenderDev
+{
+    float eps;
+    int depth;
+    uint32_t width, height;
+};
+
+// PBR (rt/PBR.h, rt/PBR.cu)
+CGS_DEV bool lightVector(float& d, V3& L, V3 P, const DevLight& lt)
+{
+    const float angle = lt.c.w;
+    if (angle < 0.01f)
+    {
+        L = {-lt.c.x, -lt.c.y, -lt.c.z};
+        d = __builtin_inff();
+        return true;
+    }
+    L = sub3({lt.b.x, lt.b.y, lt.b.z}, P);
+    d = len3(L);
+    if (d < 1e-14f || (lt.b.w > 0.0f && d > lt.b.w)) return false;
+    L = mul3(1.0f / d, L);
+    if (angle > 1.57f) return true;
+    float DL = dot3({lt.c.x, lt.c.y, lt.c.z}, L);
+    return DL < 0.0f && angle > 2.0f * acosf(DL);
+}
+CGS_DEV V3 lightColor(float d, const DevLight& lt)
+{
+    const V3 col {lt.a.x, lt.a.y, lt.a.z};
+    if (lt.c.w < 0.01f) return col;
+    float range = lt.b.w, f;
+    if (range == 0.0f)
+    {
+        f = 1.0f / d;
+        f *= f;
+    }
+    else
+    {
+        f = d / range;
+        f = 1.0f + f * (f - 2.0f);
+    }
+    return mul3(f, col);
+}
+CGS_DEV V3 schlick(V3 f0, float dotLH)
+{
+    float b = 1.0f - dotLH;
+    float b2 = b * b;
+    float s = b2 * b2 * b;
+    return {f0.x + (1.0f - f0.x) * s, f0.y + (1.0f - f0.y) * s, f0.z + (1.0f - f0.z) * s};
+}
+CGS_DEV float G1(float dotNX, float k) { return dotNX / (dotNX * (1.0f - k) + k); }
+CGS_DEV float Gsmith(float dotNL, float dotNV, float r)
+{
+    const float r1 = r + 1.0f;
+    const float k = (r1 * r1) * 0.125f;
+    return G1(dotNL, k) * G1(dotNV, k);
+}
+CGS_DEV float Dggx(float dotNH, float r)
+{
+    r = fmax_(r, 1e-3f);
+    float a = r * r;
+    float a2 = a * a;
+    float d = dotNH * dotNH * (a2 - 1.0f) + 1.0f;
+    return a2 / (0x1.921fb6p+1f * d * d);
+}
+CGS_DEV float microfacet(float dotNV, float dotNL, float dotNH, float r)
+{
+    return Gsmith(dotNL, dotNV, r) * Dggx(dotNH, r) / (4.0f * dotNL * dotNV);
+}
+
+// Ds-boundary pack_sRGB: sRGB encode, 8 bits per channel, 0xAABBGGRR.
+CGS_DEV uint32_t packSRGB(float r, float g, float b)
+{
+    auto enc = [](float c) -> uint32_t {
+        float e = c <= 0.0031308f ? 12.92f * c : 1.055f * powf(c, 1.0f / 2.4f) - 0.055f;
+        int q = (int)(e * 255.0f + 0.5f);
+        q = q < 0 ? 0 : (q > 255 ? 255 : q);
+        return (uint32_t)q;
+    };
+    return enc(r) | (enc(g) << 8) | (enc(b) << 16) | 0xFF000000u;
+}
+
+CGS_DEV float saturate(float x) { return x > 0.0f ? (x < 1.0f ? x : 1.0f) : 0.0f; }
+
+__global__ __launch_bounds__(256) void kRender(SceneDev sc, CamDev cam, RenderDev op, uint32_t* __restrict__ frame,
+                                               float* __restrict__ color, unsigned long long* __restrict__ counts)
+{
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t npx = op.width * op.height;
+    const bool active = k < npx;
+    const uint32_t i = active ? k % op.width : 0, j = active ? k / op.width : 0;
+    ClipStack stk;
+    bool overflow = false;
+    uint32_t nPrimary = active ? 1u : 0u, nShadow = 0, nSecondary = 0;
+    V3 acc {0.0f, 0.0f, 0.0f};
+    if (active)
+    {
+        // pixelRayDirection (RayTracer.cu:133-145)
+        float4 P = make_float4(cam.tlx + cam.hdx * (2.0f * (float)i), cam.tly - cam.hdy * (2.0f * (float)j), -1.0f, 1.0f);
+        M4 C;
+        C.c[0] = cam.c[0]; C.c[1] = cam.c[1]; C.c[2] = cam.c[2]; C.c[3] = cam.c[3];
+        V3 camPos {cam.px, cam.py, cam.pz};
+        V3 ro = camPos;
+        V3 rd = versor3(sub3(project4(m4mul(C, P)), camPos));
+        float rmax = __builtin_inff();
+        V3 att {1.0f, 1.0f, 1.0f};
+        int depth = op.depth;
+        bool first = true;
+        while (true)
+        {
+            if (!first) nSecondary++;
+            first = false;
+            ClosestHit h = sceneClosest(sc, ro, rd, rmax, stk, overflow);
+            if (h.object < 0)
+            {
+                acc = add3(acc, mulv3({sc.background[0], sc.background[1], sc.background[2]}, att));
+                break;
+            }
+            const DevObject& ob = sc.objects[h.object];
+            const float4 m0 = ob.mat0, m1 = ob.mat1;
+            const float metal = m0.w, rough = m1.w;
+            const V3 spec {m1.x, m1.y, m1.z};
+            const float invPi = 0x1.45f306p-2f; // std::numbers::inv_pi_v<float>
+            const V3 diffuse = mul3(1.0f - metal, mul3(invPi, {m0.x, m0.y, m0.z}));
+            V3 col = mulv3({sc.ambient[0], sc.ambient[1], sc.ambient[2]}, diffuse);
+            V3 V = neg3(rd);
+            const DevBVH surf = sc.surfaces[ob.surface];
+            V3 N = versor3(normalMat(sc.patches + 16 * (size_t)(surf.patchBase + h.patch), h.u, h.v));
+            M4 W;
+            W.c[0] = ob.w2l[0]; W.c[1] = ob.w2l[1]; W.c[2] = ob.w2l[2]; W.c[3] = ob.w2l[3];
+            N = versor3(m3tmul(W, N));
+            float dotNV = dot3(N, V);
+            if (dotNV < 0.0f)
+            {
+                N = neg3(N);
+                dotNV = -dotNV;
+            }
+            V3 Pt = add3(ro, mul3(h.t, rd));
+            for (uint32_t li = 0; li < sc.nlight; li++)
+            {
+                const DevLight lt = sc.lights[li];
+                float d;
+                V3 L;
+                if (!lightVector(d, L, Pt, lt)) continue;
+                const float dotNL = dot3(N, L);
+                if (dotNL < 1e-14f) continue;
+                nShadow++;
+                if (sceneAny(sc, add3(Pt, mul3(op.eps, L)), L, d, stk, overflow)) continue;
+                V3 H = versor3(add3(L, V));
+                float dotHN = dot3(H, N);
+                float dotHL = dot3(H, L);
+                V3 I = lightColor(d, lt);
+                V3 bs = mul3(microfacet(dotNV, dotNL, dotHN, rough), schlick(spec, dotHL));
+                col = add3(col, mulv3(mul3(dotNL, I), add3(diffuse, mul3(metal, bs))));
+            }
+            acc = add3(acc, mulv3(mul3(0x1.921fb6p+1f, col), att));
+            if (depth <= 0) break;
+            V3 refl = schlick(spec, dotNV);
+            V3 R = sub3(neg3(V), mul3(2.0f * -dotNV, N));
+            float g = Gsmith(dotNV, dotNV, rough);
+            float rr = 1.0f - rough;
+            V3 brdf = mul3(rr * rr, mul3(dotNV, add3(diffuse, mul3(metal, mul3(g, refl)))));
+            V3 a = mulv3(att, brdf);
+            if (max3(a) <= 0x1p-8f) break;
+            ro = add3(Pt, mul3(op.eps, R));
+            rd = R;
+            rmax = __builtin_inff();
+            att = a;
+            depth = depth - 1;
+        }
+        const uint32_t q = j * op.width + i;
+        if (color)
+        {
+            color[3 * (size_t)q] = acc.x;
+            color[3 * (size_t)q + 1] = acc.y;
+            color[3 * (size_t)q + 2] = acc.z;
+        }
+        frame[q] = packSRGB(saturate(acc.x), saturate(acc.y), saturate(acc.z));
+    }
+    if (counts)
+    {
+        // wave reductions of the per-lane counts, one atomic per counter per wave
+        unsigned long long v[4] = {nPrimary, nShadow, nSecondary, overflow ? 1ull : 0ull};
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+        {
+            unsigned long long s = v[c];
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+            if ((threadIdx.x & 63) == 0 && s) atomicAdd(counts + c, s);
+        }
+    }
+}
+
+} // namespace cgs
+
+// ---------------------------------------------------------------------------
+// Launchers (called by the host library)
+// ---------------------------------------------------------------------------
+namespace cgs
+{
+
+static inline uint32_t blocks(uint32_t n, uint32_t b) { return (n + b - 1) / b; }
+
+hipError_t launchTraceClosest(const SceneDev& sc, const cgs_ray* rays, uint32_t n, cgs_hit* hits, int32_t* objects,
+                              unsigned long long* overflow, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(kTraceClosest, dim3(blocks(n, 256)), dim3(256), 0, s, sc, rays, n, hits, objects, overflow);
+    return hipGetLastError();
+}
+
+hipError_t launchTraceAny(const SceneDev& sc, const cgs_ray* rays, uint32_t n, uint32_t* occluded,
+                          unsigned long long* overflow, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(kTraceAny, dim3(blocks(n, 256)), dim3(256), 0, s, sc, rays, n, occluded, overflow);
+    return hipGetLastError();
+}
+
+hipError_t launchClipBatch(const cgs_ray* rays, const float* patches, uint32_t n, float tol, cgs_hit* hits, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(kClipBatch, dim3(blocks(n, 256)), dim3(256), 0, s, rays, (const float4*)patches, n, tol, hits);
+    return hipGetLastError();
+}
+
+hipError_t launchSubdivBatch(const cgs_ray* rays, const float* patches, uint32_t n, float threshold, cgs_hit* hits,
+                             hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(kSubdivBatch, dim3(blocks(n, 256)), dim3(256), 0, s, rays, (const float4*)patches, n, threshold,
+                       hits);
+    return hipGetLastError();
+}
+
+hipError_t launchRender(const SceneDev& sc, const float* toWorld16, const float* pos3, float tlx, float tly, float hdx,
+                        float hdy, float eps, int depth, uint32_t w, uint32_t h, uint32_t* frame, float* color,
+                        unsigned long long* counts, hipStream_t s)
+{
+    CamDev cam;
+    for (int c = 0; c < 4; c++)
+        cam.c[c] = make_float4(toWorld16[4 * c], toWorld16[4 * c + 1], toWorld16[4 * c + 2], toWorld16[4 * c + 3]);
+    cam.px = pos3[0];
+    cam.py = pos3[1];
+    cam.pz = pos3[2];
+    cam.tlx = tlx;
+    cam.tly = tly;
+    cam.hdx = hdx;
+    cam.hdy = hdy;
+    RenderDev op {eps, depth, w, h};
+    uint32_t n = w * h;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(kRender, dim3(blocks(n, 256)), dim3(256), 0, s, sc, cam, op, frame, color, counts);
+    return hipGetLastError();
+}
+
+} // namespace cgs

@@ -1,0 +1,269 @@
+/*
+ * cgs.h — C ABI of the MI355X (gfx950) ray–bicubic-patch library (libcgs.so).
+ *
+ * This is the drop-in boundary for cgspline's ray–patch path.  Every entry
+ * point lists the reference interface it replaces (file:line in
+ * MachSilva/cgspline).  Plain pointers and sizes only: "device" pointers are
+ * HIP device allocations (hipMalloc, or any allocator's device memory), host
+ * pointers are ordinary memory.  Streams are hipStream_t passed as void*.
+ *
+ * Errors: every call returns a cgs_status; nothing aborts.  The C++ facade
+ * (include/cgspline/) maps the codes back to the reference's behaviour
+ * (throws, int returns, warnings).
+ */
+#ifndef CGS_H
+#define CGS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CGS_ABI_VERSION 1
+
+typedef enum cgs_status
+{
+    CGS_OK = 0,
+    CGS_ERR_INVALID = 1,        /* bad argument (null pointer, size 0 where not allowed, ...) */
+    CGS_ERR_HIP = 2,            /* a HIP runtime call failed; see cgs_last_error() */
+    CGS_ERR_HASH_FAILED = 3,    /* perfect hash: all (A,B) attempts failed (reference: runtime_error) */
+    CGS_ERR_HASH_DUPLICATE = 4, /* perfect hash: final function not injective (reference: build() == 1) */
+    CGS_ERR_BUCKET_OVERFLOW = 5,/* perfect hash: a bucket holds more than 4 keys (reference: __trap) */
+    CGS_ERR_BUSY = 6,           /* render already in flight on this context (reference: warning, no-op) */
+    CGS_ERR_EMPTY_SPLIT = 7,    /* BVH build over an empty range (reference: logic_error("invalid")) */
+    CGS_ERR_STACK_OVERFLOW = 8, /* a Bézier-clipping region stack exceeded CGS_CLIP_STACK entries */
+    CGS_ERR_NO_DEVICE = 9,      /* no HIP device */
+    CGS_ERR_OUT_OF_MEMORY = 10
+} cgs_status;
+
+/* Region-stack capacity of the GPU Bézier clipper (the reference's device
+ * variant traps at 32 entries, Spline_BezierClipping.cu:736-737). */
+#define CGS_CLIP_STACK 32
+
+/* Hit patch id reported for a ray whose clipping stack overflowed. */
+#define CGS_PATCH_OVERFLOW 0xFFFFFFFEu
+#define CGS_EMPTY 0xFFFFFFFFu
+
+/* ---------------------------------------------------------------------- */
+/* Plain-old-data records                                                  */
+/* ---------------------------------------------------------------------- */
+
+/* 32-byte ray: origin, max range, direction (rt/Intersection.h:16-24 Ray). */
+typedef struct cgs_ray
+{
+    float ox, oy, oz, tmax;
+    float dx, dy, dz, pad;
+} cgs_ray;
+
+/* 16-byte closest hit: t, patch (u,v), patch id (CGS_EMPTY on a miss)
+ * (rt/Intersection.h:28-34 Intersection: t, coordinates, index). */
+typedef struct cgs_hit
+{
+    float t, u, v;
+    uint32_t patch;
+} cgs_hit;
+
+/* 64-byte BVH node, the reference's rt::BVH::Node layout (rt/BVH.h:23-39):
+ * child boxes, then the left/right node offsets; a leaf has left = CGS_EMPTY
+ * and right = first element slot in the index array. */
+typedef struct cgs_bvh_node
+{
+    float lmin[3], lmax[3];
+    float rmin[3], rmax[3];
+    uint32_t left, right;
+    uint32_t pad[2];
+} cgs_bvh_node;
+
+/* BVH element (rt/BVH.h:41-46 ElementData). */
+typedef struct cgs_bvh_element
+{
+    float bmin[3], bmax[3];
+    float centroid[3];
+    uint32_t index;
+} cgs_bvh_element;
+
+/* Perfect hash function parameters (rt/PerfectHashFunction.h:30-70):
+ * h(k) = (A·k + B + D[k & mask]) mod table_size in 32-bit arithmetic. */
+typedef struct cgs_phf
+{
+    uint32_t A, B, mask, table_size;
+    uint32_t attempts;      /* (A,B) pairs tried */
+    uint32_t displacement_size;
+} cgs_phf;
+
+/* Scene description (host memory), the content of rt::Scene
+ * (rt/Scene.h:97-159) as MainWindow::convertScene fills it
+ * (MainWindow.cpp:363-492).  Vertices of all surfaces are concatenated;
+ * patch indices are relative to the owning surface's first vertex;
+ * point(i,j) of a patch is its index [4j+i] (Spline.h:137-141). */
+typedef struct cgs_scene_desc
+{
+    const float* vertices;        /* nv × 4 (x, y, z, w) */
+    const uint32_t* indices;      /* npatch × 16 */
+    const uint32_t* surf_vbase;   /* nsurf + 1, prefix over vertices */
+    const uint32_t* surf_pbase;   /* nsurf + 1, prefix over patches */
+    uint32_t nsurf;
+    const uint32_t* obj_surface;  /* nobj: surface of each object */
+    const float* obj_local2world; /* nobj × 16, column-major */
+    const float* obj_world2local; /* nobj × 16, column-major */
+    const float* obj_material;    /* nobj × 8: diffuse rgb, specular rgb, metalness, roughness */
+    uint32_t nobj;
+    const float* lights;          /* nlight × 12: color rgb, strength, position xyz, range, direction xyz, angle */
+    uint32_t nlight;
+    float background[3];
+    float ambient[3];
+    uint32_t hash_seed;           /* PerfectHashFunction seed (reference default 42) */
+} cgs_scene_desc;
+
+/* Pinhole camera (rt/Scene.h:29-45 Camera; RayTracer.cu:207-231). */
+typedef struct cgs_camera
+{
+    float to_world[16];   /* camera→world, column-major (Transform::matrix()) */
+    float position[3];
+    float fovy;           /* radians */
+    int32_t flip_y;
+} cgs_camera;
+
+/* RayTracer::Options (rt/RayTracer.h:26-34), non-Monte-Carlo path. */
+typedef struct cgs_render_options
+{
+    float eps;             /* secondary-ray offset, 1e-4 */
+    int32_t recursion_depth;
+} cgs_render_options;
+
+/* Per-launch ray counters written by cgs_render (device memory, 4 × u64):
+ * primary, shadow, secondary, overflowed clip stacks. */
+typedef struct cgs_ray_counts
+{
+    uint64_t primary, shadow, secondary, overflow;
+} cgs_ray_counts;
+
+typedef struct cgs_ctx cgs_ctx;
+typedef struct cgs_scene cgs_scene;
+
+/* ---------------------------------------------------------------------- */
+/* Context                                                                 */
+/* ---------------------------------------------------------------------- */
+
+int cgs_abi_version(void);
+const char* cgs_status_string(int status);
+/* Thread-local description of the last error on this thread. */
+const char* cgs_last_error(void);
+
+/* Opens device `device` (RayTracer::Options::device, rt/RayTracer.h:32). */
+cgs_status cgs_open(int device, cgs_ctx** out);
+cgs_status cgs_close(cgs_ctx* ctx);
+
+/* ---------------------------------------------------------------------- */
+/* Spline intersection entry points (Spline.h:488-508)                     */
+/* ---------------------------------------------------------------------- */
+
+/* spline::doBezierClipping (Spline.h:488-493, Spline_BezierClipping.cu:46-114)
+ * batched: ray r against patch r.  rays: n cgs_ray (device), directions are
+ * normalized as Ray3f does; patches: n × 16 float4 control points (device);
+ * hits: n cgs_hit (device), t starts at ray.tmax, patch = 1 when found else
+ * CGS_EMPTY (CGS_PATCH_OVERFLOW when the region stack overflowed).  Host
+ * (exact-region) semantics of the reference's CPU path. */
+cgs_status cgs_bezier_clip(cgs_ctx* ctx, const cgs_ray* rays, const float* patches, uint32_t n,
+                           float tol, cgs_hit* hits, void* stream);
+
+/* spline::doSubdivision (Spline.h:495-500, Spline_Subdivision.cu:168-288),
+ * batched like cgs_bezier_clip; `threshold` is the parametric size below
+ * which a region is reported (reference default 2^-4). */
+cgs_status cgs_subdivision(cgs_ctx* ctx, const cgs_ray* rays, const float* patches, uint32_t n,
+                           float threshold, cgs_hit* hits, void* stream);
+
+/* ---------------------------------------------------------------------- */
+/* BVH and perfect hash                                                    */
+/* ---------------------------------------------------------------------- */
+
+/* rt::BVH::build (rt/BVH.h:59, rt/BVH.cpp:9-108) on the host.  The library
+ * allocates *nodes (node_count), *keys (node_count) and *indices
+ * (index_count) with malloc; release them with cgs_free_host. */
+cgs_status cgs_bvh_build(const cgs_bvh_element* elements, uint32_t n, uint32_t elements_per_node,
+                         cgs_bvh_node** nodes, uint32_t** keys, uint32_t* node_count,
+                         uint32_t** indices, uint32_t* index_count);
+/* BezierSurface::buildBVH element data (rt/Primitive.cu:332-353): bounds and
+ * centroid of the projected control points of each patch.  vertices: nv × 4
+ * host floats; indices: npatch × 16.  elements: npatch entries (caller). */
+cgs_status cgs_patch_elements(const float* vertices, const uint32_t* indices, uint32_t npatch,
+                              cgs_bvh_element* elements);
+void cgs_free_host(void* p);
+
+/* PerfectHashFunction::build (rt/PerfectHashFunction.h:53, .cu:302-576) on
+ * the GPU, deterministic.  keys: n device u32; displacement: device u32
+ * buffer of cgs_phf_displacement_size(n) entries (written); out: parameters.
+ * Returns CGS_ERR_HASH_FAILED / _DUPLICATE / _BUCKET_OVERFLOW like the
+ * reference's non-zero return. */
+uint32_t cgs_phf_displacement_size(uint32_t n);
+cgs_status cgs_phf_build(cgs_ctx* ctx, const uint32_t* keys, uint32_t n, uint32_t seed,
+                         uint32_t* displacement, cgs_phf* out, void* stream);
+/* BVH::buildHashTable's table fill (rt/BVH.cpp:30-37): table[h(keys[i])] = i,
+ * CGS_EMPTY elsewhere.  table: out->table_size device u32. */
+cgs_status cgs_phf_fill_table(cgs_ctx* ctx, const cgs_phf* h, const uint32_t* displacement,
+                              const uint32_t* keys, uint32_t n, uint32_t* table, void* stream);
+/* PerfectHashFunction::hash (rt/PerfectHashFunction.h:58-63) for n keys. */
+cgs_status cgs_phf_hash(cgs_ctx* ctx, const cgs_phf* h, const uint32_t* displacement,
+                        const uint32_t* keys, uint32_t n, uint32_t* out, void* stream);
+
+/* ---------------------------------------------------------------------- */
+/* Scenes (rt::Scene + MainWindow::convertScene)                            */
+/* ---------------------------------------------------------------------- */
+
+/* Builds every surface BVH (host), the top-level BVH over object world
+ * bounds (Scene::buildBVH, rt/Scene.cu:218-232), uploads the scene and
+ * builds every perfect hash and table on the GPU (BVH::buildHashTable,
+ * rt/BVH.cpp:24-38).  The desc's host arrays are copied. */
+cgs_status cgs_scene_create(cgs_ctx* ctx, const cgs_scene_desc* desc, cgs_scene** out);
+cgs_status cgs_scene_destroy(cgs_scene* scene);
+
+/* Sizes and copies (host memory) of one surface's BVH/hash (surface >= 0) or
+ * of the top-level BVH (surface = -1), for inspection and tests. */
+cgs_status cgs_scene_bvh_info(cgs_scene* scene, int32_t surface, uint32_t* node_count,
+                              uint32_t* index_count, cgs_phf* hash);
+cgs_status cgs_scene_bvh_copy(cgs_scene* scene, int32_t surface, cgs_bvh_node* nodes, uint32_t* keys,
+                              uint32_t* indices, uint32_t* displacement, uint32_t* table);
+
+/* Closest hit of n world rays (RayTracer::Context::intersect(hit, ray),
+ * RayTracer.cu:369-423: top-level hashIntersect → per-object local ray →
+ * BezierSurface hashIntersect → doBezierClipping).  rays, hits, objects
+ * (nullable, int32 object id or -1) are device pointers. */
+cgs_status cgs_trace_closest(cgs_ctx* ctx, cgs_scene* scene, const cgs_ray* rays, uint32_t n,
+                             cgs_hit* hits, int32_t* objects, void* stream);
+/* Occlusion of n world rays up to ray.tmax (Context::intersect(ray),
+ * RayTracer.cu:425-457).  occluded: n device u32 (1 / 0). */
+cgs_status cgs_trace_any(cgs_ctx* ctx, cgs_scene* scene, const cgs_ray* rays, uint32_t n,
+                         uint32_t* occluded, void* stream);
+
+/* RayTracer::render (rt/RayTracer.h:42-43, RayTracer.cu:181-254 and the
+ * render kernel :268-618, non-Monte-Carlo): camera rays through integer
+ * pixel centres, closest hit, PBR shading with one shadow ray per light and
+ * the reflection bounce loop.  frame: width*height packed pixels (device,
+ * row stride = width); color: optional width*height*3 floats (device,
+ * unclamped radiance); counts: optional device cgs_ray_counts (accumulated).
+ * Asynchronous on `stream`; CGS_ERR_BUSY while a previous render of this
+ * context is still running (RayTracer.cu:184-190). */
+cgs_status cgs_render(cgs_ctx* ctx, cgs_scene* scene, const cgs_camera* camera,
+                      const cgs_render_options* options, uint32_t width, uint32_t height,
+                      uint32_t* frame, float* color, cgs_ray_counts* counts, void* stream);
+/* RayTracer::running() (rt/RayTracer.h:45-48): 1 while the last render runs. */
+int cgs_render_running(cgs_ctx* ctx);
+
+/* ---------------------------------------------------------------------- */
+/* Device memory helpers (for callers without their own allocator)         */
+/* ---------------------------------------------------------------------- */
+
+cgs_status cgs_malloc(cgs_ctx* ctx, size_t bytes, void** dptr);
+cgs_status cgs_free(cgs_ctx* ctx, void* dptr);
+cgs_status cgs_memcpy_h2d(cgs_ctx* ctx, void* dst, const void* src, size_t bytes, void* stream);
+cgs_status cgs_memcpy_d2h(cgs_ctx* ctx, void* dst, const void* src, size_t bytes, void* stream);
+cgs_status cgs_memset(cgs_ctx* ctx, void* dst, int value, size_t bytes, void* stream);
+cgs_status cgs_stream_sync(cgs_ctx* ctx, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CGS_H */

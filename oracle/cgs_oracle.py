@@ -1,0 +1,303 @@
+"""ctypes wrapper of the oracle library — TEST INFRASTRUCTURE ONLY.
+
+The oracle is a CPU restatement of the reference's ray–patch path
+(cgs_oracle.hpp).  Only tests/, ``__graft_entry__.smoke()`` and bench.py's
+cpu_baseline leg use it, as the checker; the product never does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle_cgs.so")
+
+_P = C.c_void_p
+_U32, _U64, _I32, _F32, _F64 = C.c_uint32, C.c_uint64, C.c_int32, C.c_float, C.c_double
+
+
+class Work(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in (
+        "nodes", "leaves", "hashProbes", "patchTests", "clipIters", "subdivs", "prunes",
+        "converged", "onHits", "maxStack")]
+
+    def asdict(self):
+        return {n: int(getattr(self, n)) for n, _ in self._fields_}
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [
+        ("vertices", _P), ("indices", _P), ("surf_vbase", _P), ("surf_pbase", _P), ("nsurf", _U32),
+        ("obj_surface", _P), ("obj_local2world", _P), ("obj_world2local", _P), ("obj_material", _P),
+        ("nobj", _U32), ("lights", _P), ("nlight", _U32),
+        ("background", _F32 * 3), ("ambient", _F32 * 3), ("seed", _U32)]
+
+
+class CameraDesc(C.Structure):
+    _fields_ = [("toWorld", _F32 * 16), ("position", _F32 * 3), ("fovy", _F32), ("flipY", C.c_int)]
+
+
+def build(force: bool = False) -> None:
+    """Compiles the oracle library with its Makefile."""
+    if force or not os.path.exists(LIB_PATH):
+        subprocess.run(["make", "-s", "-C", _HERE] + (["-B"] if force else []), check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        L.orc_clip_batch.argtypes = [C.c_int, _P, _P, _U64, _F32, _P, _P, _P, C.c_int, _F32]
+        L.orc_clip2d_candidates.argtypes = [_P, _F32, _P, C.c_int, _P]
+        L.orc_eval.argtypes = [C.c_int, _P, _F64, _F64, C.c_int, _P]
+        L.orc_subpatch2d.argtypes = [_P, C.c_int, _F32, _F32]
+        L.orc_slice_matrix.argtypes = [_P, _P]
+        L.orc_x_axis_intersection.argtypes = [_P, _P, _P]
+        L.orc_triangle_origin.argtypes = [_P, _P, _P, _P]
+        L.orc_fminf.argtypes = [_F32, _F32]
+        L.orc_fminf.restype = _F32
+        L.orc_fmaxf.argtypes = [_F32, _F32]
+        L.orc_fmaxf.restype = _F32
+        L.orc_newton.argtypes = [_P, _P, _P, _P, C.c_int]
+        L.orc_bvh_build_patches.argtypes = [_P, _P, _U32]
+        L.orc_bvh_build_patches.restype = _P
+        L.orc_bvh_build_elements.argtypes = [_P, _U32, _U32]
+        L.orc_bvh_build_elements.restype = _P
+        L.orc_bvh_sizes.argtypes = [_P, _P, _P]
+        L.orc_bvh_copy.argtypes = [_P, _P, _P, _P]
+        L.orc_bvh_free.argtypes = [_P]
+        L.orc_phf_displacement_size.argtypes = [_U64]
+        L.orc_phf_displacement_size.restype = _U32
+        L.orc_phf_build.argtypes = [_P, _U64, _U32, _P, _P]
+        L.orc_scene_create.argtypes = [C.POINTER(SceneDesc), C.POINTER(C.c_int)]
+        L.orc_scene_create.restype = _P
+        L.orc_scene_destroy.argtypes = [_P]
+        L.orc_scene_bvh_sizes.argtypes = [_P, C.c_int, _P, _P, _P, _P]
+        L.orc_scene_bvh_copy.argtypes = [_P, C.c_int, _P, _P, _P, _P, _P, _P]
+        L.orc_scene_trace.argtypes = [_P, _P, _U64, C.c_int, _P, _P, _P, _P, C.c_int]
+        L.orc_scene_render.argtypes = [_P, C.POINTER(CameraDesc), _F32, C.c_int, _U32, _U32, _P, _P, _P, _P,
+                                       C.c_int, _U32, _U32]
+        L.orc_cpu_render.argtypes = [_P, C.POINTER(CameraDesc), _F32, C.c_int, _U32, _U32, _P, _P, C.c_int,
+                                     _U32, _U32, _P]
+        L.orc_cpu_render.restype = _F64
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+# ---------------------------------------------------------------------------
+# per-patch
+# ---------------------------------------------------------------------------
+
+def clip_batch(rays, patches, tol=2.0 ** -12, fp64=False, method=0, threshold=2.0 ** -4):
+    """rays (n,8) f32, patches (n,16,4) f32 → (n,4) results (t,u,v,found), Work."""
+    rays = np.ascontiguousarray(rays, np.float32)
+    patches = np.ascontiguousarray(patches, np.float32)
+    n = rays.shape[0]
+    w = Work()
+    if fp64:
+        out = np.zeros((n, 4), np.float64)
+        lib().orc_clip_batch(1, _ptr(rays), _ptr(patches), n, tol, None, _ptr(out), C.byref(w), method, threshold)
+    else:
+        out = np.zeros((n, 4), np.float32)
+        lib().orc_clip_batch(0, _ptr(rays), _ptr(patches), n, tol, _ptr(out), None, C.byref(w), method, threshold)
+    return out, w.asdict()
+
+
+def clip2d_candidates(p2d, tol=2.0 ** -12, cap=256):
+    p2d = np.ascontiguousarray(p2d, np.float32).reshape(16, 2)
+    out = np.zeros((cap, 2), np.float32)
+    w = Work()
+    k = lib().orc_clip2d_candidates(_ptr(p2d), tol, _ptr(out), cap, C.byref(w))
+    return out[:min(k, cap)], w.asdict()
+
+
+EVAL_KINDS = {"interp_dc": 0, "interp_mat": 1, "du_dc": 2, "dv_dc": 3, "du_mat": 4, "dv_mat": 5,
+              "normal_dc": 6, "normal_mat": 7}
+
+
+def evaluate(patch, u, v, kind, fp64=False):
+    p = np.ascontiguousarray(patch, np.float32).reshape(16, 4)
+    out = np.zeros(4, np.float64)
+    rc = lib().orc_eval(int(fp64), _ptr(p), float(u), float(v), EVAL_KINDS[kind], _ptr(out))
+    assert rc == 0
+    return out
+
+
+def subpatch2d(p2d, side, a, b):
+    p = np.array(p2d, np.float32).reshape(16, 2).copy()
+    lib().orc_subpatch2d(_ptr(p), side, a, b)
+    return p
+
+
+def slice_matrix(a, b):
+    ab = np.array([a, b], np.float32)
+    m = np.zeros(16, np.float32)
+    lib().orc_slice_matrix(_ptr(ab), _ptr(m))
+    return m.reshape(4, 4)   # rows are columns of the reference's mat4
+
+
+def x_axis_intersection(A, B):
+    A = np.asarray(A, np.float32)
+    B = np.asarray(B, np.float32)
+    out = np.zeros(2, np.float32)
+    ok = lib().orc_x_axis_intersection(_ptr(A), _ptr(B), _ptr(out))
+    return bool(ok), out
+
+
+def triangle_origin(P, A, B):
+    P, A, B = (np.asarray(x, np.float32) for x in (P, A, B))
+    out = np.zeros(2, np.float32)
+    ok = lib().orc_triangle_origin(_ptr(P), _ptr(A), _ptr(B), _ptr(out))
+    return bool(ok), out
+
+
+def fminf(a, b):
+    return lib().orc_fminf(a, b)
+
+
+def fmaxf(a, b):
+    return lib().orc_fmaxf(a, b)
+
+
+def newton(patch, o, d, uvt, iters=30):
+    p = np.ascontiguousarray(patch, np.float32).reshape(16, 4)
+    o = np.asarray(o, np.float64)
+    d = np.asarray(d, np.float64)
+    x = np.array(uvt, np.float64)
+    ok = lib().orc_newton(_ptr(p), _ptr(o), _ptr(d), _ptr(x), iters)
+    return bool(ok), x
+
+
+# ---------------------------------------------------------------------------
+# BVH and hash
+# ---------------------------------------------------------------------------
+
+def _bvh_fetch(h):
+    nn, ni = C.c_uint64(), C.c_uint64()
+    lib().orc_bvh_sizes(h, C.byref(nn), C.byref(ni))
+    nodes = np.zeros((nn.value, 16), np.uint32)
+    keys = np.zeros(nn.value, np.uint32)
+    idx = np.zeros(ni.value, np.uint32)
+    lib().orc_bvh_copy(h, _ptr(nodes), _ptr(keys), _ptr(idx))
+    lib().orc_bvh_free(h)
+    return nodes, keys, idx
+
+
+def bvh_build_patches(vertices, indices):
+    v = np.ascontiguousarray(vertices, np.float32)
+    i = np.ascontiguousarray(indices, np.uint32)
+    h = lib().orc_bvh_build_patches(_ptr(v), _ptr(i), i.shape[0])
+    assert h
+    return _bvh_fetch(h)
+
+
+def bvh_build_elements(elements10, elements_per_node=1):
+    e = np.ascontiguousarray(elements10, np.float32)
+    h = lib().orc_bvh_build_elements(_ptr(e), e.shape[0], elements_per_node)
+    if not h:
+        raise ValueError("invalid")
+    return _bvh_fetch(h)
+
+
+def phf_build(keys, seed=42):
+    keys = np.ascontiguousarray(keys, np.uint32)
+    p = np.zeros(5, np.uint32)
+    D = np.zeros(lib().orc_phf_displacement_size(keys.shape[0]), np.uint32)
+    rc = lib().orc_phf_build(_ptr(keys), keys.shape[0], seed, _ptr(p), _ptr(D))
+    return rc, dict(A=int(p[0]), B=int(p[1]), mask=int(p[2]), table_size=int(p[3]), attempts=int(p[4])), D
+
+
+# ---------------------------------------------------------------------------
+# scenes
+# ---------------------------------------------------------------------------
+
+class OracleScene:
+    """The oracle's copy of a scene: BVHs, hashes and tables built on the CPU."""
+
+    def __init__(self, scene):
+        self._keep = [np.ascontiguousarray(a) for a in (
+            scene.vertices, scene.indices, scene.surf_vbase, scene.surf_pbase, scene.obj_surface,
+            scene.obj_local2world, scene.obj_world2local, scene.obj_material, scene.lights)]
+        k = self._keep
+        d = SceneDesc(_ptr(k[0]), _ptr(k[1]), _ptr(k[2]), _ptr(k[3]), scene.nsurf,
+                      _ptr(k[4]), _ptr(k[5]), _ptr(k[6]), _ptr(k[7]), scene.nobj,
+                      _ptr(k[8]), k[8].shape[0], (_F32 * 3)(*scene.background), (_F32 * 3)(*scene.ambient),
+                      scene.hash_seed)
+        st = C.c_int(0)
+        self.h = lib().orc_scene_create(C.byref(d), C.byref(st))
+        self.status = st.value
+        if not self.h:
+            raise RuntimeError(f"oracle scene build failed (status {st.value})")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_scene_destroy(self.h)
+            self.h = None
+
+    def bvh(self, surface):
+        nn, ni, nd, nt = C.c_uint64(), C.c_uint64(), C.c_uint64(), C.c_uint64()
+        lib().orc_scene_bvh_sizes(self.h, surface, C.byref(nn), C.byref(ni), C.byref(nd), C.byref(nt))
+        nodes = np.zeros((nn.value, 16), np.uint32)
+        keys = np.zeros(nn.value, np.uint32)
+        idx = np.zeros(ni.value, np.uint32)
+        p = np.zeros(5, np.uint32)
+        D = np.zeros(nd.value, np.uint32)
+        T = np.zeros(nt.value, np.uint32)
+        lib().orc_scene_bvh_copy(self.h, surface, _ptr(nodes), _ptr(keys), _ptr(idx), _ptr(p), _ptr(D), _ptr(T))
+        return dict(nodes=nodes, keys=keys, indices=idx, A=int(p[0]), B=int(p[1]), mask=int(p[2]),
+                    table_size=int(p[3]), attempts=int(p[4]), displacement=D, table=T)
+
+    def trace(self, rays, any_hit=False, threads=None):
+        rays = np.ascontiguousarray(rays, np.float32)
+        n = rays.shape[0]
+        hits = np.zeros((n, 3), np.float32)
+        patch = np.zeros(n, np.uint32)
+        obj = np.zeros(n, np.int32)
+        w = Work()
+        th = threads or os.cpu_count() or 1
+        lib().orc_scene_trace(self.h, _ptr(rays), n, int(any_hit), _ptr(hits), _ptr(patch), _ptr(obj), C.byref(w), th)
+        if any_hit:
+            return obj.astype(bool), w.asdict()
+        return hits, patch, obj, w.asdict()
+
+    @staticmethod
+    def _cam(cam):
+        return CameraDesc((_F32 * 16)(*cam.to_world), (_F32 * 3)(*cam.position), cam.fovy, int(cam.flip_y))
+
+    def render(self, cam, eps=1e-4, depth=0, rows=None, threads=None):
+        """GPU-renderer semantics; returns (color (h,w,3), frame (h,w) u32, Work, ray counts)."""
+        w, h = cam.width, cam.height
+        r0, r1 = (0, h) if rows is None else rows
+        color = np.zeros((h, w, 3), np.float32)
+        frame = np.zeros((h, w), np.uint32)
+        wk = Work()
+        rc = np.zeros(3, np.uint64)
+        th = threads or os.cpu_count() or 1
+        c = self._cam(cam)
+        lib().orc_scene_render(self.h, C.byref(c), eps, depth, w, h, _ptr(color), _ptr(frame), C.byref(wk),
+                               _ptr(rc), th, r0, r1)
+        return color, frame, wk.asdict(), dict(primary=int(rc[0]), shadow=int(rc[1]), secondary=int(rc[2]))
+
+    def cpu_render(self, cam, eps=1e-4, depth=0, rows=None, threads=None):
+        """CPURayTracer restatement; returns (seconds, color, frame, rays, shadow rays)."""
+        w, h = cam.width, cam.height
+        r0, r1 = (0, h) if rows is None else rows
+        color = np.zeros((h, w, 3), np.float32)
+        frame = np.zeros((h, w), np.uint32)
+        rc = np.zeros(2, np.uint64)
+        th = threads or os.cpu_count() or 1
+        c = self._cam(cam)
+        sec = lib().orc_cpu_render(self.h, C.byref(c), eps, depth, w, h, _ptr(color), _ptr(frame), th, r0, r1, _ptr(rc))
+        return sec, color, frame, int(rc[0]), int(rc[1])

@@ -1,0 +1,605 @@
+// SPDX-License-Identifier: MIT
+//
+// cgspline oracle — TEST INFRASTRUCTURE ONLY (see cgs_oracle.hpp).
+//
+// Scene level: BezierSurface (rt/Primitive.cu), two-level Scene (rt/Scene.cu),
+// RayTracer shading and trace loop (rt/RayTracer.cu), CPURayTracer
+// (rt/CPURayTracer.cpp) and PBR (rt/PBR.{h,cu}), restated on the CPU.
+//
+#pragma once
+
+#include <atomic>
+#include <deque>
+#include <future>
+#include <mutex>
+#include <numbers>
+#include <thread>
+
+#include "cgs_oracle.hpp"
+
+namespace orc
+{
+
+using f3 = V3<float>;
+using f4 = V4<float>;
+
+// rt/Scene.h:79-95
+struct Material
+{
+    f3 diffuse, specular;
+    float metalness, roughness;
+};
+
+// rt/Scene.h:47-77
+struct Light
+{
+    f3 color;
+    float strength;
+    f3 position;
+    float range;
+    f3 direction;
+    float angle;
+    bool isDirectional() const { return angle < 0.01f; }
+    bool isPoint() const { return angle > 1.57f; }
+};
+
+// rt/Primitive.h:90-116 (one BezierSurface per surface) + its BVH and hash.
+struct Surface
+{
+    std::vector<f4> vertices;
+    std::vector<uint32_t> indices; // 16 per patch
+    BVHData bvh;
+    PHF hash;
+    std::vector<uint32_t> table;
+    uint32_t patchCount() const { return (uint32_t)(indices.size() / 16); }
+    void patch(uint32_t p, f4 out[16]) const
+    {
+        for (int i = 0; i < 16; i++) out[i] = vertices[indices[16 * (size_t)p + i]];
+    }
+    BVHView view() const { return {bvh.nodes.data(), bvh.indices.data(), table.data(), &hash}; }
+};
+
+struct Object
+{
+    uint32_t surface;
+    M4<float> local2world, world2local;
+    Material material;
+};
+
+struct Scene
+{
+    std::vector<Surface> surfaces;
+    std::vector<Object> objects;
+    std::vector<Light> lights;
+    BVHData top;
+    PHF topHash;
+    std::vector<uint32_t> topTable;
+    f3 background {0.1f, 0.1f, 0.1f};
+    f3 ambient {0.1f, 0.1f, 0.1f};
+    BVHView topView() const { return {top.nodes.data(), top.indices.data(), topTable.data(), &topHash}; }
+};
+
+// Builds every BVH and hash table (MainWindow.cpp:363-492 convertScene order:
+// per-surface BVHs, top-level BVH, then the hash tables).  Returns 0 or the
+// first failing hash build's code.
+inline int buildScene(Scene& s, uint32_t seed = 42)
+{
+    for (auto& surf : s.surfaces)
+        surf.bvh = bvhBuild(patchElements(&surf.vertices[0].x, surf.indices.data(), surf.patchCount()), 1);
+    // rt/Scene.cu:218-232 — objects bounded by their world-space control points
+    std::vector<Element> el(s.objects.size());
+    for (size_t i = 0; i < s.objects.size(); i++)
+    {
+        const Object& o = s.objects[i];
+        const Surface& surf = s.surfaces[o.surface];
+        Element& e = el[i];
+        e.index = (uint32_t)i;
+        e.bounds = Box<float>();
+        for (uint32_t idx : surf.indices)
+        {
+            f4 q = o.local2world * surf.vertices[idx];
+            e.bounds.inflate(q.xyz());
+        }
+        e.centroid = e.bounds.center();
+    }
+    s.top = bvhBuild(el, 1);
+    for (auto& surf : s.surfaces)
+    {
+        int r = phfBuild(surf.hash, surf.bvh.keys, seed);
+        if (r) return r;
+        surf.table = hashTable(surf.hash, surf.bvh.keys);
+    }
+    int r = phfBuild(s.topHash, s.top.keys, seed);
+    if (r) return r;
+    s.topTable = hashTable(s.topHash, s.top.keys);
+    return 0;
+}
+
+struct TraceHit
+{
+    float t = std::numeric_limits<float>::infinity();
+    float u = 0, v = 0;
+    uint32_t patch = EMPTY;
+    int object = -1;
+};
+
+// rt/Primitive.cu:355-381 — closest hit against one surface (local ray).
+inline bool surfaceIntersect(const Surface& s, Hit& hit0, const Ray& ray0, Work* w)
+{
+    // Ds Ray3f{origin, direction} normalizes the direction; the result is the
+    // same for every patch of this ray, so it is computed once here.
+    const f3 dir = versor(ray0.direction);
+    auto fn = [&](Hit& hit, uint32_t index) -> bool
+    {
+        PatchHit<float> ph {hit.t, {hit.coordinates.x, hit.coordinates.y}};
+        f4 S[16];
+        s.patch(index, S);
+        if (doBezierClipping<float>(ph, ray0.origin, dir, S, 0x1p-12f, w))
+        {
+            hit.index = index;
+            hit.t = ph.distance;
+            hit.coordinates = {ph.p.x, ph.p.y, 0.0f};
+            return true;
+        }
+        return false;
+    };
+    return hashIntersect(s.view(), hit0, ray0, fn, w);
+}
+
+// rt/Primitive.cu:383-401 — any hit against one surface (local ray).
+inline bool surfaceIntersectAny(const Surface& s, const Ray& ray0, Work* w)
+{
+    const f3 dir = versor(ray0.direction);
+    auto fn = [&](uint32_t index) -> bool
+    {
+        PatchHit<float> ph {ray0.max, {0, 0}};
+        f4 S[16];
+        s.patch(index, S);
+        return doBezierClipping<float>(ph, ray0.origin, dir, S, 0x1p-12f, w);
+    };
+    return hashIntersectAny(s.view(), ray0, fn, w);
+}
+
+// World ray -> object-local ray (RayTracer.cu:379-389): t scales by |M d|.
+inline Ray localRay(const Object& o, const Ray& ray, float& dinv)
+{
+    Ray lr;
+    f4 O = o.world2local * f4 {ray.origin.x, ray.origin.y, ray.origin.z, 1.0f};
+    lr.origin = project(O);
+    lr.direction = M3<float>::from4(o.world2local) * ray.direction;
+    float d = lr.direction.length();
+    lr.max = ray.max * d;
+    d = 1 / d;
+    lr.direction = d * lr.direction;
+    dinv = d;
+    return lr;
+}
+
+// RayTracer.cu:369-423 — Context::intersect(hit, ray): returns the nearest object.
+inline int sceneIntersect(const Scene& sc, Hit& hit0, const Ray& ray0, uint32_t* patchOut, Work* w)
+{
+    int nearest = -1;
+    auto fn = [&](Hit& hit, uint32_t i) -> bool
+    {
+        const Object& o = sc.objects[i];
+        float d;
+        Ray lr = localRay(o, ray0, d);
+        Hit lh;
+        lh.t = lr.max;
+        bool b = surfaceIntersect(sc.surfaces[o.surface], lh, lr, w);
+        if (b)
+        {
+            float t = lh.t * d;
+            if (t < hit.t)
+            {
+                hit.coordinates = lh.coordinates;
+                hit.index = lh.index;
+                hit.t = t;
+                nearest = (int)i;
+                return true;
+            }
+        }
+        return false;
+    };
+    hashIntersect(sc.topView(), hit0, ray0, fn, w);
+    if (patchOut) *patchOut = hit0.index;
+    return nearest;
+}
+
+// RayTracer.cu:425-457 — Context::intersect(ray): any hit.
+inline bool sceneIntersectAny(const Scene& sc, const Ray& ray0, Work* w)
+{
+    auto fn = [&](uint32_t i) -> bool
+    {
+        const Object& o = sc.objects[i];
+        float d;
+        Ray lr = localRay(o, ray0, d);
+        return surfaceIntersectAny(sc.surfaces[o.surface], lr, w);
+    };
+    return hashIntersectAny(sc.topView(), ray0, fn, w);
+}
+
+// ---------------------------------------------------------------------------
+// PBR (rt/PBR.h, rt/PBR.cu)
+// ---------------------------------------------------------------------------
+
+inline f3 splat(float s) { return {s, s, s}; }
+
+// PBR.cu:432-454
+inline bool lightVector(float& d, f3& L, f3 P, const Light& light)
+{
+    if (light.isDirectional())
+    {
+        L = -light.direction;
+        d = std::numeric_limits<float>::infinity();
+        return true;
+    }
+    L = light.position - P;
+    d = L.length();
+    if (d < 1e-14f || (light.range > 0 && d > light.range)) return false;
+    L = (1 / d) * L;
+    if (light.isPoint()) return true;
+    float DL = light.direction.dot(L);
+    return DL < 0 && light.angle > 2 * std::acos(DL);
+}
+// PBR.cu:456-476
+inline f3 lightColor(float d, const Light& light)
+{
+    if (light.isDirectional()) return light.color;
+    float range = light.range, f;
+    if (range == 0)
+    {
+        f = 1 / d;
+        f *= f;
+    }
+    else
+    {
+        f = d / range;
+        f = 1 + f * (f - 2);
+    }
+    return f * light.color;
+}
+// PBR.h:288-295 (vec3 and float instantiations)
+inline f3 schlick(f3 f0, float dotLH)
+{
+    float b = 1 - dotLH;
+    float b2 = b * b;
+    return f0 + (splat(1) - f0) * splat(b2 * b2 * b);
+}
+// PBR.h:306-319
+inline float G1(float dotNX, float k) { return dotNX / (dotNX * (1 - k) + k); }
+inline float Gsmith(float dotNL, float dotNV, float r)
+{
+    const float r1 = r + 1;
+    const float k = (float)((double)(r1 * r1) * 0.125);
+    return G1(dotNL, k) * G1(dotNV, k);
+}
+// PBR.h:322-330
+inline float Dggx(float dotNH, float r)
+{
+    r = fmax_(r, 1e-3f);
+    float a = r * r;
+    float a2 = a * a;
+    float d = dotNH * dotNH * (a2 - 1) + 1;
+    return a2 / (std::numbers::pi_v<float> * d * d);
+}
+// PBR.h:332-341
+inline float microfacet(float dotNV, float dotNL, float dotNH, float r)
+{
+    return Gsmith(dotNL, dotNV, r) * Dggx(dotNH, r) / (4 * dotNL * dotNV);
+}
+// PBR.h:343-348
+inline f3 brdfDiffuse(const Material& m) { return std::numbers::inv_pi_v<float> * m.diffuse; }
+// PBR.h:365-375
+inline f3 brdfSpecular(float dotHL, float dotHN, float dotNV, float dotNL, float r, f3 f0)
+{
+    return microfacet(dotNV, dotNL, dotHN, r) * schlick(f0, dotHL);
+}
+// PBR.h:268-272
+inline f3 reflect(f3 I, f3 N, float dotIN) { return I - (2 * dotIN) * N; }
+
+// Primitive.cu:403-408 — shading normal from mat::normal, normalized.
+inline f3 surfaceNormal(const Surface& s, uint32_t patch, float u, float v)
+{
+    f4 S[16];
+    s.patch(patch, S);
+    return versor(normalMat<float>(S, u, v));
+}
+
+// Ds-boundary: pack_sRGB — IEC 61966-2-1 sRGB encode of each clamped channel,
+// rounded to 8 bits, packed as 0xAABBGGRR with alpha 255.
+inline uint32_t packSRGB(float r, float g, float b)
+{
+    auto enc = [](float c) -> uint32_t
+    {
+        float e = c <= 0.0031308f ? 12.92f * c : 1.055f * std::pow(c, 1.0f / 2.4f) - 0.055f;
+        int q = (int)(e * 255.0f + 0.5f);
+        return (uint32_t)std::clamp(q, 0, 255);
+    };
+    return enc(r) | (enc(g) << 8) | (enc(b) << 16) | 0xFF000000u;
+}
+
+// Camera (RayTracer.cu:207-231): pixel centres at integer (i,j).
+struct Camera
+{
+    M4<float> cameraToWorld;
+    f3 position;
+    float fovy;
+    bool flipY = false;
+};
+struct CameraSetup
+{
+    M4<float> toWorld;
+    f3 position;
+    V2<float> topLeft;
+    float half_dx, half_dy;
+};
+inline CameraSetup setupCamera(const Camera& c, uint32_t w, uint32_t h)
+{
+    CameraSetup s;
+    s.toWorld = c.cameraToWorld;
+    s.position = c.position;
+    float halfHeight = std::tan(0.5f * c.fovy);
+    s.half_dy = halfHeight / (float)h;
+    s.half_dx = s.half_dy;
+    s.topLeft.x = -s.half_dx * (float)(w - 1);
+    s.topLeft.y = halfHeight - s.half_dy;
+    if (c.flipY)
+    {
+        s.topLeft.y = -s.topLeft.y;
+        s.half_dy = -s.half_dy;
+    }
+    return s;
+}
+// RayTracer.cu:133-145
+inline f3 pixelRayDirection(const CameraSetup& c, float x, float y)
+{
+    f4 P {c.topLeft.x + c.half_dx * (2 * x), c.topLeft.y - c.half_dy * (2 * y), -1.0f, 1.0f};
+    P = c.toWorld * P;
+    return versor(project(P) - c.position);
+}
+
+struct RenderOptions
+{
+    float eps = 1e-4f;
+    int recursionDepth = 6;
+};
+
+struct RayCounts
+{
+    uint64_t primary = 0, shadow = 0, secondary = 0;
+};
+
+// RayTracer.cu:256-348 + 350-618 (non-Monte-Carlo path): the GPU renderer's
+// iterative trace loop.  Writes the unclamped colour and the packed pixel.
+inline void renderPixel(const Scene& sc, const CameraSetup& cam, const RenderOptions& op,
+    uint32_t i, uint32_t j, f3& colorOut, uint32_t& pixel, Work* w, RayCounts* rc)
+{
+    struct Payload
+    {
+        Ray ray;
+        f3 color, attenuation;
+        int depth;
+    } pl;
+    pl.color = splat(0);
+    pl.ray = {cam.position, pixelRayDirection(cam, (float)i, (float)j), std::numeric_limits<float>::infinity()};
+    pl.attenuation = splat(1);
+    pl.depth = op.recursionDepth;
+    bool first = true;
+    while (true)
+    {
+        if (rc) (first ? rc->primary : rc->secondary)++;
+        first = false;
+        Hit hit;
+        hit.t = pl.ray.max;
+        uint32_t patch = EMPTY;
+        int obj = sceneIntersect(sc, hit, pl.ray, &patch, w);
+        if (obj < 0)
+        {
+            pl.color = pl.color + sc.background * pl.attenuation;
+            break;
+        }
+        // closestHit (RayTracer.cu:471-618)
+        const Object& o = sc.objects[obj];
+        const Material& m = o.material;
+        const f3 diffuse = (1 - m.metalness) * brdfDiffuse(m);
+        f3 color = sc.ambient * diffuse;
+        f3 V = -pl.ray.direction;
+        f3 N = surfaceNormal(sc.surfaces[o.surface], hit.index, hit.coordinates.x, hit.coordinates.y);
+        N = versor(M3<float>::from4(o.world2local).transposed() * N);
+        float dotNV = N.dot(V);
+        if (dotNV < 0)
+        {
+            N = -N;
+            dotNV = -dotNV;
+        }
+        f3 P = pl.ray.origin + hit.t * pl.ray.direction;
+        for (const Light& light : sc.lights)
+        {
+            float d;
+            f3 L;
+            if (!lightVector(d, L, P, light)) continue;
+            const float dotNL = N.dot(L);
+            if (dotNL < 1e-14f) continue;
+            Ray r1 {P + op.eps * L, L, d};
+            if (rc) rc->shadow++;
+            if (sceneIntersectAny(sc, r1, w)) continue;
+            f3 H = versor(L + V);
+            float dotHN = H.dot(N);
+            float dotHL = H.dot(L);
+            f3 I = lightColor(d, light);
+            color = color + (dotNL * I) * (diffuse + m.metalness * brdfSpecular(dotHL, dotHN, dotNV, dotNL, m.roughness, m.specular));
+        }
+        pl.color = pl.color + (std::numbers::pi_v<float> * color) * pl.attenuation;
+        if (pl.depth <= 0) break;
+        f3 reflectance = schlick(m.specular, dotNV);
+        f3 R = reflect(-V, N, -dotNV);
+        float g = Gsmith(dotNV, dotNV, m.roughness);
+        float rr = 1 - m.roughness;
+        f3 brdf = (rr * rr) * (dotNV * (diffuse + m.metalness * (g * reflectance)));
+        f3 a = pl.attenuation * brdf;
+        if (a.max() <= 0x1p-8f) break;
+        pl.ray = {P + op.eps * R, R, std::numeric_limits<float>::infinity()};
+        pl.attenuation = a;
+        pl.depth = pl.depth - 1;
+    }
+    colorOut = pl.color;
+    f3 c = pl.color;
+    // __saturatef semantics: NaN -> 0
+    auto sat = [](float x) { return x > 0.0f ? (x < 1.0f ? x : 1.0f) : 0.0f; };
+    pixel = packSRGB(sat(c.x), sat(c.y), sat(c.z));
+}
+
+// ---------------------------------------------------------------------------
+// CPURayTracer (rt/CPURayTracer.cpp:186-441), non-Monte-Carlo path: the CPU
+// baseline.  Recursive trace, 32×32 tiles pulled from a mutex-guarded deque
+// by `threads` std::async workers.
+// ---------------------------------------------------------------------------
+class CPURayTracer
+{
+public:
+    struct Options
+    {
+        float eps = 1e-4f;
+        bool flipYAxis = false;
+        int recursionDepth = 8;
+        int threads = (int)std::thread::hardware_concurrency();
+        int tileSize = 32;
+    } options;
+
+    std::atomic<uint64_t> rays {0}, shadowRays {0};
+
+    // Renders rows [row0, row1) (a bounded sample of the frame for timing)
+    // and returns the wall time from entry to the last tile, in seconds
+    // (CPURayTracer.cpp:203,268).
+    double render(const Scene* scene, const Camera& camera, uint32_t w, uint32_t h,
+        float* color, uint32_t* frame, uint32_t row0 = 0, uint32_t row1 = 0xFFFFFFFFu)
+    {
+        auto t0 = std::chrono::steady_clock::now();
+        options.threads = std::clamp(options.threads, 1, 0x1000);
+        options.tileSize = std::max(options.tileSize, 8);
+        _scene = scene;
+        _w = w;
+        _h = h;
+        _color = color;
+        _frame = frame;
+        row1 = std::min(row1, h);
+        _row0 = row0;
+        _row1 = row1;
+        _cam = setupCamera(Camera {camera.cameraToWorld, camera.position, camera.fovy, options.flipYAxis}, w, h);
+        uint32_t countX = (w + options.tileSize - 1) / options.tileSize;
+        uint32_t countY = (row1 - row0 + options.tileSize - 1) / options.tileSize;
+        _work.clear();
+        for (int j = (int)countY - 1; j >= 0; j--)
+            for (int i = (int)countX - 1; i >= 0; i--)
+                _work.push_back({uint32_t(i * options.tileSize), uint32_t(row0 + j * options.tileSize)});
+        std::vector<std::future<void>> workers;
+        for (int i = 0; i < options.threads; i++)
+            workers.emplace_back(std::async(std::launch::async, &CPURayTracer::work, this));
+        for (auto& f : workers) f.wait();
+        return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+
+private:
+    struct Tile { uint32_t x, y; };
+    const Scene* _scene = nullptr;
+    uint32_t _w = 0, _h = 0, _row0 = 0, _row1 = 0;
+    float* _color = nullptr;
+    uint32_t* _frame = nullptr;
+    CameraSetup _cam;
+    std::deque<Tile> _work;
+    std::mutex _mutex;
+
+    void work()
+    {
+        while (true)
+        {
+            Tile t;
+            {
+                std::lock_guard lck(_mutex);
+                if (_work.empty()) return;
+                t = _work.back();
+                _work.pop_back();
+            }
+            uint32_t x1 = std::min(_w, t.x + options.tileSize);
+            uint32_t y1 = std::min(_row1, t.y + options.tileSize);
+            for (uint32_t j = t.y; j < y1; j++)
+                for (uint32_t i = t.x; i < x1; i++)
+                {
+                    Ray r {_cam.position, pixelRayDirection(_cam, (float)i, (float)j), std::numeric_limits<float>::infinity()};
+                    f3 c = trace(r, splat(1), options.recursionDepth);
+                    size_t k = (size_t)j * _w + i;
+                    if (_color)
+                    {
+                        _color[3 * k] = c.x;
+                        _color[3 * k + 1] = c.y;
+                        _color[3 * k + 2] = c.z;
+                    }
+                    auto cl = [](float x) { return std::clamp(x, 0.0f, 1.0f); };
+                    if (_frame) _frame[k] = packSRGB(cl(c.x), cl(c.y), cl(c.z));
+                }
+        }
+    }
+
+    // CPURayTracer.cpp:330-350
+    f3 trace(const Ray& ray, f3 attenuation, int depth)
+    {
+        rays.fetch_add(1, std::memory_order_relaxed);
+        Hit hit;
+        hit.t = ray.max;
+        int obj = sceneIntersect(*_scene, hit, ray, nullptr, nullptr);
+        if (obj < 0) return attenuation * _scene->background;
+        return closestHit(hit, ray, attenuation, (uint32_t)obj, depth);
+    }
+
+    // CPURayTracer.cpp:422-571 (non-MC branch)
+    f3 closestHit(const Hit& hit, const Ray& ray, f3 attenuation, uint32_t object, int depth)
+    {
+        const Object& o = _scene->objects[object];
+        const Material& m = o.material;
+        const f3 diffuse = (1 - m.metalness) * brdfDiffuse(m);
+        f3 color = _scene->ambient * diffuse;
+        f3 V = -ray.direction;
+        f3 N = versor(M3<float>::from4(o.world2local).transposed() *
+            surfaceNormal(_scene->surfaces[o.surface], hit.index, hit.coordinates.x, hit.coordinates.y));
+        float dotNV = N.dot(V);
+        if (dotNV < 0)
+        {
+            N = -N;
+            dotNV = -dotNV;
+        }
+        f3 P = ray.origin + hit.t * ray.direction;
+        for (const Light& light : _scene->lights)
+        {
+            float d;
+            f3 L;
+            if (!lightVector(d, L, P, light)) continue;
+            const float dotNL = N.dot(L);
+            if (dotNL < 1e-14f) continue;
+            Ray r1 {P + options.eps * L, L, d};
+            shadowRays.fetch_add(1, std::memory_order_relaxed);
+            if (sceneIntersectAny(*_scene, r1, nullptr)) continue;
+            f3 H = versor(L + V);
+            float dotHN = H.dot(N);
+            float dotHL = H.dot(L);
+            f3 I = lightColor(d, light);
+            color = color + (dotNL * I) * (diffuse + m.metalness * brdfSpecular(dotHL, dotHN, dotNV, dotNL, m.roughness, m.specular));
+        }
+        color = color * (std::numbers::pi_v<float> * attenuation);
+        if (depth <= 0) return color;
+        f3 reflectance = schlick(m.specular, dotNV);
+        f3 R = reflect(-V, N, -dotNV);
+        float g = Gsmith(dotNV, dotNV, m.roughness);
+        float rr = 1 - m.roughness;
+        f3 brdf = (rr * rr) * (dotNV * (diffuse + m.metalness * (g * reflectance)));
+        f3 a = attenuation * brdf;
+        if (a.max() > 0x1p-8f)
+        {
+            Ray r {P + options.eps * R, R, std::numeric_limits<float>::infinity()};
+            color = color + trace(r, a, depth - 1);
+        }
+        return color;
+    }
+};
+
+} // namespace orc

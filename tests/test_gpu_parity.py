@@ -1,0 +1,219 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle.
+
+Integer and index results (BVH nodes, keys, hash tables, hit patch/object
+ids) must be identical.  FP32 results (t, u, v, shading) are computed with the
+same operation order, no FMA contraction and IEEE division/sqrt on both sides,
+so they must be identical bit for bit as well; the only tolerance is ±1 in an
+8-bit channel of the packed frame (sRGB encoding uses powf, whose last bit
+differs between libm and the device library).
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import rng, say
+
+pytestmark = pytest.mark.gpu
+
+EMPTY = 0xFFFFFFFF
+
+
+def bits(a):
+    return np.ascontiguousarray(a, np.float32).view(np.uint32)
+
+
+def assert_bits_equal(a, b, what):
+    ba, bb = bits(a), bits(b)
+    bad = np.flatnonzero((ba != bb).reshape(ba.shape[0], -1).any(axis=1)) if ba.ndim > 1 else np.flatnonzero(ba != bb)
+    assert bad.size == 0, f"{what}: {bad.size} mismatches, first {bad[:5]}: {np.asarray(a)[bad[:3]]} vs {np.asarray(b)[bad[:3]]}"
+
+
+def pairs_c1(oracle, wl, step=5, extra=2, seed=3):
+    """(ray, patch) pairs from the height-field workload: every `step`-th
+    camera ray against the patch it hits plus `extra` random patches."""
+    from cgspline_amd import scenes
+    sc = wl.scene
+    rays = scenes.camera_rays(wl.camera)[::step]
+    osc = oracle.OracleScene(sc)
+    _, patch, obj, _ = osc.trace(rays)
+    P = sc.surface_patches(0)
+    r = rng(seed)
+    ids = [np.where(obj >= 0, patch, r.integers(0, P.shape[0], rays.shape[0]))]
+    for _ in range(extra):
+        ids.append(r.integers(0, P.shape[0], rays.shape[0]))
+    ids = np.concatenate(ids).astype(np.int64)
+    R = np.concatenate([rays] * (extra + 1))
+    return R, P[ids]
+
+
+def random_patch_pairs(n, seed):
+    """Random control nets (wavy, folded, degenerate rows) and rays aimed at
+    random points near them."""
+    r = rng(seed)
+    base = np.stack(np.meshgrid(np.linspace(0, 1, 4), np.linspace(0, 1, 4), indexing="xy"), -1).reshape(16, 2)
+    P = np.zeros((n, 16, 4), np.float32)
+    scale = np.exp(r.uniform(np.log(1e-2), np.log(10.0), n))[:, None]
+    P[:, :, 0] = base[None, :, 0] * scale + r.normal(0, 0.15, (n, 16)) * scale
+    P[:, :, 1] = base[None, :, 1] * scale + r.normal(0, 0.15, (n, 16)) * scale
+    P[:, :, 2] = r.normal(0, 0.4, (n, 16)) * scale
+    P[:, :, 3] = 1.0
+    deg = r.random(n) < 0.15                       # collapse a row to a point
+    P[deg, 0:4, :3] = P[deg, 0:1, :3]
+    target = P[:, :, :3].mean(axis=1) + r.normal(0, 0.3, (n, 3)) * scale
+    o = target + r.normal(0, 1, (n, 3)) * 3 * scale
+    d = target - o
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, 0:3], rays[:, 3], rays[:, 4:7] = o, np.inf, d
+    return rays, P
+
+
+def gpu_clip(ctx, rays, P):
+    out = ctx.bezier_clip(rays, P)
+    found = out.view(np.uint32)[:, 3]
+    assert not np.any(found == 0xFFFFFFFE), "clip stack overflow"
+    return out[:, :3], found == 1
+
+
+def test_clip_batch_heightfield_bitexact(ctx, oracle, c1):
+    rays, P = pairs_c1(oracle, c1)
+    say("bezier_clip heightfield pairs", rays.shape[0])
+    g, gf = gpu_clip(ctx, rays, P)
+    o, _ = oracle.clip_batch(rays, P)
+    assert np.array_equal(gf, o[:, 3] == 1)
+    assert gf.sum() > 1000
+    assert_bits_equal(g, o[:, :3], "t,u,v")
+
+
+@pytest.mark.parametrize("seed", [11, 12])
+def test_clip_batch_random_bitexact(ctx, oracle, seed):
+    rays, P = random_patch_pairs(20000, seed)
+    say("bezier_clip random patches seed", seed)
+    g, gf = gpu_clip(ctx, rays, P)
+    o, w = oracle.clip_batch(rays, P)
+    assert np.array_equal(gf, o[:, 3] == 1)
+    assert gf.sum() > 2000 and w["subdivs"] > 0 and w["prunes"] > 0
+    assert_bits_equal(g, o[:, :3], "t,u,v")
+
+
+def test_subdivision_batch_bitexact(ctx, oracle, c1):
+    rays, P = pairs_c1(oracle, c1, step=11, extra=1)
+    for thr in (2.0 ** -4, 2.0 ** -7):
+        say("subdivision threshold", thr)
+        g = ctx.subdivision(rays, P, thr)
+        o, _ = oracle.clip_batch(rays, P, method=1, threshold=thr)
+        gf = g.view(np.uint32)[:, 3]
+        assert not np.any(gf == 0xFFFFFFFE)
+        assert np.array_equal(gf == 1, o[:, 3] == 1)
+        assert_bits_equal(g[:, :3], o[:, :3], "t,u,v")
+
+
+def heap_keys(npatch):
+    keys, stack = [], [(1, npatch)]
+    while stack:
+        k, c = stack.pop()
+        keys.append(k)
+        if c > 1:
+            m = c // 2
+            stack.append(((k << 1) ^ 1, c - m))
+            stack.append((k << 1, m))
+    return np.array(keys, np.uint32)
+
+
+@pytest.mark.parametrize("keys", [
+    "heap1", "heap2", "heap3", "heap256", "heap4096", "heap33000", "shuffled4096", "random3000"])
+def test_phf_build_matches_oracle(ctx, oracle, keys):
+    r = rng(5)
+    if keys.startswith("heap"):
+        k = heap_keys(int(keys[4:]))
+    elif keys.startswith("shuffled"):
+        k = heap_keys(int(keys[8:]))
+        r.shuffle(k)
+    else:
+        k = r.choice(np.uint32(2 ** 31), int(keys[6:]), replace=False).astype(np.uint32)
+    say("phf build", keys, k.shape[0])
+    st, p, D = ctx.phf_build(k)
+    ost, op, oD = oracle.phf_build(k)
+    code = {0: 0, 1: 4, 2: 5}[ost]
+    assert st == code
+    assert p == op
+    assert np.array_equal(D, oD)
+    if st == 0:
+        h = ctx.phf_hash(p, D, k)
+        assert np.unique(h).size == k.size and h.max() < p["table_size"]
+
+
+def test_scene_bvh_and_hash_match_oracle(ctx, oracle, c2_small):
+    sc = c2_small.scene
+    say("scene create closed surfaces", sc.npatch)
+    ds = ctx.scene(sc)
+    osc = oracle.OracleScene(sc)
+    for s in [0, 1, 17, sc.nsurf - 1, -1]:
+        g, o = ds.bvh(s), osc.bvh(s)
+        for key in ("nodes", "keys", "indices", "displacement", "table"):
+            assert np.array_equal(g[key], o[key]), (s, key)
+        for key in ("A", "B", "mask", "table_size", "attempts"):
+            assert g[key] == o[key], (s, key)
+
+
+def _trace_compare(ctx, oracle, wl, rows=None, label=""):
+    from cgspline_amd import scenes
+    sc = wl.scene
+    rays = scenes.camera_rays(wl.camera, rows)
+    say("trace_closest", label, rays.shape[0], "rays")
+    ds = ctx.scene(sc)
+    g_tuv, g_p, g_o = ds.trace_closest(rays)
+    osc = oracle.OracleScene(sc)
+    o_tuv, o_p, o_o, w = osc.trace(rays)
+    assert np.array_equal(g_o, o_o), f"object ids differ at {np.flatnonzero(g_o != o_o)[:5]}"
+    assert np.array_equal(g_p, o_p)
+    assert_bits_equal(g_tuv, o_tuv, "t,u,v")
+    assert (g_o >= 0).sum() > 0
+    return ds, osc, rays, g_tuv, g_o
+
+
+def test_trace_closest_heightfield_bitexact(ctx, oracle, c1):
+    _trace_compare(ctx, oracle, c1, label="heightfield")
+
+
+def test_trace_closest_closed_surfaces_bitexact(ctx, oracle, c2_small):
+    _trace_compare(ctx, oracle, c2_small, label="closed surfaces")
+
+
+def test_trace_closest_displaced_grids_bitexact(ctx, oracle, c3_small):
+    _trace_compare(ctx, oracle, c3_small, label="displaced grids")
+
+
+def test_trace_any_bitexact(ctx, oracle, c2_small):
+    ds, osc, rays, tuv, obj = _trace_compare(ctx, oracle, c2_small, label="closed surfaces (shadow set)")
+    light = c2_small.scene.lights[0, 4:7]
+    hit = obj >= 0
+    P = rays[hit, 0:3] + tuv[hit, 0:1] * rays[hit, 4:7]
+    L = light[None, :] - P
+    d = np.linalg.norm(L, axis=1).astype(np.float32)
+    L = (L / d[:, None]).astype(np.float32)
+    sh = np.zeros((P.shape[0], 8), np.float32)
+    sh[:, 0:3] = P + 1e-4 * L
+    sh[:, 3] = d
+    sh[:, 4:7] = L
+    say("trace_any", sh.shape[0], "shadow rays")
+    g = ds.trace_any(sh)
+    o, _ = osc.trace(sh, any_hit=True)
+    assert not np.any(g == 2)
+    assert np.array_equal(g == 1, o)
+    assert 0 < o.sum() < o.size
+
+
+@pytest.mark.parametrize("name", ["c1", "c2_small", "c3_small"])
+def test_render_matches_oracle(ctx, oracle, request, name):
+    wl = request.getfixturevalue(name)
+    say("render", name, wl.camera.width, wl.camera.height, "depth", wl.depth)
+    ds = ctx.scene(wl.scene)
+    color, frame, counts = ds.render(wl.camera, eps=wl.eps, depth=wl.depth)
+    osc = oracle.OracleScene(wl.scene)
+    ocolor, oframe, _, orc = osc.render(wl.camera, eps=wl.eps, depth=wl.depth)
+    assert counts["overflow"] == 0
+    assert (counts["primary"], counts["shadow"], counts["secondary"]) == (orc["primary"], orc["shadow"], orc["secondary"])
+    assert_bits_equal(color.reshape(-1, 3), ocolor.reshape(-1, 3), "color")
+    ch = lambda f: np.stack([(f >> s) & 0xFF for s in (0, 8, 16, 24)], -1).astype(np.int32)
+    assert np.abs(ch(frame) - ch(oframe)).max() <= 1
