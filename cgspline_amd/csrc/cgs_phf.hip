@@ -81,7 +81,8 @@ __global__ __launch_bounds__(kPhfThreads) void kPhfBuild(const PhfJob* __restric
     __syncthreads();
     if (sFlag)
     {
-        if (tid == 0) results[blockIdx.x] = {0, 0, n, 0, (uint32_t)CGS_ERR_BUCKET_OVERFLOW};
+        for (uint32_t b = tid; b < M; b += kPhfThreads) D[b] = 0;
+        if (tid == 0) results[blockIdx.x] = {31, 19, n, 0, (uint32_t)CGS_ERR_BUCKET_OVERFLOW}; // member defaults (PerfectHashFunction.h:39-40)
         return;
     }
 

@@ -36,13 +36,13 @@ extern "C" {
 
 struct orc_work
 {
-    uint64_t nodes, leaves, hashProbes, patchTests, clipIters, subdivs, prunes, converged, onHits, maxStack;
+    uint64_t nodes, leaves, hashProbes, patchTests, clipIters, subdivs, prunes, converged, onHits, maxStack, objects;
 };
 
 static void putWork(orc_work* o, const Work& w)
 {
     if (!o) return;
-    *o = {w.nodes, w.leaves, w.hashProbes, w.patchTests, w.clipIters, w.subdivs, w.prunes, w.converged, w.onHits, w.maxStack};
+    *o = {w.nodes, w.leaves, w.hashProbes, w.patchTests, w.clipIters, w.subdivs, w.prunes, w.converged, w.onHits, w.maxStack, w.objects};
 }
 
 int orc_version(void) { return 1; }
@@ -500,9 +500,10 @@ int orc_scene_render(const orc_scene* h, const orc_camera* c, float eps, int dep
 }
 
 // CPURayTracer restatement (the CPU baseline): renders rows [row0,row1) with
-// `threads` workers over 32×32 tiles.  Returns wall seconds; ray counts out.
+// `threads` workers over 32×32 tiles (rows row0, row0+stride, ...).
+// Returns wall seconds; ray counts out.
 double orc_cpu_render(const orc_scene* h, const orc_camera* c, float eps, int depth, uint32_t w, uint32_t hgt,
-    float* color, uint32_t* frame, int threads, uint32_t row0, uint32_t row1, uint64_t* raycounts2)
+    float* color, uint32_t* frame, int threads, uint32_t row0, uint32_t row1, uint32_t stride, uint64_t* raycounts2)
 {
     CPURayTracer rt;
     rt.options.eps = eps;
@@ -510,7 +511,7 @@ double orc_cpu_render(const orc_scene* h, const orc_camera* c, float eps, int de
     rt.options.threads = threads;
     rt.options.flipYAxis = c->flipY != 0;
     Camera cam {mat16(c->toWorld), {c->position[0], c->position[1], c->position[2]}, c->fovy, c->flipY != 0};
-    double sec = rt.render(&h->s, cam, w, hgt, color, frame, row0, row1);
+    double sec = rt.render(&h->s, cam, w, hgt, color, frame, row0, row1, stride);
     if (raycounts2) { raycounts2[0] = rt.rays.load(); raycounts2[1] = rt.shadowRays.load(); }
     return sec;
 }

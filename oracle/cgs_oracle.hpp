@@ -186,8 +186,10 @@ struct Work
     uint64_t converged = 0;   // regions under tolerance
     uint64_t onHits = 0;      // onHit evaluations
     uint64_t maxStack = 0;    // deepest clipping stack seen
+    uint64_t objects = 0;     // top-level leaf element (object) tests
     void add(const Work& o)
     {
+        objects += o.objects;
         nodes += o.nodes; leaves += o.leaves; hashProbes += o.hashProbes;
         patchTests += o.patchTests; clipIters += o.clipIters; subdivs += o.subdivs;
         prunes += o.prunes; converged += o.converged; onHits += o.onHits;

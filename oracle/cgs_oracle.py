@@ -22,7 +22,7 @@ _U32, _U64, _I32, _F32, _F64 = C.c_uint32, C.c_uint64, C.c_int32, C.c_float, C.c
 class Work(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in (
         "nodes", "leaves", "hashProbes", "patchTests", "clipIters", "subdivs", "prunes",
-        "converged", "onHits", "maxStack")]
+        "converged", "onHits", "maxStack", "objects")]
 
     def asdict(self):
         return {n: int(getattr(self, n)) for n, _ in self._fields_}
@@ -86,7 +86,7 @@ def lib():
         L.orc_scene_render.argtypes = [_P, C.POINTER(CameraDesc), _F32, C.c_int, _U32, _U32, _P, _P, _P, _P,
                                        C.c_int, _U32, _U32]
         L.orc_cpu_render.argtypes = [_P, C.POINTER(CameraDesc), _F32, C.c_int, _U32, _U32, _P, _P, C.c_int,
-                                     _U32, _U32, _P]
+                                     _U32, _U32, _U32, _P]
         L.orc_cpu_render.restype = _F64
         _lib = L
     return _lib
@@ -290,8 +290,9 @@ class OracleScene:
                                _ptr(rc), th, r0, r1)
         return color, frame, wk.asdict(), dict(primary=int(rc[0]), shadow=int(rc[1]), secondary=int(rc[2]))
 
-    def cpu_render(self, cam, eps=1e-4, depth=0, rows=None, threads=None):
-        """CPURayTracer restatement; returns (seconds, color, frame, rays, shadow rays)."""
+    def cpu_render(self, cam, eps=1e-4, depth=0, rows=None, threads=None, stride=1):
+        """CPURayTracer restatement over rows r0, r0+stride, ... < r1; returns
+        (seconds, color, frame, rays, shadow rays)."""
         w, h = cam.width, cam.height
         r0, r1 = (0, h) if rows is None else rows
         color = np.zeros((h, w, 3), np.float32)
@@ -299,5 +300,6 @@ class OracleScene:
         rc = np.zeros(2, np.uint64)
         th = threads or os.cpu_count() or 1
         c = self._cam(cam)
-        sec = lib().orc_cpu_render(self.h, C.byref(c), eps, depth, w, h, _ptr(color), _ptr(frame), th, r0, r1, _ptr(rc))
+        sec = lib().orc_cpu_render(self.h, C.byref(c), eps, depth, w, h, _ptr(color), _ptr(frame), th, r0, r1, stride,
+                                   _ptr(rc))
         return sec, color, frame, int(rc[0]), int(rc[1])

@@ -181,6 +181,7 @@ inline int sceneIntersect(const Scene& sc, Hit& hit0, const Ray& ray0, uint32_t*
     int nearest = -1;
     auto fn = [&](Hit& hit, uint32_t i) -> bool
     {
+        if (w) w->objects++;
         const Object& o = sc.objects[i];
         float d;
         Ray lr = localRay(o, ray0, d);
@@ -211,6 +212,7 @@ inline bool sceneIntersectAny(const Scene& sc, const Ray& ray0, Work* w)
 {
     auto fn = [&](uint32_t i) -> bool
     {
+        if (w) w->objects++;
         const Object& o = sc.objects[i];
         float d;
         Ray lr = localRay(o, ray0, d);
@@ -469,12 +471,14 @@ public:
 
     std::atomic<uint64_t> rays {0}, shadowRays {0};
 
-    // Renders rows [row0, row1) (a bounded sample of the frame for timing)
-    // and returns the wall time from entry to the last tile, in seconds
-    // (CPURayTracer.cpp:203,268).
+    // Renders rows row0, row0 + stride, ... below row1 (a bounded sample of
+    // the frame for timing; stride 1 = every row) and returns the wall time
+    // from entry to the last tile, in seconds (CPURayTracer.cpp:203,268).
+    // Tiles are 32×32 over the sampled rows.
     double render(const Scene* scene, const Camera& camera, uint32_t w, uint32_t h,
-        float* color, uint32_t* frame, uint32_t row0 = 0, uint32_t row1 = 0xFFFFFFFFu)
+        float* color, uint32_t* frame, uint32_t row0 = 0, uint32_t row1 = 0xFFFFFFFFu, uint32_t stride = 1)
     {
+        _stride = std::max(1u, stride);
         auto t0 = std::chrono::steady_clock::now();
         options.threads = std::clamp(options.threads, 1, 0x1000);
         options.tileSize = std::max(options.tileSize, 8);
@@ -488,11 +492,12 @@ public:
         _row1 = row1;
         _cam = setupCamera(Camera {camera.cameraToWorld, camera.position, camera.fovy, options.flipYAxis}, w, h);
         uint32_t countX = (w + options.tileSize - 1) / options.tileSize;
-        uint32_t countY = (row1 - row0 + options.tileSize - 1) / options.tileSize;
+        _nrows = (row1 - row0 + _stride - 1) / _stride;
+        uint32_t countY = (_nrows + options.tileSize - 1) / options.tileSize;
         _work.clear();
         for (int j = (int)countY - 1; j >= 0; j--)
             for (int i = (int)countX - 1; i >= 0; i--)
-                _work.push_back({uint32_t(i * options.tileSize), uint32_t(row0 + j * options.tileSize)});
+                _work.push_back({uint32_t(i * options.tileSize), uint32_t(j * options.tileSize)});
         std::vector<std::future<void>> workers;
         for (int i = 0; i < options.threads; i++)
             workers.emplace_back(std::async(std::launch::async, &CPURayTracer::work, this));
@@ -503,7 +508,7 @@ public:
 private:
     struct Tile { uint32_t x, y; };
     const Scene* _scene = nullptr;
-    uint32_t _w = 0, _h = 0, _row0 = 0, _row1 = 0;
+    uint32_t _w = 0, _h = 0, _row0 = 0, _row1 = 0, _stride = 1, _nrows = 0;
     float* _color = nullptr;
     uint32_t* _frame = nullptr;
     CameraSetup _cam;
@@ -522,10 +527,11 @@ private:
                 _work.pop_back();
             }
             uint32_t x1 = std::min(_w, t.x + options.tileSize);
-            uint32_t y1 = std::min(_row1, t.y + options.tileSize);
-            for (uint32_t j = t.y; j < y1; j++)
+            uint32_t y1 = std::min(_nrows, t.y + options.tileSize);
+            for (uint32_t jj = t.y; jj < y1; jj++)
                 for (uint32_t i = t.x; i < x1; i++)
                 {
+                    const uint32_t j = _row0 + jj * _stride;
                     Ray r {_cam.position, pixelRayDirection(_cam, (float)i, (float)j), std::numeric_limits<float>::infinity()};
                     f3 c = trace(r, splat(1), options.recursionDepth);
                     size_t k = (size_t)j * _w + i;
