@@ -11,7 +11,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libcgs.so")
+# CGS_LIBRARY selects another build of the library (e.g. a tuning variant).
+LIB_PATH = os.environ.get("CGS_LIBRARY") or os.path.join(HERE, "libcgs.so")
 
 _P = C.c_void_p
 _U32, _I32, _U64, _F32, _SZ = C.c_uint32, C.c_int32, C.c_uint64, C.c_float, C.c_size_t

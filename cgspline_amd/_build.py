@@ -14,7 +14,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libcgs.so")
-SOURCES = ["cgs_lib.cpp", "cgs_trace.hip", "cgs_phf.hip"]
+SOURCES = ["cgs_lib.cpp", "cgs_trace.hip", "cgs_phf.hip", "cgs_stage.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++20", "-fPIC",
@@ -32,9 +32,9 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
-def build(verbose: bool = False, jobs: int = 3) -> str:
+def build(verbose: bool = False, jobs: int = 3, defines=(), lib_path: str = LIB) -> str:
     """Compiles every source to an object in csrc/_build, then links LIB."""
-    out_dir = os.path.join(CSRC, "_build")
+    out_dir = os.path.join(CSRC, "_build" + ("_" + "_".join(d.replace("=", "") for d in defines) if defines else ""))
     os.makedirs(out_dir, exist_ok=True)
     cc = hipcc()
     procs = []
@@ -42,7 +42,7 @@ def build(verbose: bool = False, jobs: int = 3) -> str:
     for src in SOURCES:
         obj = os.path.join(out_dir, src.rsplit(".", 1)[0] + ".o")
         objs.append(obj)
-        cmd = [cc, *HIPCC_FLAGS, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj]
+        cmd = [cc, *HIPCC_FLAGS, *["-D" + d for d in defines], "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))
@@ -50,15 +50,15 @@ def build(verbose: bool = False, jobs: int = 3) -> str:
             _wait(procs.pop(0))
     for p in procs:
         _wait(p)
-    tmp = LIB + ".tmp"
+    tmp = lib_path + ".tmp"
     cmd = [cc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs, "-lpthread"]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stdout.decode(errors="replace"))
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, lib_path)
+    return lib_path
 
 
 def _wait(item):

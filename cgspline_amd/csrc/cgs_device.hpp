@@ -194,8 +194,10 @@ CGS_DEV void slice1(float& c0, float& c1, float& c2, float& c3, float u, float i
     c3 = ivv * c2 + vv * p8;
 }
 
-// subpatchU (rows) / subpatchV (columns) of a 2D patch held in registers.
-CGS_DEV void subpatch2(V2 (&p)[16], bool vside, float a, float b)
+// subpatchU (rows, Spline.h:366-373) / subpatchV (columns, :375-382) of a 2D
+// patch held in registers.  The two sides are separate functions so every
+// point index is a compile-time constant and the patch never leaves VGPRs.
+CGS_DEV void subpatchU2(V2 (&p)[16], float a, float b)
 {
     float iu = 1.0f - a;
     float vv = (b - a) / iu;
@@ -203,11 +205,20 @@ CGS_DEV void subpatch2(V2 (&p)[16], bool vside, float a, float b)
 #pragma unroll
     for (int k = 0; k < 4; k++)
     {
-        // row k (U side): [4k+0..3]; column k (V side): [k, k+4, k+8, k+12]
-        const int i0 = vside ? k : 4 * k;
-        const int st = vside ? 4 : 1;
-        slice1(p[i0].x, p[i0 + st].x, p[i0 + 2 * st].x, p[i0 + 3 * st].x, a, iu, vv, ivv);
-        slice1(p[i0].y, p[i0 + st].y, p[i0 + 2 * st].y, p[i0 + 3 * st].y, a, iu, vv, ivv);
+        slice1(p[4 * k].x, p[4 * k + 1].x, p[4 * k + 2].x, p[4 * k + 3].x, a, iu, vv, ivv);
+        slice1(p[4 * k].y, p[4 * k + 1].y, p[4 * k + 2].y, p[4 * k + 3].y, a, iu, vv, ivv);
+    }
+}
+CGS_DEV void subpatchV2(V2 (&p)[16], float a, float b)
+{
+    float iu = 1.0f - a;
+    float vv = (b - a) / iu;
+    float ivv = 1.0f - vv;
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+    {
+        slice1(p[k].x, p[k + 4].x, p[k + 8].x, p[k + 12].x, a, iu, vv, ivv);
+        slice1(p[k].y, p[k + 4].y, p[k + 8].y, p[k + 12].y, a, iu, vv, ivv);
     }
 }
 
@@ -274,14 +285,21 @@ CGS_DEV ClipRay makeClipRay(V3 o, V3 dir)
     return r;
 }
 
-// Region stack in private (scratch) memory: per entry the 16 working points
-// and the region (min, max, size, side).  Exactly the state the reference's
-// host clipper keeps per std::vector entry.
+// Region stack: per entry the 16 working points and the region (min, max,
+// size, side) — exactly the state the reference's host clipper keeps per
+// std::vector entry.  It lives in a library-owned HBM buffer, one slice per
+// resident lane, interleaved by lane so the 64 lanes of a wave pushing the
+// same word touch one contiguous 512-B run.  (In private memory the 5 KB per
+// lane would make the runtime cap the number of resident waves.)
+constexpr int kStackWords = 20; // float2 words per entry: 16 points + 3 region + pad
 struct ClipStack
 {
-    V2 pts[CGS_CLIP_STACK][16];
-    float4 reg[CGS_CLIP_STACK][2];
+    float2* __restrict__ base; // this lane's first word
+    uint32_t stride;           // lanes sharing the buffer
+    CGS_DEV void put(int sp, int w, float2 v) const { base[(size_t)(sp * kStackWords + w) * stride] = v; }
+    CGS_DEV float2 get(int sp, int w) const { return base[(size_t)(sp * kStackWords + w) * stride]; }
 };
+CGS_DEV ClipStack clipStackFor(float2* buf, uint32_t lane, uint32_t lanes) { return {buf + lane, lanes}; }
 
 // doBezierClipping with host semantics (Spline_BezierClipping.cu:46-114,
 // 174-445).  S: the 16 control points (global).  best/bu/bv: current hit,
@@ -290,7 +308,7 @@ struct ClipStack
 // Returns found; sets overflow when the region stack is exhausted.
 template<bool ANY>
 CGS_DEV bool clipPatch(const float4* __restrict__ S, const ClipRay& R, float tol,
-                       float& best, float& bu, float& bv, ClipStack& stk, bool& overflow)
+                       float& best, float& bu, float& bv, const ClipStack& stk, bool& overflow)
 {
     V2 p[16];
 #pragma unroll
@@ -412,8 +430,16 @@ CGS_DEV bool clipPatch(const float4* __restrict__ S, const ClipRay& R, float tol
                 V2 q[16];
 #pragma unroll
                 for (int i = 0; i < 16; i++) q[i] = p[i];
-                subpatch2(q, vside, 0.0f, 0.5f);
-                subpatch2(p, vside, 0.5f, 1.0f);
+                if (!vside)
+                {
+                    subpatchU2(q, 0.0f, 0.5f);
+                    subpatchU2(p, 0.5f, 1.0f);
+                }
+                else
+                {
+                    subpatchV2(q, 0.0f, 0.5f);
+                    subpatchV2(p, 0.5f, 1.0f);
+                }
                 // Spline_BezierClipping.cu:373-409: the lower half keeps the
                 // old minimum and ends at the midpoint; the upper half (kept
                 // in registers, processed first) starts at the midpoint.
@@ -437,9 +463,11 @@ CGS_DEV bool clipPatch(const float4* __restrict__ S, const ClipRay& R, float tol
                 }
                 vside = !vside;
 #pragma unroll
-                for (int i = 0; i < 16; i++) stk.pts[sp][i] = q[i];
-                stk.reg[sp][0] = {omn.x, omn.y, emx.x, emx.y};
-                stk.reg[sp][1] = {sz.x, sz.y, vside ? 1.0f : 0.0f, 0.0f};
+                for (int i = 0; i < 16; i++) stk.put(sp, i, make_float2(q[i].x, q[i].y));
+                stk.put(sp, 16, make_float2(omn.x, omn.y));
+                stk.put(sp, 17, make_float2(emx.x, emx.y));
+                stk.put(sp, 18, make_float2(sz.x, sz.y));
+                stk.put(sp, 19, make_float2(vside ? 1.0f : 0.0f, 0.0f));
                 sp++;
                 continue;
             }
@@ -449,7 +477,7 @@ CGS_DEV bool clipPatch(const float4* __restrict__ S, const ClipRay& R, float tol
                 {
                     float u0 = mn.x + sz.x * lower;
                     float u1 = mx.x - sz.x * (1.0f - upper);
-                    subpatch2(p, false, lower, upper);
+                    subpatchU2(p, lower, upper);
                     mn.x = u0;
                     mx.x = u1;
                     sz.x = u1 - u0;
@@ -458,7 +486,7 @@ CGS_DEV bool clipPatch(const float4* __restrict__ S, const ClipRay& R, float tol
                 {
                     float v0 = mn.y + sz.y * lower;
                     float v1 = mx.y - sz.y * (1.0f - upper);
-                    subpatch2(p, true, lower, upper);
+                    subpatchV2(p, lower, upper);
                     mn.y = v0;
                     mx.y = v1;
                     sz.y = v1 - v0;
@@ -472,12 +500,16 @@ CGS_DEV bool clipPatch(const float4* __restrict__ S, const ClipRay& R, float tol
             if (sp == 0) break;
             sp--;
 #pragma unroll
-            for (int i = 0; i < 16; i++) p[i] = stk.pts[sp][i];
-            float4 r0 = stk.reg[sp][0], r1 = stk.reg[sp][1];
+            for (int i = 0; i < 16; i++)
+            {
+                float2 q = stk.get(sp, i);
+                p[i] = {q.x, q.y};
+            }
+            float2 r0 = stk.get(sp, 16), r1 = stk.get(sp, 17), r2 = stk.get(sp, 18), r3 = stk.get(sp, 19);
             mn = {r0.x, r0.y};
-            mx = {r0.z, r0.w};
-            sz = {r1.x, r1.y};
-            vside = r1.z != 0.0f;
+            mx = {r1.x, r1.y};
+            sz = {r2.x, r2.y};
+            vside = r3.x != 0.0f;
         }
     }
     return found;

@@ -43,15 +43,37 @@ hipError_t launchPhfFill(const cgs_phf& h, const uint32_t* D, const uint32_t* ke
                          uint32_t* table, hipStream_t s);
 hipError_t launchFill(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
 
-hipError_t launchTraceClosest(const SceneDev& sc, const cgs_ray* rays, uint32_t n, cgs_hit* hits, int32_t* objects,
-                              unsigned long long* overflow, hipStream_t s);
-hipError_t launchTraceAny(const SceneDev& sc, const cgs_ray* rays, uint32_t n, uint32_t* occluded,
+// Persistent kernels: the grid is the resident workgroups; every lane owns a
+// slice of the region-stack buffer and waves pull work from a counter.
+enum class PersistKernel
+{
+    TraceClosest,
+    TraceAny,
+    ClipBatch,
+    Render,
+};
+constexpr int kPersistKernels = 4;
+
+struct PersistArgs
+{
+    float2* stack;          // lanes × CGS_CLIP_STACK × 20 float2
+    uint32_t lanes;         // grid × 256
+    unsigned int* counter;  // zeroed before the launch
+    uint32_t grid;
+};
+
+hipError_t persistentGrid(PersistKernel k, uint32_t& blocksPerCU);
+
+hipError_t launchTraceClosest(const SceneDev& sc, const PersistArgs& pa, const cgs_ray* rays, uint32_t n, cgs_hit* hits,
+                              int32_t* objects, unsigned long long* overflow, hipStream_t s);
+hipError_t launchTraceAny(const SceneDev& sc, const PersistArgs& pa, const cgs_ray* rays, uint32_t n, uint32_t* occluded,
                           unsigned long long* overflow, hipStream_t s);
-hipError_t launchClipBatch(const cgs_ray* rays, const float* patches, uint32_t n, float tol, cgs_hit* hits, hipStream_t s);
+hipError_t launchClipBatch(const PersistArgs& pa, const cgs_ray* rays, const float* patches, uint32_t n, float tol,
+                           cgs_hit* hits, hipStream_t s);
 hipError_t launchSubdivBatch(const cgs_ray* rays, const float* patches, uint32_t n, float threshold, cgs_hit* hits,
                              hipStream_t s);
-hipError_t launchRender(const SceneDev& sc, const float* toWorld16, const float* pos3, float tlx, float tly, float hdx,
-                        float hdy, float eps, int depth, uint32_t w, uint32_t h, uint32_t* frame, float* color,
-                        unsigned long long* counts, hipStream_t s);
+hipError_t launchRender(const SceneDev& sc, const PersistArgs& pa, const float* toWorld16, const float* pos3, float tlx,
+                        float tly, float hdx, float hdy, float eps, int depth, uint32_t w, uint32_t h, uint32_t* frame,
+                        float* color, unsigned long long* counts, hipStream_t s);
 
 } // namespace cgs

@@ -25,6 +25,7 @@
 
 #include "cgs_internal.hpp"
 #include "cgs_scene_dev.hpp"
+#include "cgs_stage.hpp"
 
 using namespace cgs;
 
@@ -47,14 +48,93 @@ static cgs_status fail(cgs_status s, const std::string& msg)
             return fail(CGS_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e));     \
     } while (0)
 
+struct DeviceBuffer
+{
+    void* p = nullptr;
+    ~DeviceBuffer()
+    {
+        if (p) (void)hipFree(p);
+    }
+    hipError_t alloc(size_t bytes)
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        return hipMalloc(&p, std::max<size_t>(bytes, 256));
+    }
+    template<class T> T* as() const { return (T*)p; }
+};
+
+// Buffers of the staged tracer, grown on demand and reused across calls.
+struct StageBuffers
+{
+    static constexpr int kFields = 11;
+    size_t cap[3] = {0, 0, 0};            // slots of batch 0, 1 (closest) and 2 (shadow)
+    DeviceBuffer mem[3][kFields];
+    RayBatch batch[3] = {};
+    size_t listCap = 0;
+    DeviceBuffer lists[5];                // 0: batch-0 actives, 1: batch-1 actives, 2: shadow actives, 3/4: ping-pong
+    size_t reqCap = 0;
+    DeviceBuffer req;
+    DeviceBuffer counters;                // 32 u32 counters (+ u64 scratch)
+    size_t pixCap = 0;
+    uint32_t nlightCap = 0;
+    DeviceBuffer pix[7];
+    PixelState px = {};
+    uint32_t gridTraverse = 0, gridClip = 0, gridFinish = 0;
+    // staging policy: at most `maxRounds` traverse/clip rounds, and only while
+    // more than `tailLanes` × (in-place grid lanes) rays are active
+    int maxRounds = 1;
+    float tailLanes = 1.0f;
+};
+
 struct cgs_ctx
 {
     int device = 0;
+    int numCUs = 0;
     hipEvent_t renderDone = nullptr;
     bool renderLaunched = false;
     unsigned long long* dOverflow = nullptr; // device counter of overflowed clip stacks
+    unsigned int* dCounters = nullptr;       // one work counter per persistent kernel
+    float2* dStack = nullptr;                // clipping region stacks of the resident lanes
+    size_t stackLanes = 0;
+    uint32_t grid[kPersistKernels] = {};
+    StageBuffers stage;
+    unsigned int* hCount = nullptr;          // pinned host word for count read-backs
     std::mutex mu;
 };
+
+// Grid, stack slice and zeroed work counter for one persistent launch.
+// Launches of one context are serialized by the caller (they share the
+// stack buffer and counters).
+static cgs_status persistArgs(cgs_ctx* ctx, PersistKernel k, hipStream_t s, PersistArgs& pa)
+{
+    const int ki = (int)k;
+    if (ctx->grid[ki] == 0)
+    {
+        uint32_t bpc = 1;
+        HIP_TRY(persistentGrid(k, bpc));
+        ctx->grid[ki] = bpc * (uint32_t)ctx->numCUs;
+    }
+    const size_t lanes = (size_t)ctx->grid[ki] * 256;
+    if (lanes > ctx->stackLanes)
+    {
+        if (ctx->dStack)
+        {
+            HIP_TRY(hipDeviceSynchronize());
+            HIP_TRY(hipFree(ctx->dStack));
+            ctx->dStack = nullptr;
+            ctx->stackLanes = 0;
+        }
+        HIP_TRY(hipMalloc(&ctx->dStack, lanes * CGS_CLIP_STACK * kStackWords * sizeof(float2)));
+        ctx->stackLanes = lanes;
+    }
+    pa.stack = ctx->dStack;
+    pa.lanes = (uint32_t)lanes;
+    pa.counter = ctx->dCounters + ki;
+    pa.grid = ctx->grid[ki];
+    HIP_TRY(hipMemsetAsync(pa.counter, 0, sizeof(unsigned int), s));
+    return CGS_OK;
+}
 
 // ---------------------------------------------------------------------------
 // BVH build (rt/BVH.cpp:9-108)
@@ -290,21 +370,6 @@ template<class F> void parallelFor(size_t n, F&& f)
     for (auto& t : ts) t.join();
 }
 
-struct DeviceBuffer
-{
-    void* p = nullptr;
-    ~DeviceBuffer()
-    {
-        if (p) (void)hipFree(p);
-    }
-    hipError_t alloc(size_t bytes)
-    {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        return hipMalloc(&p, std::max<size_t>(bytes, 256));
-    }
-    template<class T> T* as() const { return (T*)p; }
-};
 
 // Batched PHF build: jobs over keys already on the device.
 struct PhfBatch
@@ -405,6 +470,143 @@ struct cgs_scene
     }
 };
 
+
+// ---------------------------------------------------------------------------
+// staged tracer driver
+// ---------------------------------------------------------------------------
+namespace
+{
+
+enum Counter
+{
+    C_LIST0 = 0, C_LIST1 = 1, C_LIST2 = 2, C_PING = 3, C_PONG = 4, C_REQ = 5, C_WORK_T = 6, C_WORK_C = 7,
+    C_SHADOW = 8, C_NEXT0 = 0, C_NEXT1 = 1,
+};
+
+cgs_status ensureBatch(cgs_ctx* ctx, int b, size_t n)
+{
+    StageBuffers& st = ctx->stage;
+    if (st.cap[b] >= n) return CGS_OK;
+    n = std::max<size_t>(n, 1024);
+    const size_t sizes[StageBuffers::kFields] = {16, 16, 16, 16, 16, 16, 16, 16, 4, 4, 4};
+    for (int f = 0; f < StageBuffers::kFields; f++) HIP_TRY(st.mem[b][f].alloc(n * sizes[f]));
+    RayBatch& rb = st.batch[b];
+    rb.od0 = st.mem[b][0].as<float4>();
+    rb.od1 = st.mem[b][1].as<float4>();
+    rb.hit = st.mem[b][2].as<float4>();
+    rb.top = st.mem[b][3].as<uint4>();
+    rb.bot = st.mem[b][4].as<uint4>();
+    rb.lr0 = st.mem[b][5].as<float4>();
+    rb.lr1 = st.mem[b][6].as<float4>();
+    rb.lh = st.mem[b][7].as<float4>();
+    rb.flags = st.mem[b][8].as<uint32_t>();
+    rb.obj = st.mem[b][9].as<uint32_t>();
+    rb.pixel = st.mem[b][10].as<uint32_t>();
+    st.cap[b] = n;
+    return CGS_OK;
+}
+
+cgs_status ensureLists(cgs_ctx* ctx, size_t n)
+{
+    StageBuffers& st = ctx->stage;
+    if (!st.counters.p)
+    {
+        HIP_TRY(st.counters.alloc(256));
+        HIP_TRY(hipMemset(st.counters.p, 0, 256));
+    }
+    if (!ctx->hCount) HIP_TRY(hipHostMalloc((void**)&ctx->hCount, 64, hipHostMallocDefault));
+    const size_t want = std::max<size_t>(n, 1024);
+    if (st.listCap < n)
+    {
+        for (auto& l : st.lists) HIP_TRY(l.alloc(want * 4));
+        st.listCap = want;
+    }
+    if (st.reqCap < n)
+    {
+        HIP_TRY(st.req.alloc(want * 8));
+        st.reqCap = want;
+    }
+    if (st.gridTraverse == 0)
+    {
+        uint32_t b0 = 1, b1 = 1, b2 = 1;
+        HIP_TRY(stageGrid(0, b0));
+        HIP_TRY(stageGrid(1, b1));
+        HIP_TRY(stageGrid(2, b2));
+        st.gridTraverse = b0 * (uint32_t)ctx->numCUs;
+        st.gridClip = b1 * (uint32_t)ctx->numCUs;
+        st.gridFinish = b2 * (uint32_t)ctx->numCUs;
+    }
+    // region stacks for the resident lanes of the clipping kernels
+    const size_t lanes = (size_t)std::max(st.gridClip, st.gridFinish) * 256;
+    if (lanes > ctx->stackLanes)
+    {
+        if (ctx->dStack)
+        {
+            HIP_TRY(hipDeviceSynchronize());
+            HIP_TRY(hipFree(ctx->dStack));
+            ctx->dStack = nullptr;
+            ctx->stackLanes = 0;
+        }
+        HIP_TRY(hipMalloc(&ctx->dStack, lanes * CGS_CLIP_STACK * kStackWords * sizeof(float2)));
+        ctx->stackLanes = lanes;
+    }
+    return CGS_OK;
+}
+
+unsigned int* counter(cgs_ctx* ctx, int c) { return ctx->stage.counters.as<unsigned int>() + c; }
+
+cgs_status readCount(cgs_ctx* ctx, int c, hipStream_t s, uint32_t& out)
+{
+    HIP_TRY(hipMemcpyAsync(ctx->hCount, counter(ctx, c), 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    out = *ctx->hCount;
+    return CGS_OK;
+}
+
+// Runs traverse/clip rounds over batch `b` starting from the active list in
+// lists[list] (count in counter `c`) while enough rays remain to fill the
+// GPU, then finishes the remaining rays with the in-place kernel.
+cgs_status stageTrace(cgs_ctx* ctx, const SceneDev& sc, int b, int list, int c, hipStream_t s, uint32_t nStart)
+{
+    StageBuffers& st = ctx->stage;
+    const RayBatch& rb = st.batch[b];
+    ClipQueue cq {st.req.as<uint2>(), counter(ctx, C_REQ)};
+    uint32_t* cur = st.lists[list].as<uint32_t>();
+    unsigned int* curCount = counter(ctx, c);
+    int ping = 3;
+    // rays one resident grid of the in-place kernel holds, scaled by the policy
+    const double tail = (double)st.gridFinish * 256.0 * st.tailLanes;
+    uint32_t left = nStart;
+    for (int round = 0; left > 0; round++)
+    {
+        if (round >= st.maxRounds || (double)left <= tail)
+        {
+            HIP_TRY(hipMemsetAsync(counter(ctx, C_WORK_T), 0, 4, s));
+            HIP_TRY(launchFinish(sc, rb, cur, curCount, counter(ctx, C_WORK_T), ctx->dStack, (uint32_t)ctx->stackLanes,
+                                 st.gridFinish, s));
+            break;
+        }
+        HIP_TRY(hipMemsetAsync(counter(ctx, C_REQ), 0, 4 * 3, s)); // req, work_t, work_c
+        HIP_TRY(launchTraverse(sc, rb, cur, curCount, cq, counter(ctx, C_WORK_T), st.gridTraverse, s));
+        uint32_t* nxt = st.lists[ping].as<uint32_t>();
+        unsigned int* nxtCount = counter(ctx, ping == 3 ? C_PING : C_PONG);
+        HIP_TRY(hipMemsetAsync(nxtCount, 0, 4, s));
+        HIP_TRY(launchClip(sc, rb, cq, ctx->dStack, (uint32_t)ctx->stackLanes, nxt, nxtCount, counter(ctx, C_WORK_C),
+                           st.gridClip, s));
+        cur = nxt;
+        curCount = nxtCount;
+        ping = ping == 3 ? 4 : 3;
+        cgs_status e = readCount(ctx, ping == 3 ? C_PONG : C_PING, s, left);
+        if (e != CGS_OK) return e;
+    }
+    return CGS_OK;
+}
+
+cgs_status stageQuery(cgs_ctx* ctx, cgs_scene* sc, const cgs_ray* rays, uint32_t n, int any, cgs_hit* hits,
+                      int32_t* objects, uint32_t* occluded, hipStream_t s);
+
+} // namespace
+
 extern "C" {
 
 int cgs_abi_version(void) { return CGS_ABI_VERSION; }
@@ -440,8 +642,13 @@ cgs_status cgs_open(int device, cgs_ctx** out)
     auto* c = new cgs_ctx;
     c->device = device;
     hipError_t e = hipEventCreateWithFlags(&c->renderDone, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&c->numCUs, hipDeviceAttributeMultiprocessorCount, device);
     if (e == hipSuccess) e = hipMalloc(&c->dOverflow, 256);
     if (e == hipSuccess) e = hipMemset(c->dOverflow, 0, 256);
+    if (const char* v = std::getenv("CGS_STAGE_ROUNDS")) c->stage.maxRounds = std::atoi(v);
+    if (const char* v = std::getenv("CGS_STAGE_TAIL")) c->stage.tailLanes = (float)std::atof(v);
+    if (e == hipSuccess) e = hipMalloc(&c->dCounters, 256);
+    if (e == hipSuccess) e = hipMemset(c->dCounters, 0, 256);
     if (e != hipSuccess)
     {
         delete c;
@@ -457,6 +664,9 @@ cgs_status cgs_close(cgs_ctx* ctx)
     (void)hipSetDevice(ctx->device);
     if (ctx->renderDone) (void)hipEventSynchronize(ctx->renderDone), (void)hipEventDestroy(ctx->renderDone);
     if (ctx->dOverflow) (void)hipFree(ctx->dOverflow);
+    if (ctx->dCounters) (void)hipFree(ctx->dCounters);
+    if (ctx->dStack) (void)hipFree(ctx->dStack);
+    if (ctx->hCount) (void)hipHostFree(ctx->hCount);
     delete ctx;
     return CGS_OK;
 }
@@ -466,7 +676,11 @@ cgs_status cgs_bezier_clip(cgs_ctx* ctx, const cgs_ray* rays, const float* patch
 {
     if (!ctx || (n && (!rays || !patches || !hits))) return fail(CGS_ERR_INVALID, "cgs_bezier_clip: null argument");
     HIP_TRY(hipSetDevice(ctx->device));
-    HIP_TRY(launchClipBatch(rays, patches, n, tol, hits, (hipStream_t)stream));
+    std::lock_guard lk(ctx->mu);
+    PersistArgs pa;
+    cgs_status st = persistArgs(ctx, PersistKernel::ClipBatch, (hipStream_t)stream, pa);
+    if (st != CGS_OK) return st;
+    HIP_TRY(launchClipBatch(pa, rays, patches, n, tol, hits, (hipStream_t)stream));
     return CGS_OK;
 }
 
@@ -780,16 +994,16 @@ cgs_status cgs_trace_closest(cgs_ctx* ctx, cgs_scene* sc, const cgs_ray* rays, u
 {
     if (!ctx || !sc || (n && (!rays || !hits))) return fail(CGS_ERR_INVALID, "cgs_trace_closest: null argument");
     HIP_TRY(hipSetDevice(ctx->device));
-    HIP_TRY(launchTraceClosest(sc->view(), rays, n, hits, objects, ctx->dOverflow, (hipStream_t)stream));
-    return CGS_OK;
+    std::lock_guard lk(ctx->mu);
+    return stageQuery(ctx, sc, rays, n, 0, hits, objects, nullptr, (hipStream_t)stream);
 }
 
 cgs_status cgs_trace_any(cgs_ctx* ctx, cgs_scene* sc, const cgs_ray* rays, uint32_t n, uint32_t* occluded, void* stream)
 {
     if (!ctx || !sc || (n && (!rays || !occluded))) return fail(CGS_ERR_INVALID, "cgs_trace_any: null argument");
     HIP_TRY(hipSetDevice(ctx->device));
-    HIP_TRY(launchTraceAny(sc->view(), rays, n, occluded, ctx->dOverflow, (hipStream_t)stream));
-    return CGS_OK;
+    std::lock_guard lk(ctx->mu);
+    return stageQuery(ctx, sc, rays, n, 1, nullptr, nullptr, occluded, (hipStream_t)stream);
 }
 
 cgs_status cgs_render(cgs_ctx* ctx, cgs_scene* sc, const cgs_camera* cam, const cgs_render_options* op, uint32_t w,
@@ -818,8 +1032,71 @@ cgs_status cgs_render(cgs_ctx* ctx, cgs_scene* sc, const cgs_camera* cam, const 
         hdy = -hdy;
     }
     hipStream_t s = (hipStream_t)stream;
-    HIP_TRY(launchRender(sc->view(), cam->to_world, cam->position, tlx, tly, hdx, hdy, op->eps, op->recursion_depth, w, h,
-                         frame, color, (unsigned long long*)counts, s));
+    if (sc->nlight > 32) return fail(CGS_ERR_INVALID, "cgs_render: at most 32 lights");
+    const size_t npix = (size_t)w * h;
+    const uint32_t nl = std::max<uint32_t>(1, sc->nlight);
+    cgs_status st;
+    if ((st = ensureBatch(ctx, 0, npix)) != CGS_OK || (st = ensureBatch(ctx, 1, npix)) != CGS_OK ||
+        (st = ensureBatch(ctx, 2, npix * nl)) != CGS_OK || (st = ensureLists(ctx, npix * nl)) != CGS_OK)
+        return st;
+    StageBuffers& S = ctx->stage;
+    if (S.pixCap < npix || S.nlightCap < nl)
+    {
+        const size_t sz[7] = {16, 16, 16, 16, 16 * (size_t)nl, 4, 4};
+        for (int f = 0; f < 7; f++) HIP_TRY(S.pix[f].alloc(npix * sz[f]));
+        S.pixCap = npix;
+        S.nlightCap = nl;
+    }
+    PixelState px;
+    px.acc = S.pix[0].as<float4>();
+    px.att = S.pix[1].as<float4>();
+    px.attNext = S.pix[2].as<float4>();
+    px.base = S.pix[3].as<float4>();
+    px.term = S.pix[4].as<float4>();
+    px.state = S.pix[5].as<uint32_t>();
+    px.nlight = sc->nlight;
+    uint32_t* occ = S.pix[6].as<uint32_t>();
+    const SceneDev view = sc->view();
+    CamDevS cd;
+    for (int c = 0; c < 4; c++)
+        cd.c[c] = make_float4(cam->to_world[4 * c], cam->to_world[4 * c + 1], cam->to_world[4 * c + 2],
+                              cam->to_world[4 * c + 3]);
+    cd.px = cam->position[0];
+    cd.py = cam->position[1];
+    cd.pz = cam->position[2];
+    cd.tlx = tlx;
+    cd.tly = tly;
+    cd.hdx = hdx;
+    cd.hdy = hdy;
+    unsigned long long* cnt = (unsigned long long*)counts;
+    HIP_TRY(hipMemsetAsync(S.counters.p, 0, 64, s));
+    HIP_TRY(launchGenPrimary(view, S.batch[0], cd, w, h, px, S.lists[0].as<uint32_t>(), counter(ctx, C_LIST0), cnt, s));
+    int cur = 0;
+    uint32_t nCur = (uint32_t)npix;
+    for (int depth = op->recursion_depth;; depth--)
+    {
+        if ((st = stageTrace(ctx, view, cur, cur, cur == 0 ? C_LIST0 : C_LIST1, s, nCur)) != CGS_OK) return st;
+        HIP_TRY(launchZero(occ, npix, s));
+        HIP_TRY(hipMemsetAsync(counter(ctx, C_SHADOW), 0, 4, s));
+        HIP_TRY(hipMemsetAsync(counter(ctx, cur == 0 ? C_LIST1 : C_LIST0), 0, 4, s));
+        HIP_TRY(launchShade(view, S.batch[cur], nCur, px, depth, op->eps, S.batch[1 - cur],
+                            S.lists[1 - cur].as<uint32_t>(), counter(ctx, cur == 0 ? C_LIST1 : C_LIST0), S.batch[2],
+                            S.lists[2].as<uint32_t>(), counter(ctx, C_SHADOW), cnt, s));
+        uint32_t nShadow = 0, nNext = 0;
+        if ((st = readCount(ctx, C_SHADOW, s, nShadow)) != CGS_OK) return st;
+        if ((st = readCount(ctx, cur == 0 ? C_LIST1 : C_LIST0, s, nNext)) != CGS_OK) return st;
+        if (nShadow)
+        {
+            if ((st = stageTrace(ctx, view, 2, 2, C_SHADOW, s, nShadow)) != CGS_OK) return st;
+            HIP_TRY(launchScatterShadow(S.batch[2], nShadow, occ, cnt, s));
+        }
+        HIP_TRY(launchAccumulate(S.batch[cur], nCur, px, occ, s));
+        HIP_TRY(launchAdvanceAtt(S.batch[1 - cur], nNext, px, s));
+        if (nNext == 0) break;
+        cur = 1 - cur;
+        nCur = nNext;
+    }
+    HIP_TRY(launchPack(px, (uint32_t)npix, frame, color, s));
     HIP_TRY(hipEventRecord(ctx->renderDone, s));
     ctx->renderLaunched = true;
     return CGS_OK;
@@ -884,3 +1161,26 @@ cgs_status cgs_stream_sync(cgs_ctx* ctx, void* stream)
 }
 
 } // extern "C"
+
+namespace
+{
+
+cgs_status stageQuery(cgs_ctx* ctx, cgs_scene* sc, const cgs_ray* rays, uint32_t n, int any, cgs_hit* hits,
+                      int32_t* objects, uint32_t* occluded, hipStream_t s)
+{
+    if (n == 0) return CGS_OK;
+    cgs_status st;
+    if ((st = ensureBatch(ctx, 0, n)) != CGS_OK || (st = ensureLists(ctx, n)) != CGS_OK) return st;
+    StageBuffers& S = ctx->stage;
+    const SceneDev view = sc->view();
+    HIP_TRY(launchLoadRays(view, S.batch[0], rays, n, any, S.lists[0].as<uint32_t>(), s));
+    HIP_TRY(hipMemcpyAsync(counter(ctx, C_LIST0), &n, 4, hipMemcpyHostToDevice, s));
+    if ((st = stageTrace(ctx, view, 0, 0, C_LIST0, s, n)) != CGS_OK) return st;
+    if (any)
+        HIP_TRY(launchStoreAny(S.batch[0], n, occluded, ctx->dOverflow, s));
+    else
+        HIP_TRY(launchStoreClosest(S.batch[0], n, hits, objects, ctx->dOverflow, s));
+    return CGS_OK;
+}
+
+} // namespace

@@ -48,7 +48,7 @@ struct ClosestHit
 
 // Context::intersect(hit, ray) (RayTracer.cu:369-423) with
 // rt::intersect(BezierSurface, hit, ray) (Primitive.cu:355-381) at the leaves.
-CGS_DEV ClosestHit sceneClosest(const SceneDev& sc, V3 o, V3 d, float tmax, ClipStack& stk, bool& overflow)
+CGS_DEV ClosestHit sceneClosest(const SceneDev& sc, V3 o, V3 d, float tmax, const ClipStack& stk, bool& overflow)
 {
     ClosestHit h {tmax, 0.0f, 0.0f, EMPTY, -1};
     const BVHRef top = bvhRef(sc, sc.top);
@@ -86,7 +86,7 @@ CGS_DEV ClosestHit sceneClosest(const SceneDev& sc, V3 o, V3 d, float tmax, Clip
 
 // Context::intersect(ray) (RayTracer.cu:425-457) with
 // rt::intersect(BezierSurface, ray) (Primitive.cu:383-401) at the leaves.
-CGS_DEV bool sceneAny(const SceneDev& sc, V3 o, V3 d, float tmax, ClipStack& stk, bool& overflow)
+CGS_DEV bool sceneAny(const SceneDev& sc, V3 o, V3 d, float tmax, const ClipStack& stk, bool& overflow)
 {
     const BVHRef top = bvhRef(sc, sc.top);
     return hashIntersectAny(top, o, d, tmax, [&](uint32_t i) -> bool {
@@ -103,24 +103,44 @@ CGS_DEV bool sceneAny(const SceneDev& sc, V3 o, V3 d, float tmax, ClipStack& stk
     });
 }
 
-// Wave-aggregated counter increment: one atomic per wave.
-CGS_DEV void countWave(unsigned long long* ctr, bool pred)
+// Sum of a per-lane count over the wave, added with one atomic.
+CGS_DEV void addWave(unsigned long long* ctr, unsigned long long v)
 {
-    unsigned long long m = __ballot(pred);
-    if (m && (threadIdx.x & 63) == (uint32_t)__builtin_ctzll(m))
-        atomicAdd(ctr, (unsigned long long)__popcll(m));
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0 && v) atomicAdd(ctr, v);
 }
 
-__global__ __launch_bounds__(256) void kTraceClosest(SceneDev sc, const cgs_ray* __restrict__ rays, uint32_t n,
+// Persistent launches: the grid holds exactly the resident waves; each wave
+// takes chunks of 64 work items from a counter.  `lane` indexes the region
+// stack slice of this lane.
+struct Persist
+{
+    float2* stack;
+    uint32_t lanes;
+    unsigned int* counter;
+};
+
+CGS_DEV uint32_t nextChunk(unsigned int* counter)
+{
+    uint32_t base = 0;
+    if ((threadIdx.x & 63) == 0) base = atomicAdd(counter, 64u);
+    return __shfl(base, 0, 64);
+}
+
+__global__ __launch_bounds__(256) void kTraceClosest(SceneDev sc, Persist ps, const cgs_ray* __restrict__ rays, uint32_t n,
                                                      cgs_hit* __restrict__ hits, int32_t* __restrict__ objects,
                                                      unsigned long long* __restrict__ overflowCount)
 {
-    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-    bool overflow = false;
-    if (r < n)
+    const uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x;
+    const ClipStack stk = clipStackFor(ps.stack, gl, ps.lanes);
+    unsigned long long nOverflow = 0;
+    for (uint32_t base = nextChunk(ps.counter); base < n; base = nextChunk(ps.counter))
     {
+        const uint32_t r = base + (threadIdx.x & 63);
+        if (r >= n) continue;
+        bool overflow = false;
         const cgs_ray ray = rays[r];
-        ClipStack stk;
         ClosestHit h = sceneClosest(sc, {ray.ox, ray.oy, ray.oz}, {ray.dx, ray.dy, ray.dz}, ray.tmax, stk, overflow);
         cgs_hit out;
         out.t = h.t;
@@ -129,39 +149,49 @@ __global__ __launch_bounds__(256) void kTraceClosest(SceneDev sc, const cgs_ray*
         out.patch = overflow ? CGS_PATCH_OVERFLOW : (h.object < 0 ? EMPTY : h.patch);
         hits[r] = out;
         if (objects) objects[r] = h.object;
+        nOverflow += overflow;
     }
-    countWave(overflowCount, overflow);
+    addWave(overflowCount, nOverflow);
 }
 
-__global__ __launch_bounds__(256) void kTraceAny(SceneDev sc, const cgs_ray* __restrict__ rays, uint32_t n,
+__global__ __launch_bounds__(256) void kTraceAny(SceneDev sc, Persist ps, const cgs_ray* __restrict__ rays, uint32_t n,
                                                  uint32_t* __restrict__ occluded,
                                                  unsigned long long* __restrict__ overflowCount)
 {
-    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-    bool overflow = false;
-    if (r < n)
+    const uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x;
+    const ClipStack stk = clipStackFor(ps.stack, gl, ps.lanes);
+    unsigned long long nOverflow = 0;
+    for (uint32_t base = nextChunk(ps.counter); base < n; base = nextChunk(ps.counter))
     {
+        const uint32_t r = base + (threadIdx.x & 63);
+        if (r >= n) continue;
+        bool overflow = false;
         const cgs_ray ray = rays[r];
-        ClipStack stk;
         bool occ = sceneAny(sc, {ray.ox, ray.oy, ray.oz}, {ray.dx, ray.dy, ray.dz}, ray.tmax, stk, overflow);
         occluded[r] = overflow ? 2u : (occ ? 1u : 0u);
+        nOverflow += overflow;
     }
-    countWave(overflowCount, overflow);
+    addWave(overflowCount, nOverflow);
 }
 
 // Batched doBezierClipping: ray r against patch r (Spline_BezierClipping.cu:46-114).
-__global__ __launch_bounds__(256) void kClipBatch(const cgs_ray* __restrict__ rays, const float4* __restrict__ patches,
-                                                  uint32_t n, float tol, cgs_hit* __restrict__ hits)
+__global__ __launch_bounds__(256) void kClipBatch(Persist ps, const cgs_ray* __restrict__ rays,
+                                                  const float4* __restrict__ patches, uint32_t n, float tol,
+                                                  cgs_hit* __restrict__ hits)
 {
-    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n) return;
-    const cgs_ray ray = rays[r];
-    ClipStack stk;
-    bool overflow = false;
-    const ClipRay cr = makeClipRay({ray.ox, ray.oy, ray.oz}, versor3({ray.dx, ray.dy, ray.dz}));
-    float t = ray.tmax, u = 0.0f, v = 0.0f;
-    bool f = clipPatch<false>(patches + 16 * (size_t)r, cr, tol, t, u, v, stk, overflow);
-    hits[r] = {t, u, v, overflow ? CGS_PATCH_OVERFLOW : (f ? 1u : EMPTY)};
+    const uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x;
+    const ClipStack stk = clipStackFor(ps.stack, gl, ps.lanes);
+    for (uint32_t base = nextChunk(ps.counter); base < n; base = nextChunk(ps.counter))
+    {
+        const uint32_t r = base + (threadIdx.x & 63);
+        if (r >= n) continue;
+        const cgs_ray ray = rays[r];
+        bool overflow = false;
+        const ClipRay cr = makeClipRay({ray.ox, ray.oy, ray.oz}, versor3({ray.dx, ray.dy, ray.dz}));
+        float t = ray.tmax, u = 0.0f, v = 0.0f;
+        bool f = clipPatch<false>(patches + 16 * (size_t)r, cr, tol, t, u, v, stk, overflow);
+        hits[r] = {t, u, v, overflow ? CGS_PATCH_OVERFLOW : (f ? 1u : EMPTY)};
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -427,24 +457,30 @@ CGS_DEV uint32_t packSRGB(float r, float g, float b)
 
 CGS_DEV float saturate(float x) { return x > 0.0f ? (x < 1.0f ? x : 1.0f) : 0.0f; }
 
-__global__ __launch_bounds__(256) void kRender(SceneDev sc, CamDev cam, RenderDev op, uint32_t* __restrict__ frame,
-                                               float* __restrict__ color, unsigned long long* __restrict__ counts)
+__global__ __launch_bounds__(256) void kRender(SceneDev sc, Persist ps, CamDev cam, RenderDev op,
+                                               uint32_t* __restrict__ frame, float* __restrict__ color,
+                                               unsigned long long* __restrict__ counts)
 {
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t npx = op.width * op.height;
-    const bool active = k < npx;
-    const uint32_t i = active ? k % op.width : 0, j = active ? k / op.width : 0;
-    ClipStack stk;
-    bool overflow = false;
-    uint32_t nPrimary = active ? 1u : 0u, nShadow = 0, nSecondary = 0;
-    V3 acc {0.0f, 0.0f, 0.0f};
-    if (active)
+    const uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x;
+    const ClipStack stk = clipStackFor(ps.stack, gl, ps.lanes);
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t tilesX = (op.width + 7) / 8, tilesY = (op.height + 7) / 8;
+    const uint32_t nItems = tilesX * tilesY * 64;
+    unsigned long long nPrimary = 0, nShadow = 0, nSecondary = 0, nOverflow = 0;
+    M4 C;
+    C.c[0] = cam.c[0]; C.c[1] = cam.c[1]; C.c[2] = cam.c[2]; C.c[3] = cam.c[3];
+    const V3 camPos {cam.px, cam.py, cam.pz};
+    for (uint32_t base = nextChunk(ps.counter); base < nItems; base = nextChunk(ps.counter))
     {
+        // one wave = one 8×8 pixel tile
+        const uint32_t tile = base / 64;
+        const uint32_t i = (tile % tilesX) * 8 + (lane & 7), j = (tile / tilesX) * 8 + (lane >> 3);
+        if (i >= op.width || j >= op.height) continue;
+        bool overflow = false;
+        nPrimary++;
+        V3 acc {0.0f, 0.0f, 0.0f};
         // pixelRayDirection (RayTracer.cu:133-145)
         float4 P = make_float4(cam.tlx + cam.hdx * (2.0f * (float)i), cam.tly - cam.hdy * (2.0f * (float)j), -1.0f, 1.0f);
-        M4 C;
-        C.c[0] = cam.c[0]; C.c[1] = cam.c[1]; C.c[2] = cam.c[2]; C.c[3] = cam.c[3];
-        V3 camPos {cam.px, cam.py, cam.pz};
         V3 ro = camPos;
         V3 rd = versor3(sub3(project4(m4mul(C, P)), camPos));
         float rmax = __builtin_inff();
@@ -521,19 +557,14 @@ __global__ __launch_bounds__(256) void kRender(SceneDev sc, CamDev cam, RenderDe
             color[3 * (size_t)q + 2] = acc.z;
         }
         frame[q] = packSRGB(saturate(acc.x), saturate(acc.y), saturate(acc.z));
+        nOverflow += overflow;
     }
     if (counts)
     {
-        // wave reductions of the per-lane counts, one atomic per counter per wave
-        unsigned long long v[4] = {nPrimary, nShadow, nSecondary, overflow ? 1ull : 0ull};
-#pragma unroll
-        for (int c = 0; c < 4; c++)
-        {
-            unsigned long long s = v[c];
-#pragma unroll
-            for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-            if ((threadIdx.x & 63) == 0 && s) atomicAdd(counts + c, s);
-        }
+        addWave(counts + 0, nPrimary);
+        addWave(counts + 1, nShadow);
+        addWave(counts + 2, nSecondary);
+        addWave(counts + 3, nOverflow);
     }
 }
 
@@ -547,26 +578,46 @@ namespace cgs
 
 static inline uint32_t blocks(uint32_t n, uint32_t b) { return (n + b - 1) / b; }
 
-hipError_t launchTraceClosest(const SceneDev& sc, const cgs_ray* rays, uint32_t n, cgs_hit* hits, int32_t* objects,
-                              unsigned long long* overflow, hipStream_t s)
+// Resident workgroups per CU of each persistent kernel (occupancy API).
+hipError_t persistentGrid(PersistKernel k, uint32_t& blocksPerCU)
+{
+    int n = 0;
+    hipError_t e = hipSuccess;
+    switch (k)
+    {
+    case PersistKernel::TraceClosest: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kTraceClosest, 256, 0); break;
+    case PersistKernel::TraceAny: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kTraceAny, 256, 0); break;
+    case PersistKernel::ClipBatch: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kClipBatch, 256, 0); break;
+    case PersistKernel::Render: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kRender, 256, 0); break;
+    }
+    blocksPerCU = n > 0 ? (uint32_t)n : 1u;
+    return e;
+}
+
+hipError_t launchTraceClosest(const SceneDev& sc, const PersistArgs& pa, const cgs_ray* rays, uint32_t n, cgs_hit* hits,
+                              int32_t* objects, unsigned long long* overflow, hipStream_t s)
 {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(kTraceClosest, dim3(blocks(n, 256)), dim3(256), 0, s, sc, rays, n, hits, objects, overflow);
+    Persist ps {pa.stack, pa.lanes, pa.counter};
+    hipLaunchKernelGGL(kTraceClosest, dim3(pa.grid), dim3(256), 0, s, sc, ps, rays, n, hits, objects, overflow);
     return hipGetLastError();
 }
 
-hipError_t launchTraceAny(const SceneDev& sc, const cgs_ray* rays, uint32_t n, uint32_t* occluded,
+hipError_t launchTraceAny(const SceneDev& sc, const PersistArgs& pa, const cgs_ray* rays, uint32_t n, uint32_t* occluded,
                           unsigned long long* overflow, hipStream_t s)
 {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(kTraceAny, dim3(blocks(n, 256)), dim3(256), 0, s, sc, rays, n, occluded, overflow);
+    Persist ps {pa.stack, pa.lanes, pa.counter};
+    hipLaunchKernelGGL(kTraceAny, dim3(pa.grid), dim3(256), 0, s, sc, ps, rays, n, occluded, overflow);
     return hipGetLastError();
 }
 
-hipError_t launchClipBatch(const cgs_ray* rays, const float* patches, uint32_t n, float tol, cgs_hit* hits, hipStream_t s)
+hipError_t launchClipBatch(const PersistArgs& pa, const cgs_ray* rays, const float* patches, uint32_t n, float tol,
+                           cgs_hit* hits, hipStream_t s)
 {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(kClipBatch, dim3(blocks(n, 256)), dim3(256), 0, s, rays, (const float4*)patches, n, tol, hits);
+    Persist ps {pa.stack, pa.lanes, pa.counter};
+    hipLaunchKernelGGL(kClipBatch, dim3(pa.grid), dim3(256), 0, s, ps, rays, (const float4*)patches, n, tol, hits);
     return hipGetLastError();
 }
 
@@ -579,9 +630,9 @@ hipError_t launchSubdivBatch(const cgs_ray* rays, const float* patches, uint32_t
     return hipGetLastError();
 }
 
-hipError_t launchRender(const SceneDev& sc, const float* toWorld16, const float* pos3, float tlx, float tly, float hdx,
-                        float hdy, float eps, int depth, uint32_t w, uint32_t h, uint32_t* frame, float* color,
-                        unsigned long long* counts, hipStream_t s)
+hipError_t launchRender(const SceneDev& sc, const PersistArgs& pa, const float* toWorld16, const float* pos3, float tlx,
+                        float tly, float hdx, float hdy, float eps, int depth, uint32_t w, uint32_t h, uint32_t* frame,
+                        float* color, unsigned long long* counts, hipStream_t s)
 {
     CamDev cam;
     for (int c = 0; c < 4; c++)
@@ -594,9 +645,9 @@ hipError_t launchRender(const SceneDev& sc, const float* toWorld16, const float*
     cam.hdx = hdx;
     cam.hdy = hdy;
     RenderDev op {eps, depth, w, h};
-    uint32_t n = w * h;
-    if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(kRender, dim3(blocks(n, 256)), dim3(256), 0, s, sc, cam, op, frame, color, counts);
+    if ((uint64_t)w * h == 0) return hipSuccess;
+    Persist ps {pa.stack, pa.lanes, pa.counter};
+    hipLaunchKernelGGL(kRender, dim3(pa.grid), dim3(256), 0, s, sc, ps, cam, op, frame, color, counts);
     return hipGetLastError();
 }
 

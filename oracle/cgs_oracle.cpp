@@ -182,6 +182,34 @@ int orc_triangle_origin(const float* P, const float* A, const float* B, float* o
     return ok;
 }
 
+// PBR building blocks (rt/PBR.{h,cu}).  kind 0: schlick(f0 rgb, dotLH) → rgb;
+// 1: G(dotNL, dotNV, r); 2: D(dotNH, r); 3: microfacet(dotNV, dotNL, dotNH, r);
+// 4: lightColor(d, light[12]) → rgb; 5: lightVector(P xyz, light[12]) →
+// (ok, d, L xyz).
+int orc_pbr(int kind, const float* a, float* out)
+{
+    auto light = [&](const float* l) {
+        return Light {{l[0], l[1], l[2]}, l[3], {l[4], l[5], l[6]}, l[7], {l[8], l[9], l[10]}, l[11]};
+    };
+    switch (kind)
+    {
+    case 0: { f3 r = schlick({a[0], a[1], a[2]}, a[3]); out[0] = r.x; out[1] = r.y; out[2] = r.z; return 0; }
+    case 1: out[0] = Gsmith(a[0], a[1], a[2]); return 0;
+    case 2: out[0] = Dggx(a[0], a[1]); return 0;
+    case 3: out[0] = microfacet(a[0], a[1], a[2], a[3]); return 0;
+    case 4: { f3 r = lightColor(a[0], light(a + 1)); out[0] = r.x; out[1] = r.y; out[2] = r.z; return 0; }
+    case 5:
+    {
+        float d = 0;
+        f3 L {0, 0, 0};
+        bool ok = lightVector(d, L, {a[0], a[1], a[2]}, light(a + 3));
+        out[0] = ok ? 1.0f : 0.0f; out[1] = d; out[2] = L.x; out[3] = L.y; out[4] = L.z;
+        return 0;
+    }
+    default: return 1;
+    }
+}
+
 float orc_fminf(float a, float b) { return fmin_(a, b); }
 float orc_fmaxf(float a, float b) { return fmax_(a, b); }
 
@@ -196,7 +224,8 @@ int orc_newton(const float* patch, const double* o, const double* d_in, double* 
     for (int k = 0; k < iters; k++)
     {
         V3<double> F = project(interpolateDC<double>(S, u, v)) - O - t * D;
-        V3<double> Su = derivativeU_DC<double>(S, u, v).xyz(), Sv = derivativeV_DC<double>(S, u, v).xyz();
+        // deCasteljauDx returns the hodograph without its factor 3 (Spline.h:183-195)
+        V3<double> Su = 3.0 * derivativeU_DC<double>(S, u, v).xyz(), Sv = 3.0 * derivativeV_DC<double>(S, u, v).xyz();
         // J = [Su Sv -D]; solve J·delta = -F (Cramer)
         V3<double> c0 = Su, c1 = Sv, c2 = -D;
         double det = c0.dot(c1.cross(c2));
@@ -422,7 +451,8 @@ int orc_scene_bvh_copy(const orc_scene* h, int surface, uint32_t* nodes16, uint3
 
 // World-space rays (n × 8: o, tmax, d, pad; d is used as given, as the
 // RayTracer's Ray).  any = 0: closest hit → hits (t,u,v) + patch + object;
-// any = 1: occlusion → object[r] = 1 when occluded.
+// any = 1: occlusion → object[r] = 1 when occluded; any = 2: closest hit with
+// the explicit-stack traversal of rt/BVH.cpp:126-189.
 int orc_scene_trace(const orc_scene* h, const float* rays, uint64_t n, int any, float* hits3,
     uint32_t* patch, int32_t* object, orc_work* work, int threads)
 {
@@ -434,7 +464,7 @@ int orc_scene_trace(const orc_scene* h, const float* rays, uint64_t n, int any, 
         {
             const float* ry = rays + 8 * r;
             Ray ray {{ry[0], ry[1], ry[2]}, {ry[4], ry[5], ry[6]}, ry[3]};
-            if (any)
+            if (any == 1)
             {
                 object[r] = sceneIntersectAny(h->s, ray, &w) ? 1 : 0;
                 continue;
@@ -442,7 +472,7 @@ int orc_scene_trace(const orc_scene* h, const float* rays, uint64_t n, int any, 
             Hit hit;
             hit.t = ray.max;
             uint32_t pid = EMPTY;
-            int o = sceneIntersect(h->s, hit, ray, &pid, &w);
+            int o = any == 2 ? sceneIntersectStack(h->s, hit, ray, &pid) : sceneIntersect(h->s, hit, ray, &pid, &w);
             hits3[3 * r] = o < 0 ? ray.max : hit.t;
             hits3[3 * r + 1] = o < 0 ? 0.0f : hit.coordinates.x;
             hits3[3 * r + 2] = o < 0 ? 0.0f : hit.coordinates.y;

@@ -62,6 +62,7 @@ def lib():
         L.orc_slice_matrix.argtypes = [_P, _P]
         L.orc_x_axis_intersection.argtypes = [_P, _P, _P]
         L.orc_triangle_origin.argtypes = [_P, _P, _P, _P]
+        L.orc_pbr.argtypes = [C.c_int, _P, _P]
         L.orc_fminf.argtypes = [_F32, _F32]
         L.orc_fminf.restype = _F32
         L.orc_fmaxf.argtypes = [_F32, _F32]
@@ -163,6 +164,16 @@ def triangle_origin(P, A, B):
     return bool(ok), out
 
 
+PBR_KINDS = {"schlick": 0, "G": 1, "D": 2, "microfacet": 3, "light_color": 4, "light_vector": 5}
+
+
+def pbr(kind, *args):
+    a = np.array(args, np.float32).ravel()
+    out = np.zeros(8, np.float32)
+    assert lib().orc_pbr(PBR_KINDS[kind], _ptr(a), _ptr(out)) == 0
+    return out
+
+
 def fminf(a, b):
     return lib().orc_fminf(a, b)
 
@@ -259,7 +270,7 @@ class OracleScene:
         return dict(nodes=nodes, keys=keys, indices=idx, A=int(p[0]), B=int(p[1]), mask=int(p[2]),
                     table_size=int(p[3]), attempts=int(p[4]), displacement=D, table=T)
 
-    def trace(self, rays, any_hit=False, threads=None):
+    def trace(self, rays, any_hit=False, threads=None, stack=False):
         rays = np.ascontiguousarray(rays, np.float32)
         n = rays.shape[0]
         hits = np.zeros((n, 3), np.float32)
@@ -267,7 +278,8 @@ class OracleScene:
         obj = np.zeros(n, np.int32)
         w = Work()
         th = threads or os.cpu_count() or 1
-        lib().orc_scene_trace(self.h, _ptr(rays), n, int(any_hit), _ptr(hits), _ptr(patch), _ptr(obj), C.byref(w), th)
+        mode = 1 if any_hit else (2 if stack else 0)
+        lib().orc_scene_trace(self.h, _ptr(rays), n, mode, _ptr(hits), _ptr(patch), _ptr(obj), C.byref(w), th)
         if any_hit:
             return obj.astype(bool), w.asdict()
         return hits, patch, obj, w.asdict()

@@ -207,6 +207,54 @@ inline int sceneIntersect(const Scene& sc, Hit& hit0, const Ray& ray0, uint32_t*
     return nearest;
 }
 
+// The same closest-hit query with the explicit-stack traversal of
+// rt/BVH.cpp:126-189 at both levels (a cross-check of the hashed one).
+inline int sceneIntersectStack(const Scene& sc, Hit& hit0, const Ray& ray0, uint32_t* patchOut)
+{
+    int nearest = -1;
+    auto fn = [&](Hit& hit, uint32_t i) -> bool
+    {
+        const Object& o = sc.objects[i];
+        float d;
+        Ray lr = localRay(o, ray0, d);
+        Hit lh;
+        lh.t = lr.max;
+        const Surface& s = sc.surfaces[o.surface];
+        const f3 dir = versor(lr.direction);
+        auto leaf = [&](Hit& h, uint32_t index) -> bool
+        {
+            PatchHit<float> ph {h.t, {h.coordinates.x, h.coordinates.y}};
+            f4 S[16];
+            s.patch(index, S);
+            if (doBezierClipping<float>(ph, lr.origin, dir, S))
+            {
+                h.index = index;
+                h.t = ph.distance;
+                h.coordinates = {ph.p.x, ph.p.y, 0.0f};
+                return true;
+            }
+            return false;
+        };
+        bool b = stackIntersect(s.view(), lh, lr, leaf);
+        if (b)
+        {
+            float t = lh.t * d;
+            if (t < hit.t)
+            {
+                hit.coordinates = lh.coordinates;
+                hit.index = lh.index;
+                hit.t = t;
+                nearest = (int)i;
+                return true;
+            }
+        }
+        return false;
+    };
+    stackIntersect(sc.topView(), hit0, ray0, fn);
+    if (patchOut) *patchOut = hit0.index;
+    return nearest;
+}
+
 // RayTracer.cu:425-457 — Context::intersect(ray): any hit.
 inline bool sceneIntersectAny(const Scene& sc, const Ray& ray0, Work* w)
 {
