@@ -61,6 +61,23 @@ def build(verbose: bool = False, jobs: int = 3, defines=(), lib_path: str = LIB)
     return lib_path
 
 
+def build_facade_check(verbose: bool = False) -> str:
+    """Compiles tests/cpp/facade_check (the C++ facade driven like the
+    reference's callers) with g++ against the in-tree libcgs.so."""
+    root = os.path.dirname(HERE)
+    src = os.path.join(root, "tests", "cpp", "facade_check.cpp")
+    exe = os.path.join(root, "tests", "cpp", "facade_check")
+    cmd = ["g++", "-std=c++20", "-O2", "-Wall", "-I", os.path.join(root, "include"), src, "-o", exe + ".tmp",
+           "-L", HERE, "-lcgs", "-Wl,-rpath,$ORIGIN/../../cgspline_amd", "-Wl,-rpath-link,/opt/rocm/lib"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+    if r.returncode != 0:
+        raise RuntimeError("facade_check build failed:\n" + r.stdout.decode(errors="replace"))
+    os.replace(exe + ".tmp", exe)
+    return exe
+
+
 def _wait(item):
     src, p = item
     out, _ = p.communicate()
@@ -74,3 +91,4 @@ def _wait(item):
 
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_facade_check(verbose=True))

@@ -93,6 +93,7 @@ struct cgs_ctx
     int numCUs = 0;
     hipEvent_t renderDone = nullptr;
     bool renderLaunched = false;
+    bool renderPersistent = false;           // render with the one-kernel persistent path
     unsigned long long* dOverflow = nullptr; // device counter of overflowed clip stacks
     unsigned int* dCounters = nullptr;       // one work counter per persistent kernel
     float2* dStack = nullptr;                // clipping region stacks of the resident lanes
@@ -647,6 +648,7 @@ cgs_status cgs_open(int device, cgs_ctx** out)
     if (e == hipSuccess) e = hipMemset(c->dOverflow, 0, 256);
     if (const char* v = std::getenv("CGS_STAGE_ROUNDS")) c->stage.maxRounds = std::atoi(v);
     if (const char* v = std::getenv("CGS_STAGE_TAIL")) c->stage.tailLanes = (float)std::atof(v);
+    if (const char* v = std::getenv("CGS_RENDER_PERSISTENT")) c->renderPersistent = std::atoi(v) != 0;
     if (e == hipSuccess) e = hipMalloc(&c->dCounters, 256);
     if (e == hipSuccess) e = hipMemset(c->dCounters, 0, 256);
     if (e != hipSuccess)
@@ -1033,6 +1035,17 @@ cgs_status cgs_render(cgs_ctx* ctx, cgs_scene* sc, const cgs_camera* cam, const 
     }
     hipStream_t s = (hipStream_t)stream;
     if (sc->nlight > 32) return fail(CGS_ERR_INVALID, "cgs_render: at most 32 lights");
+    if (ctx->renderPersistent)
+    {
+        PersistArgs pa;
+        cgs_status pst = persistArgs(ctx, PersistKernel::Render, s, pa);
+        if (pst != CGS_OK) return pst;
+        HIP_TRY(launchRender(sc->view(), pa, cam->to_world, cam->position, tlx, tly, hdx, hdy, op->eps,
+                             op->recursion_depth, w, h, frame, color, (unsigned long long*)counts, s));
+        HIP_TRY(hipEventRecord(ctx->renderDone, s));
+        ctx->renderLaunched = true;
+        return CGS_OK;
+    }
     const size_t npix = (size_t)w * h;
     const uint32_t nl = std::max<uint32_t>(1, sc->nlight);
     cgs_status st;

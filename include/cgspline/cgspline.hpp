@@ -1,0 +1,457 @@
+// SPDX-License-Identifier: MIT
+//
+// cgspline.hpp — C++ facade of libcgs with the reference's source-level names
+// (namespaces cg::spline and cg::rt), header-only, over the C ABI in cgs.h.
+//
+// The facade keeps the reference's argument meaning and error behaviour:
+//   * PerfectHashFunction::build returns non-zero on failure
+//     (rt/PerfectHashFunction.h:47-53);
+//   * BVH::buildHashTable throws std::runtime_error when the hash fails
+//     (rt/BVH.cpp:24-38); BVH::build throws std::logic_error("invalid") on an
+//     empty split (rt/BVH.cpp:84-88);
+//   * RayTracer::render on a busy tracer prints a warning and does nothing
+//     (rt/RayTracer.cu:184-190);
+//   * every other failure of the library throws cg::rt::Error (the reference
+//     aborts on CUDA errors, CUDAUtility.cpp:8-22).
+// Device memory is plain HIP device memory; streams are hipStream_t.
+//
+// What the C ABI cannot carry stays out: hashIntersect/intersect with a
+// user predicate (templates over arbitrary callables) are instantiated inside
+// the library for the Bézier-surface predicate (trace and render entry
+// points) and are not exposed as templates.
+//
+#pragma once
+
+#include <cstdint>
+#include <cstdio>
+#include <optional>
+#include <span>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../cgs.h"
+
+struct ihipStream_t;
+
+namespace cg
+{
+namespace rt
+{
+
+using stream_t = ::ihipStream_t*;   // hipStream_t
+
+class Error : public std::runtime_error
+{
+public:
+    Error(cgs_status s, const std::string& what)
+        : std::runtime_error(what + ": " + cgs_status_string(s) + " (" + cgs_last_error() + ")"), status(s)
+    {}
+    cgs_status status;
+};
+
+inline void check(cgs_status s, const char* what)
+{
+    if (s != CGS_OK) throw Error(s, what);
+}
+
+// One opened GPU (RayTracer::Options::device).  Owns the library context:
+// work queues, region stacks and staging buffers of the device path.
+class Device
+{
+public:
+    explicit Device(int device = 0) { check(cgs_open(device, &_ctx), "cgs_open"); }
+    ~Device()
+    {
+        if (_ctx) (void)cgs_close(_ctx);
+    }
+    Device(const Device&) = delete;
+    Device& operator=(const Device&) = delete;
+    Device(Device&& o) noexcept : _ctx(std::exchange(o._ctx, nullptr)) {}
+
+    cgs_ctx* handle() const noexcept { return _ctx; }
+
+    void* allocate(size_t bytes) const
+    {
+        void* p = nullptr;
+        check(cgs_malloc(_ctx, bytes, &p), "cgs_malloc");
+        return p;
+    }
+    void deallocate(void* p) const noexcept { (void)cgs_free(_ctx, p); }
+    void upload(void* dst, const void* src, size_t bytes, stream_t s = nullptr) const
+    {
+        check(cgs_memcpy_h2d(_ctx, dst, src, bytes, s), "cgs_memcpy_h2d");
+    }
+    void download(void* dst, const void* src, size_t bytes, stream_t s = nullptr) const
+    {
+        check(cgs_memcpy_d2h(_ctx, dst, src, bytes, s), "cgs_memcpy_d2h");
+    }
+    void synchronize(stream_t s = nullptr) const { check(cgs_stream_sync(_ctx, s), "cgs_stream_sync"); }
+
+private:
+    cgs_ctx* _ctx = nullptr;
+};
+
+// Device array of T (the reference's managed Array/Buffer, here explicit
+// device memory owned by one Device).
+template<class T>
+class DeviceArray
+{
+public:
+    DeviceArray() = default;
+    DeviceArray(const Device& d, size_t n) : _dev(&d), _n(n) { _p = (T*)d.allocate(sizeof(T) * (n ? n : 1)); }
+    DeviceArray(const Device& d, std::span<const T> host, stream_t s = nullptr) : DeviceArray(d, host.size())
+    {
+        if (_n) d.upload(_p, host.data(), sizeof(T) * _n, s);
+    }
+    ~DeviceArray() { reset(); }
+    DeviceArray(const DeviceArray&) = delete;
+    DeviceArray& operator=(const DeviceArray&) = delete;
+    DeviceArray(DeviceArray&& o) noexcept
+        : _dev(o._dev), _p(std::exchange(o._p, nullptr)), _n(std::exchange(o._n, 0))
+    {}
+    DeviceArray& operator=(DeviceArray&& o) noexcept
+    {
+        if (this != &o)
+        {
+            reset();
+            _dev = o._dev;
+            _p = std::exchange(o._p, nullptr);
+            _n = std::exchange(o._n, 0);
+        }
+        return *this;
+    }
+    void reset() noexcept
+    {
+        if (_p) _dev->deallocate(_p);
+        _p = nullptr;
+        _n = 0;
+    }
+    T* data() const noexcept { return _p; }
+    size_t size() const noexcept { return _n; }
+    std::vector<T> toHost(stream_t s = nullptr) const
+    {
+        std::vector<T> h(_n);
+        if (_n) _dev->download(h.data(), _p, sizeof(T) * _n, s);
+        if (s) _dev->synchronize(s);
+        return h;
+    }
+
+private:
+    const Device* _dev = nullptr;
+    T* _p = nullptr;
+    size_t _n = 0;
+};
+
+// ---------------------------------------------------------------------------
+// rt::PerfectHashFunction (rt/PerfectHashFunction.h:30-70)
+// ---------------------------------------------------------------------------
+class PerfectHashFunction
+{
+public:
+    static constexpr auto EMPTY = uint32_t(-1);
+
+    explicit PerfectHashFunction(const Device& device, uint32_t seed = 42) : _dev(&device), _seed(seed) {}
+
+    // Builds h(k) = (A·k + B + D[k & mask]) mod T for the keys (host span, as
+    // the reference's managed span is host-readable).  0 on success, non-zero
+    // on failure like the reference (PerfectHashFunction.cu:486-503).
+    int build(std::span<const uint32_t> keys, stream_t stream = nullptr)
+    {
+        DeviceArray<uint32_t> k(*_dev, keys, stream);
+        return buildDevice(k.data(), (uint32_t)keys.size(), stream);
+    }
+
+    // Same, keys already in device memory.
+    int buildDevice(const uint32_t* deviceKeys, uint32_t n, stream_t stream = nullptr)
+    {
+        _displacement = DeviceArray<uint32_t>(*_dev, cgs_phf_displacement_size(n));
+        cgs_status s = cgs_phf_build(_dev->handle(), deviceKeys, n, _seed, _displacement.data(), &_p, stream);
+        if (s == CGS_ERR_HASH_FAILED || s == CGS_ERR_HASH_DUPLICATE) return 1;
+        check(s, "PerfectHashFunction::build");
+        _host = _displacement.toHost(stream);
+        _dev->synchronize(stream);
+        return 0;
+    }
+
+    uint32_t operator()(uint32_t key) const { return hash(key); }
+    // Host evaluation (the reference's hash is __host__ __device__).
+    uint32_t hash(uint32_t key) const { return (_p.A * key + _p.B + _host[key & _p.mask]) % _p.table_size; }
+
+    size_t displacementSize() const noexcept { return _p.displacement_size; }
+    uint32_t tableSize() const noexcept { return _p.table_size; }
+    const cgs_phf& parameters() const noexcept { return _p; }
+    const uint32_t* deviceDisplacement() const noexcept { return _displacement.data(); }
+
+private:
+    const Device* _dev;
+    uint32_t _seed;
+    cgs_phf _p {31, 19, 0, 0, 0, 0};
+    DeviceArray<uint32_t> _displacement;
+    std::vector<uint32_t> _host;
+};
+
+// ---------------------------------------------------------------------------
+// rt::BVH (rt/BVH.h:20-80)
+// ---------------------------------------------------------------------------
+class BVH
+{
+public:
+    using key_t = uint32_t;
+    using Node = cgs_bvh_node;
+    using ElementData = cgs_bvh_element;
+    static constexpr uint32_t EMPTY = CGS_EMPTY;
+
+    explicit BVH(const Device* device = nullptr) : _dev(device) {}
+
+    constexpr auto root() const { return 0; }
+    std::span<const Node> nodes() const noexcept { return _nodes; }
+    std::span<const uint32_t> indices() const noexcept { return _indices; }
+    std::span<const key_t> keys() const noexcept { return _keys; }
+    std::span<const uint32_t> table() const noexcept { return _table; }
+
+    // Median split on the host (rt/BVH.cpp:9-108).  Unlike the reference the
+    // element span is not reordered in place; the result is the same.
+    void build(std::span<ElementData> elements, uint32_t elementsPerNode = 1)
+    {
+        Node* n = nullptr;
+        uint32_t* k = nullptr;
+        uint32_t* idx = nullptr;
+        uint32_t nn = 0, ni = 0;
+        cgs_status s = cgs_bvh_build(elements.data(), (uint32_t)elements.size(), elementsPerNode, &n, &k, &nn, &idx, &ni);
+        if (s == CGS_ERR_EMPTY_SPLIT) throw std::logic_error("invalid");
+        check(s, "BVH::build");
+        _nodes.assign(n, n + nn);
+        _keys.assign(k, k + nn);
+        _indices.assign(idx, idx + ni);
+        cgs_free_host(n);
+        cgs_free_host(k);
+        cgs_free_host(idx);
+        _table.clear();
+    }
+
+    // Perfect hash over the node keys and table[h(key_i)] = i
+    // (rt/BVH.cpp:24-38), built on the device.
+    void buildHashTable(stream_t stream = nullptr)
+    {
+        if (!_dev) throw std::logic_error("BVH::buildHashTable: no device");
+        _hash.emplace(*_dev);
+        DeviceArray<uint32_t> k(*_dev, std::span<const uint32_t>(_keys), stream);
+        if (_hash->buildDevice(k.data(), (uint32_t)_keys.size(), stream) != 0)
+            throw std::runtime_error("could not build the hash table");
+        DeviceArray<uint32_t> t(*_dev, _hash->tableSize());
+        check(cgs_phf_fill_table(_dev->handle(), &_hash->parameters(), _hash->deviceDisplacement(), k.data(),
+                                 (uint32_t)_keys.size(), t.data(), stream),
+              "BVH::buildHashTable");
+        _table = t.toHost(stream);
+        _dev->synchronize(stream);
+    }
+
+    const PerfectHashFunction* hash() const noexcept { return _hash ? &*_hash : nullptr; }
+
+private:
+    const Device* _dev;
+    std::vector<Node> _nodes;
+    std::vector<key_t> _keys;
+    std::vector<uint32_t> _indices;
+    std::vector<uint32_t> _table;
+    std::optional<PerfectHashFunction> _hash;
+};
+
+// ---------------------------------------------------------------------------
+// rt::Scene (rt/Scene.h:16-159): objects, lights, Bézier surfaces
+// ---------------------------------------------------------------------------
+struct Material
+{
+    float diffuse[3];
+    float specular[3];
+    float metalness;
+    float roughness;
+};
+
+struct Light
+{
+    float color[3];
+    float strength;
+    float position[3];
+    float range = 0;
+    float direction[3];
+    float angle = 1.57079637f;   // > 1.57: point light
+};
+
+struct Camera
+{
+    float toWorld[16];   // camera→world, column-major
+    float position[3];
+    float fieldOfView;   // vertical, radians
+};
+
+// Scene content as MainWindow::convertScene fills rt::Scene
+// (MainWindow.cpp:363-492); buildBVH() builds every BVH and hash and makes
+// the scene resident on the device (Scene::buildBVH, rt/Scene.cu:218-232).
+class Scene
+{
+public:
+    struct Object
+    {
+        uint32_t surface;
+        float local2world[16];
+        float world2local[16];
+        Material material;
+    };
+
+    float backgroundColor[3] {0.1f, 0.1f, 0.1f};
+    float ambientLight[3] {0.1f, 0.1f, 0.1f};
+    uint32_t hashSeed = 42;
+    std::vector<Object> objects;
+    std::vector<Light> lights;
+
+    // A BezierSurface (rt/Primitive.h:90-116): vertices (x,y,z,w) and 16
+    // indices per patch, point(i,j) = indices[4j+i].  Returns its index.
+    uint32_t addSurface(std::span<const float> vertices4, std::span<const uint32_t> indices)
+    {
+        _vbase.push_back(_vbase.back() + (uint32_t)(vertices4.size() / 4));
+        _pbase.push_back(_pbase.back() + (uint32_t)(indices.size() / 16));
+        _vertices.insert(_vertices.end(), vertices4.begin(), vertices4.end());
+        _indices.insert(_indices.end(), indices.begin(), indices.end());
+        return (uint32_t)_vbase.size() - 2;
+    }
+
+    void buildBVH(const Device& device)
+    {
+        std::vector<uint32_t> surf;
+        std::vector<float> l2w, w2l, mat, lt;
+        for (const Object& o : objects)
+        {
+            surf.push_back(o.surface);
+            l2w.insert(l2w.end(), o.local2world, o.local2world + 16);
+            w2l.insert(w2l.end(), o.world2local, o.world2local + 16);
+            const Material& m = o.material;
+            mat.insert(mat.end(), {m.diffuse[0], m.diffuse[1], m.diffuse[2], m.specular[0], m.specular[1],
+                                   m.specular[2], m.metalness, m.roughness});
+        }
+        for (const Light& l : lights)
+            lt.insert(lt.end(), {l.color[0], l.color[1], l.color[2], l.strength, l.position[0], l.position[1],
+                                 l.position[2], l.range, l.direction[0], l.direction[1], l.direction[2], l.angle});
+        cgs_scene_desc d {};
+        d.vertices = _vertices.data();
+        d.indices = _indices.data();
+        d.surf_vbase = _vbase.data();
+        d.surf_pbase = _pbase.data();
+        d.nsurf = (uint32_t)_vbase.size() - 1;
+        d.obj_surface = surf.data();
+        d.obj_local2world = l2w.data();
+        d.obj_world2local = w2l.data();
+        d.obj_material = mat.data();
+        d.nobj = (uint32_t)objects.size();
+        d.lights = lt.data();
+        d.nlight = (uint32_t)lights.size();
+        for (int c = 0; c < 3; c++)
+        {
+            d.background[c] = backgroundColor[c];
+            d.ambient[c] = ambientLight[c];
+        }
+        d.hash_seed = hashSeed;
+        release();
+        check(cgs_scene_create(device.handle(), &d, &_scene), "Scene::buildBVH");
+    }
+
+    cgs_scene* handle() const noexcept { return _scene; }
+
+    Scene() = default;
+    ~Scene() { release(); }
+    Scene(const Scene&) = delete;
+    Scene& operator=(const Scene&) = delete;
+
+private:
+    void release() noexcept
+    {
+        if (_scene) (void)cgs_scene_destroy(_scene);
+        _scene = nullptr;
+    }
+    std::vector<float> _vertices;
+    std::vector<uint32_t> _indices;
+    std::vector<uint32_t> _vbase {0}, _pbase {0};
+    cgs_scene* _scene = nullptr;
+};
+
+// Packed-sRGB frame in device memory, row stride = width.
+struct Frame
+{
+    uint32_t* data;
+    uint32_t width, height;
+};
+
+// ---------------------------------------------------------------------------
+// rt::RayTracer (rt/RayTracer.h:13-72), non-Monte-Carlo path
+// ---------------------------------------------------------------------------
+class RayTracer
+{
+public:
+    static constexpr float c_MinRadiance = 0x1p-8f;
+
+    struct Options
+    {
+        float eps = 1e-4f;
+        bool flipYAxis = false;
+        int recursionDepth = 6;
+        int device = 0;
+    };
+
+    RayTracer() : RayTracer(Options {}) {}
+    explicit RayTracer(Options&& op) : options(std::move(op)), _dev(options.device) {}
+
+    // Asynchronous on `stream`.  Busy: warning and no-op (RayTracer.cu:184-190).
+    void render(Frame* frame, const Camera* camera, const Scene* scene, stream_t stream = nullptr,
+                cgs_ray_counts* deviceCounts = nullptr)
+    {
+        cgs_camera c {};
+        for (int i = 0; i < 16; i++) c.to_world[i] = camera->toWorld[i];
+        for (int i = 0; i < 3; i++) c.position[i] = camera->position[i];
+        c.fovy = camera->fieldOfView;
+        c.flip_y = options.flipYAxis ? 1 : 0;
+        cgs_render_options o {options.eps, options.recursionDepth};
+        cgs_status s = cgs_render(_dev.handle(), scene->handle(), &c, &o, frame->width, frame->height, frame->data,
+                                  nullptr, deviceCounts, stream);
+        if (s == CGS_ERR_BUSY)
+        {
+            std::fprintf(stderr, "CUDA ray tracer not ready: wait the previous work to finish or create a new instance\n");
+            return;
+        }
+        check(s, "RayTracer::render");
+    }
+
+    bool running() const { return cgs_render_running(_dev.handle()) != 0; }
+
+    const Device& device() const noexcept { return _dev; }
+
+    Options options;
+
+private:
+    Device _dev;
+};
+
+} // namespace rt
+
+namespace spline
+{
+
+// spline::doBezierClipping (Spline.h:488-493), batched over device arrays:
+// ray r against patch r (16 float4 control points each); hits[r].patch is 1
+// on a hit, CGS_EMPTY on a miss.
+inline void doBezierClipping(const rt::Device& dev, const cgs_ray* rays, const float* patches, uint32_t n,
+                             cgs_hit* hits, float tol = 0x1p-12f, rt::stream_t stream = nullptr)
+{
+    rt::check(cgs_bezier_clip(dev.handle(), rays, patches, n, tol, hits, stream), "doBezierClipping");
+}
+
+// spline::doSubdivision (Spline.h:495-500), batched the same way.
+inline void doSubdivision(const rt::Device& dev, const cgs_ray* rays, const float* patches, uint32_t n,
+                          cgs_hit* hits, float threshold = 0x1p-4f, rt::stream_t stream = nullptr)
+{
+    rt::check(cgs_subdivision(dev.handle(), rays, patches, n, threshold, hits, stream), "doSubdivision");
+}
+
+} // namespace spline
+} // namespace cg

@@ -1,0 +1,209 @@
+// Drives the C++ facade (include/cgspline/cgspline.hpp) the way the
+// reference's own callers use rt::BVH, rt::PerfectHashFunction, rt::Scene and
+// rt::RayTracer (MainWindow::convertScene / renderScene).  Inputs and outputs
+// are little-endian binary files written and checked by tests/test_facade.py.
+//
+//   facade_check bvh    <in> <out>   BVH::build on the host (no device)
+//   facade_check hash   <in> <out>   PerfectHashFunction + BVH::buildHashTable
+//   facade_check render <in> <out>   Scene::buildBVH + RayTracer::render
+//   facade_check errors              error behaviour without a device
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <vector>
+
+#include "cgspline/cgspline.hpp"
+
+using namespace cg;
+
+namespace
+{
+
+struct Reader
+{
+    std::ifstream in;
+    explicit Reader(const char* p) : in(p, std::ios::binary)
+    {
+        if (!in) throw std::runtime_error(std::string("cannot open ") + p);
+    }
+    template<class T> T get()
+    {
+        T v;
+        in.read((char*)&v, sizeof v);
+        if (!in) throw std::runtime_error("short input");
+        return v;
+    }
+    template<class T> std::vector<T> vec(size_t n)
+    {
+        std::vector<T> v(n);
+        in.read((char*)v.data(), sizeof(T) * n);
+        if (!in) throw std::runtime_error("short input");
+        return v;
+    }
+};
+
+struct Writer
+{
+    std::ofstream out;
+    explicit Writer(const char* p) : out(p, std::ios::binary) {}
+    template<class T> void put(const T& v) { out.write((const char*)&v, sizeof v); }
+    template<class T> void vec(std::span<const T> v) { out.write((const char*)v.data(), sizeof(T) * v.size()); }
+};
+
+int runBVH(const char* inPath, const char* outPath)
+{
+    Reader r(inPath);
+    const uint32_t n = r.get<uint32_t>(), epn = r.get<uint32_t>();
+    auto el = r.vec<rt::BVH::ElementData>(n);
+    rt::BVH bvh;
+    bvh.build(el, epn);
+    Writer w(outPath);
+    w.put((uint32_t)bvh.nodes().size());
+    w.put((uint32_t)bvh.indices().size());
+    w.vec(bvh.nodes());
+    w.vec(bvh.keys());
+    w.vec(bvh.indices());
+    return 0;
+}
+
+int runHash(const char* inPath, const char* outPath)
+{
+    Reader r(inPath);
+    const uint32_t n = r.get<uint32_t>(), epn = r.get<uint32_t>();
+    auto el = r.vec<rt::BVH::ElementData>(n);
+    rt::Device dev(0);
+    rt::BVH bvh(&dev);
+    bvh.build(el, epn);
+    bvh.buildHashTable();
+    const rt::PerfectHashFunction& h = *bvh.hash();
+    // the table maps every key's hash to the key's node, EMPTY elsewhere
+    auto keys = bvh.keys();
+    std::vector<uint32_t> hashes(keys.size());
+    for (size_t i = 0; i < keys.size(); i++)
+    {
+        hashes[i] = h.hash(keys[i]);
+        if (bvh.table()[hashes[i]] != i)
+        {
+            std::fprintf(stderr, "table[h(key %zu)] = %u\n", i, bvh.table()[hashes[i]]);
+            return 3;
+        }
+    }
+    // a second function over the same keys from the host span
+    rt::PerfectHashFunction again(dev);
+    if (again.build(keys) != 0 || again.tableSize() != h.tableSize()) return 4;
+    Writer w(outPath);
+    w.put(h.parameters());
+    w.vec(std::span<const uint32_t>(hashes));
+    w.vec(bvh.table());
+    return 0;
+}
+
+int runRender(const char* inPath, const char* outPath)
+{
+    Reader r(inPath);
+    rt::Scene scene;
+    const uint32_t nsurf = r.get<uint32_t>(), nobj = r.get<uint32_t>(), nlight = r.get<uint32_t>();
+    for (uint32_t s = 0; s < nsurf; s++)
+    {
+        const uint32_t nv = r.get<uint32_t>(), np = r.get<uint32_t>();
+        auto v = r.vec<float>(4 * (size_t)nv);
+        auto ix = r.vec<uint32_t>(16 * (size_t)np);
+        scene.addSurface(v, ix);
+    }
+    for (uint32_t o = 0; o < nobj; o++)
+    {
+        rt::Scene::Object ob {};
+        ob.surface = r.get<uint32_t>();
+        auto l2w = r.vec<float>(16), w2l = r.vec<float>(16), m = r.vec<float>(8);
+        std::memcpy(ob.local2world, l2w.data(), 64);
+        std::memcpy(ob.world2local, w2l.data(), 64);
+        std::memcpy(ob.material.diffuse, &m[0], 12);
+        std::memcpy(ob.material.specular, &m[3], 12);
+        ob.material.metalness = m[6];
+        ob.material.roughness = m[7];
+        scene.objects.push_back(ob);
+    }
+    for (uint32_t l = 0; l < nlight; l++)
+    {
+        auto v = r.vec<float>(12);
+        rt::Light li {};
+        std::memcpy(&li, v.data(), sizeof li);
+        scene.lights.push_back(li);
+    }
+    auto bg = r.vec<float>(3), amb = r.vec<float>(3);
+    std::memcpy(scene.backgroundColor, bg.data(), 12);
+    std::memcpy(scene.ambientLight, amb.data(), 12);
+    scene.hashSeed = r.get<uint32_t>();
+    rt::Camera cam {};
+    auto tw = r.vec<float>(16), pos = r.vec<float>(3);
+    std::memcpy(cam.toWorld, tw.data(), 64);
+    std::memcpy(cam.position, pos.data(), 12);
+    cam.fieldOfView = r.get<float>();
+    const uint32_t width = r.get<uint32_t>(), height = r.get<uint32_t>();
+    const float eps = r.get<float>();
+    const int32_t depth = r.get<int32_t>();
+
+    rt::RayTracer tracer({.eps = eps, .flipYAxis = false, .recursionDepth = depth, .device = 0});
+    const rt::Device& dev = tracer.device();
+    scene.buildBVH(dev);
+    rt::DeviceArray<uint32_t> pixels(dev, (size_t)width * height);
+    rt::Frame frame {pixels.data(), width, height};
+    tracer.render(&frame, &cam, &scene);
+    // a second render while the first may still run is refused with a warning
+    tracer.render(&frame, &cam, &scene);
+    dev.synchronize();
+    if (tracer.running()) return 5;
+    auto host = pixels.toHost();
+    Writer w(outPath);
+    w.vec(std::span<const uint32_t>(host));
+    return 0;
+}
+
+int runErrors()
+{
+    // BVH::build over nothing: logic_error("invalid") like BVH::wrap
+    try
+    {
+        rt::BVH bvh;
+        std::vector<rt::BVH::ElementData> none;
+        bvh.build(none);
+        return 6;
+    }
+    catch (const std::logic_error& e)
+    {
+        if (std::string(e.what()) != "invalid") return 7;
+    }
+    // no device here: opening one throws rt::Error with CGS_ERR_NO_DEVICE
+    try
+    {
+        rt::Device dev(0);
+        return 8;
+    }
+    catch (const rt::Error& e)
+    {
+        if (e.status != CGS_ERR_NO_DEVICE) return 9;
+    }
+    return 0;
+}
+
+} // namespace
+
+int main(int argc, char** argv)
+{
+    try
+    {
+        const std::string mode = argc > 1 ? argv[1] : "";
+        if (mode == "bvh" && argc == 4) return runBVH(argv[2], argv[3]);
+        if (mode == "hash" && argc == 4) return runHash(argv[2], argv[3]);
+        if (mode == "render" && argc == 4) return runRender(argv[2], argv[3]);
+        if (mode == "errors") return runErrors();
+        std::fprintf(stderr, "usage: facade_check bvh|hash|render <in> <out> | errors\n");
+        return 2;
+    }
+    catch (const std::exception& e)
+    {
+        std::fprintf(stderr, "facade_check: %s\n", e.what());
+        return 1;
+    }
+}
