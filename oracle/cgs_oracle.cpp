@@ -242,6 +242,51 @@ int orc_newton(const float* patch, const double* o, const double* d_in, double* 
     return F.length() < 1e-9 * std::max(1.0, std::fabs(t)) ? 1 : 0;
 }
 
+// FP64 clipping of FP64 rays (n × 8: o, tmax, d, pad; d normalized here) —
+// the same algorithm as orc_clip_batch's FP64 instantiation without rounding
+// the ray to float first.  out: n × 4 (t, u, v, found).
+int orc_clip_batch64(const double* rays, const float* patches, uint64_t n, double tol, double* out)
+{
+    for (uint64_t r = 0; r < n; r++)
+    {
+        const double* ry = rays + 8 * r;
+        const float* pt = patches + 64 * r;
+        V3<double> o {ry[0], ry[1], ry[2]}, d {ry[4], ry[5], ry[6]};
+        d = versor(d);
+        V4<double> S[16];
+        for (int i = 0; i < 16; i++) S[i] = {pt[4 * i], pt[4 * i + 1], pt[4 * i + 2], pt[4 * i + 3]};
+        PatchHit<double> h {ry[3], {0, 0}};
+        bool f = doBezierClipping<double>(h, o, d, S, tol, nullptr);
+        out[4 * r] = h.distance;
+        out[4 * r + 1] = h.p.x;
+        out[4 * r + 2] = h.p.y;
+        out[4 * r + 3] = f ? 1.0 : 0.0;
+    }
+    return 0;
+}
+
+// Batched orc_newton from (u,v,t) seeds (n × 3, updated in place), plus the
+// cosine |N̂·d̂| of the surface normal with the ray at the polished root
+// (grazing-ray flag) and a convergence flag per ray.
+int orc_newton_batch(const float* patches, const double* rays, uint64_t n, double* uvt, double* cosnd, int* ok,
+                     int iters)
+{
+    for (uint64_t r = 0; r < n; r++)
+    {
+        const float* pt = patches + 64 * r;
+        const double* ry = rays + 8 * r;
+        ok[r] = orc_newton(pt, ry, ry + 4, uvt + 3 * r, iters);
+        V4<double> S[16];
+        for (int i = 0; i < 16; i++) S[i] = {pt[4 * i], pt[4 * i + 1], pt[4 * i + 2], pt[4 * i + 3]};
+        const double u = uvt[3 * r], v = uvt[3 * r + 1];
+        V3<double> N = derivativeU_DC<double>(S, u, v).xyz().cross(derivativeV_DC<double>(S, u, v).xyz());
+        V3<double> D = versor(V3<double> {ry[4], ry[5], ry[6]});
+        const double ln = N.length();
+        cosnd[r] = ln > 0 ? std::fabs(N.dot(D)) / ln : 0.0;
+    }
+    return 0;
+}
+
 // --- BVH ---------------------------------------------------------------------
 struct orc_bvh
 {

@@ -68,6 +68,8 @@ def lib():
         L.orc_fmaxf.argtypes = [_F32, _F32]
         L.orc_fmaxf.restype = _F32
         L.orc_newton.argtypes = [_P, _P, _P, _P, C.c_int]
+        L.orc_clip_batch64.argtypes = [_P, _P, _U64, _F64, _P]
+        L.orc_newton_batch.argtypes = [_P, _P, _U64, _P, _P, _P, C.c_int]
         L.orc_bvh_build_patches.argtypes = [_P, _P, _U32]
         L.orc_bvh_build_patches.restype = _P
         L.orc_bvh_build_elements.argtypes = [_P, _U32, _U32]
@@ -189,6 +191,30 @@ def newton(patch, o, d, uvt, iters=30):
     x = np.array(uvt, np.float64)
     ok = lib().orc_newton(_ptr(p), _ptr(o), _ptr(d), _ptr(x), iters)
     return bool(ok), x
+
+
+def clip_batch64(rays64, patches, tol=2.0 ** -12):
+    """FP64 clipping of FP64 rays (n,8) against patches (n,16,4) → (n,4) float64 (t,u,v,found)."""
+    rays64 = np.ascontiguousarray(rays64, np.float64).reshape(-1, 8)
+    patches = np.ascontiguousarray(patches, np.float32)
+    n = rays64.shape[0]
+    out = np.zeros((n, 4), np.float64)
+    if n:
+        lib().orc_clip_batch64(_ptr(rays64), _ptr(patches), n, tol, _ptr(out))
+    return out
+
+
+def newton_batch(patches, rays64, uvt, iters=30):
+    """Polishes (u,v,t) seeds (n,3) to the FP64 root; returns (uvt, |N·d| cosine, converged)."""
+    patches = np.ascontiguousarray(patches, np.float32)
+    rays64 = np.ascontiguousarray(rays64, np.float64).reshape(-1, 8)
+    x = np.array(uvt, np.float64).reshape(-1, 3).copy()
+    n = x.shape[0]
+    cos = np.zeros(n, np.float64)
+    ok = np.zeros(n, np.int32)
+    if n:
+        lib().orc_newton_batch(_ptr(patches), _ptr(rays64), n, _ptr(x), _ptr(cos), _ptr(ok), iters)
+    return x, cos, ok.astype(bool)
 
 
 # ---------------------------------------------------------------------------

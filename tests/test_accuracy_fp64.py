@@ -1,0 +1,60 @@
+"""Accuracy of the FP32 closest-hit path against FP64 (CPU; the GPU path
+equals the FP32 oracle bit for bit, checked by the GPU parity tests).
+
+Bars: no hit/miss or patch-id disagreement outside rays that the FP64 run
+flags as grazing or seam; hit t within 1e-4 relative of the FP64 run of the
+same algorithm and of the Newton-polished root.  (u,v) is bounded by the
+clipper's own stopping rule — an absolute 2^-12 on the projected sub-patch
+diagonals — not by 1e-4: the FP64 run of the same algorithm is itself up to
+~1.4e-3 from the true root in (u,v) on these workloads, so the bar for (u,v)
+is 4e-3 (about 16 × 2^-12 for patches ~1/16 wide).
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import accuracy
+
+T_REL = 1e-4
+UV_ABS = 4e-3
+
+
+@pytest.mark.parametrize("name,step", [("c1", 8), ("c2_small", 12), ("c3_small", 12)])
+def test_fp32_path_against_fp64(request, name, step):
+    from cgspline_amd import scenes
+    wl = request.getfixturevalue(name)
+    rays = scenes.camera_rays(wl.camera)[::step]
+    r = accuracy.measure(name, wl.scene, rays)
+    print("\n" + accuracy.HEADER + "\n" + r.row())
+    assert r.hits32 > 100
+    assert r.hitmiss_unflagged == 0 and r.patch_unflagged == 0
+    assert r.max_rel_t_fp64 <= T_REL and r.max_rel_t_root <= T_REL and r.max_rel_t_root32 <= T_REL
+    assert r.max_uv_fp64 <= UV_ABS and r.max_uv_root <= UV_ABS
+    assert r.newton_failed <= max(2, r.hits64 // 1000)
+
+
+def test_planted_errors_are_caught(c2_small):
+    """Mistakes planted in the FP32 results fail the bars: a t scaled by
+    1 + 2e-4, (u,v) off by 5e-3, a patch id swapped, a hit turned into a miss."""
+    from cgspline_amd import scenes
+    sc = c2_small.scene
+    rays = scenes.camera_rays(c2_small.camera)[::16]
+    h, p, o = accuracy.fp32_closest(sc, rays)
+    f = accuracy.fp64_closest(sc, rays)
+    clean = accuracy.measure("clean", sc, rays, (h, p, o), f)
+    assert clean.hitmiss_unflagged == 0 and clean.patch_unflagged == 0 and clean.max_rel_t_fp64 <= T_REL
+    apart = np.abs(np.where(np.isfinite(f.t2), f.t2, 2 * f.t) - f.t) > 1e-3 * f.t
+    k = int(np.nonzero((o >= 0) & apart)[0][0])
+    h1 = h.copy()
+    h1[k, 0] *= np.float32(1 + 2e-4)
+    assert accuracy.measure("t", sc, rays, (h1, p, o), f).max_rel_t_fp64 > T_REL
+    h2 = h.copy()
+    h2[k, 1] += np.float32(5e-3)
+    assert accuracy.measure("uv", sc, rays, (h2, p, o), f).max_uv_fp64 > UV_ABS
+    p3 = p.copy()
+    p3[k] ^= 1
+    assert accuracy.measure("patch", sc, rays, (h, p3, o), f).patch_unflagged == 1
+    o4 = o.copy()
+    o4[k] = -1
+    assert accuracy.measure("miss", sc, rays, (h, p, o4), f).hitmiss_unflagged == 1
