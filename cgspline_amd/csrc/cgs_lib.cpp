@@ -93,7 +93,10 @@ struct cgs_ctx
     int numCUs = 0;
     hipEvent_t renderDone = nullptr;
     bool renderLaunched = false;
-    bool renderPersistent = false;           // render with the one-kernel persistent path
+    // Tracer: the persistent one-kernel path (default), or the staged
+    // wavefront path (traverse / compacted clip queue / finish kernels) when
+    // CGS_STAGED=1.  Both give the same results bit for bit.
+    bool staged = false;
     unsigned long long* dOverflow = nullptr; // device counter of overflowed clip stacks
     unsigned int* dCounters = nullptr;       // one work counter per persistent kernel
     float2* dStack = nullptr;                // clipping region stacks of the resident lanes
@@ -648,7 +651,7 @@ cgs_status cgs_open(int device, cgs_ctx** out)
     if (e == hipSuccess) e = hipMemset(c->dOverflow, 0, 256);
     if (const char* v = std::getenv("CGS_STAGE_ROUNDS")) c->stage.maxRounds = std::atoi(v);
     if (const char* v = std::getenv("CGS_STAGE_TAIL")) c->stage.tailLanes = (float)std::atof(v);
-    if (const char* v = std::getenv("CGS_RENDER_PERSISTENT")) c->renderPersistent = std::atoi(v) != 0;
+    if (const char* v = std::getenv("CGS_STAGED")) c->staged = std::atoi(v) != 0;
     if (e == hipSuccess) e = hipMalloc(&c->dCounters, 256);
     if (e == hipSuccess) e = hipMemset(c->dCounters, 0, 256);
     if (e != hipSuccess)
@@ -997,7 +1000,13 @@ cgs_status cgs_trace_closest(cgs_ctx* ctx, cgs_scene* sc, const cgs_ray* rays, u
     if (!ctx || !sc || (n && (!rays || !hits))) return fail(CGS_ERR_INVALID, "cgs_trace_closest: null argument");
     HIP_TRY(hipSetDevice(ctx->device));
     std::lock_guard lk(ctx->mu);
-    return stageQuery(ctx, sc, rays, n, 0, hits, objects, nullptr, (hipStream_t)stream);
+    if (ctx->staged) return stageQuery(ctx, sc, rays, n, 0, hits, objects, nullptr, (hipStream_t)stream);
+    if (n == 0) return CGS_OK;
+    PersistArgs pa;
+    cgs_status st = persistArgs(ctx, PersistKernel::TraceClosest, (hipStream_t)stream, pa);
+    if (st != CGS_OK) return st;
+    HIP_TRY(launchTraceClosest(sc->view(), pa, rays, n, hits, objects, ctx->dOverflow, (hipStream_t)stream));
+    return CGS_OK;
 }
 
 cgs_status cgs_trace_any(cgs_ctx* ctx, cgs_scene* sc, const cgs_ray* rays, uint32_t n, uint32_t* occluded, void* stream)
@@ -1005,7 +1014,13 @@ cgs_status cgs_trace_any(cgs_ctx* ctx, cgs_scene* sc, const cgs_ray* rays, uint3
     if (!ctx || !sc || (n && (!rays || !occluded))) return fail(CGS_ERR_INVALID, "cgs_trace_any: null argument");
     HIP_TRY(hipSetDevice(ctx->device));
     std::lock_guard lk(ctx->mu);
-    return stageQuery(ctx, sc, rays, n, 1, nullptr, nullptr, occluded, (hipStream_t)stream);
+    if (ctx->staged) return stageQuery(ctx, sc, rays, n, 1, nullptr, nullptr, occluded, (hipStream_t)stream);
+    if (n == 0) return CGS_OK;
+    PersistArgs pa;
+    cgs_status st = persistArgs(ctx, PersistKernel::TraceAny, (hipStream_t)stream, pa);
+    if (st != CGS_OK) return st;
+    HIP_TRY(launchTraceAny(sc->view(), pa, rays, n, occluded, ctx->dOverflow, (hipStream_t)stream));
+    return CGS_OK;
 }
 
 cgs_status cgs_render(cgs_ctx* ctx, cgs_scene* sc, const cgs_camera* cam, const cgs_render_options* op, uint32_t w,
@@ -1035,7 +1050,7 @@ cgs_status cgs_render(cgs_ctx* ctx, cgs_scene* sc, const cgs_camera* cam, const 
     }
     hipStream_t s = (hipStream_t)stream;
     if (sc->nlight > 32) return fail(CGS_ERR_INVALID, "cgs_render: at most 32 lights");
-    if (ctx->renderPersistent)
+    if (!ctx->staged)
     {
         PersistArgs pa;
         cgs_status pst = persistArgs(ctx, PersistKernel::Render, s, pa);

@@ -128,7 +128,14 @@ CGS_DEV uint32_t nextChunk(unsigned int* counter)
     return __shfl(base, 0, 64);
 }
 
-__global__ __launch_bounds__(256) void kTraceClosest(SceneDev sc, Persist ps, const cgs_ray* __restrict__ rays, uint32_t n,
+#ifndef CGS_RENDER_WAVES
+#define CGS_RENDER_WAVES 2
+#endif
+#ifndef CGS_TRACE_WAVES
+#define CGS_TRACE_WAVES 2
+#endif
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CGS_TRACE_WAVES))) void kTraceClosest(SceneDev sc, Persist ps, const cgs_ray* __restrict__ rays, uint32_t n,
                                                      cgs_hit* __restrict__ hits, int32_t* __restrict__ objects,
                                                      unsigned long long* __restrict__ overflowCount)
 {
@@ -154,7 +161,7 @@ __global__ __launch_bounds__(256) void kTraceClosest(SceneDev sc, Persist ps, co
     addWave(overflowCount, nOverflow);
 }
 
-__global__ __launch_bounds__(256) void kTraceAny(SceneDev sc, Persist ps, const cgs_ray* __restrict__ rays, uint32_t n,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CGS_TRACE_WAVES))) void kTraceAny(SceneDev sc, Persist ps, const cgs_ray* __restrict__ rays, uint32_t n,
                                                  uint32_t* __restrict__ occluded,
                                                  unsigned long long* __restrict__ overflowCount)
 {
@@ -457,7 +464,11 @@ CGS_DEV uint32_t packSRGB(float r, float g, float b)
 
 CGS_DEV float saturate(float x) { return x > 0.0f ? (x < 1.0f ? x : 1.0f) : 0.0f; }
 
-__global__ __launch_bounds__(256) void kRender(SceneDev sc, Persist ps, CamDev cam, RenderDev op,
+// Occupancy target of the render megakernel (waves per SIMD): the compiler
+// keeps VGPR + AGPR use within 512 / CGS_RENDER_WAVES.  Two waves per SIMD
+// (256 registers, a few spilled) measured fastest on the 1M-patch frame:
+// 55 ms against 83 ms at one wave (no spills) and 69-72 ms at three or four.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CGS_RENDER_WAVES))) void kRender(SceneDev sc, Persist ps, CamDev cam, RenderDev op,
                                                uint32_t* __restrict__ frame, float* __restrict__ color,
                                                unsigned long long* __restrict__ counts)
 {
