@@ -587,6 +587,37 @@ CGS_DEV bool nodeIntersect(float& tMin, V3 bmin, V3 bmax, V3 o, V3 Dinv)
     return tmin < tmax && tmax > 0.0001f;
 }
 
+// Upper levels of a BVH staged in LDS, indexed by heap key: node `key` is
+// lds[4 * key .. 4 * key + 3] for key < keys (keys = 0: nothing staged).
+// Reading a node from here instead of HBM/L2 and skipping the hash lookup on
+// an ascent to such a key changes neither the visit order nor any result.
+typedef __attribute__((address_space(3))) const float lds_float;
+
+struct NodeCache
+{
+    lds_float* lds;     // 16 floats per key
+    uint32_t keys;
+};
+
+CGS_DEV NodeBoxes loadNodeLds(lds_float* lds, uint32_t k)
+{
+    lds_float* q = lds + 16 * k;
+    NodeBoxes n;
+    n.lmin = {q[0], q[1], q[2]};
+    n.lmax = {q[3], q[4], q[5]};
+    n.rmin = {q[6], q[7], q[8]};
+    n.rmax = {q[9], q[10], q[11]};
+    n.left = __float_as_uint(q[12]);
+    n.right = __float_as_uint(q[13]);
+    return n;
+}
+
+CGS_DEV NodeBoxes loadNodeCached(const BVHRef& b, const NodeCache& nc, uint32_t node, uint32_t key)
+{
+    if (key < nc.keys) return loadNodeLds(nc.lds, key);
+    return loadNode(b.nodes, node);
+}
+
 CGS_DEV uint32_t hashKey(const BVHRef& b, uint32_t key)
 {
     uint32_t x = b.A * key + b.B + b.displ[key & b.mask];
@@ -597,14 +628,15 @@ CGS_DEV uint32_t hashKey(const BVHRef& b, uint32_t key)
 // caller's hit.t (read through a reference so leaf hits tighten the culling);
 // `leaf(index)` tests one element and returns whether it improved the hit.
 template<class Leaf>
-CGS_DEV bool hashIntersect(const BVHRef& b, V3 o, V3 dir, const float& limit, Leaf&& leaf)
+CGS_DEV bool hashIntersect(const BVHRef& b, V3 o, V3 dir, const float& limit, Leaf&& leaf,
+                           const NodeCache& nc = NodeCache {nullptr, 0})
 {
     const V3 Dinv {1.0f / dir.x, 1.0f / dir.y, 1.0f / dir.z};
     uint32_t node = b.root, key = 1, postponed = 0;
     bool result = false;
     while (true)
     {
-        NodeBoxes n = loadNode(b.nodes, node);
+        NodeBoxes n = loadNodeCached(b, nc, node, key);
         if (n.left != EMPTY)
         {
             float L0, R0;
@@ -644,6 +676,7 @@ CGS_DEV bool hashIntersect(const BVHRef& b, V3 o, V3 dir, const float& limit, Le
         postponed >>= levels;
         key = (key >> levels) ^ 1u;
         postponed ^= 1;
+        if (key < nc.keys) continue;    // staged in LDS: read by key, no lookup
         node = b.table[hashKey(b, key)];
         if (node >= b.nodeCount) break; // a corrupt table ends the ray instead of faulting
     }
@@ -652,13 +685,14 @@ CGS_DEV bool hashIntersect(const BVHRef& b, V3 o, V3 dir, const float& limit, Le
 
 // Stackless any-hit traversal against `tmax` (rt/BVH.h:268-350).
 template<class Leaf>
-CGS_DEV bool hashIntersectAny(const BVHRef& b, V3 o, V3 dir, float tmax, Leaf&& leaf)
+CGS_DEV bool hashIntersectAny(const BVHRef& b, V3 o, V3 dir, float tmax, Leaf&& leaf,
+                              const NodeCache& nc = NodeCache {nullptr, 0})
 {
     const V3 Dinv {1.0f / dir.x, 1.0f / dir.y, 1.0f / dir.z};
     uint32_t node = b.root, key = 1, postponed = 0;
     while (true)
     {
-        NodeBoxes n = loadNode(b.nodes, node);
+        NodeBoxes n = loadNodeCached(b, nc, node, key);
         if (n.left != EMPTY)
         {
             float L0, R0;
@@ -698,6 +732,7 @@ CGS_DEV bool hashIntersectAny(const BVHRef& b, V3 o, V3 dir, float tmax, Leaf&& 
         postponed >>= levels;
         key = (key >> levels) ^ 1u;
         postponed ^= 1;
+        if (key < nc.keys) continue;    // staged in LDS: read by key, no lookup
         node = b.table[hashKey(b, key)];
         if (node >= b.nodeCount) break; // a corrupt table ends the ray instead of faulting
     }

@@ -448,7 +448,7 @@ struct cgs_scene
     std::vector<uint32_t> nodeBase, indexBase;
     std::vector<DevBVH> dev;       // per surface, then the top level
     std::vector<cgs_phf> phf;
-    DeviceBuffer patches, nodes, indices, keys, displ, tables, surfaces, objects, lights;
+    DeviceBuffer patches, nodes, indices, keys, displ, tables, surfaces, objects, lights, topByKey;
     float background[3], ambient[3];
     SceneDev view() const
     {
@@ -461,6 +461,7 @@ struct cgs_scene
         s.surfaces = surfaces.as<DevBVH>();
         s.objects = objects.as<DevObject>();
         s.lights = lights.as<DevLight>();
+        s.topByKey = topByKey.as<uint32_t>();
         s.nlight = nlight;
         s.nobj = nobj;
         s.nodeCount = nodeCount;
@@ -898,7 +899,16 @@ cgs_status cgs_scene_create(cgs_ctx* ctx, const cgs_scene_desc* d, cgs_scene** o
         o.surface = d->obj_surface[i];
         o.pad[0] = o.pad[1] = o.pad[2] = 0;
     }
+    // upper levels of the top-level BVH by heap key (LDS staging in the kernels)
+    std::vector<uint32_t> topByKey(kTopKeyCap, CGS_EMPTY);
+    {
+        const BuiltBVH& tb = sc->bvh[S];
+        for (size_t i = 0; i < tb.keys.size(); i++)
+            if (tb.keys[i] < kTopKeyCap) topByKey[tb.keys[i]] = sc->nodeBase[S] + (uint32_t)i;
+    }
     hipStream_t s = nullptr;
+    HIP_TRY(sc->topByKey.alloc(kTopKeyCap * 4));
+    HIP_TRY(hipMemcpyAsync(sc->topByKey.p, topByKey.data(), kTopKeyCap * 4, hipMemcpyHostToDevice, s));
     HIP_TRY(sc->patches.alloc(patches.size() * 4));
     HIP_TRY(sc->nodes.alloc(nodes.size() * sizeof(cgs_bvh_node)));
     HIP_TRY(sc->indices.alloc(indices.size() * 4));

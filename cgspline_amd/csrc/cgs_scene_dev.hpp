@@ -53,6 +53,11 @@ struct DevLight
     float4 c;             // direction xyz, angle
 };
 
+// Top-level nodes with heap key below kTopKeyCap are listed by key
+// (topByKey[key] = absolute node offset, EMPTY where the tree has no such
+// key) so kernels can stage the upper levels of the top-level BVH in LDS.
+constexpr uint32_t kTopKeyCap = 4096;
+
 struct SceneDev
 {
     const float4* __restrict__ patches;
@@ -63,6 +68,7 @@ struct SceneDev
     const DevBVH* __restrict__ surfaces;
     const DevObject* __restrict__ objects;
     const DevLight* __restrict__ lights;
+    const uint32_t* __restrict__ topByKey;   // kTopKeyCap entries
     uint32_t nlight;
     uint32_t nobj;
     uint32_t nodeCount;

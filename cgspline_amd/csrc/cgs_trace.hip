@@ -48,7 +48,8 @@ struct ClosestHit
 
 // Context::intersect(hit, ray) (RayTracer.cu:369-423) with
 // rt::intersect(BezierSurface, hit, ray) (Primitive.cu:355-381) at the leaves.
-CGS_DEV ClosestHit sceneClosest(const SceneDev& sc, V3 o, V3 d, float tmax, const ClipStack& stk, bool& overflow)
+CGS_DEV ClosestHit sceneClosest(const SceneDev& sc, const NodeCache& nc, V3 o, V3 d, float tmax, const ClipStack& stk,
+                                bool& overflow)
 {
     ClosestHit h {tmax, 0.0f, 0.0f, EMPTY, -1};
     const BVHRef top = bvhRef(sc, sc.top);
@@ -80,13 +81,14 @@ CGS_DEV ClosestHit sceneClosest(const SceneDev& sc, V3 o, V3 d, float tmax, cons
             }
         }
         return false;
-    });
+    }, nc);
     return h;
 }
 
 // Context::intersect(ray) (RayTracer.cu:425-457) with
 // rt::intersect(BezierSurface, ray) (Primitive.cu:383-401) at the leaves.
-CGS_DEV bool sceneAny(const SceneDev& sc, V3 o, V3 d, float tmax, const ClipStack& stk, bool& overflow)
+CGS_DEV bool sceneAny(const SceneDev& sc, const NodeCache& nc, V3 o, V3 d, float tmax, const ClipStack& stk,
+                      bool& overflow)
 {
     const BVHRef top = bvhRef(sc, sc.top);
     return hashIntersectAny(top, o, d, tmax, [&](uint32_t i) -> bool {
@@ -100,7 +102,25 @@ CGS_DEV bool sceneAny(const SceneDev& sc, V3 o, V3 d, float tmax, const ClipStac
             float best = lr.max, bu, bv;
             return clipPatch<true>(S, cr, kClipTol, best, bu, bv, stk, overflow);
         });
-    });
+    }, nc);
+}
+
+#ifndef CGS_TOP_LDS_KEYS
+#define CGS_TOP_LDS_KEYS 1024
+#endif
+static_assert(CGS_TOP_LDS_KEYS <= kTopKeyCap, "top-level LDS staging beyond the key map");
+
+// Stages the top-level BVH's nodes with heap key < CGS_TOP_LDS_KEYS (its upper
+// levels, visited by nearly every ray) in LDS: 64 B per key.
+CGS_DEV NodeCache stageTopLevel(const SceneDev& sc, float4* lds)
+{
+    for (uint32_t k = threadIdx.x; k < 4 * CGS_TOP_LDS_KEYS; k += blockDim.x)
+    {
+        const uint32_t node = sc.topByKey[k >> 2];
+        lds[k] = node != EMPTY ? sc.nodes[4 * (size_t)node + (k & 3)] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+    __syncthreads();
+    return NodeCache {(lds_float*)(float*)lds, CGS_TOP_LDS_KEYS};
 }
 
 // Sum of a per-lane count over the wave, added with one atomic.
@@ -139,6 +159,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CGS_TRACE_W
                                                      cgs_hit* __restrict__ hits, int32_t* __restrict__ objects,
                                                      unsigned long long* __restrict__ overflowCount)
 {
+    __shared__ float4 topLds[4 * CGS_TOP_LDS_KEYS];
+    const NodeCache nc = stageTopLevel(sc, topLds);
     const uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x;
     const ClipStack stk = clipStackFor(ps.stack, gl, ps.lanes);
     unsigned long long nOverflow = 0;
@@ -148,7 +170,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CGS_TRACE_W
         if (r >= n) continue;
         bool overflow = false;
         const cgs_ray ray = rays[r];
-        ClosestHit h = sceneClosest(sc, {ray.ox, ray.oy, ray.oz}, {ray.dx, ray.dy, ray.dz}, ray.tmax, stk, overflow);
+        ClosestHit h = sceneClosest(sc, nc, {ray.ox, ray.oy, ray.oz}, {ray.dx, ray.dy, ray.dz}, ray.tmax, stk, overflow);
         cgs_hit out;
         out.t = h.t;
         out.u = h.object < 0 ? 0.0f : h.u;
@@ -165,6 +187,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CGS_TRACE_W
                                                  uint32_t* __restrict__ occluded,
                                                  unsigned long long* __restrict__ overflowCount)
 {
+    __shared__ float4 topLds[4 * CGS_TOP_LDS_KEYS];
+    const NodeCache nc = stageTopLevel(sc, topLds);
     const uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x;
     const ClipStack stk = clipStackFor(ps.stack, gl, ps.lanes);
     unsigned long long nOverflow = 0;
@@ -174,7 +198,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CGS_TRACE_W
         if (r >= n) continue;
         bool overflow = false;
         const cgs_ray ray = rays[r];
-        bool occ = sceneAny(sc, {ray.ox, ray.oy, ray.oz}, {ray.dx, ray.dy, ray.dz}, ray.tmax, stk, overflow);
+        bool occ = sceneAny(sc, nc, {ray.ox, ray.oy, ray.oz}, {ray.dx, ray.dy, ray.dz}, ray.tmax, stk, overflow);
         occluded[r] = overflow ? 2u : (occ ? 1u : 0u);
         nOverflow += overflow;
     }
@@ -472,6 +496,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CGS_RENDER_
                                                uint32_t* __restrict__ frame, float* __restrict__ color,
                                                unsigned long long* __restrict__ counts)
 {
+    __shared__ float4 topLds[4 * CGS_TOP_LDS_KEYS];
+    const NodeCache nc = stageTopLevel(sc, topLds);
     const uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x;
     const ClipStack stk = clipStackFor(ps.stack, gl, ps.lanes);
     const uint32_t lane = threadIdx.x & 63;
@@ -502,7 +528,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CGS_RENDER_
         {
             if (!first) nSecondary++;
             first = false;
-            ClosestHit h = sceneClosest(sc, ro, rd, rmax, stk, overflow);
+            ClosestHit h = sceneClosest(sc, nc, ro, rd, rmax, stk, overflow);
             if (h.object < 0)
             {
                 acc = add3(acc, mulv3({sc.background[0], sc.background[1], sc.background[2]}, att));
@@ -537,7 +563,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CGS_RENDER_
                 const float dotNL = dot3(N, L);
                 if (dotNL < 1e-14f) continue;
                 nShadow++;
-                if (sceneAny(sc, add3(Pt, mul3(op.eps, L)), L, d, stk, overflow)) continue;
+                if (sceneAny(sc, nc, add3(Pt, mul3(op.eps, L)), L, d, stk, overflow)) continue;
                 V3 H = versor3(add3(L, V));
                 float dotHN = dot3(H, N);
                 float dotHL = dot3(H, L);
