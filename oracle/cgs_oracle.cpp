@@ -531,6 +531,31 @@ int orc_scene_trace(const orc_scene* h, const float* rays, uint64_t n, int any, 
     return 0;
 }
 
+// Closest-hit traces of n world rays, recording every patch test in call
+// order (single thread): up to cap tests into rays_out (8 floats each: local
+// o, initial best t, local d, 0) and patches_out (64 floats each).  Returns
+// the number of tests.
+uint64_t orc_scene_record_tests(const orc_scene* h, const float* rays, uint64_t n, float* rays_out,
+                                float* patches_out, uint64_t cap)
+{
+    PatchTestLog log;
+    g_patchLog = &log;
+    for (uint64_t r = 0; r < n; r++)
+    {
+        const float* ry = rays + 8 * r;
+        Ray ray {{ry[0], ry[1], ry[2]}, {ry[4], ry[5], ry[6]}, ry[3]};
+        Hit hit;
+        hit.t = ray.max;
+        uint32_t pid;
+        sceneIntersect(h->s, hit, ray, &pid, nullptr);
+    }
+    g_patchLog = nullptr;
+    const uint64_t k = log.rays.size() / 8, m = std::min(k, cap);
+    std::memcpy(rays_out, log.rays.data(), m * 32);
+    std::memcpy(patches_out, log.patches.data(), m * 256);
+    return k;
+}
+
 struct orc_camera
 {
     float toWorld[16];

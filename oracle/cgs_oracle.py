@@ -69,6 +69,8 @@ def lib():
         L.orc_fmaxf.restype = _F32
         L.orc_newton.argtypes = [_P, _P, _P, _P, C.c_int]
         L.orc_clip_batch64.argtypes = [_P, _P, _U64, _F64, _P]
+        L.orc_scene_record_tests.argtypes = [_P, _P, _U64, _P, _P, _U64]
+        L.orc_scene_record_tests.restype = _U64
         L.orc_newton_batch.argtypes = [_P, _P, _U64, _P, _P, _P, C.c_int]
         L.orc_bvh_build_patches.argtypes = [_P, _P, _U32]
         L.orc_bvh_build_patches.restype = _P
@@ -309,6 +311,16 @@ class OracleScene:
         if any_hit:
             return obj.astype(bool), w.asdict()
         return hits, patch, obj, w.asdict()
+
+    def record_patch_tests(self, rays, cap=1 << 22):
+        """Every patch test of the closest-hit traces of `rays`, in call order:
+        (clipper rays (k,8): local o, initial best t, local d; patches (k,16,4))."""
+        rays = np.ascontiguousarray(rays, np.float32)
+        ro = np.zeros((cap, 8), np.float32)
+        po = np.zeros((cap, 16, 4), np.float32)
+        k = lib().orc_scene_record_tests(self.h, _ptr(rays), rays.shape[0], _ptr(ro), _ptr(po), cap)
+        k = min(k, cap)
+        return ro[:k], po[:k]
 
     @staticmethod
     def _cam(cam):

@@ -123,6 +123,16 @@ struct TraceHit
     int object = -1;
 };
 
+// Optional log of every closest-hit patch test (test instrumentation: the
+// clipper's inputs — local origin, max, local direction — and the control
+// points), set per thread by orc_scene_record_tests.
+struct PatchTestLog
+{
+    std::vector<float> rays;     // 8 per test: o xyz, initial best t, d xyz, 0
+    std::vector<float> patches;  // 64 per test
+};
+inline thread_local PatchTestLog* g_patchLog = nullptr;
+
 // rt/Primitive.cu:355-381 — closest hit against one surface (local ray).
 inline bool surfaceIntersect(const Surface& s, Hit& hit0, const Ray& ray0, Work* w)
 {
@@ -134,6 +144,14 @@ inline bool surfaceIntersect(const Surface& s, Hit& hit0, const Ray& ray0, Work*
         PatchHit<float> ph {hit.t, {hit.coordinates.x, hit.coordinates.y}};
         f4 S[16];
         s.patch(index, S);
+        if (g_patchLog)
+        {
+            const float r[8] {ray0.origin.x, ray0.origin.y, ray0.origin.z, hit.t,
+                              ray0.direction.x, ray0.direction.y, ray0.direction.z, 0.0f};
+            g_patchLog->rays.insert(g_patchLog->rays.end(), r, r + 8);
+            for (int i = 0; i < 16; i++)
+                g_patchLog->patches.insert(g_patchLog->patches.end(), {S[i].x, S[i].y, S[i].z, S[i].w});
+        }
         if (doBezierClipping<float>(ph, ray0.origin, dir, S, 0x1p-12f, w))
         {
             hit.index = index;
