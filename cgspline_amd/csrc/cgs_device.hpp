@@ -194,9 +194,10 @@ CGS_DEV void slice1(float& c0, float& c1, float& c2, float& c3, float u, float i
     c3 = ivv * c2 + vv * p8;
 }
 
-// subpatchU (rows, Spline.h:366-373) / subpatchV (columns, :375-382) of a 2D
-// patch held in registers.  The two sides are separate functions so every
-// point index is a compile-time constant and the patch never leaves VGPRs.
+// subpatchU (rows, Spline.h:366-373) of a 2D patch held in registers; every
+// point index is a compile-time constant so the patch never leaves VGPRs.
+// subpatchV (columns, :375-382) is the same slice applied to the columns: the
+// clipper gets it by keeping the patch transposed while cutting in v.
 CGS_DEV void subpatchU2(V2 (&p)[16], float a, float b)
 {
     float iu = 1.0f - a;
@@ -209,19 +210,6 @@ CGS_DEV void subpatchU2(V2 (&p)[16], float a, float b)
         slice1(p[4 * k].y, p[4 * k + 1].y, p[4 * k + 2].y, p[4 * k + 3].y, a, iu, vv, ivv);
     }
 }
-CGS_DEV void subpatchV2(V2 (&p)[16], float a, float b)
-{
-    float iu = 1.0f - a;
-    float vv = (b - a) / iu;
-    float ivv = 1.0f - vv;
-#pragma unroll
-    for (int k = 0; k < 4; k++)
-    {
-        slice1(p[k].x, p[k + 4].x, p[k + 8].x, p[k + 12].x, a, iu, vv, ivv);
-        slice1(p[k].y, p[k + 4].y, p[k + 8].y, p[k + 12].y, a, iu, vv, ivv);
-    }
-}
-
 // ---------------------------------------------------------------------------
 // Hull helpers (Spline_BezierClipping.cu:127-165, Spline.h:709-737)
 // ---------------------------------------------------------------------------
@@ -307,8 +295,8 @@ CGS_DEV ClipStack clipStackFor(float2* buf, uint32_t lane, uint32_t lanes) { ret
 // stops at the first accepted candidate (the caller only needs "found").
 // Returns found; sets overflow when the region stack is exhausted.
 template<bool ANY>
-CGS_DEV bool clipPatch(const float4* __restrict__ S, const ClipRay& R, float tol,
-                       float& best, float& bu, float& bv, const ClipStack& stk, bool& overflow)
+CGS_DEV bool clipPatch(const float4* __restrict__ S, const ClipRay& R, float tol, float& best, float& bu, float& bv,
+                       const ClipStack& stk, bool& overflow)
 {
     V2 p[16];
 #pragma unroll
@@ -338,6 +326,15 @@ CGS_DEV bool clipPatch(const float4* __restrict__ S, const ClipRay& R, float tol
         return false;
     };
 
+    // The working patch is kept in the orientation of the current cut side:
+    // natural (p[4j+i] = point(i,j)) while clipping in u, transposed while
+    // clipping in v, so the curves being cut are always the rows p[4r..4r+3]
+    // and one code path serves both sides.  Transposing moves values only;
+    // every formula below gives the natural-orientation result bit for bit:
+    // the tolerance diagonals p3-p12 and p15-p0 change at most by an exact
+    // negation, L = (p15-p0) + (p12-p3) in transposed storage equals
+    // (p15-p0) - (p12-p3) in natural storage (x + (-y) == x - y), and the
+    // envelope visits the same points in the same order.
     while (true)
     {
         bool pop = false;
@@ -347,11 +344,12 @@ CGS_DEV bool clipPatch(const float4* __restrict__ S, const ClipRay& R, float tol
         float d1 = __builtin_sqrtf(t1.x * t1.x + t1.y * t1.y);
         if (d0 < tol && d1 < tol)
         {
-            // approximateIntersection (Spline.h:739-765)
+            // approximateIntersection (Spline.h:739-765) on the natural corners
+            const V2 c30 = vside ? p[12] : p[3], c03 = vside ? p[3] : p[12];
             V2 c;
-            if (triangleOrigin(c, p[0], p[3], p[12]))
+            if (triangleOrigin(c, p[0], c30, c03))
                 found |= onHit(mn.x + c.x * sz.x, mn.y + c.y * sz.y);
-            if (triangleOrigin(c, p[15], p[12], p[3]))
+            if (triangleOrigin(c, p[15], c03, c30))
                 found |= onHit(mx.x - c.x * sz.x, mx.y - c.y * sz.y);
             if (ANY && found) break;
             pop = true;
@@ -359,9 +357,8 @@ CGS_DEV bool clipPatch(const float4* __restrict__ S, const ClipRay& R, float tol
         else
         {
             V2 L {p[15].x - p[0].x, p[15].y - p[0].y};
-            V2 Dv {p[12].x - p[3].x, p[12].y - p[3].y};
-            if (!vside) L = {L.x + Dv.x, L.y + Dv.y};
-            else L = {L.x - Dv.x, L.y - Dv.y};
+            const V2 Dv {p[12].x - p[3].x, p[12].y - p[3].y};
+            L = {L.x + Dv.x, L.y + Dv.y};
             L = normalize2(L);
             const V2 N {-L.y, L.x};
             float d[16];
@@ -371,15 +368,13 @@ CGS_DEV bool clipPatch(const float4* __restrict__ S, const ClipRay& R, float tol
 #pragma unroll
             for (int i = 0; i < 4; i++)
             {
-                float y0 = vside ? d[4 * i] : d[i];
-                ybot[i] = y0;
-                ytop[i] = y0;
+                ybot[i] = d[i];
+                ytop[i] = d[i];
 #pragma unroll
                 for (int k = 1; k < 4; k++)
                 {
-                    float y = vside ? d[4 * i + k] : d[4 * k + i];
-                    ytop[i] = fmax_(ytop[i], y);
-                    ybot[i] = fmin_(ybot[i], y);
+                    ytop[i] = fmax_(ytop[i], d[4 * k + i]);
+                    ybot[i] = fmin_(ybot[i], d[4 * k + i]);
                 }
             }
             float lower = 1.0f, upper = 0.0f;
@@ -414,83 +409,79 @@ CGS_DEV bool clipPatch(const float4* __restrict__ S, const ClipRay& R, float tol
             }
             lower = lower * 0x0FFp-8f;
             upper = upper + (1.0f - upper) * 0x001p-8f;
-            float delta = upper - lower;
+            const float delta = upper - lower;
             if (delta < 0.0f)
             {
                 pop = true;
             }
-            else if (delta > 0.8f)
+            else
             {
-                // split in half; keep the upper half, push the lower half
-                if (sp >= CGS_CLIP_STACK)
+                // delta > 0.8: split in half, keep the upper half and push the
+                // lower half (Spline_BezierClipping.cu:373-409); otherwise clip
+                // to [lower, upper] (:410-432).  Both keep a slice of the rows.
+                const bool split = delta > 0.8f;
+                if (split && sp >= CGS_CLIP_STACK)
                 {
                     overflow = true;
                     break;
                 }
-                V2 q[16];
-#pragma unroll
-                for (int i = 0; i < 16; i++) q[i] = p[i];
-                if (!vside)
+                // region along the cut side (u: x, v: y)
+                const float smn = vside ? mn.y : mn.x, smx = vside ? mx.y : mx.x, ssz = vside ? sz.y : sz.x;
+                float nmn, nmx, nsz;
+                if (split)
                 {
+                    // the lower half, pushed for later: rows sliced to [0, 1/2],
+                    // stored transposed (the popped entry cuts the other side)
+                    V2 q[16];
+#pragma unroll
+                    for (int i = 0; i < 16; i++) q[i] = p[i];
                     subpatchU2(q, 0.0f, 0.5f);
-                    subpatchU2(p, 0.5f, 1.0f);
-                }
-                else
-                {
-                    subpatchV2(q, 0.0f, 0.5f);
-                    subpatchV2(p, 0.5f, 1.0f);
-                }
-                // Spline_BezierClipping.cu:373-409: the lower half keeps the
-                // old minimum and ends at the midpoint; the upper half (kept
-                // in registers, processed first) starts at the midpoint.
-                const V2 omn = mn;
-                V2 emx;
-                if (!vside)
-                {
-                    float hs = 0.5f * sz.x;
-                    float hx = mn.x + hs;
-                    emx = {hx, mx.y};
-                    mn = {hx, mn.y};
-                    sz = {hs, sz.y};
-                }
-                else
-                {
-                    float hs = 0.5f * sz.y;
-                    float hy = mn.y + hs;
-                    emx = {mx.x, hy};
-                    mn = {mn.x, hy};
-                    sz = {sz.x, hs};
-                }
-                vside = !vside;
+                    const float hs = 0.5f * ssz;
+                    const float hm = smn + hs;
+                    const V2 emx = vside ? V2 {mx.x, hm} : V2 {hm, mx.y};
+                    const V2 psz = vside ? V2 {sz.x, hs} : V2 {hs, sz.y};
 #pragma unroll
-                for (int i = 0; i < 16; i++) stk.put(sp, i, make_float2(q[i].x, q[i].y));
-                stk.put(sp, 16, make_float2(omn.x, omn.y));
-                stk.put(sp, 17, make_float2(emx.x, emx.y));
-                stk.put(sp, 18, make_float2(sz.x, sz.y));
-                stk.put(sp, 19, make_float2(vside ? 1.0f : 0.0f, 0.0f));
-                sp++;
-                continue;
-            }
-            else
-            {
-                if (!vside)
-                {
-                    float u0 = mn.x + sz.x * lower;
-                    float u1 = mx.x - sz.x * (1.0f - upper);
-                    subpatchU2(p, lower, upper);
-                    mn.x = u0;
-                    mx.x = u1;
-                    sz.x = u1 - u0;
+                    for (int r = 0; r < 4; r++)
+#pragma unroll
+                        for (int c = 0; c < 4; c++) stk.put(sp, 4 * c + r, make_float2(q[4 * r + c].x, q[4 * r + c].y));
+                    stk.put(sp, 16, make_float2(mn.x, mn.y));
+                    stk.put(sp, 17, make_float2(emx.x, emx.y));
+                    stk.put(sp, 18, make_float2(psz.x, psz.y));
+                    stk.put(sp, 19, make_float2(vside ? 0.0f : 1.0f, 0.0f));
+                    sp++;
+                    nmn = hm;
+                    nmx = smx;
+                    nsz = hs;
                 }
                 else
                 {
-                    float v0 = mn.y + sz.y * lower;
-                    float v1 = mx.y - sz.y * (1.0f - upper);
-                    subpatchV2(p, lower, upper);
-                    mn.y = v0;
-                    mx.y = v1;
-                    sz.y = v1 - v0;
+                    nmn = smn + ssz * lower;
+                    nmx = smx - ssz * (1.0f - upper);
+                    nsz = nmx - nmn;
                 }
+                subpatchU2(p, split ? 0.5f : lower, split ? 1.0f : upper);
+                if (vside)
+                {
+                    mn.y = nmn;
+                    mx.y = nmx;
+                    sz.y = nsz;
+                }
+                else
+                {
+                    mn.x = nmn;
+                    mx.x = nmx;
+                    sz.x = nsz;
+                }
+                // the other side is cut next: transpose
+#pragma unroll
+                for (int r = 0; r < 4; r++)
+#pragma unroll
+                    for (int c = r + 1; c < 4; c++)
+                    {
+                        const V2 t = p[4 * r + c];
+                        p[4 * r + c] = p[4 * c + r];
+                        p[4 * c + r] = t;
+                    }
                 vside = !vside;
                 continue;
             }
@@ -513,6 +504,14 @@ CGS_DEV bool clipPatch(const float4* __restrict__ S, const ClipRay& R, float tol
         }
     }
     return found;
+}
+
+// The same with the mode chosen per lane at run time.
+CGS_DEV bool clipPatchRT(const bool any, const float4* __restrict__ S, const ClipRay& R, float tol, float& best,
+                         float& bu, float& bv, const ClipStack& stk, bool& overflow)
+{
+    return any ? clipPatch<true>(S, R, tol, best, bu, bv, stk, overflow)
+               : clipPatch<false>(S, R, tol, best, bu, bv, stk, overflow);
 }
 
 // ---------------------------------------------------------------------------

@@ -130,8 +130,11 @@ CGS_DEV uint32_t stepToElement(const BVHRef& b, Level& L, V3 o, V3 Dinv, float l
 // INLINE = true : run the patch tests in place and finish the ray (used for
 //                 the last few rays, whose long serial traversal would
 //                 otherwise cost one staged round per patch test).
+#ifndef CGS_FINISH_WAVES
+#define CGS_FINISH_WAVES 2
+#endif
 template<bool INLINE>
-__global__ __launch_bounds__(256) void kTraverse(SceneDev sc, RayBatch rb, const uint32_t* __restrict__ active,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(INLINE ? CGS_FINISH_WAVES : 1))) void kTraverse(SceneDev sc, RayBatch rb, const uint32_t* __restrict__ active,
                                                  const unsigned int* __restrict__ nActive, ClipQueue cq,
                                                  unsigned int* __restrict__ work, float2* __restrict__ stack,
                                                  uint32_t lanes)
