@@ -361,20 +361,20 @@ CGS_DEV bool clipPatch(const float4* __restrict__ S, const ClipRay& R, float tol
             L = {L.x + Dv.x, L.y + Dv.y};
             L = normalize2(L);
             const V2 N {-L.y, L.x};
-            float d[16];
-#pragma unroll
-            for (int i = 0; i < 16; i++) d[i] = N.x * p[i].x + N.y * p[i].y;
+            // signed distances N·p folded into the envelope as they are computed
             float ytop[4], ybot[4];
 #pragma unroll
             for (int i = 0; i < 4; i++)
             {
-                ybot[i] = d[i];
-                ytop[i] = d[i];
+                const float d0 = N.x * p[i].x + N.y * p[i].y;
+                ybot[i] = d0;
+                ytop[i] = d0;
 #pragma unroll
                 for (int k = 1; k < 4; k++)
                 {
-                    ytop[i] = fmax_(ytop[i], d[4 * k + i]);
-                    ybot[i] = fmin_(ybot[i], d[4 * k + i]);
+                    const float dk = N.x * p[4 * k + i].x + N.y * p[4 * k + i].y;
+                    ytop[i] = fmax_(ytop[i], dk);
+                    ybot[i] = fmin_(ybot[i], dk);
                 }
             }
             float lower = 1.0f, upper = 0.0f;
@@ -431,19 +431,28 @@ CGS_DEV bool clipPatch(const float4* __restrict__ S, const ClipRay& R, float tol
                 if (split)
                 {
                     // the lower half, pushed for later: rows sliced to [0, 1/2],
-                    // stored transposed (the popped entry cuts the other side)
-                    V2 q[16];
+                    // stored transposed (the popped entry cuts the other side:
+                    // stored word 4c + r holds row r, point c)
+                    {
+                        // subpatchU2(copy, 0, 1/2) one row at a time
+                        const float a = 0.0f, b = 0.5f;
+                        const float iu = 1.0f - a, vv = (b - a) / iu, ivv = 1.0f - vv;
 #pragma unroll
-                    for (int i = 0; i < 16; i++) q[i] = p[i];
-                    subpatchU2(q, 0.0f, 0.5f);
+                        for (int r = 0; r < 4; r++)
+                        {
+                            V2 c0 = p[4 * r], c1 = p[4 * r + 1], c2 = p[4 * r + 2], c3 = p[4 * r + 3];
+                            slice1(c0.x, c1.x, c2.x, c3.x, a, iu, vv, ivv);
+                            slice1(c0.y, c1.y, c2.y, c3.y, a, iu, vv, ivv);
+                            stk.put(sp, r, make_float2(c0.x, c0.y));
+                            stk.put(sp, 4 + r, make_float2(c1.x, c1.y));
+                            stk.put(sp, 8 + r, make_float2(c2.x, c2.y));
+                            stk.put(sp, 12 + r, make_float2(c3.x, c3.y));
+                        }
+                    }
                     const float hs = 0.5f * ssz;
                     const float hm = smn + hs;
                     const V2 emx = vside ? V2 {mx.x, hm} : V2 {hm, mx.y};
                     const V2 psz = vside ? V2 {sz.x, hs} : V2 {hs, sz.y};
-#pragma unroll
-                    for (int r = 0; r < 4; r++)
-#pragma unroll
-                        for (int c = 0; c < 4; c++) stk.put(sp, 4 * c + r, make_float2(q[4 * r + c].x, q[4 * r + c].y));
                     stk.put(sp, 16, make_float2(mn.x, mn.y));
                     stk.put(sp, 17, make_float2(emx.x, emx.y));
                     stk.put(sp, 18, make_float2(psz.x, psz.y));

@@ -51,11 +51,8 @@ enum class PersistKernel
     TraceAny,
     ClipBatch,
     Render,
-    RenderLanes,
-    TraceClosestRefill,
-    TraceAnyRefill,
 };
-constexpr int kPersistKernels = 7;
+constexpr int kPersistKernels = 4;
 
 struct PersistArgs
 {
@@ -71,14 +68,12 @@ hipError_t launchTraceClosest(const SceneDev& sc, const PersistArgs& pa, const c
                               int32_t* objects, unsigned long long* overflow, hipStream_t s);
 hipError_t launchTraceAny(const SceneDev& sc, const PersistArgs& pa, const cgs_ray* rays, uint32_t n, uint32_t* occluded,
                           unsigned long long* overflow, hipStream_t s);
-hipError_t launchTraceRefill(const SceneDev& sc, const PersistArgs& pa, const cgs_ray* rays, uint32_t n, cgs_hit* hits,
-                             int32_t* objects, uint32_t* occluded, unsigned long long* overflow, hipStream_t s);
 hipError_t launchClipBatch(const PersistArgs& pa, const cgs_ray* rays, const float* patches, uint32_t n, float tol,
                            cgs_hit* hits, hipStream_t s);
 hipError_t launchSubdivBatch(const cgs_ray* rays, const float* patches, uint32_t n, float threshold, cgs_hit* hits,
                              hipStream_t s);
 hipError_t launchRender(const SceneDev& sc, const PersistArgs& pa, const float* toWorld16, const float* pos3, float tlx,
                         float tly, float hdx, float hdy, float eps, int depth, uint32_t w, uint32_t h, uint32_t* frame,
-                        float* color, unsigned long long* counts, bool lanes, hipStream_t s);
+                        float* color, unsigned long long* counts, hipStream_t s);
 
 } // namespace cgs

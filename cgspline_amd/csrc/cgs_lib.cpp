@@ -97,11 +97,6 @@ struct cgs_ctx
     // wavefront path (traverse / compacted clip queue / finish kernels) when
     // CGS_STAGED=1.  Both give the same results bit for bit.
     bool staged = false;
-    // Persistent render scheduling: per lane (kRenderLanes, CGS_RENDER_LANES=1)
-    // or per 8x8 tile in wave lock-step (kRender).
-    bool renderLanes = false;
-    // Ray queries with per-lane refill (kTraceRefill, CGS_TRACE_REFILL=1).
-    bool traceRefill = false;
     unsigned long long* dOverflow = nullptr; // device counter of overflowed clip stacks
     unsigned int* dCounters = nullptr;       // one work counter per persistent kernel
     float2* dStack = nullptr;                // clipping region stacks of the resident lanes
@@ -658,8 +653,6 @@ cgs_status cgs_open(int device, cgs_ctx** out)
     if (const char* v = std::getenv("CGS_STAGE_ROUNDS")) c->stage.maxRounds = std::atoi(v);
     if (const char* v = std::getenv("CGS_STAGE_TAIL")) c->stage.tailLanes = (float)std::atof(v);
     if (const char* v = std::getenv("CGS_STAGED")) c->staged = std::atoi(v) != 0;
-    if (const char* v = std::getenv("CGS_RENDER_LANES")) c->renderLanes = std::atoi(v) != 0;
-    if (const char* v = std::getenv("CGS_TRACE_REFILL")) c->traceRefill = std::atoi(v) != 0;
     if (e == hipSuccess) e = hipMalloc(&c->dCounters, 256);
     if (e == hipSuccess) e = hipMemset(c->dCounters, 0, 256);
     if (e != hipSuccess)
@@ -1020,13 +1013,9 @@ cgs_status cgs_trace_closest(cgs_ctx* ctx, cgs_scene* sc, const cgs_ray* rays, u
     if (ctx->staged) return stageQuery(ctx, sc, rays, n, 0, hits, objects, nullptr, (hipStream_t)stream);
     if (n == 0) return CGS_OK;
     PersistArgs pa;
-    cgs_status st = persistArgs(ctx, ctx->traceRefill ? PersistKernel::TraceClosestRefill : PersistKernel::TraceClosest,
-                                (hipStream_t)stream, pa);
+    cgs_status st = persistArgs(ctx, PersistKernel::TraceClosest, (hipStream_t)stream, pa);
     if (st != CGS_OK) return st;
-    if (ctx->traceRefill)
-        HIP_TRY(launchTraceRefill(sc->view(), pa, rays, n, hits, objects, nullptr, ctx->dOverflow, (hipStream_t)stream));
-    else
-        HIP_TRY(launchTraceClosest(sc->view(), pa, rays, n, hits, objects, ctx->dOverflow, (hipStream_t)stream));
+    HIP_TRY(launchTraceClosest(sc->view(), pa, rays, n, hits, objects, ctx->dOverflow, (hipStream_t)stream));
     return CGS_OK;
 }
 
@@ -1038,13 +1027,9 @@ cgs_status cgs_trace_any(cgs_ctx* ctx, cgs_scene* sc, const cgs_ray* rays, uint3
     if (ctx->staged) return stageQuery(ctx, sc, rays, n, 1, nullptr, nullptr, occluded, (hipStream_t)stream);
     if (n == 0) return CGS_OK;
     PersistArgs pa;
-    cgs_status st = persistArgs(ctx, ctx->traceRefill ? PersistKernel::TraceAnyRefill : PersistKernel::TraceAny,
-                                (hipStream_t)stream, pa);
+    cgs_status st = persistArgs(ctx, PersistKernel::TraceAny, (hipStream_t)stream, pa);
     if (st != CGS_OK) return st;
-    if (ctx->traceRefill)
-        HIP_TRY(launchTraceRefill(sc->view(), pa, rays, n, nullptr, nullptr, occluded, ctx->dOverflow, (hipStream_t)stream));
-    else
-        HIP_TRY(launchTraceAny(sc->view(), pa, rays, n, occluded, ctx->dOverflow, (hipStream_t)stream));
+    HIP_TRY(launchTraceAny(sc->view(), pa, rays, n, occluded, ctx->dOverflow, (hipStream_t)stream));
     return CGS_OK;
 }
 
@@ -1078,10 +1063,10 @@ cgs_status cgs_render(cgs_ctx* ctx, cgs_scene* sc, const cgs_camera* cam, const 
     if (!ctx->staged)
     {
         PersistArgs pa;
-        cgs_status pst = persistArgs(ctx, ctx->renderLanes ? PersistKernel::RenderLanes : PersistKernel::Render, s, pa);
+        cgs_status pst = persistArgs(ctx, PersistKernel::Render, s, pa);
         if (pst != CGS_OK) return pst;
         HIP_TRY(launchRender(sc->view(), pa, cam->to_world, cam->position, tlx, tly, hdx, hdy, op->eps,
-                             op->recursion_depth, w, h, frame, color, (unsigned long long*)counts, ctx->renderLanes, s));
+                             op->recursion_depth, w, h, frame, color, (unsigned long long*)counts, s));
         HIP_TRY(hipEventRecord(ctx->renderDone, s));
         ctx->renderLaunched = true;
         return CGS_OK;

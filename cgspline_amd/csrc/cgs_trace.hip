@@ -106,12 +106,13 @@ CGS_DEV bool sceneAny(const SceneDev& sc, const NodeCache& nc, V3 o, V3 d, float
 }
 
 #ifndef CGS_TOP_LDS_KEYS
-#define CGS_TOP_LDS_KEYS 1024
+#define CGS_TOP_LDS_KEYS 512
 #endif
 static_assert(CGS_TOP_LDS_KEYS <= kTopKeyCap, "top-level LDS staging beyond the key map");
 
 // Stages the top-level BVH's nodes with heap key < CGS_TOP_LDS_KEYS (its upper
-// levels, visited by nearly every ray) in LDS: 64 B per key.
+// levels, visited by nearly every ray) in LDS: 64 B per key.  512 keys (32 KiB
+// per workgroup) leave room for three resident workgroups per CU.
 CGS_DEV NodeCache stageTopLevel(const SceneDev& sc, float4* lds)
 {
     for (uint32_t k = threadIdx.x; k < 4 * CGS_TOP_LDS_KEYS; k += blockDim.x)
@@ -149,10 +150,10 @@ CGS_DEV uint32_t nextChunk(unsigned int* counter)
 }
 
 #ifndef CGS_RENDER_WAVES
-#define CGS_RENDER_WAVES 2
+#define CGS_RENDER_WAVES 3
 #endif
 #ifndef CGS_TRACE_WAVES
-#define CGS_TRACE_WAVES 2
+#define CGS_TRACE_WAVES 3
 #endif
 
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CGS_TRACE_WAVES))) void kTraceClosest(SceneDev sc, Persist ps, const cgs_ray* __restrict__ rays, uint32_t n,
@@ -489,9 +490,9 @@ CGS_DEV uint32_t packSRGB(float r, float g, float b)
 CGS_DEV float saturate(float x) { return x > 0.0f ? (x < 1.0f ? x : 1.0f) : 0.0f; }
 
 // Occupancy target of the render megakernel (waves per SIMD): the compiler
-// keeps VGPR + AGPR use within 512 / CGS_RENDER_WAVES.  Two waves per SIMD
-// (256 registers, a few spilled) measured fastest on the 1M-patch frame:
-// 55 ms against 83 ms at one wave (no spills) and 69-72 ms at three or four.
+// keeps VGPR + AGPR use within 512 / CGS_RENDER_WAVES.  On the 1M-patch frame
+// three waves per SIMD (168 registers) measured 39.5 ms, two waves 44.5 ms
+// and four waves 46 ms.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CGS_RENDER_WAVES))) void kRender(SceneDev sc, Persist ps, CamDev cam, RenderDev op,
                                                uint32_t* __restrict__ frame, float* __restrict__ color,
                                                unsigned long long* __restrict__ counts)
@@ -503,7 +504,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CGS_RENDER_
     const uint32_t lane = threadIdx.x & 63;
     const uint32_t tilesX = (op.width + 7) / 8, tilesY = (op.height + 7) / 8;
     const uint32_t nItems = tilesX * tilesY * 64;
-    unsigned long long nPrimary = 0, nShadow = 0, nSecondary = 0, nOverflow = 0;
+    uint32_t nPrimary = 0, nShadow = 0, nSecondary = 0, nOverflow = 0;
     M4 C;
     C.c[0] = cam.c[0]; C.c[1] = cam.c[1]; C.c[2] = cam.c[2]; C.c[3] = cam.c[3];
     const V3 camPos {cam.px, cam.py, cam.pz};
@@ -605,454 +606,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CGS_RENDER_
     }
 }
 
-// ---------------------------------------------------------------------------
-// Lane-scheduled renderer
-// ---------------------------------------------------------------------------
-// kRender runs a pixel's ray tree in wave lock-step: primary trace, shading,
-// shadow trace, reflection trace...  Most primary rays miss, so the later
-// traces of a wave run with a few lanes while costing a whole traversal each.
-// kRenderLanes gives every lane its own schedule instead: a lane takes the
-// next pixel as soon as its pixel is finished, and every loop iteration
-// advances each lane's current ray by one traversal step, whatever kind of
-// ray it is.  Per pixel the rays, their order and every arithmetic operation
-// are those of kRender, so the colours are identical.
-
-// Two-level traversal of one ray, advanced one step per call ("if-if" form):
-// a lane at the top level and a lane inside an object execute the same step.
-// A step is one node visit, one leaf element (an object entry or a patch
-// test) or one ascent.  Per ray, the sequence of node visits, patch tests and
-// hit updates is that of Context::intersect (RayTracer.cu:369-457) with
-// BVH::hashIntersect / hashIntersectAny (rt/BVH.h:178-350) at both levels and
-// rt::intersect(BezierSurface) (Primitive.cu:355-401) at the leaves.
-struct Walker
-{
-    V3 wo, wd, wDinv;                      // world ray
-    float wmax;
-    V3 lo, ld, lDinv;                      // object-local ray (ld normalized)
-    uint32_t node, key, post, leafPos;     // current level (leafPos: next element slot, EMPTY = not in a leaf)
-    uint32_t tNode, tKey, tPost, tLeafPos; // top level, saved while inside an object
-    uint32_t obj;                          // object being traversed, EMPTY at the top level
-    uint32_t bDispl, bTable, bA, bB, bMask, bT, bPatch;
-    uint64_t bTm;                          // perfect hash of the object's surface BVH
-    float ldinv, lmax;                     // 1/|M d|, local ray max
-    float lt, lu, lv;                      // best hit inside the object (closest)
-    uint32_t lp;
-    ClosestHit h;                          // closest hit so far (world t)
-    bool any, done, occluded;
-};
-
-CGS_DEV void walkInit(const SceneDev& sc, Walker& w, V3 o, V3 d, float tmax, bool any)
-{
-    w.wo = o;
-    w.wd = d;
-    w.wDinv = {1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
-    w.wmax = tmax;
-    w.node = sc.top.root;
-    w.key = 1;
-    w.post = 0;
-    w.leafPos = EMPTY;
-    w.obj = EMPTY;
-    w.h = {tmax, 0.0f, 0.0f, EMPTY, -1};
-    w.any = any;
-    w.done = false;
-    w.occluded = false;
-}
-
-// Top-level leaf element: enter object e (RayTracer.cu:379-389 local ray).
-CGS_DEV void walkEnter(const SceneDev& sc, Walker& w, uint32_t e)
-{
-    const DevObject& ob = sc.objects[e];
-    const LocalRay lr = toLocal(ob, w.wo, w.wd, w.wmax);
-    const DevBVH surf = sc.surfaces[ob.surface];
-    w.tNode = w.node;
-    w.tKey = w.key;
-    w.tPost = w.post;
-    w.tLeafPos = w.leafPos;
-    w.obj = e;
-    w.lo = lr.o;
-    w.ld = lr.d;
-    w.lDinv = {1.0f / lr.d.x, 1.0f / lr.d.y, 1.0f / lr.d.z};
-    w.ldinv = lr.dinv;
-    w.lmax = lr.max;
-    w.lt = lr.max;
-    w.lu = 0.0f;
-    w.lv = 0.0f;
-    w.lp = EMPTY;
-    w.node = surf.root;
-    w.key = 1;
-    w.post = 0;
-    w.leafPos = EMPTY;
-    w.bDispl = surf.displBase;
-    w.bTable = surf.tableBase;
-    w.bA = surf.A;
-    w.bB = surf.B;
-    w.bMask = surf.mask;
-    w.bT = surf.T;
-    w.bTm = surf.Tm;
-    w.bPatch = surf.patchBase;
-}
-
-// Object traversal finished: closest-hit update (RayTracer.cu:396-410), back
-// to the top level after the object's leaf element.
-CGS_DEV void walkExit(Walker& w)
-{
-    if (!w.any && w.lp != EMPTY)
-    {
-        const float t = w.lt * w.ldinv;
-        if (t < w.h.t) w.h = {t, w.lu, w.lv, w.lp, (int32_t)w.obj};
-    }
-    w.node = w.tNode;
-    w.key = w.tKey;
-    w.post = w.tPost;
-    w.leafPos = w.tLeafPos;
-    w.obj = EMPTY;
-}
-
-CGS_DEV void walkStep(const SceneDev& sc, const NodeCache& nc, Walker& w, const ClipStack& stk, bool& overflow)
-{
-    const bool top = w.obj == EMPTY;
-    if (w.leafPos != EMPTY)
-    {
-        const uint32_t e = sc.indices[w.leafPos];
-        if (e != EMPTY)
-        {
-            w.leafPos++;
-            if (top)
-            {
-                walkEnter(sc, w, e);
-                return;
-            }
-            // patch test (rt/Primitive.cu:355-401 → Spline_BezierClipping.cu:46-114)
-            const float4* S = sc.patches + 16 * (size_t)(w.bPatch + e);
-            const ClipRay cr = makeClipRay(w.lo, versor3(w.ld));
-            float best = w.any ? w.lmax : w.lt, bu = w.lu, bv = w.lv;
-            if (clipPatchRT(w.any, S, cr, kClipTol, best, bu, bv, stk, overflow))
-            {
-                if (w.any)
-                {
-                    w.occluded = true;
-                    w.done = true;
-                }
-                else
-                {
-                    w.lt = best;
-                    w.lu = bu;
-                    w.lv = bv;
-                    w.lp = e;
-                }
-            }
-            return;
-        }
-        w.leafPos = EMPTY;
-    }
-    else
-    {
-        const NodeBoxes n = (top && w.key < nc.keys) ? loadNodeLds(nc.lds, w.key) : loadNode(sc.nodes, w.node);
-        if (n.left == EMPTY)
-        {
-            w.leafPos = n.right;
-            return;
-        }
-        const V3 o = top ? w.wo : w.lo;
-        const V3 Dinv = top ? w.wDinv : w.lDinv;
-        const float limit = w.any ? (top ? w.wmax : w.lmax) : (top ? w.h.t : w.lt);
-        float L0, R0;
-        const bool iL = nodeIntersect(L0, n.lmin, n.lmax, o, Dinv) && L0 < limit;
-        const bool iR = nodeIntersect(R0, n.rmin, n.rmax, o, Dinv) && R0 < limit;
-        if (iL || iR)
-        {
-            w.post <<= 1;
-            const bool both = iL && iR;
-            const bool goLeft = both ? L0 < R0 : iL;
-            w.post |= both ? 1u : 0u;
-            w.key = goLeft ? w.key << 1 : (w.key << 1) ^ 1u;
-            w.node = goLeft ? n.left : n.right;
-            return;
-        }
-    }
-    // ascend (rt/BVH.h:253-262)
-    if (!w.post)
-    {
-        if (top) w.done = true;
-        else walkExit(w);
-        return;
-    }
-    const int levels = __builtin_ctz(w.post);
-    w.post >>= levels;
-    w.key = (w.key >> levels) ^ 1u;
-    w.post ^= 1;
-    if (top && w.key < nc.keys) return; // staged in LDS: read by key, no lookup
-    const uint32_t A = top ? sc.top.A : w.bA, B = top ? sc.top.B : w.bB;
-    const uint32_t mask = top ? sc.top.mask : w.bMask;
-    const uint32_t dBase = top ? sc.top.displBase : w.bDispl, tBase = top ? sc.top.tableBase : w.bTable;
-    const FastMod T {top ? sc.top.Tm : w.bTm, top ? sc.top.T : w.bT};
-    const uint32_t x = A * w.key + B + sc.displ[dBase + (w.key & mask)];
-    w.node = sc.tables[tBase + fastmod(x, T)];
-    if (w.node >= sc.nodeCount) // a corrupt table ends this level instead of faulting
-    {
-        if (top) w.done = true;
-        else walkExit(w);
-    }
-}
-
-// Ray queries with per-lane refill: a lane takes the next ray as soon as its
-// ray is finished, and each loop iteration advances every lane's ray by one
-// walker step, so a wave never waits for its slowest ray.
-template<bool ANY>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CGS_TRACE_WAVES))) void kTraceRefill(
-    SceneDev sc, Persist ps, const cgs_ray* __restrict__ rays, uint32_t n, cgs_hit* __restrict__ hits,
-    int32_t* __restrict__ objects, uint32_t* __restrict__ occluded, unsigned long long* __restrict__ overflowCount)
-{
-    __shared__ float4 topLds[4 * CGS_TOP_LDS_KEYS];
-    const NodeCache nc = stageTopLevel(sc, topLds);
-    const uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x;
-    const ClipStack stk = clipStackFor(ps.stack, gl, ps.lanes);
-    const uint32_t lane = threadIdx.x & 63;
-    unsigned long long nOverflow = 0;
-    uint32_t r = EMPTY;
-    bool exhausted = false, overflow = false;
-    Walker w;
-    w.done = true;
-    while (true)
-    {
-        if (!exhausted)
-        {
-            const bool idle = r == EMPTY;
-            const unsigned long long m = __ballot(idle);
-            if (m)
-            {
-                const uint32_t cnt = (uint32_t)__popcll(m);
-                const uint32_t leader = (uint32_t)__builtin_ctzll(m);
-                uint32_t base = 0;
-                if (lane == leader) base = atomicAdd(ps.counter, cnt);
-                base = __shfl(base, leader, 64);
-                if (base + cnt >= n) exhausted = true;
-                if (idle)
-                {
-                    const uint32_t k = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-                    if (k < n)
-                    {
-                        r = k;
-                        overflow = false;
-                        const cgs_ray ray = rays[k];
-                        walkInit(sc, w, {ray.ox, ray.oy, ray.oz}, {ray.dx, ray.dy, ray.dz}, ray.tmax, ANY);
-                    }
-                }
-            }
-        }
-        const bool busy = r != EMPTY;
-        if (!__ballot(busy))
-        {
-            if (exhausted) break;
-            continue;
-        }
-        if (!busy) continue;
-        walkStep(sc, nc, w, stk, overflow);
-        if (!w.done) continue;
-        if (ANY)
-            occluded[r] = overflow ? 2u : (w.occluded ? 1u : 0u);
-        else
-        {
-            const ClosestHit h = w.h;
-            cgs_hit out;
-            out.t = h.t;
-            out.u = h.object < 0 ? 0.0f : h.u;
-            out.v = h.object < 0 ? 0.0f : h.v;
-            out.patch = overflow ? CGS_PATCH_OVERFLOW : (h.object < 0 ? EMPTY : h.patch);
-            hits[r] = out;
-            if (objects) objects[r] = h.object;
-        }
-        nOverflow += overflow;
-        r = EMPTY;
-    }
-    addWave(overflowCount, nOverflow);
-}
-
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CGS_RENDER_WAVES))) void kRenderLanes(
-    SceneDev sc, Persist ps, CamDev cam, RenderDev op, uint32_t* __restrict__ frame, float* __restrict__ color,
-    unsigned long long* __restrict__ counts)
-{
-    __shared__ float4 topLds[4 * CGS_TOP_LDS_KEYS];
-    const NodeCache nc = stageTopLevel(sc, topLds);
-    const uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x;
-    const ClipStack stk = clipStackFor(ps.stack, gl, ps.lanes);
-    const uint32_t lane = threadIdx.x & 63;
-    const uint32_t tilesX = (op.width + 7) / 8, tilesY = (op.height + 7) / 8;
-    const uint32_t nItems = tilesX * tilesY * 64;
-    unsigned long long nPrimary = 0, nShadow = 0, nSecondary = 0, nOverflow = 0;
-    M4 C;
-    C.c[0] = cam.c[0]; C.c[1] = cam.c[1]; C.c[2] = cam.c[2]; C.c[3] = cam.c[3];
-    const V3 camPos {cam.px, cam.py, cam.pz};
-    const V3 background {sc.background[0], sc.background[1], sc.background[2]};
-    const V3 ambient {sc.ambient[0], sc.ambient[1], sc.ambient[2]};
-    const float invPi = 0x1.45f306p-2f; // std::numbers::inv_pi_v<float>
-    const float pi = 0x1.921fb6p+1f;
-
-    // per-lane pixel state
-    uint32_t item = EMPTY;          // pixel work item (8x8-tile order), EMPTY = idle
-    bool exhausted = false;         // no work items left (wave-uniform)
-    bool overflow = false;
-    V3 acc {0.0f, 0.0f, 0.0f}, att {1.0f, 1.0f, 1.0f};
-    int depth = 0;
-    // shading state of the last closest hit, kept across its shadow rays
-    V3 Pt {0.0f, 0.0f, 0.0f}, N {0.0f, 0.0f, 0.0f}, V {0.0f, 0.0f, 0.0f};
-    V3 diffuse {0.0f, 0.0f, 0.0f}, spec {0.0f, 0.0f, 0.0f}, col {0.0f, 0.0f, 0.0f}, term {0.0f, 0.0f, 0.0f};
-    float dotNV = 0.0f, metal = 0.0f, rough = 0.0f;
-    uint32_t li = 0;
-    Walker w;
-    w.done = true;
-
-    auto finishPixel = [&]() {
-        const uint32_t tile = item / 64;
-        const uint32_t i = (tile % tilesX) * 8 + (item & 7), j = (tile / tilesX) * 8 + ((item >> 3) & 7);
-        const uint32_t q = j * op.width + i;
-        if (color)
-        {
-            color[3 * (size_t)q] = acc.x;
-            color[3 * (size_t)q + 1] = acc.y;
-            color[3 * (size_t)q + 2] = acc.z;
-        }
-        frame[q] = packSRGB(saturate(acc.x), saturate(acc.y), saturate(acc.z));
-        nOverflow += overflow;
-        item = EMPTY;
-    };
-    // Launches the shadow ray of the next light that needs one, or finishes
-    // the hit: ambient + lights, then the reflection ray (RayTracer.cu:439-543).
-    auto nextLight = [&]() {
-        for (; li < sc.nlight; li++)
-        {
-            const DevLight lt = sc.lights[li];
-            float d;
-            V3 L;
-            if (!lightVector(d, L, Pt, lt)) continue;
-            const float dotNL = dot3(N, L);
-            if (dotNL < 1e-14f) continue;
-            nShadow++;
-            V3 H = versor3(add3(L, V));
-            float dotHN = dot3(H, N);
-            float dotHL = dot3(H, L);
-            V3 I = lightColor(d, lt);
-            V3 bs = mul3(microfacet(dotNV, dotNL, dotHN, rough), schlick(spec, dotHL));
-            term = mulv3(mul3(dotNL, I), add3(diffuse, mul3(metal, bs)));
-            walkInit(sc, w, add3(Pt, mul3(op.eps, L)), L, d, true);
-            return;
-        }
-        acc = add3(acc, mulv3(mul3(pi, col), att));
-        if (depth <= 0)
-        {
-            finishPixel();
-            return;
-        }
-        V3 refl = schlick(spec, dotNV);
-        V3 R = sub3(neg3(V), mul3(2.0f * -dotNV, N));
-        float g = Gsmith(dotNV, dotNV, rough);
-        float rr = 1.0f - rough;
-        V3 brdf = mul3(rr * rr, mul3(dotNV, add3(diffuse, mul3(metal, mul3(g, refl)))));
-        V3 a = mulv3(att, brdf);
-        if (max3(a) <= 0x1p-8f)
-        {
-            finishPixel();
-            return;
-        }
-        nSecondary++;
-        att = a;
-        depth = depth - 1;
-        walkInit(sc, w, add3(Pt, mul3(op.eps, R)), R, __builtin_inff(), false);
-    };
-
-    while (true)
-    {
-        // idle lanes take the next pixels (one atomic per wave)
-        if (!exhausted)
-        {
-            const bool idle = item == EMPTY;
-            const unsigned long long m = __ballot(idle);
-            if (m)
-            {
-                const uint32_t cnt = (uint32_t)__popcll(m);
-                const uint32_t leader = (uint32_t)__builtin_ctzll(m);
-                uint32_t base = 0;
-                if (lane == leader) base = atomicAdd(ps.counter, cnt);
-                base = __shfl(base, leader, 64);
-                if (base + cnt >= nItems) exhausted = true;
-                if (idle)
-                {
-                    const uint32_t k = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-                    const uint32_t tile = k / 64;
-                    const uint32_t i = (tile % tilesX) * 8 + (k & 7), j = (tile / tilesX) * 8 + ((k >> 3) & 7);
-                    if (k < nItems && i < op.width && j < op.height)
-                    {
-                        // pixelRayDirection (RayTracer.cu:133-145)
-                        item = k;
-                        nPrimary++;
-                        overflow = false;
-                        acc = {0.0f, 0.0f, 0.0f};
-                        att = {1.0f, 1.0f, 1.0f};
-                        depth = op.depth;
-                        float4 P = make_float4(cam.tlx + cam.hdx * (2.0f * (float)i), cam.tly - cam.hdy * (2.0f * (float)j),
-                                               -1.0f, 1.0f);
-                        walkInit(sc, w, camPos, versor3(sub3(project4(m4mul(C, P)), camPos)), __builtin_inff(), false);
-                    }
-                }
-            }
-        }
-        const bool busy = item != EMPTY;
-        if (!__ballot(busy))
-        {
-            if (exhausted) break;
-            continue;
-        }
-        if (!busy) continue;
-        walkStep(sc, nc, w, stk, overflow);
-        if (!w.done) continue;
-        if (w.any)
-        {
-            // shadow ray finished (RayTracer.cu:455-466)
-            if (!w.occluded) col = add3(col, term);
-            li++;
-            nextLight();
-            continue;
-        }
-        const ClosestHit h = w.h;
-        if (h.object < 0)
-        {
-            acc = add3(acc, mulv3(background, att));
-            finishPixel();
-            continue;
-        }
-        // shading of the closest hit (RayTracer.cu:425-470)
-        const DevObject& ob = sc.objects[h.object];
-        const float4 m0 = ob.mat0, m1 = ob.mat1;
-        metal = m0.w;
-        rough = m1.w;
-        spec = {m1.x, m1.y, m1.z};
-        diffuse = mul3(1.0f - metal, mul3(invPi, {m0.x, m0.y, m0.z}));
-        col = mulv3(ambient, diffuse);
-        V = neg3(w.wd);
-        const DevBVH surf = sc.surfaces[ob.surface];
-        N = versor3(normalMat(sc.patches + 16 * (size_t)(surf.patchBase + h.patch), h.u, h.v));
-        M4 W;
-        W.c[0] = ob.w2l[0]; W.c[1] = ob.w2l[1]; W.c[2] = ob.w2l[2]; W.c[3] = ob.w2l[3];
-        N = versor3(m3tmul(W, N));
-        dotNV = dot3(N, V);
-        if (dotNV < 0.0f)
-        {
-            N = neg3(N);
-            dotNV = -dotNV;
-        }
-        Pt = add3(w.wo, mul3(h.t, w.wd));
-        li = 0;
-        nextLight();
-    }
-    if (counts)
-    {
-        addWave(counts + 0, nPrimary);
-        addWave(counts + 1, nShadow);
-        addWave(counts + 2, nSecondary);
-        addWave(counts + 3, nOverflow);
-    }
-}
-
 } // namespace cgs
 
 // ---------------------------------------------------------------------------
@@ -1074,9 +627,6 @@ hipError_t persistentGrid(PersistKernel k, uint32_t& blocksPerCU)
     case PersistKernel::TraceAny: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kTraceAny, 256, 0); break;
     case PersistKernel::ClipBatch: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kClipBatch, 256, 0); break;
     case PersistKernel::Render: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kRender, 256, 0); break;
-    case PersistKernel::RenderLanes: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kRenderLanes, 256, 0); break;
-    case PersistKernel::TraceClosestRefill: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kTraceRefill<false>, 256, 0); break;
-    case PersistKernel::TraceAnyRefill: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kTraceRefill<true>, 256, 0); break;
     }
     blocksPerCU = n > 0 ? (uint32_t)n : 1u;
     return e;
@@ -1088,20 +638,6 @@ hipError_t launchTraceClosest(const SceneDev& sc, const PersistArgs& pa, const c
     if (n == 0) return hipSuccess;
     Persist ps {pa.stack, pa.lanes, pa.counter};
     hipLaunchKernelGGL(kTraceClosest, dim3(pa.grid), dim3(256), 0, s, sc, ps, rays, n, hits, objects, overflow);
-    return hipGetLastError();
-}
-
-hipError_t launchTraceRefill(const SceneDev& sc, const PersistArgs& pa, const cgs_ray* rays, uint32_t n, cgs_hit* hits,
-                             int32_t* objects, uint32_t* occluded, unsigned long long* overflow, hipStream_t s)
-{
-    if (n == 0) return hipSuccess;
-    Persist ps {pa.stack, pa.lanes, pa.counter};
-    if (occluded)
-        hipLaunchKernelGGL(kTraceRefill<true>, dim3(pa.grid), dim3(256), 0, s, sc, ps, rays, n, hits, objects, occluded,
-                           overflow);
-    else
-        hipLaunchKernelGGL(kTraceRefill<false>, dim3(pa.grid), dim3(256), 0, s, sc, ps, rays, n, hits, objects, occluded,
-                           overflow);
     return hipGetLastError();
 }
 
@@ -1134,7 +670,7 @@ hipError_t launchSubdivBatch(const cgs_ray* rays, const float* patches, uint32_t
 
 hipError_t launchRender(const SceneDev& sc, const PersistArgs& pa, const float* toWorld16, const float* pos3, float tlx,
                         float tly, float hdx, float hdy, float eps, int depth, uint32_t w, uint32_t h, uint32_t* frame,
-                        float* color, unsigned long long* counts, bool lanes, hipStream_t s)
+                        float* color, unsigned long long* counts, hipStream_t s)
 {
     CamDev cam;
     for (int c = 0; c < 4; c++)
@@ -1149,10 +685,7 @@ hipError_t launchRender(const SceneDev& sc, const PersistArgs& pa, const float* 
     RenderDev op {eps, depth, w, h};
     if ((uint64_t)w * h == 0) return hipSuccess;
     Persist ps {pa.stack, pa.lanes, pa.counter};
-    if (lanes)
-        hipLaunchKernelGGL(kRenderLanes, dim3(pa.grid), dim3(256), 0, s, sc, ps, cam, op, frame, color, counts);
-    else
-        hipLaunchKernelGGL(kRender, dim3(pa.grid), dim3(256), 0, s, sc, ps, cam, op, frame, color, counts);
+    hipLaunchKernelGGL(kRender, dim3(pa.grid), dim3(256), 0, s, sc, ps, cam, op, frame, color, counts);
     return hipGetLastError();
 }
 

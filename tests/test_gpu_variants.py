@@ -1,12 +1,7 @@
-"""The alternative GPU schedules give the default path's results bit for bit
-(the default path is checked against the oracle elsewhere):
-
-* CGS_STAGED=1: the staged wavefront tracer (traverse kernel, compacted clip
-  queue, finish kernel);
-* CGS_RENDER_LANES=1: the lane-scheduled render kernel (per-lane pixel
-  refill, one traversal step per iteration for every kind of ray);
-* CGS_TRACE_REFILL=1: ray queries with per-lane ray refill.
-"""
+"""The alternative GPU schedule gives the default path's results bit for bit
+(the default path is checked against the oracle elsewhere): CGS_STAGED=1
+selects the staged wavefront tracer (traverse kernel, compacted clip queue,
+finish kernel, separate shading passes)."""
 from __future__ import annotations
 
 import os
@@ -18,7 +13,7 @@ from conftest import say
 
 pytestmark = pytest.mark.gpu
 
-VARIANTS = {"staged": {"CGS_STAGED": "1"}, "lanes": {"CGS_RENDER_LANES": "1"}, "refill": {"CGS_TRACE_REFILL": "1"}}
+VARIANTS = {"staged": {"CGS_STAGED": "1"}}
 
 
 def open_with_env(env):
