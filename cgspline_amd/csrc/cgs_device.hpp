@@ -574,7 +574,11 @@ CGS_DEV NodeBoxes loadNode(const float4* __restrict__ nodes, uint32_t i)
     return n;
 }
 
-// _node_intersect (rt/BVH.h:147-176)
+// _node_intersect (rt/BVH.h:147-176).  The reference's "if (tn > tmin) tmin =
+// tn" and "if (tf < tmax) tmax = tf" are computed as maxNum/minNum: a NaN
+// operand (0·inf) leaves the bound unchanged in both forms, and a ±0
+// difference only reaches comparisons (tmin < tmax, L0 < limit, L0 < R0),
+// where -0 == +0; tMin is used by nothing but those comparisons.
 CGS_DEV bool nodeIntersect(float& tMin, V3 bmin, V3 bmax, V3 o, V3 Dinv)
 {
     float tmin = -__builtin_inff(), tmax = __builtin_inff();
@@ -588,8 +592,8 @@ CGS_DEV bool nodeIntersect(float& tMin, V3 bmin, V3 bmax, V3 o, V3 Dinv)
         float tb = (b1[i] - oo[i]) * di[i];
         float tn = s ? tb : ta;
         float tf = s ? ta : tb;
-        if (tn > tmin) tmin = tn;
-        if (tf < tmax) tmax = tf;
+        tmin = __builtin_fmaxf(tmin, tn);
+        tmax = __builtin_fminf(tmax, tf);
     }
     tMin = tmin;
     return tmin < tmax && tmax > 0.0001f;
@@ -650,22 +654,14 @@ CGS_DEV bool hashIntersect(const BVHRef& b, V3 o, V3 dir, const float& limit, Le
             float L0, R0;
             bool iL = nodeIntersect(L0, n.lmin, n.lmax, o, Dinv) && L0 < limit;
             bool iR = nodeIntersect(R0, n.rmin, n.rmax, o, Dinv) && R0 < limit;
-            if (iL || iR) postponed <<= 1;
-            if (iL)
+            if (iL || iR)
             {
-                if (iR)
-                {
-                    postponed |= 1;
-                    if (L0 < R0) { key = key << 1; node = n.left; }
-                    else { key = (key << 1) ^ 1u; node = n.right; }
-                }
-                else { key = key << 1; node = n.left; }
-                continue;
-            }
-            else if (iR)
-            {
-                key = (key << 1) ^ 1u;
-                node = n.right;
+                // both hit: postpone one, descend into the nearer (right on a tie)
+                const bool both = iL && iR;
+                const bool goLeft = both ? L0 < R0 : iL;
+                postponed = (postponed << 1) | (both ? 1u : 0u);
+                key = (key << 1) | (goLeft ? 0u : 1u);
+                node = goLeft ? n.left : n.right;
                 continue;
             }
         }
@@ -706,22 +702,14 @@ CGS_DEV bool hashIntersectAny(const BVHRef& b, V3 o, V3 dir, float tmax, Leaf&& 
             float L0, R0;
             bool iL = nodeIntersect(L0, n.lmin, n.lmax, o, Dinv) && L0 < tmax;
             bool iR = nodeIntersect(R0, n.rmin, n.rmax, o, Dinv) && R0 < tmax;
-            if (iL || iR) postponed <<= 1;
-            if (iL)
+            if (iL || iR)
             {
-                if (iR)
-                {
-                    postponed |= 1;
-                    if (L0 < R0) { key = key << 1; node = n.left; }
-                    else { key = (key << 1) ^ 1u; node = n.right; }
-                }
-                else { key = key << 1; node = n.left; }
-                continue;
-            }
-            else if (iR)
-            {
-                key = (key << 1) ^ 1u;
-                node = n.right;
+                // both hit: postpone one, descend into the nearer (right on a tie)
+                const bool both = iL && iR;
+                const bool goLeft = both ? L0 < R0 : iL;
+                postponed = (postponed << 1) | (both ? 1u : 0u);
+                key = (key << 1) | (goLeft ? 0u : 1u);
+                node = goLeft ? n.left : n.right;
                 continue;
             }
         }
