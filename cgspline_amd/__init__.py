@@ -14,7 +14,7 @@ import numpy as np
 from . import _abi, scenes
 from ._abi import CgsError, EMPTY, PATCH_OVERFLOW, check, lib, ptr
 
-__all__ = ["Context", "DeviceArray", "DeviceScene", "CgsError", "EMPTY", "PATCH_OVERFLOW", "scenes", "lib"]
+__all__ = ["Batch", "Context", "DeviceArray", "DeviceScene", "CgsError", "EMPTY", "PATCH_OVERFLOW", "scenes", "lib"]
 
 
 class DeviceArray:
@@ -148,6 +148,66 @@ class Context:
 
     def sync(self, stream=None):
         check(lib().cgs_stream_sync(self.h, stream), "cgs_stream_sync")
+
+    # --- batched queries over resident scenes -----------------------------------
+    @staticmethod
+    def batch_jobs(jobs) -> "C.Array":
+        """ctypes array of cgs_batch_job from dicts: scene (DeviceScene), rays
+        (device), n, and hits/objects (closest) or occluded (any=True)."""
+        arr = (_abi.BatchJob * max(1, len(jobs)))()
+        for k, j in enumerate(jobs):
+            arr[k] = _abi.BatchJob(j["scene"].h, ptr(j["rays"]), j["n"], 1 if j.get("any") else 0,
+                                   ptr(j.get("hits")), ptr(j.get("objects")), ptr(j.get("occluded")))
+        return arr
+
+    def trace_batch(self, jobs, stream=None):
+        """One persistent launch over every job (cgs_trace_batch)."""
+        arr = self.batch_jobs(jobs)
+        check(lib().cgs_trace_batch(self.h, arr, len(jobs), stream), "cgs_trace_batch")
+
+    def batch(self, jobs) -> "Batch":
+        """A fixed batch captured in a HIP graph (cgs_batch_create)."""
+        return Batch(self, jobs)
+
+    def batch_config(self) -> tuple[int, int]:
+        c = _abi.BatchConfig()
+        check(lib().cgs_batch_get_config(self.h, C.byref(c)), "cgs_batch_get_config")
+        return c.chunk, c.blocks_per_cu
+
+    def set_batch_config(self, chunk: int, blocks_per_cu: int = 0):
+        c = _abi.BatchConfig(chunk, blocks_per_cu)
+        check(lib().cgs_batch_set_config(self.h, C.byref(c)), "cgs_batch_set_config")
+
+    def batch_autotune(self, jobs) -> tuple[int, int]:
+        arr = self.batch_jobs(jobs)
+        c = _abi.BatchConfig()
+        check(lib().cgs_batch_autotune(self.h, arr, len(jobs), C.byref(c)), "cgs_batch_autotune")
+        return c.chunk, c.blocks_per_cu
+
+
+class Batch:
+    """A batched launch captured once in a HIP graph and replayed."""
+
+    def __init__(self, ctx: Context, jobs):
+        self.ctx = ctx
+        arr = ctx.batch_jobs(jobs)
+        p = C.c_void_p()
+        check(lib().cgs_batch_create(ctx.h, arr, len(jobs), C.byref(p)), "cgs_batch_create")
+        self.h = p.value
+
+    def launch(self, stream=None):
+        check(lib().cgs_batch_launch(self.h, stream), "cgs_batch_launch")
+
+    def destroy(self):
+        if self.h is not None and self.ctx.h is not None:
+            lib().cgs_batch_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
 
 
 class DeviceScene:

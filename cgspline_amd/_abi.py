@@ -34,6 +34,8 @@ EXPORTS = [
     "cgs_phf_displacement_size", "cgs_phf_build", "cgs_phf_fill_table", "cgs_phf_hash",
     "cgs_scene_create", "cgs_scene_destroy", "cgs_scene_bvh_info", "cgs_scene_bvh_copy",
     "cgs_trace_closest", "cgs_trace_any", "cgs_render", "cgs_render_running",
+    "cgs_trace_batch", "cgs_batch_create", "cgs_batch_launch", "cgs_batch_destroy", "cgs_batch_set_config",
+    "cgs_batch_get_config", "cgs_batch_autotune",
     "cgs_malloc", "cgs_free", "cgs_memcpy_h2d", "cgs_memcpy_d2h", "cgs_memset", "cgs_stream_sync",
 ]
 
@@ -72,6 +74,15 @@ class SceneDesc(C.Structure):
         ("obj_surface", _P), ("obj_local2world", _P), ("obj_world2local", _P), ("obj_material", _P),
         ("nobj", _U32), ("lights", _P), ("nlight", _U32),
         ("background", _F32 * 3), ("ambient", _F32 * 3), ("hash_seed", _U32)]
+
+
+class BatchJob(C.Structure):
+    _fields_ = [("scene", _P), ("rays", _P), ("n", _U32), ("any", _I32), ("hits", _P), ("objects", _P),
+                ("occluded", _P)]
+
+
+class BatchConfig(C.Structure):
+    _fields_ = [("chunk", _U32), ("blocks_per_cu", _U32)]
 
 
 class Camera(C.Structure):
@@ -121,6 +132,13 @@ def lib():
             "cgs_render": ([_P, _P, C.POINTER(Camera), C.POINTER(RenderOptions), _U32, _U32, _P, _P, _P, _P],
                            C.c_int),
             "cgs_render_running": ([_P], C.c_int),
+            "cgs_trace_batch": ([_P, C.POINTER(BatchJob), _U32, _P], C.c_int),
+            "cgs_batch_create": ([_P, C.POINTER(BatchJob), _U32, C.POINTER(_P)], C.c_int),
+            "cgs_batch_launch": ([_P, _P], C.c_int),
+            "cgs_batch_destroy": ([_P], C.c_int),
+            "cgs_batch_set_config": ([_P, C.POINTER(BatchConfig)], C.c_int),
+            "cgs_batch_get_config": ([_P, C.POINTER(BatchConfig)], C.c_int),
+            "cgs_batch_autotune": ([_P, C.POINTER(BatchJob), _U32, C.POINTER(BatchConfig)], C.c_int),
             "cgs_malloc": ([_P, _SZ, C.POINTER(_P)], C.c_int),
             "cgs_free": ([_P, _P], C.c_int),
             "cgs_memcpy_h2d": ([_P, _P, _P, _SZ, _P], C.c_int),

@@ -14,6 +14,7 @@ namespace cgs
 {
 
 struct SceneDev;
+struct BatchJob;
 
 // One perfect-hash build (one BVH's key set).
 struct PhfJob
@@ -51,8 +52,9 @@ enum class PersistKernel
     TraceAny,
     ClipBatch,
     Render,
+    Batch,
 };
-constexpr int kPersistKernels = 4;
+constexpr int kPersistKernels = 5;
 
 struct PersistArgs
 {
@@ -68,6 +70,10 @@ hipError_t launchTraceClosest(const SceneDev& sc, const PersistArgs& pa, const c
                               int32_t* objects, unsigned long long* overflow, hipStream_t s);
 hipError_t launchTraceAny(const SceneDev& sc, const PersistArgs& pa, const cgs_ray* rays, uint32_t n, uint32_t* occluded,
                           unsigned long long* overflow, hipStream_t s);
+// Batched queries; the counter must be zero at launch (cleared with a fill
+// kernel, so the pair can be captured in a graph).
+hipError_t launchTraceBatch(const BatchJob* jobs, uint32_t njobs, uint32_t total, uint32_t chunk, float2* stack,
+                            uint32_t lanes, unsigned int* counter, uint32_t grid, unsigned long long* overflow, hipStream_t s);
 hipError_t launchClipBatch(const PersistArgs& pa, const cgs_ray* rays, const float* patches, uint32_t n, float tol,
                            cgs_hit* hits, hipStream_t s);
 hipError_t launchSubdivBatch(const cgs_ray* rays, const float* patches, uint32_t n, float threshold, cgs_hit* hits,

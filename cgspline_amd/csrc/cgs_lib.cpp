@@ -97,6 +97,12 @@ struct cgs_ctx
     // wavefront path (traverse / compacted clip queue / finish kernels) when
     // CGS_STAGED=1.  Both give the same results bit for bit.
     bool staged = false;
+    // batched queries: launch shape, job table, completion of the last launch
+    uint32_t batchChunk = 64, batchBlocksPerCU = 0;
+    DeviceBuffer batchTable;
+    size_t batchTableCap = 0;
+    hipEvent_t batchDone = nullptr;
+    bool batchLaunched = false;
     unsigned long long* dOverflow = nullptr; // device counter of overflowed clip stacks
     unsigned int* dCounters = nullptr;       // one work counter per persistent kernel
     float2* dStack = nullptr;                // clipping region stacks of the resident lanes
@@ -647,6 +653,7 @@ cgs_status cgs_open(int device, cgs_ctx** out)
     auto* c = new cgs_ctx;
     c->device = device;
     hipError_t e = hipEventCreateWithFlags(&c->renderDone, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->batchDone, hipEventDisableTiming);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&c->numCUs, hipDeviceAttributeMultiprocessorCount, device);
     if (e == hipSuccess) e = hipMalloc(&c->dOverflow, 256);
     if (e == hipSuccess) e = hipMemset(c->dOverflow, 0, 256);
@@ -669,6 +676,8 @@ cgs_status cgs_close(cgs_ctx* ctx)
     if (!ctx) return CGS_OK;
     (void)hipSetDevice(ctx->device);
     if (ctx->renderDone) (void)hipEventSynchronize(ctx->renderDone), (void)hipEventDestroy(ctx->renderDone);
+    if (ctx->batchDone) (void)hipEventSynchronize(ctx->batchDone), (void)hipEventDestroy(ctx->batchDone);
+    (void)hipDeviceSynchronize();
     if (ctx->dOverflow) (void)hipFree(ctx->dOverflow);
     if (ctx->dCounters) (void)hipFree(ctx->dCounters);
     if (ctx->dStack) (void)hipFree(ctx->dStack);
@@ -1144,6 +1153,243 @@ int cgs_render_running(cgs_ctx* ctx)
 {
     if (!ctx || !ctx->renderLaunched) return 0;
     return hipEventQuery(ctx->renderDone) == hipErrorNotReady ? 1 : 0;
+}
+
+} // extern "C"
+
+// ---------------------------------------------------------------------------
+// batched queries
+// ---------------------------------------------------------------------------
+static cgs_status buildBatchJobs(cgs_ctx* ctx, const cgs_batch_job* jobs, uint32_t njobs, uint32_t chunk,
+                                 std::vector<BatchJob>& out, uint32_t& total)
+{
+    if (njobs && !jobs) return fail(CGS_ERR_INVALID, "batch: null jobs");
+    uint64_t sum = 0;
+    out.clear();
+    for (uint32_t j = 0; j < njobs; j++)
+    {
+        const cgs_batch_job& q = jobs[j];
+        if (!q.scene || q.scene->ctx != ctx) return fail(CGS_ERR_INVALID, "batch: job " + std::to_string(j) + " has no scene of this context");
+        if (q.n == 0) continue;
+        if (!q.rays || (q.any ? !q.occluded : !q.hits))
+            return fail(CGS_ERR_INVALID, "batch: job " + std::to_string(j) + " lacks rays or outputs");
+        BatchJob b {};
+        b.sc = q.scene->view();
+        b.rays = q.rays;
+        b.hits = q.hits;
+        b.objects = q.objects;
+        b.occluded = q.occluded;
+        b.n = q.n;
+        b.rayBase = (uint32_t)sum;
+        b.any = q.any ? 1u : 0u;
+        out.push_back(b);
+        sum += q.n;
+        if (sum + chunk >= (1ull << 32)) return fail(CGS_ERR_INVALID, "batch: more than 2^32 rays");
+    }
+    total = (uint32_t)sum;
+    return CGS_OK;
+}
+
+static cgs_status batchGrid(cgs_ctx* ctx, uint32_t blocksPerCU, uint32_t& grid)
+{
+    const int ki = (int)PersistKernel::Batch;
+    if (ctx->grid[ki] == 0)
+    {
+        uint32_t bpc = 1;
+        HIP_TRY(persistentGrid(PersistKernel::Batch, bpc));
+        ctx->grid[ki] = bpc * (uint32_t)ctx->numCUs;
+    }
+    const uint32_t maxBpc = ctx->grid[ki] / (uint32_t)ctx->numCUs;
+    const uint32_t bpc = blocksPerCU ? std::min(blocksPerCU, maxBpc) : maxBpc;
+    grid = bpc * (uint32_t)ctx->numCUs;
+    return CGS_OK;
+}
+
+static cgs_status validChunk(uint32_t chunk)
+{
+    if (chunk == 0 || chunk % 64 || chunk > 4096) return fail(CGS_ERR_INVALID, "batch: chunk must be a multiple of 64 in [64, 4096]");
+    return CGS_OK;
+}
+
+// One batched launch through the context's own buffers (waits for the
+// previous one to finish before reusing them).
+static cgs_status traceBatch(cgs_ctx* ctx, const cgs_batch_job* jobs, uint32_t njobs, uint32_t chunk, uint32_t bpc,
+                             hipStream_t s)
+{
+    std::vector<BatchJob> table;
+    uint32_t total = 0;
+    cgs_status st = buildBatchJobs(ctx, jobs, njobs, chunk, table, total);
+    if (st != CGS_OK) return st;
+    if (total == 0) return CGS_OK;
+    uint32_t grid = 0;
+    if ((st = batchGrid(ctx, bpc, grid)) != CGS_OK) return st;
+    if (ctx->batchLaunched) HIP_TRY(hipEventSynchronize(ctx->batchDone));
+    if (ctx->batchTableCap < table.size())
+    {
+        HIP_TRY(ctx->batchTable.alloc(table.size() * sizeof(BatchJob)));
+        ctx->batchTableCap = table.size();
+    }
+    // region stacks: the context's, sized for the largest persistent grid
+    PersistArgs pa;
+    if ((st = persistArgs(ctx, PersistKernel::Batch, s, pa)) != CGS_OK) return st;
+    HIP_TRY(hipMemcpyAsync(ctx->batchTable.p, table.data(), table.size() * sizeof(BatchJob), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s)); // the host table is released on return
+    HIP_TRY(launchTraceBatch(ctx->batchTable.as<BatchJob>(), (uint32_t)table.size(), total, chunk, pa.stack,
+                             std::min<uint32_t>(pa.lanes, (uint32_t)ctx->stackLanes), pa.counter, grid,
+                             ctx->dOverflow, s));
+    HIP_TRY(hipEventRecord(ctx->batchDone, s));
+    ctx->batchLaunched = true;
+    return CGS_OK;
+}
+
+struct cgs_batch
+{
+    cgs_ctx* ctx = nullptr;
+    DeviceBuffer table, counter, stack, overflow;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~cgs_batch()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+};
+
+extern "C" {
+
+cgs_status cgs_trace_batch(cgs_ctx* ctx, const cgs_batch_job* jobs, uint32_t njobs, void* stream)
+{
+    if (!ctx) return fail(CGS_ERR_INVALID, "cgs_trace_batch: null context");
+    HIP_TRY(hipSetDevice(ctx->device));
+    std::lock_guard lk(ctx->mu);
+    return traceBatch(ctx, jobs, njobs, ctx->batchChunk, ctx->batchBlocksPerCU, (hipStream_t)stream);
+}
+
+cgs_status cgs_batch_create(cgs_ctx* ctx, const cgs_batch_job* jobs, uint32_t njobs, cgs_batch** out)
+{
+    if (!ctx || !out) return fail(CGS_ERR_INVALID, "cgs_batch_create: null argument");
+    HIP_TRY(hipSetDevice(ctx->device));
+    std::lock_guard lk(ctx->mu);
+    std::vector<BatchJob> table;
+    uint32_t total = 0;
+    cgs_status st = buildBatchJobs(ctx, jobs, njobs, ctx->batchChunk, table, total);
+    if (st != CGS_OK) return st;
+    uint32_t grid = 0;
+    if ((st = batchGrid(ctx, ctx->batchBlocksPerCU, grid)) != CGS_OK) return st;
+    auto b = std::make_unique<cgs_batch>();
+    b->ctx = ctx;
+    const uint32_t lanes = grid * 256;
+    HIP_TRY(b->table.alloc(std::max<size_t>(1, table.size()) * sizeof(BatchJob)));
+    HIP_TRY(b->counter.alloc(256));
+    HIP_TRY(b->overflow.alloc(256));
+    HIP_TRY(b->stack.alloc((size_t)lanes * CGS_CLIP_STACK * kStackWords * sizeof(float2)));
+    if (!table.empty())
+        HIP_TRY(hipMemcpy(b->table.p, table.data(), table.size() * sizeof(BatchJob), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemset(b->overflow.p, 0, 256));
+    // capture: counter cleared by a fill kernel (no memset node), then the launch
+    hipStream_t cs = nullptr;
+    HIP_TRY(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) e = launchFill(b->counter.as<uint32_t>(), 0u, 1, cs);
+    if (e == hipSuccess)
+        e = launchTraceBatch(b->table.as<BatchJob>(), (uint32_t)table.size(), total, ctx->batchChunk,
+                             b->stack.as<float2>(), lanes, b->counter.as<unsigned int>(), grid,
+                             b->overflow.as<unsigned long long>(), cs);
+    hipGraph_t g = nullptr;
+    hipError_t e2 = hipStreamEndCapture(cs, &g);
+    if (e == hipSuccess) e = e2;
+    b->graph = g;
+    if (e == hipSuccess) e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+    (void)hipStreamDestroy(cs);
+    if (e != hipSuccess) return fail(CGS_ERR_HIP, std::string("cgs_batch_create: graph capture: ") + hipGetErrorString(e));
+    *out = b.release();
+    return CGS_OK;
+}
+
+cgs_status cgs_batch_launch(cgs_batch* batch, void* stream)
+{
+    if (!batch) return fail(CGS_ERR_INVALID, "cgs_batch_launch: null batch");
+    HIP_TRY(hipSetDevice(batch->ctx->device));
+    HIP_TRY(hipGraphLaunch(batch->exec, (hipStream_t)stream));
+    return CGS_OK;
+}
+
+cgs_status cgs_batch_destroy(cgs_batch* batch)
+{
+    if (!batch) return CGS_OK;
+    (void)hipSetDevice(batch->ctx->device);
+    (void)hipDeviceSynchronize();
+    delete batch;
+    return CGS_OK;
+}
+
+cgs_status cgs_batch_set_config(cgs_ctx* ctx, const cgs_batch_config* c)
+{
+    if (!ctx || !c) return fail(CGS_ERR_INVALID, "cgs_batch_set_config: null argument");
+    cgs_status st = validChunk(c->chunk);
+    if (st != CGS_OK) return st;
+    std::lock_guard lk(ctx->mu);
+    ctx->batchChunk = c->chunk;
+    ctx->batchBlocksPerCU = c->blocks_per_cu;
+    return CGS_OK;
+}
+
+cgs_status cgs_batch_get_config(cgs_ctx* ctx, cgs_batch_config* c)
+{
+    if (!ctx || !c) return fail(CGS_ERR_INVALID, "cgs_batch_get_config: null argument");
+    std::lock_guard lk(ctx->mu);
+    c->chunk = ctx->batchChunk;
+    c->blocks_per_cu = ctx->batchBlocksPerCU;
+    return CGS_OK;
+}
+
+cgs_status cgs_batch_autotune(cgs_ctx* ctx, const cgs_batch_job* jobs, uint32_t njobs, cgs_batch_config* best)
+{
+    if (!ctx) return fail(CGS_ERR_INVALID, "cgs_batch_autotune: null context");
+    HIP_TRY(hipSetDevice(ctx->device));
+    std::lock_guard lk(ctx->mu);
+    uint32_t grid = 0;
+    cgs_status st = batchGrid(ctx, 0, grid);
+    if (st != CGS_OK) return st;
+    const uint32_t maxBpc = grid / (uint32_t)ctx->numCUs;
+    hipStream_t s = nullptr;
+    HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    HIP_TRY(hipEventCreate(&t0));
+    HIP_TRY(hipEventCreate(&t1));
+    float bestMs = std::numeric_limits<float>::infinity();
+    cgs_batch_config pick {ctx->batchChunk, ctx->batchBlocksPerCU};
+    for (uint32_t chunk : {64u, 128u, 256u})
+        for (uint32_t bpc = maxBpc; bpc >= 1; bpc--)
+        {
+            float ms = 0.0f;
+            for (int rep = 0; rep < 2 && st == CGS_OK; rep++) // first run warms up, second is timed
+            {
+                hipError_t e = hipEventRecord(t0, s);
+                if (e != hipSuccess) { st = fail(CGS_ERR_HIP, hipGetErrorString(e)); break; }
+                st = traceBatch(ctx, jobs, njobs, chunk, bpc, s);
+                if (st != CGS_OK) break;
+                e = hipEventRecord(t1, s);
+                if (e == hipSuccess) e = hipEventSynchronize(t1);
+                if (e == hipSuccess) e = hipEventElapsedTime(&ms, t0, t1);
+                if (e != hipSuccess) st = fail(CGS_ERR_HIP, hipGetErrorString(e));
+            }
+            if (st != CGS_OK) break;
+            if (ms < bestMs)
+            {
+                bestMs = ms;
+                pick = {chunk, bpc == maxBpc ? 0u : bpc};
+            }
+            if (bpc == 1) break;
+        }
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    (void)hipStreamDestroy(s);
+    if (st != CGS_OK) return st;
+    ctx->batchChunk = pick.chunk;
+    ctx->batchBlocksPerCU = pick.blocks_per_cu;
+    if (best) *best = pick;
+    return CGS_OK;
 }
 
 cgs_status cgs_malloc(cgs_ctx* ctx, size_t bytes, void** dptr)

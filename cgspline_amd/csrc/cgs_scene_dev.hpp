@@ -77,6 +77,18 @@ struct SceneDev
     float ambient[3];
 };
 
+// One job of a batched launch: a resident scene and a ray range.  The jobs'
+// rays are numbered consecutively (rayBase = rays of the jobs before it).
+struct BatchJob
+{
+    SceneDev sc;
+    const cgs_ray* rays;
+    cgs_hit* hits;          // closest-hit jobs
+    int32_t* objects;       // optional object ids (closest)
+    uint32_t* occluded;     // any-hit jobs
+    uint32_t n, rayBase, any, pad;
+};
+
 CGS_DEV BVHRef bvhRef(const SceneDev& sc, const DevBVH& d)
 {
     BVHRef b;

@@ -206,6 +206,66 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CGS_TRACE_W
     addWave(overflowCount, nOverflow);
 }
 
+// Batched ray queries over many resident scenes in one launch: rays of all
+// jobs are numbered consecutively and waves take chunks of `chunk` rays (a
+// multiple of 64) from one counter, so small and large jobs share the grid
+// without a launch per scene.  Each lane finds its job by binary search over
+// the jobs' first ray numbers and runs the same per-ray query as
+// kTraceClosest / kTraceAny.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CGS_TRACE_WAVES))) void kTraceBatch(
+    const BatchJob* __restrict__ jobs, uint32_t njobs, uint32_t total, uint32_t chunk, Persist ps,
+    unsigned long long* __restrict__ overflowCount)
+{
+    const uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x;
+    const ClipStack stk = clipStackFor(ps.stack, gl, ps.lanes);
+    const NodeCache none {nullptr, 0};
+    unsigned long long nOverflow = 0;
+    while (true)
+    {
+        uint32_t base = 0;
+        if ((threadIdx.x & 63) == 0) base = atomicAdd(ps.counter, chunk);
+        base = __shfl(base, 0, 64);
+        if (base >= total) break;
+        const uint32_t end = base + chunk < total ? base + chunk : total;
+        for (uint32_t r0 = base; r0 < end; r0 += 64)
+        {
+            const uint32_t r = r0 + (threadIdx.x & 63);
+            if (r >= end) continue;
+            uint32_t lo = 0, hi = njobs;  // last job with rayBase <= r
+            while (hi - lo > 1)
+            {
+                const uint32_t mid = (lo + hi) / 2;
+                if (jobs[mid].rayBase <= r) lo = mid;
+                else hi = mid;
+            }
+            const BatchJob& J = jobs[lo];
+            const uint32_t k = r - J.rayBase;
+            const cgs_ray ray = J.rays[k];
+            bool overflow = false;
+            if (J.any)
+            {
+                const bool occ = sceneAny(J.sc, none, {ray.ox, ray.oy, ray.oz}, {ray.dx, ray.dy, ray.dz}, ray.tmax, stk,
+                                          overflow);
+                J.occluded[k] = overflow ? 2u : (occ ? 1u : 0u);
+            }
+            else
+            {
+                const ClosestHit h = sceneClosest(J.sc, none, {ray.ox, ray.oy, ray.oz}, {ray.dx, ray.dy, ray.dz},
+                                                  ray.tmax, stk, overflow);
+                cgs_hit out;
+                out.t = h.t;
+                out.u = h.object < 0 ? 0.0f : h.u;
+                out.v = h.object < 0 ? 0.0f : h.v;
+                out.patch = overflow ? CGS_PATCH_OVERFLOW : (h.object < 0 ? EMPTY : h.patch);
+                J.hits[k] = out;
+                if (J.objects) J.objects[k] = h.object;
+            }
+            nOverflow += overflow;
+        }
+    }
+    addWave(overflowCount, nOverflow);
+}
+
 // Batched doBezierClipping: ray r against patch r (Spline_BezierClipping.cu:46-114).
 __global__ __launch_bounds__(256) void kClipBatch(Persist ps, const cgs_ray* __restrict__ rays,
                                                   const float4* __restrict__ patches, uint32_t n, float tol,
@@ -627,6 +687,7 @@ hipError_t persistentGrid(PersistKernel k, uint32_t& blocksPerCU)
     case PersistKernel::TraceAny: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kTraceAny, 256, 0); break;
     case PersistKernel::ClipBatch: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kClipBatch, 256, 0); break;
     case PersistKernel::Render: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kRender, 256, 0); break;
+    case PersistKernel::Batch: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kTraceBatch, 256, 0); break;
     }
     blocksPerCU = n > 0 ? (uint32_t)n : 1u;
     return e;
@@ -647,6 +708,15 @@ hipError_t launchTraceAny(const SceneDev& sc, const PersistArgs& pa, const cgs_r
     if (n == 0) return hipSuccess;
     Persist ps {pa.stack, pa.lanes, pa.counter};
     hipLaunchKernelGGL(kTraceAny, dim3(pa.grid), dim3(256), 0, s, sc, ps, rays, n, occluded, overflow);
+    return hipGetLastError();
+}
+
+hipError_t launchTraceBatch(const BatchJob* jobs, uint32_t njobs, uint32_t total, uint32_t chunk, float2* stack,
+                            uint32_t lanes, unsigned int* counter, uint32_t grid, unsigned long long* overflow, hipStream_t s)
+{
+    if (total == 0) return hipSuccess;
+    Persist ps {stack, lanes, counter};
+    hipLaunchKernelGGL(kTraceBatch, dim3(grid), dim3(256), 0, s, jobs, njobs, total, chunk, ps, overflow);
     return hipGetLastError();
 }
 

@@ -326,6 +326,42 @@ def displaced_grids(seed=BASE_SEED + 3, grids=4096, width=2048, height=2048,
                                             "primary + shadow + 1 reflection bounce")
 
 
+@dataclasses.dataclass
+class BatchItem:
+    scene: Scene
+    rays: np.ndarray     # (n, 8) float32
+    any: bool = False
+
+
+def batched_scenes(seed=BASE_SEED + 4, count=64, patches=(1_000, 200_000), rays=(4_096, 1 << 21),
+                   any_fraction=0.0) -> list[BatchItem]:
+    """Many resident scenes with uneven ray batches: each scene is the
+    displaced-grid generator with a log-uniform patch count (whole 16×16-patch
+    grids, density of the 1M-patch scene) and a log-uniform ray count, half
+    camera rays through the scene centre and half random origins in the scene
+    box with uniform directions.  `any_fraction` of the jobs are occlusion
+    queries."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        npatch = math.exp(rng.uniform(math.log(patches[0]), math.log(patches[1])))
+        grids = max(1, int(round(npatch / 256)))
+        nray = int(round(math.exp(rng.uniform(math.log(rays[0]), math.log(rays[1])))))
+        box = 25.0 * (grids / 4096.0) ** (1.0 / 3.0)
+        side = max(8, int(math.ceil(math.sqrt(nray / 2))))
+        wl = displaced_grids(seed=int(rng.integers(1 << 31)), grids=grids, width=side, height=side, box=box)
+        cam = camera_rays(wl.camera)[: nray // 2]
+        m = nray - cam.shape[0]
+        rnd = np.zeros((m, 8), np.float32)
+        rnd[:, 0:3] = rng.uniform(0.0, box, (m, 3))
+        d = rng.normal(size=(m, 3))
+        rnd[:, 4:7] = d / np.linalg.norm(d, axis=1, keepdims=True)
+        rnd[:, 3] = np.inf
+        out.append(BatchItem(wl.scene, np.concatenate([cam, rnd]).astype(np.float32),
+                             bool(rng.random() < any_fraction)))
+    return out
+
+
 def camera_rays(cam: Camera, rows=None) -> np.ndarray:
     """World rays (n, 8) float32 (o xyz, tmax, d xyz, 0) through pixel centres,
     in float32 with the reference's camera formula (RayTracer.cu:133-145,

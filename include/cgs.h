@@ -142,6 +142,30 @@ typedef struct cgs_ray_counts
 
 typedef struct cgs_ctx cgs_ctx;
 typedef struct cgs_scene cgs_scene;
+typedef struct cgs_batch cgs_batch;
+
+/* One job of a batched query: a ray range against one resident scene
+ * (device pointers).  any = 0: closest hits into hits (and objects, nullable);
+ * any = 1: occlusion flags into occluded (1 / 0, 2 on a clip-stack overflow). */
+typedef struct cgs_batch_job
+{
+    cgs_scene* scene;
+    const cgs_ray* rays;
+    uint32_t n;
+    int32_t any;
+    cgs_hit* hits;
+    int32_t* objects;
+    uint32_t* occluded;
+} cgs_batch_job;
+
+/* Launch shape of batched queries: rays taken per wave fetch (a multiple of
+ * 64) and resident workgroups per CU (0 = as many as fit).  Neither changes
+ * any result. */
+typedef struct cgs_batch_config
+{
+    uint32_t chunk;
+    uint32_t blocks_per_cu;
+} cgs_batch_config;
 
 /* ---------------------------------------------------------------------- */
 /* Context                                                                 */
@@ -250,6 +274,31 @@ cgs_status cgs_render(cgs_ctx* ctx, cgs_scene* scene, const cgs_camera* camera,
                       uint32_t* frame, float* color, cgs_ray_counts* counts, void* stream);
 /* RayTracer::running() (rt/RayTracer.h:45-48): 1 while the last render runs. */
 int cgs_render_running(cgs_ctx* ctx);
+
+/* ---------------------------------------------------------------------- */
+/* Batched queries over many resident scenes (no counterpart in the        */
+/* reference: each job is Context::intersect over a ray range,             */
+/* RayTracer.cu:369-457, and all jobs share one persistent launch)         */
+/* ---------------------------------------------------------------------- */
+
+/* One launch over every job.  Asynchronous on `stream`; batched launches of
+ * one context run one at a time (a call waits for the previous one's kernel
+ * before reusing the context's job table). */
+cgs_status cgs_trace_batch(cgs_ctx* ctx, const cgs_batch_job* jobs, uint32_t njobs, void* stream);
+/* A fixed batch with its own job table, counter and region stacks, captured
+ * once in a HIP graph; cgs_batch_launch replays it (results as
+ * cgs_trace_batch).  Different batches may run concurrently on different
+ * streams; one batch must not be relaunched before its previous launch ends. */
+cgs_status cgs_batch_create(cgs_ctx* ctx, const cgs_batch_job* jobs, uint32_t njobs, cgs_batch** out);
+cgs_status cgs_batch_launch(cgs_batch* batch, void* stream);
+cgs_status cgs_batch_destroy(cgs_batch* batch);
+/* Launch shape of the context's later batched calls and batches. */
+cgs_status cgs_batch_set_config(cgs_ctx* ctx, const cgs_batch_config* config);
+cgs_status cgs_batch_get_config(cgs_ctx* ctx, cgs_batch_config* config);
+/* Times every candidate shape (chunk 64/128/256, all to one workgroup per CU
+ * below the maximum) on the given jobs, keeps the fastest in the context and
+ * returns it.  The jobs' outputs are overwritten with valid results. */
+cgs_status cgs_batch_autotune(cgs_ctx* ctx, const cgs_batch_job* jobs, uint32_t njobs, cgs_batch_config* best);
 
 /* ---------------------------------------------------------------------- */
 /* Device memory helpers (for callers without their own allocator)         */
