@@ -50,6 +50,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample duration")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-work-count", action="store_true", help="skip the oracle's work count (no roofline)")
+    ap.add_argument("--mode", choices=["plain", "batched", "serving"], default="plain",
+                    help="plain: the 1M-patch render (the headline line); batched: 64 resident scenes traced "
+                         "per job vs in one persistent launch; serving: continuous batching under Poisson load")
+    ap.add_argument("--scenes", type=int, default=64, help="batched/serving: number of resident scenes")
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
                     help="also run one streaming copy kernel (bitwise_not over int32) of this many MiB after the "
                          "warmup: the reference kernel that scales the HBM byte counters (FETCH_SIZE, WRITE_SIZE) "
@@ -61,6 +65,10 @@ def main():
               "(rays are independent: the multi-GPU form replicates the scene and splits the frame).",
               file=sys.stderr)
         sys.exit(2)
+    if args.mode == "batched":
+        return run_batched(args)
+    if args.mode == "serving":
+        return run_serving(args)
 
     import torch
     import cgspline_amd as cgs
@@ -220,6 +228,154 @@ def main():
                       f"median of {len(runs)} runs ({sum(runs):.1f}s CPU wall in total): CPURayTracer "
                       f"restatement (32x32 tiles, {threads} worker threads), same scene",
         }
+    print(json.dumps(out), flush=True)
+
+
+def run_batched(args):
+    """64 resident scenes (patch counts log-uniform in [1k, 200k], ray batches
+    log-uniform in [4k, 2M], half camera and half random rays): the per-job
+    form (one cgs_trace_closest per scene, back to back on one stream) against
+    one persistent batched launch, its graph replay and the autotuned shape."""
+    import torch
+    import cgspline_amd as cgs
+    from cgspline_amd import scenes
+
+    t0 = time.time()
+    items = scenes.batched_scenes(count=args.scenes)
+    npatch = sum(i.scene.npatch for i in items)
+    nray = sum(i.rays.shape[0] for i in items)
+    log(f"{len(items)} scenes, {npatch} patches, {nray} rays (generated in {time.time() - t0:.1f}s)")
+    ctx = cgs.Context(0)
+    t0 = time.time()
+    jobs = []
+    for it in items:
+        ds = ctx.scene(it.scene, hash_retries=8)
+        n = it.rays.shape[0]
+        jobs.append(dict(scene=ds, rays=cgs.DeviceArray.from_host(ctx, it.rays), n=n,
+                         hits=cgs.DeviceArray(ctx, 16 * n), objects=cgs.DeviceArray(ctx, 4 * n)))
+    log(f"scenes resident in {time.time() - t0:.1f}s")
+    stream = torch.cuda.current_stream(torch.device("cuda:0"))
+    sh = stream.cuda_stream
+
+    def timed(fn):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(args.steps):
+            fn()
+        b.record(stream)
+        b.synchronize()
+        return a.elapsed_time(b) / args.steps
+
+    def per_job():
+        for j in jobs:
+            j["scene"].trace_closest_dev(j["rays"], j["n"], j["hits"], j["objects"], stream=sh)
+
+    ms_plain = timed(per_job)
+    ms_batch = timed(lambda: ctx.trace_batch(jobs, stream=sh))
+    graph = ctx.batch(jobs)
+    ms_graph = timed(lambda: graph.launch(sh))
+    graph.destroy()
+    tuned = ctx.batch_autotune(jobs)
+    ms_tuned = timed(lambda: ctx.trace_batch(jobs, stream=sh))
+    rate = lambda ms: round(nray / ms / 1e3, 3)
+    log(f"per-job {ms_plain:.2f} ms, batched {ms_batch:.2f} ms, graph {ms_graph:.2f} ms, autotuned {tuned} "
+        f"{ms_tuned:.2f} ms")
+    best = min(ms_batch, ms_graph, ms_tuned)
+    out = {
+        "metric": f"Mrays/s over {len(items)} resident scenes (batched)",
+        "value": rate(best),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(best, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded displaced-grid scenes, camera and random rays)",
+        "config": {"workload": "batched_scenes", "scenes": len(items), "patches": npatch, "rays": nray},
+        "per_job_value": rate(ms_plain),
+        "batched_over_per_job": round(ms_plain / best, 3),
+        "forms_ms": {"per_job": round(ms_plain, 3), "batched": round(ms_batch, 3), "graph": round(ms_graph, 3),
+                     "autotuned": round(ms_tuned, 3)},
+        "autotuned_shape": {"chunk": tuned[0], "blocks_per_cu": tuned[1]},
+    }
+    print(json.dumps(out), flush=True)
+
+
+def run_serving(args):
+    """8 resident scenes (10k-1M patches, one the 1,048,576-patch scene),
+    requests of log-uniform 1k-4M rays (half camera, half random rays; 40 %
+    occlusion queries) arriving as a Poisson process at 30/60/90 % of the
+    server's measured capacity; p50/p99 latency (submit to results in host
+    memory) and sustained Mrays/s per load."""
+    import cgspline_amd as cgs
+    from cgspline_amd import scenes
+    from cgspline_amd.serving import Server, request_sizes, run_load
+
+    t0 = time.time()
+    rng = np.random.default_rng(20261202 + 5)
+    items = scenes.batched_scenes(seed=20261202 + 5, count=args.scenes - 1, patches=(10_000, 1_000_000),
+                                  rays=(1 << 20, 1 << 21))
+    big = scenes.displaced_grids()
+    items.append(scenes.BatchItem(big.scene, scenes.camera_rays(big.camera)))
+    ctx = cgs.Context(0)
+    ds = [ctx.scene(it.scene, hash_retries=8) for it in items]
+    log(f"{len(items)} resident scenes, {sum(i.scene.npatch for i in items)} patches ({time.time() - t0:.1f}s)")
+
+    def make_requests(count, seed):
+        r = np.random.default_rng(seed)
+        sizes = request_sizes(r, count)
+        reqs = []
+        for n in sizes:
+            s = int(r.integers(0, len(items)))
+            pool = items[s].rays
+            n = int(min(n, pool.shape[0]))
+            start = int(r.integers(0, pool.shape[0] - n + 1))
+            reqs.append((s, pool[start:start + n], bool(r.random() < 0.4)))
+        return reqs
+
+    budget = 8 << 30
+    sv = Server(ctx, ds, hbm_budget=budget, slots=2)
+    warm = make_requests(8, 1)
+    run_load(sv, warm, rate_hz=0.0)
+    cap_reqs = make_requests(args.steps * 4, 2)
+    lats, nrays, wall = run_load(sv, cap_reqs, rate_hz=0.0)
+    capacity = nrays / wall
+    mean_rays = nrays / len(cap_reqs)
+    log(f"capacity (all requests at once): {capacity / 1e6:.1f} Mrays/s, mean request {mean_rays:.0f} rays")
+    loads = {}
+    for frac in (0.3, 0.6, 0.9):
+        reqs = make_requests(args.steps * 4, int(frac * 100))
+        rate = frac * capacity / mean_rays
+        lats, nrays, wall = run_load(sv, reqs, rate_hz=rate, seed=int(frac * 10))
+        loads[f"{int(frac * 100)}%"] = {"offered_req_per_s": round(rate, 2), "requests": len(reqs),
+                                        "p50_ms": round(float(np.percentile(lats, 50)) * 1e3, 2),
+                                        "p99_ms": round(float(np.percentile(lats, 99)) * 1e3, 2),
+                                        "sustained_mrays_s": round(nrays / wall / 1e6, 2)}
+        log(f"load {frac:.0%}: {loads[f'{int(frac * 100)}%']}")
+    sv.close()
+    out = {
+        "metric": "Mrays/s serving capacity; p50/p99 request latency at 30/60/90% load",
+        "value": round(capacity / 1e6, 3),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": 1,
+        "ms_per_step": None,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded displaced-grid scenes; camera and random rays; Poisson arrivals)",
+        "config": {"workload": "serving", "scenes": len(items), "hbm_budget_bytes": budget, "slots": 2,
+                   "request_rays": "log-uniform [1e3, 4e6]", "occlusion_fraction": 0.4},
+        "loads": loads,
+    }
     print(json.dumps(out), flush=True)
 
 

@@ -8,6 +8,7 @@ device :class:`Context` raises.
 from __future__ import annotations
 
 import ctypes as C
+import dataclasses
 
 import numpy as np
 
@@ -143,8 +144,16 @@ class Context:
         return out.to_host(np.uint32, keys.shape)
 
     # --- scenes -------------------------------------------------------------------
-    def scene(self, scene: "scenes.Scene") -> "DeviceScene":
-        return DeviceScene(self, scene)
+    def scene(self, scene: "scenes.Scene", hash_retries: int = 0) -> "DeviceScene":
+        """Makes `scene` resident.  Like the reference's BVH::buildHashTable, a
+        perfect hash that cannot be built raises; with hash_retries > 0 the
+        build is retried with the next hash seeds (PerfectHashFunction(seed))."""
+        for k in range(hash_retries + 1):
+            try:
+                return DeviceScene(self, scene if k == 0 else dataclasses.replace(scene, hash_seed=scene.hash_seed + k))
+            except CgsError as e:
+                if e.status != 3 or k == hash_retries:
+                    raise
 
     def sync(self, stream=None):
         check(lib().cgs_stream_sync(self.h, stream), "cgs_stream_sync")

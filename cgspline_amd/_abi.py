@@ -36,6 +36,7 @@ EXPORTS = [
     "cgs_trace_closest", "cgs_trace_any", "cgs_render", "cgs_render_running",
     "cgs_trace_batch", "cgs_batch_create", "cgs_batch_launch", "cgs_batch_destroy", "cgs_batch_set_config",
     "cgs_batch_get_config", "cgs_batch_autotune",
+    "cgs_server_create", "cgs_server_submit", "cgs_server_poll", "cgs_server_wait", "cgs_server_destroy",
     "cgs_malloc", "cgs_free", "cgs_memcpy_h2d", "cgs_memcpy_d2h", "cgs_memset", "cgs_stream_sync",
 ]
 
@@ -83,6 +84,10 @@ class BatchJob(C.Structure):
 
 class BatchConfig(C.Structure):
     _fields_ = [("chunk", _U32), ("blocks_per_cu", _U32)]
+
+
+class ServerOptions(C.Structure):
+    _fields_ = [("hbm_budget", _SZ), ("slots", _U32), ("max_batch_rays", _U32)]
 
 
 class Camera(C.Structure):
@@ -139,6 +144,11 @@ def lib():
             "cgs_batch_set_config": ([_P, C.POINTER(BatchConfig)], C.c_int),
             "cgs_batch_get_config": ([_P, C.POINTER(BatchConfig)], C.c_int),
             "cgs_batch_autotune": ([_P, C.POINTER(BatchJob), _U32, C.POINTER(BatchConfig)], C.c_int),
+            "cgs_server_create": ([_P, C.POINTER(_P), _U32, C.POINTER(ServerOptions), C.POINTER(_P)], C.c_int),
+            "cgs_server_submit": ([_P, _U32, _P, _U32, _I32, _P, _P, _P, C.POINTER(_U64)], C.c_int),
+            "cgs_server_poll": ([_P, _U64, C.POINTER(_U64)], C.c_int),
+            "cgs_server_wait": ([_P, _U64, C.POINTER(_U64)], C.c_int),
+            "cgs_server_destroy": ([_P], C.c_int),
             "cgs_malloc": ([_P, _SZ, C.POINTER(_P)], C.c_int),
             "cgs_free": ([_P, _P], C.c_int),
             "cgs_memcpy_h2d": ([_P, _P, _P, _SZ, _P], C.c_int),

@@ -8,6 +8,9 @@
 #include <algorithm>
 #include <atomic>
 #include <bit>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -21,6 +24,7 @@
 #include <span>
 #include <string>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "cgs_internal.hpp"
@@ -1389,6 +1393,349 @@ cgs_status cgs_batch_autotune(cgs_ctx* ctx, const cgs_batch_job* jobs, uint32_t 
     ctx->batchChunk = pick.chunk;
     ctx->batchBlocksPerCU = pick.blocks_per_cu;
     if (best) *best = pick;
+    return CGS_OK;
+}
+
+} // extern "C"
+
+// ---------------------------------------------------------------------------
+// serving: continuous batching over resident scenes
+// ---------------------------------------------------------------------------
+namespace
+{
+
+using Clock = std::chrono::steady_clock;
+
+struct ServerRequest
+{
+    uint64_t id = 0;
+    uint32_t scene = 0, n = 0;
+    bool any = false;
+    const cgs_ray* rays = nullptr;
+    cgs_hit* hits = nullptr;
+    int32_t* objects = nullptr;
+    uint32_t* occluded = nullptr;
+    size_t bytes = 0;              // device bytes while in flight
+    Clock::time_point submitted;
+    uint64_t rayOff = 0;           // position in the slot's launch
+};
+
+// Device bytes of a request: rays in, 16-B hit + 4-B object (or flag) out.
+inline size_t requestBytes(uint32_t n) { return (size_t)n * (sizeof(cgs_ray) + sizeof(cgs_hit) + 4); }
+
+struct PinnedBuffer
+{
+    void* p = nullptr;
+    size_t cap = 0;
+    ~PinnedBuffer()
+    {
+        if (p) (void)hipHostFree(p);
+    }
+    hipError_t reserve(size_t bytes)
+    {
+        if (bytes <= cap) return hipSuccess;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+        hipError_t e = hipHostMalloc(&p, bytes, hipHostMallocDefault);
+        if (e == hipSuccess) cap = bytes;
+        return e;
+    }
+};
+
+struct ServerSlot
+{
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    DeviceBuffer rays, out, table, counter, stack, overflow;
+    size_t rayCap = 0, outCap = 0, tableCap = 0;
+    PinnedBuffer hRays, hOut, hTable;
+    std::vector<ServerRequest> reqs;
+    size_t bytes = 0;
+    uint64_t total = 0;
+    bool busy = false;
+};
+
+struct Done
+{
+    uint64_t latencyNs;
+    int ok;
+};
+
+} // namespace
+
+struct cgs_server
+{
+    cgs_ctx* ctx = nullptr;
+    std::vector<cgs_scene*> scenes;
+    std::vector<SceneDev> views;
+    size_t budget = 0;
+    uint32_t maxBatchRays = 0;
+    uint32_t grid = 0, lanes = 0, chunk = 64;
+    std::vector<ServerSlot> slots;
+    std::mutex mu;
+    std::condition_variable wake, finished;
+    std::deque<ServerRequest> pending;
+    std::unordered_map<uint64_t, Done> done;
+    uint64_t nextId = 1;
+    size_t inflight = 0;
+    bool stop = false;
+    std::thread worker;
+
+    // Copies a finished slot's results to the requests' host buffers.
+    void complete(ServerSlot& sl, int ok)
+    {
+        const Clock::time_point now = Clock::now();
+        const uint8_t* base = (const uint8_t*)sl.hOut.p;
+        for (const ServerRequest& r : sl.reqs)
+        {
+            if (ok)
+            {
+                const uint8_t* o = base + r.rayOff * (sizeof(cgs_hit) + 4);
+                if (r.any)
+                    std::memcpy(r.occluded, o + (size_t)r.n * sizeof(cgs_hit), (size_t)r.n * 4);
+                else
+                {
+                    std::memcpy(r.hits, o, (size_t)r.n * sizeof(cgs_hit));
+                    if (r.objects) std::memcpy(r.objects, o + (size_t)r.n * sizeof(cgs_hit), (size_t)r.n * 4);
+                }
+            }
+            std::lock_guard lk(mu);
+            done[r.id] = {(uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(now - r.submitted).count(), ok};
+            inflight -= r.bytes;
+        }
+        sl.reqs.clear();
+        sl.busy = false;
+        finished.notify_all();
+    }
+
+    // One batched launch of the slot's requests: rays up, trace, results down.
+    hipError_t launch(ServerSlot& sl)
+    {
+        uint64_t total = 0;
+        for (ServerRequest& r : sl.reqs)
+        {
+            r.rayOff = total;
+            total += r.n;
+        }
+        sl.total = total;
+        const size_t outBytes = total * (sizeof(cgs_hit) + 4);
+        hipError_t e = hipSuccess;
+        if (sl.rayCap < total)
+        {
+            if ((e = sl.rays.alloc(total * sizeof(cgs_ray))) != hipSuccess) return e;
+            sl.rayCap = total;
+        }
+        if (sl.outCap < total)
+        {
+            if ((e = sl.out.alloc(outBytes)) != hipSuccess) return e;
+            sl.outCap = total;
+        }
+        if (sl.tableCap < sl.reqs.size())
+        {
+            if ((e = sl.table.alloc(sl.reqs.size() * sizeof(BatchJob))) != hipSuccess) return e;
+            sl.tableCap = sl.reqs.size();
+        }
+        if ((e = sl.hRays.reserve(total * sizeof(cgs_ray))) != hipSuccess) return e;
+        if ((e = sl.hOut.reserve(outBytes)) != hipSuccess) return e;
+        if ((e = sl.hTable.reserve(sl.reqs.size() * sizeof(BatchJob))) != hipSuccess) return e;
+        cgs_ray* hr = (cgs_ray*)sl.hRays.p;
+        BatchJob* jt = (BatchJob*)sl.hTable.p;
+        uint8_t* dOut = sl.out.as<uint8_t>();
+        for (size_t k = 0; k < sl.reqs.size(); k++)
+        {
+            const ServerRequest& r = sl.reqs[k];
+            std::memcpy(hr + r.rayOff, r.rays, (size_t)r.n * sizeof(cgs_ray));
+            BatchJob b {};
+            b.sc = views[r.scene];
+            b.rays = sl.rays.as<cgs_ray>() + r.rayOff;
+            uint8_t* o = dOut + r.rayOff * (sizeof(cgs_hit) + 4);
+            b.hits = r.any ? nullptr : (cgs_hit*)o;
+            b.objects = r.any ? nullptr : (int32_t*)(o + (size_t)r.n * sizeof(cgs_hit));
+            b.occluded = r.any ? (uint32_t*)(o + (size_t)r.n * sizeof(cgs_hit)) : nullptr;
+            b.n = r.n;
+            b.rayBase = (uint32_t)r.rayOff;
+            b.any = r.any ? 1u : 0u;
+            jt[k] = b;
+        }
+        if ((e = hipMemcpyAsync(sl.rays.p, hr, total * sizeof(cgs_ray), hipMemcpyHostToDevice, sl.stream)) != hipSuccess)
+            return e;
+        if ((e = hipMemcpyAsync(sl.table.p, jt, sl.reqs.size() * sizeof(BatchJob), hipMemcpyHostToDevice,
+                                sl.stream)) != hipSuccess)
+            return e;
+        if ((e = launchFill(sl.counter.as<uint32_t>(), 0u, 1, sl.stream)) != hipSuccess) return e;
+        if ((e = launchTraceBatch(sl.table.as<BatchJob>(), (uint32_t)sl.reqs.size(), (uint32_t)total, chunk,
+                                  sl.stack.as<float2>(), lanes, sl.counter.as<unsigned int>(), grid,
+                                  sl.overflow.as<unsigned long long>(), sl.stream)) != hipSuccess)
+            return e;
+        if ((e = hipMemcpyAsync(sl.hOut.p, sl.out.p, outBytes, hipMemcpyDeviceToHost, sl.stream)) != hipSuccess) return e;
+        if ((e = hipEventRecord(sl.done, sl.stream)) != hipSuccess) return e;
+        sl.busy = true;
+        return hipSuccess;
+    }
+
+    void run()
+    {
+        (void)hipSetDevice(ctx->device);
+        while (true)
+        {
+            bool anyBusy = false;
+            for (ServerSlot& sl : slots)
+            {
+                if (!sl.busy) continue;
+                const hipError_t q = hipEventQuery(sl.done);
+                if (q == hipErrorNotReady)
+                    anyBusy = true;
+                else
+                    complete(sl, q == hipSuccess ? 1 : 0);
+            }
+            for (ServerSlot& sl : slots)
+            {
+                if (sl.busy) continue;
+                {
+                    std::lock_guard lk(mu);
+                    uint64_t rays = 0;
+                    // admit in arrival order while the byte budget (and ray cap) allows
+                    while (!pending.empty())
+                    {
+                        const ServerRequest& r = pending.front();
+                        if (inflight + r.bytes > budget && !(sl.reqs.empty() && inflight == 0)) break;
+                        if (maxBatchRays && !sl.reqs.empty() && rays + r.n > maxBatchRays) break;
+                        if (rays + r.n + chunk >= (1ull << 32)) break;
+                        inflight += r.bytes;
+                        rays += r.n;
+                        sl.reqs.push_back(r);
+                        pending.pop_front();
+                    }
+                }
+                if (sl.reqs.empty()) continue;
+                if (launch(sl) != hipSuccess)
+                {
+                    (void)hipStreamSynchronize(sl.stream);
+                    complete(sl, 0);
+                }
+                else
+                    anyBusy = true;
+            }
+            std::unique_lock lk(mu);
+            if (stop && pending.empty() && !anyBusy) break;
+            if (anyBusy)
+                wake.wait_for(lk, std::chrono::microseconds(50));
+            else
+                wake.wait(lk, [&] { return stop || !pending.empty(); });
+        }
+    }
+};
+
+extern "C" {
+
+cgs_status cgs_server_create(cgs_ctx* ctx, cgs_scene* const* scenes, uint32_t nscenes, const cgs_server_options* opt,
+                             cgs_server** out)
+{
+    if (!ctx || !out || (nscenes && !scenes)) return fail(CGS_ERR_INVALID, "cgs_server_create: null argument");
+    HIP_TRY(hipSetDevice(ctx->device));
+    auto sv = std::make_unique<cgs_server>();
+    sv->ctx = ctx;
+    for (uint32_t k = 0; k < nscenes; k++)
+    {
+        if (!scenes[k] || scenes[k]->ctx != ctx) return fail(CGS_ERR_INVALID, "cgs_server_create: scene of another context");
+        sv->scenes.push_back(scenes[k]);
+        sv->views.push_back(scenes[k]->view());
+    }
+    sv->budget = opt && opt->hbm_budget ? opt->hbm_budget : ((size_t)1 << 30);
+    sv->maxBatchRays = opt ? opt->max_batch_rays : 0;
+    const uint32_t nslots = opt && opt->slots ? opt->slots : 2;
+    {
+        std::lock_guard lk(ctx->mu);
+        cgs_status st = batchGrid(ctx, ctx->batchBlocksPerCU, sv->grid);
+        if (st != CGS_OK) return st;
+        sv->chunk = ctx->batchChunk;
+    }
+    sv->lanes = sv->grid * 256;
+    sv->slots.resize(nslots);
+    for (ServerSlot& sl : sv->slots)
+    {
+        HIP_TRY(hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking));
+        HIP_TRY(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
+        HIP_TRY(sl.counter.alloc(256));
+        HIP_TRY(sl.overflow.alloc(256));
+        HIP_TRY(hipMemset(sl.overflow.p, 0, 256));
+        HIP_TRY(sl.stack.alloc((size_t)sv->lanes * CGS_CLIP_STACK * kStackWords * sizeof(float2)));
+    }
+    cgs_server* raw = sv.get();
+    raw->worker = std::thread([raw] { raw->run(); });
+    *out = sv.release();
+    return CGS_OK;
+}
+
+cgs_status cgs_server_submit(cgs_server* sv, uint32_t scene, const cgs_ray* rays, uint32_t n, int32_t any,
+                             cgs_hit* hits, int32_t* objects, uint32_t* occluded, uint64_t* id)
+{
+    if (!sv || !id) return fail(CGS_ERR_INVALID, "cgs_server_submit: null argument");
+    if (scene >= sv->scenes.size()) return fail(CGS_ERR_INVALID, "cgs_server_submit: no such scene");
+    if (n && (!rays || (any ? !occluded : !hits))) return fail(CGS_ERR_INVALID, "cgs_server_submit: missing buffers");
+    ServerRequest r;
+    r.scene = scene;
+    r.n = n;
+    r.any = any != 0;
+    r.rays = rays;
+    r.hits = hits;
+    r.objects = objects;
+    r.occluded = occluded;
+    r.bytes = requestBytes(n);
+    if (r.bytes > sv->budget) return fail(CGS_ERR_INVALID, "cgs_server_submit: request exceeds the byte budget");
+    r.submitted = Clock::now();
+    {
+        std::lock_guard lk(sv->mu);
+        if (sv->stop) return fail(CGS_ERR_INVALID, "cgs_server_submit: server stopping");
+        r.id = sv->nextId++;
+        *id = r.id;
+        if (n == 0)
+            sv->done[r.id] = {0, 1};
+        else
+            sv->pending.push_back(r);
+    }
+    sv->wake.notify_one();
+    return CGS_OK;
+}
+
+int cgs_server_poll(cgs_server* sv, uint64_t id, uint64_t* latency_ns)
+{
+    if (!sv) return -1;
+    std::lock_guard lk(sv->mu);
+    auto it = sv->done.find(id);
+    if (it == sv->done.end()) return 0;
+    if (latency_ns) *latency_ns = it->second.latencyNs;
+    return it->second.ok ? 1 : -1;
+}
+
+cgs_status cgs_server_wait(cgs_server* sv, uint64_t id, uint64_t* latency_ns)
+{
+    if (!sv) return fail(CGS_ERR_INVALID, "cgs_server_wait: null server");
+    std::unique_lock lk(sv->mu);
+    if (id == 0 || id >= sv->nextId) return fail(CGS_ERR_INVALID, "cgs_server_wait: unknown request");
+    sv->finished.wait(lk, [&] { return sv->done.count(id) != 0; });
+    const Done d = sv->done[id];
+    if (latency_ns) *latency_ns = d.latencyNs;
+    return d.ok ? CGS_OK : fail(CGS_ERR_HIP, "cgs_server_wait: the request's launch failed");
+}
+
+cgs_status cgs_server_destroy(cgs_server* sv)
+{
+    if (!sv) return CGS_OK;
+    {
+        std::lock_guard lk(sv->mu);
+        sv->stop = true;
+    }
+    sv->wake.notify_all();
+    if (sv->worker.joinable()) sv->worker.join();
+    (void)hipSetDevice(sv->ctx->device);
+    for (ServerSlot& sl : sv->slots)
+    {
+        if (sl.stream) (void)hipStreamSynchronize(sl.stream), (void)hipStreamDestroy(sl.stream);
+        if (sl.done) (void)hipEventDestroy(sl.done);
+    }
+    delete sv;
     return CGS_OK;
 }
 

@@ -143,6 +143,7 @@ typedef struct cgs_ray_counts
 typedef struct cgs_ctx cgs_ctx;
 typedef struct cgs_scene cgs_scene;
 typedef struct cgs_batch cgs_batch;
+typedef struct cgs_server cgs_server;
 
 /* One job of a batched query: a ray range against one resident scene
  * (device pointers).  any = 0: closest hits into hits (and objects, nullable);
@@ -299,6 +300,36 @@ cgs_status cgs_batch_get_config(cgs_ctx* ctx, cgs_batch_config* config);
  * below the maximum) on the given jobs, keeps the fastest in the context and
  * returns it.  The jobs' outputs are overwritten with valid results. */
 cgs_status cgs_batch_autotune(cgs_ctx* ctx, const cgs_batch_job* jobs, uint32_t njobs, cgs_batch_config* best);
+
+/* ---------------------------------------------------------------------- */
+/* Serving: continuous batching of ray-query requests against resident     */
+/* scenes (no counterpart in the reference)                                */
+/* ---------------------------------------------------------------------- */
+
+typedef struct cgs_server_options
+{
+    size_t hbm_budget;      /* bytes of device ray + result buffers in flight (0: 1 GiB) */
+    uint32_t slots;         /* launches in flight, each on its own stream (0: 2) */
+    uint32_t max_batch_rays;/* rays per launch (0: no limit beyond the budget) */
+} cgs_server_options;
+
+/* A server over resident scenes: a worker thread admits pending requests
+ * into batched launches whenever a slot is free and the byte budget allows,
+ * so requests that arrive while others run join the next launch. */
+cgs_status cgs_server_create(cgs_ctx* ctx, cgs_scene* const* scenes, uint32_t nscenes,
+                             const cgs_server_options* options, cgs_server** out);
+/* Queues a request: n host rays against scene `scene`; any = 0: closest hits
+ * into host `hits` (and `objects`, nullable); any = 1: occlusion flags into
+ * host `occluded`.  The buffers must stay valid until the request is done.
+ * Fails with CGS_ERR_INVALID if the request alone exceeds the byte budget. */
+cgs_status cgs_server_submit(cgs_server* server, uint32_t scene, const cgs_ray* rays, uint32_t n, int32_t any,
+                             cgs_hit* hits, int32_t* objects, uint32_t* occluded, uint64_t* id);
+/* 1 when request `id` is done (outputs written), 0 while pending;
+ * latency_ns (nullable): submit to done. */
+int cgs_server_poll(cgs_server* server, uint64_t id, uint64_t* latency_ns);
+cgs_status cgs_server_wait(cgs_server* server, uint64_t id, uint64_t* latency_ns);
+/* Finishes every queued request, then stops the worker. */
+cgs_status cgs_server_destroy(cgs_server* server);
 
 /* ---------------------------------------------------------------------- */
 /* Device memory helpers (for callers without their own allocator)         */
