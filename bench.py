@@ -340,7 +340,8 @@ def run_serving(args):
         return reqs
 
     budget = 8 << 30
-    sv = Server(ctx, ds, hbm_budget=budget, slots=2)
+    max_batch = 1 << 22   # rays per launch: keeps two launches in flight so copies and tracing overlap
+    sv = Server(ctx, ds, hbm_budget=budget, slots=2, max_batch_rays=max_batch)
     warm = make_requests(8, 1)
     run_load(sv, warm, rate_hz=0.0)
     cap_reqs = make_requests(args.steps * 4, 2)
@@ -373,6 +374,7 @@ def run_serving(args):
         "dtype": "f32",
         "data": "synthetic (seeded displaced-grid scenes; camera and random rays; Poisson arrivals)",
         "config": {"workload": "serving", "scenes": len(items), "hbm_budget_bytes": budget, "slots": 2,
+                   "max_batch_rays": max_batch,
                    "request_rays": "log-uniform [1e3, 4e6]", "occlusion_fraction": 0.4},
         "loads": loads,
     }

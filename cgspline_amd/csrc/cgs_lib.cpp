@@ -1443,6 +1443,36 @@ struct PinnedBuffer
     }
 };
 
+// Host copies of one launch, cut into 4-MiB pieces and spread over worker
+// threads when there is more than a few MiB to move.
+struct CopyList
+{
+    struct Part
+    {
+        void* d;
+        const void* s;
+        size_t n;
+    };
+    std::vector<Part> parts;
+    size_t bytes = 0;
+    void add(void* d, const void* s, size_t n)
+    {
+        constexpr size_t kPiece = (size_t)4 << 20;
+        for (size_t o = 0; o < n; o += kPiece)
+            parts.push_back({(char*)d + o, (const char*)s + o, std::min(kPiece, n - o)});
+        bytes += n;
+    }
+    void run()
+    {
+        if (bytes < ((size_t)8 << 20))
+            for (const Part& p : parts) std::memcpy(p.d, p.s, p.n);
+        else
+            parallelFor(parts.size(), [&](size_t i) { std::memcpy(parts[i].d, parts[i].s, parts[i].n); });
+        parts.clear();
+        bytes = 0;
+    }
+};
+
 struct ServerSlot
 {
     hipStream_t stream = nullptr;
@@ -1485,21 +1515,26 @@ struct cgs_server
     // Copies a finished slot's results to the requests' host buffers.
     void complete(ServerSlot& sl, int ok)
     {
-        const Clock::time_point now = Clock::now();
         const uint8_t* base = (const uint8_t*)sl.hOut.p;
-        for (const ServerRequest& r : sl.reqs)
+        if (ok)
         {
-            if (ok)
+            CopyList copies;
+            for (const ServerRequest& r : sl.reqs)
             {
                 const uint8_t* o = base + r.rayOff * (sizeof(cgs_hit) + 4);
                 if (r.any)
-                    std::memcpy(r.occluded, o + (size_t)r.n * sizeof(cgs_hit), (size_t)r.n * 4);
+                    copies.add(r.occluded, o + (size_t)r.n * sizeof(cgs_hit), (size_t)r.n * 4);
                 else
                 {
-                    std::memcpy(r.hits, o, (size_t)r.n * sizeof(cgs_hit));
-                    if (r.objects) std::memcpy(r.objects, o + (size_t)r.n * sizeof(cgs_hit), (size_t)r.n * 4);
+                    copies.add(r.hits, o, (size_t)r.n * sizeof(cgs_hit));
+                    if (r.objects) copies.add(r.objects, o + (size_t)r.n * sizeof(cgs_hit), (size_t)r.n * 4);
                 }
             }
+            copies.run();
+        }
+        const Clock::time_point now = Clock::now();
+        for (const ServerRequest& r : sl.reqs)
+        {
             std::lock_guard lk(mu);
             done[r.id] = {(uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(now - r.submitted).count(), ok};
             inflight -= r.bytes;
@@ -1542,10 +1577,11 @@ struct cgs_server
         cgs_ray* hr = (cgs_ray*)sl.hRays.p;
         BatchJob* jt = (BatchJob*)sl.hTable.p;
         uint8_t* dOut = sl.out.as<uint8_t>();
+        CopyList copies;
         for (size_t k = 0; k < sl.reqs.size(); k++)
         {
             const ServerRequest& r = sl.reqs[k];
-            std::memcpy(hr + r.rayOff, r.rays, (size_t)r.n * sizeof(cgs_ray));
+            copies.add(hr + r.rayOff, r.rays, (size_t)r.n * sizeof(cgs_ray));
             BatchJob b {};
             b.sc = views[r.scene];
             b.rays = sl.rays.as<cgs_ray>() + r.rayOff;
@@ -1558,6 +1594,7 @@ struct cgs_server
             b.any = r.any ? 1u : 0u;
             jt[k] = b;
         }
+        copies.run();
         if ((e = hipMemcpyAsync(sl.rays.p, hr, total * sizeof(cgs_ray), hipMemcpyHostToDevice, sl.stream)) != hipSuccess)
             return e;
         if ((e = hipMemcpyAsync(sl.table.p, jt, sl.reqs.size() * sizeof(BatchJob), hipMemcpyHostToDevice,
