@@ -293,7 +293,8 @@ inline bool sceneIntersectAny(const Scene& sc, const Ray& ray0, Work* w)
 
 inline f3 splat(float s) { return {s, s, s}; }
 
-// PBR.cu:432-454
+// PBR.cu:21-42. A spot light (0.01 <= angle <= 1.57, Scene.h:73-76) never
+// passes: DL < 0 puts acos(DL) above pi/2, so angle > 2 acos(DL) needs angle > pi.
 inline bool lightVector(float& d, f3& L, f3 P, const Light& light)
 {
     if (light.isDirectional())
@@ -310,7 +311,7 @@ inline bool lightVector(float& d, f3& L, f3 P, const Light& light)
     float DL = light.direction.dot(L);
     return DL < 0 && light.angle > 2 * std::acos(DL);
 }
-// PBR.cu:456-476
+// PBR.cu:44-64
 inline f3 lightColor(float d, const Light& light)
 {
     if (light.isDirectional()) return light.color;
