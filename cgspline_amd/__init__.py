@@ -15,7 +15,31 @@ import numpy as np
 from . import _abi, scenes
 from ._abi import CgsError, EMPTY, PATCH_OVERFLOW, check, lib, ptr
 
-__all__ = ["Batch", "Context", "DeviceArray", "DeviceScene", "CgsError", "EMPTY", "PATCH_OVERFLOW", "scenes", "lib"]
+__all__ = ["Batch", "Context", "DeviceArray", "DeviceScene", "CgsError", "EMPTY", "PATCH_OVERFLOW", "load_be", "scenes",
+           "lib"]
+
+
+def load_be(source) -> tuple[np.ndarray, np.ndarray]:
+    """Surface::load_be (Surface.cpp:149-213) through cgs_load_be: `source`
+    is the file's text as bytes, or a path (str or os.PathLike).  Returns the
+    control points (npoints, 4) float32 with w = 1 and the zero-based patch
+    indices (npatches, 16) uint32, u fastest.  Malformed text raises CgsError
+    with the reference's message."""
+    if isinstance(source, (bytes, bytearray, memoryview)):
+        text = bytes(source)
+    else:
+        with open(source, "rb") as f:
+            text = f.read()
+    vp, ip = C.c_void_p(), C.c_void_p()
+    nv, npch = C.c_uint32(), C.c_uint32()
+    check(lib().cgs_load_be(text, len(text), C.byref(vp), C.byref(nv), C.byref(ip), C.byref(npch)), "cgs_load_be")
+    try:
+        v = np.ctypeslib.as_array(C.cast(vp, C.POINTER(C.c_float)), (max(nv.value, 1) * 4,))[: nv.value * 4].copy()
+        i = np.ctypeslib.as_array(C.cast(ip, C.POINTER(C.c_uint32)), (max(npch.value, 1) * 16,))[: npch.value * 16].copy()
+    finally:
+        lib().cgs_free_host(vp)
+        lib().cgs_free_host(ip)
+    return v.reshape(-1, 4), i.reshape(-1, 16)
 
 
 class DeviceArray:

@@ -8,10 +8,15 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <string>
+
 #include "../../include/cgs.h"
 
 namespace cgs
 {
+
+// Records `msg` as the calling thread's cgs_last_error() and returns `s`.
+cgs_status setError(cgs_status s, const std::string& msg);
 
 struct SceneDev;
 struct BatchJob;

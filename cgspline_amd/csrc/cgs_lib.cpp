@@ -43,6 +43,7 @@ static cgs_status fail(cgs_status s, const std::string& msg)
     g_lastError = msg;
     return s;
 }
+cgs_status cgs::setError(cgs_status s, const std::string& msg) { return fail(s, msg); }
 
 #define HIP_TRY(expr)                                                                        \
     do                                                                                       \

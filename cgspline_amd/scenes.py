@@ -16,6 +16,8 @@ Workloads:
 * ``displaced_grids`` — 4,096 displaced 16×16-patch grids (1,048,576 patches)
   with log-uniform scales 1e-3–1 in a 25³ box, 10% nearly-degenerate patches,
   2048² primary rays + shadow + one reflection bounce, GGX shading.
+* ``teaset``          — the reference's own `.be` models (teapot, teacup,
+  teaspoon on a `longwave` floor) arranged as in assets/scenes/Teaset01.dl.
 """
 from __future__ import annotations
 
@@ -360,6 +362,68 @@ def batched_scenes(seed=BASE_SEED + 4, count=64, patches=(1_000, 200_000), rays=
         out.append(BatchItem(wl.scene, np.concatenate([cam, rnd]).astype(np.float32),
                              bool(rng.random() < any_fraction)))
     return out
+
+
+def rotation_xyz(degrees) -> np.ndarray:
+    """R_y(y)·R_x(x)·R_z(z) for Euler angles in degrees (yaw, then pitch,
+    then roll, the usual camera order).  Single-axis rotations, the only kind
+    the teaset objects use, are the same under every order."""
+    x, y, z = (math.radians(a) for a in degrees)
+    cx, sx, cy, sy, cz, sz = math.cos(x), math.sin(x), math.cos(y), math.sin(y), math.cos(z), math.sin(z)
+    Rx = np.array([[1, 0, 0], [0, cx, -sx], [0, sx, cx]])
+    Ry = np.array([[cy, 0, sy], [0, 1, 0], [-sy, 0, cy]])
+    Rz = np.array([[cz, -sz, 0], [sz, cz, 0], [0, 0, 1]])
+    return Ry @ Rx @ Rz
+
+
+def euler_camera(position, rotation_deg, fovy_deg, width, height) -> Camera:
+    """Camera at `position` looking down its −Z axis turned by
+    :func:`rotation_xyz` (Y up)."""
+    m = np.eye(4)
+    m[:3, :3] = rotation_xyz(rotation_deg)
+    m[:3, 3] = position
+    return Camera(m.T.reshape(16).astype(np.float32), np.asarray(position, np.float32),
+                  math.radians(fovy_deg), int(width), int(height))
+
+
+def teaset(models_dir, width=320, height=240, depth=2, fovy_deg=60.0) -> Workload:
+    """assets/scenes/Teaset01.dl restated: a `longwave` floor scaled (10, 0.1,
+    10), the teapot turned −90° about X and halved, a teacup at (2.4, 0, 0),
+    a teaspoon at (3.7, 0.1, 0) turned −85° about X, one point light of
+    colour 90·(1, 0.95, 0.8) at (−10, 5, 5), the file's materials,
+    background and ambient, and its view (position (3, 2.9, 3.5), rotation
+    (−34°, 20°, −0.0012°)).  `models_dir` holds <name>.be for the four
+    models; the surfaces are read with cgs_load_be.  The view's Euler order
+    and the field of view are this module's choice (the scene file does not
+    fix them)."""
+    import os
+
+    from . import load_be
+    surf = {n: load_be(os.path.join(models_dir, n + ".be")) for n in ("longwave", "teapot", "teacup", "teaspoon")}
+    names = list(surf)
+    f0, silver = (0.04, 0.04, 0.04), (0.90, 0.91, 0.98)
+    ceramic = (0.82, 0.83, 0.84)
+    mat = {
+        "CeramicWhite": [*ceramic, *ceramic, 0.1, 0.9],
+        "Silver": [*f0, *silver, 0.9, 0.1],
+        "Floor": [*silver, *silver, 0.05, 0.9],
+    }
+    placed = [  # (model, position, rotation degrees, scale, material)
+        ("longwave", (0, 0, 0), (0, 0, 0), (10, 0.1, 10), "Floor"),
+        ("teapot", (0, 0, 0), (-90, 0, 0), (0.5, 0.5, 0.5), "CeramicWhite"),
+        ("teacup", (2.4, 0, 0), (0, 0, 0), (1, 1, 1), "CeramicWhite"),
+        ("teaspoon", (3.7, 0.1, 0), (-85, 0, 0), (1, 1, 1), "Silver"),
+    ]
+    objects = []
+    for model, pos, rot, scale, m in placed:
+        l2w, w2l = trs(pos, rotation_xyz(rot), np.asarray(scale, np.float64))
+        objects.append((names.index(model), l2w, w2l, np.array(mat[m], np.float32)))
+    light = point_light((-10.0, 5.0, 5.0), 90.0 * np.array([1.0, 0.95, 0.8]))
+    sc = assemble([surf[n] for n in names], objects, [light], name="teaset",
+                  background=(0.5, 0.5, 0.5), ambient=(0.025, 0.025, 0.025))
+    cam = euler_camera((3.0, 2.9, 3.5), (-34.0, 20.0, -0.0012), fovy_deg, width, height)
+    return Workload(sc, cam, depth=depth, desc="reference teaset models (78 patches), GGX, "
+                                               f"{depth} reflection bounces")
 
 
 def camera_rays(cam: Camera, rows=None) -> np.ndarray:

@@ -217,6 +217,17 @@ cgs_status cgs_patch_elements(const float* vertices, const uint32_t* indices, ui
                               cgs_bvh_element* elements);
 void cgs_free_host(void* p);
 
+/* Surface::load_be (Surface.cpp:149-213): parses the text of a `.be` model
+ * (the patch count; 16 one-based control-point indices per patch, u
+ * fastest; the point count; x, y, z per point; one separator character after
+ * each number).  The library allocates *vertices (npoints × 4 floats, w = 1)
+ * and *indices (npatches × 16, zero-based) with malloc; release them with
+ * cgs_free_host.  Malformed text fails with CGS_ERR_INVALID and the
+ * reference's messages; an index equal to the point count is rejected (the
+ * reference lets it through). */
+cgs_status cgs_load_be(const char* text, size_t length, float** vertices, uint32_t* npoints,
+                       uint32_t** indices, uint32_t* npatches);
+
 /* PerfectHashFunction::build (rt/PerfectHashFunction.h:53, .cu:302-576) on
  * the GPU, deterministic.  keys: n device u32; displacement: device u32
  * buffer of cgs_phf_displacement_size(n) entries (written); out: parameters.

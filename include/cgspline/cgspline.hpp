@@ -24,6 +24,9 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <istream>
+#include <iterator>
+#include <memory>
 #include <optional>
 #include <span>
 #include <stdexcept>
@@ -37,6 +40,46 @@ struct ihipStream_t;
 
 namespace cg
 {
+
+// cg::Surface (Surface.h:130-164) for one group of bicubic Bézier patches:
+// points (x, y, z, w) and 16 indices per patch, point(i, j) = indices[4j+i].
+// Ref<Surface> is a shared pointer here.
+class Surface
+{
+public:
+    Surface() = default;
+    Surface(std::vector<float> points4, std::vector<uint32_t> indices)
+        : _points(std::move(points4)), _indices(std::move(indices))
+    {}
+
+    uint32_t count() const { return (uint32_t)(_indices.size() / 16); }
+    std::span<const float> points() const { return _points; }   // 4 floats per point
+    std::span<const uint32_t> indices() const { return _indices; }
+
+    // Surface::load_be (Surface.cpp:149-213): reads a `.be` model from the
+    // stream; malformed text throws std::runtime_error with the reference's
+    // messages (an index equal to the point count included, which the
+    // reference lets through).
+    static std::shared_ptr<Surface> load_be(std::istream& input)
+    {
+        const std::string text {std::istreambuf_iterator<char>(input), std::istreambuf_iterator<char>()};
+        float* v = nullptr;
+        uint32_t* ix = nullptr;
+        uint32_t nv = 0, np = 0;
+        if (cgs_load_be(text.data(), text.size(), &v, &nv, &ix, &np) != CGS_OK)
+            throw std::runtime_error(cgs_last_error());
+        auto s = std::make_shared<Surface>(std::vector<float>(v, v + 4 * (size_t)nv),
+                                           std::vector<uint32_t>(ix, ix + 16 * (size_t)np));
+        cgs_free_host(v);
+        cgs_free_host(ix);
+        return s;
+    }
+
+private:
+    std::vector<float> _points;
+    std::vector<uint32_t> _indices;
+};
+
 namespace rt
 {
 
@@ -317,6 +360,7 @@ public:
         _indices.insert(_indices.end(), indices.begin(), indices.end());
         return (uint32_t)_vbase.size() - 2;
     }
+    uint32_t addSurface(const cg::Surface& surface) { return addSurface(surface.points(), surface.indices()); }
 
     void buildBVH(const Device& device)
     {

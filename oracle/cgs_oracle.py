@@ -8,6 +8,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import re
 import subprocess
 
 import numpy as np
@@ -353,3 +354,50 @@ class OracleScene:
         sec = lib().orc_cpu_render(self.h, C.byref(c), eps, depth, w, h, _ptr(color), _ptr(frame), th, r0, r1, stride,
                                    _ptr(rc))
         return sec, color, frame, int(rc[0]), int(rc[1])
+
+
+# ---------------------------------------------------------------------------
+# model files
+# ---------------------------------------------------------------------------
+
+_BE_INT = re.compile(rb"\s*\+?([0-9]+)")
+_BE_FLOAT = re.compile(rb"\s*([+-]?(?:[0-9]+\.?[0-9]*|\.[0-9]+)(?:[eE][+-]?[0-9]+)?)")
+
+
+class BeError(ValueError):
+    pass
+
+
+def load_be(text: bytes):
+    """Surface::load_be (Surface.cpp:149-213) restated in Python: each
+    `input >> x` skips whitespace and reads one number, each `input.ignore()`
+    drops the next character.  Returns (points (n,4) f32 with w = 1, indices
+    (p,16) u32 zero-based); raises BeError with the reference's messages,
+    and on an index equal to the point count (which the reference's
+    `points.size() < i` lets through).  Coordinates go decimal → float64 →
+    float32, which equals the direct rounding unless the decimal lies within
+    half a float64 ulp of a float32 midpoint."""
+    pos = 0
+
+    def num(rx, what):
+        nonlocal pos
+        m = rx.match(text, pos)
+        if not m:
+            raise BeError(what)
+        pos = m.end()
+        return m.group(1)
+
+    npatch = int(num(_BE_INT, "Could not read the no. patches"))
+    idx = np.zeros(16 * npatch, np.uint32)
+    for k in range(16 * npatch):
+        idx[k] = (int(num(_BE_INT, "Parsing patch failed")) - 1) & 0xFFFFFFFF
+        pos += 1
+    npts = int(num(_BE_INT, "Could not read the no. vertices"))
+    pts = np.ones((npts, 4), np.float32)
+    for i in range(npts):
+        for c in range(3):
+            pts[i, c] = np.float32(float(num(_BE_FLOAT, "Parsing point failed")))
+            pos += 1
+    if idx.size and int(idx.max()) >= npts:
+        raise BeError("Invalid/corrupted file. Indices out of range")
+    return pts, idx.reshape(-1, 16)

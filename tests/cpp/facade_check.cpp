@@ -6,11 +6,13 @@
 //   facade_check bvh    <in> <out>   BVH::build on the host (no device)
 //   facade_check hash   <in> <out>   PerfectHashFunction + BVH::buildHashTable
 //   facade_check render <in> <out>   Scene::buildBVH + RayTracer::render
+//   facade_check load_be <in> <out>  Surface::load_be on a `.be` model
 //   facade_check errors              error behaviour without a device
 #include <cstdio>
 #include <cstring>
 #include <fstream>
 #include <iostream>
+#include <sstream>
 #include <vector>
 
 #include "cgspline/cgspline.hpp"
@@ -160,8 +162,33 @@ int runRender(const char* inPath, const char* outPath)
     return 0;
 }
 
+int runLoadBe(const char* inPath, const char* outPath)
+{
+    std::ifstream in(inPath, std::ios::binary);
+    if (!in) throw std::runtime_error(std::string("cannot open ") + inPath);
+    const std::shared_ptr<Surface> s = Surface::load_be(in);
+    Writer w(outPath);
+    w.put((uint32_t)(s->points().size() / 4));
+    w.put(s->count());
+    w.vec(s->points());
+    w.vec(s->indices());
+    return 0;
+}
+
 int runErrors()
 {
+    // Surface::load_be on a truncated model: runtime_error with the
+    // reference's message
+    try
+    {
+        std::istringstream bad("3\n1,2");
+        Surface::load_be(bad);
+        return 10;
+    }
+    catch (const std::runtime_error& e)
+    {
+        if (std::string(e.what()) != "Parsing patch failed") return 11;
+    }
     // BVH::build over nothing: logic_error("invalid") like BVH::wrap
     try
     {
@@ -197,8 +224,9 @@ int main(int argc, char** argv)
         if (mode == "bvh" && argc == 4) return runBVH(argv[2], argv[3]);
         if (mode == "hash" && argc == 4) return runHash(argv[2], argv[3]);
         if (mode == "render" && argc == 4) return runRender(argv[2], argv[3]);
+        if (mode == "load_be" && argc == 4) return runLoadBe(argv[2], argv[3]);
         if (mode == "errors") return runErrors();
-        std::fprintf(stderr, "usage: facade_check bvh|hash|render <in> <out> | errors\n");
+        std::fprintf(stderr, "usage: facade_check bvh|hash|render|load_be <in> <out> | errors\n");
         return 2;
     }
     catch (const std::exception& e)

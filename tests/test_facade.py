@@ -49,6 +49,22 @@ def test_facade_bvh_build_equals_oracle(tmp_path, oracle, c1):
     assert np.array_equal(nodes, n2) and np.array_equal(keys, k2) and np.array_equal(idx, i2)
 
 
+@pytest.mark.parametrize("name", ["teapot", "teaspoon"])
+def test_facade_load_be_equals_oracle(tmp_path, oracle, name):
+    """Surface::load_be through the facade on the reference's models."""
+    src = os.path.join(conftest.ROOT, "tests", "golden", "models", name + ".be")
+    rc, out = run("load_be", src, tmp_path / "s.bin")
+    assert rc == 0, out
+    raw = (tmp_path / "s.bin").read_bytes()
+    nv, npch = np.frombuffer(raw[:8], np.uint32)
+    v = np.frombuffer(raw[8:8 + 16 * nv], np.float32).reshape(-1, 4)
+    ix = np.frombuffer(raw[8 + 16 * nv:], np.uint32).reshape(-1, 16)
+    with open(src, "rb") as f:
+        ov, oi = oracle.load_be(f.read())
+    assert ix.shape[0] == npch
+    assert np.array_equal(v.view(np.uint32), ov.view(np.uint32)) and np.array_equal(ix, oi)
+
+
 @pytest.mark.skipif(os.path.exists("/dev/kfd") and os.access("/dev/kfd", os.R_OK), reason="a device is present")
 def test_facade_error_behaviour_without_device():
     rc, out = run("errors")

@@ -1,0 +1,79 @@
+"""GPU parity on the reference's own models (tests/golden/models: teapot,
+teacup, teaspoon, longwave, read with cgs_load_be) in the Teaset01.dl
+arrangement: closest hits, shadow rays and the rendered frame against the
+oracle, bit for bit (±1 in an 8-bit frame channel, from powf in the sRGB
+encode).  The teapot's lid and bottom patches collapse a row of control
+points to a point; the spoon is a thin, strongly curved surface."""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+import pytest
+
+from conftest import say
+from test_gpu_parity import assert_bits_equal
+
+pytestmark = pytest.mark.gpu
+
+MODELS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "models")
+
+
+@pytest.fixture(scope="module")
+def teaset():
+    from cgspline_amd import scenes
+    return scenes.teaset(MODELS, width=320, height=240, depth=2)
+
+
+def test_teaset_bvh_and_hash_match_oracle(ctx, oracle, teaset):
+    sc = teaset.scene
+    say("teaset scene create", sc.npatch, "patches")
+    ds = ctx.scene(sc)
+    osc = oracle.OracleScene(sc)
+    for s in [0, 1, 2, 3, -1]:
+        g, o = ds.bvh(s), osc.bvh(s)
+        for key in ("nodes", "keys", "indices", "displacement", "table"):
+            assert np.array_equal(g[key], o[key]), (s, key)
+
+
+def test_teaset_trace_closest_and_any_bitexact(ctx, oracle, teaset):
+    from cgspline_amd import scenes
+    sc = teaset.scene
+    rays = scenes.camera_rays(teaset.camera)
+    say("teaset trace_closest", rays.shape[0], "rays")
+    ds = ctx.scene(sc)
+    tuv, patch, obj = ds.trace_closest(rays)
+    osc = oracle.OracleScene(sc)
+    otuv, opatch, oobj, _ = osc.trace(rays)
+    assert np.array_equal(obj, oobj) and np.array_equal(patch, opatch)
+    assert_bits_equal(tuv, otuv, "t,u,v")
+    assert set(np.unique(obj)) == {-1, 0, 1, 2, 3}
+    # shadow rays toward the light from every hit
+    hit = obj >= 0
+    P = rays[hit, 0:3] + tuv[hit, 0:1] * rays[hit, 4:7]
+    L = sc.lights[0, 4:7][None, :] - P
+    d = np.linalg.norm(L, axis=1).astype(np.float32)
+    L = (L / d[:, None]).astype(np.float32)
+    sh = np.zeros((P.shape[0], 8), np.float32)
+    sh[:, 0:3], sh[:, 3], sh[:, 4:7] = P + 1e-4 * L, d, L
+    say("teaset trace_any", sh.shape[0], "shadow rays")
+    g = ds.trace_any(sh)
+    o, _ = osc.trace(sh, any_hit=True)
+    assert not np.any(g == 2)
+    assert np.array_equal(g == 1, o)
+    assert 0 < o.sum() < o.size
+
+
+def test_teaset_render_matches_oracle(ctx, oracle, teaset):
+    wl = teaset
+    say("teaset render", wl.camera.width, wl.camera.height, "depth", wl.depth)
+    ds = ctx.scene(wl.scene)
+    color, frame, counts = ds.render(wl.camera, eps=wl.eps, depth=wl.depth)
+    ocolor, oframe, _, orc = oracle.OracleScene(wl.scene).render(wl.camera, eps=wl.eps, depth=wl.depth)
+    assert counts["overflow"] == 0
+    assert (counts["primary"], counts["shadow"], counts["secondary"]) == \
+        (orc["primary"], orc["shadow"], orc["secondary"])
+    assert orc["secondary"] > 0
+    assert_bits_equal(color.reshape(-1, 3), ocolor.reshape(-1, 3), "color")
+    ch = lambda f: np.stack([(f >> s) & 0xFF for s in (0, 8, 16, 24)], -1).astype(np.int32)
+    assert np.abs(ch(frame) - ch(oframe)).max() <= 1
