@@ -386,20 +386,21 @@ def euler_camera(position, rotation_deg, fovy_deg, width, height) -> Camera:
                   math.radians(fovy_deg), int(width), int(height))
 
 
-def teaset(models_dir, width=320, height=240, depth=2, fovy_deg=60.0) -> Workload:
+def teaset(assets_dir, width=320, height=240, depth=2, fovy_deg=60.0) -> Workload:
     """assets/scenes/Teaset01.dl restated: a `longwave` floor scaled (10, 0.1,
     10), the teapot turned −90° about X and halved, a teacup at (2.4, 0, 0),
     a teaspoon at (3.7, 0.1, 0) turned −85° about X, one point light of
     colour 90·(1, 0.95, 0.8) at (−10, 5, 5), the file's materials,
     background and ambient, and its view (position (3, 2.9, 3.5), rotation
-    (−34°, 20°, −0.0012°)).  `models_dir` holds <name>.be for the four
-    models; the surfaces are read with cgs_load_be.  The view's Euler order
+    (−34°, 20°, −0.0012°)).  `assets_dir` is laid out like the reference's
+    assets/ folder (bezier/teapot, ...); the surfaces are read with
+    cgs_load_be.  The view's Euler order
     and the field of view are this module's choice (the scene file does not
     fix them)."""
     import os
 
     from . import load_be
-    surf = {n: load_be(os.path.join(models_dir, n + ".be")) for n in ("longwave", "teapot", "teacup", "teaspoon")}
+    surf = {n: load_be(os.path.join(assets_dir, "bezier", n)) for n in ("longwave", "teapot", "teacup", "teaspoon")}
     names = list(surf)
     f0, silver = (0.04, 0.04, 0.04), (0.90, 0.91, 0.98)
     ceramic = (0.82, 0.83, 0.84)

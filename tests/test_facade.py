@@ -52,7 +52,7 @@ def test_facade_bvh_build_equals_oracle(tmp_path, oracle, c1):
 @pytest.mark.parametrize("name", ["teapot", "teaspoon"])
 def test_facade_load_be_equals_oracle(tmp_path, oracle, name):
     """Surface::load_be through the facade on the reference's models."""
-    src = os.path.join(conftest.ROOT, "tests", "golden", "models", name + ".be")
+    src = os.path.join(conftest.ROOT, "tests", "golden", "assets", "bezier", name)
     rc, out = run("load_be", src, tmp_path / "s.bin")
     assert rc == 0, out
     raw = (tmp_path / "s.bin").read_bytes()

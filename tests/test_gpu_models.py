@@ -1,4 +1,4 @@
-"""GPU parity on the reference's own models (tests/golden/models: teapot,
+"""GPU parity on the reference's own models (tests/golden/assets: teapot,
 teacup, teaspoon, longwave, read with cgs_load_be) in the Teaset01.dl
 arrangement: closest hits, shadow rays and the rendered frame against the
 oracle, bit for bit (±1 in an 8-bit frame channel, from powf in the sRGB
@@ -16,13 +16,13 @@ from test_gpu_parity import assert_bits_equal
 
 pytestmark = pytest.mark.gpu
 
-MODELS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "models")
+ASSETS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "assets")
 
 
 @pytest.fixture(scope="module")
 def teaset():
     from cgspline_amd import scenes
-    return scenes.teaset(MODELS, width=320, height=240, depth=2)
+    return scenes.teaset(ASSETS, width=320, height=240, depth=2)
 
 
 def test_teaset_bvh_and_hash_match_oracle(ctx, oracle, teaset):

@@ -1,6 +1,6 @@
 """The `.be` model reader (Surface::load_be, Surface.cpp:149-213) on the CPU:
 cgs_load_be against the oracle's restatement on the reference's four models
-(tests/golden/models, copied by tests/golden/make_models.py), the reference's
+(tests/golden/assets/bezier, copied by tests/golden/make_assets.py), the reference's
 error messages, separators, and the indices pinned by the teapot's
 rotational symmetry."""
 from __future__ import annotations
@@ -12,33 +12,37 @@ import os
 import numpy as np
 import pytest
 
-MODELS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "models")
+ASSETS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "assets")
 NAMES = ("teapot", "teacup", "teaspoon", "longwave")
 
 
 def meta():
-    with open(os.path.join(MODELS, "models.json")) as f:
+    with open(os.path.join(ASSETS, "assets.json")) as f:
         return json.load(f)
 
 
+def path(name) -> str:
+    return os.path.join(ASSETS, "bezier", name)
+
+
 def read(name) -> bytes:
-    with open(os.path.join(MODELS, name + ".be"), "rb") as f:
+    with open(path(name), "rb") as f:
         return f.read()
 
 
-@pytest.mark.parametrize("name", NAMES)
-def test_fixture_intact(name):
-    m = meta()[name]
-    data = read(name)
-    assert len(data) == m["bytes"] and hashlib.sha256(data).hexdigest() == m["sha256"]
+def test_fixtures_intact():
+    for rel, m in meta().items():
+        with open(os.path.join(ASSETS, rel), "rb") as f:
+            data = f.read()
+        assert len(data) == m["bytes"] and hashlib.sha256(data).hexdigest() == m["sha256"], rel
 
 
 @pytest.mark.parametrize("name", NAMES)
 def test_load_be_matches_oracle(oracle, name):
     import cgspline_amd as cgs
-    v, i = cgs.load_be(os.path.join(MODELS, name + ".be"))
+    v, i = cgs.load_be(path(name))
     ov, oi = oracle.load_be(read(name))
-    m = meta()[name]
+    m = meta()["bezier/" + name]
     assert v.shape == (m["points"], 4) and i.shape == (m["patches"], 16)
     assert np.array_equal(v.view(np.uint32), ov.view(np.uint32))
     assert np.array_equal(i, oi)
@@ -120,7 +124,7 @@ def test_teaset_scene_builds(oracle):
     """The teaset workload (Teaset01.dl restated) over the four models: the
     camera sees the floor and all three objects."""
     from cgspline_amd import scenes
-    wl = scenes.teaset(MODELS, width=96, height=72)
+    wl = scenes.teaset(ASSETS, width=96, height=72)
     assert wl.scene.npatch == 32 + 26 + 16 + 4
     _, _, obj, _ = oracle.OracleScene(wl.scene).trace(scenes.camera_rays(wl.camera))
     assert set(np.unique(obj)) == {-1, 0, 1, 2, 3}
