@@ -15,8 +15,8 @@ import numpy as np
 from . import _abi, scenes
 from ._abi import CgsError, EMPTY, PATCH_OVERFLOW, check, lib, ptr
 
-__all__ = ["Batch", "Context", "DeviceArray", "DeviceScene", "CgsError", "EMPTY", "PATCH_OVERFLOW", "load_be", "scenes",
-           "lib"]
+__all__ = ["Batch", "Context", "DeviceArray", "DeviceScene", "CgsError", "DlScene", "EMPTY", "PATCH_OVERFLOW", "load_be",
+           "read_dl", "scenes", "lib"]
 
 
 def load_be(source) -> tuple[np.ndarray, np.ndarray]:
@@ -40,6 +40,63 @@ def load_be(source) -> tuple[np.ndarray, np.ndarray]:
         lib().cgs_free_host(vp)
         lib().cgs_free_host(ip)
     return v.reshape(-1, 4), i.reshape(-1, 16)
+
+
+@dataclasses.dataclass
+class DlScene:
+    """A scene read from a `.dl` file (cgs_dl_read): the scene, the file's
+    view (camera→world, column-major, and position) and the names of its
+    objects and lights."""
+    scene: "scenes.Scene"
+    view_to_world: np.ndarray
+    view_position: np.ndarray
+    object_names: list
+    light_names: list
+
+
+def _copy(p, ctype, n, shape):
+    if n == 0:
+        return np.zeros(tuple(0 if k < 0 else k for k in shape), np.dtype(ctype))
+    return np.ctypeslib.as_array(C.cast(p, C.POINTER(ctype)), (n,)).copy().reshape(shape)
+
+
+def read_dl(source, name: str = "<text>", working_dir=None) -> DlScene:
+    """The reference's scene description language (reader/SceneReader.cpp)
+    through cgs_dl_read: `source` is a path (str or os.PathLike; model paths
+    resolve against its directory) or the text as bytes (model paths resolve
+    against `working_dir`).  Errors raise CgsError with "file:line:col:
+    message"."""
+    out = C.POINTER(_abi.DlScene)()
+    if isinstance(source, (bytes, bytearray, memoryview)):
+        text = bytes(source)
+        wd = None if working_dir is None else str(working_dir).encode()
+        check(lib().cgs_dl_parse(text, len(text), name.encode(), wd, C.byref(out)), "cgs_dl_parse")
+    else:
+        import os
+        check(lib().cgs_dl_read(os.fsencode(source), C.byref(out)), "cgs_dl_read")
+    try:
+        r = out.contents
+        d = r.scene
+        ns = d.nsurf
+        vb = _copy(d.surf_vbase, C.c_uint32, ns + 1, (ns + 1,))
+        pb = _copy(d.surf_pbase, C.c_uint32, ns + 1, (ns + 1,))
+        sc = scenes.Scene(
+            vertices=_copy(d.vertices, C.c_float, int(vb[-1]) * 4, (-1, 4)),
+            indices=_copy(d.indices, C.c_uint32, int(pb[-1]) * 16, (-1, 16)),
+            surf_vbase=vb, surf_pbase=pb,
+            obj_surface=_copy(d.obj_surface, C.c_uint32, d.nobj, (d.nobj,)),
+            obj_local2world=_copy(d.obj_local2world, C.c_float, d.nobj * 16, (-1, 16)),
+            obj_world2local=_copy(d.obj_world2local, C.c_float, d.nobj * 16, (-1, 16)),
+            obj_material=_copy(d.obj_material, C.c_float, d.nobj * 8, (-1, 8)),
+            lights=_copy(d.lights, C.c_float, d.nlight * 12, (-1, 12)),
+            background=tuple(float(x) for x in d.background), ambient=tuple(float(x) for x in d.ambient),
+            hash_seed=int(d.hash_seed), name=str(name if isinstance(source, (bytes, bytearray, memoryview)) else source))
+        res = DlScene(sc, np.array(r.view_to_world[:], np.float32), np.array(r.view_position[:], np.float32),
+                      [r.object_names[k].decode() for k in range(d.nobj)],
+                      [r.light_names[k].decode() for k in range(d.nlight)])
+    finally:
+        lib().cgs_dl_free(out)
+    return res
 
 
 class DeviceArray:

@@ -31,6 +31,7 @@ PATCH_OVERFLOW = 0xFFFFFFFE
 EXPORTS = [
     "cgs_abi_version", "cgs_status_string", "cgs_last_error", "cgs_open", "cgs_close",
     "cgs_bezier_clip", "cgs_subdivision", "cgs_bvh_build", "cgs_patch_elements", "cgs_free_host", "cgs_load_be",
+    "cgs_dl_read", "cgs_dl_parse", "cgs_dl_free",
     "cgs_phf_displacement_size", "cgs_phf_build", "cgs_phf_fill_table", "cgs_phf_hash",
     "cgs_scene_create", "cgs_scene_destroy", "cgs_scene_bvh_info", "cgs_scene_bvh_copy",
     "cgs_trace_closest", "cgs_trace_any", "cgs_render", "cgs_render_running",
@@ -75,6 +76,12 @@ class SceneDesc(C.Structure):
         ("obj_surface", _P), ("obj_local2world", _P), ("obj_world2local", _P), ("obj_material", _P),
         ("nobj", _U32), ("lights", _P), ("nlight", _U32),
         ("background", _F32 * 3), ("ambient", _F32 * 3), ("hash_seed", _U32)]
+
+
+class DlScene(C.Structure):
+    _fields_ = [("scene", SceneDesc), ("view_to_world", _F32 * 16), ("view_position", _F32 * 3),
+                ("object_names", C.POINTER(C.c_char_p)), ("light_names", C.POINTER(C.c_char_p)),
+                ("storage", _P)]
 
 
 class BatchJob(C.Structure):
@@ -126,6 +133,10 @@ def lib():
             "cgs_free_host": ([_P], None),
             "cgs_load_be": ([C.c_char_p, C.c_size_t, C.POINTER(_P), C.POINTER(_U32), C.POINTER(_P), C.POINTER(_U32)],
                             C.c_int),
+            "cgs_dl_read": ([C.c_char_p, C.POINTER(C.POINTER(DlScene))], C.c_int),
+            "cgs_dl_parse": ([C.c_char_p, C.c_size_t, C.c_char_p, C.c_char_p, C.POINTER(C.POINTER(DlScene))],
+                             C.c_int),
+            "cgs_dl_free": ([C.POINTER(DlScene)], None),
             "cgs_phf_displacement_size": ([_U32], _U32),
             "cgs_phf_build": ([_P, _P, _U32, _U32, _P, C.POINTER(PHF), _P], C.c_int),
             "cgs_phf_fill_table": ([_P, C.POINTER(PHF), _P, _P, _U32, _P, _P], C.c_int),

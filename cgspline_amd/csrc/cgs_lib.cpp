@@ -1073,7 +1073,6 @@ cgs_status cgs_render(cgs_ctx* ctx, cgs_scene* sc, const cgs_camera* cam, const 
         hdy = -hdy;
     }
     hipStream_t s = (hipStream_t)stream;
-    if (sc->nlight > 32) return fail(CGS_ERR_INVALID, "cgs_render: at most 32 lights");
     if (!ctx->staged)
     {
         PersistArgs pa;
@@ -1085,6 +1084,10 @@ cgs_status cgs_render(cgs_ctx* ctx, cgs_scene* sc, const cgs_camera* cam, const 
         ctx->renderLaunched = true;
         return CGS_OK;
     }
+    // The staged tracer keeps a pixel's pending lights in bits 0-30 of one
+    // word (bit 31 marks a hit) and its occluded lights in one word.
+    if (sc->nlight > 31)
+        return fail(CGS_ERR_INVALID, "cgs_render: the staged tracer (CGS_STAGED=1) handles at most 31 lights");
     const size_t npix = (size_t)w * h;
     const uint32_t nl = std::max<uint32_t>(1, sc->nlight);
     cgs_status st;

@@ -427,6 +427,17 @@ def teaset(assets_dir, width=320, height=240, depth=2, fovy_deg=60.0) -> Workloa
                                                f"{depth} reflection bounces")
 
 
+def from_dl(path, width=320, height=240, depth=2, fovy_deg=60.0) -> Workload:
+    """A workload from one of the reference's `.dl` scene files (read with
+    cgs_dl_read) seen from the file's view.  The field of view is not part of
+    the file; 60° is this module's choice."""
+    from . import read_dl
+    r = read_dl(path)
+    cam = Camera(r.view_to_world, r.view_position, math.radians(fovy_deg), int(width), int(height))
+    return Workload(r.scene, cam, depth=depth, desc=f"{r.scene.name}: {r.scene.nobj} objects, "
+                                                    f"{r.scene.npatch} patches, {r.scene.lights.shape[0]} lights")
+
+
 def camera_rays(cam: Camera, rows=None) -> np.ndarray:
     """World rays (n, 8) float32 (o xyz, tmax, d xyz, 0) through pixel centres,
     in float32 with the reference's camera formula (RayTracer.cu:133-145,

@@ -228,6 +228,34 @@ void cgs_free_host(void* p);
 cgs_status cgs_load_be(const char* text, size_t length, float** vertices, uint32_t* npoints,
                        uint32_t** indices, uint32_t* npatches);
 
+/* A scene read from the reference's scene description language (`.dl`;
+ * reader/DescriptionLanguage.cpp, reader/SceneReader.cpp), flattened the way
+ * MainWindow::convertScene fills rt::Scene (MainWindow.cpp:363-492): one
+ * object per surface-carrying scene object, depth first, with its world
+ * transform and material, one light per light-carrying object.  Surfaces
+ * are read with cgs_load_be; objects naming the same Surface asset share it.
+ * Triangle meshes, textures and `.se` surfaces are refused (CGS_ERR_INVALID
+ * with the reason).  Euler angles are applied yaw (y), pitch (x), roll (z). */
+typedef struct cgs_dl_scene
+{
+    cgs_scene_desc scene;            /* ready for cgs_scene_create; arrays owned by this record */
+    float view_to_world[16];         /* the file's view: camera→world, column-major */
+    float view_position[3];
+    const char* const* object_names; /* scene.nobj names, in object order */
+    const char* const* light_names;  /* scene.nlight names */
+    void* storage;                   /* owned by the library */
+} cgs_dl_scene;
+
+/* Reads the `.dl` file at `path`; model paths in it are relative to its
+ * directory.  Release the result with cgs_dl_free.  Errors carry
+ * "file:line:column: message". */
+cgs_status cgs_dl_read(const char* path, cgs_dl_scene** out);
+/* The same from text; `name` labels error messages, `working_dir` (NULL:
+ * the current directory) resolves model paths. */
+cgs_status cgs_dl_parse(const char* text, size_t length, const char* name, const char* working_dir,
+                        cgs_dl_scene** out);
+void cgs_dl_free(cgs_dl_scene* scene);
+
 /* PerfectHashFunction::build (rt/PerfectHashFunction.h:53, .cu:302-576) on
  * the GPU, deterministic.  keys: n device u32; displacement: device u32
  * buffer of cgs_phf_displacement_size(n) entries (written); out: parameters.

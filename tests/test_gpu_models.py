@@ -77,3 +77,23 @@ def test_teaset_render_matches_oracle(ctx, oracle, teaset):
     assert_bits_equal(color.reshape(-1, 3), ocolor.reshape(-1, 3), "color")
     ch = lambda f: np.stack([(f >> s) & 0xFF for s in (0, 8, 16, 24)], -1).astype(np.int32)
     assert np.abs(ch(frame) - ch(oframe)).max() <= 1
+
+
+@pytest.mark.parametrize("name", ["Teaset02", "Test32", "Test152"])
+def test_reference_scene_file_render_matches_oracle(ctx, oracle, name):
+    """The reference's own scene files (read with cgs_dl_read): Test152 is
+    152 objects, 51 point lights; the frame, colours and ray counts equal
+    the oracle's."""
+    from cgspline_amd import scenes
+    wl = scenes.from_dl(os.path.join(ASSETS, "scenes", name + ".dl"), width=160, height=120, depth=2)
+    say("scene file render", name, wl.scene.nobj, "objects", wl.scene.lights.shape[0], "lights")
+    ds = ctx.scene(wl.scene)
+    color, frame, counts = ds.render(wl.camera, eps=wl.eps, depth=wl.depth)
+    ocolor, oframe, _, orc = oracle.OracleScene(wl.scene).render(wl.camera, eps=wl.eps, depth=wl.depth)
+    assert counts["overflow"] == 0
+    assert (counts["primary"], counts["shadow"], counts["secondary"]) == \
+        (orc["primary"], orc["shadow"], orc["secondary"])
+    assert orc["shadow"] > 0 and orc["secondary"] > 0
+    assert_bits_equal(color.reshape(-1, 3), ocolor.reshape(-1, 3), "color")
+    ch = lambda f: np.stack([(f >> s) & 0xFF for s in (0, 8, 16, 24)], -1).astype(np.int32)
+    assert np.abs(ch(frame) - ch(oframe)).max() <= 1

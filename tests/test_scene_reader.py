@@ -1,0 +1,177 @@
+"""The `.dl` scene reader (cgs_dl_read / cgs_dl_parse; reader/Parser.cpp,
+reader/DescriptionLanguage.cpp, reader/SceneReader.cpp, flattened like
+MainWindow::convertScene, MainWindow.cpp:363-492) on the CPU.
+
+Pins: the reference's nine scene files read with the object and light counts
+their text declares; Teaset01.dl equals the scene restated by hand in
+``scenes.teaset``; the operators, property access, comments and the
+hierarchy's transforms on small texts with results worked out by hand; the
+reference's error messages."""
+from __future__ import annotations
+
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+ASSETS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "assets")
+SCENES = ["Teaset01", "Teaset02", "teaset", "Test32", "Test56", "Test80", "Test104", "Test128", "Test152"]
+BEZIER = os.path.join(ASSETS, "bezier")
+
+
+def strip_comments(text: str) -> str:
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return re.sub(r"//[^\n]*", "", text)
+
+
+def parse(text: str):
+    import cgspline_amd as cgs
+    return cgs.read_dl(text.encode(), name="t.dl", working_dir=BEZIER)
+
+
+def one_object(extra_scene="", obj="", before=""):
+    return parse(f"""{before}
+Surface {{ name:"w" file:"longwave" }}
+Scene {{ {extra_scene} objects:[ {{ name:"o" surface:{{ model:Get("w") }} {obj} }} ] }}""")
+
+
+@pytest.mark.parametrize("name", SCENES)
+def test_reference_scene_files(name):
+    import cgspline_amd as cgs
+    path = os.path.join(ASSETS, "scenes", name + ".dl")
+    r = cgs.read_dl(path)
+    text = strip_comments(open(path).read())
+    assert r.scene.nobj == len(re.findall(r"\bsurface\s*:", text))
+    assert r.scene.lights.shape[0] == len(re.findall(r"\blight\s*:", text))
+    assert r.scene.nsurf == 4 and r.scene.npatch == 32 + 26 + 16 + 4
+    assert len(r.object_names) == r.scene.nobj and len(r.light_names) == r.scene.lights.shape[0]
+    # every world2local inverts its local2world
+    for a, b in zip(r.scene.obj_local2world, r.scene.obj_world2local):
+        A, B = a.reshape(4, 4).T.astype(np.float64), b.reshape(4, 4).T.astype(np.float64)
+        assert np.allclose(A @ B, np.eye(4), atol=1e-5)
+
+
+def test_teaset01_equals_hand_restatement():
+    """Teaset01.dl read by the library = the same scene built by hand from
+    the file (scenes.teaset): geometry, materials and lights exactly,
+    transforms to float rounding."""
+    import cgspline_amd as cgs
+    from cgspline_amd import scenes
+    r = cgs.read_dl(os.path.join(ASSETS, "scenes", "Teaset01.dl"))
+    h = scenes.teaset(ASSETS).scene
+    a = r.scene
+    for k in ("vertices", "indices", "surf_vbase", "surf_pbase", "obj_surface", "obj_material", "lights"):
+        assert np.array_equal(getattr(a, k), getattr(h, k)), k
+    f32 = lambda t: np.float32(t)   # both reach the device as float32
+    assert np.array_equal(f32(a.background), f32(h.background)) and np.array_equal(f32(a.ambient), f32(h.ambient))
+    assert np.allclose(a.obj_local2world, h.obj_local2world, rtol=1e-6, atol=1e-7)
+    assert np.allclose(a.obj_world2local, h.obj_world2local, rtol=1e-6, atol=1e-6)
+    cam = scenes.euler_camera((3.0, 2.9, 3.5), (-34.0, 20.0, -0.0012), 60.0, 8, 8)
+    assert np.allclose(r.view_to_world, cam.to_world, atol=1e-7)
+    assert r.object_names == ["Floor", "Teapot", "Teacup", "Teaspoon"]
+
+
+def test_operators():
+    """+ - * / over int, float and vectors (DescriptionLanguage.cpp:27-235),
+    read back through material and environment values."""
+    r = parse("""
+Surface { name:"w" file:"longwave" }
+let m = Material { name:"m" diffuse: *(0.5 (1 2 3)) specular: /((2 4 6) 2) metalness: /(1 2)
+                   roughness: /(1 4.0) }
+let n = Material { name:"n" diffuse: +((1 2 3) (0.5 0.5 0.5) (1 1 1)) specular: -((1 2 3) (1 1 1))
+                   metalness: -(1.5 1) roughness: *(2 0.125) }
+Scene {
+  environment: { background: /(2 (1 2 4)) ambient: -((1 2 3)) }
+  objects:[ { surface:{ model:Get("w") material:Get("m") } } { surface:{ model:Get("w") material:n } } ]
+}""")
+    M = r.scene.obj_material
+    assert np.array_equal(M[0], np.float32([0.5, 1, 1.5, 1, 2, 3, 0, 0.25]))      # 1/2 is integer division
+    assert np.array_equal(M[1], np.float32([2.5, 3.5, 4.5, 0, 1, 2, 0.5, 0.25]))
+    # a scalar over a vector scales the vector by the scalar's reciprocal
+    assert r.scene.background == (0.5, 1.0, 2.0)
+    assert r.scene.ambient == (-1.0, -2.0, -3.0)
+
+
+def test_hierarchy_transforms_and_order():
+    """World = parent · T · R · S; objects and lights depth first, an object
+    before its children; unnamed objects are "Object N"."""
+    r = parse("""
+Surface { name:"w" file:"longwave" }
+let V = { top: { position:(1 2 3) rotation:(0 90 0) } }
+Scene {
+  view: V.top   // property access on a let-bound dictionary
+  objects: [
+    { name:"a" transform:{ position:(1 0 0) } surface:{ model:Get("w") }
+      objects: [
+        { transform:{ scale:(2 2 2) } light:{ color:(1 1 1) }
+          objects: [ { name:"c" transform:{ rotation:(0 90 0) } surface:{ model:Assets.w } } ] }
+      ] }
+    /* a block comment */
+    { name:"d" surface:{ model:Get("w") } light:{ type:"directional" } }
+  ]
+}""")
+    assert r.object_names == ["a", "c", "d"]
+    assert r.light_names == ["Object 1", "d"]
+    W = [m.reshape(4, 4).T for m in r.scene.obj_local2world]
+    assert np.allclose(W[0], [[1, 0, 0, 1], [0, 1, 0, 0], [0, 0, 1, 0], [0, 0, 0, 1]], atol=0)
+    # c: T(1,0,0) · S(2) · Ry(90°): x → -z, z → x, doubled
+    assert np.allclose(W[1], [[0, 0, 2, 1], [0, 2, 0, 0], [-2, 0, 0, 0], [0, 0, 0, 1]], atol=1e-6)
+    assert np.array_equal(W[2], np.eye(4))
+    L = r.scene.lights
+    assert np.array_equal(L[0, 4:7], [1, 0, 0]) and L[0, 11] == np.float32(3.1415)   # point light at its origin
+    assert L[1, 11] == 0.0 and np.array_equal(L[1, 8:11], [0, 0, -1])              # directional
+    v = r.view_to_world.reshape(4, 4).T
+    assert np.allclose(v[:3, 3], [1, 2, 3]) and np.allclose(v[:3, :3], [[0, 0, 1], [0, 1, 0], [-1, 0, 0]], atol=1e-7)
+
+
+def test_lights():
+    r = one_object(obj="""light:{ color:(1 2 3 4) range:5 }
+      objects:[ { name:"s" light:{ type:"spot" angle:30 } transform:{ position:(0 1 0) } } ]""")
+    L = r.scene.lights
+    assert np.array_equal(L[0, :4], [1, 2, 3, 1]) and L[0, 7] == 5
+    assert math.isclose(L[1, 11], math.radians(30), rel_tol=1e-6) and np.array_equal(L[1, 4:7], [0, 1, 0])
+
+
+@pytest.mark.parametrize("text,message", [
+    ('Scene { objects:[ { surface:{ model:Get("w") } } ] }', 'property "w" does not exist'),
+    ('let a = 1\nlet a = 2\nScene {}', 'identifier "a" already defined'),
+    ('Scene { view:{} view:{} }', 'property "view" already defined'),
+    ('Scene { objects:[ Foo ] }', 'property "Foo" undefined'),
+    ('Material { metalness:1.5 }', "'metalness' should be between 0 and 1"),
+    ('Material { roughness:-0.5 }', "'roughness' should be between 0 and 1"),
+    ('Material { refractive_index:0 }', "'refractive_index' (index of refraction) should be greater than zero"),
+    ('Surface { name:"x" }', "surface data or source not provided"),
+    ('Surface { file:"nope" }', "file 'nope' does not exist"),
+    ('Surface { file:"x.se" }', "`.se` surfaces are not supported"),
+    ('Mesh { name:"m" }', "triangle meshes are not supported"),
+    ('From_sRGB(135 206 235)', "From_sRGB function: expected 3 or 4 integer arguments"),
+    ('let a = *((1 2 3) (1 2 3))', "operator `*` cannot be called with argument types `vec3` and `vec3`"),
+    ('let a = +(1 (1 2))', "operator `+` cannot be called with argument types `int` and `vec2`"),
+    ('let a = "abc', "missing string literal ending"),
+    ('/* never closed', "unexpected end-of-file"),
+    ('let a = (1)', "a number was expected"),
+    ('let a = 1', "the file defines no Scene"),
+    ('Scene { objects:[ { light:{ type:"area" } } ] }', "Light: type must be one of"),
+    ('Scene { objects:[ { light:{ falloff:3 } } ] }', "falloff property must be an integer in [0,2]"),
+    ('Scene { objects:[ { light:{} } ] }', "the scene has no surface objects"),
+    ('let a = 1\nlet b = a.x', "expression result is not a dictionary"),
+    ('let a = 1\nlet b = a(2)', "expression result is not a function"),
+])
+def test_errors(text, message):
+    import cgspline_amd as cgs
+    with pytest.raises(cgs.CgsError, match=re.escape(message)):
+        parse(text)
+
+
+def test_error_names_the_place():
+    import cgspline_amd as cgs
+    with pytest.raises(cgs.CgsError, match=r"t\.dl:3:9: .*property \"Nope\" undefined"):
+        parse('Surface { name:"w" file:"longwave" }\n\nlet x = Nope\n')
+
+
+def test_read_missing_file():
+    import cgspline_amd as cgs
+    with pytest.raises(cgs.CgsError, match="cannot open"):
+        cgs.read_dl(os.path.join(ASSETS, "scenes", "missing.dl"))
