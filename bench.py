@@ -54,6 +54,12 @@ def main():
                     help="plain: the 1M-patch render (the headline line); batched: 64 resident scenes traced "
                          "per job vs in one persistent launch; serving: continuous batching under Poisson load")
     ap.add_argument("--scenes", type=int, default=64, help="batched/serving: number of resident scenes")
+    ap.add_argument("--scene-file", default=None,
+                    help="plain mode on one of the reference's .dl scene files instead of the 1M-patch scene "
+                         "(e.g. tests/golden/assets/scenes/Test152.dl), seen from the file's view")
+    ap.add_argument("--width", type=int, default=1920, help="--scene-file: frame width")
+    ap.add_argument("--height", type=int, default=1080, help="--scene-file: frame height")
+    ap.add_argument("--depth", type=int, default=6, help="--scene-file: recursion depth (reference default 6)")
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
                     help="also run one streaming copy kernel (bitwise_not over int32) of this many MiB after the "
                          "warmup: the reference kernel that scales the HBM byte counters (FETCH_SIZE, WRITE_SIZE) "
@@ -75,7 +81,15 @@ def main():
     from cgspline_amd import scenes, workmodel
 
     t0 = time.time()
-    wl = scenes.WORKLOADS[args.workload]()
+    if args.scene_file:
+        wl = scenes.from_dl(args.scene_file, width=args.width, height=args.height, depth=args.depth)
+        args.workload = os.path.splitext(os.path.basename(args.scene_file))[0]
+        metric = f"Mrays/s on the reference scene {args.workload} (plain launch)"
+        data = f"the reference's scene file {os.path.basename(args.scene_file)} and its .be models"
+    else:
+        wl = scenes.WORKLOADS[args.workload]()
+        metric = "Mrays/s at 1M-patch scene (plain launch)"
+        data = "synthetic (seeded displaced bicubic grids)"
     cam = wl.camera
     log(f"workload {args.workload}: {wl.scene.npatch} patches, {wl.scene.nobj} objects, "
         f"{cam.width}x{cam.height}, depth {wl.depth} (generated in {time.time() - t0:.1f}s)")
@@ -133,7 +147,7 @@ def main():
         f"(min {np.min(kernel_ms):.2f}), {rays_per_step:.0f} rays/step -> {value:.1f} Mrays/s")
 
     out = {
-        "metric": "Mrays/s at 1M-patch scene (plain launch)",
+        "metric": metric,
         "value": round(value, 3),
         "unit": "Mrays/s",
         "n_gpus": 1,
@@ -144,7 +158,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded displaced bicubic grids)",
+        "data": data,
         "config": {
             "workload": args.workload,
             "patches": wl.scene.npatch,
