@@ -562,15 +562,23 @@ struct NodeBoxes
 };
 CGS_DEV NodeBoxes loadNode(const float4* __restrict__ nodes, uint32_t i)
 {
-    float4 a = nodes[4 * (size_t)i], b = nodes[4 * (size_t)i + 1], c = nodes[4 * (size_t)i + 2],
-           d = nodes[4 * (size_t)i + 3];
+    // three 16-B loads for the boxes and one 8-B load for the child offsets
+    // (the node's last 8 bytes are padding).  The offsets' element index goes
+    // through an empty asm so the compiler cannot see that the address is
+    // 16-B aligned: otherwise it widens the 8-B load to 16 B, and the widened
+    // load's registers overlap the boxes', which serialises the node fetch
+    // into two memory round trips (s_waitcnt vmcnt(0) between the loads).
+    const float4 a = nodes[4 * (size_t)i], b = nodes[4 * (size_t)i + 1], c = nodes[4 * (size_t)i + 2];
+    size_t w = 8 * (size_t)i + 6;
+    asm("" : "+v"(w));
+    const uint2 d = reinterpret_cast<const uint2*>(nodes)[w];
     NodeBoxes n;
     n.lmin = {a.x, a.y, a.z};
     n.lmax = {a.w, b.x, b.y};
     n.rmin = {b.z, b.w, c.x};
     n.rmax = {c.y, c.z, c.w};
-    n.left = __float_as_uint(d.x);
-    n.right = __float_as_uint(d.y);
+    n.left = d.x;
+    n.right = d.y;
     return n;
 }
 
@@ -636,15 +644,54 @@ CGS_DEV uint32_t hashKey(const BVHRef& b, uint32_t key)
     return fastmod(x, b.T);
 }
 
+// Node indices of a traversal's postponed siblings, kept per lane in LDS so
+// that an ascent usually reads the sibling's index instead of hashing its
+// key (a displacement load and a table load, dependent, before the node
+// load).  The ring holds the most recent `cap` postponed siblings in order:
+// a descent that postpones one writes it (overwriting the oldest when full),
+// an ascent takes the newest.  An ascent that finds the ring empty while
+// siblings are still postponed looks the key up in the perfect hash as
+// before.  The visited nodes and their order are unchanged.
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+constexpr uint32_t kRingStride = 256;   // threads per workgroup: slot k of a lane is base[k * 256]
+
+struct SiblingRing
+{
+    lds_u32* base;   // this lane's slot 0; null: no ring, every ascent hashes
+    uint32_t cap;
+};
+
+struct RingState
+{
+    uint32_t next = 0, live = 0;
+
+    CGS_DEV void push(const SiblingRing& r, uint32_t node)
+    {
+        if (!r.base) return;
+        r.base[next * kRingStride] = node;
+        next = next + 1 == r.cap ? 0 : next + 1;
+        live += live < r.cap ? 1u : 0u;
+    }
+    CGS_DEV bool pop(const SiblingRing& r, uint32_t& node)
+    {
+        if (!live) return false;
+        next = next ? next - 1 : r.cap - 1;
+        node = r.base[next * kRingStride];
+        --live;
+        return true;
+    }
+};
+
 // Stackless closest-hit traversal (rt/BVH.h:178-266).  `limit` is the
 // caller's hit.t (read through a reference so leaf hits tighten the culling);
 // `leaf(index)` tests one element and returns whether it improved the hit.
 template<class Leaf>
 CGS_DEV bool hashIntersect(const BVHRef& b, V3 o, V3 dir, const float& limit, Leaf&& leaf,
-                           const NodeCache& nc = NodeCache {nullptr, 0})
+                           const NodeCache& nc = NodeCache {nullptr, 0}, const SiblingRing& ring = {nullptr, 0})
 {
     const V3 Dinv {1.0f / dir.x, 1.0f / dir.y, 1.0f / dir.z};
     uint32_t node = b.root, key = 1, postponed = 0;
+    RingState rs;
     bool result = false;
     while (true)
     {
@@ -659,6 +706,7 @@ CGS_DEV bool hashIntersect(const BVHRef& b, V3 o, V3 dir, const float& limit, Le
                 // both hit: postpone one, descend into the nearer (right on a tie)
                 const bool both = iL && iR;
                 const bool goLeft = both ? L0 < R0 : iL;
+                if (both) rs.push(ring, goLeft ? n.right : n.left);
                 postponed = (postponed << 1) | (both ? 1u : 0u);
                 key = (key << 1) | (goLeft ? 0u : 1u);
                 node = goLeft ? n.left : n.right;
@@ -680,6 +728,7 @@ CGS_DEV bool hashIntersect(const BVHRef& b, V3 o, V3 dir, const float& limit, Le
         postponed >>= levels;
         key = (key >> levels) ^ 1u;
         postponed ^= 1;
+        if (rs.pop(ring, node)) continue;
         if (key < nc.keys) continue;    // staged in LDS: read by key, no lookup
         node = b.table[hashKey(b, key)];
         if (node >= b.nodeCount) break; // a corrupt table ends the ray instead of faulting
@@ -690,10 +739,11 @@ CGS_DEV bool hashIntersect(const BVHRef& b, V3 o, V3 dir, const float& limit, Le
 // Stackless any-hit traversal against `tmax` (rt/BVH.h:268-350).
 template<class Leaf>
 CGS_DEV bool hashIntersectAny(const BVHRef& b, V3 o, V3 dir, float tmax, Leaf&& leaf,
-                              const NodeCache& nc = NodeCache {nullptr, 0})
+                              const NodeCache& nc = NodeCache {nullptr, 0}, const SiblingRing& ring = {nullptr, 0})
 {
     const V3 Dinv {1.0f / dir.x, 1.0f / dir.y, 1.0f / dir.z};
     uint32_t node = b.root, key = 1, postponed = 0;
+    RingState rs;
     while (true)
     {
         NodeBoxes n = loadNodeCached(b, nc, node, key);
@@ -707,6 +757,7 @@ CGS_DEV bool hashIntersectAny(const BVHRef& b, V3 o, V3 dir, float tmax, Leaf&& 
                 // both hit: postpone one, descend into the nearer (right on a tie)
                 const bool both = iL && iR;
                 const bool goLeft = both ? L0 < R0 : iL;
+                if (both) rs.push(ring, goLeft ? n.right : n.left);
                 postponed = (postponed << 1) | (both ? 1u : 0u);
                 key = (key << 1) | (goLeft ? 0u : 1u);
                 node = goLeft ? n.left : n.right;
@@ -728,6 +779,7 @@ CGS_DEV bool hashIntersectAny(const BVHRef& b, V3 o, V3 dir, float tmax, Leaf&& 
         postponed >>= levels;
         key = (key >> levels) ^ 1u;
         postponed ^= 1;
+        if (rs.pop(ring, node)) continue;
         if (key < nc.keys) continue;    // staged in LDS: read by key, no lookup
         node = b.table[hashKey(b, key)];
         if (node >= b.nodeCount) break; // a corrupt table ends the ray instead of faulting

@@ -48,8 +48,26 @@ struct ClosestHit
 
 // Context::intersect(hit, ray) (RayTracer.cu:369-423) with
 // rt::intersect(BezierSurface, hit, ray) (Primitive.cu:355-381) at the leaves.
-CGS_DEV ClosestHit sceneClosest(const SceneDev& sc, const NodeCache& nc, V3 o, V3 d, float tmax, const ClipStack& stk,
-                                bool& overflow)
+// Sibling rings of a lane for the two traversal levels (the top-level BVH
+// over objects and the patch BVH of one object), CGS_RING_SLOTS entries each.
+#ifndef CGS_RING_SLOTS
+#define CGS_RING_SLOTS 8
+#endif
+struct Rings
+{
+    SiblingRing top, bot;
+};
+constexpr uint32_t kRingWords = CGS_RING_SLOTS ? 2 * CGS_RING_SLOTS * kRingStride : 1;
+
+CGS_DEV Rings laneRings(uint32_t* lds)
+{
+    if (!CGS_RING_SLOTS) return Rings {{nullptr, 0}, {nullptr, 0}};
+    lds_u32* lane = (lds_u32*)lds + threadIdx.x;
+    return Rings {{lane, CGS_RING_SLOTS}, {lane + CGS_RING_SLOTS * kRingStride, CGS_RING_SLOTS}};
+}
+
+CGS_DEV ClosestHit sceneClosest(const SceneDev& sc, const NodeCache& nc, const Rings& rg, V3 o, V3 d, float tmax,
+                                const ClipStack& stk, bool& overflow)
 {
     ClosestHit h {tmax, 0.0f, 0.0f, EMPTY, -1};
     const BVHRef top = bvhRef(sc, sc.top);
@@ -66,7 +84,7 @@ CGS_DEV ClosestHit sceneClosest(const SceneDev& sc, const NodeCache& nc, V3 o, V
             bool f = clipPatch<false>(S, cr, kClipTol, lt, lu, lv, stk, overflow);
             if (f) lp = pid;
             return f;
-        });
+        }, NodeCache {nullptr, 0}, rg.bot);
         if (hit)
         {
             float t = lt * lr.dinv;
@@ -81,14 +99,14 @@ CGS_DEV ClosestHit sceneClosest(const SceneDev& sc, const NodeCache& nc, V3 o, V
             }
         }
         return false;
-    }, nc);
+    }, nc, rg.top);
     return h;
 }
 
 // Context::intersect(ray) (RayTracer.cu:425-457) with
 // rt::intersect(BezierSurface, ray) (Primitive.cu:383-401) at the leaves.
-CGS_DEV bool sceneAny(const SceneDev& sc, const NodeCache& nc, V3 o, V3 d, float tmax, const ClipStack& stk,
-                      bool& overflow)
+CGS_DEV bool sceneAny(const SceneDev& sc, const NodeCache& nc, const Rings& rg, V3 o, V3 d, float tmax,
+                      const ClipStack& stk, bool& overflow)
 {
     const BVHRef top = bvhRef(sc, sc.top);
     return hashIntersectAny(top, o, d, tmax, [&](uint32_t i) -> bool {
@@ -101,8 +119,8 @@ CGS_DEV bool sceneAny(const SceneDev& sc, const NodeCache& nc, V3 o, V3 d, float
             const float4* S = sc.patches + 16 * (size_t)(surf.patchBase + pid);
             float best = lr.max, bu, bv;
             return clipPatch<true>(S, cr, kClipTol, best, bu, bv, stk, overflow);
-        });
-    }, nc);
+        }, NodeCache {nullptr, 0}, rg.bot);
+    }, nc, rg.top);
 }
 
 #ifndef CGS_TOP_LDS_KEYS
@@ -161,7 +179,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CGS_TRACE_W
                                                      unsigned long long* __restrict__ overflowCount)
 {
     __shared__ float4 topLds[4 * CGS_TOP_LDS_KEYS];
+    __shared__ uint32_t ringLds[kRingWords];
     const NodeCache nc = stageTopLevel(sc, topLds);
+    const Rings rg = laneRings(ringLds);
     const uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x;
     const ClipStack stk = clipStackFor(ps.stack, gl, ps.lanes);
     unsigned long long nOverflow = 0;
@@ -171,7 +191,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CGS_TRACE_W
         if (r >= n) continue;
         bool overflow = false;
         const cgs_ray ray = rays[r];
-        ClosestHit h = sceneClosest(sc, nc, {ray.ox, ray.oy, ray.oz}, {ray.dx, ray.dy, ray.dz}, ray.tmax, stk, overflow);
+        ClosestHit h = sceneClosest(sc, nc, rg, {ray.ox, ray.oy, ray.oz}, {ray.dx, ray.dy, ray.dz}, ray.tmax, stk, overflow);
         cgs_hit out;
         out.t = h.t;
         out.u = h.object < 0 ? 0.0f : h.u;
@@ -189,7 +209,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CGS_TRACE_W
                                                  unsigned long long* __restrict__ overflowCount)
 {
     __shared__ float4 topLds[4 * CGS_TOP_LDS_KEYS];
+    __shared__ uint32_t ringLds[kRingWords];
     const NodeCache nc = stageTopLevel(sc, topLds);
+    const Rings rg = laneRings(ringLds);
     const uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x;
     const ClipStack stk = clipStackFor(ps.stack, gl, ps.lanes);
     unsigned long long nOverflow = 0;
@@ -199,7 +221,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CGS_TRACE_W
         if (r >= n) continue;
         bool overflow = false;
         const cgs_ray ray = rays[r];
-        bool occ = sceneAny(sc, nc, {ray.ox, ray.oy, ray.oz}, {ray.dx, ray.dy, ray.dz}, ray.tmax, stk, overflow);
+        bool occ = sceneAny(sc, nc, rg, {ray.ox, ray.oy, ray.oz}, {ray.dx, ray.dy, ray.dz}, ray.tmax, stk, overflow);
         occluded[r] = overflow ? 2u : (occ ? 1u : 0u);
         nOverflow += overflow;
     }
@@ -219,6 +241,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CGS_TRACE_W
     const uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x;
     const ClipStack stk = clipStackFor(ps.stack, gl, ps.lanes);
     const NodeCache none {nullptr, 0};
+    __shared__ uint32_t ringLds[kRingWords];
+    const Rings rg = laneRings(ringLds);
     unsigned long long nOverflow = 0;
     while (true)
     {
@@ -244,13 +268,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CGS_TRACE_W
             bool overflow = false;
             if (J.any)
             {
-                const bool occ = sceneAny(J.sc, none, {ray.ox, ray.oy, ray.oz}, {ray.dx, ray.dy, ray.dz}, ray.tmax, stk,
+                const bool occ = sceneAny(J.sc, none, rg, {ray.ox, ray.oy, ray.oz}, {ray.dx, ray.dy, ray.dz}, ray.tmax, stk,
                                           overflow);
                 J.occluded[k] = overflow ? 2u : (occ ? 1u : 0u);
             }
             else
             {
-                const ClosestHit h = sceneClosest(J.sc, none, {ray.ox, ray.oy, ray.oz}, {ray.dx, ray.dy, ray.dz},
+                const ClosestHit h = sceneClosest(J.sc, none, rg, {ray.ox, ray.oy, ray.oz}, {ray.dx, ray.dy, ray.dz},
                                                   ray.tmax, stk, overflow);
                 cgs_hit out;
                 out.t = h.t;
@@ -558,7 +582,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CGS_RENDER_
                                                unsigned long long* __restrict__ counts)
 {
     __shared__ float4 topLds[4 * CGS_TOP_LDS_KEYS];
+    __shared__ uint32_t ringLds[kRingWords];
     const NodeCache nc = stageTopLevel(sc, topLds);
+    const Rings rg = laneRings(ringLds);
     const uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x;
     const ClipStack stk = clipStackFor(ps.stack, gl, ps.lanes);
     const uint32_t lane = threadIdx.x & 63;
@@ -589,7 +615,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CGS_RENDER_
         {
             if (!first) nSecondary++;
             first = false;
-            ClosestHit h = sceneClosest(sc, nc, ro, rd, rmax, stk, overflow);
+            ClosestHit h = sceneClosest(sc, nc, rg, ro, rd, rmax, stk, overflow);
             if (h.object < 0)
             {
                 acc = add3(acc, mulv3({sc.background[0], sc.background[1], sc.background[2]}, att));
@@ -624,7 +650,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CGS_RENDER_
                 const float dotNL = dot3(N, L);
                 if (dotNL < 1e-14f) continue;
                 nShadow++;
-                if (sceneAny(sc, nc, add3(Pt, mul3(op.eps, L)), L, d, stk, overflow)) continue;
+                if (sceneAny(sc, nc, rg, add3(Pt, mul3(op.eps, L)), L, d, stk, overflow)) continue;
                 V3 H = versor3(add3(L, V));
                 float dotHN = dot3(H, N);
                 float dotHL = dot3(H, L);
