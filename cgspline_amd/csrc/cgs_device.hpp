@@ -657,8 +657,8 @@ constexpr uint32_t kRingStride = 256;   // threads per workgroup: slot k of a la
 
 struct SiblingRing
 {
-    lds_u32* base;   // this lane's slot 0; null: no ring, every ascent hashes
-    uint32_t cap;
+    lds_u32* base;   // this lane's slot 0
+    uint32_t cap;    // 0: no ring, every ascent hashes (the same for all lanes)
 };
 
 struct RingState
@@ -667,7 +667,7 @@ struct RingState
 
     CGS_DEV void push(const SiblingRing& r, uint32_t node)
     {
-        if (!r.base) return;
+        if (!r.cap) return;
         r.base[next * kRingStride] = node;
         next = next + 1 == r.cap ? 0 : next + 1;
         live += live < r.cap ? 1u : 0u;
@@ -682,6 +682,63 @@ struct RingState
     }
 };
 
+// One lane's place in a hashed stackless traversal (rt/BVH.h:178-350): the
+// current node and its heap key, a bit per level whose far child is still
+// postponed, and the sibling ring.
+struct TravState
+{
+    uint32_t node, key, postponed;
+    RingState rs;
+};
+
+// Moves to the most recently postponed sibling, after a leaf or a subtree the
+// ray misses.  Returns false at the end of the traversal (nothing postponed),
+// or on a table entry past the node array (a corrupt table ends the ray
+// instead of faulting).
+CGS_DEV bool ascend(const BVHRef& b, const NodeCache& nc, const SiblingRing& ring, TravState& st)
+{
+    if (!st.postponed) return false;
+    const int levels = __builtin_ctz(st.postponed);
+    st.postponed >>= levels;
+    st.key = (st.key >> levels) ^ 1u;
+    st.postponed ^= 1;
+    if (st.rs.pop(ring, st.node)) return true;
+    if (st.key < nc.keys) return true;    // staged in LDS: read by key, no lookup
+    st.node = b.table[hashKey(b, st.key)];
+    return st.node < b.nodeCount;
+}
+
+// Walks interior nodes (descending into the nearer child hit, postponing the
+// other when both are hit, right on a tie; ascending when neither is) until
+// it reaches a leaf, whose first element index it returns, or the end
+// (EMPTY).  The caller tests the leaf's elements outside this loop and then
+// ascends: the per-lane order of node visits and leaf tests is the
+// reference's, while the loop itself stays small enough to keep its
+// registers, and lanes that reach leaves together test them together.
+CGS_DEV uint32_t nextLeaf(const BVHRef& b, const NodeCache& nc, const SiblingRing& ring, V3 o, V3 Dinv,
+                          const float& limit, TravState& st)
+{
+    while (true)
+    {
+        const NodeBoxes n = loadNodeCached(b, nc, st.node, st.key);
+        if (n.left == EMPTY) return n.right;
+        float L0, R0;
+        const bool iL = nodeIntersect(L0, n.lmin, n.lmax, o, Dinv) && L0 < limit;
+        const bool iR = nodeIntersect(R0, n.rmin, n.rmax, o, Dinv) && R0 < limit;
+        if (iL || iR)
+        {
+            const bool both = iL && iR;
+            const bool goLeft = both ? L0 < R0 : iL;
+            if (both) st.rs.push(ring, goLeft ? n.right : n.left);
+            st.postponed = (st.postponed << 1) | (both ? 1u : 0u);
+            st.key = (st.key << 1) | (goLeft ? 0u : 1u);
+            st.node = goLeft ? n.left : n.right;
+            continue;
+        }
+        if (!ascend(b, nc, ring, st)) return EMPTY;
+    }
+}
+
 // Stackless closest-hit traversal (rt/BVH.h:178-266).  `limit` is the
 // caller's hit.t (read through a reference so leaf hits tighten the culling);
 // `leaf(index)` tests one element and returns whether it improved the hit.
@@ -690,48 +747,18 @@ CGS_DEV bool hashIntersect(const BVHRef& b, V3 o, V3 dir, const float& limit, Le
                            const NodeCache& nc = NodeCache {nullptr, 0}, const SiblingRing& ring = {nullptr, 0})
 {
     const V3 Dinv {1.0f / dir.x, 1.0f / dir.y, 1.0f / dir.z};
-    uint32_t node = b.root, key = 1, postponed = 0;
-    RingState rs;
+    TravState st {b.root, 1u, 0u, RingState {}};
     bool result = false;
-    while (true)
+    uint32_t i;
+    while ((i = nextLeaf(b, nc, ring, o, Dinv, limit, st)) != EMPTY)
     {
-        NodeBoxes n = loadNodeCached(b, nc, node, key);
-        if (n.left != EMPTY)
+        uint32_t e;
+        while ((e = b.indices[i]) != EMPTY)
         {
-            float L0, R0;
-            bool iL = nodeIntersect(L0, n.lmin, n.lmax, o, Dinv) && L0 < limit;
-            bool iR = nodeIntersect(R0, n.rmin, n.rmax, o, Dinv) && R0 < limit;
-            if (iL || iR)
-            {
-                // both hit: postpone one, descend into the nearer (right on a tie)
-                const bool both = iL && iR;
-                const bool goLeft = both ? L0 < R0 : iL;
-                if (both) rs.push(ring, goLeft ? n.right : n.left);
-                postponed = (postponed << 1) | (both ? 1u : 0u);
-                key = (key << 1) | (goLeft ? 0u : 1u);
-                node = goLeft ? n.left : n.right;
-                continue;
-            }
+            result |= leaf(e);
+            i++;
         }
-        else
-        {
-            uint32_t i = n.right;
-            uint32_t e;
-            while ((e = b.indices[i]) != EMPTY)
-            {
-                result |= leaf(e);
-                i++;
-            }
-        }
-        if (!postponed) break;
-        int levels = __builtin_ctz(postponed);
-        postponed >>= levels;
-        key = (key >> levels) ^ 1u;
-        postponed ^= 1;
-        if (rs.pop(ring, node)) continue;
-        if (key < nc.keys) continue;    // staged in LDS: read by key, no lookup
-        node = b.table[hashKey(b, key)];
-        if (node >= b.nodeCount) break; // a corrupt table ends the ray instead of faulting
+        if (!ascend(b, nc, ring, st)) break;
     }
     return result;
 }
@@ -742,47 +769,17 @@ CGS_DEV bool hashIntersectAny(const BVHRef& b, V3 o, V3 dir, float tmax, Leaf&& 
                               const NodeCache& nc = NodeCache {nullptr, 0}, const SiblingRing& ring = {nullptr, 0})
 {
     const V3 Dinv {1.0f / dir.x, 1.0f / dir.y, 1.0f / dir.z};
-    uint32_t node = b.root, key = 1, postponed = 0;
-    RingState rs;
-    while (true)
+    TravState st {b.root, 1u, 0u, RingState {}};
+    uint32_t i;
+    while ((i = nextLeaf(b, nc, ring, o, Dinv, tmax, st)) != EMPTY)
     {
-        NodeBoxes n = loadNodeCached(b, nc, node, key);
-        if (n.left != EMPTY)
+        uint32_t e;
+        while ((e = b.indices[i]) != EMPTY)
         {
-            float L0, R0;
-            bool iL = nodeIntersect(L0, n.lmin, n.lmax, o, Dinv) && L0 < tmax;
-            bool iR = nodeIntersect(R0, n.rmin, n.rmax, o, Dinv) && R0 < tmax;
-            if (iL || iR)
-            {
-                // both hit: postpone one, descend into the nearer (right on a tie)
-                const bool both = iL && iR;
-                const bool goLeft = both ? L0 < R0 : iL;
-                if (both) rs.push(ring, goLeft ? n.right : n.left);
-                postponed = (postponed << 1) | (both ? 1u : 0u);
-                key = (key << 1) | (goLeft ? 0u : 1u);
-                node = goLeft ? n.left : n.right;
-                continue;
-            }
+            if (leaf(e)) return true;
+            i++;
         }
-        else
-        {
-            uint32_t i = n.right;
-            uint32_t e;
-            while ((e = b.indices[i]) != EMPTY)
-            {
-                if (leaf(e)) return true;
-                i++;
-            }
-        }
-        if (!postponed) break;
-        int levels = __builtin_ctz(postponed);
-        postponed >>= levels;
-        key = (key >> levels) ^ 1u;
-        postponed ^= 1;
-        if (rs.pop(ring, node)) continue;
-        if (key < nc.keys) continue;    // staged in LDS: read by key, no lookup
-        node = b.table[hashKey(b, key)];
-        if (node >= b.nodeCount) break; // a corrupt table ends the ray instead of faulting
+        if (!ascend(b, nc, ring, st)) break;
     }
     return false;
 }

@@ -83,8 +83,11 @@ hipError_t launchClipBatch(const PersistArgs& pa, const cgs_ray* rays, const flo
                            cgs_hit* hits, hipStream_t s);
 hipError_t launchSubdivBatch(const cgs_ray* rays, const float* patches, uint32_t n, float threshold, cgs_hit* hits,
                              hipStream_t s);
+// kRender reads its camera, options and output pointers from `argsDev`, a
+// device block of kRenderArgsBytes that launchRender fills on stream `s`.
+constexpr size_t kRenderArgsBytes = 256;
 hipError_t launchRender(const SceneDev& sc, const PersistArgs& pa, const float* toWorld16, const float* pos3, float tlx,
                         float tly, float hdx, float hdy, float eps, int depth, uint32_t w, uint32_t h, uint32_t* frame,
-                        float* color, unsigned long long* counts, hipStream_t s);
+                        float* color, unsigned long long* counts, void* argsDev, hipStream_t s);
 
 } // namespace cgs

@@ -110,6 +110,7 @@ struct cgs_ctx
     bool batchLaunched = false;
     unsigned long long* dOverflow = nullptr; // device counter of overflowed clip stacks
     unsigned int* dCounters = nullptr;       // one work counter per persistent kernel
+    void* dRenderArgs = nullptr;             // kRender's argument block (one render in flight)
     float2* dStack = nullptr;                // clipping region stacks of the resident lanes
     size_t stackLanes = 0;
     uint32_t grid[kPersistKernels] = {};
@@ -667,6 +668,7 @@ cgs_status cgs_open(int device, cgs_ctx** out)
     if (const char* v = std::getenv("CGS_STAGED")) c->staged = std::atoi(v) != 0;
     if (e == hipSuccess) e = hipMalloc(&c->dCounters, 256);
     if (e == hipSuccess) e = hipMemset(c->dCounters, 0, 256);
+    if (e == hipSuccess) e = hipMalloc(&c->dRenderArgs, kRenderArgsBytes);
     if (e != hipSuccess)
     {
         delete c;
@@ -685,6 +687,7 @@ cgs_status cgs_close(cgs_ctx* ctx)
     (void)hipDeviceSynchronize();
     if (ctx->dOverflow) (void)hipFree(ctx->dOverflow);
     if (ctx->dCounters) (void)hipFree(ctx->dCounters);
+    if (ctx->dRenderArgs) (void)hipFree(ctx->dRenderArgs);
     if (ctx->dStack) (void)hipFree(ctx->dStack);
     if (ctx->hCount) (void)hipHostFree(ctx->hCount);
     delete ctx;
@@ -1079,7 +1082,7 @@ cgs_status cgs_render(cgs_ctx* ctx, cgs_scene* sc, const cgs_camera* cam, const 
         cgs_status pst = persistArgs(ctx, PersistKernel::Render, s, pa);
         if (pst != CGS_OK) return pst;
         HIP_TRY(launchRender(sc->view(), pa, cam->to_world, cam->position, tlx, tly, hdx, hdy, op->eps,
-                             op->recursion_depth, w, h, frame, color, (unsigned long long*)counts, s));
+                             op->recursion_depth, w, h, frame, color, (unsigned long long*)counts, ctx->dRenderArgs, s));
         HIP_TRY(hipEventRecord(ctx->renderDone, s));
         ctx->renderLaunched = true;
         return CGS_OK;

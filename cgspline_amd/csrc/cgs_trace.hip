@@ -497,6 +497,19 @@ struct RenderDev
     uint32_t width, height;
 };
 
+// A render's camera, options and outputs, read by kRender from device memory
+// where it needs them (once per tile, or per light for eps) rather than held
+// in scalar registers for the whole launch: the traversal loops need those
+// registers.
+struct RenderArgs
+{
+    CamDev cam;
+    RenderDev op;
+    uint32_t* frame;
+    float* color;
+    unsigned long long* counts;
+};
+
 // PBR (rt/PBR.h, rt/PBR.cu)
 CGS_DEV bool lightVector(float& d, V3& L, V3 P, const DevLight& lt)
 {
@@ -577,9 +590,7 @@ CGS_DEV float saturate(float x) { return x > 0.0f ? (x < 1.0f ? x : 1.0f) : 0.0f
 // keeps VGPR + AGPR use within 512 / CGS_RENDER_WAVES.  On the 1M-patch frame
 // three waves per SIMD (168 registers) measured 39.5 ms, two waves 44.5 ms
 // and four waves 46 ms.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CGS_RENDER_WAVES))) void kRender(SceneDev sc, Persist ps, CamDev cam, RenderDev op,
-                                               uint32_t* __restrict__ frame, float* __restrict__ color,
-                                               unsigned long long* __restrict__ counts)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CGS_RENDER_WAVES))) void kRender(SceneDev sc, Persist ps, const RenderArgs* ra)
 {
     __shared__ float4 topLds[4 * CGS_TOP_LDS_KEYS];
     __shared__ uint32_t ringLds[kRingWords];
@@ -588,14 +599,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CGS_RENDER_
     const uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x;
     const ClipStack stk = clipStackFor(ps.stack, gl, ps.lanes);
     const uint32_t lane = threadIdx.x & 63;
-    const uint32_t tilesX = (op.width + 7) / 8, tilesY = (op.height + 7) / 8;
-    const uint32_t nItems = tilesX * tilesY * 64;
+    uint32_t nItems;
+    {
+        const RenderDev op = ra->op;
+        nItems = ((op.width + 7) / 8) * ((op.height + 7) / 8) * 64;
+    }
     uint32_t nPrimary = 0, nShadow = 0, nSecondary = 0, nOverflow = 0;
-    M4 C;
-    C.c[0] = cam.c[0]; C.c[1] = cam.c[1]; C.c[2] = cam.c[2]; C.c[3] = cam.c[3];
-    const V3 camPos {cam.px, cam.py, cam.pz};
     for (uint32_t base = nextChunk(ps.counter); base < nItems; base = nextChunk(ps.counter))
     {
+        const RenderDev op = ra->op;
+        const CamDev cam = ra->cam;
+        const uint32_t tilesX = (op.width + 7) / 8;
+        M4 C;
+        C.c[0] = cam.c[0]; C.c[1] = cam.c[1]; C.c[2] = cam.c[2]; C.c[3] = cam.c[3];
+        const V3 camPos {cam.px, cam.py, cam.pz};
         // one wave = one 8×8 pixel tile
         const uint32_t tile = base / 64;
         const uint32_t i = (tile % tilesX) * 8 + (lane & 7), j = (tile / tilesX) * 8 + (lane >> 3);
@@ -650,7 +667,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CGS_RENDER_
                 const float dotNL = dot3(N, L);
                 if (dotNL < 1e-14f) continue;
                 nShadow++;
-                if (sceneAny(sc, nc, rg, add3(Pt, mul3(op.eps, L)), L, d, stk, overflow)) continue;
+                if (sceneAny(sc, nc, rg, add3(Pt, mul3(ra->op.eps, L)), L, d, stk, overflow)) continue;
                 V3 H = versor3(add3(L, V));
                 float dotHN = dot3(H, N);
                 float dotHL = dot3(H, L);
@@ -667,23 +684,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CGS_RENDER_
             V3 brdf = mul3(rr * rr, mul3(dotNV, add3(diffuse, mul3(metal, mul3(g, refl)))));
             V3 a = mulv3(att, brdf);
             if (max3(a) <= 0x1p-8f) break;
-            ro = add3(Pt, mul3(op.eps, R));
+            ro = add3(Pt, mul3(ra->op.eps, R));
             rd = R;
             rmax = __builtin_inff();
             att = a;
             depth = depth - 1;
         }
-        const uint32_t q = j * op.width + i;
-        if (color)
+        const uint32_t q = j * ra->op.width + i;
+        if (float* color = ra->color)
         {
             color[3 * (size_t)q] = acc.x;
             color[3 * (size_t)q + 1] = acc.y;
             color[3 * (size_t)q + 2] = acc.z;
         }
-        frame[q] = packSRGB(saturate(acc.x), saturate(acc.y), saturate(acc.z));
+        ra->frame[q] = packSRGB(saturate(acc.x), saturate(acc.y), saturate(acc.z));
         nOverflow += overflow;
     }
-    if (counts)
+    if (unsigned long long* counts = ra->counts)
     {
         addWave(counts + 0, nPrimary);
         addWave(counts + 1, nShadow);
@@ -766,7 +783,7 @@ hipError_t launchSubdivBatch(const cgs_ray* rays, const float* patches, uint32_t
 
 hipError_t launchRender(const SceneDev& sc, const PersistArgs& pa, const float* toWorld16, const float* pos3, float tlx,
                         float tly, float hdx, float hdy, float eps, int depth, uint32_t w, uint32_t h, uint32_t* frame,
-                        float* color, unsigned long long* counts, hipStream_t s)
+                        float* color, unsigned long long* counts, void* argsDev, hipStream_t s)
 {
     CamDev cam;
     for (int c = 0; c < 4; c++)
@@ -778,10 +795,13 @@ hipError_t launchRender(const SceneDev& sc, const PersistArgs& pa, const float* 
     cam.tly = tly;
     cam.hdx = hdx;
     cam.hdy = hdy;
-    RenderDev op {eps, depth, w, h};
     if ((uint64_t)w * h == 0) return hipSuccess;
+    const RenderArgs ra {cam, RenderDev {eps, depth, w, h}, frame, color, counts};
+    static_assert(sizeof(RenderArgs) <= kRenderArgsBytes, "render argument block");
+    hipError_t e = hipMemcpyAsync(argsDev, &ra, sizeof ra, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return e;
     Persist ps {pa.stack, pa.lanes, pa.counter};
-    hipLaunchKernelGGL(kRender, dim3(pa.grid), dim3(256), 0, s, sc, ps, cam, op, frame, color, counts);
+    hipLaunchKernelGGL(kRender, dim3(pa.grid), dim3(256), 0, s, sc, ps, (const RenderArgs*)argsDev);
     return hipGetLastError();
 }
 
