@@ -53,7 +53,8 @@ def main():
     ap.add_argument("--mode", choices=["plain", "batched", "serving"], default="plain",
                     help="plain: the 1M-patch render (the headline line); batched: 64 resident scenes traced "
                          "per job vs in one persistent launch; serving: continuous batching under Poisson load")
-    ap.add_argument("--scenes", type=int, default=64, help="batched/serving: number of resident scenes")
+    ap.add_argument("--scenes", type=int, default=None,
+                    help="batched/serving: number of resident scenes (default 64 batched, 8 serving)")
     ap.add_argument("--scene-file", default=None,
                     help="plain mode on one of the reference's .dl scene files instead of the 1M-patch scene "
                          "(e.g. tests/golden/assets/scenes/Test152.dl), seen from the file's view")
@@ -72,8 +73,10 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     if args.mode == "batched":
+        args.scenes = args.scenes or 64
         return run_batched(args)
     if args.mode == "serving":
+        args.scenes = args.scenes or 8
         return run_serving(args)
 
     import torch
