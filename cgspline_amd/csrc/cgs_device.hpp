@@ -587,6 +587,12 @@ CGS_DEV NodeBoxes loadNode(const float4* __restrict__ nodes, uint32_t i)
 // operand (0·inf) leaves the bound unchanged in both forms, and a ±0
 // difference only reaches comparisons (tmin < tmax, L0 < limit, L0 < R0),
 // where -0 == +0; tMin is used by nothing but those comparisons.
+//
+// OCT >= 0 states the sign of every component of Dinv for the whole wave
+// (bit i set: component i negative), so the near and far planes are chosen
+// at compile time; OCT < 0 reads the signs per lane.  Both compute the same
+// values.
+template<int OCT = -1>
 CGS_DEV bool nodeIntersect(float& tMin, V3 bmin, V3 bmax, V3 o, V3 Dinv)
 {
     float tmin = -__builtin_inff(), tmax = __builtin_inff();
@@ -595,7 +601,7 @@ CGS_DEV bool nodeIntersect(float& tMin, V3 bmin, V3 bmax, V3 o, V3 Dinv)
 #pragma unroll
     for (int i = 0; i < 3; i++)
     {
-        bool s = __builtin_signbit(di[i]);
+        const bool s = OCT >= 0 ? ((OCT >> i) & 1) != 0 : __builtin_signbit(di[i]);
         float ta = (b0[i] - oo[i]) * di[i];
         float tb = (b1[i] - oo[i]) * di[i];
         float tn = s ? tb : ta;
@@ -715,6 +721,7 @@ CGS_DEV bool ascend(const BVHRef& b, const NodeCache& nc, const SiblingRing& rin
 // ascends: the per-lane order of node visits and leaf tests is the
 // reference's, while the loop itself stays small enough to keep its
 // registers, and lanes that reach leaves together test them together.
+template<int OCT = -1>
 CGS_DEV uint32_t nextLeaf(const BVHRef& b, const NodeCache& nc, const SiblingRing& ring, V3 o, V3 Dinv,
                           const float& limit, TravState& st)
 {
@@ -723,8 +730,8 @@ CGS_DEV uint32_t nextLeaf(const BVHRef& b, const NodeCache& nc, const SiblingRin
         const NodeBoxes n = loadNodeCached(b, nc, st.node, st.key);
         if (n.left == EMPTY) return n.right;
         float L0, R0;
-        const bool iL = nodeIntersect(L0, n.lmin, n.lmax, o, Dinv) && L0 < limit;
-        const bool iR = nodeIntersect(R0, n.rmin, n.rmax, o, Dinv) && R0 < limit;
+        const bool iL = nodeIntersect<OCT>(L0, n.lmin, n.lmax, o, Dinv) && L0 < limit;
+        const bool iR = nodeIntersect<OCT>(R0, n.rmin, n.rmax, o, Dinv) && R0 < limit;
         if (iL || iR)
         {
             const bool both = iL && iR;
@@ -739,18 +746,52 @@ CGS_DEV uint32_t nextLeaf(const BVHRef& b, const NodeCache& nc, const SiblingRin
     }
 }
 
+// The direction octant shared by every active lane of the wave (bit i: Dinv
+// component i negative, by sign bit as nodeIntersect reads it), or -1 when
+// the lanes differ.  Rays of one 8×8 pixel tile nearly always share it.
+CGS_DEV int waveOctant(V3 Dinv)
+{
+    const int oct = (__builtin_signbit(Dinv.x) ? 1 : 0) | (__builtin_signbit(Dinv.y) ? 2 : 0) |
+                    (__builtin_signbit(Dinv.z) ? 4 : 0);
+    const int first = __builtin_amdgcn_readfirstlane(oct);
+    return __ballot(oct != first) == 0 ? first : -1;
+}
+
+// nextLeaf with the near/far planes fixed at compile time when the wave
+// shares a direction octant (a wave-uniform branch), per lane otherwise.
+CGS_DEV uint32_t nextLeafOct(int oct, const BVHRef& b, const NodeCache& nc, const SiblingRing& ring, V3 o, V3 Dinv,
+                             const float& limit, TravState& st)
+{
+    switch (oct)
+    {
+    case 0: return nextLeaf<0>(b, nc, ring, o, Dinv, limit, st);
+    case 1: return nextLeaf<1>(b, nc, ring, o, Dinv, limit, st);
+    case 2: return nextLeaf<2>(b, nc, ring, o, Dinv, limit, st);
+    case 3: return nextLeaf<3>(b, nc, ring, o, Dinv, limit, st);
+    case 4: return nextLeaf<4>(b, nc, ring, o, Dinv, limit, st);
+    case 5: return nextLeaf<5>(b, nc, ring, o, Dinv, limit, st);
+    case 6: return nextLeaf<6>(b, nc, ring, o, Dinv, limit, st);
+    case 7: return nextLeaf<7>(b, nc, ring, o, Dinv, limit, st);
+    default: return nextLeaf<-1>(b, nc, ring, o, Dinv, limit, st);
+    }
+}
+
 // Stackless closest-hit traversal (rt/BVH.h:178-266).  `limit` is the
 // caller's hit.t (read through a reference so leaf hits tighten the culling);
 // `leaf(index)` tests one element and returns whether it improved the hit.
-template<class Leaf>
+// OCTANTS: specialise the node loop on the wave's direction octant (worth its
+// code size where a wave's rays are coherent: the top level, in world space).
+template<bool OCTANTS = false, class Leaf>
 CGS_DEV bool hashIntersect(const BVHRef& b, V3 o, V3 dir, const float& limit, Leaf&& leaf,
                            const NodeCache& nc = NodeCache {nullptr, 0}, const SiblingRing& ring = {nullptr, 0})
 {
     const V3 Dinv {1.0f / dir.x, 1.0f / dir.y, 1.0f / dir.z};
     TravState st {b.root, 1u, 0u, RingState {}};
+    const int oct = OCTANTS ? waveOctant(Dinv) : -1;
     bool result = false;
     uint32_t i;
-    while ((i = nextLeaf(b, nc, ring, o, Dinv, limit, st)) != EMPTY)
+    while ((i = (OCTANTS ? nextLeafOct(oct, b, nc, ring, o, Dinv, limit, st)
+                         : nextLeaf(b, nc, ring, o, Dinv, limit, st))) != EMPTY)
     {
         uint32_t e;
         while ((e = b.indices[i]) != EMPTY)
@@ -764,14 +805,16 @@ CGS_DEV bool hashIntersect(const BVHRef& b, V3 o, V3 dir, const float& limit, Le
 }
 
 // Stackless any-hit traversal against `tmax` (rt/BVH.h:268-350).
-template<class Leaf>
+template<bool OCTANTS = false, class Leaf>
 CGS_DEV bool hashIntersectAny(const BVHRef& b, V3 o, V3 dir, float tmax, Leaf&& leaf,
                               const NodeCache& nc = NodeCache {nullptr, 0}, const SiblingRing& ring = {nullptr, 0})
 {
     const V3 Dinv {1.0f / dir.x, 1.0f / dir.y, 1.0f / dir.z};
     TravState st {b.root, 1u, 0u, RingState {}};
+    const int oct = OCTANTS ? waveOctant(Dinv) : -1;
     uint32_t i;
-    while ((i = nextLeaf(b, nc, ring, o, Dinv, tmax, st)) != EMPTY)
+    while ((i = (OCTANTS ? nextLeafOct(oct, b, nc, ring, o, Dinv, tmax, st)
+                         : nextLeaf(b, nc, ring, o, Dinv, tmax, st))) != EMPTY)
     {
         uint32_t e;
         while ((e = b.indices[i]) != EMPTY)

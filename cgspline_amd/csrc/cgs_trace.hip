@@ -71,7 +71,7 @@ CGS_DEV ClosestHit sceneClosest(const SceneDev& sc, const NodeCache& nc, const R
 {
     ClosestHit h {tmax, 0.0f, 0.0f, EMPTY, -1};
     const BVHRef top = bvhRef(sc, sc.top);
-    hashIntersect(top, o, d, h.t, [&](uint32_t i) -> bool {
+    hashIntersect<true>(top, o, d, h.t, [&](uint32_t i) -> bool {
         const DevObject& ob = sc.objects[i];
         LocalRay lr = toLocal(ob, o, d, tmax);
         const DevBVH surf = sc.surfaces[ob.surface];
@@ -109,7 +109,7 @@ CGS_DEV bool sceneAny(const SceneDev& sc, const NodeCache& nc, const Rings& rg, 
                       const ClipStack& stk, bool& overflow)
 {
     const BVHRef top = bvhRef(sc, sc.top);
-    return hashIntersectAny(top, o, d, tmax, [&](uint32_t i) -> bool {
+    return hashIntersectAny<true>(top, o, d, tmax, [&](uint32_t i) -> bool {
         const DevObject& ob = sc.objects[i];
         LocalRay lr = toLocal(ob, o, d, tmax);
         const DevBVH surf = sc.surfaces[ob.surface];
