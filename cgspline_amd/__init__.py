@@ -186,6 +186,17 @@ class Context:
         return out
 
     @staticmethod
+    def mesh_elements(vertices3: np.ndarray, triangles: np.ndarray) -> np.ndarray:
+        """Mesh::buildBVH element data (cgs_mesh_elements): (ntri, 10) float32
+        rows of bounds min, max, centroid and the triangle index."""
+        v = np.ascontiguousarray(vertices3, np.float32).reshape(-1, 3)
+        t = np.ascontiguousarray(triangles, np.uint32).reshape(-1, 3)
+        out = np.zeros((t.shape[0], 10), np.float32)
+        check(lib().cgs_mesh_elements(v.ctypes.data, v.shape[0], t.ctypes.data, t.shape[0], out.ctypes.data),
+              "cgs_mesh_elements")
+        return out
+
+    @staticmethod
     def bvh_build(elements10: np.ndarray, elements_per_node: int = 1):
         """Host BVH build; returns (nodes (N,16) u32 words, keys, indices)."""
         e = np.ascontiguousarray(elements10, np.float32)

@@ -30,7 +30,7 @@ PATCH_OVERFLOW = 0xFFFFFFFE
 # Every symbol include/cgs.h declares (checked by tests/test_abi.py).
 EXPORTS = [
     "cgs_abi_version", "cgs_status_string", "cgs_last_error", "cgs_open", "cgs_close",
-    "cgs_bezier_clip", "cgs_subdivision", "cgs_bvh_build", "cgs_patch_elements", "cgs_free_host", "cgs_load_be",
+    "cgs_bezier_clip", "cgs_subdivision", "cgs_bvh_build", "cgs_patch_elements", "cgs_mesh_elements", "cgs_free_host", "cgs_load_be",
     "cgs_dl_read", "cgs_dl_parse", "cgs_dl_free",
     "cgs_phf_displacement_size", "cgs_phf_build", "cgs_phf_fill_table", "cgs_phf_hash",
     "cgs_scene_create", "cgs_scene_destroy", "cgs_scene_bvh_info", "cgs_scene_bvh_copy",
@@ -130,6 +130,7 @@ def lib():
             "cgs_bvh_build": ([_P, _U32, _U32, C.POINTER(_P), C.POINTER(_P), C.POINTER(_U32), C.POINTER(_P),
                                C.POINTER(_U32)], C.c_int),
             "cgs_patch_elements": ([_P, _P, _U32, _P], C.c_int),
+            "cgs_mesh_elements": ([_P, _U32, _P, _U32, _P], C.c_int),
             "cgs_free_host": ([_P], None),
             "cgs_load_be": ([C.c_char_p, C.c_size_t, C.POINTER(_P), C.POINTER(_U32), C.POINTER(_P), C.POINTER(_U32)],
                             C.c_int),

@@ -58,7 +58,7 @@ using Fn = std::function<Value(const List&)>;
 
 struct Value
 {
-    enum Kind { None, Int, Float, Vec, Str, DictK, ListK, FnK, MaterialK, SurfaceK, SceneK };
+    enum Kind { None, Int, Float, Vec, Str, DictK, ListK, FnK, MaterialK, SurfaceK, MeshK, SceneK };
     Kind kind = None;
     int i = 0;
     float f = 0;
@@ -390,7 +390,7 @@ Mat4 eulerRotation(const float* deg)
 struct FlatObject
 {
     std::string name;
-    int surface;     // asset index
+    int surface;     // surface asset index; -1: a mesh (no geometry on this path)
     Material material;
     Mat4 world;
 };
@@ -439,9 +439,9 @@ public:
         _scope["Material"] = Value::ofFn([this](const List& a) { return createMaterial(a); });
         _scope["Surface"] = Value::ofFn([this](const List& a) { return createSurface(a); });
         _scope["Scene"] = Value::ofFn([this](const List& a) { return readScene(a); });
-        _scope["Mesh"] = Value::ofFn([](const List&) -> Value {
-            fail("Mesh: triangle meshes are not supported (this library traces Bezier surfaces)");
-        });
+        _scope["Mesh"] = Value::ofFn([this](const List& a) { return createMesh(a); });
+        // the reference's built-in box mesh (SceneReader.cpp:40)
+        insertAsset("Box", Value::ofAsset(Value::MeshK, 0));
         _scope["Texture"] = Value::ofFn([](const List&) -> Value { fail("Texture: textures are not supported"); });
     }
 
@@ -463,7 +463,7 @@ private:
     size_t _p = 0;
     Dict _scope;
     std::shared_ptr<Dict> _assets;
-    int _materialId = 0, _surfaceId = 0, _objectId = 0;
+    int _materialId = 0, _surfaceId = 0, _meshId = 0, _objectId = 0;
 
     const Token& cur() const { return _tok[_p]; }
     std::string where(const Token& t) const
@@ -784,6 +784,66 @@ private:
         return r;
     }
 
+    // SceneReader.cpp:133-213: a triangle mesh from `vertices` (3 numbers per
+    // vertex), optional `normals` (as many) and integer `triangles` (3 per
+    // triangle).  Meshes have no geometry on this path (the reference gives
+    // them empty bounds, Primitive.cu:47-56, so no ray reaches them); the
+    // asset is checked and kept by name only.
+    Value createMesh(const List& a)
+    {
+        if (a.empty()) fail("Mesh: expected a dictionary");
+        const Dict& d = asDict(a[0], "Mesh");
+        std::string name = "Mesh_" + std::to_string(_meshId + 1);
+        if (auto p = prop(d, "name")) name = asString(*p, "Mesh: name");
+        else ++_meshId;
+        if (prop(d, "file")) fail("not yet implemented");
+        size_t nv = 0;
+        if (auto p = prop(d, "vertices"))
+        {
+            if (p->kind != Value::ListK) fail("Mesh: vertices must be a list");
+            for (const Value& x : *p->l) (void)asNumber(x, "Mesh: vertices");
+            nv = p->l->size() / 3;
+        }
+        if (auto p = prop(d, "normals"))
+        {
+            if (p->kind != Value::ListK) fail("Mesh: normals must be a list");
+            if (p->l->size() / 3 != nv) fail("`normals` size must match `vertices` size");
+            for (const Value& x : *p->l) (void)asNumber(x, "Mesh: normals");
+        }
+        if (auto p = prop(d, "triangles"))
+        {
+            if (p->kind != Value::ListK) fail("Mesh: triangles must be a list");
+            for (const Value& x : *p->l)
+                if (x.kind != Value::Int) fail("Mesh: triangle indices must be integers");
+        }
+        const Value v = Value::ofAsset(Value::MeshK, 0);
+        insertAsset(name, v);
+        return v;
+    }
+
+    // SceneReader.cpp:242-260 (addMesh): `name` is required and names the
+    // mesh asset unless `model` gives it.
+    FlatObject meshObject(const Dict& d, const Mat4& world, const std::string& objectName)
+    {
+        const Value* nm = prop(d, "name");
+        if (!nm) fail("property \"name\" does not exist");
+        const std::string& name = asString(*nm, "mesh: name");
+        const Value* model = prop(d, "model");
+        if (!model || model->kind != Value::MeshK)
+        {
+            auto it = _assets->find(name);
+            if (it == _assets->end()) fail("property \"" + name + "\" does not exist");
+            if (it->second.kind != Value::MeshK) fail("mesh: `" + name + "` is not a Mesh");
+        }
+        Material m = kDefaultMaterial;
+        if (auto q = prop(d, "material"))
+        {
+            if (q->kind != Value::MaterialK) fail("mesh: `material` is not a Material");
+            m = materials[q->asset];
+        }
+        return FlatObject {objectName, -1, m, world};
+    }
+
     // SceneReader.cpp:420-456
     Value readScene(const List& a)
     {
@@ -876,7 +936,7 @@ private:
         Mat4 world = parent;
         if (auto p = prop(d, "transform")) world = parent * transform(asDict(*p, "object: transform"));
         if (auto p = prop(d, "light")) _scene.lights.push_back(light(asDict(*p, "object: light"), world, name));
-        if (prop(d, "mesh")) fail("Mesh: triangle meshes are not supported (this library traces Bezier surfaces)");
+        if (auto p = prop(d, "mesh")) _scene.objects.push_back(meshObject(asDict(*p, "object: mesh"), world, name));
         if (auto p = prop(d, "surface"))
         {
             const Dict& s = asDict(*p, "object: surface");
@@ -920,7 +980,7 @@ DlScene* flatten(Reader& r)
     out->pbase.push_back(0);
     for (const FlatObject& o : sc.objects)
     {
-        if (slot[o.surface] < 0)
+        if (o.surface >= 0 && slot[o.surface] < 0)
         {
             const SurfaceAsset& s = r.surfaces[o.surface];
             slot[o.surface] = int(out->vbase.size()) - 1;
@@ -929,7 +989,8 @@ DlScene* flatten(Reader& r)
             out->vbase.push_back(out->vbase.back() + uint32_t(s.vertices.size() / 4));
             out->pbase.push_back(out->pbase.back() + uint32_t(s.indices.size() / 16));
         }
-        out->objSurface.push_back(uint32_t(slot[o.surface]));
+        // a mesh object has no surface (CGS_EMPTY: empty bounds, never hit)
+        out->objSurface.push_back(o.surface < 0 ? CGS_EMPTY : uint32_t(slot[o.surface]));
         float m[16];
         o.world.store(m);
         out->l2w.insert(out->l2w.end(), m, m + 16);
@@ -981,7 +1042,9 @@ cgs_status parseInto(const std::string& text, const std::string& name, const std
     {
         Reader r(name, dir);
         r.parse(text);
-        if (r._scene.objects.empty()) fail(name + ": the scene has no surface objects");
+        bool anySurface = false;
+        for (const FlatObject& o : r._scene.objects) anySurface |= o.surface >= 0;
+        if (!anySurface) fail(name + ": the scene has no surface objects");
         *out = &flatten(r)->pub;
         return CGS_OK;
     }

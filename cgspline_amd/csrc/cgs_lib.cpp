@@ -801,7 +801,8 @@ cgs_status cgs_scene_create(cgs_ctx* ctx, const cgs_scene_desc* d, cgs_scene** o
         !d->obj_surface || !d->obj_local2world || !d->obj_world2local || !d->obj_material || (d->nlight && !d->lights))
         return fail(CGS_ERR_INVALID, "cgs_scene_create: incomplete description");
     for (uint32_t i = 0; i < d->nobj; i++)
-        if (d->obj_surface[i] >= d->nsurf) return fail(CGS_ERR_INVALID, "cgs_scene_create: object surface out of range");
+        if (d->obj_surface[i] >= d->nsurf && d->obj_surface[i] != CGS_EMPTY)
+            return fail(CGS_ERR_INVALID, "cgs_scene_create: object surface out of range");
     for (uint32_t k = 0; k < d->nsurf; k++)
         if (d->surf_pbase[k + 1] <= d->surf_pbase[k] || d->surf_vbase[k + 1] < d->surf_vbase[k])
             return fail(CGS_ERR_EMPTY_SPLIT, "cgs_scene_create: a surface has no patch");
@@ -835,16 +836,22 @@ cgs_status cgs_scene_create(cgs_ctx* ctx, const cgs_scene_desc* d, cgs_scene** o
     for (auto s : st)
         if (s != CGS_OK) return s;
 
-    // 2. top-level BVH over object world bounds (Scene::buildBVH)
+    // 2. top-level BVH over object world bounds (Scene::buildBVH, rt/Scene.cu:39-53).
+    //    An object without a surface (CGS_EMPTY: the reference's Mesh and
+    //    Sphere primitives, whose bounds() are empty, Primitive.cu:15-23,47-56)
+    //    keeps the empty box, centred at 0: it shapes the BVH like the
+    //    reference's and no ray enters it.
     {
         std::vector<cgs_bvh_element> el(d->nobj);
         for (uint32_t i = 0; i < d->nobj; i++)
         {
             const uint32_t k = d->obj_surface[i];
             const float* M = d->obj_local2world + 16 * (size_t)i;
-            const float* V = d->vertices + 4 * (size_t)d->surf_vbase[k];
             Box3 b;
-            for (uint64_t q = 16ull * d->surf_pbase[k]; q < 16ull * d->surf_pbase[k + 1]; q++)
+            const float* V = k == CGS_EMPTY ? nullptr : d->vertices + 4 * (size_t)d->surf_vbase[k];
+            const uint64_t q0 = k == CGS_EMPTY ? 0 : 16ull * d->surf_pbase[k];
+            const uint64_t q1 = k == CGS_EMPTY ? 0 : 16ull * d->surf_pbase[k + 1];
+            for (uint64_t q = q0; q < q1; q++)
             {
                 const float* v = V + 4 * (size_t)d->indices[q];
                 float p[3];

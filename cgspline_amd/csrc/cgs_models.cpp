@@ -1,7 +1,8 @@
 // SPDX-License-Identifier: MIT
 //
 // Model readers on the host: the `.be` Bézier patch format read by
-// Surface::load_be (Surface.cpp:149-213).
+// Surface::load_be (Surface.cpp:149-213); and the BVH element data of a
+// triangle mesh (Mesh::buildBVH, rt/Primitive.cu:90-108).
 //
 // Layout of a `.be` text: the patch count; per patch 16 one-based indices of
 // its control points (u fastest, Spline.h:82); the point count; x, y, z per
@@ -165,5 +166,37 @@ cgs_status cgs_load_be(const char* text, size_t length, float** vertices, uint32
     *indices = idx;
     *npoints = nPoints;
     *npatches = nPatches;
+    return CGS_OK;
+}
+
+// Mesh::buildBVH (rt/Primitive.cu:90-108): per triangle its index, the box of
+// its three vertices and the centroid (1/3)·((p0 + p1) + p2), in FP32.  The
+// reference builds the mesh BVH from these with 16 triangles per leaf.
+cgs_status cgs_mesh_elements(const float* vertices, uint32_t nvertices, const uint32_t* indices, uint32_t ntriangles,
+                             cgs_bvh_element* elements)
+{
+    if (ntriangles && (!vertices || !indices || !elements))
+        return cgs::setError(CGS_ERR_INVALID, "cgs_mesh_elements: null argument");
+    for (size_t k = 0; k < 3 * (size_t)ntriangles; ++k)
+        if (indices[k] >= nvertices) return cgs::setError(CGS_ERR_INVALID, "cgs_mesh_elements: vertex index out of range");
+    const float third = 1.0f / 3.0f;
+    for (uint32_t i = 0; i < ntriangles; ++i)
+    {
+        const float* p[3] = {vertices + 3 * (size_t)indices[3 * (size_t)i], vertices + 3 * (size_t)indices[3 * (size_t)i + 1],
+                             vertices + 3 * (size_t)indices[3 * (size_t)i + 2]};
+        cgs_bvh_element& e = elements[i];
+        for (int c = 0; c < 3; ++c)
+        {
+            e.bmin[c] = 3.402823466e+38f;    // the empty box, then inflate by each vertex
+            e.bmax[c] = -3.402823466e+38f;
+            for (int v = 0; v < 3; ++v)
+            {
+                if (p[v][c] < e.bmin[c]) e.bmin[c] = p[v][c];
+                if (p[v][c] > e.bmax[c]) e.bmax[c] = p[v][c];
+            }
+            e.centroid[c] = third * ((p[0][c] + p[1][c]) + p[2][c]);
+        }
+        e.index = i;
+    }
     return CGS_OK;
 }

@@ -233,9 +233,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(INLINE ? CG
                         flags |= F_DONE;
                         break;
                     }
-                    // enter object i: local ray (RayTracer.cu:379-389)
+                    // enter object i: local ray (RayTracer.cu:379-389); an object
+                    // without geometry (its box is empty, so only a NaN ray gets
+                    // here) has nothing to test
                     obj = i;
                     const DevObject& ob = sc.objects[i];
+                    if (ob.surface == EMPTY) continue;
                     M4 M;
                     M.c[0] = ob.w2l[0]; M.c[1] = ob.w2l[1]; M.c[2] = ob.w2l[2]; M.c[3] = ob.w2l[3];
                     V3 lo = project4(m4mul(M, make_float4(o.x, o.y, o.z, 1.0f)));

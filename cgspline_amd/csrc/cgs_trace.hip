@@ -73,6 +73,7 @@ CGS_DEV ClosestHit sceneClosest(const SceneDev& sc, const NodeCache& nc, const R
     const BVHRef top = bvhRef(sc, sc.top);
     hashIntersect<true>(top, o, d, h.t, [&](uint32_t i) -> bool {
         const DevObject& ob = sc.objects[i];
+        if (ob.surface == EMPTY) return false;   // no geometry (empty box: only a NaN ray gets here)
         LocalRay lr = toLocal(ob, o, d, tmax);
         const DevBVH surf = sc.surfaces[ob.surface];
         const BVHRef b = bvhRef(sc, surf);
@@ -111,6 +112,7 @@ CGS_DEV bool sceneAny(const SceneDev& sc, const NodeCache& nc, const Rings& rg, 
     const BVHRef top = bvhRef(sc, sc.top);
     return hashIntersectAny<true>(top, o, d, tmax, [&](uint32_t i) -> bool {
         const DevObject& ob = sc.objects[i];
+        if (ob.surface == EMPTY) return false;   // no geometry (empty box: only a NaN ray gets here)
         LocalRay lr = toLocal(ob, o, d, tmax);
         const DevBVH surf = sc.surfaces[ob.surface];
         const BVHRef b = bvhRef(sc, surf);

@@ -105,7 +105,10 @@ typedef struct cgs_scene_desc
     const uint32_t* surf_vbase;   /* nsurf + 1, prefix over vertices */
     const uint32_t* surf_pbase;   /* nsurf + 1, prefix over patches */
     uint32_t nsurf;
-    const uint32_t* obj_surface;  /* nobj: surface of each object */
+    const uint32_t* obj_surface;  /* nobj: surface of each object; CGS_EMPTY: an object without
+                                   * geometry on this path (the reference's Mesh and Sphere, whose
+                                   * bounds() are empty, Primitive.cu:15-23,47-56): it takes part in
+                                   * the top-level BVH with an empty box and is never hit */
     const float* obj_local2world; /* nobj × 16, column-major */
     const float* obj_world2local; /* nobj × 16, column-major */
     const float* obj_material;    /* nobj × 8: diffuse rgb, specular rgb, metalness, roughness */
@@ -216,6 +219,12 @@ cgs_status cgs_bvh_build(const cgs_bvh_element* elements, uint32_t n, uint32_t e
 cgs_status cgs_patch_elements(const float* vertices, const uint32_t* indices, uint32_t npatch,
                               cgs_bvh_element* elements);
 void cgs_free_host(void* p);
+/* Mesh::buildBVH element data (rt/Primitive.cu:90-108): per triangle the box
+ * of its vertices and the centroid (1/3)((p0+p1)+p2).  vertices: nvertices
+ * × 3 host floats; indices: ntriangles × 3.  Build the mesh BVH from them with
+ * cgs_bvh_build(..., elements_per_node = 16), as the reference does. */
+cgs_status cgs_mesh_elements(const float* vertices, uint32_t nvertices, const uint32_t* indices, uint32_t ntriangles,
+                             cgs_bvh_element* elements);
 
 /* Surface::load_be (Surface.cpp:149-213): parses the text of a `.be` model
  * (the patch count; 16 one-based control-point indices per patch, u

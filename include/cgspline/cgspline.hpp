@@ -338,7 +338,7 @@ class Scene
 public:
     struct Object
     {
-        uint32_t surface;
+        uint32_t surface;   // CGS_EMPTY: a mesh (no geometry here; empty bounds, never hit, as in the reference)
         float local2world[16];
         float world2local[16];
         Material material;

@@ -86,19 +86,25 @@ inline int buildScene(Scene& s, uint32_t seed = 42)
 {
     for (auto& surf : s.surfaces)
         surf.bvh = bvhBuild(patchElements(&surf.vertices[0].x, surf.indices.data(), surf.patchCount()), 1);
-    // rt/Scene.cu:218-232 — objects bounded by their world-space control points
+    // rt/Scene.cu:39-53 — objects bounded by their world-space control points.
+    // An object without a surface (EMPTY: the reference's Mesh and Sphere,
+    // whose bounds() are empty, Primitive.cu:15-23,47-56) keeps the empty box:
+    // its centroid is the box centre (0) and no ray ever enters its leaf.
     std::vector<Element> el(s.objects.size());
     for (size_t i = 0; i < s.objects.size(); i++)
     {
         const Object& o = s.objects[i];
-        const Surface& surf = s.surfaces[o.surface];
         Element& e = el[i];
         e.index = (uint32_t)i;
         e.bounds = Box<float>();
-        for (uint32_t idx : surf.indices)
+        if (o.surface != EMPTY)
         {
-            f4 q = o.local2world * surf.vertices[idx];
-            e.bounds.inflate(q.xyz());
+            const Surface& surf = s.surfaces[o.surface];
+            for (uint32_t idx : surf.indices)
+            {
+                f4 q = o.local2world * surf.vertices[idx];
+                e.bounds.inflate(q.xyz());
+            }
         }
         e.centroid = e.bounds.center();
     }
@@ -201,6 +207,7 @@ inline int sceneIntersect(const Scene& sc, Hit& hit0, const Ray& ray0, uint32_t*
     {
         if (w) w->objects++;
         const Object& o = sc.objects[i];
+        if (o.surface == EMPTY) return false;   // no geometry (never reached: empty box)
         float d;
         Ray lr = localRay(o, ray0, d);
         Hit lh;
@@ -234,6 +241,7 @@ inline int sceneIntersectStack(const Scene& sc, Hit& hit0, const Ray& ray0, uint
     {
         const Object& o = sc.objects[i];
         float d;
+        if (o.surface == EMPTY) return false;   // no geometry (never reached: empty box)
         Ray lr = localRay(o, ray0, d);
         Hit lh;
         lh.t = lr.max;
@@ -280,6 +288,7 @@ inline bool sceneIntersectAny(const Scene& sc, const Ray& ray0, Work* w)
     {
         if (w) w->objects++;
         const Object& o = sc.objects[i];
+        if (o.surface == EMPTY) return false;   // no geometry (never reached: empty box)
         float d;
         Ray lr = localRay(o, ray0, d);
         return surfaceIntersectAny(sc.surfaces[o.surface], lr, w);

@@ -217,3 +217,32 @@ def test_render_matches_oracle(ctx, oracle, request, name):
     assert_bits_equal(color.reshape(-1, 3), ocolor.reshape(-1, 3), "color")
     ch = lambda f: np.stack([(f >> s) & 0xFF for s in (0, 8, 16, 24)], -1).astype(np.int32)
     assert np.abs(ch(frame) - ch(oframe)).max() <= 1
+
+
+def test_geometryless_objects_match_oracle(ctx, oracle, c2_small):
+    """A scene mixing surfaces with objects that have no geometry (the
+    reference's mesh and sphere primitives: empty bounds, never hit): the
+    top-level BVH and hash, the hits and the frame equal the oracle's."""
+    from test_oracle_bvh import with_geometryless_objects
+    from cgspline_amd import scenes
+    sc2, keep = with_geometryless_objects(c2_small.scene)
+    say("scene with geometry-less objects", sc2.nobj, "objects")
+    ds = ctx.scene(sc2)
+    osc = oracle.OracleScene(sc2)
+    g, o = ds.bvh(-1), osc.bvh(-1)
+    for key in ("nodes", "keys", "indices", "displacement", "table"):
+        assert np.array_equal(g[key], o[key]), key
+    rays = scenes.camera_rays(c2_small.camera)[::3]
+    say("trace_closest with geometry-less objects", rays.shape[0])
+    tuv, patch, obj = ds.trace_closest(rays)
+    otuv, opatch, oobj, _ = osc.trace(rays)
+    assert np.array_equal(obj, oobj) and np.array_equal(patch, opatch)
+    assert_bits_equal(tuv, otuv, "t,u,v")
+    assert not np.isin(obj, np.flatnonzero(~keep)).any()
+    wl = c2_small
+    say("render with geometry-less objects")
+    color, frame, counts = ds.render(wl.camera, eps=wl.eps, depth=wl.depth)
+    ocolor, oframe, _, orc = osc.render(wl.camera, eps=wl.eps, depth=wl.depth)
+    assert (counts["primary"], counts["shadow"], counts["secondary"]) == \
+        (orc["primary"], orc["shadow"], orc["secondary"])
+    assert_bits_equal(color.reshape(-1, 3), ocolor.reshape(-1, 3), "color")

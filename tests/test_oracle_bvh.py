@@ -233,3 +233,71 @@ def test_phf_fails_at_two_million_heap_keys(oracle):
     keys = heap_keys(1 << 20)
     st, p, _ = oracle.phf_build(keys)
     assert st == 1 and p["attempts"] == 16
+
+
+def with_geometryless_objects(sc, where=(0, 2, 5)):
+    """The scene with objects that have no surface (the reference's mesh and
+    sphere primitives, whose bounds are empty) inserted at `where`."""
+    import dataclasses
+    n = sc.nobj + len(where)
+    keep = np.ones(n, bool)
+    keep[list(where)] = False
+    def spread(a, fill):
+        out = np.empty((n,) + a.shape[1:], a.dtype)
+        out[keep] = a
+        out[~keep] = fill
+        return out
+    eye = np.eye(4, dtype=np.float32).reshape(16)
+    return dataclasses.replace(
+        sc, obj_surface=spread(sc.obj_surface, 0xFFFFFFFF),
+        obj_local2world=spread(sc.obj_local2world, eye), obj_world2local=spread(sc.obj_world2local, eye),
+        obj_material=spread(sc.obj_material, 0.5)), keep
+
+
+def test_geometryless_objects_are_never_hit(oracle, c2_small):
+    """Objects without geometry take part in the top-level build with an
+    empty box (Scene.cu:39-53 with Mesh::bounds() = {}): the top-level BVH
+    gains their leaves, and every ray hits what it hit without them."""
+    from cgspline_amd import scenes
+    sc = c2_small.scene
+    rays = scenes.camera_rays(c2_small.camera)[::7]
+    base = oracle.OracleScene(sc)
+    tuv0, p0, o0, _ = base.trace(rays)
+    sc2, keep = with_geometryless_objects(sc)
+    ext = oracle.OracleScene(sc2)
+    top = ext.bvh(-1)
+    assert top["nodes"].shape[0] == 2 * sc2.nobj - 1
+    tuv1, p1, o1, _ = ext.trace(rays)
+    ids = np.flatnonzero(keep)                  # new id of each original object
+    assert not np.isin(o1, np.flatnonzero(~keep)).any()
+    assert np.array_equal(np.where(o0 >= 0, ids[np.maximum(o0, 0)], -1), o1)
+    assert np.array_equal(p0, p1) and np.array_equal(tuv0.view(np.uint32), tuv1.view(np.uint32))
+    occ0, _ = base.trace(rays, any_hit=True)
+    occ1, _ = ext.trace(rays, any_hit=True)
+    assert np.array_equal(occ0, occ1)
+
+
+def test_mesh_elements_and_bvh_match_oracle(oracle):
+    """Mesh::buildBVH (rt/Primitive.cu:90-108): cgs_mesh_elements equals the
+    element data restated here in float32 (vertex box; centroid
+    (1/3)((p0+p1)+p2)); the 16-triangles-per-leaf BVH built from it equals the
+    oracle's, and each leaf holds at most 16 triangles."""
+    import cgspline_amd as cgs
+    r = rng(21)
+    V = r.normal(size=(400, 3)).astype(np.float32) * np.float32(3)
+    T = r.integers(0, 400, size=(700, 3)).astype(np.uint32)
+    el = cgs.Context.mesh_elements(V, T)
+    p0, p1, p2 = V[T[:, 0]], V[T[:, 1]], V[T[:, 2]]
+    assert np.array_equal(el[:, 0:3], np.minimum(np.minimum(p0, p1), p2))
+    assert np.array_equal(el[:, 3:6], np.maximum(np.maximum(p0, p1), p2))
+    assert np.array_equal(el[:, 6:9], np.float32(1.0 / 3.0) * ((p0 + p1) + p2))
+    assert np.array_equal(el[:, 9].view(np.uint32), np.arange(700, dtype=np.uint32))
+    n1, k1, i1 = cgs.Context.bvh_build(el, 16)
+    n2, k2, i2 = oracle.bvh_build_elements(el, 16)
+    assert np.array_equal(n1, n2) and np.array_equal(k1, k2) and np.array_equal(i1, i2)
+    leaves = n1.view(np.uint32).reshape(-1, 16)
+    assert (leaves[:, 12] == 0xFFFFFFFF).sum() >= 700 // 16
+    runs = np.diff(np.flatnonzero(np.append(i1 == 0xFFFFFFFF, True)))
+    assert runs.max() <= 17
+    with pytest.raises(cgs.CgsError, match="out of range"):
+        cgs.Context.mesh_elements(V[:3], np.array([[0, 1, 3]], np.uint32))

@@ -126,6 +126,30 @@ Scene {
     assert np.allclose(v[:3, 3], [1, 2, 3]) and np.allclose(v[:3, :3], [[0, 0, 1], [0, 1, 0], [-1, 0, 0]], atol=1e-7)
 
 
+def test_meshes_are_objects_without_geometry():
+    """`mesh:` objects (SceneReader.cpp:242-260, the built-in Box or a Mesh
+    asset) become objects without a surface (obj_surface = CGS_EMPTY): the
+    reference gives meshes empty bounds (Primitive.cu:47-56), so they shape
+    the top-level BVH and are never hit.  Order and transforms as for
+    surfaces."""
+    r = parse("""
+Surface { name:"w" file:"longwave" }
+let m = Material { name:"m" diffuse:(1 0 0) }
+Mesh { name:"tri" vertices:[0 0 0 1 0 0 0 1 0] normals:[0 0 1 0 0 1 0 0 1] triangles:[0 1 2] }
+Scene { objects:[ { name:"floor" mesh:{ name:"Box" material:m } transform:{ position:(0 1 0) } }
+                  { name:"s" surface:{ model:Get("w") } }
+                  { name:"t" mesh:{ name:"tri" } } ] }""")
+    assert r.object_names == ["floor", "s", "t"]
+    assert r.scene.obj_surface.tolist() == [0xFFFFFFFF, 0, 0xFFFFFFFF]
+    assert r.scene.nsurf == 1
+    assert np.array_equal(r.scene.obj_local2world[0].reshape(4, 4).T[:3, 3], [0, 1, 0])
+    assert np.array_equal(r.scene.obj_material[0, :3], [1, 0, 0])
+    # a scene of meshes only has nothing to trace
+    import cgspline_amd as cgs
+    with pytest.raises(cgs.CgsError, match="no surface objects"):
+        parse('Scene { objects:[ { mesh:{ name:"Box" } } ] }')
+
+
 def test_lights():
     r = one_object(obj="""light:{ color:(1 2 3 4) range:5 }
       objects:[ { name:"s" light:{ type:"spot" angle:30 } transform:{ position:(0 1 0) } } ]""")
@@ -145,7 +169,11 @@ def test_lights():
     ('Surface { name:"x" }', "surface data or source not provided"),
     ('Surface { file:"nope" }', "file 'nope' does not exist"),
     ('Surface { file:"x.se" }', "`.se` surfaces are not supported"),
-    ('Mesh { name:"m" }', "triangle meshes are not supported"),
+    ('Mesh { file:"box.obj" }', "not yet implemented"),
+    ('Mesh { vertices:[0 0 0 1 0 0] normals:[0 0 1] }', "`normals` size must match `vertices` size"),
+    ('Mesh { vertices:[0 0 0 1 0 0 0 1 0] triangles:[0 1 2.0] }', "triangle indices must be integers"),
+    ('Scene { objects:[ { mesh:{ model:Get("Box") } } ] }', 'property "name" does not exist'),
+    ('Scene { objects:[ { mesh:{ name:"Cone" } } ] }', 'property "Cone" does not exist'),
     ('From_sRGB(135 206 235)', "From_sRGB function: expected 3 or 4 integer arguments"),
     ('let a = *((1 2 3) (1 2 3))', "operator `*` cannot be called with argument types `vec3` and `vec3`"),
     ('let a = +(1 (1 2))', "operator `+` cannot be called with argument types `int` and `vec2`"),
