@@ -360,10 +360,13 @@ class DeviceScene:
     def trace_any_dev(self, rays, n, occluded, stream=None):
         check(lib().cgs_trace_any(self.ctx.h, self.h, ptr(rays), n, ptr(occluded), stream), "cgs_trace_any")
 
-    def render_dev(self, camera: "scenes.Camera", frame, color=None, counts=None, eps=1e-4, depth=0, stream=None):
+    def render_dev(self, camera: "scenes.Camera", frame, color=None, counts=None, eps=1e-4, depth=0, stream=None,
+                   stride=0):
+        """cgs_render into device buffers; `stride` is the frame's row pitch in
+        pixels (0: the width), pixel (x, y) at frame[stride*y + x]."""
         cam = _abi.Camera((C.c_float * 16)(*camera.to_world), (C.c_float * 3)(*camera.position), camera.fovy,
                           int(camera.flip_y))
-        op = _abi.RenderOptions(eps, depth)
+        op = _abi.RenderOptions(eps, depth, stride)
         return lib().cgs_render(self.ctx.h, self.h, C.byref(cam), C.byref(op), camera.width, camera.height,
                                 ptr(frame), ptr(color), ptr(counts), stream)
 

@@ -102,7 +102,7 @@ class Camera(C.Structure):
 
 
 class RenderOptions(C.Structure):
-    _fields_ = [("eps", _F32), ("recursion_depth", _I32)]
+    _fields_ = [("eps", _F32), ("recursion_depth", _I32), ("frame_stride", _U32)]
 
 
 class RayCounts(C.Structure):

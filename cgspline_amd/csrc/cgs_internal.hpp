@@ -88,6 +88,6 @@ hipError_t launchSubdivBatch(const cgs_ray* rays, const float* patches, uint32_t
 constexpr size_t kRenderArgsBytes = 256;
 hipError_t launchRender(const SceneDev& sc, const PersistArgs& pa, const float* toWorld16, const float* pos3, float tlx,
                         float tly, float hdx, float hdy, float eps, int depth, uint32_t w, uint32_t h, uint32_t* frame,
-                        float* color, unsigned long long* counts, void* argsDev, hipStream_t s);
+                        float* color, unsigned long long* counts, uint32_t stride, void* argsDev, hipStream_t s);
 
 } // namespace cgs

@@ -1062,6 +1062,8 @@ cgs_status cgs_render(cgs_ctx* ctx, cgs_scene* sc, const cgs_camera* cam, const 
 {
     if (!ctx || !sc || !cam || !op || !frame) return fail(CGS_ERR_INVALID, "cgs_render: null argument");
     if ((uint64_t)w * h >= (1ull << 32)) return fail(CGS_ERR_INVALID, "cgs_render: frame too large");
+    const uint32_t stride = op->frame_stride ? op->frame_stride : w;
+    if (stride < w) return fail(CGS_ERR_INVALID, "cgs_render: frame stride smaller than the width");
     HIP_TRY(hipSetDevice(ctx->device));
     std::lock_guard lk(ctx->mu);
     if (ctx->renderLaunched)
@@ -1089,7 +1091,8 @@ cgs_status cgs_render(cgs_ctx* ctx, cgs_scene* sc, const cgs_camera* cam, const 
         cgs_status pst = persistArgs(ctx, PersistKernel::Render, s, pa);
         if (pst != CGS_OK) return pst;
         HIP_TRY(launchRender(sc->view(), pa, cam->to_world, cam->position, tlx, tly, hdx, hdy, op->eps,
-                             op->recursion_depth, w, h, frame, color, (unsigned long long*)counts, ctx->dRenderArgs, s));
+                             op->recursion_depth, w, h, frame, color, (unsigned long long*)counts, stride,
+                             ctx->dRenderArgs, s));
         HIP_TRY(hipEventRecord(ctx->renderDone, s));
         ctx->renderLaunched = true;
         return CGS_OK;
@@ -1161,7 +1164,7 @@ cgs_status cgs_render(cgs_ctx* ctx, cgs_scene* sc, const cgs_camera* cam, const 
         cur = 1 - cur;
         nCur = nNext;
     }
-    HIP_TRY(launchPack(px, (uint32_t)npix, frame, color, s));
+    HIP_TRY(launchPack(px, (uint32_t)npix, w, stride, frame, color, s));
     HIP_TRY(hipEventRecord(ctx->renderDone, s));
     ctx->renderLaunched = true;
     return CGS_OK;

@@ -672,12 +672,13 @@ CGS_DEV uint32_t packSRGBS(float r, float g, float b)
 }
 CGS_DEV float sat(float x) { return x > 0.0f ? (x < 1.0f ? x : 1.0f) : 0.0f; }
 
-__global__ void kPack(PixelState px, uint32_t npix, uint32_t* __restrict__ frame, float* __restrict__ color)
+__global__ void kPack(PixelState px, uint32_t npix, uint32_t width, uint32_t stride, uint32_t* __restrict__ frame,
+                      float* __restrict__ color)
 {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= npix) return;
     const float4 a = px.acc[p];
-    frame[p] = packSRGBS(sat(a.x), sat(a.y), sat(a.z));
+    frame[(size_t)(p / width) * stride + p % width] = packSRGBS(sat(a.x), sat(a.y), sat(a.z));
     if (color)
     {
         color[3 * (size_t)p] = a.x;
@@ -793,10 +794,11 @@ hipError_t launchAdvanceAtt(const RayBatch& next, uint32_t n, const PixelState& 
     return hipGetLastError();
 }
 
-hipError_t launchPack(const PixelState& px, uint32_t npix, uint32_t* frame, float* color, hipStream_t s)
+hipError_t launchPack(const PixelState& px, uint32_t npix, uint32_t width, uint32_t stride, uint32_t* frame, float* color,
+                      hipStream_t s)
 {
     if (npix == 0) return hipSuccess;
-    hipLaunchKernelGGL(kPack, dim3(nb(npix, 256)), dim3(256), 0, s, px, npix, frame, color);
+    hipLaunchKernelGGL(kPack, dim3(nb(npix, 256)), dim3(256), 0, s, px, npix, width, stride, frame, color);
     return hipGetLastError();
 }
 

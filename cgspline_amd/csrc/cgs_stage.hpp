@@ -78,7 +78,8 @@ hipError_t launchScatterShadow(const RayBatch& shadow, uint32_t n, uint32_t* occ
                                hipStream_t s);
 hipError_t launchAccumulate(const RayBatch& rb, uint32_t n, const PixelState& px, const uint32_t* occ, hipStream_t s);
 hipError_t launchAdvanceAtt(const RayBatch& next, uint32_t n, const PixelState& px, hipStream_t s);
-hipError_t launchPack(const PixelState& px, uint32_t npix, uint32_t* frame, float* color, hipStream_t s);
+hipError_t launchPack(const PixelState& px, uint32_t npix, uint32_t width, uint32_t stride, uint32_t* frame, float* color,
+                      hipStream_t s);
 hipError_t launchZero(uint32_t* p, size_t n, hipStream_t s);
 
 } // namespace cgs

@@ -497,6 +497,7 @@ struct RenderDev
     float eps;
     int depth;
     uint32_t width, height;
+    uint32_t stride;      // frame row pitch in pixels (>= width)
 };
 
 // A render's camera, options and outputs, read by kRender from device memory
@@ -699,7 +700,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CGS_RENDER_
             color[3 * (size_t)q + 1] = acc.y;
             color[3 * (size_t)q + 2] = acc.z;
         }
-        ra->frame[q] = packSRGB(saturate(acc.x), saturate(acc.y), saturate(acc.z));
+        ra->frame[(size_t)j * ra->op.stride + i] = packSRGB(saturate(acc.x), saturate(acc.y), saturate(acc.z));
         nOverflow += overflow;
     }
     if (unsigned long long* counts = ra->counts)
@@ -785,7 +786,7 @@ hipError_t launchSubdivBatch(const cgs_ray* rays, const float* patches, uint32_t
 
 hipError_t launchRender(const SceneDev& sc, const PersistArgs& pa, const float* toWorld16, const float* pos3, float tlx,
                         float tly, float hdx, float hdy, float eps, int depth, uint32_t w, uint32_t h, uint32_t* frame,
-                        float* color, unsigned long long* counts, void* argsDev, hipStream_t s)
+                        float* color, unsigned long long* counts, uint32_t stride, void* argsDev, hipStream_t s)
 {
     CamDev cam;
     for (int c = 0; c < 4; c++)
@@ -798,7 +799,7 @@ hipError_t launchRender(const SceneDev& sc, const PersistArgs& pa, const float* 
     cam.hdx = hdx;
     cam.hdy = hdy;
     if ((uint64_t)w * h == 0) return hipSuccess;
-    const RenderArgs ra {cam, RenderDev {eps, depth, w, h}, frame, color, counts};
+    const RenderArgs ra {cam, RenderDev {eps, depth, w, h, stride}, frame, color, counts};
     static_assert(sizeof(RenderArgs) <= kRenderArgsBytes, "render argument block");
     hipError_t e = hipMemcpyAsync(argsDev, &ra, sizeof ra, hipMemcpyHostToDevice, s);
     if (e != hipSuccess) return e;

@@ -21,7 +21,7 @@
 extern "C" {
 #endif
 
-#define CGS_ABI_VERSION 1
+#define CGS_ABI_VERSION 2
 
 typedef enum cgs_status
 {
@@ -134,6 +134,8 @@ typedef struct cgs_render_options
 {
     float eps;             /* secondary-ray offset, 1e-4 */
     int32_t recursion_depth;
+    uint32_t frame_stride; /* pixels from one frame row to the next (rt::Frame::stride,
+                            * Frame.h:29-94, a row padded to an alignment); 0: the width */
 } cgs_render_options;
 
 /* Per-launch ray counters written by cgs_render (device memory, 4 × u64):
@@ -313,9 +315,11 @@ cgs_status cgs_trace_any(cgs_ctx* ctx, cgs_scene* scene, const cgs_ray* rays, ui
 /* RayTracer::render (rt/RayTracer.h:42-43, RayTracer.cu:181-254 and the
  * render kernel :268-618, non-Monte-Carlo): camera rays through integer
  * pixel centres, closest hit, PBR shading with one shadow ray per light and
- * the reflection bounce loop.  frame: width*height packed pixels (device,
- * row stride = width); color: optional width*height*3 floats (device,
- * unclamped radiance); counts: optional device cgs_ray_counts (accumulated).
+ * the reflection bounce loop.  frame: `height` rows of packed pixels
+ * (device), options->frame_stride (or width) pixels apart, pixel (x, y) at
+ * frame[stride*y + x] like rt::Frame::at; the padding is not written.
+ * color: optional width*height*3 floats (device, unclamped radiance, dense);
+ * counts: optional device cgs_ray_counts (accumulated).
  * Asynchronous on `stream`; CGS_ERR_BUSY while a previous render of this
  * context is still running (RayTracer.cu:184-190). */
 cgs_status cgs_render(cgs_ctx* ctx, cgs_scene* scene, const cgs_camera* camera,

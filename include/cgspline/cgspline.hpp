@@ -421,10 +421,21 @@ private:
 };
 
 // Packed-sRGB frame in device memory, row stride = width.
+// rt::Frame (Frame.h:29-94) over caller-owned device memory: pixel (x, y) at
+// data[stride*y + x]; stride 0 means the width.  alignTo gives the stride of
+// a Frame built with an alignment (a power of two), as the reference pads.
 struct Frame
 {
     uint32_t* data;
     uint32_t width, height;
+    uint32_t stride = 0;
+
+    static constexpr uint32_t alignTo(uint32_t n, uint32_t alignment)
+    {
+        if (!alignment) alignment = 1;
+        const uint32_t rem = n & (alignment - 1);
+        return rem ? n + (alignment - rem) : n;
+    }
 };
 
 // ---------------------------------------------------------------------------
@@ -455,7 +466,7 @@ public:
         for (int i = 0; i < 3; i++) c.position[i] = camera->position[i];
         c.fovy = camera->fieldOfView;
         c.flip_y = options.flipYAxis ? 1 : 0;
-        cgs_render_options o {options.eps, options.recursionDepth};
+        cgs_render_options o {options.eps, options.recursionDepth, frame->stride};
         cgs_status s = cgs_render(_dev.handle(), scene->handle(), &c, &o, frame->width, frame->height, frame->data,
                                   nullptr, deviceCounts, stream);
         if (s == CGS_ERR_BUSY)

@@ -34,7 +34,7 @@ def test_library_exports_every_declared_symbol():
 def test_abi_version_and_status_strings():
     import cgspline_amd
     lib = cgspline_amd.lib()
-    assert lib.cgs_abi_version() == 1
+    assert lib.cgs_abi_version() == 2
     assert lib.cgs_status_string(3) == b"hash table build failed"
     assert lib.cgs_status_string(7) == b"invalid"        # BVH::wrap's logic_error text
 

@@ -246,3 +246,26 @@ def test_geometryless_objects_match_oracle(ctx, oracle, c2_small):
     assert (counts["primary"], counts["shadow"], counts["secondary"]) == \
         (orc["primary"], orc["shadow"], orc["secondary"])
     assert_bits_equal(color.reshape(-1, 3), ocolor.reshape(-1, 3), "color")
+
+
+@pytest.mark.parametrize("pad", [13, 64])
+def test_render_into_padded_frame(ctx, oracle, c1, pad):
+    """rt::Frame rows padded to a stride (Frame.h:29-94): pixel (x, y) goes to
+    frame[stride*y + x]; the rows equal the dense render's, the padding is
+    left as it was."""
+    import cgspline_amd as cgs
+    wl = c1
+    w, h = wl.camera.width, wl.camera.height
+    stride = w + pad
+    say("render into a padded frame, stride", stride)
+    ds = ctx.scene(wl.scene)
+    _, dense, _ = ds.render(wl.camera, eps=wl.eps, depth=wl.depth)
+    sentinel = np.full(stride * h, 0x5A5A5A5A, np.uint32)
+    df = cgs.DeviceArray.from_host(ctx, sentinel)
+    cgs.check(ds.render_dev(wl.camera, df, eps=wl.eps, depth=wl.depth, stride=stride), "cgs_render")
+    ctx.sync()
+    out = df.to_host(np.uint32, (h, stride))
+    assert np.array_equal(out[:, :w], dense.reshape(h, w))
+    assert np.all(out[:, w:] == 0x5A5A5A5A)
+    with pytest.raises(cgs.CgsError, match="stride"):
+        cgs.check(ds.render_dev(wl.camera, df, eps=wl.eps, depth=wl.depth, stride=w - 1), "cgs_render")

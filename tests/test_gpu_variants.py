@@ -63,3 +63,20 @@ def test_variant_render_equals_default(ctx, variant_ctx, request, name):
     cb, fb, nb = vctx.scene(wl.scene).render(wl.camera, eps=wl.eps, depth=wl.depth)
     assert na == nb
     assert np.array_equal(ca.view(np.uint32), cb.view(np.uint32)) and np.array_equal(fa, fb)
+
+
+def test_variant_render_into_padded_frame(ctx, variant_ctx, c1):
+    """The variant writes padded frames like the default tracer."""
+    import cgspline_amd as cgs
+    vname, vctx = variant_ctx
+    wl = c1
+    w, h = wl.camera.width, wl.camera.height
+    stride = w + 7
+    say("padded render", vname, stride)
+    _, dense, _ = ctx.scene(wl.scene).render(wl.camera, eps=wl.eps, depth=wl.depth)
+    ds = vctx.scene(wl.scene)
+    df = cgs.DeviceArray.from_host(vctx, np.full(stride * h, 0x5A5A5A5A, np.uint32))
+    cgs.check(ds.render_dev(wl.camera, df, eps=wl.eps, depth=wl.depth, stride=stride), "cgs_render")
+    vctx.sync()
+    out = df.to_host(np.uint32, (h, stride))
+    assert np.array_equal(out[:, :w], dense.reshape(h, w)) and np.all(out[:, w:] == 0x5A5A5A5A)
