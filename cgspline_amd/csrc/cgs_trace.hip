@@ -48,22 +48,31 @@ struct ClosestHit
 
 // Context::intersect(hit, ray) (RayTracer.cu:369-423) with
 // rt::intersect(BezierSurface, hit, ray) (Primitive.cu:355-381) at the leaves.
-// Sibling rings of a lane for the two traversal levels (the top-level BVH
-// over objects and the patch BVH of one object), CGS_RING_SLOTS entries each.
-#ifndef CGS_RING_SLOTS
-#define CGS_RING_SLOTS 8
+// Sibling rings of a lane for the two traversal levels: the top-level BVH
+// over objects (CGS_RING_TOP entries; 4096 objects make it 13 levels deep)
+// and the patch BVH of one object (CGS_RING_BOT; 256 patches, 9 levels).
+#ifndef CGS_RING_TOP
+#define CGS_RING_TOP 8
+#endif
+#ifndef CGS_RING_BOT
+#define CGS_RING_BOT 8
+#endif
+// Octant-specialised node loops in the patch BVHs too (object-local rays:
+// lanes in different objects rarely share an octant).
+#ifndef CGS_BOT_OCTANTS
+#define CGS_BOT_OCTANTS false
 #endif
 struct Rings
 {
     SiblingRing top, bot;
 };
-constexpr uint32_t kRingWords = CGS_RING_SLOTS ? 2 * CGS_RING_SLOTS * kRingStride : 1;
+constexpr uint32_t kRingWords = CGS_RING_TOP + CGS_RING_BOT ? (CGS_RING_TOP + CGS_RING_BOT) * kRingStride : 1;
 
 CGS_DEV Rings laneRings(uint32_t* lds)
 {
-    if (!CGS_RING_SLOTS) return Rings {{nullptr, 0}, {nullptr, 0}};
     lds_u32* lane = (lds_u32*)lds + threadIdx.x;
-    return Rings {{lane, CGS_RING_SLOTS}, {lane + CGS_RING_SLOTS * kRingStride, CGS_RING_SLOTS}};
+    return Rings {{CGS_RING_TOP ? lane : nullptr, CGS_RING_TOP},
+                  {CGS_RING_BOT ? lane + CGS_RING_TOP * kRingStride : nullptr, CGS_RING_BOT}};
 }
 
 CGS_DEV ClosestHit sceneClosest(const SceneDev& sc, const NodeCache& nc, const Rings& rg, V3 o, V3 d, float tmax,
@@ -80,7 +89,7 @@ CGS_DEV ClosestHit sceneClosest(const SceneDev& sc, const NodeCache& nc, const R
         const ClipRay cr = makeClipRay(lr.o, versor3(lr.d));
         float lt = lr.max, lu = 0.0f, lv = 0.0f;
         uint32_t lp = EMPTY;
-        bool hit = hashIntersect(b, lr.o, lr.d, lt, [&](uint32_t pid) -> bool {
+        bool hit = hashIntersect<CGS_BOT_OCTANTS>(b, lr.o, lr.d, lt, [&](uint32_t pid) -> bool {
             const float4* S = sc.patches + 16 * (size_t)(surf.patchBase + pid);
             bool f = clipPatch<false>(S, cr, kClipTol, lt, lu, lv, stk, overflow);
             if (f) lp = pid;
@@ -117,7 +126,7 @@ CGS_DEV bool sceneAny(const SceneDev& sc, const NodeCache& nc, const Rings& rg, 
         const DevBVH surf = sc.surfaces[ob.surface];
         const BVHRef b = bvhRef(sc, surf);
         const ClipRay cr = makeClipRay(lr.o, versor3(lr.d));
-        return hashIntersectAny(b, lr.o, lr.d, lr.max, [&](uint32_t pid) -> bool {
+        return hashIntersectAny<CGS_BOT_OCTANTS>(b, lr.o, lr.d, lr.max, [&](uint32_t pid) -> bool {
             const float4* S = sc.patches + 16 * (size_t)(surf.patchBase + pid);
             float best = lr.max, bu, bv;
             return clipPatch<true>(S, cr, kClipTol, best, bu, bv, stk, overflow);
