@@ -55,6 +55,8 @@ def main():
                          "per job vs in one persistent launch; serving: continuous batching under Poisson load")
     ap.add_argument("--scenes", type=int, default=None,
                     help="batched/serving: number of resident scenes (default 64 batched, 8 serving)")
+    ap.add_argument("--slots", type=int, default=3, help="serving: launch slots (streams) of the server")
+    ap.add_argument("--max-batch-rays", type=int, default=1 << 22, help="serving: rays per launch")
     ap.add_argument("--scene-file", default=None,
                     help="plain mode on one of the reference's .dl scene files instead of the 1M-patch scene "
                          "(e.g. tests/golden/assets/scenes/Test152.dl), seen from the file's view")
@@ -357,8 +359,8 @@ def run_serving(args):
         return reqs
 
     budget = 8 << 30
-    max_batch = 1 << 22   # rays per launch: keeps two launches in flight so copies and tracing overlap
-    sv = Server(ctx, ds, hbm_budget=budget, slots=2, max_batch_rays=max_batch)
+    max_batch = args.max_batch_rays   # rays per launch: several launches in flight so copies and tracing overlap
+    sv = Server(ctx, ds, hbm_budget=budget, slots=args.slots, max_batch_rays=max_batch)
     warm = make_requests(8, 1)
     run_load(sv, warm, rate_hz=0.0)
     cap_reqs = make_requests(args.steps * 4, 2)
@@ -390,7 +392,7 @@ def run_serving(args):
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded displaced-grid scenes; camera and random rays; Poisson arrivals)",
-        "config": {"workload": "serving", "scenes": len(items), "hbm_budget_bytes": budget, "slots": 2,
+        "config": {"workload": "serving", "scenes": len(items), "hbm_budget_bytes": budget, "slots": args.slots,
                    "max_batch_rays": max_batch,
                    "request_rays": "log-uniform [1e3, 4e6]", "occlusion_fraction": 0.4},
         "loads": loads,

@@ -1698,7 +1698,7 @@ cgs_status cgs_server_create(cgs_ctx* ctx, cgs_scene* const* scenes, uint32_t ns
     }
     sv->budget = opt && opt->hbm_budget ? opt->hbm_budget : ((size_t)1 << 30);
     sv->maxBatchRays = opt ? opt->max_batch_rays : 0;
-    const uint32_t nslots = opt && opt->slots ? opt->slots : 2;
+    const uint32_t nslots = opt && opt->slots ? opt->slots : 3;
     {
         std::lock_guard lk(ctx->mu);
         cgs_status st = batchGrid(ctx, ctx->batchBlocksPerCU, sv->grid);

@@ -17,7 +17,7 @@ class Server:
     """cgs_server over the given DeviceScenes.  Requests are host arrays; the
     output arrays are owned by the returned request record until it is done."""
 
-    def __init__(self, ctx, scenes, hbm_budget: int = 1 << 30, slots: int = 2, max_batch_rays: int = 0):
+    def __init__(self, ctx, scenes, hbm_budget: int = 1 << 30, slots: int = 3, max_batch_rays: int = 0):
         self.ctx = ctx
         self.scenes = list(scenes)
         arr = (C.c_void_p * max(1, len(self.scenes)))(*[s.h for s in self.scenes])

@@ -361,7 +361,7 @@ cgs_status cgs_batch_autotune(cgs_ctx* ctx, const cgs_batch_job* jobs, uint32_t 
 typedef struct cgs_server_options
 {
     size_t hbm_budget;      /* bytes of device ray + result buffers in flight (0: 1 GiB) */
-    uint32_t slots;         /* launches in flight, each on its own stream (0: 2) */
+    uint32_t slots;         /* launches in flight, each on its own stream (0: 3) */
     uint32_t max_batch_rays;/* rays per launch (0: no limit beyond the budget) */
 } cgs_server_options;
 
