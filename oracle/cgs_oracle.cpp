@@ -210,6 +210,40 @@ int orc_pbr(int kind, const float* a, float* out)
     }
 }
 
+// The oracle's restatements of the absent Ds library's boundary functions,
+// for the tests that pin them.  kind 0: versor(v xyz) -> xyz;
+// 1: triangleIntersect(o, d, p0, p1, p2) -> ok, t, b1, b2, b0;
+// 2: packSRGB(r, g, b) -> the packed pixel (bits in out[0]);
+// 3: boxIntersect(mn, mx, o, d) -> ok, tmin, tmax.
+int orc_ds(int kind, const float* a, float* out)
+{
+    auto v3 = [&](int i) { return f3 {a[i], a[i + 1], a[i + 2]}; };
+    switch (kind)
+    {
+    case 0: { f3 r = versor(v3(0)); out[0] = r.x; out[1] = r.y; out[2] = r.z; return 0; }
+    case 1:
+    {
+        f3 uvw {0, 0, 0};
+        float t = 0;
+        const bool ok = triangleIntersect(v3(0), v3(3), v3(6), v3(9), v3(12), uvw, t);
+        out[0] = ok ? 1.0f : 0.0f; out[1] = t; out[2] = uvw.x; out[3] = uvw.y; out[4] = uvw.z;
+        return 0;
+    }
+    case 2: { const uint32_t q = packSRGB(a[0], a[1], a[2]); std::memcpy(out, &q, 4); return 0; }
+    case 3:
+    {
+        Box<float> b;
+        b.mn = v3(0);
+        b.mx = v3(3);
+        float tmin = 0, tmax = 0;
+        const bool ok = boxIntersect(b, v3(6), v3(9), tmin, tmax);
+        out[0] = ok ? 1.0f : 0.0f; out[1] = tmin; out[2] = tmax;
+        return 0;
+    }
+    }
+    return -1;
+}
+
 float orc_fminf(float a, float b) { return fmin_(a, b); }
 float orc_fmaxf(float a, float b) { return fmax_(a, b); }
 

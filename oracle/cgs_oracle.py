@@ -64,6 +64,7 @@ def lib():
         L.orc_x_axis_intersection.argtypes = [_P, _P, _P]
         L.orc_triangle_origin.argtypes = [_P, _P, _P, _P]
         L.orc_pbr.argtypes = [C.c_int, _P, _P]
+        L.orc_ds.argtypes = [C.c_int, _P, _P]
         L.orc_fminf.argtypes = [_F32, _F32]
         L.orc_fminf.restype = _F32
         L.orc_fmaxf.argtypes = [_F32, _F32]
@@ -167,6 +168,17 @@ def triangle_origin(P, A, B):
     out = np.zeros(2, np.float32)
     ok = lib().orc_triangle_origin(_ptr(P), _ptr(A), _ptr(B), _ptr(out))
     return bool(ok), out
+
+
+DS_KINDS = {"versor": 0, "triangle": 1, "srgb": 2, "box": 3}
+
+
+def ds(kind, *args):
+    """The restated Ds boundary functions (orc_ds); returns 8 floats."""
+    a = np.array(args, np.float32).ravel()
+    out = np.zeros(8, np.float32)
+    assert lib().orc_ds(DS_KINDS[kind], _ptr(a), _ptr(out)) == 0
+    return out
 
 
 PBR_KINDS = {"schlick": 0, "G": 1, "D": 2, "microfacet": 3, "light_color": 4, "light_vector": 5}
