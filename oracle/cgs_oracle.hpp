@@ -15,9 +15,11 @@
 //
 // Ds boundary.  The reference's vector/bounds/ray types come from the absent
 // Ds submodule.  This file defines them from first principles (see the
-// "Ds-boundary" comments); results that depend on those definitions are
-// "parity unpinned" with respect to the reference binary and are pinned here by
-// closed forms and invariants instead (tests/test_oracle_*.py).
+// "Ds-boundary" comments), and tests/test_oracle_ds.py pins them by
+// mathematics: versor, the slab test, Möller–Trumbore, the sRGB encode.
+// Parity unpinned: the empty box's convention (+max, -max), which only shows
+// where a box stays empty (the top-level element of an object without
+// geometry: centroid 0 here, NaN under ±infinity).
 //
 #pragma once
 
