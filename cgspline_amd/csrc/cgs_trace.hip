@@ -810,6 +810,8 @@ hipError_t launchRender(const SceneDev& sc, const PersistArgs& pa, const float* 
     if ((uint64_t)w * h == 0) return hipSuccess;
     const RenderArgs ra {cam, RenderDev {eps, depth, w, h, stride}, frame, color, counts};
     static_assert(sizeof(RenderArgs) <= kRenderArgsBytes, "render argument block");
+    // `ra` is pageable (the stack): HIP performs such a copy synchronously,
+    // so it has been read when the call returns.
     hipError_t e = hipMemcpyAsync(argsDev, &ra, sizeof ra, hipMemcpyHostToDevice, s);
     if (e != hipSuccess) return e;
     Persist ps {pa.stack, pa.lanes, pa.counter};
