@@ -29,7 +29,6 @@ import numpy as np
 BASE_SEED = 20261202
 
 _GRID_N = 16               # patches per side of a height-field grid
-_CTRL = 3 * _GRID_N + 1    # 49 control points per side
 
 
 @dataclasses.dataclass
@@ -145,25 +144,27 @@ def point_light(position, color) -> np.ndarray:
 # surfaces
 # ---------------------------------------------------------------------------
 
-def heightfield_grid(rng, amplitude=0.1):
-    """49×49 control grid over [0,1]², z = amp·Σ_k sin(2π(a_k x + b_k y) + φ_k),
-    made C1 across patch seams (z_i = (z_{i-1} + z_{i+1})/2 at i ≡ 0 mod 3,
-    along x then y).  Returns (vertices (2401,4) f32, indices (256,16) u32)."""
+def heightfield_grid(rng, amplitude=0.1, n=_GRID_N):
+    """(3n+1)² control grid over [0,1]² (n patches per side, 49×49 for the
+    default 16), z = amp·Σ_k sin(2π(a_k x + b_k y) + φ_k), made C1 across
+    patch seams (z_i = (z_{i-1} + z_{i+1})/2 at i ≡ 0 mod 3, along x then y).
+    Returns (vertices ((3n+1)²,4) f32, indices (n²,16) u32)."""
     a = rng.uniform(0.5, 2.0, 4)
     b = rng.uniform(0.5, 2.0, 4)
     ph = rng.uniform(0.0, 2 * np.pi, 4)
-    g = np.arange(_CTRL) / (_CTRL - 1)
+    ctrl = 3 * n + 1
+    g = np.arange(ctrl) / (ctrl - 1)
     x, y = np.meshgrid(g, g, indexing="xy")   # x varies along axis 1 (ix), y along axis 0 (iy)
     z = amplitude * np.sum(np.sin(2 * np.pi * (a[:, None, None] * x + b[:, None, None] * y)
                                   + ph[:, None, None]), axis=0)
-    for i in range(3, _CTRL - 1, 3):
+    for i in range(3, ctrl - 1, 3):
         z[:, i] = 0.5 * (z[:, i - 1] + z[:, i + 1])
-    for i in range(3, _CTRL - 1, 3):
+    for i in range(3, ctrl - 1, 3):
         z[i, :] = 0.5 * (z[i - 1, :] + z[i + 1, :])
     verts = np.stack([x, y, z, np.ones_like(z)], axis=-1).reshape(-1, 4).astype(np.float32)
-    I, J = np.meshgrid(np.arange(_GRID_N), np.arange(_GRID_N), indexing="xy")
+    I, J = np.meshgrid(np.arange(n), np.arange(n), indexing="xy")
     aa, bb = np.meshgrid(np.arange(4), np.arange(4), indexing="xy")   # aa = u (fast), bb = v
-    idx = (3 * I.reshape(-1, 1) + aa.reshape(1, -1)) + _CTRL * (3 * J.reshape(-1, 1) + bb.reshape(1, -1))
+    idx = (3 * I.reshape(-1, 1) + aa.reshape(1, -1)) + ctrl * (3 * J.reshape(-1, 1) + bb.reshape(1, -1))
     return verts, idx.astype(np.uint32)
 
 
@@ -325,6 +326,21 @@ def displaced_grids(seed=BASE_SEED + 3, grids=4096, width=2048, height=2048,
                   [point_light((c, -0.5 * box, 1.5 * box), (3e4, 3e4, 3e4))], name="displaced_grids")
     cam = make_camera((c, -0.85 * box, c), (c, c, c), (0.0, 0.0, 1.0), 60.0, width, height)
     return Workload(sc, cam, depth=1, desc=f"{grids} displaced grids ({grids * 256} patches), GGX, "
+                                            "primary + shadow + 1 reflection bounce")
+
+
+def grazing_heightfield(seed=BASE_SEED + 5, n=128, width=128, height=128) -> Workload:
+    """One n×n-patch height field (16,384 patches at n=128: a 14-level BVH)
+    seen at a grazing angle, so that rays find both children of a node hit at
+    many levels in a row and postpone up to 12 siblings at once.  Shading
+    adds shadow rays and one reflection bounce."""
+    rng = np.random.default_rng(seed)
+    v, i = heightfield_grid(rng, amplitude=0.01, n=n)
+    mat = np.array([0.8, 0.7, 0.6, 0.04, 0.04, 0.04, 0.0, 0.5], np.float32)
+    sc = assemble([(v, i)], [(0, IDENTITY16, IDENTITY16, mat)],
+                  [point_light((0.5, 2.0, 0.3), (4.0, 4.0, 4.0))], name="grazing_heightfield")
+    cam = make_camera((0.5, -0.1, 0.04), (0.5, 0.5, -0.02), (0.0, 0.0, 1.0), 40.0, width, height)
+    return Workload(sc, cam, depth=1, desc=f"{n * n} height-field patches at a grazing angle, "
                                             "primary + shadow + 1 reflection bounce")
 
 

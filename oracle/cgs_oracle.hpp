@@ -189,9 +189,11 @@ struct Work
     uint64_t onHits = 0;      // onHit evaluations
     uint64_t maxStack = 0;    // deepest clipping stack seen
     uint64_t objects = 0;     // top-level leaf element (object) tests
+    uint64_t maxPostponed = 0; // most siblings postponed at once in one traversal
     void add(const Work& o)
     {
         objects += o.objects;
+        maxPostponed = std::max(maxPostponed, o.maxPostponed);
         nodes += o.nodes; leaves += o.leaves; hashProbes += o.hashProbes;
         patchTests += o.patchTests; clipIters += o.clipIters; subdivs += o.subdivs;
         prunes += o.prunes; converged += o.converged; onHits += o.onHits;
@@ -1117,6 +1119,7 @@ bool hashIntersect(const BVHView& bv, Hit& hit, const Ray& ray, F&& fn, Work* w 
                 if (iR)
                 {
                     postponed |= 1;
+                    if (w) w->maxPostponed = std::max<uint64_t>(w->maxPostponed, std::popcount(postponed));
                     if (L0 < R0) { key = key << 1; node = bv.nodes + node->left; }
                     else { key = (key << 1) ^ 1u; node = bv.nodes + node->right; }
                 }
@@ -1172,6 +1175,7 @@ bool hashIntersectAny(const BVHView& bv, const Ray& ray, F&& fn, Work* w = nullp
                 if (iR)
                 {
                     postponed |= 1;
+                    if (w) w->maxPostponed = std::max<uint64_t>(w->maxPostponed, std::popcount(postponed));
                     if (L0 < R0) { key = key << 1; node = bv.nodes + node->left; }
                     else { key = (key << 1) ^ 1u; node = bv.nodes + node->right; }
                 }

@@ -57,5 +57,13 @@ def c3_small():
     return scenes.displaced_grids(grids=96, width=256, height=256, box=6.0)
 
 
+@pytest.fixture(scope="session")
+def grazing():
+    """A 128×128-patch height field at a grazing angle (128² rays): deep
+    enough that traversals postpone more than 8 siblings at once."""
+    from cgspline_amd import scenes
+    return scenes.grazing_heightfield()
+
+
 def rng(seed=0):
     return np.random.default_rng(seed)

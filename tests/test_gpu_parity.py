@@ -184,9 +184,9 @@ def test_trace_closest_displaced_grids_bitexact(ctx, oracle, c3_small):
     _trace_compare(ctx, oracle, c3_small, label="displaced grids")
 
 
-def test_trace_any_bitexact(ctx, oracle, c2_small):
-    ds, osc, rays, tuv, obj = _trace_compare(ctx, oracle, c2_small, label="closed surfaces (shadow set)")
-    light = c2_small.scene.lights[0, 4:7]
+def _shadow_rays(scene, rays, tuv, obj):
+    """Rays from each hit point towards the first light, ending at it."""
+    light = scene.lights[0, 4:7]
     hit = obj >= 0
     P = rays[hit, 0:3] + tuv[hit, 0:1] * rays[hit, 4:7]
     L = light[None, :] - P
@@ -196,15 +196,40 @@ def test_trace_any_bitexact(ctx, oracle, c2_small):
     sh[:, 0:3] = P + 1e-4 * L
     sh[:, 3] = d
     sh[:, 4:7] = L
+    return sh
+
+
+def _any_compare(ds, osc, sh):
     say("trace_any", sh.shape[0], "shadow rays")
     g = ds.trace_any(sh)
-    o, _ = osc.trace(sh, any_hit=True)
+    o, w = osc.trace(sh, any_hit=True)
     assert not np.any(g == 2)
     assert np.array_equal(g == 1, o)
+    return o, w
+
+
+def test_trace_any_bitexact(ctx, oracle, c2_small):
+    ds, osc, rays, tuv, obj = _trace_compare(ctx, oracle, c2_small, label="closed surfaces (shadow set)")
+    o, _ = _any_compare(ds, osc, _shadow_rays(c2_small.scene, rays, tuv, obj))
     assert 0 < o.sum() < o.size
 
 
-@pytest.mark.parametrize("name", ["c1", "c2_small", "c3_small"])
+def test_trace_past_the_sibling_ring_bitexact(ctx, oracle, grazing):
+    """Traversals that postpone more siblings at once than the device keeps
+    in its per-lane LDS ring (the 8 most recent) find the older ones through
+    the perfect hash, as the reference finds every sibling (rt/BVH.h:
+    178-266, 268-350): closest hits and shadow answers still equal the
+    oracle's.  The oracle's count of postponed siblings shows that the ring
+    overflowed."""
+    ds, osc, rays, tuv, obj = _trace_compare(ctx, oracle, grazing, label="grazing height field")
+    _, _, _, w = osc.trace(rays)
+    assert w["maxPostponed"] > 8, w
+    o, w_any = _any_compare(ds, osc, _shadow_rays(grazing.scene, rays, tuv, obj))
+    assert w_any["maxPostponed"] > 8, w_any
+    assert 0 < o.sum() < o.size
+
+
+@pytest.mark.parametrize("name", ["c1", "c2_small", "c3_small", "grazing"])
 def test_render_matches_oracle(ctx, oracle, request, name):
     wl = request.getfixturevalue(name)
     say("render", name, wl.camera.width, wl.camera.height, "depth", wl.depth)

@@ -38,7 +38,7 @@ def variant_ctx(request):
     c.close()
 
 
-@pytest.mark.parametrize("name", ["c2_small", "c3_small"])
+@pytest.mark.parametrize("name", ["c2_small", "c3_small", "grazing"])
 def test_variant_trace_equals_default(ctx, variant_ctx, request, name):
     from cgspline_amd import scenes
     vname, vctx = variant_ctx
@@ -54,7 +54,7 @@ def test_variant_trace_equals_default(ctx, variant_ctx, request, name):
     assert np.array_equal(a.trace_any(rays), b.trace_any(rays))
 
 
-@pytest.mark.parametrize("name", ["c1", "c2_small", "c3_small"])
+@pytest.mark.parametrize("name", ["c1", "c2_small", "c3_small", "grazing"])
 def test_variant_render_equals_default(ctx, variant_ctx, request, name):
     vname, vctx = variant_ctx
     wl = request.getfixturevalue(name)

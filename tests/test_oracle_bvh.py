@@ -108,6 +108,24 @@ def test_hashed_traversal_equals_stack_and_brute_force(oracle, c1):
     assert hit.sum() > 100
 
 
+def test_grazing_rays_postpone_many_siblings(oracle, grazing):
+    """The grazing height field, where traversals postpone more than 8
+    siblings at once: the hashed traversal still equals the stack traversal,
+    and the count of siblings postponed at once stays within the deepest
+    leaf's depth (one postponement per interior node on a root-to-leaf
+    path)."""
+    from cgspline_amd import scenes
+    osc = oracle.OracleScene(grazing.scene)
+    rays = scenes.camera_rays(grazing.camera)
+    h, p, o, w = osc.trace(rays)
+    hs, ps, os_, _ = osc.trace(rays, stack=True)
+    assert np.array_equal(o, os_) and np.array_equal(p, ps)
+    assert np.array_equal(h.view(np.uint32), hs.view(np.uint32))
+    depth = max(int(k).bit_length() - 1 for s in (0, -1) for k in osc.bvh(s)["keys"])
+    assert 8 < w["maxPostponed"] <= depth, (w["maxPostponed"], depth)
+    assert (o >= 0).sum() > rays.shape[0] // 4
+
+
 def test_any_hit_agrees_with_closest(oracle, c2_small):
     from cgspline_amd import scenes
     osc = oracle.OracleScene(c2_small.scene)
