@@ -25,7 +25,8 @@ def bits(a):
 
 def assert_bits_equal(a, b, what):
     ba, bb = bits(a), bits(b)
-    bad = np.flatnonzero((ba != bb).reshape(ba.shape[0], -1).any(axis=1)) if ba.ndim > 1 else np.flatnonzero(ba != bb)
+    bad = np.flatnonzero((ba != bb).reshape(ba.shape[0], int(np.prod(ba.shape[1:]))).any(axis=1)) if ba.ndim > 1 \
+        else np.flatnonzero(ba != bb)
     assert bad.size == 0, f"{what}: {bad.size} mismatches, first {bad[:5]}: {np.asarray(a)[bad[:3]]} vs {np.asarray(b)[bad[:3]]}"
 
 
