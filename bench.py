@@ -151,6 +151,20 @@ def main():
     log(f"{args.steps} steps: {ms_step:.2f} ms/step wall, kernel {np.mean(kernel_ms):.2f} ms "
         f"(min {np.min(kernel_ms):.2f}), {rays_per_step:.0f} rays/step -> {value:.1f} Mrays/s")
 
+    # Outside the timed region: the same frames with each one downloaded to
+    # pinned host memory after its render (what a caller that shows or saves
+    # every frame pays over PCIe).
+    host_frame = torch.empty(frame.numel(), dtype=torch.int32, pin_memory=True)
+    dl_steps = max(1, min(args.steps, 5))
+    torch.cuda.synchronize()
+    t_dl = time.perf_counter()
+    for _ in range(dl_steps):
+        step()
+        host_frame.copy_(frame, non_blocking=True)
+        torch.cuda.synchronize()
+    dl_ms = (time.perf_counter() - t_dl) * 1e3 / dl_steps
+    log(f"render + frame download ({frame.numel() * 4} B): {dl_ms:.2f} ms/frame")
+
     out = {
         "metric": metric,
         "value": round(value, 3),
@@ -175,6 +189,13 @@ def main():
             "primary_shadow_secondary_per_step": [int(c[0]) // args.steps, int(c[1]) // args.steps,
                                                   int(c[2]) // args.steps],
             "scene_build_ms": round(build_s * 1e3, 1),
+        },
+        "pcie_inclusive": {
+            "value": round(rays_per_step / (dl_ms / 1e3) / 1e6, 3),
+            "unit": "Mrays/s",
+            "ms_per_step": round(dl_ms, 4),
+            "frame_bytes": frame.numel() * 4,
+            "note": "render then frame download to pinned host memory, outside the timed region",
         },
     }
 
