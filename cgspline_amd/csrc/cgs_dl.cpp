@@ -507,8 +507,19 @@ private:
     static bool isOperator(int k) { return k == '+' || k == '-' || k == '*' || k == '/'; }
 
     // Value <- Expr / Dict / List / Literal
+    // Lists, dictionaries and call arguments nest through value(); a file
+    // nested deeper than this is refused rather than exhausting the stack.
+    static constexpr int kMaxNesting = 256;
+    int _nesting = 0;
+
     Value value()
     {
+        struct Leave
+        {
+            int& n;
+            ~Leave() { --n; }
+        } leave {++_nesting};
+        if (_nesting > kMaxNesting) error("values nested more than " + std::to_string(kMaxNesting) + " deep");
         const int k = cur().kind;
         if (k == T_ID || isOperator(k) || k == '.') return expr();
         if (k == '{') return dict();

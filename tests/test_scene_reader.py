@@ -199,6 +199,16 @@ def test_error_names_the_place():
         parse('Surface { name:"w" file:"longwave" }\n\nlet x = Nope\n')
 
 
+def test_deep_nesting_is_refused_not_a_crash():
+    """Nested lists 255 deep parse; 100,000 deep are refused with an error
+    instead of running the parser out of stack."""
+    import cgspline_amd as cgs
+    with pytest.raises(cgs.CgsError, match="the file defines no Scene"):
+        parse("let a = " + "[" * 255 + "]" * 255)
+    with pytest.raises(cgs.CgsError, match="values nested more than 256 deep"):
+        parse("let a = " + "[" * 100_000 + "]" * 100_000)
+
+
 def test_read_missing_file():
     import cgspline_amd as cgs
     with pytest.raises(cgs.CgsError, match="cannot open"):
