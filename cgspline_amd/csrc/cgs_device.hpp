@@ -582,6 +582,31 @@ CGS_DEV NodeBoxes loadNode(const float4* __restrict__ nodes, uint32_t i)
     return n;
 }
 
+// loadNode for a BVH whose nodes lie in the first 4 GiB of the node array
+// (the top-level BVH, placed first): 32-bit byte offsets from the array's
+// wave-uniform base, so each load takes the base in scalar registers and the
+// address costs one shift instead of 64-bit arithmetic.  The child offsets
+// go through an empty asm for the reason given in loadNode.
+CGS_DEV NodeBoxes loadNodeLow(const float4* __restrict__ nodes, uint32_t i)
+{
+    const char* base = reinterpret_cast<const char*>(nodes);
+    const uint32_t off = i * 64u;
+    const float4 a = *reinterpret_cast<const float4*>(base + off);
+    const float4 b = *reinterpret_cast<const float4*>(base + off + 16);
+    const float4 c = *reinterpret_cast<const float4*>(base + off + 32);
+    uint32_t offd = off + 48u;
+    asm("" : "+v"(offd));
+    const uint2 d = *reinterpret_cast<const uint2*>(base + offd);
+    NodeBoxes n;
+    n.lmin = {a.x, a.y, a.z};
+    n.lmax = {a.w, b.x, b.y};
+    n.rmin = {b.z, b.w, c.x};
+    n.rmax = {c.y, c.z, c.w};
+    n.left = d.x;
+    n.right = d.y;
+    return n;
+}
+
 // _node_intersect (rt/BVH.h:147-176).  The reference's "if (tn > tmin) tmin =
 // tn" and "if (tf < tmax) tmax = tf" are computed as maxNum/minNum: a NaN
 // operand (0·inf) leaves the bound unchanged in both forms, and a ±0
@@ -638,10 +663,11 @@ CGS_DEV NodeBoxes loadNodeLds(lds_float* lds, uint32_t k)
     return n;
 }
 
+template<bool LOW = false>
 CGS_DEV NodeBoxes loadNodeCached(const BVHRef& b, const NodeCache& nc, uint32_t node, uint32_t key)
 {
     if (key < nc.keys) return loadNodeLds(nc.lds, key);
-    return loadNode(b.nodes, node);
+    return LOW ? loadNodeLow(b.nodes, node) : loadNode(b.nodes, node);
 }
 
 CGS_DEV uint32_t hashKey(const BVHRef& b, uint32_t key)
@@ -721,13 +747,13 @@ CGS_DEV bool ascend(const BVHRef& b, const NodeCache& nc, const SiblingRing& rin
 // ascends: the per-lane order of node visits and leaf tests is the
 // reference's, while the loop itself stays small enough to keep its
 // registers, and lanes that reach leaves together test them together.
-template<int OCT = -1>
+template<int OCT = -1, bool LOW = false>
 CGS_DEV uint32_t nextLeaf(const BVHRef& b, const NodeCache& nc, const SiblingRing& ring, V3 o, V3 Dinv,
                           const float& limit, TravState& st)
 {
     while (true)
     {
-        const NodeBoxes n = loadNodeCached(b, nc, st.node, st.key);
+        const NodeBoxes n = loadNodeCached<LOW>(b, nc, st.node, st.key);
         if (n.left == EMPTY) return n.right;
         float L0, R0;
         const bool iL = nodeIntersect<OCT>(L0, n.lmin, n.lmax, o, Dinv) && L0 < limit;
@@ -759,20 +785,21 @@ CGS_DEV int waveOctant(V3 Dinv)
 
 // nextLeaf with the near/far planes fixed at compile time when the wave
 // shares a direction octant (a wave-uniform branch), per lane otherwise.
+template<bool LOW>
 CGS_DEV uint32_t nextLeafOct(int oct, const BVHRef& b, const NodeCache& nc, const SiblingRing& ring, V3 o, V3 Dinv,
                              const float& limit, TravState& st)
 {
     switch (oct)
     {
-    case 0: return nextLeaf<0>(b, nc, ring, o, Dinv, limit, st);
-    case 1: return nextLeaf<1>(b, nc, ring, o, Dinv, limit, st);
-    case 2: return nextLeaf<2>(b, nc, ring, o, Dinv, limit, st);
-    case 3: return nextLeaf<3>(b, nc, ring, o, Dinv, limit, st);
-    case 4: return nextLeaf<4>(b, nc, ring, o, Dinv, limit, st);
-    case 5: return nextLeaf<5>(b, nc, ring, o, Dinv, limit, st);
-    case 6: return nextLeaf<6>(b, nc, ring, o, Dinv, limit, st);
-    case 7: return nextLeaf<7>(b, nc, ring, o, Dinv, limit, st);
-    default: return nextLeaf<-1>(b, nc, ring, o, Dinv, limit, st);
+    case 0: return nextLeaf<0, LOW>(b, nc, ring, o, Dinv, limit, st);
+    case 1: return nextLeaf<1, LOW>(b, nc, ring, o, Dinv, limit, st);
+    case 2: return nextLeaf<2, LOW>(b, nc, ring, o, Dinv, limit, st);
+    case 3: return nextLeaf<3, LOW>(b, nc, ring, o, Dinv, limit, st);
+    case 4: return nextLeaf<4, LOW>(b, nc, ring, o, Dinv, limit, st);
+    case 5: return nextLeaf<5, LOW>(b, nc, ring, o, Dinv, limit, st);
+    case 6: return nextLeaf<6, LOW>(b, nc, ring, o, Dinv, limit, st);
+    case 7: return nextLeaf<7, LOW>(b, nc, ring, o, Dinv, limit, st);
+    default: return nextLeaf<-1, LOW>(b, nc, ring, o, Dinv, limit, st);
     }
 }
 
@@ -781,7 +808,8 @@ CGS_DEV uint32_t nextLeafOct(int oct, const BVHRef& b, const NodeCache& nc, cons
 // `leaf(index)` tests one element and returns whether it improved the hit.
 // OCTANTS: specialise the node loop on the wave's direction octant (worth its
 // code size where a wave's rays are coherent: the top level, in world space).
-template<bool OCTANTS = false, class Leaf>
+// LOW: the BVH's nodes lie in the first 4 GiB of the node array (loadNodeLow).
+template<bool OCTANTS = false, bool LOW = false, class Leaf>
 CGS_DEV bool hashIntersect(const BVHRef& b, V3 o, V3 dir, const float& limit, Leaf&& leaf,
                            const NodeCache& nc = NodeCache {nullptr, 0}, const SiblingRing& ring = {nullptr, 0})
 {
@@ -790,8 +818,8 @@ CGS_DEV bool hashIntersect(const BVHRef& b, V3 o, V3 dir, const float& limit, Le
     const int oct = OCTANTS ? waveOctant(Dinv) : -1;
     bool result = false;
     uint32_t i;
-    while ((i = (OCTANTS ? nextLeafOct(oct, b, nc, ring, o, Dinv, limit, st)
-                         : nextLeaf(b, nc, ring, o, Dinv, limit, st))) != EMPTY)
+    while ((i = (OCTANTS ? nextLeafOct<LOW>(oct, b, nc, ring, o, Dinv, limit, st)
+                         : nextLeaf<-1, LOW>(b, nc, ring, o, Dinv, limit, st))) != EMPTY)
     {
         uint32_t e;
         while ((e = b.indices[i]) != EMPTY)
@@ -805,7 +833,7 @@ CGS_DEV bool hashIntersect(const BVHRef& b, V3 o, V3 dir, const float& limit, Le
 }
 
 // Stackless any-hit traversal against `tmax` (rt/BVH.h:268-350).
-template<bool OCTANTS = false, class Leaf>
+template<bool OCTANTS = false, bool LOW = false, class Leaf>
 CGS_DEV bool hashIntersectAny(const BVHRef& b, V3 o, V3 dir, float tmax, Leaf&& leaf,
                               const NodeCache& nc = NodeCache {nullptr, 0}, const SiblingRing& ring = {nullptr, 0})
 {
@@ -813,8 +841,8 @@ CGS_DEV bool hashIntersectAny(const BVHRef& b, V3 o, V3 dir, float tmax, Leaf&& 
     TravState st {b.root, 1u, 0u, RingState {}};
     const int oct = OCTANTS ? waveOctant(Dinv) : -1;
     uint32_t i;
-    while ((i = (OCTANTS ? nextLeafOct(oct, b, nc, ring, o, Dinv, tmax, st)
-                         : nextLeaf(b, nc, ring, o, Dinv, tmax, st))) != EMPTY)
+    while ((i = (OCTANTS ? nextLeafOct<LOW>(oct, b, nc, ring, o, Dinv, tmax, st)
+                         : nextLeaf<-1, LOW>(b, nc, ring, o, Dinv, tmax, st))) != EMPTY)
     {
         uint32_t e;
         while ((e = b.indices[i]) != EMPTY)

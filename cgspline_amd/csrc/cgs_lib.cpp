@@ -868,12 +868,17 @@ cgs_status cgs_scene_create(cgs_ctx* ctx, const cgs_scene_desc* d, cgs_scene** o
         if (s != CGS_OK) return s;
     }
 
-    // 3. layout: absolute node offsets, leaf slots and patch bases
+    // 3. layout: absolute node offsets, leaf slots and patch bases.  The
+    // top-level BVH (index S) comes first in the node array: the kernels
+    // address its nodes with 32-bit byte offsets from the array's start.
+    if (sc->bvh[S].nodes.size() > (1u << 26))
+        return fail(CGS_ERR_INVALID, "cgs_scene_create: top-level BVH beyond 2^26 nodes");
     uint64_t nodeTotal = 0, indexTotal = 0, keyTotal = 0;
     sc->nodeBase.resize(S + 1);
     sc->indexBase.resize(S + 1);
-    for (uint32_t k = 0; k <= S; k++)
+    for (uint32_t j = 0; j <= S; j++)
     {
+        const uint32_t k = j == 0 ? S : j - 1;
         sc->nodeBase[k] = (uint32_t)nodeTotal;
         sc->indexBase[k] = (uint32_t)indexTotal;
         nodeTotal += sc->bvh[k].nodes.size();

@@ -80,7 +80,7 @@ CGS_DEV ClosestHit sceneClosest(const SceneDev& sc, const NodeCache& nc, const R
 {
     ClosestHit h {tmax, 0.0f, 0.0f, EMPTY, -1};
     const BVHRef top = bvhRef(sc, sc.top);
-    hashIntersect<true>(top, o, d, h.t, [&](uint32_t i) -> bool {
+    hashIntersect<true, true>(top, o, d, h.t, [&](uint32_t i) -> bool {
         const DevObject& ob = sc.objects[i];
         if (ob.surface == EMPTY) return false;   // no geometry (empty box: only a NaN ray gets here)
         LocalRay lr = toLocal(ob, o, d, tmax);
@@ -119,7 +119,7 @@ CGS_DEV bool sceneAny(const SceneDev& sc, const NodeCache& nc, const Rings& rg, 
                       const ClipStack& stk, bool& overflow)
 {
     const BVHRef top = bvhRef(sc, sc.top);
-    return hashIntersectAny<true>(top, o, d, tmax, [&](uint32_t i) -> bool {
+    return hashIntersectAny<true, true>(top, o, d, tmax, [&](uint32_t i) -> bool {
         const DevObject& ob = sc.objects[i];
         if (ob.surface == EMPTY) return false;   // no geometry (empty box: only a NaN ray gets here)
         LocalRay lr = toLocal(ob, o, d, tmax);
