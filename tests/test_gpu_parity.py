@@ -122,8 +122,12 @@ def heap_keys(npatch):
 
 
 @pytest.mark.parametrize("keys", [
-    "heap1", "heap2", "heap3", "heap256", "heap4096", "heap33000", "shuffled4096", "random3000"])
+    "heap1", "heap2", "heap3", "heap256", "heap4096", "heap33000", "shuffled4096", "random3000",
+    "heap524288", "heap1048576"])
 def test_phf_build_matches_oracle(ctx, oracle, keys):
+    """The GPU builder against the oracle, up to the largest BVHs: 2^19
+    patches (1,048,575 keys) hash after 7 attempts; 2^20 patches (2,097,151
+    keys) fail all 16, and both report the same status and parameters."""
     r = rng(5)
     if keys.startswith("heap"):
         k = heap_keys(int(keys[4:]))
