@@ -86,8 +86,13 @@ hipError_t launchSubdivBatch(const cgs_ray* rays, const float* patches, uint32_t
 // kRender reads its camera, options and output pointers from `argsDev`, a
 // device block of kRenderArgsBytes that launchRender fills on stream `s`.
 constexpr size_t kRenderArgsBytes = 256;
+// `order` (rank -> tile) sets the order in which the waves take the tiles
+// (null: row by row); `tileTicks`, when set, receives each tile's time.
 hipError_t launchRender(const SceneDev& sc, const PersistArgs& pa, const float* toWorld16, const float* pos3, float tlx,
                         float tly, float hdx, float hdy, float eps, int depth, uint32_t w, uint32_t h, uint32_t* frame,
-                        float* color, unsigned long long* counts, uint32_t stride, void* argsDev, hipStream_t s);
+                        float* color, unsigned long long* counts, uint32_t stride, const uint32_t* order,
+                        uint32_t* tileTicks, void* argsDev, hipStream_t s);
+// order[0..n) = the tiles 0..n-1, longest `ticks` first.
+hipError_t launchTileOrder(const uint32_t* ticks, uint32_t n, uint32_t* order, hipStream_t s);
 
 } // namespace cgs

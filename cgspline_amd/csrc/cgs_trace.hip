@@ -520,6 +520,8 @@ struct RenderArgs
     uint32_t* frame;
     float* color;
     unsigned long long* counts;
+    const uint32_t* order;   // rank -> tile, or null for row order
+    uint32_t* tileTicks;     // per tile: its wave's time in wall-clock ticks, or null
 };
 
 // PBR (rt/PBR.h, rt/PBR.cu)
@@ -625,8 +627,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CGS_RENDER_
         M4 C;
         C.c[0] = cam.c[0]; C.c[1] = cam.c[1]; C.c[2] = cam.c[2]; C.c[3] = cam.c[3];
         const V3 camPos {cam.px, cam.py, cam.pz};
-        // one wave = one 8×8 pixel tile
-        const uint32_t tile = base / 64;
+        // one wave = one 8×8 pixel tile, taken in the order of ra->order
+        const uint32_t rank = base / 64;
+        const uint32_t tile = ra->order ? ra->order[rank] : rank;
+        const unsigned long long tileStart = wall_clock64();
         const uint32_t i = (tile % tilesX) * 8 + (lane & 7), j = (tile / tilesX) * 8 + (lane >> 3);
         if (i >= op.width || j >= op.height) continue;
         bool overflow = false;
@@ -711,6 +715,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CGS_RENDER_
         }
         ra->frame[(size_t)j * ra->op.stride + i] = packSRGB(saturate(acc.x), saturate(acc.y), saturate(acc.z));
         nOverflow += overflow;
+        // The tile's time, for ordering the next render of this view.  Lane 0
+        // holds the tile's first pixel, which always lies inside the frame.
+        if (uint32_t* ticks = ra->tileTicks)
+        {
+            const unsigned long long dt = wall_clock64() - tileStart;
+            if (lane == 0) ticks[tile] = dt > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)dt;
+        }
     }
     if (unsigned long long* counts = ra->counts)
     {
@@ -795,7 +806,8 @@ hipError_t launchSubdivBatch(const cgs_ray* rays, const float* patches, uint32_t
 
 hipError_t launchRender(const SceneDev& sc, const PersistArgs& pa, const float* toWorld16, const float* pos3, float tlx,
                         float tly, float hdx, float hdy, float eps, int depth, uint32_t w, uint32_t h, uint32_t* frame,
-                        float* color, unsigned long long* counts, uint32_t stride, void* argsDev, hipStream_t s)
+                        float* color, unsigned long long* counts, uint32_t stride, const uint32_t* order,
+                        uint32_t* tileTicks, void* argsDev, hipStream_t s)
 {
     CamDev cam;
     for (int c = 0; c < 4; c++)
@@ -808,7 +820,7 @@ hipError_t launchRender(const SceneDev& sc, const PersistArgs& pa, const float* 
     cam.hdx = hdx;
     cam.hdy = hdy;
     if ((uint64_t)w * h == 0) return hipSuccess;
-    const RenderArgs ra {cam, RenderDev {eps, depth, w, h, stride}, frame, color, counts};
+    const RenderArgs ra {cam, RenderDev {eps, depth, w, h, stride}, frame, color, counts, order, tileTicks};
     static_assert(sizeof(RenderArgs) <= kRenderArgsBytes, "render argument block");
     // `ra` is pageable (the stack): HIP performs such a copy synchronously,
     // so it has been read when the call returns.
@@ -816,6 +828,46 @@ hipError_t launchRender(const SceneDev& sc, const PersistArgs& pa, const float* 
     if (e != hipSuccess) return e;
     Persist ps {pa.stack, pa.lanes, pa.counter};
     hipLaunchKernelGGL(kRender, dim3(pa.grid), dim3(256), 0, s, sc, ps, (const RenderArgs*)argsDev);
+    return hipGetLastError();
+}
+
+// Tile order for the next render of a view: longest first, by a counting
+// sort on the tile times (four buckets per octave; order within a bucket is
+// arbitrary).  One workgroup; every tile gets exactly one rank.
+constexpr int kOrderBuckets = 128;
+CGS_DEV uint32_t timeBucket(uint32_t t)
+{
+    if (t < 2) return 0;
+    const uint32_t l = 31 - __builtin_clz(t);
+    return 4 * l + (l >= 2 ? (t >> (l - 2)) & 3 : (t << (2 - l)) & 3);
+}
+
+__global__ __launch_bounds__(1024) void kTileOrder(const uint32_t* __restrict__ ticks, uint32_t n,
+                                                   uint32_t* __restrict__ order)
+{
+    __shared__ uint32_t count[kOrderBuckets];
+    for (uint32_t b = threadIdx.x; b < kOrderBuckets; b += blockDim.x) count[b] = 0;
+    __syncthreads();
+    for (uint32_t t = threadIdx.x; t < n; t += blockDim.x) atomicAdd(&count[timeBucket(ticks[t])], 1u);
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        uint32_t first = 0;   // the longest bucket first
+        for (int b = kOrderBuckets - 1; b >= 0; b--)
+        {
+            const uint32_t c = count[b];
+            count[b] = first;
+            first += c;
+        }
+    }
+    __syncthreads();
+    for (uint32_t t = threadIdx.x; t < n; t += blockDim.x) order[atomicAdd(&count[timeBucket(ticks[t])], 1u)] = t;
+}
+
+hipError_t launchTileOrder(const uint32_t* ticks, uint32_t n, uint32_t* order, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(kTileOrder, dim3(1), dim3(1024), 0, s, ticks, n, order);
     return hipGetLastError();
 }
 

@@ -278,6 +278,31 @@ def test_geometryless_objects_match_oracle(ctx, oracle, c2_small):
     assert_bits_equal(color.reshape(-1, 3), ocolor.reshape(-1, 3), "color")
 
 
+@pytest.mark.parametrize("name", ["c3_small", "grazing"])
+def test_repeated_renders_equal_the_first(ctx, oracle, request, name):
+    """Each render records its tiles' times and the next render of the same
+    view takes the tiles longest first.  The frame, the colours and the ray
+    counts do not depend on that order: renders 2 and 3 (reordered), a render
+    of another view in between (row order again) and the render after it all
+    equal the oracle's."""
+    import dataclasses
+    wl = request.getfixturevalue(name)
+    ds = ctx.scene(wl.scene)
+    osc = oracle.OracleScene(wl.scene)
+    ocolor, oframe, _, orc = osc.render(wl.camera, eps=wl.eps, depth=wl.depth)
+    other = dataclasses.replace(wl.camera, width=wl.camera.width // 2 + 3)
+    for k, cam in enumerate([wl.camera, wl.camera, wl.camera, other, wl.camera, wl.camera]):
+        say("render", name, "pass", k, cam.width, cam.height)
+        color, frame, counts = ds.render(cam, eps=wl.eps, depth=wl.depth)
+        if cam is other:
+            continue
+        assert (counts["primary"], counts["shadow"], counts["secondary"]) == \
+            (orc["primary"], orc["shadow"], orc["secondary"]), k
+        assert_bits_equal(color.reshape(-1, 3), ocolor.reshape(-1, 3), f"color, pass {k}")
+        ch = lambda f: np.stack([(f >> s) & 0xFF for s in (0, 8, 16, 24)], -1).astype(np.int32)
+        assert np.abs(ch(frame) - ch(oframe)).max() <= 1, k
+
+
 @pytest.mark.parametrize("pad", [13, 64])
 def test_render_into_padded_frame(ctx, oracle, c1, pad):
     """rt::Frame rows padded to a stride (Frame.h:29-94): pixel (x, y) goes to
