@@ -463,12 +463,12 @@ struct cgs_scene
     DeviceBuffer patches, nodes, indices, keys, displ, tables, surfaces, objects, lights, topByKey;
     float background[3], ambient[3];
     // Tile scheduling of the render: each render records its tiles' times,
-    // and the next render of the same view (camera, frame size, options)
-    // takes the tiles longest first.  Results do not depend on the order.
+    // and the next render of a frame of the same size takes the tiles longest
+    // first by them (an earlier view's order is still a good guess for a
+    // nearby one).  Results do not depend on the order.
     DeviceBuffer tileTicks, tileOrder;
-    uint32_t tileCap = 0;
+    uint32_t tileCap = 0, orderW = 0, orderH = 0;
     bool orderReady = false;
-    std::vector<float> orderView;
     SceneDev view() const
     {
         SceneDev s {};
@@ -1110,18 +1110,15 @@ cgs_status cgs_render(cgs_ctx* ctx, cgs_scene* sc, const cgs_camera* cam, const 
             sc->tileCap = nTiles;
             sc->orderReady = false;
         }
-        // the view that decides each tile's work
-        std::vector<float> view(cam->to_world, cam->to_world + 16);
-        view.insert(view.end(), cam->position, cam->position + 3);
-        view.insert(view.end(), {cam->fovy, (float)cam->flip_y, (float)w, (float)h, op->eps, (float)op->recursion_depth});
-        const bool ordered = sc->orderReady && view == sc->orderView;
+        const bool ordered = sc->orderReady && sc->orderW == w && sc->orderH == h;
         HIP_TRY(launchRender(sc->view(), pa, cam->to_world, cam->position, tlx, tly, hdx, hdy, op->eps,
                              op->recursion_depth, w, h, frame, color, (unsigned long long*)counts, stride,
                              ordered ? sc->tileOrder.as<uint32_t>() : nullptr, sc->tileTicks.as<uint32_t>(),
                              ctx->dRenderArgs, s));
         HIP_TRY(launchTileOrder(sc->tileTicks.as<uint32_t>(), nTiles, sc->tileOrder.as<uint32_t>(), s));
         sc->orderReady = true;
-        sc->orderView = std::move(view);
+        sc->orderW = w;
+        sc->orderH = h;
         HIP_TRY(hipEventRecord(ctx->renderDone, s));
         ctx->renderLaunched = true;
         return CGS_OK;

@@ -323,9 +323,10 @@ cgs_status cgs_trace_any(cgs_ctx* ctx, cgs_scene* scene, const cgs_ray* rays, ui
  * Asynchronous on `stream`; CGS_ERR_BUSY while a previous render of this
  * context is still running (RayTracer.cu:184-190).
  * Scheduling: each render records how long each 8×8 pixel tile took, and the
- * next render of the same view (camera, size, eps, depth) on this scene takes
- * the tiles longest first; the first render of a view takes them row by row.
- * The frame, the colours and the counts are the same in any order. */
+ * next render of a frame of the same size on this scene takes the tiles
+ * longest first by those times; a scene's first render at a size takes them
+ * row by row.  The frame, the colours and the counts are the same in any
+ * order. */
 cgs_status cgs_render(cgs_ctx* ctx, cgs_scene* scene, const cgs_camera* camera,
                       const cgs_render_options* options, uint32_t width, uint32_t height,
                       uint32_t* frame, float* color, cgs_ray_counts* counts, void* stream);

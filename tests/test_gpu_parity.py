@@ -280,21 +280,22 @@ def test_geometryless_objects_match_oracle(ctx, oracle, c2_small):
 
 @pytest.mark.parametrize("name", ["c3_small", "grazing"])
 def test_repeated_renders_equal_the_first(ctx, oracle, request, name):
-    """Each render records its tiles' times and the next render of the same
-    view takes the tiles longest first.  The frame, the colours and the ray
-    counts do not depend on that order: renders 2 and 3 (reordered), a render
-    of another view in between (row order again) and the render after it all
-    equal the oracle's."""
+    """Each render records its tiles' times and the next render of a frame
+    of the same size takes the tiles longest first.  The frame, the colours
+    and the ray counts do not depend on that order: renders 2 and 3
+    (reordered), renders of another size and of another view of the same size
+    in between, and the renders after them all equal the oracle's."""
     import dataclasses
     wl = request.getfixturevalue(name)
     ds = ctx.scene(wl.scene)
     osc = oracle.OracleScene(wl.scene)
     ocolor, oframe, _, orc = osc.render(wl.camera, eps=wl.eps, depth=wl.depth)
     other = dataclasses.replace(wl.camera, width=wl.camera.width // 2 + 3)
-    for k, cam in enumerate([wl.camera, wl.camera, wl.camera, other, wl.camera, wl.camera]):
+    moved = dataclasses.replace(wl.camera, position=wl.camera.position + np.float32(0.25))
+    for k, cam in enumerate([wl.camera, wl.camera, wl.camera, other, wl.camera, moved, wl.camera, wl.camera]):
         say("render", name, "pass", k, cam.width, cam.height)
         color, frame, counts = ds.render(cam, eps=wl.eps, depth=wl.depth)
-        if cam is other:
+        if cam is other or cam is moved:
             continue
         assert (counts["primary"], counts["shadow"], counts["secondary"]) == \
             (orc["primary"], orc["shadow"], orc["secondary"]), k
