@@ -292,10 +292,19 @@ def test_repeated_renders_equal_the_first(ctx, oracle, request, name):
     ocolor, oframe, _, orc = osc.render(wl.camera, eps=wl.eps, depth=wl.depth)
     other = dataclasses.replace(wl.camera, width=wl.camera.width // 2 + 3)
     moved = dataclasses.replace(wl.camera, position=wl.camera.position + np.float32(0.25))
-    for k, cam in enumerate([wl.camera, wl.camera, wl.camera, other, wl.camera, moved, wl.camera, wl.camera]):
+    first = {}
+    seq = [wl.camera, wl.camera, wl.camera, other, other, wl.camera, moved, wl.camera, wl.camera]
+    for k, cam in enumerate(seq):
         say("render", name, "pass", k, cam.width, cam.height)
         color, frame, counts = ds.render(cam, eps=wl.eps, depth=wl.depth)
         if cam is other or cam is moved:
+            # a frame whose width is not a multiple of 8 (edge tiles), reordered
+            # on its second render: equal to its first render
+            if id(cam) in first:
+                c0, f0, n0 = first[id(cam)]
+                assert n0 == counts and np.array_equal(f0, frame), k
+                assert np.array_equal(c0.view(np.uint32), color.view(np.uint32)), k
+            first[id(cam)] = (color, frame, counts)
             continue
         assert (counts["primary"], counts["shadow"], counts["secondary"]) == \
             (orc["primary"], orc["shadow"], orc["secondary"]), k
