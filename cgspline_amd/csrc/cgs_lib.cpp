@@ -1044,6 +1044,7 @@ cgs_status cgs_trace_closest(cgs_ctx* ctx, cgs_scene* sc, const cgs_ray* rays, u
                              void* stream)
 {
     if (!ctx || !sc || (n && (!rays || !hits))) return fail(CGS_ERR_INVALID, "cgs_trace_closest: null argument");
+    if (sc->ctx != ctx) return fail(CGS_ERR_INVALID, "cgs_trace_closest: scene of another context");
     HIP_TRY(hipSetDevice(ctx->device));
     std::lock_guard lk(ctx->mu);
     if (ctx->staged) return stageQuery(ctx, sc, rays, n, 0, hits, objects, nullptr, (hipStream_t)stream);
@@ -1058,6 +1059,7 @@ cgs_status cgs_trace_closest(cgs_ctx* ctx, cgs_scene* sc, const cgs_ray* rays, u
 cgs_status cgs_trace_any(cgs_ctx* ctx, cgs_scene* sc, const cgs_ray* rays, uint32_t n, uint32_t* occluded, void* stream)
 {
     if (!ctx || !sc || (n && (!rays || !occluded))) return fail(CGS_ERR_INVALID, "cgs_trace_any: null argument");
+    if (sc->ctx != ctx) return fail(CGS_ERR_INVALID, "cgs_trace_any: scene of another context");
     HIP_TRY(hipSetDevice(ctx->device));
     std::lock_guard lk(ctx->mu);
     if (ctx->staged) return stageQuery(ctx, sc, rays, n, 1, nullptr, nullptr, occluded, (hipStream_t)stream);
@@ -1073,6 +1075,7 @@ cgs_status cgs_render(cgs_ctx* ctx, cgs_scene* sc, const cgs_camera* cam, const 
                       uint32_t h, uint32_t* frame, float* color, cgs_ray_counts* counts, void* stream)
 {
     if (!ctx || !sc || !cam || !op || !frame) return fail(CGS_ERR_INVALID, "cgs_render: null argument");
+    if (sc->ctx != ctx) return fail(CGS_ERR_INVALID, "cgs_render: scene of another context");
     if ((uint64_t)w * h >= (1ull << 32)) return fail(CGS_ERR_INVALID, "cgs_render: frame too large");
     const uint32_t stride = op->frame_stride ? op->frame_stride : w;
     if (stride < w) return fail(CGS_ERR_INVALID, "cgs_render: frame stride smaller than the width");

@@ -173,3 +173,33 @@ def test_empty_inputs(ctx, c2_small):
     assert ds.trace_any(none).shape == (0,)
     assert ctx.bezier_clip(none, np.zeros((0, 16, 4), np.float32)).shape == (0, 4)
     assert ctx.subdivision(none, np.zeros((0, 16, 4), np.float32)).shape == (0, 4)
+
+
+def test_scene_of_another_context_is_refused(ctx, c1):
+    """A scene belongs to the context that created it: tracing or rendering
+    it through another context is refused before anything is launched."""
+    import ctypes as C
+    import cgspline_amd as cgs
+    from cgspline_amd import _abi
+    other = cgs.Context(0)
+    try:
+        ds = other.scene(c1.scene)
+        rays = np.zeros((4, 8), np.float32)
+        rays[:, 6] = -1.0
+        dr = cgs.DeviceArray.from_host(ctx, rays)
+        dh = cgs.DeviceArray(ctx, 16 * 4)
+        say("trace and render through another context")
+        lib = _abi.lib()
+        invalid = 1
+        assert lib.cgs_trace_closest(ctx.h, ds.h, dr.dptr, 4, dh.dptr, None, None) == invalid
+        assert b"another context" in lib.cgs_last_error()
+        assert lib.cgs_trace_any(ctx.h, ds.h, dr.dptr, 4, dh.dptr, None) == invalid
+        cam = c1.camera
+        cd = _abi.Camera((C.c_float * 16)(*cam.to_world), (C.c_float * 3)(*cam.position), cam.fovy, int(cam.flip_y))
+        op = _abi.RenderOptions(1e-4, 0, 0)
+        df = cgs.DeviceArray(ctx, 4 * cam.width * cam.height)
+        assert lib.cgs_render(ctx.h, ds.h, C.byref(cd), C.byref(op), cam.width, cam.height, df.dptr,
+                              None, None, None) == invalid
+        assert b"another context" in lib.cgs_last_error()
+    finally:
+        other.close()
