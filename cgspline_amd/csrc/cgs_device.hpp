@@ -553,6 +553,7 @@ struct BVHRef
     uint32_t A, B, mask;
     FastMod T;
     uint32_t nodeCount;                  // size of `nodes`, bounds table lookups
+    uint32_t finiteBoxes;                // 1: every child box of an inner node is finite
 };
 
 struct NodeBoxes
@@ -616,23 +617,37 @@ CGS_DEV NodeBoxes loadNodeLow(const float4* __restrict__ nodes, uint32_t i)
 // OCT >= 0 states the sign of every component of Dinv for the whole wave
 // (bit i set: component i negative), so the near and far planes are chosen
 // at compile time; OCT < 0 reads the signs per lane.  Both compute the same
-// values.
+// values.  OCT >= 0 also drops the -inf / +inf seeds of tmin and tmax:
+// maxNum/minNum over the three axes equal the seeded chains unless all
+// three distances are NaN, and waveOctant picks OCT >= 0 only for a wave
+// whose every ray has an axis with a finite origin and a finite, non-zero
+// Dinv against a BVH of finite boxes, where that axis's distances are never
+// NaN.
 template<int OCT = -1>
 CGS_DEV bool nodeIntersect(float& tMin, V3 bmin, V3 bmax, V3 o, V3 Dinv)
 {
-    float tmin = -__builtin_inff(), tmax = __builtin_inff();
     const float b0[3] = {bmin.x, bmin.y, bmin.z}, b1[3] = {bmax.x, bmax.y, bmax.z};
     const float oo[3] = {o.x, o.y, o.z}, di[3] = {Dinv.x, Dinv.y, Dinv.z};
+    float tn[3], tf[3];
 #pragma unroll
     for (int i = 0; i < 3; i++)
     {
         const bool s = OCT >= 0 ? ((OCT >> i) & 1) != 0 : __builtin_signbit(di[i]);
         float ta = (b0[i] - oo[i]) * di[i];
         float tb = (b1[i] - oo[i]) * di[i];
-        float tn = s ? tb : ta;
-        float tf = s ? ta : tb;
-        tmin = __builtin_fmaxf(tmin, tn);
-        tmax = __builtin_fminf(tmax, tf);
+        tn[i] = s ? tb : ta;
+        tf[i] = s ? ta : tb;
+    }
+    float tmin, tmax;
+    if (OCT >= 0)
+    {
+        tmin = __builtin_fmaxf(__builtin_fmaxf(tn[0], tn[1]), tn[2]);
+        tmax = __builtin_fminf(__builtin_fminf(tf[0], tf[1]), tf[2]);
+    }
+    else
+    {
+        tmin = __builtin_fmaxf(__builtin_fmaxf(__builtin_fmaxf(-__builtin_inff(), tn[0]), tn[1]), tn[2]);
+        tmax = __builtin_fminf(__builtin_fminf(__builtin_fminf(__builtin_inff(), tf[0]), tf[1]), tf[2]);
     }
     tMin = tmin;
     return tmin < tmax && tmax > 0.0001f;
@@ -774,13 +789,20 @@ CGS_DEV uint32_t nextLeaf(const BVHRef& b, const NodeCache& nc, const SiblingRin
 
 // The direction octant shared by every active lane of the wave (bit i: Dinv
 // component i negative, by sign bit as nodeIntersect reads it), or -1 when
-// the lanes differ.  Rays of one 8×8 pixel tile nearly always share it.
-CGS_DEV int waveOctant(V3 Dinv)
+// the lanes differ, when some lane's ray has no axis with a finite origin
+// and a finite, non-zero Dinv, or when the BVH has a non-finite box (the
+// conditions under which nodeIntersect<OCT >= 0> equals the generic form).
+// Rays of one 8×8 pixel tile nearly always share an octant.
+CGS_DEV int waveOctant(V3 o, V3 Dinv, uint32_t finiteBoxes)
 {
     const int oct = (__builtin_signbit(Dinv.x) ? 1 : 0) | (__builtin_signbit(Dinv.y) ? 2 : 0) |
                     (__builtin_signbit(Dinv.z) ? 4 : 0);
+    auto axis = [](float oi, float di) {
+        return __builtin_isfinite(oi) && __builtin_isfinite(di) && di != 0.0f;
+    };
+    const bool ok = axis(o.x, Dinv.x) || axis(o.y, Dinv.y) || axis(o.z, Dinv.z);
     const int first = __builtin_amdgcn_readfirstlane(oct);
-    return __ballot(oct != first) == 0 ? first : -1;
+    return (__ballot(oct != first || !ok) == 0 && finiteBoxes) ? first : -1;
 }
 
 // nextLeaf with the near/far planes fixed at compile time when the wave
@@ -815,7 +837,7 @@ CGS_DEV bool hashIntersect(const BVHRef& b, V3 o, V3 dir, const float& limit, Le
 {
     const V3 Dinv {1.0f / dir.x, 1.0f / dir.y, 1.0f / dir.z};
     TravState st {b.root, 1u, 0u, RingState {}};
-    const int oct = OCTANTS ? waveOctant(Dinv) : -1;
+    const int oct = OCTANTS ? waveOctant(o, Dinv, b.finiteBoxes) : -1;
     bool result = false;
     uint32_t i;
     while ((i = (OCTANTS ? nextLeafOct<LOW>(oct, b, nc, ring, o, Dinv, limit, st)
@@ -839,7 +861,7 @@ CGS_DEV bool hashIntersectAny(const BVHRef& b, V3 o, V3 dir, float tmax, Leaf&& 
 {
     const V3 Dinv {1.0f / dir.x, 1.0f / dir.y, 1.0f / dir.z};
     TravState st {b.root, 1u, 0u, RingState {}};
-    const int oct = OCTANTS ? waveOctant(Dinv) : -1;
+    const int oct = OCTANTS ? waveOctant(o, Dinv, b.finiteBoxes) : -1;
     uint32_t i;
     while ((i = (OCTANTS ? nextLeafOct<LOW>(oct, b, nc, ring, o, Dinv, tmax, st)
                          : nextLeaf<-1, LOW>(b, nc, ring, o, Dinv, tmax, st))) != EMPTY)

@@ -989,6 +989,13 @@ cgs_status cgs_scene_create(cgs_ctx* ctx, const cgs_scene_desc* d, cgs_scene** o
         b.T = r.T;
         b.patchBase = k < S ? d->surf_pbase[k] : 0;
         b.Tm = fastmodM(r.T);
+        b.finiteBoxes = 1;
+        for (const cgs_bvh_node& n : sc->bvh[k].nodes)
+            if (n.left != CGS_EMPTY)
+                for (int q = 0; q < 3; q++)
+                    if (!std::isfinite(n.lmin[q]) || !std::isfinite(n.lmax[q]) ||
+                        !std::isfinite(n.rmin[q]) || !std::isfinite(n.rmax[q]))
+                        b.finiteBoxes = 0;
         sc->phf[k] = cgs_phf {r.A, r.B, j.mask, r.T, r.attempts, j.M};
     }
     HIP_TRY(sc->surfaces.alloc(S * sizeof(DevBVH)));

@@ -32,7 +32,8 @@ struct DevBVH
     uint32_t T;           // table size
     uint32_t patchBase;   // first patch of the surface (surfaces only)
     uint64_t Tm;          // fastmod multiplier for T
-    uint64_t pad;
+    uint32_t finiteBoxes; // 1: every child box of an inner node is finite
+    uint32_t pad;
 };
 static_assert(sizeof(DevBVH) == 48, "DevBVH layout");
 
@@ -102,6 +103,7 @@ CGS_DEV BVHRef bvhRef(const SceneDev& sc, const DevBVH& d)
     b.mask = d.mask;
     b.T = {d.Tm, d.T};
     b.nodeCount = sc.nodeCount;
+    b.finiteBoxes = d.finiteBoxes;
     return b;
 }
 
