@@ -203,3 +203,44 @@ def test_scene_of_another_context_is_refused(ctx, c1):
         assert b"another context" in lib.cgs_last_error()
     finally:
         other.close()
+
+
+def test_degenerate_rays_bitexact(ctx, oracle, c2_small):
+    """Rays with no usable axis: a zero direction (all three reciprocals
+    infinite) from origins on top-level box corners, NaN directions and
+    infinite origins.  For these the octant-specialised node loop would differ
+    from the reference's seeded slab test, so the wave falls back to the
+    generic loop; every result still equals the oracle's."""
+    sc = c2_small.scene
+    ds = ctx.scene(sc)
+    osc = oracle.OracleScene(sc)
+    planes = _box_planes(osc)
+    r = rng(47)
+    n = 512
+    o = np.stack([r.choice(planes[a], n) for a in range(3)], -1).astype(np.float32)
+    d = np.zeros((n, 3), np.float32)
+    d[r.random((n, 3)) < 0.5] = -0.0
+    rays = _rays(o, d)
+    _compare_closest(ds, osc, rays, "zero directions from box corners")
+    _compare_any(ds, osc, rays, "zero directions from box corners")
+    mixed = _rays(o, np.tile(np.float32([0.3, 0.4, 0.5]), (n, 1)))
+    mixed[::3, 4:7] = np.nan                      # NaN directions in every wave
+    mixed[1::5, 0] = np.inf                       # infinite origins
+    _compare_closest(ds, osc, mixed, "NaN directions and infinite origins")
+    _compare_any(ds, osc, mixed, "NaN directions and infinite origins")
+
+
+def test_non_finite_geometry_bitexact(ctx, oracle, c1):
+    """A control point at +inf makes the top-level box infinite: the
+    octant-specialised loop is not used for that BVH, and hits equal the
+    oracle's."""
+    import dataclasses
+    from cgspline_amd import scenes
+    v = c1.scene.vertices.copy()
+    v[c1.scene.indices[17, 5], 2] = np.inf
+    sc = dataclasses.replace(c1.scene, vertices=v)
+    ds = ctx.scene(sc)
+    osc = oracle.OracleScene(sc)
+    rays = scenes.camera_rays(c1.camera)[::7]
+    obj = _compare_closest(ds, osc, rays, "infinite control point")
+    assert (obj >= 0).sum() > 100
