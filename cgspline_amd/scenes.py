@@ -344,6 +344,37 @@ def grazing_heightfield(seed=BASE_SEED + 5, n=128, width=128, height=128) -> Wor
                                             "primary + shadow + 1 reflection bounce")
 
 
+def stacked_arcs(n=180_000):
+    """n objects stacked along +y, each a quarter-cylinder patch (axis y,
+    radius 1, height 1) whose box [0,1]×[k,k+1]×[0,1] a ray along +y at
+    x, z ≈ 0.3 crosses without touching the surface, and on top one bumped
+    square patch that the ray hits.  The top-level BVH over the objects is
+    about 18 levels deep and such a ray finds both children hit at every
+    level.  (n = 180,000: the top level's 359,999 heap keys hash; 2^17 of
+    them do not.)  Returns (scene, rays (64, 8))."""
+    k = 0.5523
+    arc = [(1.0, 0.0), (1.0, k), (k, 1.0), (0.0, 1.0)]
+    cyl = np.array([[x, j / 3.0, z, 1.0] for j in range(4) for (x, z) in arc], np.float32)
+    bump = np.array([[i / 3.0, 0.3 if (0 < i < 3 and 0 < j < 3) else 0.0, j / 3.0, 1.0]
+                     for j in range(4) for i in range(4)], np.float32)
+    idx = np.arange(16, dtype=np.uint32).reshape(1, 16)
+    mat = np.array([0.8, 0.8, 0.8, 0.04, 0.04, 0.04, 0.0, 0.5], np.float32)
+    objects = []
+    for i in range(n):
+        l2w, w2l = trs(np.array([0.0, float(i), 0.0]), np.eye(3), 1.0)
+        objects.append((1 if i == n - 1 else 0, l2w, w2l, mat))
+    sc = assemble([(cyl, idx), (bump, idx)], objects, [point_light((0.5, -2.0, 0.5), (1.0, 1.0, 1.0))],
+                  name="stacked_arcs")
+    rng = np.random.default_rng(BASE_SEED + 6)
+    rays = np.zeros((64, 8), np.float32)
+    rays[:, 0] = rng.uniform(0.2, 0.4, 64)
+    rays[:, 1] = -1.0
+    rays[:, 2] = rng.uniform(0.2, 0.4, 64)
+    rays[:, 3] = np.inf
+    rays[:, 5] = 1.0
+    return sc, rays
+
+
 @dataclasses.dataclass
 class BatchItem:
     scene: Scene

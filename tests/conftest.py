@@ -67,3 +67,11 @@ def grazing():
 
 def rng(seed=0):
     return np.random.default_rng(seed)
+
+
+@pytest.fixture(scope="session")
+def stacked():
+    """180,000 objects stacked along y and 64 rays through all of them
+    (scenes.stacked_arcs)."""
+    from cgspline_amd import scenes
+    return scenes.stacked_arcs()

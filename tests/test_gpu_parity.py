@@ -234,6 +234,26 @@ def test_trace_past_the_sibling_ring_bitexact(ctx, oracle, grazing):
     assert 0 < o.sum() < o.size
 
 
+def test_trace_deep_top_level_bitexact(ctx, oracle, stacked):
+    """180,000 stacked objects: rays postpone 17 top-level siblings at once,
+    so past the ring's 8 the traversal reads a sibling whose heap key is 512
+    or more (not staged in LDS) through the top level's perfect hash.  Hits
+    and occlusion equal the oracle's."""
+    sc, rays = stacked
+    say("scene create: stacked arcs", sc.nobj, "objects")
+    ds = ctx.scene(sc)
+    osc = oracle.OracleScene(sc)
+    say("trace_closest through", sc.nobj, "objects")
+    g_tuv, g_p, g_o = ds.trace_closest(rays)
+    o_tuv, o_p, o_o, w = osc.trace(rays)
+    assert w["maxPostponed"] >= 17
+    assert np.array_equal(g_o, o_o) and np.array_equal(g_p, o_p)
+    assert_bits_equal(g_tuv, o_tuv, "t,u,v")
+    assert (g_o == sc.nobj - 1).all()
+    say("trace_any through", sc.nobj, "objects")
+    _any_compare(ds, osc, rays)
+
+
 @pytest.mark.parametrize("name", ["c1", "c2_small", "c3_small", "grazing"])
 def test_render_matches_oracle(ctx, oracle, request, name):
     wl = request.getfixturevalue(name)

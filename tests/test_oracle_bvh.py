@@ -126,6 +126,21 @@ def test_grazing_rays_postpone_many_siblings(oracle, grazing):
     assert (o >= 0).sum() > rays.shape[0] // 4
 
 
+def test_stacked_objects_postpone_past_the_lds_levels(oracle, stacked):
+    """The stacked-arcs scene: rays along the stack find both children hit
+    at every level of the top-level BVH, 17 siblings postponed at once.  A
+    traversal that keeps the 8 most recent in a ring and the heap keys below
+    512 (9 levels) in LDS must then find a sibling at depth 9 or deeper
+    through the perfect hash.  Every ray ends on the top patch."""
+    sc, rays = stacked
+    osc = oracle.OracleScene(sc)
+    h, p, o, w = osc.trace(rays)
+    assert w["maxPostponed"] >= 17, w
+    assert (o == sc.nobj - 1).all()
+    occ, wa = osc.trace(rays, any_hit=True)
+    assert occ.all() and wa["maxPostponed"] >= 17
+
+
 def test_any_hit_agrees_with_closest(oracle, c2_small):
     from cgspline_amd import scenes
     osc = oracle.OracleScene(c2_small.scene)
