@@ -90,6 +90,33 @@ def test_batch_equals_separate_queries(ctx, small_batch):
     ctx.set_batch_config(64, 0)
 
 
+def test_batch_of_deep_traversals_equals_oracle(ctx, oracle, grazing, stacked):
+    """Jobs whose traversals overflow the sibling ring, in the batched
+    kernel, which stages no BVH levels in LDS (every sibling the ring has
+    overwritten comes from the perfect hash): grazing rays on the 128×128-patch
+    height field, and rays through 180,000 stacked objects, closest and any
+    hit.  Every job equals the oracle."""
+    from cgspline_amd import scenes
+    sc_s, rays_s = stacked
+    rays_g = scenes.camera_rays(grazing.camera)[::3]
+    items = [scenes.BatchItem(grazing.scene, rays_g), scenes.BatchItem(sc_s, rays_s),
+             scenes.BatchItem(sc_s, rays_s, any=True)]
+    say("batch of deep traversals", sum(i.rays.shape[0] for i in items), "rays")
+    jobs = upload(ctx, items)
+    scrub(ctx, jobs)
+    ctx.trace_batch(jobs)
+    got = results(ctx, jobs)
+    for it, g in zip(items, got):
+        osc = oracle.OracleScene(it.scene)
+        if it.any:
+            o, w = osc.trace(it.rays, any_hit=True)
+            assert np.array_equal(g == 1, o) and w["maxPostponed"] > 8
+        else:
+            h, p, o, w = osc.trace(it.rays)
+            assert np.array_equal(g[1], o) and np.array_equal(g[0][:, 3], p)
+            assert np.array_equal(g[0][:, :3], h.view(np.uint32)) and w["maxPostponed"] > 8
+
+
 def test_graph_replays_match_eager(ctx, small_batch):
     jobs = upload(ctx, small_batch)
     ref = reference(small_batch, jobs)
