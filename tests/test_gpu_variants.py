@@ -54,6 +54,20 @@ def test_variant_trace_equals_default(ctx, variant_ctx, request, name):
     assert np.array_equal(a.trace_any(rays), b.trace_any(rays))
 
 
+def test_variant_deep_top_level_equals_default(ctx, variant_ctx, stacked):
+    """Rays through 180,000 stacked objects (17 top-level siblings postponed
+    at once) give the default tracer's hits and occlusion."""
+    vname, vctx = variant_ctx
+    sc, rays = stacked
+    say("trace", vname, "stacked arcs")
+    a, b = ctx.scene(sc), vctx.scene(sc)
+    ha, pa, oa = a.trace_closest(rays)
+    hb, pb, ob = b.trace_closest(rays)
+    assert np.array_equal(pa, pb) and np.array_equal(oa, ob)
+    assert np.array_equal(ha.view(np.uint32), hb.view(np.uint32))
+    assert np.array_equal(a.trace_any(rays), b.trace_any(rays))
+
+
 @pytest.mark.parametrize("name", ["c1", "c2_small", "c3_small", "grazing"])
 def test_variant_render_equals_default(ctx, variant_ctx, request, name):
     vname, vctx = variant_ctx
