@@ -76,7 +76,7 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+        print(f"bench.py: the library runs on one GPU; --gpus {args.gpus} is not supported "
               "(rays are independent: the multi-GPU form replicates the scene and splits the frame).",
               file=sys.stderr)
         sys.exit(2)

@@ -2,7 +2,7 @@
 
 The oracle is a CPU restatement of the reference's ray–patch path
 (cgs_oracle.hpp).  Only tests/, ``__graft_entry__.smoke()`` and bench.py's
-cpu_baseline leg use it, as the checker; the product never does.
+cpu_baseline leg use it, as the reference results compare against; the product never does.
 """
 from __future__ import annotations
 
