@@ -318,7 +318,7 @@ class OracleScene:
         patch = np.zeros(n, np.uint32)
         obj = np.zeros(n, np.int32)
         w = Work()
-        th = threads or os.cpu_count() or 1
+        th = threads or min(16, os.cpu_count() or 1)
         mode = 1 if any_hit else (2 if stack else 0)
         lib().orc_scene_trace(self.h, _ptr(rays), n, mode, _ptr(hits), _ptr(patch), _ptr(obj), C.byref(w), th)
         if any_hit:
@@ -347,7 +347,7 @@ class OracleScene:
         frame = np.zeros((h, w), np.uint32)
         wk = Work()
         rc = np.zeros(3, np.uint64)
-        th = threads or os.cpu_count() or 1
+        th = threads or min(16, os.cpu_count() or 1)
         c = self._cam(cam)
         lib().orc_scene_render(self.h, C.byref(c), eps, depth, w, h, _ptr(color), _ptr(frame), C.byref(wk),
                                _ptr(rc), th, r0, r1)
@@ -361,7 +361,7 @@ class OracleScene:
         color = np.zeros((h, w, 3), np.float32)
         frame = np.zeros((h, w), np.uint32)
         rc = np.zeros(2, np.uint64)
-        th = threads or os.cpu_count() or 1
+        th = threads or min(16, os.cpu_count() or 1)
         c = self._cam(cam)
         sec = lib().orc_cpu_render(self.h, C.byref(c), eps, depth, w, h, _ptr(color), _ptr(frame), th, r0, r1, stride,
                                    _ptr(rc))

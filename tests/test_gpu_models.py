@@ -79,7 +79,8 @@ def test_teaset_render_matches_oracle(ctx, oracle, teaset):
     assert np.abs(ch(frame) - ch(oframe)).max() <= 1
 
 
-@pytest.mark.parametrize("name", ["Teaset02", "Test32", "Test152"])
+@pytest.mark.parametrize("name", ["Teaset01", "Teaset02", "teaset", "Test32", "Test56", "Test80", "Test104",
+                                  "Test128", "Test152"])
 def test_reference_scene_file_render_matches_oracle(ctx, oracle, name):
     """The reference's own scene files (read with cgs_dl_read): Test152 is
     152 objects, 51 point lights; the frame, colours and ray counts equal
@@ -97,3 +98,23 @@ def test_reference_scene_file_render_matches_oracle(ctx, oracle, name):
     assert_bits_equal(color.reshape(-1, 3), ocolor.reshape(-1, 3), "color")
     ch = lambda f: np.stack([(f >> s) & 0xFF for s in (0, 8, 16, 24)], -1).astype(np.int32)
     assert np.abs(ch(frame) - ch(oframe)).max() <= 1
+
+
+def test_test152_benchmark_frame_matches_oracle(ctx, oracle):
+    """The Test152 benchmark frame itself: 1920×1080, recursion depth 6, 91.6 M
+    rays.  Rendered twice (the second render takes the tiles longest first),
+    both equal the oracle's frame, colours and ray counts."""
+    from cgspline_amd import scenes
+    wl = scenes.from_dl(os.path.join(ASSETS, "scenes", "Test152.dl"), width=1920, height=1080, depth=6)
+    ds = ctx.scene(wl.scene)
+    say("oracle render of Test152 at 1920x1080, depth 6")
+    ocolor, oframe, _, orc = oracle.OracleScene(wl.scene).render(wl.camera, eps=wl.eps, depth=wl.depth)
+    ch = lambda f: np.stack([(f >> s) & 0xFF for s in (0, 8, 16, 24)], -1).astype(np.int32)
+    for k in range(2):
+        say("render Test152 at 1920x1080, depth 6, pass", k)
+        color, frame, counts = ds.render(wl.camera, eps=wl.eps, depth=wl.depth)
+        assert counts["overflow"] == 0
+        assert (counts["primary"], counts["shadow"], counts["secondary"]) == \
+            (orc["primary"], orc["shadow"], orc["secondary"])
+        assert_bits_equal(color.reshape(-1, 3), ocolor.reshape(-1, 3), f"color, pass {k}")
+        assert np.abs(ch(frame) - ch(oframe)).max() <= 1

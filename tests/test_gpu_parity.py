@@ -254,6 +254,27 @@ def test_trace_deep_top_level_bitexact(ctx, oracle, stacked):
     _any_compare(ds, osc, rays)
 
 
+def test_bench_frame_matches_oracle(ctx, oracle):
+    """The benchmark frame itself: 4,096 displaced grids (1,048,576 patches),
+    2048×2048, depth 1.  Rendered twice (the second render takes the tiles
+    longest first), both equal the oracle's frame, colours and ray counts."""
+    from cgspline_amd import scenes
+    wl = scenes.displaced_grids()
+    say("scene create: 1M-patch bench scene")
+    ds = ctx.scene(wl.scene)
+    say("oracle render of the bench frame")
+    ocolor, oframe, _, orc = oracle.OracleScene(wl.scene).render(wl.camera, eps=wl.eps, depth=wl.depth)
+    ch = lambda f: np.stack([(f >> s) & 0xFF for s in (0, 8, 16, 24)], -1).astype(np.int32)
+    for k in range(2):
+        say("render the bench frame, pass", k)
+        color, frame, counts = ds.render(wl.camera, eps=wl.eps, depth=wl.depth)
+        assert counts["overflow"] == 0
+        assert (counts["primary"], counts["shadow"], counts["secondary"]) == \
+            (orc["primary"], orc["shadow"], orc["secondary"])
+        assert_bits_equal(color.reshape(-1, 3), ocolor.reshape(-1, 3), f"color, pass {k}")
+        assert np.abs(ch(frame) - ch(oframe)).max() <= 1
+
+
 @pytest.mark.parametrize("name", ["c1", "c2_small", "c3_small", "grazing"])
 def test_render_matches_oracle(ctx, oracle, request, name):
     wl = request.getfixturevalue(name)
