@@ -250,6 +250,15 @@ class Context:
     def sync(self, stream=None):
         check(lib().cgs_stream_sync(self.h, stream), "cgs_stream_sync")
 
+    def create_stream(self) -> int:
+        """A non-blocking HIP stream on this context's device (cgs_stream_create)."""
+        s = C.c_void_p()
+        check(lib().cgs_stream_create(self.h, C.byref(s)), "cgs_stream_create")
+        return s.value
+
+    def destroy_stream(self, stream: int):
+        check(lib().cgs_stream_destroy(self.h, stream), "cgs_stream_destroy")
+
     # --- batched queries over resident scenes -----------------------------------
     @staticmethod
     def batch_jobs(jobs) -> "C.Array":
@@ -312,7 +321,15 @@ class Batch:
 
 
 class DeviceScene:
-    """A scene resident in HBM (cgs_scene_create)."""
+    """A scene resident in HBM (cgs_scene_create).  Any context on the same
+    device may trace or render it: `via(ctx)` gives a view through another
+    context."""
+
+    def via(self, ctx: Context) -> "DeviceScene":
+        """This resident scene used through another context (no copy)."""
+        v = object.__new__(DeviceScene)
+        v.ctx, v.scene, v.h, v._owner = ctx, self.scene, self.h, self
+        return v
 
     def __init__(self, ctx: Context, scene: "scenes.Scene"):
         self.ctx = ctx
@@ -329,6 +346,9 @@ class DeviceScene:
         self.h = p.value
 
     def destroy(self):
+        if getattr(self, "_owner", None) is not None:   # a view: the owner destroys
+            self.h = None
+            return
         if self.h is not None and self.ctx.h is not None:
             lib().cgs_scene_destroy(self.h)
         self.h = None

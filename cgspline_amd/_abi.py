@@ -39,6 +39,7 @@ EXPORTS = [
     "cgs_batch_get_config", "cgs_batch_autotune",
     "cgs_server_create", "cgs_server_submit", "cgs_server_poll", "cgs_server_wait", "cgs_server_destroy",
     "cgs_malloc", "cgs_free", "cgs_memcpy_h2d", "cgs_memcpy_d2h", "cgs_memset", "cgs_stream_sync",
+    "cgs_stream_create", "cgs_stream_destroy",
 ]
 
 
@@ -169,6 +170,8 @@ def lib():
             "cgs_memcpy_d2h": ([_P, _P, _P, _SZ, _P], C.c_int),
             "cgs_memset": ([_P, _P, C.c_int, _SZ, _P], C.c_int),
             "cgs_stream_sync": ([_P, _P], C.c_int),
+            "cgs_stream_create": ([_P, C.POINTER(_P)], C.c_int),
+            "cgs_stream_destroy": ([_P, _P], C.c_int),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)

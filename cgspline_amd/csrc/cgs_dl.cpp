@@ -756,7 +756,7 @@ private:
         return v;
     }
 
-    // SceneReader.cpp:215-240 and Surface::load (Surface.cpp:350-373): a
+    // SceneReader.cpp:215-242 and Surface::load (Surface.cpp:350-373): a
     // name ending in ".se" is a `.se` surface, anything else a `.be` model.
     Value createSurface(const List& a)
     {
@@ -832,7 +832,7 @@ private:
         return v;
     }
 
-    // SceneReader.cpp:242-260 (addMesh): `name` is required and names the
+    // SceneReader.cpp:244-263 (addMesh): `name` is required and names the
     // mesh asset unless `model` gives it.
     FlatObject meshObject(const Dict& d, const Mat4& world, const std::string& objectName)
     {

@@ -8,6 +8,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+#include <bit>
 #include <string>
 
 #include "../../include/cgs.h"
@@ -40,7 +42,18 @@ struct PhfResult
     uint32_t A, B, T, attempts, status;
 };
 
-inline uint32_t phfScratchWords(uint32_t M, uint32_t tcap) { return 13u * M + 2u * tcap; }
+constexpr uint32_t phfScratchWords(uint32_t M, uint32_t tcap) { return 13u * M + 2u * tcap; }
+
+// Displacement-table size of n keys: bit_floor(max(2, (n-1) >> 1))
+// (PerfectHashFunction.cu:234-248).
+constexpr uint32_t phfDisplSize(uint64_t n)
+{
+    int s = int(n - 1) >> 1;
+    if (s < 2) s = 2;
+    return std::bit_floor((unsigned)s);
+}
+// Table growth after a failed function: max(4, 0.05·n) (PerfectHashFunction.cu:251,481).
+constexpr uint32_t phfGrowthStep(uint64_t n) { return std::max<uint32_t>(4u, (uint32_t)(0.05f * (float)n)); }
 
 hipError_t launchPhfBuild(const PhfJob* jobs, uint32_t njobs, const uint32_t* keys, const uint32_t* ab, uint32_t* scratch,
                           uint32_t* displ, uint32_t* tables, PhfResult* results, hipStream_t s);

@@ -25,11 +25,12 @@
 #include <string>
 #include <thread>
 #include <unordered_map>
+#include <unordered_set>
 #include <vector>
 
 #include "cgs_internal.hpp"
+#include "cgs_limits.hpp"
 #include "cgs_scene_dev.hpp"
-#include "cgs_stage.hpp"
 
 using namespace cgs;
 
@@ -69,27 +70,18 @@ struct DeviceBuffer
     template<class T> T* as() const { return (T*)p; }
 };
 
-// Buffers of the staged tracer, grown on demand and reused across calls.
-struct StageBuffers
+// Tile scheduling state of one scene's renders through one context: each
+// render records its tiles' times, and the next render of a frame of the same
+// size takes the tiles longest first by them (an earlier view's order is still
+// a good guess for a nearby one).  Results do not depend on the order.  Kept
+// per context, so two contexts may render one scene at the same time.
+struct TileState
 {
-    static constexpr int kFields = 11;
-    size_t cap[3] = {0, 0, 0};            // slots of batch 0, 1 (closest) and 2 (shadow)
-    DeviceBuffer mem[3][kFields];
-    RayBatch batch[3] = {};
-    size_t listCap = 0;
-    DeviceBuffer lists[5];                // 0: batch-0 actives, 1: batch-1 actives, 2: shadow actives, 3/4: ping-pong
-    size_t reqCap = 0;
-    DeviceBuffer req;
-    DeviceBuffer counters;                // 32 u32 counters (+ u64 scratch)
-    size_t pixCap = 0;
-    uint32_t nlightCap = 0;
-    DeviceBuffer pix[7];
-    PixelState px = {};
-    uint32_t gridTraverse = 0, gridClip = 0, gridFinish = 0;
-    // staging policy: at most `maxRounds` traverse/clip rounds, and only while
-    // more than `tailLanes` × (in-place grid lanes) rays are active
-    int maxRounds = 1;
-    float tailLanes = 1.0f;
+    uint64_t scene = 0;            // cgs_scene::serial
+    DeviceBuffer ticks, order;
+    uint32_t cap = 0, w = 0, h = 0;
+    bool ready = false;
+    uint64_t used = 0;             // last use (eviction order)
 };
 
 struct cgs_ctx
@@ -98,10 +90,12 @@ struct cgs_ctx
     int numCUs = 0;
     hipEvent_t renderDone = nullptr;
     bool renderLaunched = false;
-    // Tracer: the persistent one-kernel path (default), or the staged
-    // wavefront path (traverse / compacted clip queue / finish kernels) when
-    // CGS_STAGED=1.  Both give the same results bit for bit.
-    bool staged = false;
+    // Persistent launches of this context share its region stacks, work
+    // counters and render argument block.  Each one waits (on the GPU) for
+    // the previous one, whatever stream either was issued on, so calls on
+    // different streams never overlap on those buffers.
+    hipEvent_t persistDone = nullptr;
+    bool persistLaunched = false;
     // batched queries: launch shape, job table, completion of the last launch
     uint32_t batchChunk = 64, batchBlocksPerCU = 0;
     DeviceBuffer batchTable;
@@ -114,15 +108,17 @@ struct cgs_ctx
     float2* dStack = nullptr;                // clipping region stacks of the resident lanes
     size_t stackLanes = 0;
     uint32_t grid[kPersistKernels] = {};
-    StageBuffers stage;
+    std::vector<std::unique_ptr<TileState>> tiles;
+    uint64_t tileClock = 0;
     unsigned int* hCount = nullptr;          // pinned host word for count read-backs
     std::mutex mu;
 };
 
-// Grid, stack slice and zeroed work counter for one persistent launch.
-// Launches of one context are serialized by the caller (they share the
-// stack buffer and counters).
-static cgs_status persistArgs(cgs_ctx* ctx, PersistKernel k, hipStream_t s, PersistArgs& pa)
+// Grid, stack slice and zeroed work counter for one persistent launch over
+// `items` work items.  The caller holds ctx->mu, launches the kernel on `s`
+// and then calls persistEnd.  The counter grows by 64 per wave fetch past
+// the last item, so items + 64 × (resident waves) must stay below 2^32.
+static cgs_status persistArgs(cgs_ctx* ctx, PersistKernel k, hipStream_t s, uint64_t items, PersistArgs& pa)
 {
     const int ki = (int)k;
     if (ctx->grid[ki] == 0)
@@ -131,6 +127,8 @@ static cgs_status persistArgs(cgs_ctx* ctx, PersistKernel k, hipStream_t s, Pers
         HIP_TRY(persistentGrid(k, bpc));
         ctx->grid[ki] = bpc * (uint32_t)ctx->numCUs;
     }
+    if (!counterFits(items, ctx->grid[ki], 64))
+        return fail(CGS_ERR_INVALID, "work of " + std::to_string(items) + " items exceeds the 32-bit work counter");
     const size_t lanes = (size_t)ctx->grid[ki] * 256;
     if (lanes > ctx->stackLanes)
     {
@@ -148,8 +146,40 @@ static cgs_status persistArgs(cgs_ctx* ctx, PersistKernel k, hipStream_t s, Pers
     pa.lanes = (uint32_t)lanes;
     pa.counter = ctx->dCounters + ki;
     pa.grid = ctx->grid[ki];
-    HIP_TRY(hipMemsetAsync(pa.counter, 0, sizeof(unsigned int), s));
+    if (ctx->persistLaunched) HIP_TRY(hipStreamWaitEvent(s, ctx->persistDone, 0));
+    HIP_TRY(launchFill(pa.counter, 0u, 1, s));
     return CGS_OK;
+}
+
+// Marks the end of a persistent launch on `s` (the next one waits for it).
+static cgs_status persistEnd(cgs_ctx* ctx, hipStream_t s)
+{
+    HIP_TRY(hipEventRecord(ctx->persistDone, s));
+    ctx->persistLaunched = true;
+    return CGS_OK;
+}
+
+// The tile state of `serial` in this context (created on first use; the
+// least recently used of more than 16 is dropped).
+static TileState& tileState(cgs_ctx* ctx, uint64_t serial)
+{
+    for (auto& t : ctx->tiles)
+        if (t->scene == serial)
+        {
+            t->used = ++ctx->tileClock;
+            return *t;
+        }
+    if (ctx->tiles.size() >= 16)
+    {
+        auto lru = std::min_element(ctx->tiles.begin(), ctx->tiles.end(),
+                                    [](const auto& a, const auto& b) { return a->used < b->used; });
+        ctx->tiles.erase(lru);
+    }
+    ctx->tiles.push_back(std::make_unique<TileState>());
+    TileState& t = *ctx->tiles.back();
+    t.scene = serial;
+    t.used = ++ctx->tileClock;
+    return t;
 }
 
 // ---------------------------------------------------------------------------
@@ -313,7 +343,7 @@ cgs_status buildBVH(std::vector<cgs_bvh_element>& el, uint32_t epn, BuiltBVH& ou
     return CGS_OK;
 }
 
-// BezierSurface::buildBVH element data (rt/Primitive.cu:332-353).
+// BezierSurface::buildBVH element data (rt/Primitive.cu:199-220).
 void patchElements(const float* verts, const uint32_t* idx, uint32_t npatch, cgs_bvh_element* out)
 {
     for (uint32_t i = 0; i < npatch; i++)
@@ -336,14 +366,13 @@ void patchElements(const float* verts, const uint32_t* idx, uint32_t npatch, cgs
 }
 
 // PerfectHashFunction's (A,B) draws (rt/PerfectHashFunction.h:45-46,
-// .cu:452-473): mt19937(seed), uniform_int_distribution<uint32_t>(0, n),
+// .cu:386-400): mt19937(seed), uniform_int_distribution<uint32_t>(0, n),
 // A = 0 replaced by 31, redrawn while gcd(n, A) > 1.  All 16 functions'
-// pairs, in order.
-std::vector<uint32_t> drawAB(uint32_t n, uint32_t seed)
+// pairs, in order, into ab[0..32).
+void drawAB(uint32_t n, uint32_t seed, uint32_t* ab)
 {
     std::mt19937 mt(seed);
     std::uniform_int_distribution<uint32_t> d(0, n);
-    std::vector<uint32_t> ab;
     for (int fn = 0; fn < 16; fn++)
     {
         uint32_t A, B;
@@ -353,20 +382,21 @@ std::vector<uint32_t> drawAB(uint32_t n, uint32_t seed)
             if (A == 0) A = 31;
             B = d(mt);
         } while (std::gcd((size_t)n, (size_t)A) > 1);
-        ab.push_back(A);
-        ab.push_back(B);
+        ab[2 * fn] = A;
+        ab[2 * fn + 1] = B;
     }
+}
+
+std::vector<uint32_t> drawAB(uint32_t n, uint32_t seed)
+{
+    std::vector<uint32_t> ab(32);
+    drawAB(n, seed, ab.data());
     return ab;
 }
 
-uint32_t displSize(uint64_t n)
-{
-    int s = int(n - 1) >> 1;
-    if (s < 2) s = 2;
-    return std::bit_floor((unsigned)s);
-}
+uint32_t displSize(uint64_t n) { return phfDisplSize(n); }
 
-uint32_t growthStep(uint64_t n) { return std::max<uint32_t>(4u, (uint32_t)(0.05f * (float)n)); }
+uint32_t growthStep(uint64_t n) { return phfGrowthStep(n); }
 
 template<class F> void parallelFor(size_t n, F&& f)
 {
@@ -452,9 +482,14 @@ uint64_t fastmodM(uint32_t T) { return 0xFFFFFFFFFFFFFFFFull / T + 1; }
 // ---------------------------------------------------------------------------
 // scene
 // ---------------------------------------------------------------------------
+static std::atomic<uint64_t> g_sceneSerial {0};
+
+// A resident scene.  Its device arrays are read-only after cgs_scene_create,
+// so any context on the same device may trace or render it.
 struct cgs_scene
 {
-    cgs_ctx* ctx = nullptr;
+    int device = 0;
+    uint64_t serial = 0;           // unique per scene of this process
     uint32_t nsurf = 0, nobj = 0, nlight = 0, nodeCount = 0;
     std::vector<BuiltBVH> bvh;     // per surface, then the top level (local offsets)
     std::vector<uint32_t> nodeBase, indexBase;
@@ -462,13 +497,6 @@ struct cgs_scene
     std::vector<cgs_phf> phf;
     DeviceBuffer patches, nodes, indices, keys, displ, tables, surfaces, objects, lights, topByKey;
     float background[3], ambient[3];
-    // Tile scheduling of the render: each render records its tiles' times,
-    // and the next render of a frame of the same size takes the tiles longest
-    // first by them (an earlier view's order is still a good guess for a
-    // nearby one).  Results do not depend on the order.
-    DeviceBuffer tileTicks, tileOrder;
-    uint32_t tileCap = 0, orderW = 0, orderH = 0;
-    bool orderReady = false;
     SceneDev view() const
     {
         SceneDev s {};
@@ -494,142 +522,6 @@ struct cgs_scene
     }
 };
 
-
-// ---------------------------------------------------------------------------
-// staged tracer driver
-// ---------------------------------------------------------------------------
-namespace
-{
-
-enum Counter
-{
-    C_LIST0 = 0, C_LIST1 = 1, C_LIST2 = 2, C_PING = 3, C_PONG = 4, C_REQ = 5, C_WORK_T = 6, C_WORK_C = 7,
-    C_SHADOW = 8, C_NEXT0 = 0, C_NEXT1 = 1,
-};
-
-cgs_status ensureBatch(cgs_ctx* ctx, int b, size_t n)
-{
-    StageBuffers& st = ctx->stage;
-    if (st.cap[b] >= n) return CGS_OK;
-    n = std::max<size_t>(n, 1024);
-    const size_t sizes[StageBuffers::kFields] = {16, 16, 16, 16, 16, 16, 16, 16, 4, 4, 4};
-    for (int f = 0; f < StageBuffers::kFields; f++) HIP_TRY(st.mem[b][f].alloc(n * sizes[f]));
-    RayBatch& rb = st.batch[b];
-    rb.od0 = st.mem[b][0].as<float4>();
-    rb.od1 = st.mem[b][1].as<float4>();
-    rb.hit = st.mem[b][2].as<float4>();
-    rb.top = st.mem[b][3].as<uint4>();
-    rb.bot = st.mem[b][4].as<uint4>();
-    rb.lr0 = st.mem[b][5].as<float4>();
-    rb.lr1 = st.mem[b][6].as<float4>();
-    rb.lh = st.mem[b][7].as<float4>();
-    rb.flags = st.mem[b][8].as<uint32_t>();
-    rb.obj = st.mem[b][9].as<uint32_t>();
-    rb.pixel = st.mem[b][10].as<uint32_t>();
-    st.cap[b] = n;
-    return CGS_OK;
-}
-
-cgs_status ensureLists(cgs_ctx* ctx, size_t n)
-{
-    StageBuffers& st = ctx->stage;
-    if (!st.counters.p)
-    {
-        HIP_TRY(st.counters.alloc(256));
-        HIP_TRY(hipMemset(st.counters.p, 0, 256));
-    }
-    if (!ctx->hCount) HIP_TRY(hipHostMalloc((void**)&ctx->hCount, 64, hipHostMallocDefault));
-    const size_t want = std::max<size_t>(n, 1024);
-    if (st.listCap < n)
-    {
-        for (auto& l : st.lists) HIP_TRY(l.alloc(want * 4));
-        st.listCap = want;
-    }
-    if (st.reqCap < n)
-    {
-        HIP_TRY(st.req.alloc(want * 8));
-        st.reqCap = want;
-    }
-    if (st.gridTraverse == 0)
-    {
-        uint32_t b0 = 1, b1 = 1, b2 = 1;
-        HIP_TRY(stageGrid(0, b0));
-        HIP_TRY(stageGrid(1, b1));
-        HIP_TRY(stageGrid(2, b2));
-        st.gridTraverse = b0 * (uint32_t)ctx->numCUs;
-        st.gridClip = b1 * (uint32_t)ctx->numCUs;
-        st.gridFinish = b2 * (uint32_t)ctx->numCUs;
-    }
-    // region stacks for the resident lanes of the clipping kernels
-    const size_t lanes = (size_t)std::max(st.gridClip, st.gridFinish) * 256;
-    if (lanes > ctx->stackLanes)
-    {
-        if (ctx->dStack)
-        {
-            HIP_TRY(hipDeviceSynchronize());
-            HIP_TRY(hipFree(ctx->dStack));
-            ctx->dStack = nullptr;
-            ctx->stackLanes = 0;
-        }
-        HIP_TRY(hipMalloc(&ctx->dStack, lanes * CGS_CLIP_STACK * kStackWords * sizeof(float2)));
-        ctx->stackLanes = lanes;
-    }
-    return CGS_OK;
-}
-
-unsigned int* counter(cgs_ctx* ctx, int c) { return ctx->stage.counters.as<unsigned int>() + c; }
-
-cgs_status readCount(cgs_ctx* ctx, int c, hipStream_t s, uint32_t& out)
-{
-    HIP_TRY(hipMemcpyAsync(ctx->hCount, counter(ctx, c), 4, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    out = *ctx->hCount;
-    return CGS_OK;
-}
-
-// Runs traverse/clip rounds over batch `b` starting from the active list in
-// lists[list] (count in counter `c`) while enough rays remain to fill the
-// GPU, then finishes the remaining rays with the in-place kernel.
-cgs_status stageTrace(cgs_ctx* ctx, const SceneDev& sc, int b, int list, int c, hipStream_t s, uint32_t nStart)
-{
-    StageBuffers& st = ctx->stage;
-    const RayBatch& rb = st.batch[b];
-    ClipQueue cq {st.req.as<uint2>(), counter(ctx, C_REQ)};
-    uint32_t* cur = st.lists[list].as<uint32_t>();
-    unsigned int* curCount = counter(ctx, c);
-    int ping = 3;
-    // rays one resident grid of the in-place kernel holds, scaled by the policy
-    const double tail = (double)st.gridFinish * 256.0 * st.tailLanes;
-    uint32_t left = nStart;
-    for (int round = 0; left > 0; round++)
-    {
-        if (round >= st.maxRounds || (double)left <= tail)
-        {
-            HIP_TRY(hipMemsetAsync(counter(ctx, C_WORK_T), 0, 4, s));
-            HIP_TRY(launchFinish(sc, rb, cur, curCount, counter(ctx, C_WORK_T), ctx->dStack, (uint32_t)ctx->stackLanes,
-                                 st.gridFinish, s));
-            break;
-        }
-        HIP_TRY(hipMemsetAsync(counter(ctx, C_REQ), 0, 4 * 3, s)); // req, work_t, work_c
-        HIP_TRY(launchTraverse(sc, rb, cur, curCount, cq, counter(ctx, C_WORK_T), st.gridTraverse, s));
-        uint32_t* nxt = st.lists[ping].as<uint32_t>();
-        unsigned int* nxtCount = counter(ctx, ping == 3 ? C_PING : C_PONG);
-        HIP_TRY(hipMemsetAsync(nxtCount, 0, 4, s));
-        HIP_TRY(launchClip(sc, rb, cq, ctx->dStack, (uint32_t)ctx->stackLanes, nxt, nxtCount, counter(ctx, C_WORK_C),
-                           st.gridClip, s));
-        cur = nxt;
-        curCount = nxtCount;
-        ping = ping == 3 ? 4 : 3;
-        cgs_status e = readCount(ctx, ping == 3 ? C_PONG : C_PING, s, left);
-        if (e != CGS_OK) return e;
-    }
-    return CGS_OK;
-}
-
-cgs_status stageQuery(cgs_ctx* ctx, cgs_scene* sc, const cgs_ray* rays, uint32_t n, int any, cgs_hit* hits,
-                      int32_t* objects, uint32_t* occluded, hipStream_t s);
-
-} // namespace
 
 extern "C" {
 
@@ -667,12 +559,10 @@ cgs_status cgs_open(int device, cgs_ctx** out)
     c->device = device;
     hipError_t e = hipEventCreateWithFlags(&c->renderDone, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->batchDone, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&c->persistDone, hipEventDisableTiming);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&c->numCUs, hipDeviceAttributeMultiprocessorCount, device);
     if (e == hipSuccess) e = hipMalloc(&c->dOverflow, 256);
     if (e == hipSuccess) e = hipMemset(c->dOverflow, 0, 256);
-    if (const char* v = std::getenv("CGS_STAGE_ROUNDS")) c->stage.maxRounds = std::atoi(v);
-    if (const char* v = std::getenv("CGS_STAGE_TAIL")) c->stage.tailLanes = (float)std::atof(v);
-    if (const char* v = std::getenv("CGS_STAGED")) c->staged = std::atoi(v) != 0;
     if (e == hipSuccess) e = hipMalloc(&c->dCounters, 256);
     if (e == hipSuccess) e = hipMemset(c->dCounters, 0, 256);
     if (e == hipSuccess) e = hipMalloc(&c->dRenderArgs, kRenderArgsBytes);
@@ -691,6 +581,7 @@ cgs_status cgs_close(cgs_ctx* ctx)
     (void)hipSetDevice(ctx->device);
     if (ctx->renderDone) (void)hipEventSynchronize(ctx->renderDone), (void)hipEventDestroy(ctx->renderDone);
     if (ctx->batchDone) (void)hipEventSynchronize(ctx->batchDone), (void)hipEventDestroy(ctx->batchDone);
+    if (ctx->persistDone) (void)hipEventSynchronize(ctx->persistDone), (void)hipEventDestroy(ctx->persistDone);
     (void)hipDeviceSynchronize();
     if (ctx->dOverflow) (void)hipFree(ctx->dOverflow);
     if (ctx->dCounters) (void)hipFree(ctx->dCounters);
@@ -707,11 +598,12 @@ cgs_status cgs_bezier_clip(cgs_ctx* ctx, const cgs_ray* rays, const float* patch
     if (!ctx || (n && (!rays || !patches || !hits))) return fail(CGS_ERR_INVALID, "cgs_bezier_clip: null argument");
     HIP_TRY(hipSetDevice(ctx->device));
     std::lock_guard lk(ctx->mu);
+    if (n == 0) return CGS_OK;
     PersistArgs pa;
-    cgs_status st = persistArgs(ctx, PersistKernel::ClipBatch, (hipStream_t)stream, pa);
+    cgs_status st = persistArgs(ctx, PersistKernel::ClipBatch, (hipStream_t)stream, n, pa);
     if (st != CGS_OK) return st;
     HIP_TRY(launchClipBatch(pa, rays, patches, n, tol, hits, (hipStream_t)stream));
-    return CGS_OK;
+    return persistEnd(ctx, (hipStream_t)stream);
 }
 
 cgs_status cgs_subdivision(cgs_ctx* ctx, const cgs_ray* rays, const float* patches, uint32_t n, float threshold,
@@ -815,7 +707,8 @@ cgs_status cgs_scene_create(cgs_ctx* ctx, const cgs_scene_desc* d, cgs_scene** o
             return fail(CGS_ERR_EMPTY_SPLIT, "cgs_scene_create: a surface has no patch");
     HIP_TRY(hipSetDevice(ctx->device));
     auto sc = std::make_unique<cgs_scene>();
-    sc->ctx = ctx;
+    sc->device = ctx->device;
+    sc->serial = ++g_sceneSerial;
     sc->nsurf = d->nsurf;
     sc->nobj = d->nobj;
     sc->nlight = d->nlight;
@@ -1008,7 +901,7 @@ cgs_status cgs_scene_create(cgs_ctx* ctx, const cgs_scene_desc* d, cgs_scene** o
 cgs_status cgs_scene_destroy(cgs_scene* scene)
 {
     if (!scene) return CGS_OK;
-    (void)hipSetDevice(scene->ctx->device);
+    (void)hipSetDevice(scene->device);
     (void)hipDeviceSynchronize();
     delete scene;
     return CGS_OK;
@@ -1033,7 +926,7 @@ cgs_status cgs_scene_bvh_copy(cgs_scene* sc, int32_t surface, cgs_bvh_node* node
     if (nodes) std::memcpy(nodes, b.nodes.data(), b.nodes.size() * sizeof(cgs_bvh_node));
     if (keys) std::memcpy(keys, b.keys.data(), b.keys.size() * 4);
     if (indices) std::memcpy(indices, b.indices.data(), b.indices.size() * 4);
-    HIP_TRY(hipSetDevice(sc->ctx->device));
+    HIP_TRY(hipSetDevice(sc->device));
     if (displacement)
         HIP_TRY(hipMemcpy(displacement, sc->displ.as<uint32_t>() + sc->dev[k].displBase, sc->phf[k].displacement_size * 4,
                           hipMemcpyDeviceToHost));
@@ -1051,38 +944,36 @@ cgs_status cgs_trace_closest(cgs_ctx* ctx, cgs_scene* sc, const cgs_ray* rays, u
                              void* stream)
 {
     if (!ctx || !sc || (n && (!rays || !hits))) return fail(CGS_ERR_INVALID, "cgs_trace_closest: null argument");
-    if (sc->ctx != ctx) return fail(CGS_ERR_INVALID, "cgs_trace_closest: scene of another context");
+    if (sc->device != ctx->device) return fail(CGS_ERR_INVALID, "cgs_trace_closest: scene on another device");
     HIP_TRY(hipSetDevice(ctx->device));
     std::lock_guard lk(ctx->mu);
-    if (ctx->staged) return stageQuery(ctx, sc, rays, n, 0, hits, objects, nullptr, (hipStream_t)stream);
     if (n == 0) return CGS_OK;
     PersistArgs pa;
-    cgs_status st = persistArgs(ctx, PersistKernel::TraceClosest, (hipStream_t)stream, pa);
+    cgs_status st = persistArgs(ctx, PersistKernel::TraceClosest, (hipStream_t)stream, n, pa);
     if (st != CGS_OK) return st;
     HIP_TRY(launchTraceClosest(sc->view(), pa, rays, n, hits, objects, ctx->dOverflow, (hipStream_t)stream));
-    return CGS_OK;
+    return persistEnd(ctx, (hipStream_t)stream);
 }
 
 cgs_status cgs_trace_any(cgs_ctx* ctx, cgs_scene* sc, const cgs_ray* rays, uint32_t n, uint32_t* occluded, void* stream)
 {
     if (!ctx || !sc || (n && (!rays || !occluded))) return fail(CGS_ERR_INVALID, "cgs_trace_any: null argument");
-    if (sc->ctx != ctx) return fail(CGS_ERR_INVALID, "cgs_trace_any: scene of another context");
+    if (sc->device != ctx->device) return fail(CGS_ERR_INVALID, "cgs_trace_any: scene on another device");
     HIP_TRY(hipSetDevice(ctx->device));
     std::lock_guard lk(ctx->mu);
-    if (ctx->staged) return stageQuery(ctx, sc, rays, n, 1, nullptr, nullptr, occluded, (hipStream_t)stream);
     if (n == 0) return CGS_OK;
     PersistArgs pa;
-    cgs_status st = persistArgs(ctx, PersistKernel::TraceAny, (hipStream_t)stream, pa);
+    cgs_status st = persistArgs(ctx, PersistKernel::TraceAny, (hipStream_t)stream, n, pa);
     if (st != CGS_OK) return st;
     HIP_TRY(launchTraceAny(sc->view(), pa, rays, n, occluded, ctx->dOverflow, (hipStream_t)stream));
-    return CGS_OK;
+    return persistEnd(ctx, (hipStream_t)stream);
 }
 
 cgs_status cgs_render(cgs_ctx* ctx, cgs_scene* sc, const cgs_camera* cam, const cgs_render_options* op, uint32_t w,
                       uint32_t h, uint32_t* frame, float* color, cgs_ray_counts* counts, void* stream)
 {
     if (!ctx || !sc || !cam || !op || !frame) return fail(CGS_ERR_INVALID, "cgs_render: null argument");
-    if (sc->ctx != ctx) return fail(CGS_ERR_INVALID, "cgs_render: scene of another context");
+    if (sc->device != ctx->device) return fail(CGS_ERR_INVALID, "cgs_render: scene on another device");
     if ((uint64_t)w * h >= (1ull << 32)) return fail(CGS_ERR_INVALID, "cgs_render: frame too large");
     const uint32_t stride = op->frame_stride ? op->frame_stride : w;
     if (stride < w) return fail(CGS_ERR_INVALID, "cgs_render: frame stride smaller than the width");
@@ -1095,7 +986,7 @@ cgs_status cgs_render(cgs_ctx* ctx, cgs_scene* sc, const cgs_camera* cam, const 
             return fail(CGS_ERR_BUSY, "CUDA ray tracer not ready: wait the previous work to finish or create a new instance");
         HIP_TRY(q);
     }
-    // camera setup (RayTracer.cu:221-237), on the host
+    // camera setup (RayTracer.cu:141-163), on the host
     const float halfHeight = std::tan(0.5f * cam->fovy);
     float hdy = halfHeight / (float)h;
     const float hdx = hdy;
@@ -1107,103 +998,30 @@ cgs_status cgs_render(cgs_ctx* ctx, cgs_scene* sc, const cgs_camera* cam, const 
         hdy = -hdy;
     }
     hipStream_t s = (hipStream_t)stream;
-    if (!ctx->staged)
+    if (w == 0 || h == 0) return CGS_OK;
+    PersistArgs pa;
+    cgs_status pst = persistArgs(ctx, PersistKernel::Render, s, renderItems(w, h), pa);
+    if (pst != CGS_OK) return pst;
+    const uint32_t nTiles = ((w + 7) / 8) * ((h + 7) / 8);
+    TileState& ts = tileState(ctx, sc->serial);
+    if (nTiles > ts.cap)
     {
-        PersistArgs pa;
-        cgs_status pst = persistArgs(ctx, PersistKernel::Render, s, pa);
-        if (pst != CGS_OK) return pst;
-        const uint32_t nTiles = ((w + 7) / 8) * ((h + 7) / 8);
-        if (nTiles > sc->tileCap)
-        {
-            HIP_TRY(sc->tileTicks.alloc((size_t)nTiles * 4));
-            HIP_TRY(sc->tileOrder.alloc((size_t)nTiles * 4));
-            sc->tileCap = nTiles;
-            sc->orderReady = false;
-        }
-        const bool ordered = sc->orderReady && sc->orderW == w && sc->orderH == h;
-        HIP_TRY(launchRender(sc->view(), pa, cam->to_world, cam->position, tlx, tly, hdx, hdy, op->eps,
-                             op->recursion_depth, w, h, frame, color, (unsigned long long*)counts, stride,
-                             ordered ? sc->tileOrder.as<uint32_t>() : nullptr, sc->tileTicks.as<uint32_t>(),
-                             ctx->dRenderArgs, s));
-        HIP_TRY(launchTileOrder(sc->tileTicks.as<uint32_t>(), nTiles, sc->tileOrder.as<uint32_t>(), s));
-        sc->orderReady = true;
-        sc->orderW = w;
-        sc->orderH = h;
-        HIP_TRY(hipEventRecord(ctx->renderDone, s));
-        ctx->renderLaunched = true;
-        return CGS_OK;
+        HIP_TRY(ts.ticks.alloc((size_t)nTiles * 4));
+        HIP_TRY(ts.order.alloc((size_t)nTiles * 4));
+        ts.cap = nTiles;
+        ts.ready = false;
     }
-    // The staged tracer keeps a pixel's pending lights in bits 0-30 of one
-    // word (bit 31 marks a hit) and its occluded lights in one word.
-    if (sc->nlight > 31)
-        return fail(CGS_ERR_INVALID, "cgs_render: the staged tracer (CGS_STAGED=1) handles at most 31 lights");
-    const size_t npix = (size_t)w * h;
-    const uint32_t nl = std::max<uint32_t>(1, sc->nlight);
-    cgs_status st;
-    if ((st = ensureBatch(ctx, 0, npix)) != CGS_OK || (st = ensureBatch(ctx, 1, npix)) != CGS_OK ||
-        (st = ensureBatch(ctx, 2, npix * nl)) != CGS_OK || (st = ensureLists(ctx, npix * nl)) != CGS_OK)
-        return st;
-    StageBuffers& S = ctx->stage;
-    if (S.pixCap < npix || S.nlightCap < nl)
-    {
-        const size_t sz[7] = {16, 16, 16, 16, 16 * (size_t)nl, 4, 4};
-        for (int f = 0; f < 7; f++) HIP_TRY(S.pix[f].alloc(npix * sz[f]));
-        S.pixCap = npix;
-        S.nlightCap = nl;
-    }
-    PixelState px;
-    px.acc = S.pix[0].as<float4>();
-    px.att = S.pix[1].as<float4>();
-    px.attNext = S.pix[2].as<float4>();
-    px.base = S.pix[3].as<float4>();
-    px.term = S.pix[4].as<float4>();
-    px.state = S.pix[5].as<uint32_t>();
-    px.nlight = sc->nlight;
-    uint32_t* occ = S.pix[6].as<uint32_t>();
-    const SceneDev view = sc->view();
-    CamDevS cd;
-    for (int c = 0; c < 4; c++)
-        cd.c[c] = make_float4(cam->to_world[4 * c], cam->to_world[4 * c + 1], cam->to_world[4 * c + 2],
-                              cam->to_world[4 * c + 3]);
-    cd.px = cam->position[0];
-    cd.py = cam->position[1];
-    cd.pz = cam->position[2];
-    cd.tlx = tlx;
-    cd.tly = tly;
-    cd.hdx = hdx;
-    cd.hdy = hdy;
-    unsigned long long* cnt = (unsigned long long*)counts;
-    HIP_TRY(hipMemsetAsync(S.counters.p, 0, 64, s));
-    HIP_TRY(launchGenPrimary(view, S.batch[0], cd, w, h, px, S.lists[0].as<uint32_t>(), counter(ctx, C_LIST0), cnt, s));
-    int cur = 0;
-    uint32_t nCur = (uint32_t)npix;
-    for (int depth = op->recursion_depth;; depth--)
-    {
-        if ((st = stageTrace(ctx, view, cur, cur, cur == 0 ? C_LIST0 : C_LIST1, s, nCur)) != CGS_OK) return st;
-        HIP_TRY(launchZero(occ, npix, s));
-        HIP_TRY(hipMemsetAsync(counter(ctx, C_SHADOW), 0, 4, s));
-        HIP_TRY(hipMemsetAsync(counter(ctx, cur == 0 ? C_LIST1 : C_LIST0), 0, 4, s));
-        HIP_TRY(launchShade(view, S.batch[cur], nCur, px, depth, op->eps, S.batch[1 - cur],
-                            S.lists[1 - cur].as<uint32_t>(), counter(ctx, cur == 0 ? C_LIST1 : C_LIST0), S.batch[2],
-                            S.lists[2].as<uint32_t>(), counter(ctx, C_SHADOW), cnt, s));
-        uint32_t nShadow = 0, nNext = 0;
-        if ((st = readCount(ctx, C_SHADOW, s, nShadow)) != CGS_OK) return st;
-        if ((st = readCount(ctx, cur == 0 ? C_LIST1 : C_LIST0, s, nNext)) != CGS_OK) return st;
-        if (nShadow)
-        {
-            if ((st = stageTrace(ctx, view, 2, 2, C_SHADOW, s, nShadow)) != CGS_OK) return st;
-            HIP_TRY(launchScatterShadow(S.batch[2], nShadow, occ, cnt, s));
-        }
-        HIP_TRY(launchAccumulate(S.batch[cur], nCur, px, occ, s));
-        HIP_TRY(launchAdvanceAtt(S.batch[1 - cur], nNext, px, s));
-        if (nNext == 0) break;
-        cur = 1 - cur;
-        nCur = nNext;
-    }
-    HIP_TRY(launchPack(px, (uint32_t)npix, w, stride, frame, color, s));
+    const bool ordered = ts.ready && ts.w == w && ts.h == h;
+    HIP_TRY(launchRender(sc->view(), pa, cam->to_world, cam->position, tlx, tly, hdx, hdy, op->eps, op->recursion_depth,
+                         w, h, frame, color, (unsigned long long*)counts, stride,
+                         ordered ? ts.order.as<uint32_t>() : nullptr, ts.ticks.as<uint32_t>(), ctx->dRenderArgs, s));
+    HIP_TRY(launchTileOrder(ts.ticks.as<uint32_t>(), nTiles, ts.order.as<uint32_t>(), s));
+    ts.ready = true;
+    ts.w = w;
+    ts.h = h;
     HIP_TRY(hipEventRecord(ctx->renderDone, s));
     ctx->renderLaunched = true;
-    return CGS_OK;
+    return persistEnd(ctx, s);
 }
 
 int cgs_render_running(cgs_ctx* ctx)
@@ -1217,7 +1035,9 @@ int cgs_render_running(cgs_ctx* ctx)
 // ---------------------------------------------------------------------------
 // batched queries
 // ---------------------------------------------------------------------------
-static cgs_status buildBatchJobs(cgs_ctx* ctx, const cgs_batch_job* jobs, uint32_t njobs, uint32_t chunk,
+// The job table of a batched launch of `grid` workgroups taking `chunk` rays
+// per wave fetch.
+static cgs_status buildBatchJobs(cgs_ctx* ctx, const cgs_batch_job* jobs, uint32_t njobs, uint32_t chunk, uint32_t grid,
                                  std::vector<BatchJob>& out, uint32_t& total)
 {
     if (njobs && !jobs) return fail(CGS_ERR_INVALID, "batch: null jobs");
@@ -1226,7 +1046,8 @@ static cgs_status buildBatchJobs(cgs_ctx* ctx, const cgs_batch_job* jobs, uint32
     for (uint32_t j = 0; j < njobs; j++)
     {
         const cgs_batch_job& q = jobs[j];
-        if (!q.scene || q.scene->ctx != ctx) return fail(CGS_ERR_INVALID, "batch: job " + std::to_string(j) + " has no scene of this context");
+        if (!q.scene || q.scene->device != ctx->device)
+            return fail(CGS_ERR_INVALID, "batch: job " + std::to_string(j) + " has no scene on this context's device");
         if (q.n == 0) continue;
         if (!q.rays || (q.any ? !q.occluded : !q.hits))
             return fail(CGS_ERR_INVALID, "batch: job " + std::to_string(j) + " lacks rays or outputs");
@@ -1241,7 +1062,8 @@ static cgs_status buildBatchJobs(cgs_ctx* ctx, const cgs_batch_job* jobs, uint32
         b.any = q.any ? 1u : 0u;
         out.push_back(b);
         sum += q.n;
-        if (sum + chunk >= (1ull << 32)) return fail(CGS_ERR_INVALID, "batch: more than 2^32 rays");
+        if (!counterFits(sum, grid, chunk))
+            return fail(CGS_ERR_INVALID, "batch: " + std::to_string(sum) + " rays exceed the 32-bit work counter");
     }
     total = (uint32_t)sum;
     return CGS_OK;
@@ -1273,13 +1095,13 @@ static cgs_status validChunk(uint32_t chunk)
 static cgs_status traceBatch(cgs_ctx* ctx, const cgs_batch_job* jobs, uint32_t njobs, uint32_t chunk, uint32_t bpc,
                              hipStream_t s)
 {
+    uint32_t grid = 0;
+    cgs_status st = batchGrid(ctx, bpc, grid);
+    if (st != CGS_OK) return st;
     std::vector<BatchJob> table;
     uint32_t total = 0;
-    cgs_status st = buildBatchJobs(ctx, jobs, njobs, chunk, table, total);
-    if (st != CGS_OK) return st;
+    if ((st = buildBatchJobs(ctx, jobs, njobs, chunk, grid, table, total)) != CGS_OK) return st;
     if (total == 0) return CGS_OK;
-    uint32_t grid = 0;
-    if ((st = batchGrid(ctx, bpc, grid)) != CGS_OK) return st;
     if (ctx->batchLaunched) HIP_TRY(hipEventSynchronize(ctx->batchDone));
     if (ctx->batchTableCap < table.size())
     {
@@ -1288,7 +1110,7 @@ static cgs_status traceBatch(cgs_ctx* ctx, const cgs_batch_job* jobs, uint32_t n
     }
     // region stacks: the context's, sized for the largest persistent grid
     PersistArgs pa;
-    if ((st = persistArgs(ctx, PersistKernel::Batch, s, pa)) != CGS_OK) return st;
+    if ((st = persistArgs(ctx, PersistKernel::Batch, s, total, pa)) != CGS_OK) return st;
     HIP_TRY(hipMemcpyAsync(ctx->batchTable.p, table.data(), table.size() * sizeof(BatchJob), hipMemcpyHostToDevice, s));
     HIP_TRY(hipStreamSynchronize(s)); // the host table is released on return
     HIP_TRY(launchTraceBatch(ctx->batchTable.as<BatchJob>(), (uint32_t)table.size(), total, chunk, pa.stack,
@@ -1296,7 +1118,7 @@ static cgs_status traceBatch(cgs_ctx* ctx, const cgs_batch_job* jobs, uint32_t n
                              ctx->dOverflow, s));
     HIP_TRY(hipEventRecord(ctx->batchDone, s));
     ctx->batchLaunched = true;
-    return CGS_OK;
+    return persistEnd(ctx, s);
 }
 
 struct cgs_batch
@@ -1327,12 +1149,12 @@ cgs_status cgs_batch_create(cgs_ctx* ctx, const cgs_batch_job* jobs, uint32_t nj
     if (!ctx || !out) return fail(CGS_ERR_INVALID, "cgs_batch_create: null argument");
     HIP_TRY(hipSetDevice(ctx->device));
     std::lock_guard lk(ctx->mu);
+    uint32_t grid = 0;
+    cgs_status st = batchGrid(ctx, ctx->batchBlocksPerCU, grid);
+    if (st != CGS_OK) return st;
     std::vector<BatchJob> table;
     uint32_t total = 0;
-    cgs_status st = buildBatchJobs(ctx, jobs, njobs, ctx->batchChunk, table, total);
-    if (st != CGS_OK) return st;
-    uint32_t grid = 0;
-    if ((st = batchGrid(ctx, ctx->batchBlocksPerCU, grid)) != CGS_OK) return st;
+    if ((st = buildBatchJobs(ctx, jobs, njobs, ctx->batchChunk, grid, table, total)) != CGS_OK) return st;
     auto b = std::make_unique<cgs_batch>();
     b->ctx = ctx;
     const uint32_t lanes = grid * 256;
@@ -1559,6 +1381,9 @@ struct cgs_server
     std::mutex mu;
     std::condition_variable wake, finished;
     std::deque<ServerRequest> pending;
+    // Requests not yet done (queued or in a launch), and done requests not
+    // yet collected by cgs_server_wait / cgs_server_poll, which erase them.
+    std::unordered_set<uint64_t> open;
     std::unordered_map<uint64_t, Done> done;
     uint64_t nextId = 1;
     size_t inflight = 0;
@@ -1590,6 +1415,7 @@ struct cgs_server
         {
             std::lock_guard lk(mu);
             done[r.id] = {(uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(now - r.submitted).count(), ok};
+            open.erase(r.id);
             inflight -= r.bytes;
         }
         sl.reqs.clear();
@@ -1691,7 +1517,7 @@ struct cgs_server
                         const ServerRequest& r = pending.front();
                         if (inflight + r.bytes > budget && !(sl.reqs.empty() && inflight == 0)) break;
                         if (maxBatchRays && !sl.reqs.empty() && rays + r.n > maxBatchRays) break;
-                        if (rays + r.n + chunk >= (1ull << 32)) break;
+                        if (!counterFits(rays + r.n, grid, chunk)) break;
                         inflight += r.bytes;
                         rays += r.n;
                         sl.reqs.push_back(r);
@@ -1728,7 +1554,8 @@ cgs_status cgs_server_create(cgs_ctx* ctx, cgs_scene* const* scenes, uint32_t ns
     sv->ctx = ctx;
     for (uint32_t k = 0; k < nscenes; k++)
     {
-        if (!scenes[k] || scenes[k]->ctx != ctx) return fail(CGS_ERR_INVALID, "cgs_server_create: scene of another context");
+        if (!scenes[k] || scenes[k]->device != ctx->device)
+            return fail(CGS_ERR_INVALID, "cgs_server_create: scene on another device");
         sv->scenes.push_back(scenes[k]);
         sv->views.push_back(scenes[k]->view());
     }
@@ -1774,6 +1601,10 @@ cgs_status cgs_server_submit(cgs_server* sv, uint32_t scene, const cgs_ray* rays
     r.occluded = occluded;
     r.bytes = requestBytes(n);
     if (r.bytes > sv->budget) return fail(CGS_ERR_INVALID, "cgs_server_submit: request exceeds the byte budget");
+    // a request must fit one launch's 32-bit work counter on its own, or it
+    // would never be admitted and would block every request behind it
+    if (!counterFits(n, sv->grid, sv->chunk))
+        return fail(CGS_ERR_INVALID, "cgs_server_submit: request exceeds the 32-bit work counter of a launch");
     r.submitted = Clock::now();
     {
         std::lock_guard lk(sv->mu);
@@ -1783,7 +1614,10 @@ cgs_status cgs_server_submit(cgs_server* sv, uint32_t scene, const cgs_ray* rays
         if (n == 0)
             sv->done[r.id] = {0, 1};
         else
+        {
+            sv->open.insert(r.id);
             sv->pending.push_back(r);
+        }
     }
     sv->wake.notify_one();
     return CGS_OK;
@@ -1794,18 +1628,22 @@ int cgs_server_poll(cgs_server* sv, uint64_t id, uint64_t* latency_ns)
     if (!sv) return -1;
     std::lock_guard lk(sv->mu);
     auto it = sv->done.find(id);
-    if (it == sv->done.end()) return 0;
-    if (latency_ns) *latency_ns = it->second.latencyNs;
-    return it->second.ok ? 1 : -1;
+    if (it == sv->done.end()) return sv->open.count(id) ? 0 : -1;
+    const Done d = it->second;
+    sv->done.erase(it);
+    if (latency_ns) *latency_ns = d.latencyNs;
+    return d.ok ? 1 : -1;
 }
 
 cgs_status cgs_server_wait(cgs_server* sv, uint64_t id, uint64_t* latency_ns)
 {
     if (!sv) return fail(CGS_ERR_INVALID, "cgs_server_wait: null server");
     std::unique_lock lk(sv->mu);
-    if (id == 0 || id >= sv->nextId) return fail(CGS_ERR_INVALID, "cgs_server_wait: unknown request");
+    if (!sv->done.count(id) && !sv->open.count(id))
+        return fail(CGS_ERR_INVALID, "cgs_server_wait: unknown or already collected request");
     sv->finished.wait(lk, [&] { return sv->done.count(id) != 0; });
     const Done d = sv->done[id];
+    sv->done.erase(id);
     if (latency_ns) *latency_ns = d.latencyNs;
     return d.ok ? CGS_OK : fail(CGS_ERR_HIP, "cgs_server_wait: the request's launch failed");
 }
@@ -1873,6 +1711,26 @@ cgs_status cgs_memset(cgs_ctx* ctx, void* dst, int value, size_t bytes, void* st
     return CGS_OK;
 }
 
+cgs_status cgs_stream_create(cgs_ctx* ctx, void** stream)
+{
+    if (!ctx || !stream) return fail(CGS_ERR_INVALID, "cgs_stream_create: null argument");
+    HIP_TRY(hipSetDevice(ctx->device));
+    hipStream_t s = nullptr;
+    HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    *stream = s;
+    return CGS_OK;
+}
+
+cgs_status cgs_stream_destroy(cgs_ctx* ctx, void* stream)
+{
+    if (!ctx) return fail(CGS_ERR_INVALID, "cgs_stream_destroy: null context");
+    if (!stream) return CGS_OK;
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+    HIP_TRY(hipStreamDestroy((hipStream_t)stream));
+    return CGS_OK;
+}
+
 cgs_status cgs_stream_sync(cgs_ctx* ctx, void* stream)
 {
     if (!ctx) return fail(CGS_ERR_INVALID, "cgs_stream_sync: null context");
@@ -1882,26 +1740,3 @@ cgs_status cgs_stream_sync(cgs_ctx* ctx, void* stream)
 }
 
 } // extern "C"
-
-namespace
-{
-
-cgs_status stageQuery(cgs_ctx* ctx, cgs_scene* sc, const cgs_ray* rays, uint32_t n, int any, cgs_hit* hits,
-                      int32_t* objects, uint32_t* occluded, hipStream_t s)
-{
-    if (n == 0) return CGS_OK;
-    cgs_status st;
-    if ((st = ensureBatch(ctx, 0, n)) != CGS_OK || (st = ensureLists(ctx, n)) != CGS_OK) return st;
-    StageBuffers& S = ctx->stage;
-    const SceneDev view = sc->view();
-    HIP_TRY(launchLoadRays(view, S.batch[0], rays, n, any, S.lists[0].as<uint32_t>(), s));
-    HIP_TRY(hipMemcpyAsync(counter(ctx, C_LIST0), &n, 4, hipMemcpyHostToDevice, s));
-    if ((st = stageTrace(ctx, view, 0, 0, C_LIST0, s, n)) != CGS_OK) return st;
-    if (any)
-        HIP_TRY(launchStoreAny(S.batch[0], n, occluded, ctx->dOverflow, s));
-    else
-        HIP_TRY(launchStoreClosest(S.batch[0], n, hits, objects, ctx->dOverflow, s));
-    return CGS_OK;
-}
-
-} // namespace

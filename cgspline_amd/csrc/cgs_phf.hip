@@ -69,7 +69,7 @@ __global__ __launch_bounds__(kPhfThreads) void kPhfBuild(const PhfJob* __restric
     for (uint32_t b = tid; b < M; b += kPhfThreads) bsize[b] = 0;
     __syncthreads();
 
-    // 1. buckets (init_and_bucketsort, PerfectHashFunction.cu:100-128)
+    // 1. buckets (init_and_bucketsort, PerfectHashFunction.cu:27-55)
     for (uint32_t i = tid; i < n; i += kPhfThreads)
     {
         uint32_t k = keys[J.keyOff + i];
@@ -126,7 +126,7 @@ __global__ __launch_bounds__(kPhfThreads) void kPhfBuild(const PhfJob* __restric
     }
     __syncthreads();
 
-    // 3. functions (PerfectHashFunction.cu:456-557)
+    // 3. functions (PerfectHashFunction.cu:383-484)
     uint32_t T = n;
     uint32_t A = 31, B = 19;
     int fn = 0;
@@ -234,7 +234,7 @@ __global__ __launch_bounds__(kPhfThreads) void kPhfBuild(const PhfJob* __restric
     }
     const uint32_t attempts = ok ? fn + 1 : kMaxFunctions;
 
-    // 4. injectivity check and table fill (BVH::buildHashTable, BVH.cpp:30-37)
+    // 4. injectivity check and table fill (BVH::buildHashTable, BVH.cpp:24-38)
     for (uint32_t i = tid; i < T; i += kPhfThreads) table[i] = EMPTY;
     if (tid == 0) sFlag = 0;
     __syncthreads();

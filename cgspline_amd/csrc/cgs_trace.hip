@@ -23,7 +23,7 @@ struct LocalRay
     float max, dinv;
 };
 
-// RayTracer.cu:379-389: world ray -> object-local ray; t scales by |M d|.
+// RayTracer.cu:305-314,333: world ray -> object-local ray; t scales by |M d|.
 CGS_DEV LocalRay toLocal(const DevObject& ob, V3 o, V3 d, float tmax)
 {
     M4 M;
@@ -46,8 +46,8 @@ struct ClosestHit
     int32_t object;
 };
 
-// Context::intersect(hit, ray) (RayTracer.cu:369-423) with
-// rt::intersect(BezierSurface, hit, ray) (Primitive.cu:355-381) at the leaves.
+// Context::intersect(hit, ray) (RayTracer.cu:296-350) with
+// rt::intersect(BezierSurface, hit, ray) (Primitive.cu:222-248) at the leaves.
 // Sibling rings of a lane for the two traversal levels: the top-level BVH
 // over objects (CGS_RING_TOP entries; 4096 objects make it 13 levels deep)
 // and the patch BVH of one object (CGS_RING_BOT; 256 patches, 9 levels).
@@ -113,8 +113,8 @@ CGS_DEV ClosestHit sceneClosest(const SceneDev& sc, const NodeCache& nc, const R
     return h;
 }
 
-// Context::intersect(ray) (RayTracer.cu:425-457) with
-// rt::intersect(BezierSurface, ray) (Primitive.cu:383-401) at the leaves.
+// Context::intersect(ray) (RayTracer.cu:352-383) with
+// rt::intersect(BezierSurface, ray) (Primitive.cu:250-268) at the leaves.
 CGS_DEV bool sceneAny(const SceneDev& sc, const NodeCache& nc, const Rings& rg, V3 o, V3 d, float tmax,
                       const ClipStack& stk, bool& overflow)
 {
@@ -490,7 +490,7 @@ __global__ __launch_bounds__(256) void kSubdivBatch(const cgs_ray* __restrict__ 
 }
 
 // ---------------------------------------------------------------------------
-// Renderer (RayTracer.cu:268-618, non-Monte-Carlo path)
+// Renderer (RayTracer.cu:194-544, non-Monte-Carlo path)
 // ---------------------------------------------------------------------------
 
 struct CamDev
@@ -636,7 +636,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CGS_RENDER_
         bool overflow = false;
         nPrimary++;
         V3 acc {0.0f, 0.0f, 0.0f};
-        // pixelRayDirection (RayTracer.cu:133-145)
+        // pixelRayDirection (RayTracer.cu:59-71)
         float4 P = make_float4(cam.tlx + cam.hdx * (2.0f * (float)i), cam.tly - cam.hdy * (2.0f * (float)j), -1.0f, 1.0f);
         V3 ro = camPos;
         V3 rd = versor3(sub3(project4(m4mul(C, P)), camPos));

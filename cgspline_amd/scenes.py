@@ -70,7 +70,7 @@ class Scene:
 
 @dataclasses.dataclass
 class Camera:
-    """Pinhole camera: looks along its local -Z, up +Y (RayTracer.cu:133-145)."""
+    """Pinhole camera: looks along its local -Z, up +Y (RayTracer.cu:59-71,141-163)."""
 
     to_world: np.ndarray   # (16,) float32, column-major camera→world
     position: np.ndarray   # (3,) float32
@@ -487,7 +487,7 @@ def from_dl(path, width=320, height=240, depth=2, fovy_deg=60.0) -> Workload:
 
 def camera_rays(cam: Camera, rows=None) -> np.ndarray:
     """World rays (n, 8) float32 (o xyz, tmax, d xyz, 0) through pixel centres,
-    in float32 with the reference's camera formula (RayTracer.cu:133-145,
+    in float32 with the reference's camera formula (RayTracer.cu:59-71,
     207-231)."""
     f32 = np.float32
     w, h = cam.width, cam.height

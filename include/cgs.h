@@ -120,7 +120,7 @@ typedef struct cgs_scene_desc
     uint32_t hash_seed;           /* PerfectHashFunction seed (reference default 42) */
 } cgs_scene_desc;
 
-/* Pinhole camera (rt/Scene.h:29-45 Camera; RayTracer.cu:207-231). */
+/* Pinhole camera (rt/Scene.h:29-45 Camera; RayTracer.cu:59-71,141-163). */
 typedef struct cgs_camera
 {
     float to_world[16];   /* camera→world, column-major (Transform::matrix()) */
@@ -215,7 +215,7 @@ cgs_status cgs_subdivision(cgs_ctx* ctx, const cgs_ray* rays, const float* patch
 cgs_status cgs_bvh_build(const cgs_bvh_element* elements, uint32_t n, uint32_t elements_per_node,
                          cgs_bvh_node** nodes, uint32_t** keys, uint32_t* node_count,
                          uint32_t** indices, uint32_t* index_count);
-/* BezierSurface::buildBVH element data (rt/Primitive.cu:332-353): bounds and
+/* BezierSurface::buildBVH element data (rt/Primitive.cu:199-220): bounds and
  * centroid of the projected control points of each patch.  vertices: nv × 4
  * host floats; indices: npatch × 16.  elements: npatch entries (caller). */
 cgs_status cgs_patch_elements(const float* vertices, const uint32_t* indices, uint32_t npatch,
@@ -267,7 +267,7 @@ cgs_status cgs_dl_parse(const char* text, size_t length, const char* name, const
                         cgs_dl_scene** out);
 void cgs_dl_free(cgs_dl_scene* scene);
 
-/* PerfectHashFunction::build (rt/PerfectHashFunction.h:53, .cu:302-576) on
+/* PerfectHashFunction::build (rt/PerfectHashFunction.h:53, .cu:229-503) on
  * the GPU, deterministic.  keys: n device u32; displacement: device u32
  * buffer of cgs_phf_displacement_size(n) entries (written); out: parameters.
  * Returns CGS_ERR_HASH_FAILED / _DUPLICATE / _BUCKET_OVERFLOW like the
@@ -275,7 +275,7 @@ void cgs_dl_free(cgs_dl_scene* scene);
 uint32_t cgs_phf_displacement_size(uint32_t n);
 cgs_status cgs_phf_build(cgs_ctx* ctx, const uint32_t* keys, uint32_t n, uint32_t seed,
                          uint32_t* displacement, cgs_phf* out, void* stream);
-/* BVH::buildHashTable's table fill (rt/BVH.cpp:30-37): table[h(keys[i])] = i,
+/* BVH::buildHashTable's table fill (rt/BVH.cpp:24-38): table[h(keys[i])] = i,
  * CGS_EMPTY elsewhere.  table: out->table_size device u32. */
 cgs_status cgs_phf_fill_table(cgs_ctx* ctx, const cgs_phf* h, const uint32_t* displacement,
                               const uint32_t* keys, uint32_t n, uint32_t* table, void* stream);
@@ -288,7 +288,7 @@ cgs_status cgs_phf_hash(cgs_ctx* ctx, const cgs_phf* h, const uint32_t* displace
 /* ---------------------------------------------------------------------- */
 
 /* Builds every surface BVH (host), the top-level BVH over object world
- * bounds (Scene::buildBVH, rt/Scene.cu:218-232), uploads the scene and
+ * bounds (Scene::buildBVH, rt/Scene.cu:39-53), uploads the scene and
  * builds every perfect hash and table on the GPU (BVH::buildHashTable,
  * rt/BVH.cpp:24-38).  The desc's host arrays are copied. */
 cgs_status cgs_scene_create(cgs_ctx* ctx, const cgs_scene_desc* desc, cgs_scene** out);
@@ -301,19 +301,26 @@ cgs_status cgs_scene_bvh_info(cgs_scene* scene, int32_t surface, uint32_t* node_
 cgs_status cgs_scene_bvh_copy(cgs_scene* scene, int32_t surface, cgs_bvh_node* nodes, uint32_t* keys,
                               uint32_t* indices, uint32_t* displacement, uint32_t* table);
 
+/* Scenes are read-only once created: any context on the scene's device may
+ * trace or render it, and several contexts may do so at the same time.
+ * The trace, clip, render and batched calls of one context share its region
+ * stacks and work counters: each launch waits on the GPU for the context's
+ * previous one, whatever streams they were issued on.  Contexts overlap
+ * freely with each other. */
+
 /* Closest hit of n world rays (RayTracer::Context::intersect(hit, ray),
- * RayTracer.cu:369-423: top-level hashIntersect → per-object local ray →
+ * RayTracer.cu:296-350: top-level hashIntersect → per-object local ray →
  * BezierSurface hashIntersect → doBezierClipping).  rays, hits, objects
  * (nullable, int32 object id or -1) are device pointers. */
 cgs_status cgs_trace_closest(cgs_ctx* ctx, cgs_scene* scene, const cgs_ray* rays, uint32_t n,
                              cgs_hit* hits, int32_t* objects, void* stream);
 /* Occlusion of n world rays up to ray.tmax (Context::intersect(ray),
- * RayTracer.cu:425-457).  occluded: n device u32 (1 / 0). */
+ * RayTracer.cu:352-383).  occluded: n device u32 (1 / 0). */
 cgs_status cgs_trace_any(cgs_ctx* ctx, cgs_scene* scene, const cgs_ray* rays, uint32_t n,
                          uint32_t* occluded, void* stream);
 
-/* RayTracer::render (rt/RayTracer.h:42-43, RayTracer.cu:181-254 and the
- * render kernel :268-618, non-Monte-Carlo): camera rays through integer
+/* RayTracer::render (rt/RayTracer.h:42-43, RayTracer.cu:107-180 and the
+ * render kernel :194-544, non-Monte-Carlo): camera rays through integer
  * pixel centres, closest hit, PBR shading with one shadow ray per light and
  * the reflection bounce loop.  frame: `height` rows of packed pixels
  * (device), options->frame_stride (or width) pixels apart, pixel (x, y) at
@@ -321,7 +328,7 @@ cgs_status cgs_trace_any(cgs_ctx* ctx, cgs_scene* scene, const cgs_ray* rays, ui
  * color: optional width*height*3 floats (device, unclamped radiance, dense);
  * counts: optional device cgs_ray_counts (accumulated).
  * Asynchronous on `stream`; CGS_ERR_BUSY while a previous render of this
- * context is still running (RayTracer.cu:184-190).
+ * context is still running (RayTracer.cu:109-115).
  * Scheduling: each render records how long each 8×8 pixel tile took, and the
  * next render of a frame of the same size on this scene takes the tiles
  * longest first by those times; a scene's first render at a size takes them
@@ -336,12 +343,13 @@ int cgs_render_running(cgs_ctx* ctx);
 /* ---------------------------------------------------------------------- */
 /* Batched queries over many resident scenes (no counterpart in the        */
 /* reference: each job is Context::intersect over a ray range,             */
-/* RayTracer.cu:369-457, and all jobs share one persistent launch)         */
+/* RayTracer.cu:296-383, and all jobs share one persistent launch)         */
 /* ---------------------------------------------------------------------- */
 
 /* One launch over every job.  Asynchronous on `stream`; batched launches of
  * one context run one at a time (a call waits for the previous one's kernel
- * before reusing the context's job table). */
+ * before reusing the context's job table).  Refused (CGS_ERR_INVALID) when
+ * the jobs' rays, plus one chunk per resident wave, reach 2^32. */
 cgs_status cgs_trace_batch(cgs_ctx* ctx, const cgs_batch_job* jobs, uint32_t njobs, void* stream);
 /* A fixed batch with its own job table, counter and region stacks, captured
  * once in a HIP graph; cgs_batch_launch replays it (results as
@@ -378,12 +386,17 @@ cgs_status cgs_server_create(cgs_ctx* ctx, cgs_scene* const* scenes, uint32_t ns
 /* Queues a request: n host rays against scene `scene`; any = 0: closest hits
  * into host `hits` (and `objects`, nullable); any = 1: occlusion flags into
  * host `occluded`.  The buffers must stay valid until the request is done.
- * Fails with CGS_ERR_INVALID if the request alone exceeds the byte budget. */
+ * Fails with CGS_ERR_INVALID if the request alone exceeds the byte budget or
+ * one launch's 32-bit work counter (n plus one chunk per resident wave). */
 cgs_status cgs_server_submit(cgs_server* server, uint32_t scene, const cgs_ray* rays, uint32_t n, int32_t any,
                              cgs_hit* hits, int32_t* objects, uint32_t* occluded, uint64_t* id);
-/* 1 when request `id` is done (outputs written), 0 while pending;
- * latency_ns (nullable): submit to done. */
+/* 1 when request `id` is done (outputs written), 0 while pending, -1 when
+ * its launch failed or the id is unknown or already collected; latency_ns
+ * (nullable): submit to done.  A result of 1 or -1 collects the request: the
+ * server forgets it, so a long-running server keeps no record per request. */
 int cgs_server_poll(cgs_server* server, uint64_t id, uint64_t* latency_ns);
+/* Waits for request `id` and collects it like cgs_server_poll;
+ * CGS_ERR_INVALID for an unknown or already collected id. */
 cgs_status cgs_server_wait(cgs_server* server, uint64_t id, uint64_t* latency_ns);
 /* Finishes every queued request, then stops the worker. */
 cgs_status cgs_server_destroy(cgs_server* server);
@@ -398,6 +411,9 @@ cgs_status cgs_memcpy_h2d(cgs_ctx* ctx, void* dst, const void* src, size_t bytes
 cgs_status cgs_memcpy_d2h(cgs_ctx* ctx, void* dst, const void* src, size_t bytes, void* stream);
 cgs_status cgs_memset(cgs_ctx* ctx, void* dst, int value, size_t bytes, void* stream);
 cgs_status cgs_stream_sync(cgs_ctx* ctx, void* stream);
+/* A non-blocking stream on the context's device (hipStream_t as void*). */
+cgs_status cgs_stream_create(cgs_ctx* ctx, void** stream);
+cgs_status cgs_stream_destroy(cgs_ctx* ctx, void* stream);
 
 #ifdef __cplusplus
 }

@@ -10,7 +10,7 @@
 //     (rt/BVH.cpp:24-38); BVH::build throws std::logic_error("invalid") on an
 //     empty split (rt/BVH.cpp:84-88);
 //   * RayTracer::render on a busy tracer prints a warning and does nothing
-//     (rt/RayTracer.cu:184-190);
+//     (rt/RayTracer.cu:109-115);
 //   * every other failure of the library throws cg::rt::Error (the reference
 //     aborts on CUDA errors, CUDAUtility.cpp:8-22).
 // Device memory is plain HIP device memory; streams are hipStream_t.
@@ -26,7 +26,9 @@
 #include <cstdio>
 #include <istream>
 #include <iterator>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <optional>
 #include <span>
 #include <stdexcept>
@@ -100,11 +102,25 @@ inline void check(cgs_status s, const char* what)
 }
 
 // One opened GPU (RayTracer::Options::device).  Owns the library context:
-// work queues, region stacks and staging buffers of the device path.
+// work queues, region stacks and staging buffers of the device path.  Data
+// made resident through one Device (scenes) may be used through any other
+// Device of the same GPU.
 class Device
 {
 public:
     explicit Device(int device = 0) { check(cgs_open(device, &_ctx), "cgs_open"); }
+
+    // A Device per GPU for callers that do not bring their own (scene
+    // builds, PatchBVH): opened on first use and kept until the process ends.
+    static const Device& shared(int device = 0)
+    {
+        static std::mutex mu;
+        static std::map<int, Device*> open;
+        std::lock_guard lk(mu);
+        Device*& d = open[device];
+        if (!d) d = new Device(device);
+        return *d;
+    }
     ~Device()
     {
         if (_ctx) (void)cgs_close(_ctx);
@@ -131,6 +147,13 @@ public:
         check(cgs_memcpy_d2h(_ctx, dst, src, bytes, s), "cgs_memcpy_d2h");
     }
     void synchronize(stream_t s = nullptr) const { check(cgs_stream_sync(_ctx, s), "cgs_stream_sync"); }
+    stream_t createStream() const
+    {
+        void* s = nullptr;
+        check(cgs_stream_create(_ctx, &s), "cgs_stream_create");
+        return (stream_t)s;
+    }
+    void destroyStream(stream_t s) const { check(cgs_stream_destroy(_ctx, s), "cgs_stream_destroy"); }
 
 private:
     cgs_ctx* _ctx = nullptr;
@@ -332,7 +355,7 @@ struct Camera
 
 // Scene content as MainWindow::convertScene fills rt::Scene
 // (MainWindow.cpp:363-492); buildBVH() builds every BVH and hash and makes
-// the scene resident on the device (Scene::buildBVH, rt/Scene.cu:218-232).
+// the scene resident on the device (Scene::buildBVH, rt/Scene.cu:39-53).
 class Scene
 {
 public:
@@ -361,6 +384,10 @@ public:
         return (uint32_t)_vbase.size() - 2;
     }
     uint32_t addSurface(const cg::Surface& surface) { return addSurface(surface.points(), surface.indices()); }
+
+    // Builds on the GPU's shared Device (Device::shared); any RayTracer on
+    // that GPU can render the result.
+    void buildBVH(int device = 0) { buildBVH(Device::shared(device)); }
 
     void buildBVH(const Device& device)
     {
@@ -457,7 +484,7 @@ public:
     RayTracer() : RayTracer(Options {}) {}
     explicit RayTracer(Options&& op) : options(std::move(op)), _dev(options.device) {}
 
-    // Asynchronous on `stream`.  Busy: warning and no-op (RayTracer.cu:184-190).
+    // Asynchronous on `stream`.  Busy: warning and no-op (RayTracer.cu:109-115).
     void render(Frame* frame, const Camera* camera, const Scene* scene, stream_t stream = nullptr,
                 cgs_ray_counts* deviceCounts = nullptr)
     {

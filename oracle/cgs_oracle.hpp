@@ -870,7 +870,7 @@ inline BVHData bvhBuild(std::vector<Element> el, uint32_t elementsPerNode = 1)
     return b;
 }
 
-// Primitive.cu:332-353 — per-patch element data: bounds and centroid of the
+// BezierSurface::buildBVH (Primitive.cu:199-220) — per-patch element data: bounds and centroid of the
 // projected control points (centroid accumulated as += (1/16)·p).
 inline std::vector<Element> patchElements(const float* vertices4, const uint32_t* indices, uint32_t patchCount)
 {
@@ -901,7 +901,7 @@ inline std::vector<Element> patchElements(const float* vertices4, const uint32_t
 // atomics), so its output is not reproducible even by itself.  This is the
 // deterministic restatement both this oracle and the HIP build implement:
 //  * hash family, table sizing, growth, (A,B) draws and attempt budget as in
-//    the reference (PerfectHashFunction.h:45-63, .cu:302-557);
+//    the reference (PerfectHashFunction.h:45-63, .cu:229-484);
 //  * buckets = key & mask; keys kept in input order inside a bucket;
 //  * size classes 4..1; inside a class buckets are ranked g by ascending id;
 //  * rounds: every pending bucket proposes d = a·(g+1) (a = its attempt
@@ -922,7 +922,7 @@ struct PHF
 };
 
 // Returns 0 on success, 1 when the final function is not injective
-// (reference: "DUPLICATE FOUND!", PerfectHashFunction.cu:559-575), 2 when a
+// (reference: "DUPLICATE FOUND!", PerfectHashFunction.cu:486-502), 2 when a
 // bucket holds more than 4 keys (reference: __trap in init_and_bucketsort).
 inline int phfBuild(PHF& h, std::span<const uint32_t> keys, uint32_t seed = 42)
 {
@@ -1034,7 +1034,7 @@ inline int phfBuild(PHF& h, std::span<const uint32_t> keys, uint32_t seed = 42)
         if (failures == 0) break;
         T = T + step;
     }
-    // PerfectHashFunction.cu:559-575 — injectivity check on the final function
+    // PerfectHashFunction.cu:486-502 — injectivity check on the final function
     // (after a last failure the table size has grown past the one used).
     if (h.tableSize != T) h.tableSize = T;
     std::vector<uint8_t> seen(h.tableSize, 0);

@@ -139,7 +139,7 @@ struct PatchTestLog
 };
 inline thread_local PatchTestLog* g_patchLog = nullptr;
 
-// rt/Primitive.cu:355-381 — closest hit against one surface (local ray).
+// rt/Primitive.cu:222-248 — closest hit against one surface (local ray).
 inline bool surfaceIntersect(const Surface& s, Hit& hit0, const Ray& ray0, Work* w)
 {
     // Ds Ray3f{origin, direction} normalizes the direction; the result is the
@@ -170,7 +170,7 @@ inline bool surfaceIntersect(const Surface& s, Hit& hit0, const Ray& ray0, Work*
     return hashIntersect(s.view(), hit0, ray0, fn, w);
 }
 
-// rt/Primitive.cu:383-401 — any hit against one surface (local ray).
+// rt/Primitive.cu:250-268 — any hit against one surface (local ray).
 inline bool surfaceIntersectAny(const Surface& s, const Ray& ray0, Work* w)
 {
     const f3 dir = versor(ray0.direction);
@@ -184,7 +184,7 @@ inline bool surfaceIntersectAny(const Surface& s, const Ray& ray0, Work* w)
     return hashIntersectAny(s.view(), ray0, fn, w);
 }
 
-// World ray -> object-local ray (RayTracer.cu:379-389): t scales by |M d|.
+// World ray -> object-local ray (RayTracer.cu:305-314,333): t scales by |M d|.
 inline Ray localRay(const Object& o, const Ray& ray, float& dinv)
 {
     Ray lr;
@@ -199,7 +199,7 @@ inline Ray localRay(const Object& o, const Ray& ray, float& dinv)
     return lr;
 }
 
-// RayTracer.cu:369-423 — Context::intersect(hit, ray): returns the nearest object.
+// RayTracer.cu:296-350 — Context::intersect(hit, ray): returns the nearest object.
 inline int sceneIntersect(const Scene& sc, Hit& hit0, const Ray& ray0, uint32_t* patchOut, Work* w)
 {
     int nearest = -1;
@@ -281,7 +281,7 @@ inline int sceneIntersectStack(const Scene& sc, Hit& hit0, const Ray& ray0, uint
     return nearest;
 }
 
-// RayTracer.cu:425-457 — Context::intersect(ray): any hit.
+// RayTracer.cu:352-383 — Context::intersect(ray): any hit.
 inline bool sceneIntersectAny(const Scene& sc, const Ray& ray0, Work* w)
 {
     auto fn = [&](uint32_t i) -> bool
@@ -337,14 +337,14 @@ inline f3 lightColor(float d, const Light& light)
     }
     return f * light.color;
 }
-// PBR.h:288-295 (vec3 and float instantiations)
+// schlick (PBR.h:53-61; vec3 and float instantiations)
 inline f3 schlick(f3 f0, float dotLH)
 {
     float b = 1 - dotLH;
     float b2 = b * b;
     return f0 + (splat(1) - f0) * splat(b2 * b2 * b);
 }
-// PBR.h:306-319
+// G1, G (PBR.h:72-85)
 inline float G1(float dotNX, float k) { return dotNX / (dotNX * (1 - k) + k); }
 inline float Gsmith(float dotNL, float dotNV, float r)
 {
@@ -352,7 +352,7 @@ inline float Gsmith(float dotNL, float dotNV, float r)
     const float k = (float)((double)(r1 * r1) * 0.125);
     return G1(dotNL, k) * G1(dotNV, k);
 }
-// PBR.h:322-330
+// D (PBR.h:87-96)
 inline float Dggx(float dotNH, float r)
 {
     r = fmax_(r, 1e-3f);
@@ -361,22 +361,22 @@ inline float Dggx(float dotNH, float r)
     float d = dotNH * dotNH * (a2 - 1) + 1;
     return a2 / (std::numbers::pi_v<float> * d * d);
 }
-// PBR.h:332-341
+// BRDF_microfacet (PBR.h:98-107)
 inline float microfacet(float dotNV, float dotNL, float dotNH, float r)
 {
     return Gsmith(dotNL, dotNV, r) * Dggx(dotNH, r) / (4 * dotNL * dotNV);
 }
-// PBR.h:343-348
+// BRDF_diffuse (PBR.h:109-114)
 inline f3 brdfDiffuse(const Material& m) { return std::numbers::inv_pi_v<float> * m.diffuse; }
-// PBR.h:365-375
+// BRDF_specular (PBR.h:131-141)
 inline f3 brdfSpecular(float dotHL, float dotHN, float dotNV, float dotNL, float r, f3 f0)
 {
     return microfacet(dotNV, dotNL, dotHN, r) * schlick(f0, dotHL);
 }
-// PBR.h:268-272
+// reflect (PBR.h:34-38)
 inline f3 reflect(f3 I, f3 N, float dotIN) { return I - (2 * dotIN) * N; }
 
-// Primitive.cu:403-408 — shading normal from mat::normal, normalized.
+// Primitive.cu:270-275 — shading normal from mat::normal, normalized.
 inline f3 surfaceNormal(const Surface& s, uint32_t patch, float u, float v)
 {
     f4 S[16];
@@ -397,7 +397,7 @@ inline uint32_t packSRGB(float r, float g, float b)
     return enc(r) | (enc(g) << 8) | (enc(b) << 16) | 0xFF000000u;
 }
 
-// Camera (RayTracer.cu:207-231): pixel centres at integer (i,j).
+// Camera (RayTracer.cu:141-163): pixel centres at integer (i,j).
 struct Camera
 {
     M4<float> cameraToWorld;
@@ -429,7 +429,7 @@ inline CameraSetup setupCamera(const Camera& c, uint32_t w, uint32_t h)
     }
     return s;
 }
-// RayTracer.cu:133-145
+// Context::pixelRayDirection (RayTracer.cu:59-71)
 inline f3 pixelRayDirection(const CameraSetup& c, float x, float y)
 {
     f4 P {c.topLeft.x + c.half_dx * (2 * x), c.topLeft.y - c.half_dy * (2 * y), -1.0f, 1.0f};
@@ -448,7 +448,7 @@ struct RayCounts
     uint64_t primary = 0, shadow = 0, secondary = 0;
 };
 
-// RayTracer.cu:256-348 + 350-618 (non-Monte-Carlo path): the GPU renderer's
+// render kernel and Context::trace/miss/closestHit (RayTracer.cu:194-294,385-544, non-Monte-Carlo path): the GPU renderer's
 // iterative trace loop.  Writes the unclamped colour and the packed pixel.
 inline void renderPixel(const Scene& sc, const CameraSetup& cam, const RenderOptions& op,
     uint32_t i, uint32_t j, f3& colorOut, uint32_t& pixel, Work* w, RayCounts* rc)
@@ -477,7 +477,7 @@ inline void renderPixel(const Scene& sc, const CameraSetup& cam, const RenderOpt
             pl.color = pl.color + sc.background * pl.attenuation;
             break;
         }
-        // closestHit (RayTracer.cu:471-618)
+        // closestHit (RayTracer.cu:398-544)
         const Object& o = sc.objects[obj];
         const Material& m = o.material;
         const f3 diffuse = (1 - m.metalness) * brdfDiffuse(m);
@@ -529,7 +529,7 @@ inline void renderPixel(const Scene& sc, const CameraSetup& cam, const RenderOpt
 }
 
 // ---------------------------------------------------------------------------
-// CPURayTracer (rt/CPURayTracer.cpp:186-441), non-Monte-Carlo path: the CPU
+// CPURayTracer (rt/CPURayTracer.cpp:55-440), non-Monte-Carlo path: the CPU
 // baseline.  Recursive trace, 32×32 tiles pulled from a mutex-guarded deque
 // by `threads` std::async workers.
 // ---------------------------------------------------------------------------
@@ -549,7 +549,7 @@ public:
 
     // Renders rows row0, row0 + stride, ... below row1 (a bounded sample of
     // the frame for timing; stride 1 = every row) and returns the wall time
-    // from entry to the last tile, in seconds (CPURayTracer.cpp:203,268).
+    // from entry to the last tile, in seconds (CPURayTracer.cpp:72,137).
     // Tiles are 32×32 over the sampled rows.
     double render(const Scene* scene, const Camera& camera, uint32_t w, uint32_t h,
         float* color, uint32_t* frame, uint32_t row0 = 0, uint32_t row1 = 0xFFFFFFFFu, uint32_t stride = 1)
@@ -623,7 +623,7 @@ private:
         }
     }
 
-    // CPURayTracer.cpp:330-350
+    // CPURayTracer::trace (CPURayTracer.cpp:199-219)
     f3 trace(const Ray& ray, f3 attenuation, int depth)
     {
         rays.fetch_add(1, std::memory_order_relaxed);
@@ -634,7 +634,7 @@ private:
         return closestHit(hit, ray, attenuation, (uint32_t)obj, depth);
     }
 
-    // CPURayTracer.cpp:422-571 (non-MC branch)
+    // CPURayTracer::closestHit (CPURayTracer.cpp:291-440, non-MC branch)
     f3 closestHit(const Hit& hit, const Ray& ray, f3 attenuation, uint32_t object, int depth)
     {
         const Object& o = _scene->objects[object];

@@ -5,8 +5,8 @@ path equals the FP32 oracle bit for bit, which the GPU parity tests check):
 
 (ii) against the same algorithm in FP64: every patch whose control-point box
      the FP64 object-local ray crosses is clipped in FP64 (Bézier clipping,
-     tolerance 2^-12, rt/Primitive.cu:355-381 + Spline_BezierClipping.cu) and
-     the nearest world t kept (RayTracer.cu:369-423 scaling t by 1/|M d|);
+     tolerance 2^-12, rt/Primitive.cu:222-248 + Spline_BezierClipping.cu) and
+     the nearest world t kept (RayTracer.cu:296-350 scaling t by 1/|M d|);
 (iii) against the true root: the FP64 hit is polished with Newton on
      S(u,v) - O - t·d = 0 (the clipper's method error).
 

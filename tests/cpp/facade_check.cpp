@@ -106,12 +106,14 @@ int runRender(const char* inPath, const char* outPath)
     Reader r(inPath);
     rt::Scene scene;
     const uint32_t nsurf = r.get<uint32_t>(), nobj = r.get<uint32_t>(), nlight = r.get<uint32_t>();
+    std::vector<std::pair<std::vector<float>, std::vector<uint32_t>>> surfaces;
     for (uint32_t s = 0; s < nsurf; s++)
     {
         const uint32_t nv = r.get<uint32_t>(), np = r.get<uint32_t>();
         auto v = r.vec<float>(4 * (size_t)nv);
         auto ix = r.vec<uint32_t>(16 * (size_t)np);
         scene.addSurface(v, ix);
+        surfaces.emplace_back(std::move(v), std::move(ix));
     }
     for (uint32_t o = 0; o < nobj; o++)
     {
@@ -159,6 +161,34 @@ int runRender(const char* inPath, const char* outPath)
     auto host = pixels.toHost();
     Writer w(outPath);
     w.vec(std::span<const uint32_t>(host));
+
+    // The reference's usage across instances: a scene built on its own
+    // (the GPU's shared Device) and rendered by two RayTracers at once, each
+    // on a stream of its own.
+    rt::Scene shared;
+    shared.objects = scene.objects;
+    shared.lights = scene.lights;
+    std::memcpy(shared.backgroundColor, scene.backgroundColor, 12);
+    std::memcpy(shared.ambientLight, scene.ambientLight, 12);
+    shared.hashSeed = scene.hashSeed;
+    for (uint32_t s = 0; s < nsurf; s++) shared.addSurface(surfaces[s].first, surfaces[s].second);
+    shared.buildBVH();
+    rt::RayTracer t1({.eps = eps, .flipYAxis = false, .recursionDepth = depth, .device = 0});
+    rt::RayTracer t2({.eps = eps, .flipYAxis = false, .recursionDepth = depth, .device = 0});
+    rt::DeviceArray<uint32_t> p1(t1.device(), (size_t)width * height), p2(t2.device(), (size_t)width * height);
+    rt::Frame f1 {p1.data(), width, height}, f2 {p2.data(), width, height};
+    rt::stream_t s1 = t1.device().createStream(), s2 = t2.device().createStream();
+    for (int k = 0; k < 3; k++)
+    {
+        t1.render(&f1, &cam, &shared, s1);
+        t2.render(&f2, &cam, &shared, s2);
+        t1.device().synchronize(s1);
+        t2.device().synchronize(s2);
+    }
+    t1.device().destroyStream(s1);
+    t2.device().destroyStream(s2);
+    w.vec(std::span<const uint32_t>(p1.toHost()));
+    w.vec(std::span<const uint32_t>(p2.toHost()));
     return 0;
 }
 

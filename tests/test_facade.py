@@ -123,7 +123,10 @@ def test_facade_render_matches_oracle(tmp_path, oracle, c1):
         f.write(np.array([wl.depth], np.int32).tobytes())
     rc, out = run("render", tmp_path / "in", tmp_path / "out")
     assert rc == 0, out
-    frame = np.fromfile(tmp_path / "out", np.uint32).reshape(cam.height, cam.width)
+    # the tracer's own frame, then the frames of two RayTracers rendering one
+    # independently built scene at the same time on two streams
+    frames = np.fromfile(tmp_path / "out", np.uint32).reshape(3, cam.height, cam.width)
     _, oframe, _, _ = oracle.OracleScene(sc).render(cam, eps=wl.eps, depth=wl.depth)
     ch = lambda fr: np.stack([(fr >> s) & 0xFF for s in (0, 8, 16, 24)], -1).astype(np.int32)
-    assert np.abs(ch(frame) - ch(oframe)).max() <= 1
+    assert np.abs(ch(frames[0]) - ch(oframe)).max() <= 1
+    assert np.array_equal(frames[1], frames[0]) and np.array_equal(frames[2], frames[0])

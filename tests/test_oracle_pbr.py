@@ -46,7 +46,7 @@ def test_ggx_peak_and_roughness_clamp(oracle):
         a2 = r ** 4
         # n·h = 1: D = 1/(π α²); FP32 d = (α²-1)+1 cancels to ~ulp(1)/α² relative
         assert math.isclose(oracle.pbr("D", 1.0, r)[0], 1 / (math.pi * a2), rel_tol=4 * 2 ** -23 / a2)
-    # roughness below 1e-3 is clamped (PBR.h:322-330)
+    # roughness below 1e-3 is clamped (PBR.h:87-96)
     assert oracle.pbr("D", 0.999, 0.0)[0] == oracle.pbr("D", 0.999, 1e-3)[0]
 
 

@@ -127,7 +127,7 @@ Scene {
 
 
 def test_meshes_are_objects_without_geometry():
-    """`mesh:` objects (SceneReader.cpp:242-260, the built-in Box or a Mesh
+    """`mesh:` objects (SceneReader.cpp:244-263, the built-in Box or a Mesh
     asset) become objects without a surface (obj_surface = CGS_EMPTY): the
     reference gives meshes empty bounds (Primitive.cu:47-56), so they shape
     the top-level BVH and are never hit.  Order and transforms as for
