@@ -488,7 +488,7 @@ def from_dl(path, width=320, height=240, depth=2, fovy_deg=60.0) -> Workload:
 def camera_rays(cam: Camera, rows=None) -> np.ndarray:
     """World rays (n, 8) float32 (o xyz, tmax, d xyz, 0) through pixel centres,
     in float32 with the reference's camera formula (RayTracer.cu:59-71,
-    207-231)."""
+    141-163)."""
     f32 = np.float32
     w, h = cam.width, cam.height
     half_h = f32(math.tan(f32(0.5) * f32(cam.fovy)))
@@ -496,6 +496,8 @@ def camera_rays(cam: Camera, rows=None) -> np.ndarray:
     half_dx = half_dy
     tl_x = f32(-half_dx * f32(w - 1))
     tl_y = f32(half_h - half_dy)
+    if cam.flip_y:
+        tl_y, half_dy = -tl_y, -half_dy
     js = np.arange(h) if rows is None else np.asarray(rows)
     ii, jj = np.meshgrid(np.arange(w, dtype=f32), js.astype(f32), indexing="xy")
     px = tl_x + half_dx * (f32(2) * ii)

@@ -600,6 +600,23 @@ struct orc_camera
     int flipY;
 };
 
+// Context::pixelRayDirection (RayTracer.cu:59-71) after the render's camera
+// setup (RayTracer.cu:141-163) for a w x h frame, at n fractional pixel
+// coordinates xy (2 floats each); dirs3: n unit directions.
+int orc_pixel_rays(const orc_camera* c, uint32_t w, uint32_t hgt, const float* xy, uint64_t n, float* dirs3)
+{
+    Camera cam {mat16(c->toWorld), {c->position[0], c->position[1], c->position[2]}, c->fovy, c->flipY != 0};
+    CameraSetup cs = setupCamera(cam, w, hgt);
+    for (uint64_t k = 0; k < n; k++)
+    {
+        f3 d = pixelRayDirection(cs, xy[2 * k], xy[2 * k + 1]);
+        dirs3[3 * k] = d.x;
+        dirs3[3 * k + 1] = d.y;
+        dirs3[3 * k + 2] = d.z;
+    }
+    return 0;
+}
+
 // The GPU renderer's semantics (RayTracer.cu), one pixel per iteration.
 // color: w*h*3 floats (unclamped), frame: w*h packed pixels (stride w).
 int orc_scene_render(const orc_scene* h, const orc_camera* c, float eps, int depth, uint32_t w, uint32_t hgt,

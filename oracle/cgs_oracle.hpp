@@ -21,7 +21,23 @@
 // where a box stays empty (the top-level element of an object without
 // geometry: centroid 0 here, NaN under ±infinity).
 //
+// Planted mistakes.  CGS_ORACLE_PLANT selects one deliberate error (0, the
+// default and the only value the Makefile builds, selects none).  The tests
+// build copies with a nonzero value to show that the mathematical pins of
+// the oracle fail on a plausible mistake (tests/planted.py):
+//   1 subdivision reports a region's corner instead of its centre
+//   2 subdivision compares a region's sides with <= instead of < threshold
+//   3 camera: tan(fovy) instead of tan(fovy / 2)
+//   4 camera: pixel centres at i + 1/2 instead of the integer i
+//   5 shading: the factor pi on a hit's colour dropped
+//   6 shading: the reflection attenuation without (1 - roughness)^2
+//   7 scene: an object's local t not rescaled by 1 / |M d|
+//
 #pragma once
+
+#ifndef CGS_ORACLE_PLANT
+#define CGS_ORACLE_PLANT 0
+#endif
 
 #include <algorithm>
 #include <bit>
@@ -737,7 +753,10 @@ bool doSubdivision(PatchHit<T>& hit, V3<T> o, V3<T> d, const V4<T> S[16], T thre
         if (!boxIntersect(pb, o, d, tmin, tmax)) continue;
         T x = T(0.5f) * (e.mn.x + e.mx.x);
         T y = T(0.5f) * (e.mn.y + e.mx.y);
-        if ((e.mx.x - e.mn.x) < threshold && (e.mx.y - e.mn.y) < threshold)
+        const bool small = CGS_ORACLE_PLANT == 2
+                               ? (e.mx.x - e.mn.x) <= threshold && (e.mx.y - e.mn.y) <= threshold
+                               : (e.mx.x - e.mn.x) < threshold && (e.mx.y - e.mn.y) < threshold;
+        if (small)
         {
             V3<T> p00 = project(sub[0]), p10 = project(sub[3]), p01 = project(sub[12]), p11 = project(sub[15]);
             V3<T> uvw, uvw1;
@@ -756,7 +775,7 @@ bool doSubdivision(PatchHit<T>& hit, V3<T> o, V3<T> d, const V4<T> S[16], T thre
             if (b0 && t < hit.distance)
             {
                 hit.distance = t;
-                hit.p = {x, y};
+                hit.p = CGS_ORACLE_PLANT == 1 ? V2<T> {e.mn.x, e.mn.y} : V2<T> {x, y};
                 found = true;
             }
         }

@@ -14,7 +14,9 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "liboracle_cgs.so")
+# CGS_ORACLE_LIB loads another build of the oracle (the tests' planted-mistake
+# copies, tests/planted.py).
+LIB_PATH = os.environ.get("CGS_ORACLE_LIB") or os.path.join(_HERE, "liboracle_cgs.so")
 
 _P = C.c_void_p
 _U32, _U64, _I32, _F32, _F64 = C.c_uint32, C.c_uint64, C.c_int32, C.c_float, C.c_double
@@ -95,6 +97,7 @@ def lib():
         L.orc_cpu_render.argtypes = [_P, C.POINTER(CameraDesc), _F32, C.c_int, _U32, _U32, _P, _P, C.c_int,
                                      _U32, _U32, _U32, _P]
         L.orc_cpu_render.restype = _F64
+        L.orc_pixel_rays.argtypes = [C.POINTER(CameraDesc), _U32, _U32, _P, _U64, _P]
         _lib = L
     return _lib
 
@@ -334,6 +337,16 @@ class OracleScene:
         k = lib().orc_scene_record_tests(self.h, _ptr(rays), rays.shape[0], _ptr(ro), _ptr(po), cap)
         k = min(k, cap)
         return ro[:k], po[:k]
+
+    @staticmethod
+    def pixel_rays(cam, xy):
+        """Unit directions of the camera's rays through fractional pixel
+        coordinates xy (n,2) of its width x height frame (orc_pixel_rays)."""
+        xy = np.ascontiguousarray(xy, np.float32).reshape(-1, 2)
+        out = np.zeros((xy.shape[0], 3), np.float32)
+        c = OracleScene._cam(cam)
+        lib().orc_pixel_rays(C.byref(c), cam.width, cam.height, _ptr(xy), xy.shape[0], _ptr(out))
+        return out
 
     @staticmethod
     def _cam(cam):

@@ -215,7 +215,7 @@ inline int sceneIntersect(const Scene& sc, Hit& hit0, const Ray& ray0, uint32_t*
         bool b = surfaceIntersect(sc.surfaces[o.surface], lh, lr, w);
         if (b)
         {
-            float t = lh.t * d;
+            float t = CGS_ORACLE_PLANT == 7 ? lh.t : lh.t * d;
             if (t < hit.t)
             {
                 hit.coordinates = lh.coordinates;
@@ -417,7 +417,7 @@ inline CameraSetup setupCamera(const Camera& c, uint32_t w, uint32_t h)
     CameraSetup s;
     s.toWorld = c.cameraToWorld;
     s.position = c.position;
-    float halfHeight = std::tan(0.5f * c.fovy);
+    float halfHeight = std::tan(CGS_ORACLE_PLANT == 3 ? c.fovy : 0.5f * c.fovy);
     s.half_dy = halfHeight / (float)h;
     s.half_dx = s.half_dy;
     s.topLeft.x = -s.half_dx * (float)(w - 1);
@@ -432,6 +432,11 @@ inline CameraSetup setupCamera(const Camera& c, uint32_t w, uint32_t h)
 // Context::pixelRayDirection (RayTracer.cu:59-71)
 inline f3 pixelRayDirection(const CameraSetup& c, float x, float y)
 {
+    if constexpr (CGS_ORACLE_PLANT == 4)
+    {
+        x += 0.5f;
+        y += 0.5f;
+    }
     f4 P {c.topLeft.x + c.half_dx * (2 * x), c.topLeft.y - c.half_dy * (2 * y), -1.0f, 1.0f};
     P = c.toWorld * P;
     return versor(project(P) - c.position);
@@ -508,12 +513,12 @@ inline void renderPixel(const Scene& sc, const CameraSetup& cam, const RenderOpt
             f3 I = lightColor(d, light);
             color = color + (dotNL * I) * (diffuse + m.metalness * brdfSpecular(dotHL, dotHN, dotNV, dotNL, m.roughness, m.specular));
         }
-        pl.color = pl.color + (std::numbers::pi_v<float> * color) * pl.attenuation;
+        pl.color = pl.color + ((CGS_ORACLE_PLANT == 5 ? 1.0f : std::numbers::pi_v<float>) * color) * pl.attenuation;
         if (pl.depth <= 0) break;
         f3 reflectance = schlick(m.specular, dotNV);
         f3 R = reflect(-V, N, -dotNV);
         float g = Gsmith(dotNV, dotNV, m.roughness);
-        float rr = 1 - m.roughness;
+        float rr = CGS_ORACLE_PLANT == 6 ? 1.0f : 1 - m.roughness;
         f3 brdf = (rr * rr) * (dotNV * (diffuse + m.metalness * (g * reflectance)));
         f3 a = pl.attenuation * brdf;
         if (a.max() <= 0x1p-8f) break;
@@ -672,7 +677,7 @@ private:
         f3 reflectance = schlick(m.specular, dotNV);
         f3 R = reflect(-V, N, -dotNV);
         float g = Gsmith(dotNV, dotNV, m.roughness);
-        float rr = 1 - m.roughness;
+        float rr = CGS_ORACLE_PLANT == 6 ? 1.0f : 1 - m.roughness;
         f3 brdf = (rr * rr) * (dotNV * (diffuse + m.metalness * (g * reflectance)));
         f3 a = attenuation * brdf;
         if (a.max() > 0x1p-8f)

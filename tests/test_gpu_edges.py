@@ -175,34 +175,28 @@ def test_empty_inputs(ctx, c2_small):
     assert ctx.subdivision(none, np.zeros((0, 16, 4), np.float32)).shape == (0, 4)
 
 
-def test_scene_of_another_context_is_refused(ctx, c1):
-    """A scene belongs to the context that created it: tracing or rendering
-    it through another context is refused before anything is launched."""
-    import ctypes as C
+def test_scene_outlives_the_context_that_made_it(ctx, oracle, c1):
+    """A scene's device arrays are the scene's, not its creating context's:
+    after that context is closed the scene still traces and renders through
+    another one, with the oracle's results."""
     import cgspline_amd as cgs
-    from cgspline_amd import _abi
+    from cgspline_amd import scenes
     other = cgs.Context(0)
-    try:
-        ds = other.scene(c1.scene)
-        rays = np.zeros((4, 8), np.float32)
-        rays[:, 6] = -1.0
-        dr = cgs.DeviceArray.from_host(ctx, rays)
-        dh = cgs.DeviceArray(ctx, 16 * 4)
-        say("trace and render through another context")
-        lib = _abi.lib()
-        invalid = 1
-        assert lib.cgs_trace_closest(ctx.h, ds.h, dr.dptr, 4, dh.dptr, None, None) == invalid
-        assert b"another context" in lib.cgs_last_error()
-        assert lib.cgs_trace_any(ctx.h, ds.h, dr.dptr, 4, dh.dptr, None) == invalid
-        cam = c1.camera
-        cd = _abi.Camera((C.c_float * 16)(*cam.to_world), (C.c_float * 3)(*cam.position), cam.fovy, int(cam.flip_y))
-        op = _abi.RenderOptions(1e-4, 0, 0)
-        df = cgs.DeviceArray(ctx, 4 * cam.width * cam.height)
-        assert lib.cgs_render(ctx.h, ds.h, C.byref(cd), C.byref(op), cam.width, cam.height, df.dptr,
-                              None, None, None) == invalid
-        assert b"another context" in lib.cgs_last_error()
-    finally:
-        other.close()
+    ds = other.scene(c1.scene)
+    say("close the creating context, then trace and render through another")
+    other.close()
+    v = ds.via(ctx)
+    rays = scenes.camera_rays(c1.camera)[::7]
+    h, p, o = v.trace_closest(rays)
+    oh, op, oo, _ = oracle.OracleScene(c1.scene).trace(rays)
+    assert np.array_equal(p, op) and np.array_equal(o, oo)
+    assert np.array_equal(h.view(np.uint32), oh.view(np.uint32))
+    color, _, _ = v.render(c1.camera, eps=c1.eps, depth=c1.depth)
+    ocolor, _, _, _ = oracle.OracleScene(c1.scene).render(c1.camera, eps=c1.eps, depth=c1.depth)
+    assert np.array_equal(color.view(np.uint32), ocolor.view(np.uint32))
+    lib = cgs.lib()
+    lib.cgs_scene_destroy(ds.h)
+    ds.h = None
 
 
 def test_degenerate_rays_bitexact(ctx, oracle, c2_small):
