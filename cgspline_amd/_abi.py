@@ -32,7 +32,7 @@ EXPORTS = [
     "cgs_abi_version", "cgs_status_string", "cgs_last_error", "cgs_open", "cgs_close",
     "cgs_bezier_clip", "cgs_subdivision", "cgs_bvh_build", "cgs_patch_elements", "cgs_mesh_elements", "cgs_free_host", "cgs_load_be",
     "cgs_dl_read", "cgs_dl_parse", "cgs_dl_free",
-    "cgs_phf_displacement_size", "cgs_phf_build", "cgs_phf_fill_table", "cgs_phf_hash",
+    "cgs_phf_displacement_size", "cgs_phf_build", "cgs_phf_build_static", "cgs_phf_fill_table", "cgs_phf_hash",
     "cgs_scene_create", "cgs_scene_destroy", "cgs_scene_bvh_info", "cgs_scene_bvh_copy",
     "cgs_trace_closest", "cgs_trace_any", "cgs_render", "cgs_render_running",
     "cgs_trace_batch", "cgs_batch_create", "cgs_batch_launch", "cgs_batch_destroy", "cgs_batch_set_config",
@@ -141,6 +141,7 @@ def lib():
             "cgs_dl_free": ([C.POINTER(DlScene)], None),
             "cgs_phf_displacement_size": ([_U32], _U32),
             "cgs_phf_build": ([_P, _P, _U32, _U32, _P, C.POINTER(PHF), _P], C.c_int),
+            "cgs_phf_build_static": ([_P, _P, _U32, _U32, _P, C.POINTER(PHF), _P], C.c_int),
             "cgs_phf_fill_table": ([_P, C.POINTER(PHF), _P, _P, _U32, _P, _P], C.c_int),
             "cgs_phf_hash": ([_P, C.POINTER(PHF), _P, _P, _U32, _P, _P], C.c_int),
             "cgs_scene_create": ([_P, C.POINTER(SceneDesc), C.POINTER(_P)], C.c_int),

@@ -55,6 +55,25 @@ constexpr uint32_t phfDisplSize(uint64_t n)
 // Table growth after a failed function: max(4, 0.05·n) (PerfectHashFunction.cu:251,481).
 constexpr uint32_t phfGrowthStep(uint64_t n) { return std::max<uint32_t>(4u, (uint32_t)(0.05f * (float)n)); }
 
+// Device scratch of cgs_phf_build_static, a file-scope array in cgs_phf.hip
+// sized for kStaticPhfMaxKeys keys, in words: the job record, the 16 (A,B)
+// pairs, the result, the build's working arrays and the growing table.
+constexpr uint32_t kStaticPhfMaxKeys = 1u << 19;
+constexpr uint32_t kStaticPhfM = phfDisplSize(kStaticPhfMaxKeys);
+constexpr uint32_t kStaticPhfTcap = kStaticPhfMaxKeys + 16u * phfGrowthStep(kStaticPhfMaxKeys);
+constexpr uint32_t kStaticPhfJobWord = 0, kStaticPhfAbWord = 16, kStaticPhfResultWord = 48, kStaticPhfScratchWord = 64;
+constexpr uint32_t kStaticPhfScratchWords = phfScratchWords(kStaticPhfM, kStaticPhfTcap);
+constexpr uint32_t kStaticPhfTableWord = (kStaticPhfScratchWord + kStaticPhfScratchWords + 63u) & ~63u;
+constexpr uint32_t kStaticPhfWords = kStaticPhfTableWord + kStaticPhfTcap;
+static_assert(sizeof(PhfJob) <= 4 * (kStaticPhfAbWord - kStaticPhfJobWord), "static scratch: job record");
+static_assert(sizeof(PhfResult) <= 4 * (kStaticPhfScratchWord - kStaticPhfResultWord), "static scratch: result record");
+
+// Records the scratch array's address on `device` (the current device) once
+// per process; cgs_open calls it.  phfStaticScratch returns that address, or
+// null before the first cgs_open on the device.
+hipError_t phfStaticInit(int device);
+uint32_t* phfStaticScratch(int device);
+
 hipError_t launchPhfBuild(const PhfJob* jobs, uint32_t njobs, const uint32_t* keys, const uint32_t* ab, uint32_t* scratch,
                           uint32_t* displ, uint32_t* tables, PhfResult* results, hipStream_t s);
 hipError_t launchPhfHash(const cgs_phf& h, const uint32_t* D, const uint32_t* keys, uint32_t n, uint32_t* out, hipStream_t s);

@@ -560,6 +560,8 @@ cgs_status cgs_open(int device, cgs_ctx** out)
     hipError_t e = hipEventCreateWithFlags(&c->renderDone, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->batchDone, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&c->persistDone, hipEventDisableTiming);
+    // the no-allocation hash build's scratch array: its address on this device
+    if (e == hipSuccess) e = phfStaticInit(device);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&c->numCUs, hipDeviceAttributeMultiprocessorCount, device);
     if (e == hipSuccess) e = hipMalloc(&c->dOverflow, 256);
     if (e == hipSuccess) e = hipMemset(c->dOverflow, 0, 256);
@@ -670,6 +672,48 @@ cgs_status cgs_phf_build(cgs_ctx* ctx, const uint32_t* keys, uint32_t n, uint32_
     if (r[0].status != CGS_OK)
         return fail((cgs_status)r[0].status, r[0].status == CGS_ERR_HASH_DUPLICATE ? "PerfectHashFunction: DUPLICATE FOUND!"
                                                                                   : "PerfectHashFunction: bucket overflow");
+    return CGS_OK;
+}
+
+cgs_status cgs_phf_build_static(cgs_ctx* ctx, const uint32_t* keys, uint32_t n, uint32_t seed, uint32_t* displacement,
+                                cgs_phf* out, void* stream)
+{
+    if (!ctx || !out || !displacement || (n && !keys)) return fail(CGS_ERR_INVALID, "cgs_phf_build_static: null argument");
+    if (n == 0) return fail(CGS_ERR_INVALID, "cgs_phf_build_static: empty key set");
+    if (n > kStaticPhfMaxKeys)
+        return fail(CGS_ERR_INVALID, "cgs_phf_build_static: " + std::to_string(n) + " keys exceed its static scratch (at most " +
+                                         std::to_string(kStaticPhfMaxKeys) + " keys); use cgs_phf_build");
+    HIP_TRY(hipSetDevice(ctx->device));
+    uint32_t* base = phfStaticScratch(ctx->device);
+    if (!base) return fail(CGS_ERR_HIP, "cgs_phf_build_static: no scratch address for this device");
+    hipStream_t s = (hipStream_t)stream;
+    PhfJob j {};
+    j.n = n;
+    j.M = displSize(n);
+    j.mask = j.M - 1;
+    j.step = growthStep(n);
+    j.tcap = n + 16 * j.step;
+    if (phfScratchWords(j.M, j.tcap) > kStaticPhfScratchWords || j.tcap > kStaticPhfTcap)
+        return fail(CGS_ERR_INVALID, "cgs_phf_build_static: scratch layout too small");
+    uint32_t ab[32];
+    drawAB(n, seed, ab);
+    PhfResult r {};
+    HIP_TRY(hipMemcpyAsync(base + kStaticPhfJobWord, &j, sizeof j, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(base + kStaticPhfAbWord, ab, sizeof ab, hipMemcpyHostToDevice, s));
+    HIP_TRY(launchPhfBuild((const PhfJob*)(base + kStaticPhfJobWord), 1, keys, base + kStaticPhfAbWord,
+                           base + kStaticPhfScratchWord, displacement, base + kStaticPhfTableWord,
+                           (PhfResult*)(base + kStaticPhfResultWord), s));
+    HIP_TRY(hipMemcpyAsync(&r, base + kStaticPhfResultWord, sizeof r, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    out->A = r.A;
+    out->B = r.B;
+    out->mask = j.mask;
+    out->table_size = r.T;
+    out->attempts = r.attempts;
+    out->displacement_size = j.M;
+    if (r.status != CGS_OK)
+        return fail((cgs_status)r.status, r.status == CGS_ERR_HASH_DUPLICATE ? "PerfectHashFunction: DUPLICATE FOUND!"
+                                                                            : "PerfectHashFunction: bucket overflow");
     return CGS_OK;
 }
 

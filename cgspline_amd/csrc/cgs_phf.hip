@@ -22,6 +22,8 @@
 //
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "cgs_device.hpp"
 #include "cgs_internal.hpp"
 
@@ -274,6 +276,39 @@ __global__ void kFill(uint32_t* __restrict__ p, uint32_t v, size_t n)
 {
     size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Scratch of the no-allocation build (cgs_phf_build_static): one array per
+// device for the life of the process, kStaticPhfWords words (17.3 MiB, room
+// for one build of up to kStaticPhfMaxKeys keys).  Host code takes its
+// address once per device with hipGetSymbolAddress and passes that address
+// to the kernels; every build on a device shares the array, so builds that
+// overlap on one device race.  kPhfBuild initialises every word it reads.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) uint32_t scratch[kStaticPhfWords];
+
+constexpr int kStaticMaxDevices = 64;
+static std::mutex g_staticMu;
+static uint32_t* g_staticPtr[kStaticMaxDevices] = {};
+
+hipError_t phfStaticInit(int device)
+{
+    if (device < 0 || device >= kStaticMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard lk(g_staticMu);
+    if (g_staticPtr[device]) return hipSuccess;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_staticPtr[device] = (uint32_t*)p;
+    return hipSuccess;
+}
+
+uint32_t* phfStaticScratch(int device)
+{
+    if (device < 0 || device >= kStaticMaxDevices) return nullptr;
+    std::lock_guard lk(g_staticMu);
+    return g_staticPtr[device];
 }
 
 hipError_t launchPhfBuild(const PhfJob* jobs, uint32_t njobs, const uint32_t* keys, const uint32_t* ab, uint32_t* scratch,

@@ -275,6 +275,16 @@ void cgs_dl_free(cgs_dl_scene* scene);
 uint32_t cgs_phf_displacement_size(uint32_t n);
 cgs_status cgs_phf_build(cgs_ctx* ctx, const uint32_t* keys, uint32_t n, uint32_t seed,
                          uint32_t* displacement, cgs_phf* out, void* stream);
+/* No counterpart in the reference: cgs_phf_build for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments, results and errors as
+ * cgs_phf_build; it allocates nothing.  Its working memory is one static
+ * device array per GPU (17.3 MiB, held until the process ends), so n is at
+ * most 524,288 keys (CGS_ERR_INVALID above that).  Every call on a GPU
+ * shares that array: calls that overlap on one GPU race, and a caller must
+ * order them (one stream, or events).  Synchronizes `stream`. */
+cgs_status cgs_phf_build_static(cgs_ctx* ctx, const uint32_t* keys, uint32_t n, uint32_t seed,
+                                uint32_t* displacement, cgs_phf* out, void* stream);
 /* BVH::buildHashTable's table fill (rt/BVH.cpp:24-38): table[h(keys[i])] = i,
  * CGS_EMPTY elsewhere.  table: out->table_size device u32. */
 cgs_status cgs_phf_fill_table(cgs_ctx* ctx, const cgs_phf* h, const uint32_t* displacement,
