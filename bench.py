@@ -7,16 +7,30 @@ bounce (+ its shadow ray), GGX shading, fp32.  The scene (BVHs and perfect
 hashes) is built and resident in HBM before the timed region; `value` counts
 every ray the launch traces (primary + shadow + secondary).
 
-Prints one JSON line.  Extra objects: `roofline` (algorithmic bytes of the
-render kernel from the oracle's event counts ÷ the kernel time measured with
-HIP events on the launch stream), `cpu_baseline` (the CPU restatement of
-the reference's CPURayTracer on a bounded row sample, all host threads),
-`first_render` and `pcie_inclusive`.
+Prints one JSON line.  Besides the contract's fields:
+
+* `roofline`: the render kernel against its bound, FP32 VALU: algorithmic
+  FP32 operations per launch (the oracle's event counts × the per-event
+  operation counts of cgspline_amd/workmodel.py) ÷ the kernel's time measured
+  with HIP events on the launch stream.  `traffic` is the measured HBM bytes
+  per launch (profiles/kRender_displaced_grids_hbm_traffic.md) and `hbm` that
+  traffic over the same kernel time; `issue` gives the SQ counters of
+  profiles/kRender_displaced_grids_valu.md.
+* `cpu_baseline`: the CPU restatement of the reference's CPURayTracer, built
+  with -march=native on this host, on the same frame.
+* `first_render`, `pcie_inclusive`: the scene's first render (tiles row by
+  row) and the render with the frame downloaded after each step.
+* `moving_camera`: an orbit of the same view, a small step per frame, each
+  render taking its tiles in the order the previous frame measured.
+* `batched`: 64 resident scenes (1k–200k patches, 4k–2M rays), one
+  persistent batched launch against one trace call per scene on the same rays.
+* `serving`: 8 resident scenes (one of them this 1M-patch scene) under
+  Poisson arrivals of 1k–4M-ray requests at 30/60/90 % of capacity: p50/p99
+  latency and sustained Mrays/s.
 
 Each render records its tiles' times and the next render of the same frame
 size takes the tiles longest first; the warmup renders set that order up, and
-every timed render still traces every ray.  `first_render` is the scene's
-first render, whose tiles go row by row.
+every timed render still traces every ray.
 """
 from __future__ import annotations
 
@@ -47,6 +61,43 @@ def measured_traffic(workload):
         return int(json.load(f)["traffic_bytes_per_launch"])
 
 
+def measured_issue(workload):
+    """The SQ counter summary of kRender on this workload
+    (profiles/kRender_<workload>_valu.json), or None."""
+    path = os.path.join(ROOT, "profiles", f"kRender_{workload}_valu.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def native_oracle():
+    """Builds the oracle with -march=native for this host's CPU baseline (the
+    reference's CXX flags, CMakeLists.txt:83) into a temporary directory;
+    returns (path, flags), or (None, reason) when the build fails and the
+    prebuilt x86-64-v3 library is used instead."""
+    import subprocess
+    import tempfile
+    out = os.path.join(tempfile.mkdtemp(prefix="cgs_bench_"), "liboracle_native.so")
+    flags = ["-O3", "-march=native", "-ffp-contract=off", "-fno-fast-math", "-std=c++20", "-fPIC"]
+    r = subprocess.run(["g++", *flags, "-shared", "-o", out, os.path.join(ROOT, "oracle", "cgs_oracle.cpp"),
+                        "-lpthread"], stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+    if r.returncode != 0:
+        return None, "g++ -march=native failed; the prebuilt -march=x86-64-v3 oracle was used"
+    return out, " ".join(flags)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -69,6 +120,8 @@ def main():
     ap.add_argument("--width", type=int, default=1920, help="--scene-file: frame width")
     ap.add_argument("--height", type=int, default=1080, help="--scene-file: frame height")
     ap.add_argument("--depth", type=int, default=6, help="--scene-file: recursion depth (reference default 6)")
+    ap.add_argument("--no-extras", dest="extras", action="store_false",
+                    help="plain mode: skip the moving-camera, batched and serving sub-objects")
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
                     help="also run one streaming copy kernel (bitwise_not over int32) of this many MiB after the "
                          "warmup: the reference kernel that scales the HBM byte counters (FETCH_SIZE, WRITE_SIZE) "
@@ -222,11 +275,18 @@ def main():
     }
 
     osc = None
+    threads = max(1, min(16, os.cpu_count() or 1))
     if not args.no_work_count or not args.no_cpu_baseline:
+        build = "prebuilt -O3 -march=x86-64-v3 -ffp-contract=off"
+        if not args.no_cpu_baseline:
+            t0 = time.time()
+            lib, build = native_oracle()
+            if lib:
+                os.environ["CGS_ORACLE_LIB"] = lib
+            log(f"oracle for the CPU baseline: {build} ({time.time() - t0:.1f}s)")
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import cgs_oracle
         osc = cgs_oracle.OracleScene(wl.scene)
-    threads = max(1, min(16, os.cpu_count() or 1))
 
     if not args.no_work_count:
         t0 = time.time()
@@ -237,26 +297,36 @@ def main():
             log(f"WARNING: oracle ray count {oracle_rays} != GPU {rays_per_step}")
         nbytes = workmodel.bytes_of(work, cam.width * cam.height, work["objects"])
         nflops = workmodel.flops_of(work)
-        achieved = nbytes / avg_kernel_s / 1e9
-        out["roofline"] = {
-            "bound": "hbm",
-            "achieved": round(achieved, 2),
-            "peak": workmodel.HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / workmodel.HBM_PEAK_GBS, 4),
-            "traffic": measured_traffic(args.workload),
+        tflops = nflops / avg_kernel_s / 1e12
+        traffic = measured_traffic(args.workload)
+        roof = {
+            "bound": "valu",
+            "achieved": round(tflops, 3),
+            "peak": workmodel.FP32_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": round(tflops / workmodel.FP32_PEAK_TFLOPS, 4),
+            "traffic": traffic,
             "kernel": "kRender",
-            "algorithmic_bytes_per_launch": int(nbytes),
             "kernel_ms": round(avg_kernel_s * 1e3, 4),
-            "valu": {
-                "achieved": round(nflops / avg_kernel_s / 1e12, 3),
-                "peak": workmodel.FP32_PEAK_TFLOPS,
-                "unit": "TFLOP/s",
-                "frac": round(nflops / avg_kernel_s / 1e12 / workmodel.FP32_PEAK_TFLOPS, 4),
-                "algorithmic_flops_per_launch": int(nflops),
-            },
-            "work_counts": work,
+            "algorithmic_flops_per_launch": int(nflops),
+            "peak_nonpacked": workmodel.FP32_PEAK_NONPACKED_TFLOPS,
+            "frac_nonpacked": round(tflops / workmodel.FP32_PEAK_NONPACKED_TFLOPS, 4),
+            "note": "FP32 VALU bound: algorithmic FP32 operations of the launch (oracle event counts x "
+                    "workmodel.py) over the kernel time measured with HIP events; 157.3 TFLOP/s assumes "
+                    "packed FP32, which this build disables, so 78.6 is the attainable peak",
         }
+        if traffic is not None:
+            roof["hbm"] = {"achieved": round(traffic / avg_kernel_s / 1e9, 2), "peak": workmodel.HBM_PEAK_GBS,
+                           "unit": "GB/s", "frac": round(traffic / avg_kernel_s / 1e9 / workmodel.HBM_PEAK_GBS, 4),
+                           "note": "measured HBM bytes per launch (FETCH_SIZE / 0.5 + WRITE_SIZE, "
+                                   "profiles/kRender_displaced_grids_hbm_traffic.md) over the kernel time; "
+                                   f"the algorithmic node, hash and patch bytes are {nbytes / 1e9:.1f} GB, "
+                                   "served from L2 and the MALL"}
+        issue = measured_issue(args.workload)
+        if issue is not None:
+            roof["issue"] = issue
+        roof["work_counts"] = work
+        out["roofline"] = roof
 
     if not args.no_cpu_baseline:
         # bounded sample: every `stride`-th row of the frame (uniform over the
@@ -289,8 +359,180 @@ def main():
             "sample": f"{what} {cam.width}x{h} frame ({nrows * cam.width} pixels, {nr + ns} rays per run), "
                       f"median of {len(runs)} runs ({sum(runs):.1f}s CPU wall in total): CPURayTracer "
                       f"restatement (32x32 tiles, {threads} worker threads), same scene",
+            "host_cpus": os.cpu_count(),
+            "cpu_model": cpu_model(),
+            "build": build,
+            "threads_note": f"{threads} threads: the host CPU share one GPU is allotted on this machine "
+                            f"({os.cpu_count()} logical CPUs are visible)",
+            "gpu_over_cpu": round(value / cpu_value, 2),
         }
+
+    if args.extras and not args.scene_file:
+        out["moving_camera"] = moving_camera(ds, wl, frame, counts, stream, args)
+        out["batched"] = batched_leg(ctx, args)
+        out["serving"] = serving_leg(ctx, args, ds, wl)
+        if "batched" in out and out.get("value"):
+            out["batched"]["batched_over_plain_render"] = round(out["batched"]["value"] / value, 3)
     print(json.dumps(out), flush=True)
+
+
+def moving_camera(ds, wl, frame, counts, stream, args):
+    """An orbit about the vertical through the point of the view axis nearest
+    the objects' mean position, 0.25 degrees per frame: each render takes its
+    tiles in the order the previous frame measured (the first in the static
+    view's order)."""
+    import math as _m
+    import torch
+    import cgspline_amd as cgs
+    from cgspline_amd import scenes
+    cam0 = wl.camera
+    pos = np.asarray(cam0.position, np.float64)
+    M = cam0.to_world.reshape(4, 4).astype(np.float64)
+    fwd = -M[2, :3] / np.linalg.norm(M[2, :3])
+    mean = wl.scene.obj_local2world[:, 12:15].astype(np.float64).mean(axis=0)
+    centre = pos + max(1.0, float((mean - pos) @ fwd)) * fwd
+    rel = pos - centre
+    frames = max(4, args.steps)
+    cams = []
+    for k in range(1, frames + 1):
+        a = _m.radians(0.25 * k)
+        R = np.array([[_m.cos(a), -_m.sin(a), 0.0], [_m.sin(a), _m.cos(a), 0.0], [0.0, 0.0, 1.0]])
+        p = centre + R @ rel
+        cams.append(scenes.make_camera(p, centre, (0.0, 0.0, 1.0), _m.degrees(cam0.fovy), cam0.width, cam0.height))
+    sh = stream.cuda_stream
+    counts.zero_()
+    torch.cuda.synchronize()
+    ms = []
+    t0 = time.perf_counter()
+    for c in cams:
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        cgs.check(ds.render_dev(c, frame, None, counts, eps=wl.eps, depth=wl.depth, stream=sh), "cgs_render")
+        b.record(stream)
+        b.synchronize()
+        ms.append(a.elapsed_time(b))
+    wall = time.perf_counter() - t0
+    cnt = counts.cpu().numpy().astype(np.int64)
+    rays = int(cnt[0] + cnt[1] + cnt[2])
+    res = {"value": round(rays / wall / 1e6, 3), "unit": "Mrays/s", "frames": frames,
+           "ms_per_frame": round(wall * 1e3 / frames, 4), "kernel_ms_mean": round(float(np.mean(ms)), 4),
+           "kernel_ms_max": round(float(np.max(ms)), 4), "degrees_per_frame": 0.25,
+           "note": "orbit about a point on the view axis; each render takes its tiles in the order measured "
+                   "by the previous frame"}
+    log(f"moving camera: {res}")
+    return res
+
+
+def batched_leg(ctx, args):
+    """The 64-scene batched workload: one persistent batched launch (and its
+    autotuned shape and graph replay) against one cgs_trace_closest per scene
+    on the same rays, results compared."""
+    import torch
+    import cgspline_amd as cgs
+    from cgspline_amd import scenes
+    t0 = time.time()
+    items = scenes.batched_scenes()
+    npatch = sum(i.scene.npatch for i in items)
+    nray = sum(i.rays.shape[0] for i in items)
+    jobs = []
+    for it in items:
+        d = ctx.scene(it.scene, hash_retries=8)
+        n = it.rays.shape[0]
+        jobs.append(dict(scene=d, rays=cgs.DeviceArray.from_host(ctx, it.rays), n=n,
+                         hits=cgs.DeviceArray(ctx, 16 * n), objects=cgs.DeviceArray(ctx, 4 * n)))
+    log(f"batched: {len(items)} scenes, {npatch} patches, {nray} rays resident ({time.time() - t0:.1f}s)")
+    stream = torch.cuda.current_stream(torch.device("cuda:0"))
+    sh = stream.cuda_stream
+    reps = max(3, min(args.steps, 10))
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(reps):
+            fn()
+        b.record(stream)
+        b.synchronize()
+        return a.elapsed_time(b) / reps
+
+    def per_job():
+        for j in jobs:
+            j["scene"].trace_closest_dev(j["rays"], j["n"], j["hits"], j["objects"], stream=sh)
+
+    ms_per_job = timed(per_job)
+    ref = [j["hits"].to_host(np.uint32, (j["n"], 4)) for j in jobs]
+    tuned = ctx.batch_autotune(jobs)
+    ms_batch = timed(lambda: ctx.trace_batch(jobs, stream=sh))
+    same = all(np.array_equal(j["hits"].to_host(np.uint32, (j["n"], 4)), r) for j, r in zip(jobs, ref))
+    graph = ctx.batch(jobs)
+    ms_graph = timed(lambda: graph.launch(sh))
+    graph.destroy()
+    best = min(ms_batch, ms_graph)
+    res = {"value": round(nray / best / 1e3, 3), "unit": "Mrays/s", "scenes": len(items), "patches": npatch,
+           "rays": nray, "ms_per_pass": round(best, 4), "per_job_value": round(nray / ms_per_job / 1e3, 3),
+           "batched_over_per_job": round(ms_per_job / best, 3),
+           "forms_ms": {"per_job": round(ms_per_job, 3), "batched": round(ms_batch, 3), "graph": round(ms_graph, 3)},
+           "autotuned_shape": {"chunk": tuned[0], "blocks_per_cu": tuned[1]},
+           "results_equal_per_job": bool(same),
+           "note": "closest hits of half camera, half random rays; the batched results equal the per-job "
+                   "results bit for bit (same per-ray algorithm), so the accuracy is equal"}
+    log(f"batched: {res}")
+    for j in jobs:
+        j["scene"].destroy()
+    return res
+
+
+def serving_leg(ctx, args, big_ds, big_wl):
+    """8 resident scenes (this 1M-patch scene and 7 of 10k-1M patches):
+    capacity, then Poisson arrivals at 30/60/90 % of it."""
+    from cgspline_amd import scenes
+    from cgspline_amd.serving import Server, request_sizes, run_load
+    t0 = time.time()
+    items = scenes.batched_scenes(seed=20261202 + 5, count=7, patches=(10_000, 1_000_000), rays=(1 << 20, 1 << 21))
+    ds = [ctx.scene(it.scene, hash_retries=8) for it in items] + [big_ds]
+    pools = [it.rays for it in items] + [scenes.camera_rays(big_wl.camera)]
+    log(f"serving: {len(ds)} resident scenes ({time.time() - t0:.1f}s)")
+
+    def make_requests(count, seed):
+        r = np.random.default_rng(seed)
+        sizes = request_sizes(r, count)
+        reqs = []
+        for n in sizes:
+            k = int(r.integers(0, len(ds)))
+            pool = pools[k]
+            n = int(min(n, pool.shape[0]))
+            start = int(r.integers(0, pool.shape[0] - n + 1))
+            reqs.append((k, pool[start:start + n], bool(r.random() < 0.4)))
+        return reqs
+
+    budget = 8 << 30
+    sv = Server(ctx, ds, hbm_budget=budget, slots=args.slots, max_batch_rays=args.max_batch_rays)
+    run_load(sv, make_requests(8, 1), rate_hz=0.0)
+    nreq = max(24, 2 * args.steps)
+    cap = make_requests(nreq, 2)
+    _, nrays, wall = run_load(sv, cap, rate_hz=0.0)
+    capacity = nrays / wall
+    mean_rays = nrays / len(cap)
+    loads = {}
+    for frac in (0.3, 0.6, 0.9):
+        reqs = make_requests(nreq, int(frac * 100))
+        rate = frac * capacity / mean_rays
+        lats, nr, w = run_load(sv, reqs, rate_hz=rate, seed=int(frac * 10))
+        loads[f"{int(frac * 100)}%"] = {"offered_req_per_s": round(rate, 2), "requests": len(reqs),
+                                        "p50_ms": round(float(np.percentile(lats, 50)) * 1e3, 2),
+                                        "p99_ms": round(float(np.percentile(lats, 99)) * 1e3, 2),
+                                        "sustained_mrays_s": round(nr / w / 1e6, 2)}
+    sv.close()
+    res = {"capacity": round(capacity / 1e6, 3), "unit": "Mrays/s", "scenes": len(ds),
+           "patches": sum(int(d.scene.npatch) for d in ds), "hbm_budget_bytes": budget, "slots": args.slots,
+           "max_batch_rays": args.max_batch_rays, "request_rays": "log-uniform [1e3, 4e6]",
+           "occlusion_fraction": 0.4, "loads": loads,
+           "note": "latency is submit to results in the caller's host arrays (host copies and PCIe included)"}
+    log(f"serving: {res}")
+    for d in ds[:-1]:
+        d.destroy()
+    return res
 
 
 def run_batched(args):

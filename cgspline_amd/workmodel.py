@@ -39,9 +39,12 @@ B_NODE, B_PROBE, B_LEAF, B_PATCH, B_OBJECT, B_PIXEL, B_COLOR = 64, 8, 8, 256, 16
 
 F_NODE, F_PROJ, F_CLIP, F_SUBDIV, F_PRUNE, F_CONVERGED, F_ONHIT = 54, 248, 551, 835, 259, 54, 222
 
-# MI355X published peaks
+# MI355X published peaks.  The FP32 vector peak counts packed (two-wide)
+# FP32 instructions; without them (this library is built with packed FP32
+# disabled) a CU issues 64 FP32 FMA lanes per clock: 256 CUs x 2.4 GHz x 128.
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
+FP32_PEAK_NONPACKED_TFLOPS = 78.6
 
 
 def bytes_of(work: dict, pixels: int, objects_visited: int = 0, color: bool = False) -> float:
