@@ -37,17 +37,21 @@ extern "C" {
 struct orc_work
 {
     uint64_t nodes, leaves, hashProbes, patchTests, clipIters, subdivs, prunes, converged, onHits, maxStack, objects,
-        maxPostponed;
+        maxPostponed, overflows;
 };
 
 static void putWork(orc_work* o, const Work& w)
 {
     if (!o) return;
     *o = {w.nodes, w.leaves, w.hashProbes, w.patchTests, w.clipIters, w.subdivs, w.prunes, w.converged, w.onHits, w.maxStack, w.objects,
-          w.maxPostponed};
+          w.maxPostponed, w.overflows};
 }
 
 int orc_version(void) { return 1; }
+
+// Caps the clip's region stack of the calling thread (0: unbounded, the
+// reference's semantics); see clipStackCap.
+void orc_set_clip_stack_cap(int cap) { clipStackCap = cap; }
 
 // --- per-patch intersection ------------------------------------------------
 // rays: n × 8 floats (ox, oy, oz, tmax, dx, dy, dz, pad); the direction is

@@ -206,8 +206,10 @@ struct Work
     uint64_t maxStack = 0;    // deepest clipping stack seen
     uint64_t objects = 0;     // top-level leaf element (object) tests
     uint64_t maxPostponed = 0; // most siblings postponed at once in one traversal
+    uint64_t overflows = 0;   // clips stopped by the optional stack cap (clipStackCap)
     void add(const Work& o)
     {
+        overflows += o.overflows;
         objects += o.objects;
         maxPostponed = std::max(maxPostponed, o.maxPostponed);
         nodes += o.nodes; leaves += o.leaves; hashProbes += o.hashProbes;
@@ -456,6 +458,13 @@ inline bool approximateIntersection(F&& onHit, const V2<T> p[16], V2<T> pmin, V2
     return found;
 }
 
+// Optional cap on the clip's region stack (0: unbounded, the reference's
+// host semantics).  With a cap the clip stops, as the GPU clipper does, when
+// a split would store more than `cap` regions below the current one, and
+// counts an overflow; test infrastructure for finding such rays without
+// running an unbounded clip.
+inline thread_local int clipStackCap = 0;
+
 template<class T> struct ClipConst
 {
     static constexpr T over3[4] {T(0.0f), T(1.0f / 3.0f), T(2.0f / 3.0f), T(1.0f)};
@@ -584,6 +593,11 @@ bool clip2D(F&& onHit, const V2<T> patch[16], T tol, Work* w = nullptr)
         else if (delta > T(0.8f))
         {
             // :373-409 — split in half; the upper half is pushed (processed first)
+            if (clipStackCap > 0 && S.size() - 1 >= (size_t)clipStackCap)
+            {
+                if (w) w->overflows++;
+                return found;
+            }
             if (w) w->subdivs++;
             State s1;
             for (int i = 0; i < 16; i++) s1.p[i] = e.p[i];

@@ -25,7 +25,7 @@ _U32, _U64, _I32, _F32, _F64 = C.c_uint32, C.c_uint64, C.c_int32, C.c_float, C.c
 class Work(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in (
         "nodes", "leaves", "hashProbes", "patchTests", "clipIters", "subdivs", "prunes",
-        "converged", "onHits", "maxStack", "objects", "maxPostponed")]
+        "converged", "onHits", "maxStack", "objects", "maxPostponed", "overflows")]
 
     def asdict(self):
         return {n: int(getattr(self, n)) for n, _ in self._fields_}
@@ -97,6 +97,7 @@ def lib():
         L.orc_cpu_render.argtypes = [_P, C.POINTER(CameraDesc), _F32, C.c_int, _U32, _U32, _P, _P, C.c_int,
                                      _U32, _U32, _U32, _P]
         L.orc_cpu_render.restype = _F64
+        L.orc_set_clip_stack_cap.argtypes = [C.c_int]
         L.orc_pixel_rays.argtypes = [C.POINTER(CameraDesc), _U32, _U32, _P, _U64, _P]
         _lib = L
     return _lib
@@ -123,6 +124,13 @@ def clip_batch(rays, patches, tol=2.0 ** -12, fp64=False, method=0, threshold=2.
         out = np.zeros((n, 4), np.float32)
         lib().orc_clip_batch(0, _ptr(rays), _ptr(patches), n, tol, _ptr(out), None, C.byref(w), method, threshold)
     return out, w.asdict()
+
+
+def set_clip_stack_cap(cap: int) -> None:
+    """Caps the clipper's region stack on this thread like the GPU clipper
+    (stop and count an overflow when a split would store more than `cap`
+    regions); 0 restores the reference's unbounded stack."""
+    lib().orc_set_clip_stack_cap(int(cap))
 
 
 def clip2d_candidates(p2d, tol=2.0 ** -12, cap=256):

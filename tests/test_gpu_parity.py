@@ -375,3 +375,19 @@ def test_render_into_padded_frame(ctx, oracle, c1, pad):
     assert np.all(out[:, w:] == 0x5A5A5A5A)
     with pytest.raises(cgs.CgsError, match="stride"):
         cgs.check(ds.render_dev(wl.camera, df, eps=wl.eps, depth=wl.depth, stride=w - 1), "cgs_render")
+
+
+def test_trace_equals_golden_c1_hits(ctx, c1):
+    """The plain CPU workload's committed golden hits
+    (tests/golden/c1_heightfield_hits.npz), bit for bit."""
+    import os
+    import conftest
+    from cgspline_amd import scenes
+    g = np.load(os.path.join(conftest.ROOT, "tests", "golden", "c1_heightfield_hits.npz"))
+    rays = scenes.camera_rays(c1.camera)
+    say("golden c1 hits", rays.shape[0], "rays")
+    tuv, patch, obj = ctx.scene(c1.scene).trace_closest(rays)
+    assert np.array_equal(np.where(obj >= 0, patch, np.uint32(EMPTY)), g["patch"])
+    for k, name in enumerate("tuv"):
+        hit = obj >= 0
+        assert np.array_equal(bits(tuv[hit, k]), g[name][hit].view(np.uint32)), name
