@@ -15,8 +15,8 @@ import numpy as np
 from . import _abi, scenes
 from ._abi import CgsError, EMPTY, PATCH_OVERFLOW, check, lib, ptr
 
-__all__ = ["Batch", "Context", "DeviceArray", "DeviceScene", "CgsError", "DlScene", "EMPTY", "PATCH_OVERFLOW", "load_be",
-           "read_dl", "scenes", "lib"]
+__all__ = ["Batch", "CLIPPING", "Context", "DeviceArray", "DeviceScene", "CgsError", "DlScene", "EMPTY", "PATCH_OVERFLOW",
+           "SUBDIVISION", "load_be", "read_dl", "scenes", "lib", "trace_options"]
 
 
 def load_be(source) -> tuple[np.ndarray, np.ndarray]:
@@ -97,6 +97,18 @@ def read_dl(source, name: str = "<text>", working_dir=None) -> DlScene:
     finally:
         lib().cgs_dl_free(out)
     return res
+
+
+CLIPPING = 0
+SUBDIVISION = 1
+
+
+def trace_options(method=CLIPPING, clip_tol=0.0, subdiv_depth=0, max_clip_iters=0, polish=False):
+    """A cgs_trace_options record; every default (zero) is the reference's
+    leaf call: clipping at 2^-12, no cap, no polish."""
+    if method not in (CLIPPING, SUBDIVISION):
+        raise ValueError(f"unknown leaf method {method!r}")
+    return _abi.TraceOptions(int(method), float(clip_tol), int(subdiv_depth), int(max_clip_iters), int(bool(polish)))
 
 
 class DeviceArray:
@@ -373,12 +385,22 @@ class DeviceScene:
                     attempts=h.attempts, displacement=D, table=T)
 
     # device-pointer entry points (for resident buffers)
-    def trace_closest_dev(self, rays, n, hits, objects=None, stream=None):
-        check(lib().cgs_trace_closest(self.ctx.h, self.h, ptr(rays), n, ptr(hits), ptr(objects), stream),
-              "cgs_trace_closest")
+    def trace_closest_dev(self, rays, n, hits, objects=None, stream=None, **options):
+        """cgs_trace_closest, or cgs_trace_closest_ex with trace_options(**options)."""
+        if options:
+            check(lib().cgs_trace_closest_ex(self.ctx.h, self.h, ptr(rays), n, ptr(hits), ptr(objects),
+                                             C.byref(trace_options(**options)), stream), "cgs_trace_closest_ex")
+        else:
+            check(lib().cgs_trace_closest(self.ctx.h, self.h, ptr(rays), n, ptr(hits), ptr(objects), stream),
+                  "cgs_trace_closest")
 
-    def trace_any_dev(self, rays, n, occluded, stream=None):
-        check(lib().cgs_trace_any(self.ctx.h, self.h, ptr(rays), n, ptr(occluded), stream), "cgs_trace_any")
+    def trace_any_dev(self, rays, n, occluded, stream=None, **options):
+        """cgs_trace_any, or cgs_trace_any_ex with trace_options(**options)."""
+        if options:
+            check(lib().cgs_trace_any_ex(self.ctx.h, self.h, ptr(rays), n, ptr(occluded),
+                                         C.byref(trace_options(**options)), stream), "cgs_trace_any_ex")
+        else:
+            check(lib().cgs_trace_any(self.ctx.h, self.h, ptr(rays), n, ptr(occluded), stream), "cgs_trace_any")
 
     def render_dev(self, camera: "scenes.Camera", frame, color=None, counts=None, eps=1e-4, depth=0, stream=None,
                    stride=0):
@@ -391,24 +413,27 @@ class DeviceScene:
                                 ptr(frame), ptr(color), ptr(counts), stream)
 
     # host convenience wrappers
-    def trace_closest(self, rays: np.ndarray):
-        """Closest hits of host rays (n, 8): returns (t,u,v (n,3), patch (n,), object (n,))."""
+    def trace_closest(self, rays: np.ndarray, **options):
+        """Closest hits of host rays (n, 8): returns (t,u,v (n,3), patch (n,), object (n,)).
+        options: trace_options' keywords (method, clip_tol, subdiv_depth,
+        max_clip_iters, polish)."""
         rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
         n = rays.shape[0]
         dr = DeviceArray.from_host(self.ctx, rays)
         dh = DeviceArray(self.ctx, 16 * n)
         do = DeviceArray(self.ctx, 4 * n)
-        self.trace_closest_dev(dr, n, dh, do)
+        self.trace_closest_dev(dr, n, dh, do, **options)
         self.ctx.sync()
         h = dh.to_host(np.float32, (n, 4))
         return h[:, :3].copy(), h.view(np.uint32)[:, 3].copy(), do.to_host(np.int32, (n,))
 
-    def trace_any(self, rays: np.ndarray) -> np.ndarray:
+    def trace_any(self, rays: np.ndarray, **options) -> np.ndarray:
+        """Occlusion of host rays (n, 8): 1 / 0 (2: clip-stack overflow)."""
         rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
         n = rays.shape[0]
         dr = DeviceArray.from_host(self.ctx, rays)
         do = DeviceArray(self.ctx, 4 * n)
-        self.trace_any_dev(dr, n, do)
+        self.trace_any_dev(dr, n, do, **options)
         self.ctx.sync()
         return do.to_host(np.uint32, (n,))
 

@@ -34,7 +34,8 @@ EXPORTS = [
     "cgs_dl_read", "cgs_dl_parse", "cgs_dl_free",
     "cgs_phf_displacement_size", "cgs_phf_build", "cgs_phf_build_static", "cgs_phf_fill_table", "cgs_phf_hash",
     "cgs_scene_create", "cgs_scene_destroy", "cgs_scene_bvh_info", "cgs_scene_bvh_copy",
-    "cgs_trace_closest", "cgs_trace_any", "cgs_render", "cgs_render_running",
+    "cgs_trace_closest", "cgs_trace_any", "cgs_trace_closest_ex", "cgs_trace_any_ex", "cgs_render",
+    "cgs_render_running",
     "cgs_trace_batch", "cgs_batch_create", "cgs_batch_launch", "cgs_batch_destroy", "cgs_batch_set_config",
     "cgs_batch_get_config", "cgs_batch_autotune",
     "cgs_server_create", "cgs_server_submit", "cgs_server_poll", "cgs_server_wait", "cgs_server_destroy",
@@ -106,6 +107,11 @@ class RenderOptions(C.Structure):
     _fields_ = [("eps", _F32), ("recursion_depth", _I32), ("frame_stride", _U32)]
 
 
+class TraceOptions(C.Structure):
+    _fields_ = [("method", _U32), ("clip_tol", _F32), ("subdiv_depth", _U32), ("max_clip_iters", _U32),
+                ("polish", _U32), ("reserved", _U32 * 3)]
+
+
 class RayCounts(C.Structure):
     _fields_ = [("primary", _U64), ("shadow", _U64), ("secondary", _U64), ("overflow", _U64)]
 
@@ -150,6 +156,8 @@ def lib():
             "cgs_scene_bvh_copy": ([_P, _I32, _P, _P, _P, _P, _P], C.c_int),
             "cgs_trace_closest": ([_P, _P, _P, _U32, _P, _P, _P], C.c_int),
             "cgs_trace_any": ([_P, _P, _P, _U32, _P, _P], C.c_int),
+            "cgs_trace_closest_ex": ([_P, _P, _P, _U32, _P, _P, C.POINTER(TraceOptions), _P], C.c_int),
+            "cgs_trace_any_ex": ([_P, _P, _P, _U32, _P, C.POINTER(TraceOptions), _P], C.c_int),
             "cgs_render": ([_P, _P, C.POINTER(Camera), C.POINTER(RenderOptions), _U32, _U32, _P, _P, _P, _P],
                            C.c_int),
             "cgs_render_running": ([_P], C.c_int),

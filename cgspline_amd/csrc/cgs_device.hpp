@@ -294,9 +294,11 @@ CGS_DEV ClipStack clipStackFor(float2* buf, uint32_t lane, uint32_t lanes) { ret
 // updated by accepted candidates (t < best && V·d > 0).  With ANY the test
 // stops at the first accepted candidate (the caller only needs "found").
 // Returns found; sets overflow when the region stack is exhausted.
-template<bool ANY>
+// CAPPED (an opt-in mode with no counterpart in the reference): the test
+// stops after maxIters passes of its loop and keeps what it found so far.
+template<bool ANY, bool CAPPED = false>
 CGS_DEV bool clipPatch(const float4* __restrict__ S, const ClipRay& R, float tol, float& best, float& bu, float& bv,
-                       const ClipStack& stk, bool& overflow)
+                       const ClipStack& stk, bool& overflow, uint32_t maxIters = 0)
 {
     V2 p[16];
 #pragma unroll
@@ -335,8 +337,10 @@ CGS_DEV bool clipPatch(const float4* __restrict__ S, const ClipRay& R, float tol
     // negation, L = (p15-p0) + (p12-p3) in transposed storage equals
     // (p15-p0) - (p12-p3) in natural storage (x + (-y) == x - y), and the
     // envelope visits the same points in the same order.
+    uint32_t iters = 0;
     while (true)
     {
+        if (CAPPED && iters++ >= maxIters) break;
         bool pop = false;
         V2 t0 {p[3].x - p[12].x, p[3].y - p[12].y};
         V2 t1 {p[15].x - p[0].x, p[15].y - p[0].y};
@@ -513,6 +517,116 @@ CGS_DEV bool clipPatch(const float4* __restrict__ S, const ClipRay& R, float tol
         }
     }
     return found;
+}
+
+// Newton polish of a closest hit (an opt-in refinement with no counterpart
+// in the reference).  The clipper stops when the projected sub-patch is
+// below an absolute 2^-12, which leaves (u, v) uncertain by about
+// 2^-12 / |dS/du|; and in FP32 an object-local ray far from a small object
+// (its origin thousands of local units away) is itself uncertain by an ulp
+// of that distance.  So the polish runs in FP64 (the GPU's FP64 vector rate
+// is that of its unpacked FP32): the local ray o_l + t M3 d is rebuilt in
+// FP64 from the FP32 world ray and world→local matrix, with t the world ray
+// parameter, and up to six Newton steps on S(u,v) - o_l - t M3 d = 0
+// (Cramer's rule for the 3x3 step, the quotient rule for w) refine (u, v, t).
+// The result is kept when the steps converge (the last below 1e-9 in u and
+// v) inside the parameter square with t > 0; otherwise the hit stands.
+struct D3 { double x, y, z; };
+CGS_DEV D3 dsub(D3 a, D3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+CGS_DEV D3 dmul(double s, D3 a) { return {s * a.x, s * a.y, s * a.z}; }
+CGS_DEV double ddot(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+CGS_DEV D3 dcross(D3 a, D3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+
+// x.x*a + x.y*b + x.z*c + x.w*d of the points' four components, in FP64
+CGS_DEV void dcomb4(const double* a, const double* b, const double* c, const double* d, const double* x, double* r)
+{
+#pragma unroll
+    for (int k = 0; k < 4; k++) r[k] = ((x[0] * a[k] + x[1] * b[k]) + x[2] * c[k]) + x[3] * d[k];
+}
+CGS_DEV void dbasis(double t, double* b, double* db)
+{
+    const double t2 = t * t, t3 = t2 * t;
+    // the cubic Bernstein polynomials and their derivatives in power form
+    b[0] = ((1.0 - 3.0 * t) + 3.0 * t2) - t3;
+    b[1] = (3.0 * t - 6.0 * t2) + 3.0 * t3;
+    b[2] = 3.0 * t2 - 3.0 * t3;
+    b[3] = t3;
+    db[0] = (-3.0 + 6.0 * t) - 3.0 * t2;
+    db[1] = (3.0 - 12.0 * t) + 9.0 * t2;
+    db[2] = 6.0 * t - 9.0 * t2;
+    db[3] = 3.0 * t2;
+}
+
+CGS_DEV bool polishHit64(const float4* __restrict__ S, const float4 (&w2l)[4], V3 ow, V3 dw, float& t, float& u, float& v)
+{
+    // local origin (project of M (o, 1)) and the unnormalised local direction
+    double M[4][4];
+#pragma unroll
+    for (int c = 0; c < 4; c++)
+    {
+        M[c][0] = w2l[c].x; M[c][1] = w2l[c].y; M[c][2] = w2l[c].z; M[c][3] = w2l[c].w;
+    }
+    const double ox = ow.x, oy = ow.y, oz = ow.z, dx = dw.x, dy = dw.y, dz = dw.z;
+    double O4[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) O4[k] = ((ox * M[0][k] + oy * M[1][k]) + oz * M[2][k]) + M[3][k];
+    const D3 o {O4[0] / O4[3], O4[1] / O4[3], O4[2] / O4[3]};
+    const D3 D {(dx * M[0][0] + dy * M[1][0]) + dz * M[2][0], (dx * M[0][1] + dy * M[1][1]) + dz * M[2][1],
+                (dx * M[0][2] + dy * M[1][2]) + dz * M[2][2]};
+    double pu = u, pv = v, pt = t;
+    bool converged = false;
+    for (int it = 0; it < 6; it++)
+    {
+        double bu[4], dbu[4], bv[4], dbv[4];
+        dbasis(pu, bu, dbu);
+        dbasis(pv, bv, dbv);
+        // row j of the control net, combined along u (read per step rather
+        // than held: 64 doubles of registers)
+        double a[4][4], ad[4][4];
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+        {
+            double q[4][4];
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+            {
+                const float4 c = S[4 * j + i];
+                q[i][0] = c.x; q[i][1] = c.y; q[i][2] = c.z; q[i][3] = c.w;
+            }
+            dcomb4(q[0], q[1], q[2], q[3], bu, a[j]);
+            dcomb4(q[0], q[1], q[2], q[3], dbu, ad[j]);
+        }
+        double X[4], Xu[4], Xv[4];
+        dcomb4(a[0], a[1], a[2], a[3], bv, X);
+        dcomb4(ad[0], ad[1], ad[2], ad[3], bv, Xu);
+        dcomb4(a[0], a[1], a[2], a[3], dbv, Xv);
+        const double iw = 1.0 / X[3];
+        const D3 p {X[0] * iw, X[1] * iw, X[2] * iw};
+        const D3 Pu {(Xu[0] - p.x * Xu[3]) * iw, (Xu[1] - p.y * Xu[3]) * iw, (Xu[2] - p.z * Xu[3]) * iw};
+        const D3 Pv {(Xv[0] - p.x * Xv[3]) * iw, (Xv[1] - p.y * Xv[3]) * iw, (Xv[2] - p.z * Xv[3]) * iw};
+        const D3 F = dsub(dsub(p, o), dmul(pt, D));
+        const D3 mD {-D.x, -D.y, -D.z};
+        const double det = ddot(Pu, dcross(Pv, mD));
+        if (!(det > 1e-300 || det < -1e-300)) return false;
+        const double idet = 1.0 / det;
+        const D3 nF {-F.x, -F.y, -F.z};
+        const double du = ddot(nF, dcross(Pv, mD)) * idet;
+        const double dv = ddot(Pu, dcross(nF, mD)) * idet;
+        const double dt = ddot(Pu, dcross(Pv, nF)) * idet;
+        pu += du;
+        pv += dv;
+        pt += dt;
+        if ((du < 1e-9 && du > -1e-9) && (dv < 1e-9 && dv > -1e-9))
+        {
+            converged = true;
+            break;
+        }
+    }
+    if (!converged || !(pu >= 0.0 && pu <= 1.0 && pv >= 0.0 && pv <= 1.0 && pt > 0.0)) return false;
+    t = (float)pt;
+    u = (float)pu;
+    v = (float)pv;
+    return true;
 }
 
 // The same with the mode chosen per lane at run time.

@@ -103,10 +103,21 @@ struct PersistArgs
 
 hipError_t persistentGrid(PersistKernel k, uint32_t& blocksPerCU);
 
+// Leaf method of a ray query (cgs_trace_options): the default is Bézier
+// clipping at the reference's tolerance, unpolished and uncapped.
+struct TraceMode
+{
+    uint32_t method = CGS_METHOD_CLIPPING;
+    uint32_t polish = 0;
+    float tol = 0x1p-12f;
+    float threshold = 0x1p-4f;
+    uint32_t maxIters = 0;
+};
+
 hipError_t launchTraceClosest(const SceneDev& sc, const PersistArgs& pa, const cgs_ray* rays, uint32_t n, cgs_hit* hits,
-                              int32_t* objects, unsigned long long* overflow, hipStream_t s);
+                              int32_t* objects, unsigned long long* overflow, hipStream_t s, const TraceMode& m);
 hipError_t launchTraceAny(const SceneDev& sc, const PersistArgs& pa, const cgs_ray* rays, uint32_t n, uint32_t* occluded,
-                          unsigned long long* overflow, hipStream_t s);
+                          unsigned long long* overflow, hipStream_t s, const TraceMode& m);
 // Batched queries; the counter must be zero at launch (cleared with a fill
 // kernel, so the pair can be captured in a graph).
 hipError_t launchTraceBatch(const BatchJob* jobs, uint32_t njobs, uint32_t total, uint32_t chunk, float2* stack,

@@ -984,33 +984,67 @@ cgs_status cgs_scene_bvh_copy(cgs_scene* sc, int32_t surface, cgs_bvh_node* node
     return CGS_OK;
 }
 
-cgs_status cgs_trace_closest(cgs_ctx* ctx, cgs_scene* sc, const cgs_ray* rays, uint32_t n, cgs_hit* hits, int32_t* objects,
-                             void* stream)
+// The trace mode of a query's options (null: the default).
+static cgs_status traceMode(const cgs_trace_options* o, TraceMode& m, const char* what)
+{
+    m = TraceMode {};
+    if (!o) return CGS_OK;
+    if (o->method != CGS_METHOD_CLIPPING && o->method != CGS_METHOD_SUBDIVISION)
+        return fail(CGS_ERR_INVALID, std::string(what) + ": unknown method");
+    if (o->subdiv_depth > 16) return fail(CGS_ERR_INVALID, std::string(what) + ": subdivision depth above 16");
+    if (!(o->clip_tol >= 0.0f) || !std::isfinite(o->clip_tol))
+        return fail(CGS_ERR_INVALID, std::string(what) + ": bad clipping tolerance");
+    m.method = o->method;
+    m.polish = o->polish ? 1u : 0u;
+    if (o->clip_tol > 0.0f) m.tol = o->clip_tol;
+    if (o->subdiv_depth) m.threshold = std::ldexp(1.0f, 1 - (int)o->subdiv_depth);
+    m.maxIters = o->max_clip_iters;
+    return CGS_OK;
+}
+
+cgs_status cgs_trace_closest_ex(cgs_ctx* ctx, cgs_scene* sc, const cgs_ray* rays, uint32_t n, cgs_hit* hits,
+                                int32_t* objects, const cgs_trace_options* options, void* stream)
 {
     if (!ctx || !sc || (n && (!rays || !hits))) return fail(CGS_ERR_INVALID, "cgs_trace_closest: null argument");
     if (sc->device != ctx->device) return fail(CGS_ERR_INVALID, "cgs_trace_closest: scene on another device");
+    TraceMode m;
+    cgs_status st = traceMode(options, m, "cgs_trace_closest");
+    if (st != CGS_OK) return st;
     HIP_TRY(hipSetDevice(ctx->device));
     std::lock_guard lk(ctx->mu);
     if (n == 0) return CGS_OK;
     PersistArgs pa;
-    cgs_status st = persistArgs(ctx, PersistKernel::TraceClosest, (hipStream_t)stream, n, pa);
-    if (st != CGS_OK) return st;
-    HIP_TRY(launchTraceClosest(sc->view(), pa, rays, n, hits, objects, ctx->dOverflow, (hipStream_t)stream));
+    if ((st = persistArgs(ctx, PersistKernel::TraceClosest, (hipStream_t)stream, n, pa)) != CGS_OK) return st;
+    HIP_TRY(launchTraceClosest(sc->view(), pa, rays, n, hits, objects, ctx->dOverflow, (hipStream_t)stream, m));
     return persistEnd(ctx, (hipStream_t)stream);
+}
+
+cgs_status cgs_trace_any_ex(cgs_ctx* ctx, cgs_scene* sc, const cgs_ray* rays, uint32_t n, uint32_t* occluded,
+                            const cgs_trace_options* options, void* stream)
+{
+    if (!ctx || !sc || (n && (!rays || !occluded))) return fail(CGS_ERR_INVALID, "cgs_trace_any: null argument");
+    if (sc->device != ctx->device) return fail(CGS_ERR_INVALID, "cgs_trace_any: scene on another device");
+    TraceMode m;
+    cgs_status st = traceMode(options, m, "cgs_trace_any");
+    if (st != CGS_OK) return st;
+    HIP_TRY(hipSetDevice(ctx->device));
+    std::lock_guard lk(ctx->mu);
+    if (n == 0) return CGS_OK;
+    PersistArgs pa;
+    if ((st = persistArgs(ctx, PersistKernel::TraceAny, (hipStream_t)stream, n, pa)) != CGS_OK) return st;
+    HIP_TRY(launchTraceAny(sc->view(), pa, rays, n, occluded, ctx->dOverflow, (hipStream_t)stream, m));
+    return persistEnd(ctx, (hipStream_t)stream);
+}
+
+cgs_status cgs_trace_closest(cgs_ctx* ctx, cgs_scene* sc, const cgs_ray* rays, uint32_t n, cgs_hit* hits, int32_t* objects,
+                             void* stream)
+{
+    return cgs_trace_closest_ex(ctx, sc, rays, n, hits, objects, nullptr, stream);
 }
 
 cgs_status cgs_trace_any(cgs_ctx* ctx, cgs_scene* sc, const cgs_ray* rays, uint32_t n, uint32_t* occluded, void* stream)
 {
-    if (!ctx || !sc || (n && (!rays || !occluded))) return fail(CGS_ERR_INVALID, "cgs_trace_any: null argument");
-    if (sc->device != ctx->device) return fail(CGS_ERR_INVALID, "cgs_trace_any: scene on another device");
-    HIP_TRY(hipSetDevice(ctx->device));
-    std::lock_guard lk(ctx->mu);
-    if (n == 0) return CGS_OK;
-    PersistArgs pa;
-    cgs_status st = persistArgs(ctx, PersistKernel::TraceAny, (hipStream_t)stream, n, pa);
-    if (st != CGS_OK) return st;
-    HIP_TRY(launchTraceAny(sc->view(), pa, rays, n, occluded, ctx->dOverflow, (hipStream_t)stream));
-    return persistEnd(ctx, (hipStream_t)stream);
+    return cgs_trace_any_ex(ctx, sc, rays, n, occluded, nullptr, stream);
 }
 
 cgs_status cgs_render(cgs_ctx* ctx, cgs_scene* sc, const cgs_camera* cam, const cgs_render_options* op, uint32_t w,

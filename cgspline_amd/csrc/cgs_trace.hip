@@ -17,6 +17,25 @@ namespace cgs
 
 constexpr float kClipTol = 0x1p-12f; // Spline.h:493
 
+// Leaf method of the ray queries.  Clip: Bézier clipping (the reference's
+// leaf call, Primitive.cu:234,260); ClipCapped: the same stopped after a
+// number of loop passes; Subdiv: subdivision (the commented-out alternative
+// at Primitive.cu:236,263, Spline_Subdivision.cu:168-288).  The last two,
+// and the Newton polish, are opt-in modes; the default is Clip, unpolished.
+enum LeafMethod : int
+{
+    Clip = 0,
+    ClipCapped = 1,
+    Subdiv = 2,
+};
+
+struct TraceParams
+{
+    float tol;          // clipping tolerance (Spline.h:493 default 2^-12)
+    float threshold;    // subdivision threshold (Spline.h:500 default 2^-4)
+    uint32_t maxIters;  // ClipCapped: loop passes per patch test
+};
+
 struct LocalRay
 {
     V3 o, d;
@@ -37,288 +56,6 @@ CGS_DEV LocalRay toLocal(const DevObject& ob, V3 o, V3 d, float tmax)
     l.d = mul3(dinv, ld);
     l.dinv = dinv;
     return l;
-}
-
-struct ClosestHit
-{
-    float t, u, v;
-    uint32_t patch;
-    int32_t object;
-};
-
-// Context::intersect(hit, ray) (RayTracer.cu:296-350) with
-// rt::intersect(BezierSurface, hit, ray) (Primitive.cu:222-248) at the leaves.
-// Sibling rings of a lane for the two traversal levels: the top-level BVH
-// over objects (CGS_RING_TOP entries; 4096 objects make it 13 levels deep)
-// and the patch BVH of one object (CGS_RING_BOT; 256 patches, 9 levels).
-#ifndef CGS_RING_TOP
-#define CGS_RING_TOP 8
-#endif
-#ifndef CGS_RING_BOT
-#define CGS_RING_BOT 8
-#endif
-// Octant-specialised node loops in the patch BVHs too (object-local rays:
-// lanes in different objects rarely share an octant).
-#ifndef CGS_BOT_OCTANTS
-#define CGS_BOT_OCTANTS false
-#endif
-struct Rings
-{
-    SiblingRing top, bot;
-};
-constexpr uint32_t kRingWords = CGS_RING_TOP + CGS_RING_BOT ? (CGS_RING_TOP + CGS_RING_BOT) * kRingStride : 1;
-
-CGS_DEV Rings laneRings(uint32_t* lds)
-{
-    lds_u32* lane = (lds_u32*)lds + threadIdx.x;
-    return Rings {{CGS_RING_TOP ? lane : nullptr, CGS_RING_TOP},
-                  {CGS_RING_BOT ? lane + CGS_RING_TOP * kRingStride : nullptr, CGS_RING_BOT}};
-}
-
-CGS_DEV ClosestHit sceneClosest(const SceneDev& sc, const NodeCache& nc, const Rings& rg, V3 o, V3 d, float tmax,
-                                const ClipStack& stk, bool& overflow)
-{
-    ClosestHit h {tmax, 0.0f, 0.0f, EMPTY, -1};
-    const BVHRef top = bvhRef(sc, sc.top);
-    hashIntersect<true, true>(top, o, d, h.t, [&](uint32_t i) -> bool {
-        const DevObject& ob = sc.objects[i];
-        if (ob.surface == EMPTY) return false;   // no geometry (empty box: only a NaN ray gets here)
-        LocalRay lr = toLocal(ob, o, d, tmax);
-        const DevBVH surf = sc.surfaces[ob.surface];
-        const BVHRef b = bvhRef(sc, surf);
-        const ClipRay cr = makeClipRay(lr.o, versor3(lr.d));
-        float lt = lr.max, lu = 0.0f, lv = 0.0f;
-        uint32_t lp = EMPTY;
-        bool hit = hashIntersect<CGS_BOT_OCTANTS>(b, lr.o, lr.d, lt, [&](uint32_t pid) -> bool {
-            const float4* S = sc.patches + 16 * (size_t)(surf.patchBase + pid);
-            bool f = clipPatch<false>(S, cr, kClipTol, lt, lu, lv, stk, overflow);
-            if (f) lp = pid;
-            return f;
-        }, NodeCache {nullptr, 0}, rg.bot);
-        if (hit)
-        {
-            float t = lt * lr.dinv;
-            if (t < h.t)
-            {
-                h.t = t;
-                h.u = lu;
-                h.v = lv;
-                h.patch = lp;
-                h.object = (int32_t)i;
-                return true;
-            }
-        }
-        return false;
-    }, nc, rg.top);
-    return h;
-}
-
-// Context::intersect(ray) (RayTracer.cu:352-383) with
-// rt::intersect(BezierSurface, ray) (Primitive.cu:250-268) at the leaves.
-CGS_DEV bool sceneAny(const SceneDev& sc, const NodeCache& nc, const Rings& rg, V3 o, V3 d, float tmax,
-                      const ClipStack& stk, bool& overflow)
-{
-    const BVHRef top = bvhRef(sc, sc.top);
-    return hashIntersectAny<true, true>(top, o, d, tmax, [&](uint32_t i) -> bool {
-        const DevObject& ob = sc.objects[i];
-        if (ob.surface == EMPTY) return false;   // no geometry (empty box: only a NaN ray gets here)
-        LocalRay lr = toLocal(ob, o, d, tmax);
-        const DevBVH surf = sc.surfaces[ob.surface];
-        const BVHRef b = bvhRef(sc, surf);
-        const ClipRay cr = makeClipRay(lr.o, versor3(lr.d));
-        return hashIntersectAny<CGS_BOT_OCTANTS>(b, lr.o, lr.d, lr.max, [&](uint32_t pid) -> bool {
-            const float4* S = sc.patches + 16 * (size_t)(surf.patchBase + pid);
-            float best = lr.max, bu, bv;
-            return clipPatch<true>(S, cr, kClipTol, best, bu, bv, stk, overflow);
-        }, NodeCache {nullptr, 0}, rg.bot);
-    }, nc, rg.top);
-}
-
-#ifndef CGS_TOP_LDS_KEYS
-#define CGS_TOP_LDS_KEYS 512
-#endif
-static_assert(CGS_TOP_LDS_KEYS <= kTopKeyCap, "top-level LDS staging beyond the key map");
-
-// Stages the top-level BVH's nodes with heap key < CGS_TOP_LDS_KEYS (its upper
-// levels, visited by nearly every ray) in LDS: 64 B per key.  512 keys (32 KiB
-// per workgroup) leave room for three resident workgroups per CU.
-CGS_DEV NodeCache stageTopLevel(const SceneDev& sc, float4* lds)
-{
-    for (uint32_t k = threadIdx.x; k < 4 * CGS_TOP_LDS_KEYS; k += blockDim.x)
-    {
-        const uint32_t node = sc.topByKey[k >> 2];
-        lds[k] = node != EMPTY ? sc.nodes[4 * (size_t)node + (k & 3)] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    }
-    __syncthreads();
-    return NodeCache {(lds_float*)(float*)lds, CGS_TOP_LDS_KEYS};
-}
-
-// Sum of a per-lane count over the wave, added with one atomic.
-CGS_DEV void addWave(unsigned long long* ctr, unsigned long long v)
-{
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    if ((threadIdx.x & 63) == 0 && v) atomicAdd(ctr, v);
-}
-
-// Persistent launches: the grid holds exactly the resident waves; each wave
-// takes chunks of 64 work items from a counter.  `lane` indexes the region
-// stack slice of this lane.
-struct Persist
-{
-    float2* stack;
-    uint32_t lanes;
-    unsigned int* counter;
-};
-
-CGS_DEV uint32_t nextChunk(unsigned int* counter)
-{
-    uint32_t base = 0;
-    if ((threadIdx.x & 63) == 0) base = atomicAdd(counter, 64u);
-    return __shfl(base, 0, 64);
-}
-
-#ifndef CGS_RENDER_WAVES
-#define CGS_RENDER_WAVES 3
-#endif
-#ifndef CGS_TRACE_WAVES
-#define CGS_TRACE_WAVES 3
-#endif
-
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CGS_TRACE_WAVES))) void kTraceClosest(SceneDev sc, Persist ps, const cgs_ray* __restrict__ rays, uint32_t n,
-                                                     cgs_hit* __restrict__ hits, int32_t* __restrict__ objects,
-                                                     unsigned long long* __restrict__ overflowCount)
-{
-    __shared__ float4 topLds[4 * CGS_TOP_LDS_KEYS];
-    __shared__ uint32_t ringLds[kRingWords];
-    const NodeCache nc = stageTopLevel(sc, topLds);
-    const Rings rg = laneRings(ringLds);
-    const uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x;
-    const ClipStack stk = clipStackFor(ps.stack, gl, ps.lanes);
-    unsigned long long nOverflow = 0;
-    for (uint32_t base = nextChunk(ps.counter); base < n; base = nextChunk(ps.counter))
-    {
-        const uint32_t r = base + (threadIdx.x & 63);
-        if (r >= n) continue;
-        bool overflow = false;
-        const cgs_ray ray = rays[r];
-        ClosestHit h = sceneClosest(sc, nc, rg, {ray.ox, ray.oy, ray.oz}, {ray.dx, ray.dy, ray.dz}, ray.tmax, stk, overflow);
-        cgs_hit out;
-        out.t = h.t;
-        out.u = h.object < 0 ? 0.0f : h.u;
-        out.v = h.object < 0 ? 0.0f : h.v;
-        out.patch = overflow ? CGS_PATCH_OVERFLOW : (h.object < 0 ? EMPTY : h.patch);
-        hits[r] = out;
-        if (objects) objects[r] = h.object;
-        nOverflow += overflow;
-    }
-    addWave(overflowCount, nOverflow);
-}
-
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CGS_TRACE_WAVES))) void kTraceAny(SceneDev sc, Persist ps, const cgs_ray* __restrict__ rays, uint32_t n,
-                                                 uint32_t* __restrict__ occluded,
-                                                 unsigned long long* __restrict__ overflowCount)
-{
-    __shared__ float4 topLds[4 * CGS_TOP_LDS_KEYS];
-    __shared__ uint32_t ringLds[kRingWords];
-    const NodeCache nc = stageTopLevel(sc, topLds);
-    const Rings rg = laneRings(ringLds);
-    const uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x;
-    const ClipStack stk = clipStackFor(ps.stack, gl, ps.lanes);
-    unsigned long long nOverflow = 0;
-    for (uint32_t base = nextChunk(ps.counter); base < n; base = nextChunk(ps.counter))
-    {
-        const uint32_t r = base + (threadIdx.x & 63);
-        if (r >= n) continue;
-        bool overflow = false;
-        const cgs_ray ray = rays[r];
-        bool occ = sceneAny(sc, nc, rg, {ray.ox, ray.oy, ray.oz}, {ray.dx, ray.dy, ray.dz}, ray.tmax, stk, overflow);
-        occluded[r] = overflow ? 2u : (occ ? 1u : 0u);
-        nOverflow += overflow;
-    }
-    addWave(overflowCount, nOverflow);
-}
-
-// Batched ray queries over many resident scenes in one launch: rays of all
-// jobs are numbered consecutively and waves take chunks of `chunk` rays (a
-// multiple of 64) from one counter, so small and large jobs share the grid
-// without a launch per scene.  Each lane finds its job by binary search over
-// the jobs' first ray numbers and runs the same per-ray query as
-// kTraceClosest / kTraceAny.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CGS_TRACE_WAVES))) void kTraceBatch(
-    const BatchJob* __restrict__ jobs, uint32_t njobs, uint32_t total, uint32_t chunk, Persist ps,
-    unsigned long long* __restrict__ overflowCount)
-{
-    const uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x;
-    const ClipStack stk = clipStackFor(ps.stack, gl, ps.lanes);
-    const NodeCache none {nullptr, 0};
-    __shared__ uint32_t ringLds[kRingWords];
-    const Rings rg = laneRings(ringLds);
-    unsigned long long nOverflow = 0;
-    while (true)
-    {
-        uint32_t base = 0;
-        if ((threadIdx.x & 63) == 0) base = atomicAdd(ps.counter, chunk);
-        base = __shfl(base, 0, 64);
-        if (base >= total) break;
-        const uint32_t end = base + chunk < total ? base + chunk : total;
-        for (uint32_t r0 = base; r0 < end; r0 += 64)
-        {
-            const uint32_t r = r0 + (threadIdx.x & 63);
-            if (r >= end) continue;
-            uint32_t lo = 0, hi = njobs;  // last job with rayBase <= r
-            while (hi - lo > 1)
-            {
-                const uint32_t mid = (lo + hi) / 2;
-                if (jobs[mid].rayBase <= r) lo = mid;
-                else hi = mid;
-            }
-            const BatchJob& J = jobs[lo];
-            const uint32_t k = r - J.rayBase;
-            const cgs_ray ray = J.rays[k];
-            bool overflow = false;
-            if (J.any)
-            {
-                const bool occ = sceneAny(J.sc, none, rg, {ray.ox, ray.oy, ray.oz}, {ray.dx, ray.dy, ray.dz}, ray.tmax, stk,
-                                          overflow);
-                J.occluded[k] = overflow ? 2u : (occ ? 1u : 0u);
-            }
-            else
-            {
-                const ClosestHit h = sceneClosest(J.sc, none, rg, {ray.ox, ray.oy, ray.oz}, {ray.dx, ray.dy, ray.dz},
-                                                  ray.tmax, stk, overflow);
-                cgs_hit out;
-                out.t = h.t;
-                out.u = h.object < 0 ? 0.0f : h.u;
-                out.v = h.object < 0 ? 0.0f : h.v;
-                out.patch = overflow ? CGS_PATCH_OVERFLOW : (h.object < 0 ? EMPTY : h.patch);
-                J.hits[k] = out;
-                if (J.objects) J.objects[k] = h.object;
-            }
-            nOverflow += overflow;
-        }
-    }
-    addWave(overflowCount, nOverflow);
-}
-
-// Batched doBezierClipping: ray r against patch r (Spline_BezierClipping.cu:46-114).
-__global__ __launch_bounds__(256) void kClipBatch(Persist ps, const cgs_ray* __restrict__ rays,
-                                                  const float4* __restrict__ patches, uint32_t n, float tol,
-                                                  cgs_hit* __restrict__ hits)
-{
-    const uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x;
-    const ClipStack stk = clipStackFor(ps.stack, gl, ps.lanes);
-    for (uint32_t base = nextChunk(ps.counter); base < n; base = nextChunk(ps.counter))
-    {
-        const uint32_t r = base + (threadIdx.x & 63);
-        if (r >= n) continue;
-        const cgs_ray ray = rays[r];
-        bool overflow = false;
-        const ClipRay cr = makeClipRay({ray.ox, ray.oy, ray.oz}, versor3({ray.dx, ray.dy, ray.dz}));
-        float t = ray.tmax, u = 0.0f, v = 0.0f;
-        bool f = clipPatch<false>(patches + 16 * (size_t)r, cr, tol, t, u, v, stk, overflow);
-        hits[r] = {t, u, v, overflow ? CGS_PATCH_OVERFLOW : (f ? 1u : EMPTY)};
-    }
 }
 
 // ---------------------------------------------------------------------------
@@ -407,6 +144,405 @@ CGS_DEV void projBox(const float4 (&p)[16], V3& mn, V3& mx)
 
 constexpr int kSubdivStack = 64;
 
+// The subdivision test of one patch against a ray (o, d unit), host
+// semantics: returns found; best/bu/bv updated by a nearer triangle hit;
+// overflow when the region stack is exhausted.
+template<bool ANY>
+CGS_DEV bool subdividePatch(const float4* __restrict__ S, V3 o, V3 d, float threshold, float& best, float& bu,
+                            float& bv, bool& overflow)
+{
+    bool found = false;
+    float4 sub[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) sub[i] = S[i];
+    V3 bmn, bmx;
+    projBox(sub, bmn, bmx);
+    if (!boxHit(bmn, bmx, o, d)) return false;
+    float4 st[kSubdivStack]; // (umin, vmin, umax, vmax)
+    uint8_t side[kSubdivStack];
+    int sp = 0;
+    st[sp] = {0.0f, 0.0f, 0.5f, 1.0f}; side[sp++] = 0;
+    st[sp] = {0.5f, 0.0f, 1.0f, 1.0f}; side[sp++] = 0;
+    while (sp > 0)
+    {
+        sp--;
+        const float4 e = st[sp];
+        const bool es = side[sp] != 0;
+#pragma unroll
+        for (int i = 0; i < 16; i++) sub[i] = S[i];
+        subpatch4(sub, true, e.y, e.w);
+        subpatch4(sub, false, e.x, e.z);
+        projBox(sub, bmn, bmx);
+        if (!boxHit(bmn, bmx, o, d)) continue;
+        float x = 0.5f * (e.x + e.z);
+        float y = 0.5f * (e.y + e.w);
+        if ((e.z - e.x) < threshold && (e.w - e.y) < threshold)
+        {
+            V3 p00 = project4(sub[0]), p10 = project4(sub[3]), p01 = project4(sub[12]), p11 = project4(sub[15]);
+            float t = 0.0f, t1 = 0.0f;
+            bool b0 = triHit(o, d, p10, p01, p00, t);
+            bool b1 = triHit(o, d, p01, p10, p11, t1);
+            if (b0)
+            {
+                if (b1 && t1 < t) t = t1;
+            }
+            else if (b1)
+            {
+                t = t1;
+                b0 = true;
+            }
+            if (b0 && t < best)
+            {
+                best = t;
+                bu = x;
+                bv = y;
+                found = true;
+                if (ANY) break;
+            }
+        }
+        else
+        {
+            if (sp + 2 > kSubdivStack)
+            {
+                overflow = true;
+                break;
+            }
+            if (es)
+            {
+                st[sp] = {e.x, e.y, x, e.w}; side[sp++] = !es;
+                st[sp] = {x, e.y, e.z, e.w}; side[sp++] = !es;
+            }
+            else
+            {
+                st[sp] = {e.x, e.y, e.z, y}; side[sp++] = !es;
+                st[sp] = {e.x, y, e.z, e.w}; side[sp++] = !es;
+            }
+        }
+    }
+    return found;
+}
+
+// One patch test of a scene query by the chosen leaf method.
+template<int METHOD, bool ANY>
+CGS_DEV bool leafTest(const float4* __restrict__ S, const ClipRay& cr, const TraceParams& tp, float& best, float& bu,
+                      float& bv, const ClipStack& stk, bool& overflow)
+{
+    bool f;
+    if constexpr (METHOD == Subdiv)
+        f = subdividePatch<ANY>(S, cr.o, cr.d, tp.threshold, best, bu, bv, overflow);
+    else if constexpr (METHOD == ClipCapped)
+        f = clipPatch<ANY, true>(S, cr, tp.tol, best, bu, bv, stk, overflow, tp.maxIters);
+    else
+        f = clipPatch<ANY>(S, cr, tp.tol, best, bu, bv, stk, overflow);
+    return f;
+}
+
+struct ClosestHit
+{
+    float t, u, v;
+    uint32_t patch;
+    int32_t object;
+};
+
+// Context::intersect(hit, ray) (RayTracer.cu:296-350) with
+// rt::intersect(BezierSurface, hit, ray) (Primitive.cu:222-248) at the leaves.
+// Sibling rings of a lane for the two traversal levels: the top-level BVH
+// over objects (CGS_RING_TOP entries; 4096 objects make it 13 levels deep)
+// and the patch BVH of one object (CGS_RING_BOT; 256 patches, 9 levels).
+#ifndef CGS_RING_TOP
+#define CGS_RING_TOP 8
+#endif
+#ifndef CGS_RING_BOT
+#define CGS_RING_BOT 8
+#endif
+// Octant-specialised node loops in the patch BVHs too (object-local rays:
+// lanes in different objects rarely share an octant).
+#ifndef CGS_BOT_OCTANTS
+#define CGS_BOT_OCTANTS false
+#endif
+struct Rings
+{
+    SiblingRing top, bot;
+};
+constexpr uint32_t kRingWords = CGS_RING_TOP + CGS_RING_BOT ? (CGS_RING_TOP + CGS_RING_BOT) * kRingStride : 1;
+
+CGS_DEV Rings laneRings(uint32_t* lds)
+{
+    lds_u32* lane = (lds_u32*)lds + threadIdx.x;
+    return Rings {{CGS_RING_TOP ? lane : nullptr, CGS_RING_TOP},
+                  {CGS_RING_BOT ? lane + CGS_RING_TOP * kRingStride : nullptr, CGS_RING_BOT}};
+}
+
+template<int METHOD = Clip>
+CGS_DEV ClosestHit sceneClosest(const SceneDev& sc, const NodeCache& nc, const Rings& rg, V3 o, V3 d, float tmax,
+                                const ClipStack& stk, bool& overflow, const TraceParams& tp = {kClipTol, 0x1p-4f, 0})
+{
+    ClosestHit h {tmax, 0.0f, 0.0f, EMPTY, -1};
+    const BVHRef top = bvhRef(sc, sc.top);
+    hashIntersect<true, true>(top, o, d, h.t, [&](uint32_t i) -> bool {
+        const DevObject& ob = sc.objects[i];
+        if (ob.surface == EMPTY) return false;   // no geometry (empty box: only a NaN ray gets here)
+        LocalRay lr = toLocal(ob, o, d, tmax);
+        const DevBVH surf = sc.surfaces[ob.surface];
+        const BVHRef b = bvhRef(sc, surf);
+        const ClipRay cr = makeClipRay(lr.o, versor3(lr.d));
+        float lt = lr.max, lu = 0.0f, lv = 0.0f;
+        uint32_t lp = EMPTY;
+        bool hit = hashIntersect<CGS_BOT_OCTANTS>(b, lr.o, lr.d, lt, [&](uint32_t pid) -> bool {
+            const float4* S = sc.patches + 16 * (size_t)(surf.patchBase + pid);
+            bool f;
+            // the default keeps the direct call: routing it through leafTest
+            // costs the render and trace kernels hundreds of spilled VGPRs
+            if constexpr (METHOD == Clip)
+                f = clipPatch<false>(S, cr, kClipTol, lt, lu, lv, stk, overflow);
+            else
+                f = leafTest<METHOD, false>(S, cr, tp, lt, lu, lv, stk, overflow);
+            if (f) lp = pid;
+            return f;
+        }, NodeCache {nullptr, 0}, rg.bot);
+        if (hit)
+        {
+            float t = lt * lr.dinv;
+            if (t < h.t)
+            {
+                h.t = t;
+                h.u = lu;
+                h.v = lv;
+                h.patch = lp;
+                h.object = (int32_t)i;
+                return true;
+            }
+        }
+        return false;
+    }, nc, rg.top);
+    return h;
+}
+
+// Context::intersect(ray) (RayTracer.cu:352-383) with
+// rt::intersect(BezierSurface, ray) (Primitive.cu:250-268) at the leaves.
+template<int METHOD = Clip>
+CGS_DEV bool sceneAny(const SceneDev& sc, const NodeCache& nc, const Rings& rg, V3 o, V3 d, float tmax,
+                      const ClipStack& stk, bool& overflow, const TraceParams& tp = {kClipTol, 0x1p-4f, 0})
+{
+    const BVHRef top = bvhRef(sc, sc.top);
+    return hashIntersectAny<true, true>(top, o, d, tmax, [&](uint32_t i) -> bool {
+        const DevObject& ob = sc.objects[i];
+        if (ob.surface == EMPTY) return false;   // no geometry (empty box: only a NaN ray gets here)
+        LocalRay lr = toLocal(ob, o, d, tmax);
+        const DevBVH surf = sc.surfaces[ob.surface];
+        const BVHRef b = bvhRef(sc, surf);
+        const ClipRay cr = makeClipRay(lr.o, versor3(lr.d));
+        return hashIntersectAny<CGS_BOT_OCTANTS>(b, lr.o, lr.d, lr.max, [&](uint32_t pid) -> bool {
+            const float4* S = sc.patches + 16 * (size_t)(surf.patchBase + pid);
+            float best = lr.max, bu, bv;
+            if constexpr (METHOD == Clip)
+                return clipPatch<true>(S, cr, kClipTol, best, bu, bv, stk, overflow);
+            else
+                return leafTest<METHOD, true>(S, cr, tp, best, bu, bv, stk, overflow);
+        }, NodeCache {nullptr, 0}, rg.bot);
+    }, nc, rg.top);
+}
+
+#ifndef CGS_TOP_LDS_KEYS
+#define CGS_TOP_LDS_KEYS 512
+#endif
+static_assert(CGS_TOP_LDS_KEYS <= kTopKeyCap, "top-level LDS staging beyond the key map");
+
+// Stages the top-level BVH's nodes with heap key < CGS_TOP_LDS_KEYS (its upper
+// levels, visited by nearly every ray) in LDS: 64 B per key.  512 keys (32 KiB
+// per workgroup) leave room for three resident workgroups per CU.
+CGS_DEV NodeCache stageTopLevel(const SceneDev& sc, float4* lds)
+{
+    for (uint32_t k = threadIdx.x; k < 4 * CGS_TOP_LDS_KEYS; k += blockDim.x)
+    {
+        const uint32_t node = sc.topByKey[k >> 2];
+        lds[k] = node != EMPTY ? sc.nodes[4 * (size_t)node + (k & 3)] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+    __syncthreads();
+    return NodeCache {(lds_float*)(float*)lds, CGS_TOP_LDS_KEYS};
+}
+
+// Sum of a per-lane count over the wave, added with one atomic.
+CGS_DEV void addWave(unsigned long long* ctr, unsigned long long v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0 && v) atomicAdd(ctr, v);
+}
+
+// Persistent launches: the grid holds exactly the resident waves; each wave
+// takes chunks of 64 work items from a counter.  `lane` indexes the region
+// stack slice of this lane.
+struct Persist
+{
+    float2* stack;
+    uint32_t lanes;
+    unsigned int* counter;
+};
+
+CGS_DEV uint32_t nextChunk(unsigned int* counter)
+{
+    uint32_t base = 0;
+    if ((threadIdx.x & 63) == 0) base = atomicAdd(counter, 64u);
+    return __shfl(base, 0, 64);
+}
+
+#ifndef CGS_RENDER_WAVES
+#define CGS_RENDER_WAVES 3
+#endif
+#ifndef CGS_TRACE_WAVES
+#define CGS_TRACE_WAVES 3
+#endif
+
+template<int METHOD, bool POLISH>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CGS_TRACE_WAVES))) void kTraceClosest(SceneDev sc, Persist ps, const cgs_ray* __restrict__ rays, uint32_t n,
+                                                     cgs_hit* __restrict__ hits, int32_t* __restrict__ objects,
+                                                     unsigned long long* __restrict__ overflowCount, TraceParams tp)
+{
+    __shared__ float4 topLds[4 * CGS_TOP_LDS_KEYS];
+    __shared__ uint32_t ringLds[kRingWords];
+    const NodeCache nc = stageTopLevel(sc, topLds);
+    const Rings rg = laneRings(ringLds);
+    const uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x;
+    const ClipStack stk = clipStackFor(ps.stack, gl, ps.lanes);
+    unsigned long long nOverflow = 0;
+    for (uint32_t base = nextChunk(ps.counter); base < n; base = nextChunk(ps.counter))
+    {
+        const uint32_t r = base + (threadIdx.x & 63);
+        if (r >= n) continue;
+        bool overflow = false;
+        const cgs_ray ray = rays[r];
+        ClosestHit h = sceneClosest<METHOD>(sc, nc, rg, {ray.ox, ray.oy, ray.oz}, {ray.dx, ray.dy, ray.dz}, ray.tmax,
+                                            stk, overflow, tp);
+        if constexpr (POLISH)
+            if (h.object >= 0 && !overflow)
+            {
+                // the reported hit only: the traversal's decisions are the reference's
+                const DevObject& ob = sc.objects[h.object];
+                const DevBVH surf = sc.surfaces[ob.surface];
+                polishHit64(sc.patches + 16 * (size_t)(surf.patchBase + h.patch), ob.w2l, {ray.ox, ray.oy, ray.oz},
+                            {ray.dx, ray.dy, ray.dz}, h.t, h.u, h.v);
+            }
+        cgs_hit out;
+        out.t = h.t;
+        out.u = h.object < 0 ? 0.0f : h.u;
+        out.v = h.object < 0 ? 0.0f : h.v;
+        out.patch = overflow ? CGS_PATCH_OVERFLOW : (h.object < 0 ? EMPTY : h.patch);
+        hits[r] = out;
+        if (objects) objects[r] = h.object;
+        nOverflow += overflow;
+    }
+    addWave(overflowCount, nOverflow);
+}
+
+template<int METHOD>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CGS_TRACE_WAVES))) void kTraceAny(SceneDev sc, Persist ps, const cgs_ray* __restrict__ rays, uint32_t n,
+                                                 uint32_t* __restrict__ occluded,
+                                                 unsigned long long* __restrict__ overflowCount, TraceParams tp)
+{
+    __shared__ float4 topLds[4 * CGS_TOP_LDS_KEYS];
+    __shared__ uint32_t ringLds[kRingWords];
+    const NodeCache nc = stageTopLevel(sc, topLds);
+    const Rings rg = laneRings(ringLds);
+    const uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x;
+    const ClipStack stk = clipStackFor(ps.stack, gl, ps.lanes);
+    unsigned long long nOverflow = 0;
+    for (uint32_t base = nextChunk(ps.counter); base < n; base = nextChunk(ps.counter))
+    {
+        const uint32_t r = base + (threadIdx.x & 63);
+        if (r >= n) continue;
+        bool overflow = false;
+        const cgs_ray ray = rays[r];
+        bool occ = sceneAny<METHOD>(sc, nc, rg, {ray.ox, ray.oy, ray.oz}, {ray.dx, ray.dy, ray.dz}, ray.tmax, stk, overflow,
+                                    tp);
+        occluded[r] = overflow ? 2u : (occ ? 1u : 0u);
+        nOverflow += overflow;
+    }
+    addWave(overflowCount, nOverflow);
+}
+
+// Batched ray queries over many resident scenes in one launch: rays of all
+// jobs are numbered consecutively and waves take chunks of `chunk` rays (a
+// multiple of 64) from one counter, so small and large jobs share the grid
+// without a launch per scene.  Each lane finds its job by binary search over
+// the jobs' first ray numbers and runs the same per-ray query as
+// kTraceClosest / kTraceAny.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CGS_TRACE_WAVES))) void kTraceBatch(
+    const BatchJob* __restrict__ jobs, uint32_t njobs, uint32_t total, uint32_t chunk, Persist ps,
+    unsigned long long* __restrict__ overflowCount)
+{
+    const uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x;
+    const ClipStack stk = clipStackFor(ps.stack, gl, ps.lanes);
+    const NodeCache none {nullptr, 0};
+    __shared__ uint32_t ringLds[kRingWords];
+    const Rings rg = laneRings(ringLds);
+    unsigned long long nOverflow = 0;
+    while (true)
+    {
+        uint32_t base = 0;
+        if ((threadIdx.x & 63) == 0) base = atomicAdd(ps.counter, chunk);
+        base = __shfl(base, 0, 64);
+        if (base >= total) break;
+        const uint32_t end = base + chunk < total ? base + chunk : total;
+        for (uint32_t r0 = base; r0 < end; r0 += 64)
+        {
+            const uint32_t r = r0 + (threadIdx.x & 63);
+            if (r >= end) continue;
+            uint32_t lo = 0, hi = njobs;  // last job with rayBase <= r
+            while (hi - lo > 1)
+            {
+                const uint32_t mid = (lo + hi) / 2;
+                if (jobs[mid].rayBase <= r) lo = mid;
+                else hi = mid;
+            }
+            const BatchJob& J = jobs[lo];
+            const uint32_t k = r - J.rayBase;
+            const cgs_ray ray = J.rays[k];
+            bool overflow = false;
+            if (J.any)
+            {
+                const bool occ = sceneAny(J.sc, none, rg, {ray.ox, ray.oy, ray.oz}, {ray.dx, ray.dy, ray.dz}, ray.tmax, stk,
+                                          overflow);
+                J.occluded[k] = overflow ? 2u : (occ ? 1u : 0u);
+            }
+            else
+            {
+                const ClosestHit h = sceneClosest(J.sc, none, rg, {ray.ox, ray.oy, ray.oz}, {ray.dx, ray.dy, ray.dz},
+                                                  ray.tmax, stk, overflow);
+                cgs_hit out;
+                out.t = h.t;
+                out.u = h.object < 0 ? 0.0f : h.u;
+                out.v = h.object < 0 ? 0.0f : h.v;
+                out.patch = overflow ? CGS_PATCH_OVERFLOW : (h.object < 0 ? EMPTY : h.patch);
+                J.hits[k] = out;
+                if (J.objects) J.objects[k] = h.object;
+            }
+            nOverflow += overflow;
+        }
+    }
+    addWave(overflowCount, nOverflow);
+}
+
+// Batched doBezierClipping: ray r against patch r (Spline_BezierClipping.cu:46-114).
+__global__ __launch_bounds__(256) void kClipBatch(Persist ps, const cgs_ray* __restrict__ rays,
+                                                  const float4* __restrict__ patches, uint32_t n, float tol,
+                                                  cgs_hit* __restrict__ hits)
+{
+    const uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x;
+    const ClipStack stk = clipStackFor(ps.stack, gl, ps.lanes);
+    for (uint32_t base = nextChunk(ps.counter); base < n; base = nextChunk(ps.counter))
+    {
+        const uint32_t r = base + (threadIdx.x & 63);
+        if (r >= n) continue;
+        const cgs_ray ray = rays[r];
+        bool overflow = false;
+        const ClipRay cr = makeClipRay({ray.ox, ray.oy, ray.oz}, versor3({ray.dx, ray.dy, ray.dz}));
+        float t = ray.tmax, u = 0.0f, v = 0.0f;
+        bool f = clipPatch<false>(patches + 16 * (size_t)r, cr, tol, t, u, v, stk, overflow);
+        hits[r] = {t, u, v, overflow ? CGS_PATCH_OVERFLOW : (f ? 1u : EMPTY)};
+    }
+}
+
 __global__ __launch_bounds__(256) void kSubdivBatch(const cgs_ray* __restrict__ rays, const float4* __restrict__ patches,
                                                     uint32_t n, float threshold, cgs_hit* __restrict__ hits)
 {
@@ -415,77 +551,9 @@ __global__ __launch_bounds__(256) void kSubdivBatch(const cgs_ray* __restrict__ 
     const cgs_ray ray = rays[r];
     const V3 o {ray.ox, ray.oy, ray.oz};
     const V3 d = versor3({ray.dx, ray.dy, ray.dz});
-    const float4* S = patches + 16 * (size_t)r;
     float best = ray.tmax, bu = 0.0f, bv = 0.0f;
-    bool found = false, overflow = false;
-    float4 sub[16];
-#pragma unroll
-    for (int i = 0; i < 16; i++) sub[i] = S[i];
-    V3 bmn, bmx;
-    projBox(sub, bmn, bmx);
-    if (boxHit(bmn, bmx, o, d))
-    {
-        float4 st[kSubdivStack]; // (umin, vmin, umax, vmax); side in the sign bit of nothing: separate array
-        uint8_t side[kSubdivStack];
-        int sp = 0;
-        st[sp] = {0.0f, 0.0f, 0.5f, 1.0f}; side[sp++] = 0;
-        st[sp] = {0.5f, 0.0f, 1.0f, 1.0f}; side[sp++] = 0;
-        while (sp > 0)
-        {
-            sp--;
-            const float4 e = st[sp];
-            const bool es = side[sp] != 0;
-#pragma unroll
-            for (int i = 0; i < 16; i++) sub[i] = S[i];
-            subpatch4(sub, true, e.y, e.w);
-            subpatch4(sub, false, e.x, e.z);
-            projBox(sub, bmn, bmx);
-            if (!boxHit(bmn, bmx, o, d)) continue;
-            float x = 0.5f * (e.x + e.z);
-            float y = 0.5f * (e.y + e.w);
-            if ((e.z - e.x) < threshold && (e.w - e.y) < threshold)
-            {
-                V3 p00 = project4(sub[0]), p10 = project4(sub[3]), p01 = project4(sub[12]), p11 = project4(sub[15]);
-                float t = 0.0f, t1 = 0.0f;
-                bool b0 = triHit(o, d, p10, p01, p00, t);
-                bool b1 = triHit(o, d, p01, p10, p11, t1);
-                if (b0)
-                {
-                    if (b1 && t1 < t) t = t1;
-                }
-                else if (b1)
-                {
-                    t = t1;
-                    b0 = true;
-                }
-                if (b0 && t < best)
-                {
-                    best = t;
-                    bu = x;
-                    bv = y;
-                    found = true;
-                }
-            }
-            else
-            {
-                if (sp + 2 > kSubdivStack)
-                {
-                    overflow = true;
-                    break;
-                }
-                if (es)
-                {
-                    st[sp] = {e.x, e.y, x, e.w}; side[sp++] = !es;
-                    st[sp] = {x, e.y, e.z, e.w}; side[sp++] = !es;
-                }
-                else
-                {
-                    st[sp] = {e.x, e.y, e.z, y}; side[sp++] = !es;
-                    st[sp] = {e.x, y, e.z, e.w}; side[sp++] = !es;
-                }
-            }
-        }
-    }
+    bool overflow = false;
+    const bool found = subdividePatch<false>(patches + 16 * (size_t)r, o, d, threshold, best, bu, bv, overflow);
     hits[r] = {best, bu, bv, overflow ? CGS_PATCH_OVERFLOW : (found ? 1u : EMPTY)};
 }
 
@@ -749,8 +817,8 @@ hipError_t persistentGrid(PersistKernel k, uint32_t& blocksPerCU)
     hipError_t e = hipSuccess;
     switch (k)
     {
-    case PersistKernel::TraceClosest: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kTraceClosest, 256, 0); break;
-    case PersistKernel::TraceAny: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kTraceAny, 256, 0); break;
+    case PersistKernel::TraceClosest: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kTraceClosest<Clip, false>, 256, 0); break;
+    case PersistKernel::TraceAny: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kTraceAny<Clip>, 256, 0); break;
     case PersistKernel::ClipBatch: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kClipBatch, 256, 0); break;
     case PersistKernel::Render: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kRender, 256, 0); break;
     case PersistKernel::Batch: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kTraceBatch, 256, 0); break;
@@ -759,21 +827,52 @@ hipError_t persistentGrid(PersistKernel k, uint32_t& blocksPerCU)
     return e;
 }
 
+// Trace launches with an explicit leaf method.  The opt-in modes use the
+// default kernels' grid (the same VGPR budget and LDS; they spill to scratch
+// where the default kernels do not).  The default kernels clip at the fixed
+// 2^-12, so a clipping run with another tolerance takes the capped kernels
+// with a cap no run reaches (2^32 - 1 passes).
+static TraceParams traceParams(const TraceMode& m)
+{
+    return {m.tol, m.threshold, m.maxIters ? m.maxIters : 0xFFFFFFFFu};
+}
+
 hipError_t launchTraceClosest(const SceneDev& sc, const PersistArgs& pa, const cgs_ray* rays, uint32_t n, cgs_hit* hits,
-                              int32_t* objects, unsigned long long* overflow, hipStream_t s)
+                              int32_t* objects, unsigned long long* overflow, hipStream_t s, const TraceMode& m)
 {
     if (n == 0) return hipSuccess;
     Persist ps {pa.stack, pa.lanes, pa.counter};
-    hipLaunchKernelGGL(kTraceClosest, dim3(pa.grid), dim3(256), 0, s, sc, ps, rays, n, hits, objects, overflow);
+    const TraceParams tp = traceParams(m);
+#define CGS_LAUNCH_CLOSEST(M, P) \
+    hipLaunchKernelGGL((kTraceClosest<M, P>), dim3(pa.grid), dim3(256), 0, s, sc, ps, rays, n, hits, objects, overflow, tp)
+    if (m.method == CGS_METHOD_SUBDIVISION)
+    {
+        if (m.polish) CGS_LAUNCH_CLOSEST(Subdiv, true);
+        else CGS_LAUNCH_CLOSEST(Subdiv, false);
+    }
+    else if (m.maxIters || m.tol != kClipTol)
+    {
+        if (m.polish) CGS_LAUNCH_CLOSEST(ClipCapped, true);
+        else CGS_LAUNCH_CLOSEST(ClipCapped, false);
+    }
+    else if (m.polish) CGS_LAUNCH_CLOSEST(Clip, true);
+    else CGS_LAUNCH_CLOSEST(Clip, false);
+#undef CGS_LAUNCH_CLOSEST
     return hipGetLastError();
 }
 
 hipError_t launchTraceAny(const SceneDev& sc, const PersistArgs& pa, const cgs_ray* rays, uint32_t n, uint32_t* occluded,
-                          unsigned long long* overflow, hipStream_t s)
+                          unsigned long long* overflow, hipStream_t s, const TraceMode& m)
 {
     if (n == 0) return hipSuccess;
     Persist ps {pa.stack, pa.lanes, pa.counter};
-    hipLaunchKernelGGL(kTraceAny, dim3(pa.grid), dim3(256), 0, s, sc, ps, rays, n, occluded, overflow);
+    const TraceParams tp = traceParams(m);
+    if (m.method == CGS_METHOD_SUBDIVISION)
+        hipLaunchKernelGGL((kTraceAny<Subdiv>), dim3(pa.grid), dim3(256), 0, s, sc, ps, rays, n, occluded, overflow, tp);
+    else if (m.maxIters || m.tol != kClipTol)
+        hipLaunchKernelGGL((kTraceAny<ClipCapped>), dim3(pa.grid), dim3(256), 0, s, sc, ps, rays, n, occluded, overflow, tp);
+    else
+        hipLaunchKernelGGL((kTraceAny<Clip>), dim3(pa.grid), dim3(256), 0, s, sc, ps, rays, n, occluded, overflow, tp);
     return hipGetLastError();
 }
 

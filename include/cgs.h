@@ -138,6 +138,39 @@ typedef struct cgs_render_options
                             * Frame.h:29-94, a row padded to an alignment); 0: the width */
 } cgs_render_options;
 
+/* Leaf method of a ray query (cgs_trace_closest_ex / cgs_trace_any_ex).
+ * CGS_METHOD_CLIPPING: Bézier clipping, the reference's leaf call
+ * (Primitive.cu:234,260); CGS_METHOD_SUBDIVISION: subdivision, its
+ * commented-out alternative (Primitive.cu:236,263; Spline_Subdivision.cu:168-288). */
+#define CGS_METHOD_CLIPPING 0u
+#define CGS_METHOD_SUBDIVISION 1u
+
+/* Options of a ray query.  A zeroed record, or a null pointer, is the
+ * default: clipping at the reference's tolerance 2^-12 (Spline.h:493), no
+ * cap, no polish; its results are the reference algorithm's bit for bit.
+ *   clip_tol: clipping tolerance (0: 2^-12);
+ *   subdiv_depth: subdivision halvings per parameter side, 1..16 (0: the
+ *     reference's threshold 2^-4, which is depth 5: cells of 1/32); depth D
+ *     is the threshold 2^(1-D);
+ *   max_clip_iters: stop each patch test after this many clipping-loop
+ *     passes, keeping what it found (0: no cap; an opt-in trade of accuracy
+ *     for a bounded cost per patch, no counterpart in the reference);
+ *   polish: refine the reported closest hit by FP64 Newton steps on
+ *     S(u,v) = o_l + t M d in the hit object's frame, after the query (no
+ *     counterpart in the reference; hits, patches and objects are the
+ *     unpolished query's, t/u/v come within ~1e-7 of the true root where
+ *     the clipper's 2^-12 stopping rule leaves up to ~1e-3 in (u,v)); a hit
+ *     whose steps do not converge inside the patch keeps its clipped values. */
+typedef struct cgs_trace_options
+{
+    uint32_t method;
+    float clip_tol;
+    uint32_t subdiv_depth;
+    uint32_t max_clip_iters;
+    uint32_t polish;
+    uint32_t reserved[3];
+} cgs_trace_options;
+
 /* Per-launch ray counters written by cgs_render (device memory, 4 × u64):
  * primary, shadow, secondary, overflowed clip stacks. */
 typedef struct cgs_ray_counts
@@ -328,6 +361,14 @@ cgs_status cgs_trace_closest(cgs_ctx* ctx, cgs_scene* scene, const cgs_ray* rays
  * RayTracer.cu:352-383).  occluded: n device u32 (1 / 0). */
 cgs_status cgs_trace_any(cgs_ctx* ctx, cgs_scene* scene, const cgs_ray* rays, uint32_t n,
                          uint32_t* occluded, void* stream);
+
+/* cgs_trace_closest / cgs_trace_any with a leaf method and its parameters
+ * (options may be null: the default).  CGS_ERR_INVALID for a depth above 16
+ * or a negative or non-finite tolerance. */
+cgs_status cgs_trace_closest_ex(cgs_ctx* ctx, cgs_scene* scene, const cgs_ray* rays, uint32_t n, cgs_hit* hits,
+                                int32_t* objects, const cgs_trace_options* options, void* stream);
+cgs_status cgs_trace_any_ex(cgs_ctx* ctx, cgs_scene* scene, const cgs_ray* rays, uint32_t n, uint32_t* occluded,
+                            const cgs_trace_options* options, void* stream);
 
 /* RayTracer::render (rt/RayTracer.h:42-43, RayTracer.cu:107-180 and the
  * render kernel :194-544, non-Monte-Carlo): camera rays through integer

@@ -538,9 +538,30 @@ int orc_scene_bvh_copy(const orc_scene* h, int surface, uint32_t* nodes16, uint3
 // RayTracer's Ray).  any = 0: closest hit → hits (t,u,v) + patch + object;
 // any = 1: occlusion → object[r] = 1 when occluded; any = 2: closest hit with
 // the explicit-stack traversal of rt/BVH.cpp:126-189.
-int orc_scene_trace(const orc_scene* h, const float* rays, uint64_t n, int any, float* hits3,
-    uint32_t* patch, int32_t* object, orc_work* work, int threads)
+// Leaf method of a trace (cgs_trace_options's meaning): method 0 clipping,
+// 1 subdivision; clip tolerance; subdivision threshold; clip pass cap (0:
+// none); Newton polish of closest hits.
+struct orc_trace_opts
 {
+    int method;
+    float tol;
+    float threshold;
+    uint32_t maxIters;
+    int polish;
+};
+
+int orc_scene_trace_ex(const orc_scene* h, const float* rays, uint64_t n, int any, float* hits3,
+    uint32_t* patch, int32_t* object, orc_work* work, int threads, const orc_trace_opts* opts)
+{
+    TraceOpts op;
+    if (opts)
+    {
+        op.method = opts->method;
+        op.tol = opts->tol;
+        op.threshold = opts->threshold;
+        op.maxIters = opts->maxIters;
+        op.polish = opts->polish != 0;
+    }
     std::mutex mu;
     Work total;
     parallelFor(n, threads, [&](uint64_t a, uint64_t b) {
@@ -551,13 +572,21 @@ int orc_scene_trace(const orc_scene* h, const float* rays, uint64_t n, int any, 
             Ray ray {{ry[0], ry[1], ry[2]}, {ry[4], ry[5], ry[6]}, ry[3]};
             if (any == 1)
             {
-                object[r] = sceneIntersectAny(h->s, ray, &w) ? 1 : 0;
+                object[r] = sceneIntersectAny(h->s, ray, &w, op) ? 1 : 0;
                 continue;
             }
             Hit hit;
             hit.t = ray.max;
             uint32_t pid = EMPTY;
-            int o = any == 2 ? sceneIntersectStack(h->s, hit, ray, &pid) : sceneIntersect(h->s, hit, ray, &pid, &w);
+            const uint64_t overflowsBefore = w.overflows;
+            int o = any == 2 ? sceneIntersectStack(h->s, hit, ray, &pid) : sceneIntersect(h->s, hit, ray, &pid, &w, op);
+            if (op.polish && o >= 0 && w.overflows == overflowsBefore)
+            {
+                const Object& ob = h->s.objects[o];
+                f4 S[16];
+                h->s.surfaces[ob.surface].patch(pid, S);
+                polishHit64(S, ob.world2local, ray.origin, ray.direction, hit.t, hit.coordinates.x, hit.coordinates.y);
+            }
             hits3[3 * r] = o < 0 ? ray.max : hit.t;
             hits3[3 * r + 1] = o < 0 ? 0.0f : hit.coordinates.x;
             hits3[3 * r + 2] = o < 0 ? 0.0f : hit.coordinates.y;
@@ -569,6 +598,12 @@ int orc_scene_trace(const orc_scene* h, const float* rays, uint64_t n, int any, 
     });
     putWork(work, total);
     return 0;
+}
+
+int orc_scene_trace(const orc_scene* h, const float* rays, uint64_t n, int any, float* hits3,
+    uint32_t* patch, int32_t* object, orc_work* work, int threads)
+{
+    return orc_scene_trace_ex(h, rays, n, any, hits3, patch, object, work, threads, nullptr);
 }
 
 // Closest-hit traces of n world rays, recording every patch test in call

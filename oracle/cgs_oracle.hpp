@@ -32,6 +32,8 @@
 //   5 shading: the factor pi on a hit's colour dropped
 //   6 shading: the reflection attenuation without (1 - roughness)^2
 //   7 scene: an object's local t not rescaled by 1 / |M d|
+//   8 polish: the residual S - o + t d instead of S - o - t d
+//   9 polish: the u derivative without the quotient rule's -p w_u term
 //
 #pragma once
 
@@ -478,7 +480,7 @@ template<> struct ClipConst<double>
 // full states (the working patch is stored per entry, so a popped region is
 // exactly the patch that was split, not a recomputation).
 template<class T, class F>
-bool clip2D(F&& onHit, const V2<T> patch[16], T tol, Work* w = nullptr)
+bool clip2D(F&& onHit, const V2<T> patch[16], T tol, Work* w = nullptr, uint32_t maxIters = 0)
 {
     struct State
     {
@@ -496,8 +498,11 @@ bool clip2D(F&& onHit, const V2<T> patch[16], T tol, Work* w = nullptr)
     for (int i = 0; i < 16; i++) S.back().p[i] = patch[i];
     if (w) w->maxStack = std::max<uint64_t>(w->maxStack, 1);
 
+    uint32_t iters = 0;
     do
     {
+        // opt-in cap on the loop passes (no counterpart in the reference)
+        if (maxIters && iters++ >= maxIters) break;
         State& e = S.back();
         V2<T>* p = e.p;
         // Spline_BezierClipping.cu:240-248 — tolerance on the two diagonals
@@ -668,7 +673,7 @@ template<class T> struct PatchHit
 // the Ray3f direction (versor of the caller's direction).
 template<class T>
 bool doBezierClipping(PatchHit<T>& hit, V3<T> origin, V3<T> dir, const V4<T> S[16],
-    T tol = T(0x1p-12f), Work* w = nullptr)
+    T tol = T(0x1p-12f), Work* w = nullptr, uint32_t maxIters = 0)
 {
     if (w) w->patchTests++;
     V2<T> p2D[16];
@@ -692,7 +697,116 @@ bool doBezierClipping(PatchHit<T>& hit, V3<T> origin, V3<T> dir, const V4<T> S[1
         }
         return false;
     };
-    return clip2D<T>(onHit, p2D, tol, w);
+    return clip2D<T>(onHit, p2D, tol, w, maxIters);
+}
+
+// Newton polish of a closest hit (an opt-in refinement with no counterpart
+// in the reference; the library's polishHit64, cgs_device.hpp, with the same
+// FP64 operations in the same order).  The object-local ray is rebuilt in
+// FP64 from the FP32 world ray and world→local matrix, o_l + t M3 d with t
+// the world parameter; up to six Newton steps on S(u,v) - o_l - t M3 d = 0
+// (Cramer's rule, the quotient rule for w) refine (u, v, t), kept when the
+// last step is below 1e-9 in u and v and lands inside the parameter square
+// with t > 0.
+namespace polish
+{
+inline void comb4(const double* a, const double* b, const double* c, const double* d, const double* x, double* r)
+{
+    for (int k = 0; k < 4; k++) r[k] = ((x[0] * a[k] + x[1] * b[k]) + x[2] * c[k]) + x[3] * d[k];
+}
+
+// the cubic Bernstein polynomials and their derivatives in power form
+inline void basis(double t, double* b, double* db)
+{
+    const double t2 = t * t, t3 = t2 * t;
+    b[0] = ((1.0 - 3.0 * t) + 3.0 * t2) - t3;
+    b[1] = (3.0 * t - 6.0 * t2) + 3.0 * t3;
+    b[2] = 3.0 * t2 - 3.0 * t3;
+    b[3] = t3;
+    db[0] = (-3.0 + 6.0 * t) - 3.0 * t2;
+    db[1] = (3.0 - 12.0 * t) + 9.0 * t2;
+    db[2] = 6.0 * t - 9.0 * t2;
+    db[3] = 3.0 * t2;
+}
+} // namespace polish
+
+inline bool polishHit64(const V4<float> S[16], const M4<float>& w2l, V3<float> ow, V3<float> dw, float& t, float& u,
+                        float& v)
+{
+    using V3d = V3<double>;
+    double M[4][4];
+    for (int c = 0; c < 4; c++)
+    {
+        M[c][0] = w2l.c[c].x;
+        M[c][1] = w2l.c[c].y;
+        M[c][2] = w2l.c[c].z;
+        M[c][3] = w2l.c[c].w;
+    }
+    const double ox = ow.x, oy = ow.y, oz = ow.z, dx = dw.x, dy = dw.y, dz = dw.z;
+    double O4[4];
+    for (int k = 0; k < 4; k++) O4[k] = ((ox * M[0][k] + oy * M[1][k]) + oz * M[2][k]) + M[3][k];
+    const V3d o {O4[0] / O4[3], O4[1] / O4[3], O4[2] / O4[3]};
+    const V3d D {(dx * M[0][0] + dy * M[1][0]) + dz * M[2][0], (dx * M[0][1] + dy * M[1][1]) + dz * M[2][1],
+                 (dx * M[0][2] + dy * M[1][2]) + dz * M[2][2]};
+    double pu = u, pv = v, pt = t;
+    bool converged = false;
+    for (int it = 0; it < 6; it++)
+    {
+        double bu[4], dbu[4], bv[4], dbv[4];
+        polish::basis(pu, bu, dbu);
+        polish::basis(pv, bv, dbv);
+        double a[4][4], ad[4][4];
+        for (int j = 0; j < 4; j++)
+        {
+            double q[4][4];
+            for (int i = 0; i < 4; i++)
+            {
+                const V4<float>& c = S[4 * j + i];
+                q[i][0] = c.x;
+                q[i][1] = c.y;
+                q[i][2] = c.z;
+                q[i][3] = c.w;
+            }
+            polish::comb4(q[0], q[1], q[2], q[3], bu, a[j]);
+            polish::comb4(q[0], q[1], q[2], q[3], dbu, ad[j]);
+        }
+        double X[4], Xu[4], Xv[4];
+        polish::comb4(a[0], a[1], a[2], a[3], bv, X);
+        polish::comb4(ad[0], ad[1], ad[2], ad[3], bv, Xu);
+        polish::comb4(a[0], a[1], a[2], a[3], dbv, Xv);
+        const double iw = 1.0 / X[3];
+        const V3d p {X[0] * iw, X[1] * iw, X[2] * iw};
+        const V3d Pu = CGS_ORACLE_PLANT == 9
+                           ? V3d {Xu[0] * iw, Xu[1] * iw, Xu[2] * iw}
+                           : V3d {(Xu[0] - p.x * Xu[3]) * iw, (Xu[1] - p.y * Xu[3]) * iw, (Xu[2] - p.z * Xu[3]) * iw};
+        const V3d Pv {(Xv[0] - p.x * Xv[3]) * iw, (Xv[1] - p.y * Xv[3]) * iw, (Xv[2] - p.z * Xv[3]) * iw};
+        const double sgn = CGS_ORACLE_PLANT == 8 ? -1.0 : 1.0;
+        const V3d F {(p.x - o.x) - sgn * pt * D.x, (p.y - o.y) - sgn * pt * D.y, (p.z - o.z) - sgn * pt * D.z};
+        const V3d mD {-D.x, -D.y, -D.z};
+        auto dot = [](V3d a, V3d b) { return a.x * b.x + a.y * b.y + a.z * b.z; };
+        auto cross = [](V3d a, V3d b) -> V3d
+        { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; };
+        const double det = dot(Pu, cross(Pv, mD));
+        if (!(det > 1e-300 || det < -1e-300)) return false;
+        const double idet = 1.0 / det;
+        const V3d nF {-F.x, -F.y, -F.z};
+        const double du = dot(nF, cross(Pv, mD)) * idet;
+        const double dv = dot(Pu, cross(nF, mD)) * idet;
+        const double dt = dot(Pu, cross(Pv, nF)) * idet;
+        pu += du;
+        pv += dv;
+        pt += dt;
+        if ((du < 1e-9 && du > -1e-9) && (dv < 1e-9 && dv > -1e-9))
+        {
+            converged = true;
+            break;
+        }
+    }
+    if (!converged || !(pu >= 0.0 && pu <= 1.0 && pv >= 0.0 && pv <= 1.0 && pt > 0.0)) return false;
+    t = (float)pt;
+    u = (float)pu;
+    v = (float)pv;
+    return true;
 }
 
 // ---------------------------------------------------------------------------

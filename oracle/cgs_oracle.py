@@ -39,6 +39,10 @@ class SceneDesc(C.Structure):
         ("background", _F32 * 3), ("ambient", _F32 * 3), ("seed", _U32)]
 
 
+class TraceOpts(C.Structure):
+    _fields_ = [("method", C.c_int), ("tol", _F32), ("threshold", _F32), ("maxIters", _U32), ("polish", C.c_int)]
+
+
 class CameraDesc(C.Structure):
     _fields_ = [("toWorld", _F32 * 16), ("position", _F32 * 3), ("fovy", _F32), ("flipY", C.c_int)]
 
@@ -98,6 +102,7 @@ def lib():
                                      _U32, _U32, _U32, _P]
         L.orc_cpu_render.restype = _F64
         L.orc_set_clip_stack_cap.argtypes = [C.c_int]
+        L.orc_scene_trace_ex.argtypes = [_P, _P, _U64, C.c_int, _P, _P, _P, _P, C.c_int, C.POINTER(TraceOpts)]
         L.orc_pixel_rays.argtypes = [C.POINTER(CameraDesc), _U32, _U32, _P, _U64, _P]
         _lib = L
     return _lib
@@ -322,7 +327,13 @@ class OracleScene:
         return dict(nodes=nodes, keys=keys, indices=idx, A=int(p[0]), B=int(p[1]), mask=int(p[2]),
                     table_size=int(p[3]), attempts=int(p[4]), displacement=D, table=T)
 
-    def trace(self, rays, any_hit=False, threads=None, stack=False):
+    def trace(self, rays, any_hit=False, threads=None, stack=False, method=0, clip_tol=0.0, subdiv_depth=0,
+              max_clip_iters=0, polish=False):
+        """Closest (or any) hits of world rays, with cgs_trace_options' meaning:
+        method 0 clips (clip_tol, 0: the reference's 2^-12), 1 subdivides
+        (subdiv_depth D: threshold 2^(1-D), 0: the reference's 2^-4);
+        max_clip_iters caps each clip; polish refines closest hits."""
+        tol = clip_tol if clip_tol > 0 else 2.0 ** -12
         rays = np.ascontiguousarray(rays, np.float32)
         n = rays.shape[0]
         hits = np.zeros((n, 3), np.float32)
@@ -331,7 +342,10 @@ class OracleScene:
         w = Work()
         th = threads or min(16, os.cpu_count() or 1)
         mode = 1 if any_hit else (2 if stack else 0)
-        lib().orc_scene_trace(self.h, _ptr(rays), n, mode, _ptr(hits), _ptr(patch), _ptr(obj), C.byref(w), th)
+        thr = 2.0 ** (1 - subdiv_depth) if subdiv_depth else 2.0 ** -4
+        op = TraceOpts(int(method), float(tol), float(thr), int(max_clip_iters), int(bool(polish)))
+        lib().orc_scene_trace_ex(self.h, _ptr(rays), n, mode, _ptr(hits), _ptr(patch), _ptr(obj), C.byref(w), th,
+                                 C.byref(op))
         if any_hit:
             return obj.astype(bool), w.asdict()
         return hits, patch, obj, w.asdict()

@@ -139,8 +139,29 @@ struct PatchTestLog
 };
 inline thread_local PatchTestLog* g_patchLog = nullptr;
 
+// Leaf method of a query, as cgs_trace_options selects it: Bézier clipping at
+// the reference's tolerance by default (Primitive.cu:234,260); subdivision
+// (the commented-out alternative, Primitive.cu:236,263); the opt-in clip
+// pass cap (no counterpart in the reference).  The opt-in Newton polish
+// (polishHit64) refines the reported closest hit after the query, once per
+// ray, so it changes no traversal decision.
+struct TraceOpts
+{
+    int method = 0;          // 0 clipping, 1 subdivision
+    float tol = 0x1p-12f;
+    float threshold = 0x1p-4f;
+    uint32_t maxIters = 0;
+    bool polish = false;
+};
+
+inline bool leafTest(const TraceOpts& op, PatchHit<float>& ph, f3 origin, f3 dir, const f4 S[16], Work* w)
+{
+    return op.method == 1 ? doSubdivision<float>(ph, origin, dir, S, op.threshold)
+                          : doBezierClipping<float>(ph, origin, dir, S, op.tol, w, op.maxIters);
+}
+
 // rt/Primitive.cu:222-248 — closest hit against one surface (local ray).
-inline bool surfaceIntersect(const Surface& s, Hit& hit0, const Ray& ray0, Work* w)
+inline bool surfaceIntersect(const Surface& s, Hit& hit0, const Ray& ray0, Work* w, const TraceOpts& op = {})
 {
     // Ds Ray3f{origin, direction} normalizes the direction; the result is the
     // same for every patch of this ray, so it is computed once here.
@@ -158,7 +179,7 @@ inline bool surfaceIntersect(const Surface& s, Hit& hit0, const Ray& ray0, Work*
             for (int i = 0; i < 16; i++)
                 g_patchLog->patches.insert(g_patchLog->patches.end(), {S[i].x, S[i].y, S[i].z, S[i].w});
         }
-        if (doBezierClipping<float>(ph, ray0.origin, dir, S, 0x1p-12f, w))
+        if (leafTest(op, ph, ray0.origin, dir, S, w))
         {
             hit.index = index;
             hit.t = ph.distance;
@@ -171,7 +192,7 @@ inline bool surfaceIntersect(const Surface& s, Hit& hit0, const Ray& ray0, Work*
 }
 
 // rt/Primitive.cu:250-268 — any hit against one surface (local ray).
-inline bool surfaceIntersectAny(const Surface& s, const Ray& ray0, Work* w)
+inline bool surfaceIntersectAny(const Surface& s, const Ray& ray0, Work* w, const TraceOpts& op = {})
 {
     const f3 dir = versor(ray0.direction);
     auto fn = [&](uint32_t index) -> bool
@@ -179,7 +200,7 @@ inline bool surfaceIntersectAny(const Surface& s, const Ray& ray0, Work* w)
         PatchHit<float> ph {ray0.max, {0, 0}};
         f4 S[16];
         s.patch(index, S);
-        return doBezierClipping<float>(ph, ray0.origin, dir, S, 0x1p-12f, w);
+        return leafTest(op, ph, ray0.origin, dir, S, w);
     };
     return hashIntersectAny(s.view(), ray0, fn, w);
 }
@@ -200,7 +221,8 @@ inline Ray localRay(const Object& o, const Ray& ray, float& dinv)
 }
 
 // RayTracer.cu:296-350 — Context::intersect(hit, ray): returns the nearest object.
-inline int sceneIntersect(const Scene& sc, Hit& hit0, const Ray& ray0, uint32_t* patchOut, Work* w)
+inline int sceneIntersect(const Scene& sc, Hit& hit0, const Ray& ray0, uint32_t* patchOut, Work* w,
+                          const TraceOpts& op = {})
 {
     int nearest = -1;
     auto fn = [&](Hit& hit, uint32_t i) -> bool
@@ -212,7 +234,7 @@ inline int sceneIntersect(const Scene& sc, Hit& hit0, const Ray& ray0, uint32_t*
         Ray lr = localRay(o, ray0, d);
         Hit lh;
         lh.t = lr.max;
-        bool b = surfaceIntersect(sc.surfaces[o.surface], lh, lr, w);
+        bool b = surfaceIntersect(sc.surfaces[o.surface], lh, lr, w, op);
         if (b)
         {
             float t = CGS_ORACLE_PLANT == 7 ? lh.t : lh.t * d;
@@ -282,7 +304,7 @@ inline int sceneIntersectStack(const Scene& sc, Hit& hit0, const Ray& ray0, uint
 }
 
 // RayTracer.cu:352-383 — Context::intersect(ray): any hit.
-inline bool sceneIntersectAny(const Scene& sc, const Ray& ray0, Work* w)
+inline bool sceneIntersectAny(const Scene& sc, const Ray& ray0, Work* w, const TraceOpts& op = {})
 {
     auto fn = [&](uint32_t i) -> bool
     {
@@ -291,7 +313,7 @@ inline bool sceneIntersectAny(const Scene& sc, const Ray& ray0, Work* w)
         if (o.surface == EMPTY) return false;   // no geometry (never reached: empty box)
         float d;
         Ray lr = localRay(o, ray0, d);
-        return surfaceIntersectAny(sc.surfaces[o.surface], lr, w);
+        return surfaceIntersectAny(sc.surfaces[o.surface], lr, w, op);
     };
     return hashIntersectAny(sc.topView(), ray0, fn, w);
 }

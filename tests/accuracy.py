@@ -136,6 +136,7 @@ class Report:
     max_uv_root: float
     max_rel_t_root32: float
     newton_failed: int
+    max_uv_root32: float = 0.0
 
     def row(self) -> str:
         return (f"| {self.name} | {self.rays} | {self.hits32} | {self.grazing} / {self.seam} | "
@@ -149,10 +150,11 @@ HEADER = ("| workload | rays | hits | grazing / seam | hit-miss / patch disagree
           "FP32 vs root: rel t |\n|---|---|---|---|---|---|---|---|---|---|")
 
 
-def fp32_closest(scene, rays):
-    """(t,u,v), patch, object of the FP32 path (the oracle's restatement)."""
+def fp32_closest(scene, rays, **options):
+    """(t,u,v), patch, object of the FP32 path (the oracle's restatement),
+    with the trace options of cgs_trace_options (e.g. polish=True)."""
     import cgs_oracle
-    h32, p32, o32, _ = cgs_oracle.OracleScene(scene).trace(rays)
+    h32, p32, o32, _ = cgs_oracle.OracleScene(scene).trace(rays, **options)
     return h32, p32, o32
 
 
@@ -197,13 +199,14 @@ def _measure(name, scene, rays, fp32, fp64):
     uv_root = np.maximum(np.abs(f.u - root[:, 0]), np.abs(f.v - root[:, 1]))
     pr32 = polished & same
     rel_root32 = np.abs(h32[:, 0].astype(np.float64) - root[:, 2]) / root[:, 2]
+    uv_root32 = np.maximum(np.abs(h32[:, 1] - root[:, 0]), np.abs(h32[:, 2] - root[:, 1]))
     mx = lambda a, m: float(a[m].max()) if m.any() else 0.0
     return Report(name, int(rays.shape[0]), int(hit32.sum()), int(hit64.sum()), int(grazing.sum()), int(seam.sum()),
                   int((hm & ~flagged).sum()), int((pd & ~flagged).sum()), int((hm & flagged).sum()),
                   int((pd & flagged).sum()), mx(rel, good), mx(duv, good),
                   float(np.percentile(rel[good], 99)) if good.any() else 0.0,
                   mx(rel_root, polished), mx(uv_root, polished), mx(rel_root32, pr32),
-                  int((hit64 & ~ok).sum()))
+                  int((hit64 & ~ok).sum()), mx(uv_root32, pr32))
 
 
 def workloads():
