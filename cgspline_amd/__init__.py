@@ -357,6 +357,11 @@ class DeviceScene:
         check(lib().cgs_scene_create(ctx.h, C.byref(d), C.byref(p)), "cgs_scene_create")
         self.h = p.value
 
+    @property
+    def culled(self) -> bool:
+        """Whether the queries find objects through the cull tree (cgs_scene_culled)."""
+        return lib().cgs_scene_culled(self.h) == 1
+
     def destroy(self):
         if getattr(self, "_owner", None) is not None:   # a view: the owner destroys
             self.h = None

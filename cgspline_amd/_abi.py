@@ -33,7 +33,7 @@ EXPORTS = [
     "cgs_bezier_clip", "cgs_subdivision", "cgs_bvh_build", "cgs_patch_elements", "cgs_mesh_elements", "cgs_free_host", "cgs_load_be",
     "cgs_dl_read", "cgs_dl_parse", "cgs_dl_free",
     "cgs_phf_displacement_size", "cgs_phf_build", "cgs_phf_build_static", "cgs_phf_fill_table", "cgs_phf_hash",
-    "cgs_scene_create", "cgs_scene_destroy", "cgs_scene_bvh_info", "cgs_scene_bvh_copy",
+    "cgs_scene_create", "cgs_scene_destroy", "cgs_scene_bvh_info", "cgs_scene_bvh_copy", "cgs_scene_culled",
     "cgs_trace_closest", "cgs_trace_any", "cgs_trace_closest_ex", "cgs_trace_any_ex", "cgs_render",
     "cgs_render_running",
     "cgs_trace_batch", "cgs_batch_create", "cgs_batch_launch", "cgs_batch_destroy", "cgs_batch_set_config",
@@ -154,6 +154,7 @@ def lib():
             "cgs_scene_destroy": ([_P], C.c_int),
             "cgs_scene_bvh_info": ([_P, _I32, C.POINTER(_U32), C.POINTER(_U32), C.POINTER(PHF)], C.c_int),
             "cgs_scene_bvh_copy": ([_P, _I32, _P, _P, _P, _P, _P], C.c_int),
+            "cgs_scene_culled": ([_P], C.c_int),
             "cgs_trace_closest": ([_P, _P, _P, _U32, _P, _P, _P], C.c_int),
             "cgs_trace_any": ([_P, _P, _P, _U32, _P, _P], C.c_int),
             "cgs_trace_closest_ex": ([_P, _P, _P, _U32, _P, _P, C.POINTER(TraceOptions), _P], C.c_int),

@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <bit>
 #include <string>
+#include <vector>
 
 #include "../../include/cgs.h"
 
@@ -80,6 +81,18 @@ hipError_t launchPhfHash(const cgs_phf& h, const uint32_t* D, const uint32_t* ke
 hipError_t launchPhfFill(const cgs_phf& h, const uint32_t* D, const uint32_t* keys, uint32_t n, uint32_t base,
                          uint32_t* table, hipStream_t s);
 hipError_t launchFill(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
+
+// A cull tree over the leaf boxes of a BVH (cgs_cull.cpp): nodes 8 floats,
+// links 16 words and elements 8 floats each.  buildCullTree returns false
+// (and builds nothing) when the BVH's root is a leaf or no leaf box is finite.
+struct CullTreeHost
+{
+    std::vector<float> nodes;
+    std::vector<uint32_t> links;
+    std::vector<float> elems;
+    uint32_t nodeCount = 0;
+};
+bool buildCullTree(const cgs_bvh_node* refNodes, size_t nnodes, const uint32_t* indices, CullTreeHost& out);
 
 // Persistent kernels: the grid is the resident workgroups; every lane owns a
 // slice of the region-stack buffer and waves pull work from a counter.

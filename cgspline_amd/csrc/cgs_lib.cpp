@@ -496,6 +496,8 @@ struct cgs_scene
     std::vector<DevBVH> dev;       // per surface, then the top level
     std::vector<cgs_phf> phf;
     DeviceBuffer patches, nodes, indices, keys, displ, tables, surfaces, objects, lights, topByKey;
+    DeviceBuffer cullNodes, cullLinks, cullElems;
+    bool cull = false;             // the top-level cull tree is built
     float background[3], ambient[3];
     SceneDev view() const
     {
@@ -509,6 +511,9 @@ struct cgs_scene
         s.objects = objects.as<DevObject>();
         s.lights = lights.as<DevLight>();
         s.topByKey = topByKey.as<uint32_t>();
+        s.cullNodes = cull ? cullNodes.as<float4>() : nullptr;
+        s.cullLinks = cull ? cullLinks.as<uint2>() : nullptr;
+        s.cullElems = cull ? cullElems.as<float4>() : nullptr;
         s.nlight = nlight;
         s.nobj = nobj;
         s.nodeCount = nodeCount;
@@ -737,6 +742,70 @@ cgs_status cgs_phf_hash(cgs_ctx* ctx, const cgs_phf* h, const uint32_t* displace
     return CGS_OK;
 }
 
+// Whether the queries may enumerate a scene's objects through a cull tree
+// (cgs_cull.cpp).  The enumeration skips an object whose top-level box the
+// ray enters well beyond the nearest hit found so far, which is sound when a
+// hit on an object lies inside that object's box: every control point of
+// weight 1 (the box is then the control points' box, and a Bézier patch lies
+// in its control hull) and affine object transforms whose world→local matrix
+// undoes local→world on the object's box corners to within 1e-5 of the box's
+// size plus distance from the origin, in world units.
+// CGS_CULL=0 in the environment turns the tree off (a diagnostic: the
+// results are the same).
+static bool cullAllowed(const cgs_scene_desc* d)
+{
+    if (const char* e = std::getenv("CGS_CULL"))
+        if (e[0] == '0') return false;
+    if (d->nobj < 2) return false;
+    for (uint64_t v = 0; v < d->surf_vbase[d->nsurf]; v++)
+        if (d->vertices[4 * v + 3] != 1.0f) return false;
+    for (uint32_t i = 0; i < d->nobj; i++)
+    {
+        const uint32_t k = d->obj_surface[i];
+        if (k == CGS_EMPTY) continue;
+        const float* L = d->obj_local2world + 16 * (size_t)i;
+        const float* W = d->obj_world2local + 16 * (size_t)i;
+        for (const float* M : {L, W})
+        {
+            if (M[3] != 0.0f || M[7] != 0.0f || M[11] != 0.0f || M[15] != 1.0f) return false;
+            for (int q = 0; q < 16; q++)
+                if (!std::isfinite(M[q])) return false;
+        }
+        // the object's local control-point box, mapped out and back
+        double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        const float* V = d->vertices + 4 * (size_t)d->surf_vbase[k];
+        for (uint64_t q = 16ull * d->surf_pbase[k]; q < 16ull * d->surf_pbase[k + 1]; q++)
+            for (int a = 0; a < 3; a++)
+            {
+                lo[a] = std::min(lo[a], (double)V[4 * (size_t)d->indices[q] + a]);
+                hi[a] = std::max(hi[a], (double)V[4 * (size_t)d->indices[q] + a]);
+            }
+        // the round trip's error, mapped back to world units, against the
+        // box's size and distance from the origin
+        double err = 0.0, mag = 0.0, wlo[3] = {INFINITY, INFINITY, INFINITY}, whi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (int c = 0; c < 8; c++)
+        {
+            const double p[3] = {(c & 1) ? hi[0] : lo[0], (c & 2) ? hi[1] : lo[1], (c & 4) ? hi[2] : lo[2]};
+            double cw[3], e[3];
+            for (int r = 0; r < 3; r++)
+            {
+                cw[r] = (double)L[r] * p[0] + (double)L[4 + r] * p[1] + (double)L[8 + r] * p[2] + (double)L[12 + r];
+                wlo[r] = std::min(wlo[r], cw[r]);
+                whi[r] = std::max(whi[r], cw[r]);
+                mag = std::max(mag, std::fabs(cw[r]));
+            }
+            for (int r = 0; r < 3; r++)
+                e[r] = (double)W[r] * cw[0] + (double)W[4 + r] * cw[1] + (double)W[8 + r] * cw[2] + (double)W[12 + r] - p[r];
+            for (int r = 0; r < 3; r++)
+                err = std::max(err, std::fabs((double)L[r] * e[0] + (double)L[4 + r] * e[1] + (double)L[8 + r] * e[2]));
+        }
+        double ext = 0.0;
+        for (int r = 0; r < 3; r++) ext = std::max(ext, whi[r] - wlo[r]);
+        if (!(err <= 1e-5 * (mag + ext))) return false;
+    }
+    return true;
+}
+
 cgs_status cgs_scene_create(cgs_ctx* ctx, const cgs_scene_desc* d, cgs_scene** out)
 {
     if (!ctx || !d || !out) return fail(CGS_ERR_INVALID, "cgs_scene_create: null argument");
@@ -811,6 +880,11 @@ cgs_status cgs_scene_create(cgs_ctx* ctx, const cgs_scene_desc* d, cgs_scene** o
         cgs_status s = buildBVH(el, 1, sc->bvh[S]);
         if (s != CGS_OK) return s;
     }
+
+    // 2b. the cull tree over the top-level leaf boxes, where the scene allows one
+    CullTreeHost cullTree;
+    sc->cull = cullAllowed(d) && buildCullTree(sc->bvh[S].nodes.data(), sc->bvh[S].nodes.size(),
+                                              sc->bvh[S].indices.data(), cullTree);
 
     // 3. layout: absolute node offsets, leaf slots and patch bases.  The
     // top-level BVH (index S) comes first in the node array: the kernels
@@ -896,6 +970,15 @@ cgs_status cgs_scene_create(cgs_ctx* ctx, const cgs_scene_desc* d, cgs_scene** o
     if (d->nlight)
         HIP_TRY(hipMemcpyAsync(sc->lights.p, d->lights, d->nlight * sizeof(DevLight), hipMemcpyHostToDevice, s));
     sc->nlight = d->nlight;
+    if (sc->cull)
+    {
+        HIP_TRY(sc->cullNodes.alloc(cullTree.nodes.size() * 4));
+        HIP_TRY(sc->cullLinks.alloc(cullTree.links.size() * 4));
+        HIP_TRY(sc->cullElems.alloc(cullTree.elems.size() * 4));
+        HIP_TRY(hipMemcpyAsync(sc->cullNodes.p, cullTree.nodes.data(), cullTree.nodes.size() * 4, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(sc->cullLinks.p, cullTree.links.data(), cullTree.links.size() * 4, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(sc->cullElems.p, cullTree.elems.data(), cullTree.elems.size() * 4, hipMemcpyHostToDevice, s));
+    }
 
     // 5. perfect hashes for every BVH in one launch (BVH::buildHashTable)
     PhfBatch batch;
@@ -960,6 +1043,8 @@ cgs_status cgs_scene_bvh_info(cgs_scene* sc, int32_t surface, uint32_t* node_cou
     if (hash) *hash = sc->phf[k];
     return CGS_OK;
 }
+
+int cgs_scene_culled(cgs_scene* sc) { return sc ? (sc->cull ? 1 : 0) : -1; }
 
 cgs_status cgs_scene_bvh_copy(cgs_scene* sc, int32_t surface, cgs_bvh_node* nodes, uint32_t* keys, uint32_t* indices,
                               uint32_t* displacement, uint32_t* table)

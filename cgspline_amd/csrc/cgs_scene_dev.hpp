@@ -13,6 +13,7 @@
 //   displ    : perfect-hash displacement tables, concatenated
 //   tables   : perfect-hash node tables, concatenated
 //   surfaces : one DevSurface per surface;  objects : one DevObject per object
+//   cull*    : the cull tree over the top-level BVH's leaf boxes (optional)
 //
 #pragma once
 
@@ -70,6 +71,12 @@ struct SceneDev
     const DevObject* __restrict__ objects;
     const DevLight* __restrict__ lights;
     const uint32_t* __restrict__ topByKey;   // kTopKeyCap entries
+    // cull tree over the top-level leaf boxes (cgs_cull.cpp), or null where
+    // the scene does not allow one (the queries then use the reference's
+    // traversal alone)
+    const float4* __restrict__ cullNodes;
+    const uint2* __restrict__ cullLinks;
+    const float4* __restrict__ cullElems;
     uint32_t nlight;
     uint32_t nobj;
     uint32_t nodeCount;

@@ -3,9 +3,11 @@
 // Ray-query and render kernels for gfx950.
 //
 // One ray per lane; 256-lane workgroups (4 waves).  A lane runs the reference's
-// complete per-ray algorithm — top-level hashed traversal, per-object local
-// ray, per-object hashed traversal, Bézier clipping at each leaf — so every
-// result equals the CPU restatement's bit for bit.
+// per-ray algorithm — top-level traversal, per-object local ray, per-object
+// hashed traversal, Bézier clipping at each leaf — so every result equals the
+// CPU restatement's bit for bit.  The objects a ray reaches are found through
+// a cull tree where the scene has one, with the reference's top-level hashed
+// traversal for the rays whose answer could depend on the visit order.
 //
 #include <hip/hip_runtime.h>
 
@@ -244,8 +246,6 @@ struct ClosestHit
     int32_t object;
 };
 
-// Context::intersect(hit, ray) (RayTracer.cu:296-350) with
-// rt::intersect(BezierSurface, hit, ray) (Primitive.cu:222-248) at the leaves.
 // Sibling rings of a lane for the two traversal levels: the top-level BVH
 // over objects (CGS_RING_TOP entries; 4096 objects make it 13 levels deep)
 // and the patch BVH of one object (CGS_RING_BOT; 256 patches, 9 levels).
@@ -273,73 +273,241 @@ CGS_DEV Rings laneRings(uint32_t* lds)
                   {CGS_RING_BOT ? lane + CGS_RING_TOP * kRingStride : nullptr, CGS_RING_BOT}};
 }
 
+// rt::intersect(BezierSurface, hit, ray) (Primitive.cu:222-248) for object
+// i, from RayTracer.cu:305-333: the object-local ray, the per-object hashed
+// traversal from t = ray.max·|M d|, and the leaf test at each patch.  On a
+// hit, t is the world distance (local t / |M d|).
+template<int METHOD>
+CGS_DEV bool objectClosest(const SceneDev& sc, const Rings& rg, uint32_t i, V3 o, V3 d, float tmax,
+                           const ClipStack& stk, bool& overflow, const TraceParams& tp, float& t, float& u, float& v,
+                           uint32_t& patch)
+{
+    const DevObject& ob = sc.objects[i];
+    if (ob.surface == EMPTY) return false;   // no geometry (empty box: only a NaN ray gets here)
+    LocalRay lr = toLocal(ob, o, d, tmax);
+    const DevBVH surf = sc.surfaces[ob.surface];
+    const BVHRef b = bvhRef(sc, surf);
+    const ClipRay cr = makeClipRay(lr.o, versor3(lr.d));
+    float lt = lr.max, lu = 0.0f, lv = 0.0f;
+    uint32_t lp = EMPTY;
+    bool hit = hashIntersect<CGS_BOT_OCTANTS>(b, lr.o, lr.d, lt, [&](uint32_t pid) -> bool {
+        const float4* S = sc.patches + 16 * (size_t)(surf.patchBase + pid);
+        bool f;
+        // the default keeps the direct call: routing it through leafTest
+        // costs the render and trace kernels hundreds of spilled VGPRs
+        if constexpr (METHOD == Clip)
+            f = clipPatch<false>(S, cr, kClipTol, lt, lu, lv, stk, overflow);
+        else
+            f = leafTest<METHOD, false>(S, cr, tp, lt, lu, lv, stk, overflow);
+        if (f) lp = pid;
+        return f;
+    }, NodeCache {nullptr, 0}, rg.bot);
+    if (!hit) return false;
+    t = lt * lr.dinv;
+    u = lu;
+    v = lv;
+    patch = lp;
+    return true;
+}
+
+// rt::intersect(BezierSurface, ray) (Primitive.cu:250-268) for object i.
+template<int METHOD>
+CGS_DEV bool objectAny(const SceneDev& sc, const Rings& rg, uint32_t i, V3 o, V3 d, float tmax, const ClipStack& stk,
+                       bool& overflow, const TraceParams& tp)
+{
+    const DevObject& ob = sc.objects[i];
+    if (ob.surface == EMPTY) return false;   // no geometry (empty box: only a NaN ray gets here)
+    LocalRay lr = toLocal(ob, o, d, tmax);
+    const DevBVH surf = sc.surfaces[ob.surface];
+    const BVHRef b = bvhRef(sc, surf);
+    const ClipRay cr = makeClipRay(lr.o, versor3(lr.d));
+    return hashIntersectAny<CGS_BOT_OCTANTS>(b, lr.o, lr.d, lr.max, [&](uint32_t pid) -> bool {
+        const float4* S = sc.patches + 16 * (size_t)(surf.patchBase + pid);
+        float best = lr.max, bu, bv;
+        if constexpr (METHOD == Clip)
+            return clipPatch<true>(S, cr, kClipTol, best, bu, bv, stk, overflow);
+        else
+            return leafTest<METHOD, true>(S, cr, tp, best, bu, bv, stk, overflow);
+    }, NodeCache {nullptr, 0}, rg.bot);
+}
+
+// ---------------------------------------------------------------------------
+// Object enumeration through the cull tree (cgs_cull.cpp)
+// ---------------------------------------------------------------------------
+//
+// The reference's top-level traversal reaches object X only if the ray
+// passes X's leaf box with entry L0 below the hit distance at that moment
+// (rt/BVH.h:147-266), and every ancestor box contains X's box, so it passes
+// there whenever it passes X's (the slab test is monotone in the box).
+// Each object's own result does not depend on the visit order (its traversal
+// starts from ray.max·|M d|, RayTracer.cu:311-314).  So:
+//  - any hit: the reference's answer is whether some object whose leaf box
+//    the ray passes with L0 < ray.max has a hit, in any order;
+//  - closest: enumerating every such object whose L0 is not far beyond the
+//    nearest hit found so far (a hit lies inside its object's box, up to
+//    rounding, for which the slack below leaves a wide margin) finds the
+//    nearest hit W.  The reference reaches W when L0_W < t_W (its limit is
+//    never below t_W), and then reports W unless another object has a hit at
+//    exactly t_W, where its visit order decides.
+// A lane falls back to the reference's traversal for a tie, for L0_W >= t_W,
+// for a clip-stack overflow on the way (so that overflows are reported as in
+// the reference order), and for rays on which the slab test could meet a
+// NaN (a zero or non-finite direction component, a non-finite origin).
+
+// The cull tree's node and element records, read per lane.
+struct CullIter
+{
+    uint32_t node, elem, elemEnd;
+};
+
+CGS_DEV bool cullable(V3 o, V3 Dinv)
+{
+    auto ok = [](float oi, float di) { return __builtin_isfinite(oi) && __builtin_isfinite(di) && di != 0.0f; };
+    return ok(o.x, Dinv.x) && ok(o.y, Dinv.y) && ok(o.z, Dinv.z);
+}
+
+// The next object whose reference leaf box the ray passes with entry
+// below tmax and at most lim; its entry in L0.  EMPTY at the end.
+CGS_DEV uint32_t cullNext(const SceneDev& sc, int oct, V3 o, V3 Dinv, float lim, float tmax, CullIter& it, float& L0)
+{
+    while (true)
+    {
+        while (it.elem < it.elemEnd)
+        {
+            const float4 a = sc.cullElems[2 * (size_t)it.elem], b = sc.cullElems[2 * (size_t)it.elem + 1];
+            it.elem++;
+            float l;
+            if (nodeIntersect<-1>(l, {a.x, a.y, a.z}, {b.x, b.y, b.z}, o, Dinv) && l < tmax && l <= lim)
+            {
+                L0 = l;
+                return __float_as_uint(a.w);
+            }
+        }
+        if (it.node == EMPTY) return EMPTY;
+        const float4 a = sc.cullNodes[2 * (size_t)it.node], b = sc.cullNodes[2 * (size_t)it.node + 1];
+        const uint2 l = sc.cullLinks[8 * (size_t)it.node + oct];
+        float tn;
+        const bool pass = nodeIntersect<-1>(tn, {a.x, a.y, a.z}, {b.x, b.y, b.z}, o, Dinv) && tn <= lim;
+        const uint32_t count = __float_as_uint(b.w);
+        if (pass && count)
+        {
+            it.elem = __float_as_uint(a.w);
+            it.elemEnd = it.elem + count;
+        }
+        it.node = pass && !count ? l.x : l.y;
+    }
+}
+
+// How far beyond the nearest hit so far an object's box entry may lie and
+// the object still be tested: 2^-10 of the distance plus the origin's
+// magnitude, orders of magnitude above the FP32 rounding of a hit distance
+// or a box entry.
+CGS_DEV float cullLimit(float best, float mag) { return best + (best + mag) * 0x1p-10f; }
+
+CGS_DEV int dirOctant(V3 Dinv)
+{
+    return (__builtin_signbit(Dinv.x) ? 1 : 0) | (__builtin_signbit(Dinv.y) ? 2 : 0) | (__builtin_signbit(Dinv.z) ? 4 : 0);
+}
+
+template<int METHOD>
+CGS_DEV ClosestHit closestCulled(const SceneDev& sc, const Rings& rg, V3 o, V3 d, V3 Dinv, float tmax,
+                                 const ClipStack& stk, const TraceParams& tp, bool& exact)
+{
+    ClosestHit h {tmax, 0.0f, 0.0f, EMPTY, -1};
+    const int oct = dirOctant(Dinv);
+    const float mag = fmax_(__builtin_fabsf(o.x), fmax_(__builtin_fabsf(o.y), __builtin_fabsf(o.z)));
+    float L0W = 0.0f, L0;
+    bool tie = false, overflow = false;
+    CullIter it {0u, 0u, 0u};
+    uint32_t i;
+    while ((i = cullNext(sc, oct, o, Dinv, cullLimit(h.t, mag), tmax, it, L0)) != EMPTY)
+    {
+        float t, u, v;
+        uint32_t p;
+        if (!objectClosest<METHOD>(sc, rg, i, o, d, tmax, stk, overflow, tp, t, u, v, p)) continue;
+        if (t < h.t)
+        {
+            h = {t, u, v, p, (int32_t)i};
+            L0W = L0;
+            tie = false;
+        }
+        else if (t == h.t)
+            tie = true;
+    }
+    exact = overflow || tie || (h.object >= 0 && !(L0W < h.t));
+    return h;
+}
+
+template<int METHOD>
+CGS_DEV bool anyCulled(const SceneDev& sc, const Rings& rg, V3 o, V3 d, V3 Dinv, float tmax, const ClipStack& stk,
+                       const TraceParams& tp, bool& exact)
+{
+    const int oct = dirOctant(Dinv);
+    bool overflow = false;
+    CullIter it {0u, 0u, 0u};
+    float L0;
+    uint32_t i;
+    while ((i = cullNext(sc, oct, o, Dinv, tmax, tmax, it, L0)) != EMPTY)
+    {
+        const bool hit = objectAny<METHOD>(sc, rg, i, o, d, tmax, stk, overflow, tp);
+        if (overflow) break;
+        if (hit) return true;
+    }
+    exact = overflow;
+    return false;
+}
+
+// Context::intersect(hit, ray) (RayTracer.cu:296-350): the nearest object
+// hit, through the cull tree where it applies, else (or where its answer
+// could depend on the visit order) by the reference's top-level traversal.
 template<int METHOD = Clip>
 CGS_DEV ClosestHit sceneClosest(const SceneDev& sc, const NodeCache& nc, const Rings& rg, V3 o, V3 d, float tmax,
                                 const ClipStack& stk, bool& overflow, const TraceParams& tp = {kClipTol, 0x1p-4f, 0})
 {
+    if (sc.cullNodes)
+    {
+        const V3 Dinv {1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
+        if (cullable(o, Dinv))
+        {
+            bool exact = false;
+            const ClosestHit h = closestCulled<METHOD>(sc, rg, o, d, Dinv, tmax, stk, tp, exact);
+            if (!exact) return h;
+        }
+    }
     ClosestHit h {tmax, 0.0f, 0.0f, EMPTY, -1};
     const BVHRef top = bvhRef(sc, sc.top);
     hashIntersect<true, true>(top, o, d, h.t, [&](uint32_t i) -> bool {
-        const DevObject& ob = sc.objects[i];
-        if (ob.surface == EMPTY) return false;   // no geometry (empty box: only a NaN ray gets here)
-        LocalRay lr = toLocal(ob, o, d, tmax);
-        const DevBVH surf = sc.surfaces[ob.surface];
-        const BVHRef b = bvhRef(sc, surf);
-        const ClipRay cr = makeClipRay(lr.o, versor3(lr.d));
-        float lt = lr.max, lu = 0.0f, lv = 0.0f;
-        uint32_t lp = EMPTY;
-        bool hit = hashIntersect<CGS_BOT_OCTANTS>(b, lr.o, lr.d, lt, [&](uint32_t pid) -> bool {
-            const float4* S = sc.patches + 16 * (size_t)(surf.patchBase + pid);
-            bool f;
-            // the default keeps the direct call: routing it through leafTest
-            // costs the render and trace kernels hundreds of spilled VGPRs
-            if constexpr (METHOD == Clip)
-                f = clipPatch<false>(S, cr, kClipTol, lt, lu, lv, stk, overflow);
-            else
-                f = leafTest<METHOD, false>(S, cr, tp, lt, lu, lv, stk, overflow);
-            if (f) lp = pid;
-            return f;
-        }, NodeCache {nullptr, 0}, rg.bot);
-        if (hit)
+        float t, u, v;
+        uint32_t p;
+        if (objectClosest<METHOD>(sc, rg, i, o, d, tmax, stk, overflow, tp, t, u, v, p) && t < h.t)
         {
-            float t = lt * lr.dinv;
-            if (t < h.t)
-            {
-                h.t = t;
-                h.u = lu;
-                h.v = lv;
-                h.patch = lp;
-                h.object = (int32_t)i;
-                return true;
-            }
+            h = {t, u, v, p, (int32_t)i};
+            return true;
         }
         return false;
     }, nc, rg.top);
     return h;
 }
 
-// Context::intersect(ray) (RayTracer.cu:352-383) with
-// rt::intersect(BezierSurface, ray) (Primitive.cu:250-268) at the leaves.
+// Context::intersect(ray) (RayTracer.cu:352-383), through the cull tree
+// where it applies.
 template<int METHOD = Clip>
 CGS_DEV bool sceneAny(const SceneDev& sc, const NodeCache& nc, const Rings& rg, V3 o, V3 d, float tmax,
                       const ClipStack& stk, bool& overflow, const TraceParams& tp = {kClipTol, 0x1p-4f, 0})
 {
+    if (sc.cullNodes)
+    {
+        const V3 Dinv {1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
+        if (cullable(o, Dinv))
+        {
+            bool exact = false;
+            if (anyCulled<METHOD>(sc, rg, o, d, Dinv, tmax, stk, tp, exact)) return true;
+            if (!exact) return false;
+        }
+    }
     const BVHRef top = bvhRef(sc, sc.top);
     return hashIntersectAny<true, true>(top, o, d, tmax, [&](uint32_t i) -> bool {
-        const DevObject& ob = sc.objects[i];
-        if (ob.surface == EMPTY) return false;   // no geometry (empty box: only a NaN ray gets here)
-        LocalRay lr = toLocal(ob, o, d, tmax);
-        const DevBVH surf = sc.surfaces[ob.surface];
-        const BVHRef b = bvhRef(sc, surf);
-        const ClipRay cr = makeClipRay(lr.o, versor3(lr.d));
-        return hashIntersectAny<CGS_BOT_OCTANTS>(b, lr.o, lr.d, lr.max, [&](uint32_t pid) -> bool {
-            const float4* S = sc.patches + 16 * (size_t)(surf.patchBase + pid);
-            float best = lr.max, bu, bv;
-            if constexpr (METHOD == Clip)
-                return clipPatch<true>(S, cr, kClipTol, best, bu, bv, stk, overflow);
-            else
-                return leafTest<METHOD, true>(S, cr, tp, best, bu, bv, stk, overflow);
-        }, NodeCache {nullptr, 0}, rg.bot);
+        return objectAny<METHOD>(sc, rg, i, o, d, tmax, stk, overflow, tp);
     }, nc, rg.top);
 }
 

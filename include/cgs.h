@@ -344,6 +344,18 @@ cgs_status cgs_scene_bvh_info(cgs_scene* scene, int32_t surface, uint32_t* node_
 cgs_status cgs_scene_bvh_copy(cgs_scene* scene, int32_t surface, cgs_bvh_node* nodes, uint32_t* keys,
                               uint32_t* indices, uint32_t* displacement, uint32_t* table);
 
+/* 1 when the scene's ray queries find the objects a ray reaches through a
+ * cull tree (an SAH BVH over the top-level leaf boxes) instead of walking the
+ * reference's top-level BVH; 0 otherwise, -1 for a null scene.  No
+ * counterpart in the reference, and no effect on any result: a ray whose
+ * answer could depend on the visit order (an exact tie between objects, a
+ * hit before its object's box entry, a clip-stack overflow, a direction
+ * component of zero) is traced by the reference's traversal.  Built when the
+ * scene has two or more objects, every control point has weight 1 and every
+ * object transform is affine with a consistent inverse; CGS_CULL=0 in the
+ * environment at scene creation turns it off. */
+int cgs_scene_culled(cgs_scene* scene);
+
 /* Scenes are read-only once created: any context on the scene's device may
  * trace or render it, and several contexts may do so at the same time.
  * The trace, clip, render and batched calls of one context share its region

@@ -37,14 +37,14 @@ extern "C" {
 struct orc_work
 {
     uint64_t nodes, leaves, hashProbes, patchTests, clipIters, subdivs, prunes, converged, onHits, maxStack, objects,
-        maxPostponed, overflows;
+        maxPostponed, overflows, topNodes;
 };
 
 static void putWork(orc_work* o, const Work& w)
 {
     if (!o) return;
     *o = {w.nodes, w.leaves, w.hashProbes, w.patchTests, w.clipIters, w.subdivs, w.prunes, w.converged, w.onHits, w.maxStack, w.objects,
-          w.maxPostponed, w.overflows};
+          w.maxPostponed, w.overflows, w.topNodes};
 }
 
 int orc_version(void) { return 1; }

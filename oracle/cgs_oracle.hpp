@@ -209,9 +209,11 @@ struct Work
     uint64_t objects = 0;     // top-level leaf element (object) tests
     uint64_t maxPostponed = 0; // most siblings postponed at once in one traversal
     uint64_t overflows = 0;   // clips stopped by the optional stack cap (clipStackCap)
+    uint64_t topNodes = 0;    // the part of `nodes` visited in scenes' top-level BVHs
     void add(const Work& o)
     {
         overflows += o.overflows;
+        topNodes += o.topNodes;
         objects += o.objects;
         maxPostponed = std::max(maxPostponed, o.maxPostponed);
         nodes += o.nodes; leaves += o.leaves; hashProbes += o.hashProbes;

@@ -25,7 +25,7 @@ _U32, _U64, _I32, _F32, _F64 = C.c_uint32, C.c_uint64, C.c_int32, C.c_float, C.c
 class Work(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in (
         "nodes", "leaves", "hashProbes", "patchTests", "clipIters", "subdivs", "prunes",
-        "converged", "onHits", "maxStack", "objects", "maxPostponed", "overflows")]
+        "converged", "onHits", "maxStack", "objects", "maxPostponed", "overflows", "topNodes")]
 
     def asdict(self):
         return {n: int(getattr(self, n)) for n, _ in self._fields_}

@@ -225,6 +225,8 @@ inline int sceneIntersect(const Scene& sc, Hit& hit0, const Ray& ray0, uint32_t*
                           const TraceOpts& op = {})
 {
     int nearest = -1;
+    uint64_t bottom = 0;
+    const uint64_t before = w ? w->nodes : 0;
     auto fn = [&](Hit& hit, uint32_t i) -> bool
     {
         if (w) w->objects++;
@@ -234,7 +236,9 @@ inline int sceneIntersect(const Scene& sc, Hit& hit0, const Ray& ray0, uint32_t*
         Ray lr = localRay(o, ray0, d);
         Hit lh;
         lh.t = lr.max;
+        const uint64_t n0 = w ? w->nodes : 0;
         bool b = surfaceIntersect(sc.surfaces[o.surface], lh, lr, w, op);
+        if (w) bottom += w->nodes - n0;
         if (b)
         {
             float t = CGS_ORACLE_PLANT == 7 ? lh.t : lh.t * d;
@@ -250,6 +254,7 @@ inline int sceneIntersect(const Scene& sc, Hit& hit0, const Ray& ray0, uint32_t*
         return false;
     };
     hashIntersect(sc.topView(), hit0, ray0, fn, w);
+    if (w) w->topNodes += w->nodes - before - bottom;
     if (patchOut) *patchOut = hit0.index;
     return nearest;
 }
@@ -306,6 +311,8 @@ inline int sceneIntersectStack(const Scene& sc, Hit& hit0, const Ray& ray0, uint
 // RayTracer.cu:352-383 — Context::intersect(ray): any hit.
 inline bool sceneIntersectAny(const Scene& sc, const Ray& ray0, Work* w, const TraceOpts& op = {})
 {
+    uint64_t bottom = 0;
+    const uint64_t before = w ? w->nodes : 0;
     auto fn = [&](uint32_t i) -> bool
     {
         if (w) w->objects++;
@@ -313,9 +320,14 @@ inline bool sceneIntersectAny(const Scene& sc, const Ray& ray0, Work* w, const T
         if (o.surface == EMPTY) return false;   // no geometry (never reached: empty box)
         float d;
         Ray lr = localRay(o, ray0, d);
-        return surfaceIntersectAny(sc.surfaces[o.surface], lr, w, op);
+        const uint64_t n0 = w ? w->nodes : 0;
+        const bool b = surfaceIntersectAny(sc.surfaces[o.surface], lr, w, op);
+        if (w) bottom += w->nodes - n0;
+        return b;
     };
-    return hashIntersectAny(sc.topView(), ray0, fn, w);
+    const bool any = hashIntersectAny(sc.topView(), ray0, fn, w);
+    if (w) w->topNodes += w->nodes - before - bottom;
+    return any;
 }
 
 // ---------------------------------------------------------------------------
