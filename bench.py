@@ -9,10 +9,12 @@ every ray the launch traces (primary + shadow + secondary).
 
 Prints one JSON line.  Besides the contract's fields:
 
-* `roofline`: the render kernel against its bound, FP32 VALU: algorithmic
-  FP32 operations per launch (the oracle's event counts × the per-event
-  operation counts of cgspline_amd/workmodel.py) ÷ the kernel's time measured
-  with HIP events on the launch stream.  `traffic` is the measured HBM bytes
+* `roofline`: the render kernel against its bound, FP32 VALU: FP32
+  operations per launch (the library's own event counts, from its
+  CGS_WORK_COUNTS build rendering the same frame in a child process, × the
+  per-event operation counts of cgspline_amd/workmodel.py) ÷ the kernel's
+  time measured with HIP events on the launch stream; beside it the
+  reference algorithm's operations on the same rays (the oracle's counts).  `traffic` is the measured HBM bytes
   per launch (profiles/kRender_displaced_grids_hbm_traffic.md) and `hbm` that
   traffic over the same kernel time; `issue` gives the SQ counters of
   profiles/kRender_displaced_grids_valu.md.
@@ -69,6 +71,46 @@ def measured_issue(workload):
         return None
     with open(path) as f:
         return json.load(f)
+
+
+_COUNT_CHILD = r"""
+import json, os, sys
+sys.path.insert(0, os.environ["CGS_ROOT"])
+import ctypes as C
+import cgspline_amd as cgs
+from cgspline_amd import scenes, workmodel
+a = json.loads(sys.argv[1])
+if a["scene_file"]:
+    wl = scenes.from_dl(a["scene_file"], width=a["width"], height=a["height"], depth=a["depth"])
+else:
+    wl = scenes.WORKLOADS[a["workload"]]()
+ctx = cgs.Context(0)
+ds = ctx.scene(wl.scene)
+out = (C.c_uint64 * len(workmodel.GPU_EVENTS))()
+cgs.check(cgs.lib().cgs_work_counts(ctx.h, out, len(out), 1), "cgs_work_counts")
+color, frame, counts = ds.render(wl.camera, eps=wl.eps, depth=wl.depth)
+cgs.check(cgs.lib().cgs_work_counts(ctx.h, out, len(out), 1), "cgs_work_counts")
+print(json.dumps({"events": dict(zip(workmodel.GPU_EVENTS, [int(x) for x in out])), "rays": counts}))
+"""
+
+
+def gpu_work_counts(args):
+    """Work events of one render of the workload, counted by the library's
+    CGS_WORK_COUNTS=1 build (cgspline_amd/libcgs_counts.so) in a child
+    process (the counting build is never the one timed), or None."""
+    import subprocess
+    from cgspline_amd import _build
+    if not os.path.exists(_build.COUNTING_LIB):
+        return None
+    env = dict(os.environ, CGS_LIBRARY=_build.COUNTING_LIB, CGS_ROOT=ROOT)
+    a = {"workload": args.workload, "scene_file": args.scene_file, "width": args.width, "height": args.height,
+         "depth": args.depth}
+    r = subprocess.run([sys.executable, "-c", _COUNT_CHILD, json.dumps(a)], env=env, stdout=subprocess.PIPE,
+                       stderr=subprocess.PIPE, text=True, timeout=600)
+    if r.returncode != 0:
+        log(f"work count child failed ({r.returncode}): {r.stderr[-400:]}")
+        return None
+    return json.loads(r.stdout.strip().splitlines()[-1])
 
 
 def cpu_model():
@@ -295,8 +337,20 @@ def main():
         oracle_rays = rc["primary"] + rc["shadow"] + rc["secondary"]
         if oracle_rays != int(round(rays_per_step)):
             log(f"WARNING: oracle ray count {oracle_rays} != GPU {rays_per_step}")
-        nbytes = workmodel.bytes_of(work, cam.width * cam.height, work["objects"])
-        nflops = workmodel.flops_of(work)
+        ref_flops = workmodel.flops_of(work)
+        t0 = time.time()
+        gc = gpu_work_counts(args)
+        log(f"library work count ({time.time() - t0:.1f}s): {gc}")
+        if gc is not None:
+            events = gc["events"]
+            nbytes = workmodel.bytes_of_counts(events, cam.width * cam.height)
+            nflops = workmodel.flops_of_counts(events)
+            source = "the library's own event counts (its CGS_WORK_COUNTS build, one render of this frame)"
+        else:
+            events = None
+            nbytes = workmodel.bytes_of(work, cam.width * cam.height, work["objects"])
+            nflops = ref_flops
+            source = "the oracle's event counts of the reference algorithm"
         tflops = nflops / avg_kernel_s / 1e12
         traffic = measured_traffic(args.workload)
         roof = {
@@ -311,10 +365,16 @@ def main():
             "algorithmic_flops_per_launch": int(nflops),
             "peak_nonpacked": workmodel.FP32_PEAK_NONPACKED_TFLOPS,
             "frac_nonpacked": round(tflops / workmodel.FP32_PEAK_NONPACKED_TFLOPS, 4),
-            "note": "FP32 VALU bound: algorithmic FP32 operations of the launch (oracle event counts x "
-                    "workmodel.py) over the kernel time measured with HIP events; 157.3 TFLOP/s assumes "
-                    "packed FP32, which this build disables, so 78.6 is the attainable peak",
+            "reference_algorithm_flops_per_launch": int(ref_flops),
+            "reference_equivalent_tflops": round(ref_flops / avg_kernel_s / 1e12, 3),
+            "note": "FP32 VALU bound: FP32 operations of the launch (" + source + " x the per-event costs of "
+                    "workmodel.py) over the kernel time measured with HIP events; 157.3 TFLOP/s assumes packed "
+                    "FP32, which this build disables, so 78.6 is the attainable peak.  The reference algorithm "
+                    "on the same rays (its top-level BVH walked node by node, which the cull tree replaces) "
+                    "would take reference_algorithm_flops_per_launch.",
         }
+        if events is not None:
+            roof["events"] = events
         if traffic is not None:
             roof["hbm"] = {"achieved": round(traffic / avg_kernel_s / 1e9, 2), "peak": workmodel.HBM_PEAK_GBS,
                            "unit": "GB/s", "frac": round(traffic / avg_kernel_s / 1e9 / workmodel.HBM_PEAK_GBS, 4),

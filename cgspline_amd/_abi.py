@@ -35,7 +35,7 @@ EXPORTS = [
     "cgs_phf_displacement_size", "cgs_phf_build", "cgs_phf_build_static", "cgs_phf_fill_table", "cgs_phf_hash",
     "cgs_scene_create", "cgs_scene_destroy", "cgs_scene_bvh_info", "cgs_scene_bvh_copy", "cgs_scene_culled",
     "cgs_trace_closest", "cgs_trace_any", "cgs_trace_closest_ex", "cgs_trace_any_ex", "cgs_render",
-    "cgs_render_running",
+    "cgs_render_running", "cgs_work_counts",
     "cgs_trace_batch", "cgs_batch_create", "cgs_batch_launch", "cgs_batch_destroy", "cgs_batch_set_config",
     "cgs_batch_get_config", "cgs_batch_autotune",
     "cgs_server_create", "cgs_server_submit", "cgs_server_poll", "cgs_server_wait", "cgs_server_destroy",
@@ -162,6 +162,7 @@ def lib():
             "cgs_render": ([_P, _P, C.POINTER(Camera), C.POINTER(RenderOptions), _U32, _U32, _P, _P, _P, _P],
                            C.c_int),
             "cgs_render_running": ([_P], C.c_int),
+            "cgs_work_counts": ([_P, _P, _U32, C.c_int], C.c_int),
             "cgs_trace_batch": ([_P, C.POINTER(BatchJob), _U32, _P], C.c_int),
             "cgs_batch_create": ([_P, C.POINTER(BatchJob), _U32, C.POINTER(_P)], C.c_int),
             "cgs_batch_launch": ([_P, _P], C.c_int),

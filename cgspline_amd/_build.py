@@ -61,6 +61,16 @@ def build(verbose: bool = False, jobs: int = 3, defines=(), lib_path: str = LIB)
     return lib_path
 
 
+COUNTING_LIB = os.path.join(HERE, "libcgs_counts.so")
+
+
+def build_counting(verbose: bool = False) -> str:
+    """The work-counting variant (CGS_WORK_COUNTS=1): the same kernels, each
+    counting its work events (cgs_work_counts), for the work model's counts
+    of a launch.  Never the one timed: counting slows the kernels."""
+    return build(verbose, defines=("CGS_WORK_COUNTS=1",), lib_path=COUNTING_LIB)
+
+
 def build_facade_check(verbose: bool = False) -> str:
     """Compiles tests/cpp/facade_check (the C++ facade driven like the
     reference's callers) with g++ against the in-tree libcgs.so."""

@@ -25,6 +25,40 @@ namespace cgs
 {
 
 constexpr uint32_t EMPTY = 0xFFFFFFFFu;
+
+// Event counts of a launch, for the work model: a build with
+// CGS_WORK_COUNTS=1 counts each event in a per-workgroup LDS slot, and the
+// kernels that call workBegin/workEnd add the slots to a device total.  In
+// the default build every CGS_COUNT is empty.
+#ifndef CGS_WORK_COUNTS
+#define CGS_WORK_COUNTS 0
+#endif
+enum WorkEvent : int
+{
+    kWorkCullNode,    // cull-tree node box tested
+    kWorkCullElem,    // cull-tree element (object leaf box) tested
+    kWorkNode,        // BVH inner node visited (two child boxes tested)
+    kWorkProbe,       // perfect-hash lookup on an ascent
+    kWorkLeaf,        // leaf element visited
+    kWorkPatch,       // patch test (projection)
+    kWorkClip,        // clipping pass that clipped the region
+    kWorkSplit,       // clipping pass that split it
+    kWorkPrune,       // clipping pass whose hull missed
+    kWorkConverged,   // region under tolerance
+    kWorkOnHit,       // candidate evaluated
+    kWorkObject,      // object-local ray set up
+    kWorkEvents
+};
+#if CGS_WORK_COUNTS
+CGS_DEV unsigned long long* workSlots()
+{
+    __shared__ unsigned long long slots[kWorkEvents];
+    return slots;
+}
+#define CGS_COUNT(e) atomicAdd(::cgs::workSlots() + (e), 1ull)
+#else
+#define CGS_COUNT(e) ((void)0)
+#endif
 constexpr float FLT_EPS = 1.1920928955078125e-07f; // 2^-23
 
 struct V2 { float x, y; };
@@ -300,6 +334,7 @@ template<bool ANY, bool CAPPED = false>
 CGS_DEV bool clipPatch(const float4* __restrict__ S, const ClipRay& R, float tol, float& best, float& bu, float& bv,
                        const ClipStack& stk, bool& overflow, uint32_t maxIters = 0)
 {
+    CGS_COUNT(kWorkPatch);
     V2 p[16];
 #pragma unroll
     for (int i = 0; i < 16; i++)
@@ -315,6 +350,7 @@ CGS_DEV bool clipPatch(const float4* __restrict__ S, const ClipRay& R, float tol
 
     auto onHit = [&](float eu, float ev) -> bool
     {
+        CGS_COUNT(kWorkOnHit);
         float4 X = interpolateMat(S, eu, ev);
         V3 V = sub3(project4(X), R.o);
         float t = len3(V);
@@ -348,6 +384,7 @@ CGS_DEV bool clipPatch(const float4* __restrict__ S, const ClipRay& R, float tol
         float d1 = __builtin_sqrtf(t1.x * t1.x + t1.y * t1.y);
         if (d0 < tol && d1 < tol)
         {
+            CGS_COUNT(kWorkConverged);
             // approximateIntersection (Spline.h:739-765) on the natural corners
             const V2 c30 = vside ? p[12] : p[3], c03 = vside ? p[3] : p[12];
             V2 c;
@@ -416,6 +453,7 @@ CGS_DEV bool clipPatch(const float4* __restrict__ S, const ClipRay& R, float tol
             const float delta = upper - lower;
             if (delta < 0.0f)
             {
+                CGS_COUNT(kWorkPrune);
                 pop = true;
             }
             else
@@ -424,6 +462,7 @@ CGS_DEV bool clipPatch(const float4* __restrict__ S, const ClipRay& R, float tol
                 // lower half (Spline_BezierClipping.cu:373-409); otherwise clip
                 // to [lower, upper] (:410-432).  Both keep a slice of the rows.
                 const bool split = delta > 0.8f;
+                CGS_COUNT(split ? kWorkSplit : kWorkClip);
                 if (split && sp >= CGS_CLIP_STACK)
                 {
                     overflow = true;
@@ -865,6 +904,7 @@ CGS_DEV bool ascend(const BVHRef& b, const NodeCache& nc, const SiblingRing& rin
     st.postponed ^= 1;
     if (st.rs.pop(ring, st.node)) return true;
     if (st.key < nc.keys) return true;    // staged in LDS: read by key, no lookup
+    CGS_COUNT(kWorkProbe);
     st.node = b.table[hashKey(b, st.key)];
     return st.node < b.nodeCount;
 }
@@ -884,6 +924,7 @@ CGS_DEV uint32_t nextLeaf(const BVHRef& b, const NodeCache& nc, const SiblingRin
     {
         const NodeBoxes n = loadNodeCached<LOW>(b, nc, st.node, st.key);
         if (n.left == EMPTY) return n.right;
+        CGS_COUNT(kWorkNode);
         float L0, R0;
         const bool iL = nodeIntersect<OCT>(L0, n.lmin, n.lmax, o, Dinv) && L0 < limit;
         const bool iR = nodeIntersect<OCT>(R0, n.rmin, n.rmax, o, Dinv) && R0 < limit;
@@ -960,6 +1001,7 @@ CGS_DEV bool hashIntersect(const BVHRef& b, V3 o, V3 dir, const float& limit, Le
         uint32_t e;
         while ((e = b.indices[i]) != EMPTY)
         {
+            CGS_COUNT(kWorkLeaf);
             result |= leaf(e);
             i++;
         }
@@ -983,6 +1025,7 @@ CGS_DEV bool hashIntersectAny(const BVHRef& b, V3 o, V3 dir, float tmax, Leaf&& 
         uint32_t e;
         while ((e = b.indices[i]) != EMPTY)
         {
+            CGS_COUNT(kWorkLeaf);
             if (leaf(e)) return true;
             i++;
         }

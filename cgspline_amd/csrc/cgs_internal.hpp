@@ -148,6 +148,9 @@ hipError_t launchRender(const SceneDev& sc, const PersistArgs& pa, const float* 
                         float tly, float hdx, float hdy, float eps, int depth, uint32_t w, uint32_t h, uint32_t* frame,
                         float* color, unsigned long long* counts, uint32_t stride, const uint32_t* order,
                         uint32_t* tileTicks, void* argsDev, hipStream_t s);
+// The render's work-event totals (a CGS_WORK_COUNTS=1 build; zeros
+// otherwise), kWorkEvents of them (cgs_device.hpp); cleared when `reset`.
+hipError_t workCounts(unsigned long long* out, bool reset);
 // order[0..n) = the tiles 0..n-1, longest `ticks` first.
 hipError_t launchTileOrder(const uint32_t* ticks, uint32_t n, uint32_t* order, hipStream_t s);
 

@@ -1187,6 +1187,18 @@ cgs_status cgs_render(cgs_ctx* ctx, cgs_scene* sc, const cgs_camera* cam, const 
     return persistEnd(ctx, s);
 }
 
+cgs_status cgs_work_counts(cgs_ctx* ctx, uint64_t* out, uint32_t n, int reset)
+{
+    if (!ctx || (n && !out)) return fail(CGS_ERR_INVALID, "cgs_work_counts: null argument");
+    if (!CGS_WORK_COUNTS) return fail(CGS_ERR_INVALID, "cgs_work_counts: this build does not count work events");
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipDeviceSynchronize());
+    unsigned long long all[kWorkEvents];
+    HIP_TRY(workCounts(all, reset != 0));
+    for (uint32_t k = 0; k < n && k < (uint32_t)kWorkEvents; k++) out[k] = all[k];
+    return CGS_OK;
+}
+
 int cgs_render_running(cgs_ctx* ctx)
 {
     if (!ctx || !ctx->renderLaunched) return 0;

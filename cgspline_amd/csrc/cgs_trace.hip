@@ -284,6 +284,7 @@ CGS_DEV bool objectClosest(const SceneDev& sc, const Rings& rg, uint32_t i, V3 o
 {
     const DevObject& ob = sc.objects[i];
     if (ob.surface == EMPTY) return false;   // no geometry (empty box: only a NaN ray gets here)
+    CGS_COUNT(kWorkObject);
     LocalRay lr = toLocal(ob, o, d, tmax);
     const DevBVH surf = sc.surfaces[ob.surface];
     const BVHRef b = bvhRef(sc, surf);
@@ -317,6 +318,7 @@ CGS_DEV bool objectAny(const SceneDev& sc, const Rings& rg, uint32_t i, V3 o, V3
 {
     const DevObject& ob = sc.objects[i];
     if (ob.surface == EMPTY) return false;   // no geometry (empty box: only a NaN ray gets here)
+    CGS_COUNT(kWorkObject);
     LocalRay lr = toLocal(ob, o, d, tmax);
     const DevBVH surf = sc.surfaces[ob.surface];
     const BVHRef b = bvhRef(sc, surf);
@@ -374,6 +376,7 @@ CGS_DEV uint32_t cullNext(const SceneDev& sc, int oct, V3 o, V3 Dinv, float lim,
     {
         while (it.elem < it.elemEnd)
         {
+            CGS_COUNT(kWorkCullElem);
             const float4 a = sc.cullElems[2 * (size_t)it.elem], b = sc.cullElems[2 * (size_t)it.elem + 1];
             it.elem++;
             float l;
@@ -384,6 +387,7 @@ CGS_DEV uint32_t cullNext(const SceneDev& sc, int oct, V3 o, V3 Dinv, float lim,
             }
         }
         if (it.node == EMPTY) return EMPTY;
+        CGS_COUNT(kWorkCullNode);
         const float4 a = sc.cullNodes[2 * (size_t)it.node], b = sc.cullNodes[2 * (size_t)it.node + 1];
         const uint2 l = sc.cullLinks[8 * (size_t)it.node + oct];
         float tn;
@@ -509,6 +513,25 @@ CGS_DEV bool sceneAny(const SceneDev& sc, const NodeCache& nc, const Rings& rg, 
     return hashIntersectAny<true, true>(top, o, d, tmax, [&](uint32_t i) -> bool {
         return objectAny<METHOD>(sc, rg, i, o, d, tmax, stk, overflow, tp);
     }, nc, rg.top);
+}
+
+// Device totals of the work events (CGS_WORK_COUNTS builds).
+__device__ unsigned long long g_workCounts[kWorkEvents];
+
+// Clears the workgroup's event slots (the caller then passes a barrier
+// before any event) / adds them to g_workCounts at the end of a kernel.
+CGS_DEV void workBegin()
+{
+#if CGS_WORK_COUNTS
+    if (threadIdx.x < kWorkEvents) workSlots()[threadIdx.x] = 0;
+#endif
+}
+CGS_DEV void workEnd()
+{
+#if CGS_WORK_COUNTS
+    __syncthreads();
+    if (threadIdx.x < kWorkEvents) atomicAdd(g_workCounts + threadIdx.x, workSlots()[threadIdx.x]);
+#endif
 }
 
 #ifndef CGS_TOP_LDS_KEYS
@@ -844,6 +867,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CGS_RENDER_
 {
     __shared__ float4 topLds[4 * CGS_TOP_LDS_KEYS];
     __shared__ uint32_t ringLds[kRingWords];
+    workBegin();
     const NodeCache nc = stageTopLevel(sc, topLds);
     const Rings rg = laneRings(ringLds);
     const uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x;
@@ -966,6 +990,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CGS_RENDER_
         addWave(counts + 2, nSecondary);
         addWave(counts + 3, nOverflow);
     }
+    workEnd();
 }
 
 } // namespace cgs
@@ -1136,6 +1161,19 @@ hipError_t launchTileOrder(const uint32_t* ticks, uint32_t n, uint32_t* order, h
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(kTileOrder, dim3(1), dim3(1024), 0, s, ticks, n, order);
     return hipGetLastError();
+}
+
+// The work-event totals of the CGS_WORK_COUNTS build (zeros otherwise):
+// copies kWorkEvents totals to `out` and clears them when `reset`.
+hipError_t workCounts(unsigned long long* out, bool reset)
+{
+    hipError_t e = hipMemcpyFromSymbol(out, HIP_SYMBOL(g_workCounts), sizeof(g_workCounts));
+    if (e == hipSuccess && reset)
+    {
+        const unsigned long long zero[kWorkEvents] = {};
+        e = hipMemcpyToSymbol(HIP_SYMBOL(g_workCounts), zero, sizeof zero);
+    }
+    return e;
 }
 
 } // namespace cgs

@@ -47,6 +47,35 @@ FP32_PEAK_TFLOPS = 157.3
 FP32_PEAK_NONPACKED_TFLOPS = 78.6
 
 
+# The events a launch of the library counts (cgs_work_counts, a
+# CGS_WORK_COUNTS=1 build), in the ABI's order.
+GPU_EVENTS = ("cullNodes", "cullElems", "nodes", "hashProbes", "leaves", "patchTests", "clips", "splits", "prunes",
+              "converged", "onHits", "objects")
+
+# Per event of the library's own traversal: a cull-tree node or element box
+# test is one slab test (3 axes × (2 sub + 2 mul + 2 select + 2 compare) + 3
+# compares = 27 operations) reading 32 B (+ 8 B of links for a node); an
+# object-local ray costs the world→local transform (28), its projection (3),
+# the direction's transform and normalisation (15 + 10) and the clipping
+# frame (2 cross products and versors, ≈ 40): ≈ 100.
+F_CULL, F_OBJECT = 27, 100
+B_CULL_NODE, B_CULL_ELEM = 40, 32
+
+
+def flops_of_counts(c: dict) -> float:
+    """FP32 operations of a launch from the library's own event counts."""
+    return (F_CULL * (c["cullNodes"] + c["cullElems"]) + F_NODE * c["nodes"] + F_PROJ * c["patchTests"]
+            + F_CLIP * c["clips"] + F_SUBDIV * c["splits"] + F_PRUNE * c["prunes"]
+            + F_CONVERGED * c["converged"] + F_ONHIT * c["onHits"] + F_OBJECT * c["objects"])
+
+
+def bytes_of_counts(c: dict, pixels: int, color: bool = False) -> float:
+    """Algorithmic bytes of a launch from the library's own event counts."""
+    return (B_CULL_NODE * c["cullNodes"] + B_CULL_ELEM * c["cullElems"] + B_NODE * c["nodes"]
+            + B_PROBE * c["hashProbes"] + B_LEAF * c["leaves"] + B_PATCH * c["patchTests"]
+            + B_OBJECT * c["objects"] + pixels * (B_PIXEL + (B_COLOR if color else 0)))
+
+
 def bytes_of(work: dict, pixels: int, objects_visited: int = 0, color: bool = False) -> float:
     """Algorithmic bytes of a render/trace launch from oracle work counts."""
     return (B_NODE * work["nodes"] + B_PROBE * work["hashProbes"] + B_LEAF * work["leaves"]
@@ -55,7 +84,8 @@ def bytes_of(work: dict, pixels: int, objects_visited: int = 0, color: bool = Fa
 
 
 def flops_of(work: dict) -> float:
-    """Algorithmic FP32 operations of a launch from oracle work counts."""
+    """FP32 operations of the reference algorithm on a launch's rays, from
+    the oracle's work counts (the reference's traversal at both levels)."""
     plain_clips = work["clipIters"] - work["subdivs"] - work["prunes"]
     return (F_NODE * work["nodes"] + F_PROJ * work["patchTests"] + F_CLIP * plain_clips
             + F_SUBDIV * work["subdivs"] + F_PRUNE * work["prunes"] + F_CONVERGED * work["converged"]

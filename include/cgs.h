@@ -403,6 +403,15 @@ cgs_status cgs_render(cgs_ctx* ctx, cgs_scene* scene, const cgs_camera* camera,
 /* RayTracer::running() (rt/RayTracer.h:45-48): 1 while the last render runs. */
 int cgs_render_running(cgs_ctx* ctx);
 
+/* Work-event totals of the renders on `ctx`'s device since the last reset,
+ * for the work model (no counterpart in the reference): cull-tree node and
+ * element boxes tested, BVH inner nodes visited, perfect-hash probes, leaf
+ * elements visited, patch tests, clipping passes that clipped / split /
+ * pruned, converged regions, candidates evaluated, object-local rays (12
+ * values, in that order, into out[0..n)).  Only a library built with
+ * CGS_WORK_COUNTS=1 counts (CGS_ERR_INVALID from any other build). */
+cgs_status cgs_work_counts(cgs_ctx* ctx, uint64_t* out, uint32_t n, int reset);
+
 /* ---------------------------------------------------------------------- */
 /* Batched queries over many resident scenes (no counterpart in the        */
 /* reference: each job is Context::intersect over a ray range,             */
