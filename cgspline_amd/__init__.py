@@ -16,7 +16,7 @@ from . import _abi, scenes
 from ._abi import CgsError, EMPTY, PATCH_OVERFLOW, check, lib, ptr
 
 __all__ = ["Batch", "CLIPPING", "Context", "DeviceArray", "DeviceScene", "CgsError", "DlScene", "EMPTY", "PATCH_OVERFLOW",
-           "SUBDIVISION", "load_be", "read_dl", "scenes", "lib", "trace_options"]
+           "PatchBVH", "SUBDIVISION", "load_be", "read_dl", "scenes", "lib", "trace_options"]
 
 
 def load_be(source) -> tuple[np.ndarray, np.ndarray]:
@@ -188,6 +188,36 @@ class Context:
         check(lib().cgs_stream_sync(self.h, None))
         return dh.to_host(np.float32, (n, 4))
 
+    def patch_eval(self, kind: int, patches: np.ndarray, uv: np.ndarray) -> np.ndarray:
+        """cgs_patch_eval: patch r (16 × 4 floats) at uv[r] by `kind`
+        (CGS_EVAL_*: interpolate, derivatives, normal; de Casteljau or matrix
+        form); returns (n, 4) float32."""
+        patches = np.ascontiguousarray(patches, np.float32).reshape(-1, 64)
+        uv = np.ascontiguousarray(uv, np.float32).reshape(-1, 2)
+        n = patches.shape[0]
+        assert uv.shape[0] == n
+        dp, du = DeviceArray.from_host(self, patches), DeviceArray.from_host(self, uv)
+        do = DeviceArray(self, n * 16)
+        check(lib().cgs_patch_eval(self.h, int(kind), dp.dptr, du.dptr, n, do.dptr, None), "cgs_patch_eval")
+        self.sync()
+        return do.to_host(np.float32, (n, 4))
+
+    def subpatch(self, patches: np.ndarray, ranges: np.ndarray) -> np.ndarray:
+        """cgs_patch_subpatch: patch r cut to ranges[r] = (umin, umax, vmin,
+        vmax); returns (n, 16, 4) float32."""
+        patches = np.ascontiguousarray(patches, np.float32).reshape(-1, 64)
+        ranges = np.ascontiguousarray(ranges, np.float32).reshape(-1, 4)
+        n = patches.shape[0]
+        assert ranges.shape[0] == n
+        dp, dr = DeviceArray.from_host(self, patches), DeviceArray.from_host(self, ranges)
+        do = DeviceArray(self, n * 256)
+        check(lib().cgs_patch_subpatch(self.h, dp.dptr, dr.dptr, n, do.dptr, None), "cgs_patch_subpatch")
+        self.sync()
+        return do.to_host(np.float32, (n, 16, 4))
+
+    def patch_bvh(self, vertices: np.ndarray, indices: np.ndarray) -> "PatchBVH":
+        return PatchBVH(self, vertices, indices)
+
     # --- BVH / hash -------------------------------------------------------------
     @staticmethod
     def patch_elements(vertices: np.ndarray, indices: np.ndarray) -> np.ndarray:
@@ -324,6 +354,49 @@ class Batch:
         if self.h is not None and self.ctx.h is not None:
             lib().cgs_batch_destroy(self.h)
         self.h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+class PatchBVH:
+    """cg::PatchBVH over one surface's Bézier patches (cgs_patchbvh_create):
+    closest and any hits of rays in the surface's frame."""
+
+    def __init__(self, ctx: Context, vertices: np.ndarray, indices: np.ndarray):
+        v = np.ascontiguousarray(vertices, np.float32).reshape(-1, 4)
+        i = np.ascontiguousarray(indices, np.uint32).reshape(-1, 16)
+        p = C.c_void_p()
+        check(lib().cgs_patchbvh_create(ctx.h, v.ctypes.data, v.shape[0], i.ctypes.data, i.shape[0], C.byref(p)),
+              "cgs_patchbvh_create")
+        self.ctx, self.h = ctx, p.value
+
+    def intersect(self, rays: np.ndarray):
+        """Closest hits of host rays (n, 8): (t,u,v (n,3), patch (n,))."""
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        n = rays.shape[0]
+        dr, dh = DeviceArray.from_host(self.ctx, rays), DeviceArray(self.ctx, 16 * n)
+        check(lib().cgs_patchbvh_intersect(self.ctx.h, self.h, dr.dptr, n, dh.dptr, None), "cgs_patchbvh_intersect")
+        self.ctx.sync()
+        h = dh.to_host(np.float32, (n, 4))
+        return h[:, :3].copy(), h.view(np.uint32)[:, 3].copy()
+
+    def intersect_any(self, rays: np.ndarray) -> np.ndarray:
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        n = rays.shape[0]
+        dr, do = DeviceArray.from_host(self.ctx, rays), DeviceArray(self.ctx, 4 * n)
+        check(lib().cgs_patchbvh_intersect_any(self.ctx.h, self.h, dr.dptr, n, do.dptr, None),
+              "cgs_patchbvh_intersect_any")
+        self.ctx.sync()
+        return do.to_host(np.uint32, (n,))
+
+    def destroy(self):
+        if self.h is not None:
+            lib().cgs_patchbvh_destroy(self.h)
+            self.h = None
 
     def __del__(self):
         try:

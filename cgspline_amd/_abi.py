@@ -30,7 +30,8 @@ PATCH_OVERFLOW = 0xFFFFFFFE
 # Every symbol include/cgs.h declares (checked by tests/test_abi.py).
 EXPORTS = [
     "cgs_abi_version", "cgs_status_string", "cgs_last_error", "cgs_open", "cgs_close",
-    "cgs_bezier_clip", "cgs_subdivision", "cgs_bvh_build", "cgs_patch_elements", "cgs_mesh_elements", "cgs_free_host", "cgs_load_be",
+    "cgs_bezier_clip", "cgs_subdivision", "cgs_patch_eval", "cgs_patch_subpatch",
+    "cgs_patchbvh_create", "cgs_patchbvh_destroy", "cgs_patchbvh_intersect", "cgs_patchbvh_intersect_any", "cgs_bvh_build", "cgs_patch_elements", "cgs_mesh_elements", "cgs_free_host", "cgs_load_be",
     "cgs_dl_read", "cgs_dl_parse", "cgs_dl_free",
     "cgs_phf_displacement_size", "cgs_phf_build", "cgs_phf_build_static", "cgs_phf_fill_table", "cgs_phf_hash",
     "cgs_scene_create", "cgs_scene_destroy", "cgs_scene_bvh_info", "cgs_scene_bvh_copy", "cgs_scene_culled",
@@ -134,6 +135,12 @@ def lib():
             "cgs_close": ([_P], C.c_int),
             "cgs_bezier_clip": ([_P, _P, _P, _U32, _F32, _P, _P], C.c_int),
             "cgs_subdivision": ([_P, _P, _P, _U32, _F32, _P, _P], C.c_int),
+            "cgs_patch_eval": ([_P, _U32, _P, _P, _U32, _P, _P], C.c_int),
+            "cgs_patch_subpatch": ([_P, _P, _P, _U32, _P, _P], C.c_int),
+            "cgs_patchbvh_create": ([_P, _P, _U32, _P, _U32, C.POINTER(_P)], C.c_int),
+            "cgs_patchbvh_destroy": ([_P], C.c_int),
+            "cgs_patchbvh_intersect": ([_P, _P, _P, _U32, _P, _P], C.c_int),
+            "cgs_patchbvh_intersect_any": ([_P, _P, _P, _U32, _P, _P], C.c_int),
             "cgs_bvh_build": ([_P, _U32, _U32, C.POINTER(_P), C.POINTER(_P), C.POINTER(_U32), C.POINTER(_P),
                                C.POINTER(_U32)], C.c_int),
             "cgs_patch_elements": ([_P, _P, _U32, _P], C.c_int),

@@ -103,8 +103,9 @@ enum class PersistKernel
     ClipBatch,
     Render,
     Batch,
+    Pick,
 };
-constexpr int kPersistKernels = 5;
+constexpr int kPersistKernels = 6;
 
 struct PersistArgs
 {
@@ -139,6 +140,12 @@ hipError_t launchClipBatch(const PersistArgs& pa, const cgs_ray* rays, const flo
                            cgs_hit* hits, hipStream_t s);
 hipError_t launchSubdivBatch(const cgs_ray* rays, const float* patches, uint32_t n, float threshold, cgs_hit* hits,
                              hipStream_t s);
+hipError_t launchPatchEval(uint32_t kind, const float* patches, const float* uv, uint32_t n, float* out, hipStream_t s);
+hipError_t launchSubpatch(const float* patches, const float* ranges, uint32_t n, float* out, hipStream_t s);
+// PatchBVH queries against surface 0 of `sc`: closest hits into `hits`, or
+// (occluded non-null) any hits into `occluded`.
+hipError_t launchPick(const SceneDev& sc, const PersistArgs& pa, const cgs_ray* rays, uint32_t n, cgs_hit* hits,
+                      uint32_t* occluded, hipStream_t s);
 // kRender reads its camera, options and output pointers from `argsDev`, a
 // device block of kRenderArgsBytes that launchRender fills on stream `s`.
 constexpr size_t kRenderArgsBytes = 256;

@@ -622,6 +622,25 @@ cgs_status cgs_subdivision(cgs_ctx* ctx, const cgs_ray* rays, const float* patch
     return CGS_OK;
 }
 
+cgs_status cgs_patch_eval(cgs_ctx* ctx, uint32_t kind, const float* patches, const float* uv, uint32_t n, float* out,
+                          void* stream)
+{
+    if (!ctx || (n && (!patches || !uv || !out))) return fail(CGS_ERR_INVALID, "cgs_patch_eval: null argument");
+    if (kind > CGS_EVAL_NORMAL_MAT) return fail(CGS_ERR_INVALID, "cgs_patch_eval: unknown kind");
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(launchPatchEval(kind, patches, uv, n, out, (hipStream_t)stream));
+    return CGS_OK;
+}
+
+cgs_status cgs_patch_subpatch(cgs_ctx* ctx, const float* patches, const float* ranges, uint32_t n, float* out,
+                              void* stream)
+{
+    if (!ctx || (n && (!patches || !ranges || !out))) return fail(CGS_ERR_INVALID, "cgs_patch_subpatch: null argument");
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(launchSubpatch(patches, ranges, n, out, (hipStream_t)stream));
+    return CGS_OK;
+}
+
 cgs_status cgs_bvh_build(const cgs_bvh_element* elements, uint32_t n, uint32_t epn, cgs_bvh_node** nodes,
                          uint32_t** keys, uint32_t* node_count, uint32_t** indices, uint32_t* index_count)
 {
@@ -1045,6 +1064,83 @@ cgs_status cgs_scene_bvh_info(cgs_scene* sc, int32_t surface, uint32_t* node_cou
 }
 
 int cgs_scene_culled(cgs_scene* sc) { return sc ? (sc->cull ? 1 : 0) : -1; }
+
+} // extern "C"
+
+// A PatchBVH is a scene of its one surface, as one object in its own frame:
+// the surface's BVH, hash and gathered patches are what the pick kernel reads.
+struct cgs_patchbvh
+{
+    cgs_scene* scene = nullptr;
+    int device = 0;
+};
+
+extern "C" {
+
+cgs_status cgs_patchbvh_create(cgs_ctx* ctx, const float* vertices, uint32_t nv, const uint32_t* indices, uint32_t np,
+                               cgs_patchbvh** out)
+{
+    if (!ctx || !out || !vertices || !indices || nv == 0 || np == 0)
+        return fail(CGS_ERR_INVALID, "cgs_patchbvh_create: empty or null surface");
+    static const float I[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+    const uint32_t vbase[2] = {0, nv}, pbase[2] = {0, np}, surf = 0;
+    const float mat[8] = {};
+    cgs_scene_desc d {};
+    d.vertices = vertices;
+    d.indices = indices;
+    d.surf_vbase = vbase;
+    d.surf_pbase = pbase;
+    d.nsurf = 1;
+    d.obj_surface = &surf;
+    d.obj_local2world = I;
+    d.obj_world2local = I;
+    d.obj_material = mat;
+    d.nobj = 1;
+    d.hash_seed = 42;
+    auto pb = std::make_unique<cgs_patchbvh>();
+    cgs_status st = cgs_scene_create(ctx, &d, &pb->scene);
+    if (st != CGS_OK) return st;
+    pb->device = ctx->device;
+    *out = pb.release();
+    return CGS_OK;
+}
+
+cgs_status cgs_patchbvh_destroy(cgs_patchbvh* pb)
+{
+    if (!pb) return CGS_OK;
+    cgs_status st = cgs_scene_destroy(pb->scene);
+    delete pb;
+    return st;
+}
+
+static cgs_status patchbvhQuery(cgs_ctx* ctx, cgs_patchbvh* pb, const cgs_ray* rays, uint32_t n, cgs_hit* hits,
+                                uint32_t* occluded, void* stream, const char* what)
+{
+    if (!ctx || !pb || (n && (!rays || (!hits && !occluded)))) return fail(CGS_ERR_INVALID, std::string(what) + ": null argument");
+    if (pb->device != ctx->device) return fail(CGS_ERR_INVALID, std::string(what) + ": PatchBVH on another device");
+    HIP_TRY(hipSetDevice(ctx->device));
+    std::lock_guard lk(ctx->mu);
+    if (n == 0) return CGS_OK;
+    PersistArgs pa;
+    cgs_status st = persistArgs(ctx, PersistKernel::Pick, (hipStream_t)stream, n, pa);
+    if (st != CGS_OK) return st;
+    HIP_TRY(launchPick(pb->scene->view(), pa, rays, n, hits, occluded, (hipStream_t)stream));
+    return persistEnd(ctx, (hipStream_t)stream);
+}
+
+cgs_status cgs_patchbvh_intersect(cgs_ctx* ctx, cgs_patchbvh* pb, const cgs_ray* rays, uint32_t n, cgs_hit* hits,
+                                  void* stream)
+{
+    if (!hits && n) return fail(CGS_ERR_INVALID, "cgs_patchbvh_intersect: null hits");
+    return patchbvhQuery(ctx, pb, rays, n, hits, nullptr, stream, "cgs_patchbvh_intersect");
+}
+
+cgs_status cgs_patchbvh_intersect_any(cgs_ctx* ctx, cgs_patchbvh* pb, const cgs_ray* rays, uint32_t n,
+                                      uint32_t* occluded, void* stream)
+{
+    if (!occluded && n) return fail(CGS_ERR_INVALID, "cgs_patchbvh_intersect_any: null output");
+    return patchbvhQuery(ctx, pb, rays, n, nullptr, occluded, stream, "cgs_patchbvh_intersect_any");
+}
 
 cgs_status cgs_scene_bvh_copy(cgs_scene* sc, int32_t surface, cgs_bvh_node* nodes, uint32_t* keys, uint32_t* indices,
                               uint32_t* displacement, uint32_t* table)

@@ -749,6 +749,207 @@ __global__ __launch_bounds__(256) void kSubdivBatch(const cgs_ray* __restrict__ 
 }
 
 // ---------------------------------------------------------------------------
+// Patch evaluation (Spline.h:165-254 de Casteljau, Spline.cu:10-27 normal,
+// SplineMat.h:125-187 matrix form) and subpatch (Spline.h:384-396)
+// ---------------------------------------------------------------------------
+
+// deCasteljau (Spline.h:168-178) over 4 float4 points
+CGS_DEV float4 lerp4(float4 a, float4 b, float t, float u)
+{
+    return {t * a.x + u * b.x, t * a.y + u * b.y, t * a.z + u * b.z, t * a.w + u * b.w};
+}
+CGS_DEV float4 deCasteljau4(float4 (&p)[4], float u)
+{
+    const float t = 1.0f - u;
+#pragma unroll
+    for (int k = 1; k < 4; ++k)
+#pragma unroll
+        for (int i = 0; i < 4 - k; ++i) p[i] = lerp4(p[i], p[i + 1], t, u);
+    return p[0];
+}
+// deCasteljauDx (Spline.h:183-195): the hodograph without its factor 3
+CGS_DEV float4 deCasteljauDx4(float4 (&p)[4], float u)
+{
+    const float t = 1.0f - u;
+#pragma unroll
+    for (int i = 1; i < 4; ++i) p[i - 1] = {p[i].x - p[i - 1].x, p[i].y - p[i - 1].y, p[i].z - p[i - 1].z, p[i].w - p[i - 1].w};
+#pragma unroll
+    for (int k = 2; k < 4; ++k)
+#pragma unroll
+        for (int i = 0; i < 4 - k; ++i) p[i] = lerp4(p[i], p[i + 1], t, u);
+    return p[0];
+}
+// interpolate (Spline.h:197-214), derivativeU (:216-234), derivativeV (:236-254)
+CGS_DEV float4 interpolateDC(const float4* __restrict__ S, float u, float v)
+{
+    float4 q[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+    {
+        float4 p[4] = {S[4 * j], S[4 * j + 1], S[4 * j + 2], S[4 * j + 3]};
+        q[j] = deCasteljau4(p, u);
+    }
+    return deCasteljau4(q, v);
+}
+CGS_DEV float4 derivativeU_DC(const float4* __restrict__ S, float u, float v)
+{
+    float4 q[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+    {
+        float4 p[4] = {S[4 * j], S[4 * j + 1], S[4 * j + 2], S[4 * j + 3]};
+        q[j] = deCasteljauDx4(p, u);
+    }
+    return deCasteljau4(q, v);
+}
+CGS_DEV float4 derivativeV_DC(const float4* __restrict__ S, float u, float v)
+{
+    float4 p[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+    {
+        float4 q[4] = {S[i], S[4 + i], S[8 + i], S[12 + i]};
+        p[i] = deCasteljauDx4(q, v);
+    }
+    return deCasteljau4(p, u);
+}
+// normal (Spline.cu:10-27): a vanishing derivative (a degenerate edge) is
+// taken 1e-4 inside the patch instead
+CGS_DEV V3 normalDC(const float4* __restrict__ S, float u, float v)
+{
+    const float e = 0.0001f;
+    float4 dU = derivativeU_DC(S, u, v);
+    float4 dV = derivativeV_DC(S, u, v);
+    if (dU.x == 0.0f && dU.y == 0.0f && dU.z == 0.0f) dU = derivativeU_DC(S, u, __builtin_fabsf(v - e));
+    if (dV.x == 0.0f && dV.y == 0.0f && dV.z == 0.0f) dV = derivativeV_DC(S, __builtin_fabsf(u - e), v);
+    return cross3({dU.x, dU.y, dU.z}, {dV.x, dV.y, dV.z});
+}
+// mat::derivativeU (SplineMat.h:142-158) / mat::derivativeV (:160-176)
+CGS_DEV float4 derivativeUMat(const float4* __restrict__ S, float u, float v)
+{
+    float4 x = dbasisMul(evalVecDt(u));
+    float4 y = basisMul(evalVec(v));
+    float4 a0 = comb4(S[0], S[1], S[2], S[3], x);
+    float4 a1 = comb4(S[4], S[5], S[6], S[7], x);
+    float4 a2 = comb4(S[8], S[9], S[10], S[11], x);
+    float4 a3 = comb4(S[12], S[13], S[14], S[15], x);
+    return comb4(a0, a1, a2, a3, y);
+}
+CGS_DEV float4 derivativeVMat(const float4* __restrict__ S, float u, float v)
+{
+    float4 xb = basisMul(evalVec(u));
+    float4 yd = dbasisMul(evalVecDt(v));
+    float4 b0 = comb4(S[0], S[4], S[8], S[12], yd);
+    float4 b1 = comb4(S[1], S[5], S[9], S[13], yd);
+    float4 b2 = comb4(S[2], S[6], S[10], S[14], yd);
+    float4 b3 = comb4(S[3], S[7], S[11], S[15], yd);
+    return comb4(b0, b1, b2, b3, xb);
+}
+
+// One evaluation per item: patch r at (u, v) = uv[r], by `kind` (CGS_EVAL_*).
+__global__ __launch_bounds__(256) void kPatchEval(uint32_t kind, const float4* __restrict__ patches,
+                                                  const float2* __restrict__ uv, uint32_t n, float4* __restrict__ out)
+{
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const float4* S = patches + 16 * (size_t)r;
+    const float2 t = uv[r];
+    float4 o;
+    switch (kind)
+    {
+    case CGS_EVAL_INTERPOLATE: o = interpolateDC(S, t.x, t.y); break;
+    case CGS_EVAL_INTERPOLATE_MAT: o = interpolateMat(S, t.x, t.y); break;
+    case CGS_EVAL_DERIVATIVE_U: o = derivativeU_DC(S, t.x, t.y); break;
+    case CGS_EVAL_DERIVATIVE_V: o = derivativeV_DC(S, t.x, t.y); break;
+    case CGS_EVAL_DERIVATIVE_U_MAT: o = derivativeUMat(S, t.x, t.y); break;
+    case CGS_EVAL_DERIVATIVE_V_MAT: o = derivativeVMat(S, t.x, t.y); break;
+    case CGS_EVAL_NORMAL:
+    {
+        const V3 nv = normalDC(S, t.x, t.y);
+        o = {nv.x, nv.y, nv.z, 0.0f};
+        break;
+    }
+    default:
+    {
+        const V3 nv = normalMat(S, t.x, t.y);
+        o = {nv.x, nv.y, nv.z, 0.0f};
+        break;
+    }
+    }
+    out[r] = o;
+}
+
+// subpatch (Spline.h:384-396): subpatchV to [vmin, vmax], then subpatchU to
+// [umin, umax]; ranges[r] = (umin, umax, vmin, vmax).
+__global__ __launch_bounds__(256) void kSubpatch(const float4* __restrict__ patches, const float4* __restrict__ ranges,
+                                                 uint32_t n, float4* __restrict__ out)
+{
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    float4 p[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) p[i] = patches[16 * (size_t)r + i];
+    const float4 g = ranges[r];
+    subpatch4(p, true, g.z, g.w);
+    subpatch4(p, false, g.x, g.y);
+#pragma unroll
+    for (int i = 0; i < 16; i++) out[16 * (size_t)r + i] = p[i];
+}
+
+// ---------------------------------------------------------------------------
+// cg::PatchBVH (PatchBVH.h:13-28, PatchBVH.cpp:84-121): rays against the
+// patches of one surface, in the surface's own frame
+// ---------------------------------------------------------------------------
+//
+// The ray is a Ray3f: its direction normalized once.  Closest (intersect(ray,
+// hit)): hit.distance starts at ray.tmax and each leaf clips its patch
+// against it.  Any (intersect(ray)): a leaf clips with hit.distance = inf
+// (PatchBVH.cpp:93-94), so a patch whose box the traversal reaches within
+// ray.tmax counts with a hit at any distance in front.
+template<bool ANY>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CGS_TRACE_WAVES))) void kPick(SceneDev sc, Persist ps, const cgs_ray* __restrict__ rays, uint32_t n,
+                                                            cgs_hit* __restrict__ hits, uint32_t* __restrict__ occluded)
+{
+    __shared__ uint32_t ringLds[kRingWords];
+    const Rings rg = laneRings(ringLds);
+    const uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x;
+    const ClipStack stk = clipStackFor(ps.stack, gl, ps.lanes);
+    const DevBVH surf = sc.surfaces[0];
+    const BVHRef b = bvhRef(sc, surf);
+    for (uint32_t base = nextChunk(ps.counter); base < n; base = nextChunk(ps.counter))
+    {
+        const uint32_t r = base + (threadIdx.x & 63);
+        if (r >= n) continue;
+        const cgs_ray ray = rays[r];
+        const V3 o {ray.ox, ray.oy, ray.oz};
+        const V3 d = versor3({ray.dx, ray.dy, ray.dz});
+        const ClipRay cr = makeClipRay(o, d);
+        bool overflow = false;
+        if (ANY)
+        {
+            const bool f = hashIntersectAny<true>(b, o, d, ray.tmax, [&](uint32_t pid) -> bool {
+                float best = __builtin_inff(), bu, bv;
+                return clipPatch<true>(sc.patches + 16 * (size_t)(surf.patchBase + pid), cr, kClipTol, best, bu, bv, stk,
+                                       overflow);
+            }, NodeCache {nullptr, 0}, rg.bot);
+            occluded[r] = overflow ? 2u : (f ? 1u : 0u);
+        }
+        else
+        {
+            float t = ray.tmax, u = 0.0f, v = 0.0f;
+            uint32_t patch = EMPTY;
+            hashIntersect<true>(b, o, d, t, [&](uint32_t pid) -> bool {
+                const bool f = clipPatch<false>(sc.patches + 16 * (size_t)(surf.patchBase + pid), cr, kClipTol, t, u, v,
+                                                stk, overflow);
+                if (f) patch = pid;
+                return f;
+            }, NodeCache {nullptr, 0}, rg.bot);
+            hits[r] = {t, patch == EMPTY ? 0.0f : u, patch == EMPTY ? 0.0f : v, overflow ? CGS_PATCH_OVERFLOW : patch};
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Renderer (RayTracer.cu:194-544, non-Monte-Carlo path)
 // ---------------------------------------------------------------------------
 
@@ -1015,6 +1216,7 @@ hipError_t persistentGrid(PersistKernel k, uint32_t& blocksPerCU)
     case PersistKernel::ClipBatch: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kClipBatch, 256, 0); break;
     case PersistKernel::Render: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kRender, 256, 0); break;
     case PersistKernel::Batch: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kTraceBatch, 256, 0); break;
+    case PersistKernel::Pick: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kPick<false>, 256, 0); break;
     }
     blocksPerCU = n > 0 ? (uint32_t)n : 1u;
     return e;
@@ -1093,6 +1295,34 @@ hipError_t launchSubdivBatch(const cgs_ray* rays, const float* patches, uint32_t
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(kSubdivBatch, dim3(blocks(n, 256)), dim3(256), 0, s, rays, (const float4*)patches, n, threshold,
                        hits);
+    return hipGetLastError();
+}
+
+hipError_t launchPatchEval(uint32_t kind, const float* patches, const float* uv, uint32_t n, float* out, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(kPatchEval, dim3(blocks(n, 256)), dim3(256), 0, s, kind, (const float4*)patches,
+                       (const float2*)uv, n, (float4*)out);
+    return hipGetLastError();
+}
+
+hipError_t launchSubpatch(const float* patches, const float* ranges, uint32_t n, float* out, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(kSubpatch, dim3(blocks(n, 256)), dim3(256), 0, s, (const float4*)patches, (const float4*)ranges,
+                       n, (float4*)out);
+    return hipGetLastError();
+}
+
+hipError_t launchPick(const SceneDev& sc, const PersistArgs& pa, const cgs_ray* rays, uint32_t n, cgs_hit* hits,
+                      uint32_t* occluded, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    Persist ps {pa.stack, pa.lanes, pa.counter};
+    if (occluded)
+        hipLaunchKernelGGL(kPick<true>, dim3(pa.grid), dim3(256), 0, s, sc, ps, rays, n, hits, occluded);
+    else
+        hipLaunchKernelGGL(kPick<false>, dim3(pa.grid), dim3(256), 0, s, sc, ps, rays, n, hits, occluded);
     return hipGetLastError();
 }
 

@@ -238,6 +238,58 @@ cgs_status cgs_bezier_clip(cgs_ctx* ctx, const cgs_ray* rays, const float* patch
 cgs_status cgs_subdivision(cgs_ctx* ctx, const cgs_ray* rays, const float* patches, uint32_t n,
                            float threshold, cgs_hit* hits, void* stream);
 
+/* Patch evaluation, batched: patch r (16 float4 control points, device) at
+ * (u, v) = uv[2r], uv[2r+1] (device) into out[4r..4r+3] (device).  The kinds
+ * are the reference's header functions, FP32, same operation order:
+ *   CGS_EVAL_INTERPOLATE       spline::interpolate, de Casteljau (Spline.h:197-214)
+ *   CGS_EVAL_INTERPOLATE_MAT   spline::mat::interpolate (SplineMat.h:125-140)
+ *   CGS_EVAL_DERIVATIVE_U/V    spline::derivativeU/V (Spline.h:216-254): the
+ *                              hodograph without its factor 3
+ *   CGS_EVAL_DERIVATIVE_U/V_MAT  spline::mat::derivativeU/V (SplineMat.h:142-176)
+ *   CGS_EVAL_NORMAL            spline::normal with its degenerate-edge step of
+ *                              1e-4 (Spline.cu:10-27); w = 0
+ *   CGS_EVAL_NORMAL_MAT        spline::mat::normal (SplineMat.h:178-187); w = 0 */
+#define CGS_EVAL_INTERPOLATE 0u
+#define CGS_EVAL_INTERPOLATE_MAT 1u
+#define CGS_EVAL_DERIVATIVE_U 2u
+#define CGS_EVAL_DERIVATIVE_V 3u
+#define CGS_EVAL_DERIVATIVE_U_MAT 4u
+#define CGS_EVAL_DERIVATIVE_V_MAT 5u
+#define CGS_EVAL_NORMAL 6u
+#define CGS_EVAL_NORMAL_MAT 7u
+cgs_status cgs_patch_eval(cgs_ctx* ctx, uint32_t kind, const float* patches, const float* uv, uint32_t n, float* out,
+                          void* stream);
+
+/* spline::subpatch (Spline.h:384-396), batched: patch r (device) cut to
+ * [umin, umax] × [vmin, vmax] = ranges[4r..4r+3] (device), subpatchV then
+ * subpatchU, into out (n × 16 float4, device). */
+cgs_status cgs_patch_subpatch(cgs_ctx* ctx, const float* patches, const float* ranges, uint32_t n, float* out,
+                              void* stream);
+
+/* cg::PatchBVH (PatchBVH.h:13-28, PatchBVH.cpp:7-121) over one surface of
+ * bicubic Bézier patches (host points nv × 4, indices np × 16, point(i,j) =
+ * [4j+i]): one patch per leaf, boxed by its control points
+ * (PatchBVH.cpp:66-75), resident on ctx's device.  The tree is the
+ * reference's rt::BVH (median split) with its perfect hash; the reference's
+ * BVHBase comes from its absent Ds library, and only the hits, not the tree,
+ * are its results. */
+typedef struct cgs_patchbvh cgs_patchbvh;
+cgs_status cgs_patchbvh_create(cgs_ctx* ctx, const float* vertices, uint32_t nv, const uint32_t* indices, uint32_t np,
+                               cgs_patchbvh** out);
+cgs_status cgs_patchbvh_destroy(cgs_patchbvh* bvh);
+/* BVHBase::intersect(ray, hit) with PatchBVH::intersectLeaf (PatchBVH.cpp:104-121):
+ * the closest patch hit of each ray (a Ray3f: its direction normalized once),
+ * hit.distance starting at ray.tmax.  hits (device): t, (u, v), patch index
+ * (CGS_EMPTY: a miss; CGS_PATCH_OVERFLOW: a region-stack overflow). */
+cgs_status cgs_patchbvh_intersect(cgs_ctx* ctx, cgs_patchbvh* bvh, const cgs_ray* rays, uint32_t n, cgs_hit* hits,
+                                  void* stream);
+/* BVHBase::intersect(ray) with PatchBVH::intersectLeaf (PatchBVH.cpp:84-102):
+ * 1 when a patch whose box the traversal reaches within ray.tmax has a hit in
+ * front of the origin (at any distance: the leaf clips from distance inf),
+ * else 0; 2 on a region-stack overflow.  occluded: n u32 (device). */
+cgs_status cgs_patchbvh_intersect_any(cgs_ctx* ctx, cgs_patchbvh* bvh, const cgs_ray* rays, uint32_t n,
+                                      uint32_t* occluded, void* stream);
+
 /* ---------------------------------------------------------------------- */
 /* BVH and perfect hash                                                    */
 /* ---------------------------------------------------------------------- */

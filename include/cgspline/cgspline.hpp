@@ -22,6 +22,8 @@
 //
 #pragma once
 
+#include <array>
+#include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <istream>
@@ -516,8 +518,142 @@ private:
 
 } // namespace rt
 
+// ---------------------------------------------------------------------------
+// Per-call forms of the reference's header functions
+// ---------------------------------------------------------------------------
+
+// Ray3f of the reference's Ds library: origin and direction (normalized where
+// the reference's constructor normalizes it, by the library) and the range.
+struct Ray3f
+{
+    float origin[3];
+    float direction[3];
+    float tMax = INFINITY;
+};
+
+// Intersection as the per-patch functions and PatchBVH fill it: distance
+// (the current best on entry), the patch coordinates (u, v), the patch index
+// and the object.
+struct Intersection
+{
+    float distance = INFINITY;
+    float p[2] = {0.0f, 0.0f};
+    uint32_t triangleIndex = CGS_EMPTY;
+    const void* object = nullptr;
+};
+
+namespace detail
+{
+
+// A small device staging area per GPU for the per-call functions (one call
+// at a time per GPU; guarded).
+struct Staging
+{
+    std::mutex mu;
+    void* p = nullptr;
+    static constexpr size_t bytes = 1024;
+};
+inline Staging& staging(const rt::Device& dev)
+{
+    static std::mutex mu;
+    static std::map<const rt::Device*, Staging*> all;
+    std::lock_guard lk(mu);
+    Staging*& s = all[&dev];
+    if (!s)
+    {
+        s = new Staging;
+        s->p = dev.allocate(Staging::bytes);
+    }
+    return *s;
+}
+
+// One ray against one patch gathered from buffer[patch[k]] (vec4 points)
+// by a batched entry point of size 1; hit is updated when found.
+inline bool perPatch(decltype(&cgs_bezier_clip) fn, Intersection& hit, const Ray3f& ray, const float* buffer4,
+                     const uint32_t patch[16], float param, const rt::Device& dev)
+{
+    cgs_ray r {ray.origin[0], ray.origin[1], ray.origin[2], hit.distance,
+               ray.direction[0], ray.direction[1], ray.direction[2], 0.0f};
+    float pts[64];
+    for (int k = 0; k < 16; k++)
+        for (int c = 0; c < 4; c++) pts[4 * k + c] = buffer4[4 * (size_t)patch[k] + c];
+    Staging& st = staging(dev);
+    std::lock_guard lk(st.mu);
+    char* base = (char*)st.p;
+    dev.upload(base, &r, sizeof r);
+    dev.upload(base + 64, pts, sizeof pts);
+    rt::check(fn(dev.handle(), (const cgs_ray*)base, (const float*)(base + 64), 1, param, (cgs_hit*)(base + 512), nullptr),
+              "per-patch intersection");
+    cgs_hit h;
+    dev.download(&h, base + 512, sizeof h);
+    dev.synchronize();
+    if (h.patch == CGS_PATCH_OVERFLOW) throw rt::Error(CGS_ERR_STACK_OVERFLOW, "doBezierClipping");
+    if (h.patch != 1u) return false;
+    hit.distance = h.t;
+    hit.p[0] = h.u;
+    hit.p[1] = h.v;
+    return true;
+}
+
+} // namespace detail
+
 namespace spline
 {
+
+// spline::doBezierClipping (Spline.h:488-493) for one ray and one patch:
+// buffer holds vec4 points (4 floats each), patch 16 indices into it.  The
+// hit's distance is the current best; on a hit nearer than it, distance and
+// p (u, v) are updated and true is returned.
+inline bool doBezierClipping(Intersection& hit, const Ray3f& ray, const float* buffer4, const uint32_t patch[16],
+                             float tol = 0x1p-12f, const rt::Device& dev = rt::Device::shared())
+{
+    return detail::perPatch(&cgs_bezier_clip, hit, ray, buffer4, patch, tol, dev);
+}
+
+// spline::doSubdivision (Spline.h:495-500) for one ray and one patch.
+inline bool doSubdivision(Intersection& hit, const Ray3f& ray, const float* buffer4, const uint32_t patch[16],
+                          float tol = 0x1p-4f, const rt::Device& dev = rt::Device::shared())
+{
+    return detail::perPatch(&cgs_subdivision, hit, ray, buffer4, patch, tol, dev);
+}
+
+// The header evaluations (Spline.h:197-254, Spline.cu:10-27,
+// SplineMat.h:125-187), batched over device arrays: patches n × 16 float4,
+// uv n × 2, out n × 4 (kind: CGS_EVAL_*).
+inline void evaluate(uint32_t kind, const float* patches, const float* uv, uint32_t n, float* out,
+                     const rt::Device& dev = rt::Device::shared(), rt::stream_t stream = nullptr)
+{
+    rt::check(cgs_patch_eval(dev.handle(), kind, patches, uv, n, out, stream), "spline evaluation");
+}
+
+// spline::subpatch (Spline.h:384-396), batched: ranges n × (umin, umax, vmin, vmax).
+inline void subpatch(const float* patches, const float* ranges, uint32_t n, float* out,
+                     const rt::Device& dev = rt::Device::shared(), rt::stream_t stream = nullptr)
+{
+    rt::check(cgs_patch_subpatch(dev.handle(), patches, ranges, n, out, stream), "spline::subpatch");
+}
+
+// One patch gathered from buffer[patch[k]] at (u, v) by `kind`, on the GPU:
+// spline::interpolate (CGS_EVAL_INTERPOLATE), derivativeU/V, normal, and
+// their mat:: forms.  Returns x, y, z, w (w = 0 for normals).
+inline std::array<float, 4> evaluate(uint32_t kind, const float* buffer4, const uint32_t patch[16], float u, float v,
+                                     const rt::Device& dev = rt::Device::shared())
+{
+    float pts[64];
+    for (int k = 0; k < 16; k++)
+        for (int c = 0; c < 4; c++) pts[4 * k + c] = buffer4[4 * (size_t)patch[k] + c];
+    const float uv[2] = {u, v};
+    detail::Staging& st = detail::staging(dev);
+    std::lock_guard lk(st.mu);
+    char* base = (char*)st.p;
+    dev.upload(base, pts, sizeof pts);
+    dev.upload(base + 256, uv, sizeof uv);
+    evaluate(kind, (const float*)base, (const float*)(base + 256), 1, (float*)(base + 512), dev);
+    std::array<float, 4> out;
+    dev.download(out.data(), base + 512, sizeof(float) * 4);
+    dev.synchronize();
+    return out;
+}
 
 // spline::doBezierClipping (Spline.h:488-493), batched over device arrays:
 // ray r against patch r (16 float4 control points each); hits[r].patch is 1
@@ -536,4 +672,91 @@ inline void doSubdivision(const rt::Device& dev, const cgs_ray* rays, const floa
 }
 
 } // namespace spline
+
+// ---------------------------------------------------------------------------
+// cg::PatchBVH (PatchBVH.h:13-28, PatchBVH.cpp:7-121): a BVH over one
+// surface's Bézier patches, resident on a Device, with the queries of
+// BVHBase that PatchBVH's leaves serve: the closest hit (intersect(ray, hit))
+// and whether anything is hit (intersect(ray)).
+// ---------------------------------------------------------------------------
+class PatchBVH
+{
+public:
+    explicit PatchBVH(const Surface* s, const rt::Device& dev = rt::Device::shared()) : _surface(s), _dev(&dev)
+    {
+        const auto pts = s->points();
+        const auto idx = s->indices();
+        rt::check(cgs_patchbvh_create(dev.handle(), pts.data(), (uint32_t)(pts.size() / 4), idx.data(),
+                                      (uint32_t)(idx.size() / 16), &_bvh),
+                  "PatchBVH");
+    }
+    ~PatchBVH()
+    {
+        if (_bvh) (void)cgs_patchbvh_destroy(_bvh);
+    }
+    PatchBVH(const PatchBVH&) = delete;
+    PatchBVH& operator=(const PatchBVH&) = delete;
+
+    const Surface* surface() const { return _surface; }
+
+    // The closest patch hit nearer than hit.distance: fills distance, p (u, v),
+    // triangleIndex (the patch) and object (this), as intersectLeaf does.
+    bool intersect(const Ray3f& ray, Intersection& hit) const
+    {
+        cgs_ray r {ray.origin[0], ray.origin[1], ray.origin[2], std::min(hit.distance, ray.tMax),
+                   ray.direction[0], ray.direction[1], ray.direction[2], 0.0f};
+        cgs_hit h = one(r, false);
+        if (h.patch == CGS_EMPTY) return false;
+        if (h.patch == CGS_PATCH_OVERFLOW) throw rt::Error(CGS_ERR_STACK_OVERFLOW, "PatchBVH::intersect");
+        hit.distance = h.t;
+        hit.p[0] = h.u;
+        hit.p[1] = h.v;
+        hit.triangleIndex = h.patch;
+        hit.object = this;
+        return true;
+    }
+
+    // Whether a patch whose box the ray reaches within ray.tMax is hit.
+    bool intersect(const Ray3f& ray) const
+    {
+        cgs_ray r {ray.origin[0], ray.origin[1], ray.origin[2], ray.tMax,
+                   ray.direction[0], ray.direction[1], ray.direction[2], 0.0f};
+        const cgs_hit h = one(r, true);
+        if (h.patch == 2u) throw rt::Error(CGS_ERR_STACK_OVERFLOW, "PatchBVH::intersect");
+        return h.patch == 1u;
+    }
+
+    // Batched over device arrays (n rays, n hits or n occlusion words).
+    void intersect(const cgs_ray* rays, uint32_t n, cgs_hit* hits, rt::stream_t stream = nullptr) const
+    {
+        rt::check(cgs_patchbvh_intersect(_dev->handle(), _bvh, rays, n, hits, stream), "PatchBVH::intersect");
+    }
+    void intersectAny(const cgs_ray* rays, uint32_t n, uint32_t* occluded, rt::stream_t stream = nullptr) const
+    {
+        rt::check(cgs_patchbvh_intersect_any(_dev->handle(), _bvh, rays, n, occluded, stream), "PatchBVH::intersect");
+    }
+
+private:
+    cgs_hit one(const cgs_ray& r, bool any) const
+    {
+        detail::Staging& st = detail::staging(*_dev);
+        std::lock_guard lk(st.mu);
+        char* base = (char*)st.p;
+        _dev->upload(base, &r, sizeof r);
+        if (any)
+            intersectAny((const cgs_ray*)base, 1, (uint32_t*)(base + 512));
+        else
+            intersect((const cgs_ray*)base, 1, (cgs_hit*)(base + 512));
+        cgs_hit h {};
+        _dev->download(&h, base + 512, any ? sizeof(uint32_t) : sizeof h);
+        _dev->synchronize();
+        if (any) h.patch = *(const uint32_t*)&h.t;
+        return h;
+    }
+
+    const Surface* _surface;
+    const rt::Device* _dev;
+    cgs_patchbvh* _bvh = nullptr;
+};
+
 } // namespace cg

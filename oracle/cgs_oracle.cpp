@@ -153,6 +153,23 @@ int orc_eval(int fp64, const float* patch, double u, double v, int kind, double*
     return fp64 ? run(0.0) : run(0.0f);
 }
 
+// Spline.h:384-396 — subpatch of a 16-point (x,y,z,w) patch in place:
+// subpatchV to [vmin, vmax], then subpatchU to [umin, umax].
+void orc_subpatch(float* patch64, float umin, float umax, float vmin, float vmax)
+{
+    V4<float> P[16];
+    for (int i = 0; i < 16; i++) P[i] = {patch64[4 * i], patch64[4 * i + 1], patch64[4 * i + 2], patch64[4 * i + 3]};
+    subpatchV(P, vmin, vmax);
+    subpatchU(P, umin, umax);
+    for (int i = 0; i < 16; i++)
+    {
+        patch64[4 * i] = P[i].x;
+        patch64[4 * i + 1] = P[i].y;
+        patch64[4 * i + 2] = P[i].z;
+        patch64[4 * i + 3] = P[i].w;
+    }
+}
+
 // Slices the 16 (x,y) points of a patch: side 0 = U (rows), 1 = V (columns).
 void orc_subpatch2d(float* p2d, int side, float a, float b)
 {

@@ -66,6 +66,7 @@ def lib():
         L.orc_clip2d_candidates.argtypes = [_P, _F32, _P, C.c_int, _P]
         L.orc_eval.argtypes = [C.c_int, _P, _F64, _F64, C.c_int, _P]
         L.orc_subpatch2d.argtypes = [_P, C.c_int, _F32, _F32]
+        L.orc_subpatch.argtypes = [_P, _F32, _F32, _F32, _F32]
         L.orc_slice_matrix.argtypes = [_P, _P]
         L.orc_x_axis_intersection.argtypes = [_P, _P, _P]
         L.orc_triangle_origin.argtypes = [_P, _P, _P, _P]
@@ -156,6 +157,13 @@ def evaluate(patch, u, v, kind, fp64=False):
     rc = lib().orc_eval(int(fp64), _ptr(p), float(u), float(v), EVAL_KINDS[kind], _ptr(out))
     assert rc == 0
     return out
+
+
+def subpatch(patch, umin, umax, vmin, vmax):
+    """Spline.h:384-396 on a (16, 4) float32 patch (subpatchV, then subpatchU)."""
+    p = np.array(patch, np.float32).reshape(16, 4).copy()
+    lib().orc_subpatch(_ptr(p), umin, umax, vmin, vmax)
+    return p
 
 
 def subpatch2d(p2d, side, a, b):

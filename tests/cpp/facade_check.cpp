@@ -205,6 +205,57 @@ int runLoadBe(const char* inPath, const char* outPath)
     return 0;
 }
 
+// The per-call functions and PatchBVH the way a picking caller uses them:
+// a .be model, rays (n × 8 floats: origin, tMax, direction, pad); for each
+// ray: PatchBVH::intersect(ray, hit) (found, distance, u, v, patch),
+// PatchBVH::intersect(ray), spline::doBezierClipping and doSubdivision of the
+// ray against patch k mod count (found, distance, u, v, each), and
+// spline::evaluate(CGS_EVAL_NORMAL) of that patch at (0.3, 0.6).
+int runPick(const char* modelPath, const char* raysPath, const char* outPath)
+{
+    std::ifstream in(modelPath, std::ios::binary);
+    if (!in) throw std::runtime_error(std::string("cannot open ") + modelPath);
+    const std::shared_ptr<Surface> s = Surface::load_be(in);
+    std::ifstream rf(raysPath, std::ios::binary | std::ios::ate);
+    if (!rf) throw std::runtime_error(std::string("cannot open ") + raysPath);
+    const size_t nbytes = (size_t)rf.tellg();
+    rf.seekg(0);
+    std::vector<float> r(nbytes / 4);
+    rf.read((char*)r.data(), (std::streamsize)(r.size() * 4));
+    const PatchBVH bvh(s.get());
+    Writer w(outPath);
+    const uint32_t n = (uint32_t)(r.size() / 8);
+    for (uint32_t k = 0; k < n; k++)
+    {
+        const float* q = &r[8 * k];
+        Ray3f ray {{q[0], q[1], q[2]}, {q[4], q[5], q[6]}, q[3]};
+        Intersection hit;
+        const bool f = bvh.intersect(ray, hit);
+        w.put((uint32_t)f);
+        w.put(hit.distance);
+        w.put(hit.p[0]);
+        w.put(hit.p[1]);
+        w.put(hit.triangleIndex);
+        if (f && hit.object != &bvh) return 20;
+        w.put((uint32_t)bvh.intersect(ray));
+        const uint32_t* patch = s->indices().data() + 16 * (size_t)(k % s->count());
+        for (int m = 0; m < 2; m++)
+        {
+            Intersection h;
+            h.distance = q[3];
+            const bool g = m == 0 ? spline::doBezierClipping(h, ray, s->points().data(), patch)
+                                  : spline::doSubdivision(h, ray, s->points().data(), patch);
+            w.put((uint32_t)g);
+            w.put(h.distance);
+            w.put(h.p[0]);
+            w.put(h.p[1]);
+        }
+        const auto nrm = spline::evaluate(CGS_EVAL_NORMAL, s->points().data(), patch, 0.3f, 0.6f);
+        for (float x : nrm) w.put(x);
+    }
+    return 0;
+}
+
 int runErrors()
 {
     // Surface::load_be on a truncated model: runtime_error with the
@@ -256,7 +307,8 @@ int main(int argc, char** argv)
         if (mode == "render" && argc == 4) return runRender(argv[2], argv[3]);
         if (mode == "load_be" && argc == 4) return runLoadBe(argv[2], argv[3]);
         if (mode == "errors") return runErrors();
-        std::fprintf(stderr, "usage: facade_check bvh|hash|render|load_be <in> <out> | errors\n");
+        if (mode == "pick" && argc == 5) return runPick(argv[2], argv[3], argv[4]);
+        std::fprintf(stderr, "usage: facade_check bvh|hash|render|load_be <in> <out> | pick <model> <rays> <out> | errors\n");
         return 2;
     }
     catch (const std::exception& e)

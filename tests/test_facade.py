@@ -130,3 +130,44 @@ def test_facade_render_matches_oracle(tmp_path, oracle, c1):
     ch = lambda fr: np.stack([(fr >> s) & 0xFF for s in (0, 8, 16, 24)], -1).astype(np.int32)
     assert np.abs(ch(frames[0]) - ch(oframe)).max() <= 1
     assert np.array_equal(frames[1], frames[0]) and np.array_equal(frames[2], frames[0])
+
+
+@pytest.mark.gpu
+def test_facade_pick_and_per_call_functions(tmp_path, oracle):
+    """cg::PatchBVH::intersect (closest and any), the per-call
+    spline::doBezierClipping / doSubdivision and spline::evaluate, driven from
+    C++ on the teapot, against the Python binding's PatchBVH and the oracle."""
+    import cgspline_amd as cgs
+    from test_gpu_patchbvh import ASSETS, _pick_rays
+    model = os.path.join(ASSETS, "teapot")
+    v, i = cgs.load_be(model)
+    rays = _pick_rays(v, 300, 75)
+    rays[::3, 3] = 5.0
+    rays.tofile(tmp_path / "rays")
+    say("facade pick", rays.shape[0])
+    rc, out = run("pick", model, tmp_path / "rays", tmp_path / "out")
+    assert rc == 0, out
+    rec = np.fromfile(tmp_path / "out", np.uint32).reshape(rays.shape[0], 18)
+    f32 = rec.view(np.float32)
+    ctx = cgs.Context(0)
+    pb = ctx.patch_bvh(v, i)
+    tuv, patch = pb.intersect(rays)
+    occ = pb.intersect_any(rays)
+    found = rec[:, 0] == 1
+    assert np.array_equal(found, patch != 0xFFFFFFFF)
+    assert np.array_equal(rec[found, 1:4], tuv[found].view(np.uint32)) and np.array_equal(rec[found, 4], patch[found])
+    assert np.array_equal(rec[:, 5], occ)
+    assert found.sum() > 50
+    P = v[i][np.arange(rays.shape[0]) % i.shape[0]]
+    for col, fn in ((6, lambda: oracle.clip_batch(rays, P)[0]),
+                    (10, lambda: oracle.clip_batch(rays, P, method=1)[0])):
+        o = fn()
+        of = o[:, 3] == 1
+        assert np.array_equal(rec[:, col] == 1, of)
+        assert np.array_equal(f32[of, col + 1:col + 4], o[of, :3])
+    nrm = np.stack([oracle.evaluate(P[k], np.float32(0.3), np.float32(0.6), "normal_dc") for k in range(P.shape[0])])
+    nrm = nrm.astype(np.float32)
+    nrm[:, 3] = 0.0
+    assert np.array_equal(rec[:, 14:18], nrm.view(np.uint32))
+    pb.destroy()
+    ctx.close()

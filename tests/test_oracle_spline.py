@@ -179,3 +179,25 @@ def test_triangle_origin_barycentrics(oracle):
             hits += 1
             assert np.allclose(c, uv, atol=1e-5)
     assert hits > 10
+
+
+def test_subpatch_is_a_reparametrisation(oracle):
+    """spline::subpatch (Spline.h:384-396): the sub-patch's surface at (s, t)
+    is the patch's at (umin + s (umax - umin), vmin + t (vmax - vmin)), in
+    homogeneous coordinates (FP64 Bernstein evaluation; FP32 slicing error
+    only); the full range leaves the patch as it is."""
+    r = rng(12)
+    for k in range(40):
+        P = rand_patch(r, w=(k % 2 == 1))
+        a = np.sort(r.random(2)).astype(np.float32)
+        b = np.sort(r.random(2)).astype(np.float32)
+        Q = oracle.subpatch(P, a[0], a[1], b[0], b[1])
+        for s, t in r.random((6, 2)):
+            want = surface(P, a[0] + s * (a[1] - a[0]), b[0] + t * (b[1] - b[0]))
+            got = surface(Q, s, t)
+            assert np.allclose(got, want, rtol=2e-5, atol=2e-5), (k, s, t)
+        assert np.array_equal(oracle.subpatch(P, 0.0, 1.0, 0.0, 1.0), P)
+    # a planted transposition (u and v ranges swapped) fails the identity
+    P = rand_patch(r)
+    Q = oracle.subpatch(P, 0.1, 0.4, 0.5, 0.9)
+    assert not np.allclose(surface(Q, 0.3, 0.7), surface(P, 0.5 + 0.3 * 0.4, 0.1 + 0.7 * 0.3), rtol=1e-3)
