@@ -337,6 +337,40 @@ class Context:
         return c.chunk, c.blocks_per_cu
 
 
+    def set_batch_trace_options(self, **options):
+        """Leaf method of later batched launches (trace_options' keywords;
+        none: the reference's clipping)."""
+        op = trace_options(**options) if options else None
+        check(lib().cgs_batch_set_trace_options(self.h, C.byref(op) if op is not None else None),
+              "cgs_batch_set_trace_options")
+
+    def batch_trace_options(self) -> dict:
+        op = _abi.TraceOptions()
+        check(lib().cgs_batch_get_trace_options(self.h, C.byref(op)), "cgs_batch_get_trace_options")
+        return _options_dict(op)
+
+    def batch_autotune_accuracy(self, jobs, max_rel_t=1e-4, max_uv=1e-4, max_disagree=0.0, allow_polish=True):
+        """Accuracy-gated tuning of the batched leaf method
+        (cgs_batch_autotune_accuracy): returns (the kept options, a list of
+        every candidate's options, ms, max_rel_t, max_uv, disagree,
+        admissible)."""
+        arr = self.batch_jobs(jobs)
+        lim = _abi.AccuracyLimits(max_rel_t, max_uv, max_disagree, int(bool(allow_polish)))
+        best = _abi.TraceOptions()
+        rep = (_abi.AutotuneEntry * 64)()
+        cnt = C.c_uint32()
+        check(lib().cgs_batch_autotune_accuracy(self.h, arr, len(jobs), C.byref(lim), C.byref(best), rep, 64,
+                                                C.byref(cnt)), "cgs_batch_autotune_accuracy")
+        report = [dict(options=_options_dict(e.options), ms=e.ms, max_rel_t=e.max_rel_t, max_uv=e.max_uv,
+                       disagree=e.disagree, admissible=bool(e.admissible)) for e in rep[:cnt.value]]
+        return _options_dict(best), report
+
+
+def _options_dict(op) -> dict:
+    return dict(method=int(op.method), clip_tol=float(op.clip_tol), subdiv_depth=int(op.subdiv_depth),
+                max_clip_iters=int(op.max_clip_iters), polish=bool(op.polish))
+
+
 class Batch:
     """A batched launch captured once in a HIP graph and replayed."""
 
@@ -514,6 +548,33 @@ class DeviceScene:
         self.trace_any_dev(dr, n, do, **options)
         self.ctx.sync()
         return do.to_host(np.uint32, (n,))
+
+    def shade_dev(self, rays, n, color, pixels=None, hits=None, objects=None, counts=None, eps=1e-4, depth=0,
+                  stream=None):
+        """cgs_shade into device buffers."""
+        op = _abi.RenderOptions(eps, depth, 0)
+        check(lib().cgs_shade(self.ctx.h, self.h, ptr(rays), n, C.byref(op), ptr(color), ptr(pixels), ptr(hits),
+                              ptr(objects), ptr(counts), stream), "cgs_shade")
+
+    def shade(self, rays: np.ndarray, eps=1e-4, depth=0):
+        """Host rays (n, 8) traced and shaded as the render shades camera rays:
+        returns (color (n,3), packed pixels (n,), first hits t,u,v (n,3),
+        patch (n,), object (n,), counts dict)."""
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        n = rays.shape[0]
+        dr = DeviceArray.from_host(self.ctx, rays)
+        dc = DeviceArray(self.ctx, 12 * n)
+        dp = DeviceArray(self.ctx, 4 * n)
+        dh = DeviceArray(self.ctx, 16 * n)
+        do = DeviceArray(self.ctx, 4 * n)
+        dn = DeviceArray.from_host(self.ctx, np.zeros(4, np.uint64))
+        self.shade_dev(dr, n, dc, dp, dh, do, dn, eps, depth)
+        self.ctx.sync()
+        h = dh.to_host(np.float32, (n, 4))
+        cnt = dn.to_host(np.uint64, (4,))
+        return (dc.to_host(np.float32, (n, 3)), dp.to_host(np.uint32, (n,)), h[:, :3].copy(),
+                h.view(np.uint32)[:, 3].copy(), do.to_host(np.int32, (n,)),
+                dict(primary=int(cnt[0]), shadow=int(cnt[1]), secondary=int(cnt[2]), overflow=int(cnt[3])))
 
     def render(self, camera: "scenes.Camera", eps=1e-4, depth=0):
         """Returns (color (h,w,3) f32, frame (h,w) u32, counts dict)."""

@@ -36,10 +36,11 @@ EXPORTS = [
     "cgs_phf_displacement_size", "cgs_phf_build", "cgs_phf_build_static", "cgs_phf_fill_table", "cgs_phf_hash",
     "cgs_scene_create", "cgs_scene_destroy", "cgs_scene_bvh_info", "cgs_scene_bvh_copy", "cgs_scene_culled",
     "cgs_trace_closest", "cgs_trace_any", "cgs_trace_closest_ex", "cgs_trace_any_ex", "cgs_render",
-    "cgs_render_running", "cgs_work_counts",
+    "cgs_render_running", "cgs_shade", "cgs_work_counts",
     "cgs_trace_batch", "cgs_batch_create", "cgs_batch_launch", "cgs_batch_destroy", "cgs_batch_set_config",
-    "cgs_batch_get_config", "cgs_batch_autotune",
-    "cgs_server_create", "cgs_server_submit", "cgs_server_poll", "cgs_server_wait", "cgs_server_destroy",
+    "cgs_batch_get_config", "cgs_batch_autotune", "cgs_batch_set_trace_options", "cgs_batch_get_trace_options",
+    "cgs_batch_autotune_accuracy",
+    "cgs_server_create", "cgs_server_submit", "cgs_server_submit_shaded", "cgs_server_poll", "cgs_server_wait", "cgs_server_destroy",
     "cgs_malloc", "cgs_free", "cgs_memcpy_h2d", "cgs_memcpy_d2h", "cgs_memset", "cgs_stream_sync",
     "cgs_stream_create", "cgs_stream_destroy",
 ]
@@ -113,6 +114,15 @@ class TraceOptions(C.Structure):
                 ("polish", _U32), ("reserved", _U32 * 3)]
 
 
+class AccuracyLimits(C.Structure):
+    _fields_ = [("max_rel_t", _F32), ("max_uv", _F32), ("max_disagree", _F32), ("allow_polish", _U32)]
+
+
+class AutotuneEntry(C.Structure):
+    _fields_ = [("options", TraceOptions), ("ms", _F32), ("max_rel_t", _F32), ("max_uv", _F32), ("disagree", _F32),
+                ("admissible", _U32)]
+
+
 class RayCounts(C.Structure):
     _fields_ = [("primary", _U64), ("shadow", _U64), ("secondary", _U64), ("overflow", _U64)]
 
@@ -169,6 +179,7 @@ def lib():
             "cgs_render": ([_P, _P, C.POINTER(Camera), C.POINTER(RenderOptions), _U32, _U32, _P, _P, _P, _P],
                            C.c_int),
             "cgs_render_running": ([_P], C.c_int),
+            "cgs_shade": ([_P, _P, _P, _U32, C.POINTER(RenderOptions), _P, _P, _P, _P, _P, _P], C.c_int),
             "cgs_work_counts": ([_P, _P, _U32, C.c_int], C.c_int),
             "cgs_trace_batch": ([_P, C.POINTER(BatchJob), _U32, _P], C.c_int),
             "cgs_batch_create": ([_P, C.POINTER(BatchJob), _U32, C.POINTER(_P)], C.c_int),
@@ -177,8 +188,15 @@ def lib():
             "cgs_batch_set_config": ([_P, C.POINTER(BatchConfig)], C.c_int),
             "cgs_batch_get_config": ([_P, C.POINTER(BatchConfig)], C.c_int),
             "cgs_batch_autotune": ([_P, C.POINTER(BatchJob), _U32, C.POINTER(BatchConfig)], C.c_int),
+            "cgs_batch_set_trace_options": ([_P, C.POINTER(TraceOptions)], C.c_int),
+            "cgs_batch_get_trace_options": ([_P, C.POINTER(TraceOptions)], C.c_int),
+            "cgs_batch_autotune_accuracy": ([_P, C.POINTER(BatchJob), _U32, C.POINTER(AccuracyLimits),
+                                             C.POINTER(TraceOptions), C.POINTER(AutotuneEntry), _U32,
+                                             C.POINTER(_U32)], C.c_int),
             "cgs_server_create": ([_P, C.POINTER(_P), _U32, C.POINTER(ServerOptions), C.POINTER(_P)], C.c_int),
             "cgs_server_submit": ([_P, _U32, _P, _U32, _I32, _P, _P, _P, C.POINTER(_U64)], C.c_int),
+            "cgs_server_submit_shaded": ([_P, _U32, _P, _U32, C.POINTER(RenderOptions), _P, _P, _P, _P,
+                                          C.POINTER(_U64)], C.c_int),
             "cgs_server_poll": ([_P, _U64, C.POINTER(_U64)], C.c_int),
             "cgs_server_wait": ([_P, _U64, C.POINTER(_U64)], C.c_int),
             "cgs_server_destroy": ([_P], C.c_int),

@@ -23,6 +23,7 @@ cgs_status setError(cgs_status s, const std::string& msg);
 
 struct SceneDev;
 struct BatchJob;
+struct ShadeJob;
 
 // One perfect-hash build (one BVH's key set).
 struct PhfJob
@@ -104,8 +105,9 @@ enum class PersistKernel
     Render,
     Batch,
     Pick,
+    Shade,
 };
-constexpr int kPersistKernels = 6;
+constexpr int kPersistKernels = 7;
 
 struct PersistArgs
 {
@@ -132,10 +134,12 @@ hipError_t launchTraceClosest(const SceneDev& sc, const PersistArgs& pa, const c
                               int32_t* objects, unsigned long long* overflow, hipStream_t s, const TraceMode& m);
 hipError_t launchTraceAny(const SceneDev& sc, const PersistArgs& pa, const cgs_ray* rays, uint32_t n, uint32_t* occluded,
                           unsigned long long* overflow, hipStream_t s, const TraceMode& m);
-// Batched queries; the counter must be zero at launch (cleared with a fill
-// kernel, so the pair can be captured in a graph).
+// Batched queries with leaf method `m` (closest-hit jobs polished when it
+// asks); the counter must be zero at launch (cleared with a fill kernel, so
+// the pair can be captured in a graph).
 hipError_t launchTraceBatch(const BatchJob* jobs, uint32_t njobs, uint32_t total, uint32_t chunk, float2* stack,
-                            uint32_t lanes, unsigned int* counter, uint32_t grid, unsigned long long* overflow, hipStream_t s);
+                            uint32_t lanes, unsigned int* counter, uint32_t grid, unsigned long long* overflow, hipStream_t s,
+                            const TraceMode& m = TraceMode {});
 hipError_t launchClipBatch(const PersistArgs& pa, const cgs_ray* rays, const float* patches, uint32_t n, float tol,
                            cgs_hit* hits, hipStream_t s);
 hipError_t launchSubdivBatch(const cgs_ray* rays, const float* patches, uint32_t n, float threshold, cgs_hit* hits,
@@ -146,6 +150,12 @@ hipError_t launchSubpatch(const float* patches, const float* ranges, uint32_t n,
 // (occluded non-null) any hits into `occluded`.
 hipError_t launchPick(const SceneDev& sc, const PersistArgs& pa, const cgs_ray* rays, uint32_t n, cgs_hit* hits,
                       uint32_t* occluded, hipStream_t s);
+// Shaded ray queries over `jobs` (consecutive ray numbers, `total` rays); with
+// `one` non-null every job is on that scene.  The counter must be zero at
+// launch.  counts (nullable): cgs_ray_counts, accumulated.
+hipError_t launchShade(const SceneDev* one, const ShadeJob* jobs, uint32_t njobs, uint32_t total, uint32_t chunk,
+                       float2* stack, uint32_t lanes, unsigned int* counter, uint32_t grid, unsigned long long* counts,
+                       hipStream_t s);
 // kRender reads its camera, options and output pointers from `argsDev`, a
 // device block of kRenderArgsBytes that launchRender fills on stream `s`.
 constexpr size_t kRenderArgsBytes = 256;

@@ -98,6 +98,8 @@ struct cgs_ctx
     bool persistLaunched = false;
     // batched queries: launch shape, job table, completion of the last launch
     uint32_t batchChunk = 64, batchBlocksPerCU = 0;
+    TraceMode batchMode;                     // leaf method of batched launches
+    cgs_trace_options batchOptions {};       // ... as the caller set it (zero: the default)
     DeviceBuffer batchTable;
     size_t batchTableCap = 0;
     hipEvent_t batchDone = nullptr;
@@ -105,6 +107,7 @@ struct cgs_ctx
     unsigned long long* dOverflow = nullptr; // device counter of overflowed clip stacks
     unsigned int* dCounters = nullptr;       // one work counter per persistent kernel
     void* dRenderArgs = nullptr;             // kRender's argument block (one render in flight)
+    ShadeJob* dShadeJob = nullptr;           // cgs_shade's job (persistent launches run one at a time)
     float2* dStack = nullptr;                // clipping region stacks of the resident lanes
     size_t stackLanes = 0;
     uint32_t grid[kPersistKernels] = {};
@@ -573,6 +576,7 @@ cgs_status cgs_open(int device, cgs_ctx** out)
     if (e == hipSuccess) e = hipMalloc(&c->dCounters, 256);
     if (e == hipSuccess) e = hipMemset(c->dCounters, 0, 256);
     if (e == hipSuccess) e = hipMalloc(&c->dRenderArgs, kRenderArgsBytes);
+    if (e == hipSuccess) e = hipMalloc(&c->dShadeJob, sizeof(ShadeJob));
     if (e != hipSuccess)
     {
         delete c;
@@ -593,6 +597,7 @@ cgs_status cgs_close(cgs_ctx* ctx)
     if (ctx->dOverflow) (void)hipFree(ctx->dOverflow);
     if (ctx->dCounters) (void)hipFree(ctx->dCounters);
     if (ctx->dRenderArgs) (void)hipFree(ctx->dRenderArgs);
+    if (ctx->dShadeJob) (void)hipFree(ctx->dShadeJob);
     if (ctx->dStack) (void)hipFree(ctx->dStack);
     if (ctx->hCount) (void)hipHostFree(ctx->hCount);
     delete ctx;
@@ -1283,6 +1288,40 @@ cgs_status cgs_render(cgs_ctx* ctx, cgs_scene* sc, const cgs_camera* cam, const 
     return persistEnd(ctx, s);
 }
 
+cgs_status cgs_shade(cgs_ctx* ctx, cgs_scene* sc, const cgs_ray* rays, uint32_t n, const cgs_render_options* op,
+                     float* color, uint32_t* pixels, cgs_hit* hits, int32_t* objects, cgs_ray_counts* counts,
+                     void* stream)
+{
+    if (!ctx || !sc || !op || (n && (!rays || !color))) return fail(CGS_ERR_INVALID, "cgs_shade: null argument");
+    if (sc->device != ctx->device) return fail(CGS_ERR_INVALID, "cgs_shade: scene on another device");
+    HIP_TRY(hipSetDevice(ctx->device));
+    std::lock_guard lk(ctx->mu);
+    if (n == 0) return CGS_OK;
+    hipStream_t s = (hipStream_t)stream;
+    PersistArgs pa;
+    cgs_status st = persistArgs(ctx, PersistKernel::Shade, s, n, pa);
+    if (st != CGS_OK) return st;
+    const SceneDev view = sc->view();
+    ShadeJob job {};
+    job.sc = view;
+    job.rays = rays;
+    job.color = color;
+    job.pixels = pixels;
+    job.hits = hits;
+    job.objects = objects;
+    job.n = n;
+    job.rayBase = 0;
+    job.eps = op->eps;
+    job.depth = op->recursion_depth;
+    // `job` is pageable (the stack): HIP performs such a copy synchronously,
+    // so it has been read when the call returns; the copy itself runs in
+    // stream order after the previous persistent launch.
+    HIP_TRY(hipMemcpyAsync(ctx->dShadeJob, &job, sizeof job, hipMemcpyHostToDevice, s));
+    HIP_TRY(launchShade(&view, ctx->dShadeJob, 1, n, 64, pa.stack, pa.lanes, pa.counter, pa.grid,
+                        (unsigned long long*)counts, s));
+    return persistEnd(ctx, s);
+}
+
 cgs_status cgs_work_counts(cgs_ctx* ctx, uint64_t* out, uint32_t n, int reset)
 {
     if (!ctx || (n && !out)) return fail(CGS_ERR_INVALID, "cgs_work_counts: null argument");
@@ -1364,7 +1403,7 @@ static cgs_status validChunk(uint32_t chunk)
 // One batched launch through the context's own buffers (waits for the
 // previous one to finish before reusing them).
 static cgs_status traceBatch(cgs_ctx* ctx, const cgs_batch_job* jobs, uint32_t njobs, uint32_t chunk, uint32_t bpc,
-                             hipStream_t s)
+                             hipStream_t s, const TraceMode& mode)
 {
     uint32_t grid = 0;
     cgs_status st = batchGrid(ctx, bpc, grid);
@@ -1386,7 +1425,7 @@ static cgs_status traceBatch(cgs_ctx* ctx, const cgs_batch_job* jobs, uint32_t n
     HIP_TRY(hipStreamSynchronize(s)); // the host table is released on return
     HIP_TRY(launchTraceBatch(ctx->batchTable.as<BatchJob>(), (uint32_t)table.size(), total, chunk, pa.stack,
                              std::min<uint32_t>(pa.lanes, (uint32_t)ctx->stackLanes), pa.counter, grid,
-                             ctx->dOverflow, s));
+                             ctx->dOverflow, s, mode));
     HIP_TRY(hipEventRecord(ctx->batchDone, s));
     ctx->batchLaunched = true;
     return persistEnd(ctx, s);
@@ -1412,7 +1451,7 @@ cgs_status cgs_trace_batch(cgs_ctx* ctx, const cgs_batch_job* jobs, uint32_t njo
     if (!ctx) return fail(CGS_ERR_INVALID, "cgs_trace_batch: null context");
     HIP_TRY(hipSetDevice(ctx->device));
     std::lock_guard lk(ctx->mu);
-    return traceBatch(ctx, jobs, njobs, ctx->batchChunk, ctx->batchBlocksPerCU, (hipStream_t)stream);
+    return traceBatch(ctx, jobs, njobs, ctx->batchChunk, ctx->batchBlocksPerCU, (hipStream_t)stream, ctx->batchMode);
 }
 
 cgs_status cgs_batch_create(cgs_ctx* ctx, const cgs_batch_job* jobs, uint32_t njobs, cgs_batch** out)
@@ -1444,7 +1483,7 @@ cgs_status cgs_batch_create(cgs_ctx* ctx, const cgs_batch_job* jobs, uint32_t nj
     if (e == hipSuccess)
         e = launchTraceBatch(b->table.as<BatchJob>(), (uint32_t)table.size(), total, ctx->batchChunk,
                              b->stack.as<float2>(), lanes, b->counter.as<unsigned int>(), grid,
-                             b->overflow.as<unsigned long long>(), cs);
+                             b->overflow.as<unsigned long long>(), cs, ctx->batchMode);
     hipGraph_t g = nullptr;
     hipError_t e2 = hipStreamEndCapture(cs, &g);
     if (e == hipSuccess) e = e2;
@@ -1517,7 +1556,7 @@ cgs_status cgs_batch_autotune(cgs_ctx* ctx, const cgs_batch_job* jobs, uint32_t 
             {
                 hipError_t e = hipEventRecord(t0, s);
                 if (e != hipSuccess) { st = fail(CGS_ERR_HIP, hipGetErrorString(e)); break; }
-                st = traceBatch(ctx, jobs, njobs, chunk, bpc, s);
+                st = traceBatch(ctx, jobs, njobs, chunk, bpc, s, ctx->batchMode);
                 if (st != CGS_OK) break;
                 e = hipEventRecord(t1, s);
                 if (e == hipSuccess) e = hipEventSynchronize(t1);
@@ -1542,6 +1581,198 @@ cgs_status cgs_batch_autotune(cgs_ctx* ctx, const cgs_batch_job* jobs, uint32_t 
     return CGS_OK;
 }
 
+cgs_status cgs_batch_set_trace_options(cgs_ctx* ctx, const cgs_trace_options* options)
+{
+    if (!ctx) return fail(CGS_ERR_INVALID, "cgs_batch_set_trace_options: null context");
+    TraceMode m;
+    cgs_status st = traceMode(options, m, "cgs_batch_set_trace_options");
+    if (st != CGS_OK) return st;
+    std::lock_guard lk(ctx->mu);
+    ctx->batchMode = m;
+    ctx->batchOptions = options ? *options : cgs_trace_options {};
+    return CGS_OK;
+}
+
+cgs_status cgs_batch_get_trace_options(cgs_ctx* ctx, cgs_trace_options* options)
+{
+    if (!ctx || !options) return fail(CGS_ERR_INVALID, "cgs_batch_get_trace_options: null argument");
+    std::lock_guard lk(ctx->mu);
+    *options = ctx->batchOptions;
+    return CGS_OK;
+}
+
+} // extern "C"
+
+// The outputs of a batch's jobs, read back to the host.
+namespace
+{
+struct JobResults
+{
+    std::vector<cgs_hit> hits;
+    std::vector<int32_t> objects;
+    std::vector<uint32_t> occluded;
+};
+
+cgs_status readJobResults(const cgs_batch_job* jobs, uint32_t njobs, std::vector<JobResults>& out)
+{
+    out.resize(njobs);
+    for (uint32_t j = 0; j < njobs; j++)
+    {
+        const cgs_batch_job& q = jobs[j];
+        JobResults& r = out[j];
+        if (q.n == 0) continue;
+        if (q.any)
+        {
+            r.occluded.resize(q.n);
+            HIP_TRY(hipMemcpy(r.occluded.data(), q.occluded, (size_t)q.n * 4, hipMemcpyDeviceToHost));
+        }
+        else
+        {
+            r.hits.resize(q.n);
+            HIP_TRY(hipMemcpy(r.hits.data(), q.hits, (size_t)q.n * sizeof(cgs_hit), hipMemcpyDeviceToHost));
+            if (q.objects)
+            {
+                r.objects.resize(q.n);
+                HIP_TRY(hipMemcpy(r.objects.data(), q.objects, (size_t)q.n * 4, hipMemcpyDeviceToHost));
+            }
+        }
+    }
+    return CGS_OK;
+}
+
+// A candidate's results against the reference's: the largest relative t and
+// (u,v) differences over rays both hit on the same patch, and the rays whose
+// hit or miss, object, or occlusion differ, or whose patch differs with t
+// outside the limit (a hit on the neighbouring patch of a seam at the same
+// distance agrees; its (u,v) is on another patch and is not compared).
+void compareResults(const cgs_batch_job* jobs, uint32_t njobs, const std::vector<JobResults>& ref,
+                    const std::vector<JobResults>& got, float maxRelT, cgs_autotune_entry& e)
+{
+    double relT = 0.0, uv = 0.0;
+    uint64_t disagree = 0, total = 0;
+    for (uint32_t j = 0; j < njobs; j++)
+    {
+        const cgs_batch_job& q = jobs[j];
+        total += q.n;
+        if (q.any)
+        {
+            for (uint32_t k = 0; k < q.n; k++) disagree += ref[j].occluded[k] != got[j].occluded[k];
+            continue;
+        }
+        for (uint32_t k = 0; k < q.n; k++)
+        {
+            const cgs_hit& a = ref[j].hits[k];
+            const cgs_hit& b = got[j].hits[k];
+            const bool ha = a.patch != EMPTY && a.patch != CGS_PATCH_OVERFLOW;
+            const bool hb = b.patch != EMPTY && b.patch != CGS_PATCH_OVERFLOW;
+            if (!ha && !hb) continue;
+            if (ha != hb || (!ref[j].objects.empty() && ref[j].objects[k] != got[j].objects[k]))
+            {
+                disagree++;
+                continue;
+            }
+            const double dt = std::fabs((double)b.t - (double)a.t) / std::max(std::fabs((double)a.t), 1e-30);
+            if (a.patch != b.patch)
+            {
+                if (dt > maxRelT) disagree++;
+                continue;
+            }
+            relT = std::max(relT, dt);
+            uv = std::max(uv, std::max(std::fabs((double)b.u - a.u), std::fabs((double)b.v - a.v)));
+        }
+    }
+    e.max_rel_t = (float)relT;
+    e.max_uv = (float)uv;
+    e.disagree = total ? (float)((double)disagree / (double)total) : 0.0f;
+}
+} // namespace
+
+extern "C" {
+
+cgs_status cgs_batch_autotune_accuracy(cgs_ctx* ctx, const cgs_batch_job* jobs, uint32_t njobs,
+                                       const cgs_accuracy_limits* limits, cgs_trace_options* best,
+                                       cgs_autotune_entry* report, uint32_t report_cap, uint32_t* report_count)
+{
+    if (!ctx || !limits || (njobs && !jobs) || (report_cap && !report))
+        return fail(CGS_ERR_INVALID, "cgs_batch_autotune_accuracy: null argument");
+    if (!(limits->max_rel_t >= 0.0f) || !(limits->max_uv >= 0.0f) || !(limits->max_disagree >= 0.0f))
+        return fail(CGS_ERR_INVALID, "cgs_batch_autotune_accuracy: negative or NaN limit");
+    HIP_TRY(hipSetDevice(ctx->device));
+    std::lock_guard lk(ctx->mu);
+    // candidates: the reference's clipping, clipping with a pass cap, and
+    // subdivision at several depths, each also with the FP64 polish if allowed
+    std::vector<cgs_trace_options> cands;
+    for (uint32_t polish : {0u, 1u})
+    {
+        if (polish && !limits->allow_polish) break;
+        cands.push_back({CGS_METHOD_CLIPPING, 0.0f, 0, 0, polish, {}});
+        for (uint32_t it : {8u, 12u, 16u, 24u, 32u}) cands.push_back({CGS_METHOD_CLIPPING, 0.0f, 0, it, polish, {}});
+        for (uint32_t d : {4u, 5u, 6u, 8u, 10u, 12u}) cands.push_back({CGS_METHOD_SUBDIVISION, 0.0f, d, 0, polish, {}});
+    }
+    hipStream_t s = nullptr;
+    HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    HIP_TRY(hipEventCreate(&t0));
+    HIP_TRY(hipEventCreate(&t1));
+    cgs_status st = CGS_OK;
+    // the reference results: the reference's clipping, polished when allowed
+    std::vector<JobResults> ref, got;
+    {
+        TraceMode rm;
+        rm.polish = limits->allow_polish ? 1u : 0u;
+        st = traceBatch(ctx, jobs, njobs, ctx->batchChunk, ctx->batchBlocksPerCU, s, rm);
+        if (st == CGS_OK && hipStreamSynchronize(s) != hipSuccess) st = fail(CGS_ERR_HIP, "cgs_batch_autotune_accuracy");
+        if (st == CGS_OK) st = readJobResults(jobs, njobs, ref);
+    }
+    std::vector<cgs_autotune_entry> entries;
+    int pick = -1;
+    for (size_t c = 0; c < cands.size() && st == CGS_OK; c++)
+    {
+        TraceMode m;
+        if ((st = traceMode(&cands[c], m, "cgs_batch_autotune_accuracy")) != CGS_OK) break;
+        cgs_autotune_entry e {};
+        e.options = cands[c];
+        // one run for the results, then the faster of two timed runs
+        st = traceBatch(ctx, jobs, njobs, ctx->batchChunk, ctx->batchBlocksPerCU, s, m);
+        if (st == CGS_OK && hipStreamSynchronize(s) != hipSuccess) st = fail(CGS_ERR_HIP, "cgs_batch_autotune_accuracy");
+        if (st == CGS_OK) st = readJobResults(jobs, njobs, got);
+        if (st != CGS_OK) break;
+        compareResults(jobs, njobs, ref, got, limits->max_rel_t, e);
+        float bestMs = std::numeric_limits<float>::infinity();
+        for (int rep = 0; rep < 2 && st == CGS_OK; rep++)
+        {
+            hipError_t he = hipEventRecord(t0, s);
+            if (he == hipSuccess) st = traceBatch(ctx, jobs, njobs, ctx->batchChunk, ctx->batchBlocksPerCU, s, m);
+            if (st != CGS_OK) break;
+            if (he == hipSuccess) he = hipEventRecord(t1, s);
+            if (he == hipSuccess) he = hipEventSynchronize(t1);
+            float ms = 0.0f;
+            if (he == hipSuccess) he = hipEventElapsedTime(&ms, t0, t1);
+            if (he != hipSuccess) st = fail(CGS_ERR_HIP, hipGetErrorString(he));
+            bestMs = std::min(bestMs, ms);
+        }
+        e.ms = bestMs;
+        e.admissible = e.max_rel_t <= limits->max_rel_t && e.max_uv <= limits->max_uv &&
+                               e.disagree <= limits->max_disagree ? 1u : 0u;
+        entries.push_back(e);
+        if (e.admissible && (pick < 0 || e.ms < entries[pick].ms)) pick = (int)entries.size() - 1;
+    }
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    (void)hipStreamDestroy(s);
+    if (st != CGS_OK) return st;
+    if (report_count) *report_count = (uint32_t)entries.size();
+    for (size_t k = 0; k < entries.size() && k < report_cap; k++) report[k] = entries[k];
+    // the reference candidate agrees with the reference exactly, so one is always admissible
+    if (pick < 0) return fail(CGS_ERR_INVALID, "cgs_batch_autotune_accuracy: no candidate within the limits");
+    TraceMode m;
+    (void)traceMode(&entries[pick].options, m, "cgs_batch_autotune_accuracy");
+    ctx->batchMode = m;
+    ctx->batchOptions = entries[pick].options;
+    if (best) *best = entries[pick].options;
+    return CGS_OK;
+}
+
 } // extern "C"
 
 // ---------------------------------------------------------------------------
@@ -1552,22 +1783,37 @@ namespace
 
 using Clock = std::chrono::steady_clock;
 
+enum class RequestKind
+{
+    Closest,
+    Any,
+    Shade,
+};
+
 struct ServerRequest
 {
     uint64_t id = 0;
     uint32_t scene = 0, n = 0;
-    bool any = false;
+    RequestKind kind = RequestKind::Closest;
     const cgs_ray* rays = nullptr;
     cgs_hit* hits = nullptr;
     int32_t* objects = nullptr;
     uint32_t* occluded = nullptr;
+    float* color = nullptr;        // shaded requests
+    uint32_t* pixels = nullptr;
+    float eps = 0.0f;
+    int32_t depth = 0;
     size_t bytes = 0;              // device bytes while in flight
     Clock::time_point submitted;
-    uint64_t rayOff = 0;           // position in the slot's launch
+    uint64_t rayOff = 0;           // position in the slot's launch (among requests of its kernel)
+    uint64_t outOff = 0;           // byte offset of its results in the slot's output buffer
 };
 
-// Device bytes of a request: rays in, 16-B hit + 4-B object (or flag) out.
-inline size_t requestBytes(uint32_t n) { return (size_t)n * (sizeof(cgs_ray) + sizeof(cgs_hit) + 4); }
+// Result bytes per ray: a 16-B hit + a 4-B object id (or occlusion flag);
+// shaded requests add 12 B of colour and a 4-B packed pixel.
+inline size_t resultBytes(RequestKind k) { return sizeof(cgs_hit) + 4 + (k == RequestKind::Shade ? 16 : 0); }
+// Device bytes of a request: rays in, results out.
+inline size_t requestBytes(uint32_t n, RequestKind k) { return (size_t)n * (sizeof(cgs_ray) + resultBytes(k)); }
 
 struct PinnedBuffer
 {
@@ -1623,9 +1869,9 @@ struct ServerSlot
 {
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
-    DeviceBuffer rays, out, table, counter, stack, overflow;
-    size_t rayCap = 0, outCap = 0, tableCap = 0;
-    PinnedBuffer hRays, hOut, hTable;
+    DeviceBuffer rays, out, table, shadeTable, counter, stack, overflow;
+    size_t rayCap = 0, outCap = 0, tableCap = 0, shadeTableCap = 0;
+    PinnedBuffer hRays, hOut, hTable, hShadeTable;
     std::vector<ServerRequest> reqs;
     size_t bytes = 0;
     uint64_t total = 0;
@@ -1648,6 +1894,7 @@ struct cgs_server
     size_t budget = 0;
     uint32_t maxBatchRays = 0;
     uint32_t grid = 0, lanes = 0, chunk = 64;
+    uint32_t shadeGrid = 0;        // kShade's resident workgroups (at most grid: the stacks hold grid × 256 lanes)
     std::vector<ServerSlot> slots;
     std::mutex mu;
     std::condition_variable wake, finished;
@@ -1670,13 +1917,20 @@ struct cgs_server
             CopyList copies;
             for (const ServerRequest& r : sl.reqs)
             {
-                const uint8_t* o = base + r.rayOff * (sizeof(cgs_hit) + 4);
-                if (r.any)
-                    copies.add(r.occluded, o + (size_t)r.n * sizeof(cgs_hit), (size_t)r.n * 4);
+                // per request: hits, then object ids (or flags), then colours and pixels
+                const uint8_t* o = base + r.outOff;
+                const size_t n = r.n;
+                if (r.kind == RequestKind::Any)
+                    copies.add(r.occluded, o + n * sizeof(cgs_hit), n * 4);
                 else
                 {
-                    copies.add(r.hits, o, (size_t)r.n * sizeof(cgs_hit));
-                    if (r.objects) copies.add(r.objects, o + (size_t)r.n * sizeof(cgs_hit), (size_t)r.n * 4);
+                    if (r.hits) copies.add(r.hits, o, n * sizeof(cgs_hit));
+                    if (r.objects) copies.add(r.objects, o + n * sizeof(cgs_hit), n * 4);
+                }
+                if (r.kind == RequestKind::Shade)
+                {
+                    copies.add(r.color, o + n * (sizeof(cgs_hit) + 4), n * 12);
+                    if (r.pixels) copies.add(r.pixels, o + n * (sizeof(cgs_hit) + 16), n * 4);
                 }
             }
             copies.run();
@@ -1694,67 +1948,117 @@ struct cgs_server
         finished.notify_all();
     }
 
-    // One batched launch of the slot's requests: rays up, trace, results down.
+    // One batched launch of the slot's requests: rays up, then kTraceBatch
+    // over the closest and occlusion requests and kShade over the shaded
+    // ones (each numbering its own requests' rays from 0, with its own
+    // counter), results down.
     hipError_t launch(ServerSlot& sl)
     {
-        uint64_t total = 0;
+        uint64_t total = 0, outBytes = 0;
+        size_t nShade = 0;
         for (ServerRequest& r : sl.reqs)
         {
             r.rayOff = total;
+            r.outOff = outBytes;
             total += r.n;
+            outBytes += (uint64_t)r.n * resultBytes(r.kind);
+            nShade += r.kind == RequestKind::Shade;
         }
+        const size_t nTrace = sl.reqs.size() - nShade;
         sl.total = total;
-        const size_t outBytes = total * (sizeof(cgs_hit) + 4);
         hipError_t e = hipSuccess;
         if (sl.rayCap < total)
         {
             if ((e = sl.rays.alloc(total * sizeof(cgs_ray))) != hipSuccess) return e;
             sl.rayCap = total;
         }
-        if (sl.outCap < total)
+        if (sl.outCap < outBytes)
         {
             if ((e = sl.out.alloc(outBytes)) != hipSuccess) return e;
-            sl.outCap = total;
+            sl.outCap = outBytes;
         }
-        if (sl.tableCap < sl.reqs.size())
+        if (sl.tableCap < nTrace)
         {
-            if ((e = sl.table.alloc(sl.reqs.size() * sizeof(BatchJob))) != hipSuccess) return e;
-            sl.tableCap = sl.reqs.size();
+            if ((e = sl.table.alloc(nTrace * sizeof(BatchJob))) != hipSuccess) return e;
+            sl.tableCap = nTrace;
+        }
+        if (sl.shadeTableCap < nShade)
+        {
+            if ((e = sl.shadeTable.alloc(nShade * sizeof(ShadeJob))) != hipSuccess) return e;
+            sl.shadeTableCap = nShade;
         }
         if ((e = sl.hRays.reserve(total * sizeof(cgs_ray))) != hipSuccess) return e;
         if ((e = sl.hOut.reserve(outBytes)) != hipSuccess) return e;
-        if ((e = sl.hTable.reserve(sl.reqs.size() * sizeof(BatchJob))) != hipSuccess) return e;
+        if ((e = sl.hTable.reserve(std::max<size_t>(1, nTrace) * sizeof(BatchJob))) != hipSuccess) return e;
+        if ((e = sl.hShadeTable.reserve(std::max<size_t>(1, nShade) * sizeof(ShadeJob))) != hipSuccess) return e;
         cgs_ray* hr = (cgs_ray*)sl.hRays.p;
         BatchJob* jt = (BatchJob*)sl.hTable.p;
+        ShadeJob* js = (ShadeJob*)sl.hShadeTable.p;
         uint8_t* dOut = sl.out.as<uint8_t>();
+        cgs_ray* dRays = sl.rays.as<cgs_ray>();
         CopyList copies;
-        for (size_t k = 0; k < sl.reqs.size(); k++)
+        uint32_t traceRays = 0, shadeRays = 0, kt = 0, ks = 0;
+        for (const ServerRequest& r : sl.reqs)
         {
-            const ServerRequest& r = sl.reqs[k];
             copies.add(hr + r.rayOff, r.rays, (size_t)r.n * sizeof(cgs_ray));
-            BatchJob b {};
-            b.sc = views[r.scene];
-            b.rays = sl.rays.as<cgs_ray>() + r.rayOff;
-            uint8_t* o = dOut + r.rayOff * (sizeof(cgs_hit) + 4);
-            b.hits = r.any ? nullptr : (cgs_hit*)o;
-            b.objects = r.any ? nullptr : (int32_t*)(o + (size_t)r.n * sizeof(cgs_hit));
-            b.occluded = r.any ? (uint32_t*)(o + (size_t)r.n * sizeof(cgs_hit)) : nullptr;
-            b.n = r.n;
-            b.rayBase = (uint32_t)r.rayOff;
-            b.any = r.any ? 1u : 0u;
-            jt[k] = b;
+            uint8_t* o = dOut + r.outOff;
+            const size_t n = r.n;
+            if (r.kind == RequestKind::Shade)
+            {
+                ShadeJob j {};
+                j.sc = views[r.scene];
+                j.rays = dRays + r.rayOff;
+                j.hits = r.hits ? (cgs_hit*)o : nullptr;
+                j.objects = r.objects ? (int32_t*)(o + n * sizeof(cgs_hit)) : nullptr;
+                j.color = (float*)(o + n * (sizeof(cgs_hit) + 4));
+                j.pixels = r.pixels ? (uint32_t*)(o + n * (sizeof(cgs_hit) + 16)) : nullptr;
+                j.n = r.n;
+                j.rayBase = shadeRays;
+                j.eps = r.eps;
+                j.depth = r.depth;
+                js[ks++] = j;
+                shadeRays += r.n;
+            }
+            else
+            {
+                const bool any = r.kind == RequestKind::Any;
+                BatchJob b {};
+                b.sc = views[r.scene];
+                b.rays = dRays + r.rayOff;
+                b.hits = any ? nullptr : (cgs_hit*)o;
+                b.objects = any ? nullptr : (int32_t*)(o + n * sizeof(cgs_hit));
+                b.occluded = any ? (uint32_t*)(o + n * sizeof(cgs_hit)) : nullptr;
+                b.n = r.n;
+                b.rayBase = traceRays;
+                b.any = any ? 1u : 0u;
+                jt[kt++] = b;
+                traceRays += r.n;
+            }
         }
         copies.run();
         if ((e = hipMemcpyAsync(sl.rays.p, hr, total * sizeof(cgs_ray), hipMemcpyHostToDevice, sl.stream)) != hipSuccess)
             return e;
-        if ((e = hipMemcpyAsync(sl.table.p, jt, sl.reqs.size() * sizeof(BatchJob), hipMemcpyHostToDevice,
-                                sl.stream)) != hipSuccess)
-            return e;
-        if ((e = launchFill(sl.counter.as<uint32_t>(), 0u, 1, sl.stream)) != hipSuccess) return e;
-        if ((e = launchTraceBatch(sl.table.as<BatchJob>(), (uint32_t)sl.reqs.size(), (uint32_t)total, chunk,
-                                  sl.stack.as<float2>(), lanes, sl.counter.as<unsigned int>(), grid,
-                                  sl.overflow.as<unsigned long long>(), sl.stream)) != hipSuccess)
-            return e;
+        if ((e = launchFill(sl.counter.as<uint32_t>(), 0u, 2, sl.stream)) != hipSuccess) return e;
+        if (nTrace)
+        {
+            if ((e = hipMemcpyAsync(sl.table.p, jt, nTrace * sizeof(BatchJob), hipMemcpyHostToDevice, sl.stream)) !=
+                hipSuccess)
+                return e;
+            if ((e = launchTraceBatch(sl.table.as<BatchJob>(), (uint32_t)nTrace, traceRays, chunk, sl.stack.as<float2>(),
+                                      lanes, sl.counter.as<unsigned int>(), grid, sl.overflow.as<unsigned long long>(),
+                                      sl.stream)) != hipSuccess)
+                return e;
+        }
+        if (nShade)
+        {
+            if ((e = hipMemcpyAsync(sl.shadeTable.p, js, nShade * sizeof(ShadeJob), hipMemcpyHostToDevice,
+                                    sl.stream)) != hipSuccess)
+                return e;
+            if ((e = launchShade(nullptr, sl.shadeTable.as<ShadeJob>(), (uint32_t)nShade, shadeRays, chunk,
+                                 sl.stack.as<float2>(), lanes, sl.counter.as<unsigned int>() + 1, shadeGrid, nullptr,
+                                 sl.stream)) != hipSuccess)
+                return e;
+        }
         if ((e = hipMemcpyAsync(sl.hOut.p, sl.out.p, outBytes, hipMemcpyDeviceToHost, sl.stream)) != hipSuccess) return e;
         if ((e = hipEventRecord(sl.done, sl.stream)) != hipSuccess) return e;
         sl.busy = true;
@@ -1840,6 +2144,11 @@ cgs_status cgs_server_create(cgs_ctx* ctx, cgs_scene* const* scenes, uint32_t ns
         sv->chunk = ctx->batchChunk;
     }
     sv->lanes = sv->grid * 256;
+    {
+        uint32_t bpc = 1;
+        HIP_TRY(persistentGrid(PersistKernel::Shade, bpc));
+        sv->shadeGrid = std::min(sv->grid, bpc * (uint32_t)ctx->numCUs);
+    }
     sv->slots.resize(nslots);
     for (ServerSlot& sl : sv->slots)
     {
@@ -1856,33 +2165,23 @@ cgs_status cgs_server_create(cgs_ctx* ctx, cgs_scene* const* scenes, uint32_t ns
     return CGS_OK;
 }
 
-cgs_status cgs_server_submit(cgs_server* sv, uint32_t scene, const cgs_ray* rays, uint32_t n, int32_t any,
-                             cgs_hit* hits, int32_t* objects, uint32_t* occluded, uint64_t* id)
+// Queues a checked request (the submit entry points fill in its outputs).
+static cgs_status serverQueue(cgs_server* sv, ServerRequest& r, uint64_t* id, const char* who)
 {
-    if (!sv || !id) return fail(CGS_ERR_INVALID, "cgs_server_submit: null argument");
-    if (scene >= sv->scenes.size()) return fail(CGS_ERR_INVALID, "cgs_server_submit: no such scene");
-    if (n && (!rays || (any ? !occluded : !hits))) return fail(CGS_ERR_INVALID, "cgs_server_submit: missing buffers");
-    ServerRequest r;
-    r.scene = scene;
-    r.n = n;
-    r.any = any != 0;
-    r.rays = rays;
-    r.hits = hits;
-    r.objects = objects;
-    r.occluded = occluded;
-    r.bytes = requestBytes(n);
-    if (r.bytes > sv->budget) return fail(CGS_ERR_INVALID, "cgs_server_submit: request exceeds the byte budget");
+    if (r.scene >= sv->scenes.size()) return fail(CGS_ERR_INVALID, std::string(who) + ": no such scene");
+    r.bytes = requestBytes(r.n, r.kind);
+    if (r.bytes > sv->budget) return fail(CGS_ERR_INVALID, std::string(who) + ": request exceeds the byte budget");
     // a request must fit one launch's 32-bit work counter on its own, or it
     // would never be admitted and would block every request behind it
-    if (!counterFits(n, sv->grid, sv->chunk))
-        return fail(CGS_ERR_INVALID, "cgs_server_submit: request exceeds the 32-bit work counter of a launch");
+    if (!counterFits(r.n, sv->grid, sv->chunk))
+        return fail(CGS_ERR_INVALID, std::string(who) + ": request exceeds the 32-bit work counter of a launch");
     r.submitted = Clock::now();
     {
         std::lock_guard lk(sv->mu);
-        if (sv->stop) return fail(CGS_ERR_INVALID, "cgs_server_submit: server stopping");
+        if (sv->stop) return fail(CGS_ERR_INVALID, std::string(who) + ": server stopping");
         r.id = sv->nextId++;
         *id = r.id;
-        if (n == 0)
+        if (r.n == 0)
             sv->done[r.id] = {0, 1};
         else
         {
@@ -1892,6 +2191,42 @@ cgs_status cgs_server_submit(cgs_server* sv, uint32_t scene, const cgs_ray* rays
     }
     sv->wake.notify_one();
     return CGS_OK;
+}
+
+cgs_status cgs_server_submit(cgs_server* sv, uint32_t scene, const cgs_ray* rays, uint32_t n, int32_t any,
+                             cgs_hit* hits, int32_t* objects, uint32_t* occluded, uint64_t* id)
+{
+    if (!sv || !id) return fail(CGS_ERR_INVALID, "cgs_server_submit: null argument");
+    if (n && (!rays || (any ? !occluded : !hits))) return fail(CGS_ERR_INVALID, "cgs_server_submit: missing buffers");
+    ServerRequest r;
+    r.scene = scene;
+    r.n = n;
+    r.kind = any ? RequestKind::Any : RequestKind::Closest;
+    r.rays = rays;
+    r.hits = hits;
+    r.objects = objects;
+    r.occluded = occluded;
+    return serverQueue(sv, r, id, "cgs_server_submit");
+}
+
+cgs_status cgs_server_submit_shaded(cgs_server* sv, uint32_t scene, const cgs_ray* rays, uint32_t n,
+                                    const cgs_render_options* options, float* color, uint32_t* pixels, cgs_hit* hits,
+                                    int32_t* objects, uint64_t* id)
+{
+    if (!sv || !id || !options) return fail(CGS_ERR_INVALID, "cgs_server_submit_shaded: null argument");
+    if (n && (!rays || !color)) return fail(CGS_ERR_INVALID, "cgs_server_submit_shaded: missing buffers");
+    ServerRequest r;
+    r.scene = scene;
+    r.n = n;
+    r.kind = RequestKind::Shade;
+    r.rays = rays;
+    r.hits = hits;
+    r.objects = objects;
+    r.color = color;
+    r.pixels = pixels;
+    r.eps = options->eps;
+    r.depth = options->recursion_depth;
+    return serverQueue(sv, r, id, "cgs_server_submit_shaded");
 }
 
 int cgs_server_poll(cgs_server* sv, uint64_t id, uint64_t* latency_ns)

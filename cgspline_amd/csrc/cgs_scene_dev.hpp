@@ -97,6 +97,21 @@ struct BatchJob
     uint32_t n, rayBase, any, pad;
 };
 
+// One shading job: n world rays against one scene, each traced and shaded as
+// the render traces and shades a camera ray (RayTracer.cu:194-544).
+struct ShadeJob
+{
+    SceneDev sc;
+    const cgs_ray* rays;
+    float* color;           // 3 floats per ray (unclamped radiance)
+    uint32_t* pixels;       // packed sRGB per ray, optional
+    cgs_hit* hits;          // the first segment's closest hit, optional
+    int32_t* objects;       // its object id, optional
+    uint32_t n, rayBase;
+    float eps;
+    int32_t depth;
+};
+
 CGS_DEV BVHRef bvhRef(const SceneDev& sc, const DevBVH& d)
 {
     BVHRef b;

@@ -657,10 +657,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CGS_TRACE_W
 // multiple of 64) from one counter, so small and large jobs share the grid
 // without a launch per scene.  Each lane finds its job by binary search over
 // the jobs' first ray numbers and runs the same per-ray query as
-// kTraceClosest / kTraceAny.
+// kTraceClosest / kTraceAny, with the same leaf methods and polish.
+template<int METHOD, bool POLISH>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CGS_TRACE_WAVES))) void kTraceBatch(
     const BatchJob* __restrict__ jobs, uint32_t njobs, uint32_t total, uint32_t chunk, Persist ps,
-    unsigned long long* __restrict__ overflowCount)
+    unsigned long long* __restrict__ overflowCount, TraceParams tp)
 {
     const uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x;
     const ClipStack stk = clipStackFor(ps.stack, gl, ps.lanes);
@@ -692,14 +693,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CGS_TRACE_W
             bool overflow = false;
             if (J.any)
             {
-                const bool occ = sceneAny(J.sc, none, rg, {ray.ox, ray.oy, ray.oz}, {ray.dx, ray.dy, ray.dz}, ray.tmax, stk,
-                                          overflow);
+                const bool occ = sceneAny<METHOD>(J.sc, none, rg, {ray.ox, ray.oy, ray.oz}, {ray.dx, ray.dy, ray.dz},
+                                                  ray.tmax, stk, overflow, tp);
                 J.occluded[k] = overflow ? 2u : (occ ? 1u : 0u);
             }
             else
             {
-                const ClosestHit h = sceneClosest(J.sc, none, rg, {ray.ox, ray.oy, ray.oz}, {ray.dx, ray.dy, ray.dz},
-                                                  ray.tmax, stk, overflow);
+                ClosestHit h = sceneClosest<METHOD>(J.sc, none, rg, {ray.ox, ray.oy, ray.oz}, {ray.dx, ray.dy, ray.dz},
+                                                    ray.tmax, stk, overflow, tp);
+                if constexpr (POLISH)
+                    if (h.object >= 0 && !overflow)
+                    {
+                        const DevObject& ob = J.sc.objects[h.object];
+                        const DevBVH surf = J.sc.surfaces[ob.surface];
+                        polishHit64(J.sc.patches + 16 * (size_t)(surf.patchBase + h.patch), ob.w2l,
+                                    {ray.ox, ray.oy, ray.oz}, {ray.dx, ray.dy, ray.dz}, h.t, h.u, h.v);
+                    }
                 cgs_hit out;
                 out.t = h.t;
                 out.u = h.object < 0 ? 0.0f : h.u;
@@ -1060,6 +1069,88 @@ CGS_DEV uint32_t packSRGB(float r, float g, float b)
 
 CGS_DEV float saturate(float x) { return x > 0.0f ? (x < 1.0f ? x : 1.0f) : 0.0f; }
 
+// The radiance carried back along one ray: the render kernel's iterative
+// trace loop with Context::trace / miss / closestHit (RayTracer.cu:194-294,
+// 385-544, non-Monte-Carlo): closest hit, ambient + one shadow ray per light
+// with the PBR terms, then the reflection bounce while depth remains and the
+// attenuation exceeds 2^-8.  The first segment ends at rmax, bounces at
+// infinity.  eps is read from device memory per use (the traversal loops need
+// the registers a kernel argument would hold for the whole launch).  With
+// RECORD, the first segment's hit is returned in `first`.
+template<bool RECORD>
+CGS_DEV V3 rayRadiance(const SceneDev& sc, const NodeCache& nc, const Rings& rg, V3 ro, V3 rd, float rmax, int depth,
+                       const float* eps, const ClipStack& stk, bool& overflow, uint32_t& nShadow,
+                       uint32_t& nSecondary, ClosestHit& first)
+{
+    V3 acc {0.0f, 0.0f, 0.0f};
+    V3 att {1.0f, 1.0f, 1.0f};
+    bool primary = true;
+    while (true)
+    {
+        if (!primary) nSecondary++;
+        ClosestHit h = sceneClosest(sc, nc, rg, ro, rd, rmax, stk, overflow);
+        if (RECORD && primary) first = h;
+        primary = false;
+        if (h.object < 0)
+        {
+            acc = add3(acc, mulv3({sc.background[0], sc.background[1], sc.background[2]}, att));
+            break;
+        }
+        const DevObject& ob = sc.objects[h.object];
+        const float4 m0 = ob.mat0, m1 = ob.mat1;
+        const float metal = m0.w, rough = m1.w;
+        const V3 spec {m1.x, m1.y, m1.z};
+        const float invPi = 0x1.45f306p-2f; // std::numbers::inv_pi_v<float>
+        const V3 diffuse = mul3(1.0f - metal, mul3(invPi, {m0.x, m0.y, m0.z}));
+        V3 col = mulv3({sc.ambient[0], sc.ambient[1], sc.ambient[2]}, diffuse);
+        V3 V = neg3(rd);
+        const DevBVH surf = sc.surfaces[ob.surface];
+        V3 N = versor3(normalMat(sc.patches + 16 * (size_t)(surf.patchBase + h.patch), h.u, h.v));
+        M4 W;
+        W.c[0] = ob.w2l[0]; W.c[1] = ob.w2l[1]; W.c[2] = ob.w2l[2]; W.c[3] = ob.w2l[3];
+        N = versor3(m3tmul(W, N));
+        float dotNV = dot3(N, V);
+        if (dotNV < 0.0f)
+        {
+            N = neg3(N);
+            dotNV = -dotNV;
+        }
+        V3 Pt = add3(ro, mul3(h.t, rd));
+        for (uint32_t li = 0; li < sc.nlight; li++)
+        {
+            const DevLight lt = sc.lights[li];
+            float d;
+            V3 L;
+            if (!lightVector(d, L, Pt, lt)) continue;
+            const float dotNL = dot3(N, L);
+            if (dotNL < 1e-14f) continue;
+            nShadow++;
+            if (sceneAny(sc, nc, rg, add3(Pt, mul3(*eps, L)), L, d, stk, overflow)) continue;
+            V3 H = versor3(add3(L, V));
+            float dotHN = dot3(H, N);
+            float dotHL = dot3(H, L);
+            V3 I = lightColor(d, lt);
+            V3 bs = mul3(microfacet(dotNV, dotNL, dotHN, rough), schlick(spec, dotHL));
+            col = add3(col, mulv3(mul3(dotNL, I), add3(diffuse, mul3(metal, bs))));
+        }
+        acc = add3(acc, mulv3(mul3(0x1.921fb6p+1f, col), att));
+        if (depth <= 0) break;
+        V3 refl = schlick(spec, dotNV);
+        V3 R = sub3(neg3(V), mul3(2.0f * -dotNV, N));
+        float g = Gsmith(dotNV, dotNV, rough);
+        float rr = 1.0f - rough;
+        V3 brdf = mul3(rr * rr, mul3(dotNV, add3(diffuse, mul3(metal, mul3(g, refl)))));
+        V3 a = mulv3(att, brdf);
+        if (max3(a) <= 0x1p-8f) break;
+        ro = add3(Pt, mul3(*eps, R));
+        rd = R;
+        rmax = __builtin_inff();
+        att = a;
+        depth = depth - 1;
+    }
+    return acc;
+}
+
 // Occupancy target of the render megakernel (waves per SIMD): the compiler
 // keeps VGPR + AGPR use within 512 / CGS_RENDER_WAVES.  On the 1M-patch frame
 // three waves per SIMD (168 registers) measured 39.5 ms, two waves 44.5 ms
@@ -1096,77 +1187,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CGS_RENDER_
         if (i >= op.width || j >= op.height) continue;
         bool overflow = false;
         nPrimary++;
-        V3 acc {0.0f, 0.0f, 0.0f};
         // pixelRayDirection (RayTracer.cu:59-71)
         float4 P = make_float4(cam.tlx + cam.hdx * (2.0f * (float)i), cam.tly - cam.hdy * (2.0f * (float)j), -1.0f, 1.0f);
-        V3 ro = camPos;
-        V3 rd = versor3(sub3(project4(m4mul(C, P)), camPos));
-        float rmax = __builtin_inff();
-        V3 att {1.0f, 1.0f, 1.0f};
-        int depth = op.depth;
-        bool first = true;
-        while (true)
-        {
-            if (!first) nSecondary++;
-            first = false;
-            ClosestHit h = sceneClosest(sc, nc, rg, ro, rd, rmax, stk, overflow);
-            if (h.object < 0)
-            {
-                acc = add3(acc, mulv3({sc.background[0], sc.background[1], sc.background[2]}, att));
-                break;
-            }
-            const DevObject& ob = sc.objects[h.object];
-            const float4 m0 = ob.mat0, m1 = ob.mat1;
-            const float metal = m0.w, rough = m1.w;
-            const V3 spec {m1.x, m1.y, m1.z};
-            const float invPi = 0x1.45f306p-2f; // std::numbers::inv_pi_v<float>
-            const V3 diffuse = mul3(1.0f - metal, mul3(invPi, {m0.x, m0.y, m0.z}));
-            V3 col = mulv3({sc.ambient[0], sc.ambient[1], sc.ambient[2]}, diffuse);
-            V3 V = neg3(rd);
-            const DevBVH surf = sc.surfaces[ob.surface];
-            V3 N = versor3(normalMat(sc.patches + 16 * (size_t)(surf.patchBase + h.patch), h.u, h.v));
-            M4 W;
-            W.c[0] = ob.w2l[0]; W.c[1] = ob.w2l[1]; W.c[2] = ob.w2l[2]; W.c[3] = ob.w2l[3];
-            N = versor3(m3tmul(W, N));
-            float dotNV = dot3(N, V);
-            if (dotNV < 0.0f)
-            {
-                N = neg3(N);
-                dotNV = -dotNV;
-            }
-            V3 Pt = add3(ro, mul3(h.t, rd));
-            for (uint32_t li = 0; li < sc.nlight; li++)
-            {
-                const DevLight lt = sc.lights[li];
-                float d;
-                V3 L;
-                if (!lightVector(d, L, Pt, lt)) continue;
-                const float dotNL = dot3(N, L);
-                if (dotNL < 1e-14f) continue;
-                nShadow++;
-                if (sceneAny(sc, nc, rg, add3(Pt, mul3(ra->op.eps, L)), L, d, stk, overflow)) continue;
-                V3 H = versor3(add3(L, V));
-                float dotHN = dot3(H, N);
-                float dotHL = dot3(H, L);
-                V3 I = lightColor(d, lt);
-                V3 bs = mul3(microfacet(dotNV, dotNL, dotHN, rough), schlick(spec, dotHL));
-                col = add3(col, mulv3(mul3(dotNL, I), add3(diffuse, mul3(metal, bs))));
-            }
-            acc = add3(acc, mulv3(mul3(0x1.921fb6p+1f, col), att));
-            if (depth <= 0) break;
-            V3 refl = schlick(spec, dotNV);
-            V3 R = sub3(neg3(V), mul3(2.0f * -dotNV, N));
-            float g = Gsmith(dotNV, dotNV, rough);
-            float rr = 1.0f - rough;
-            V3 brdf = mul3(rr * rr, mul3(dotNV, add3(diffuse, mul3(metal, mul3(g, refl)))));
-            V3 a = mulv3(att, brdf);
-            if (max3(a) <= 0x1p-8f) break;
-            ro = add3(Pt, mul3(ra->op.eps, R));
-            rd = R;
-            rmax = __builtin_inff();
-            att = a;
-            depth = depth - 1;
-        }
+        const V3 rd = versor3(sub3(project4(m4mul(C, P)), camPos));
+        ClosestHit unused;
+        const V3 acc = rayRadiance<false>(sc, nc, rg, camPos, rd, __builtin_inff(), op.depth, &ra->op.eps, stk, overflow,
+                                          nShadow, nSecondary, unused);
         const uint32_t q = j * ra->op.width + i;
         if (float* color = ra->color)
         {
@@ -1194,6 +1220,86 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CGS_RENDER_
     workEnd();
 }
 
+// Shaded ray queries: each lane traces and shades one given world ray as the
+// render does a camera ray (rayRadiance), over one or many jobs.  Rays of all
+// jobs are numbered consecutively and waves take chunks of `chunk` rays from
+// one counter; each lane finds its job by binary search over the jobs' first
+// ray numbers.  With ONE_SCENE every job is on scene `sc0` (a kernel
+// argument, like the render's) and the top-level BVH's upper levels are
+// staged in LDS as in the render; otherwise each job's own scene is used.
+template<bool ONE_SCENE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CGS_RENDER_WAVES))) void kShade(
+    SceneDev sc0, const ShadeJob* __restrict__ jobs, uint32_t njobs, uint32_t total, uint32_t chunk, Persist ps,
+    unsigned long long* __restrict__ counts)
+{
+    __shared__ uint32_t ringLds[kRingWords];
+    NodeCache nc {nullptr, 0};
+    if constexpr (ONE_SCENE)
+    {
+        __shared__ float4 topLds[4 * CGS_TOP_LDS_KEYS];
+        nc = stageTopLevel(sc0, topLds);
+    }
+    const Rings rg = laneRings(ringLds);
+    const uint32_t gl = blockIdx.x * blockDim.x + threadIdx.x;
+    const ClipStack stk = clipStackFor(ps.stack, gl, ps.lanes);
+    uint32_t nPrimary = 0, nShadow = 0, nSecondary = 0, nOverflow = 0;
+    while (true)
+    {
+        uint32_t base = 0;
+        if ((threadIdx.x & 63) == 0) base = atomicAdd(ps.counter, chunk);
+        base = __shfl(base, 0, 64);
+        if (base >= total) break;
+        const uint32_t end = base + chunk < total ? base + chunk : total;
+        for (uint32_t r0 = base; r0 < end; r0 += 64)
+        {
+            const uint32_t r = r0 + (threadIdx.x & 63);
+            if (r >= end) continue;
+            uint32_t lo = 0, hi = njobs;  // last job with rayBase <= r
+            while (hi - lo > 1)
+            {
+                const uint32_t mid = (lo + hi) / 2;
+                if (jobs[mid].rayBase <= r) lo = mid;
+                else hi = mid;
+            }
+            const ShadeJob& J = jobs[lo];
+            const uint32_t k = r - J.rayBase;
+            const cgs_ray ray = J.rays[k];
+            bool overflow = false;
+            nPrimary++;
+            ClosestHit h;
+            V3 acc;
+            if constexpr (ONE_SCENE)
+                acc = rayRadiance<true>(sc0, nc, rg, {ray.ox, ray.oy, ray.oz}, {ray.dx, ray.dy, ray.dz}, ray.tmax,
+                                        J.depth, &J.eps, stk, overflow, nShadow, nSecondary, h);
+            else
+                acc = rayRadiance<true>(J.sc, nc, rg, {ray.ox, ray.oy, ray.oz}, {ray.dx, ray.dy, ray.dz}, ray.tmax,
+                                        J.depth, &J.eps, stk, overflow, nShadow, nSecondary, h);
+            J.color[3 * (size_t)k] = acc.x;
+            J.color[3 * (size_t)k + 1] = acc.y;
+            J.color[3 * (size_t)k + 2] = acc.z;
+            if (uint32_t* px = J.pixels) px[k] = packSRGB(saturate(acc.x), saturate(acc.y), saturate(acc.z));
+            if (cgs_hit* hits = J.hits)
+            {
+                cgs_hit out;
+                out.t = h.t;
+                out.u = h.object < 0 ? 0.0f : h.u;
+                out.v = h.object < 0 ? 0.0f : h.v;
+                out.patch = overflow ? CGS_PATCH_OVERFLOW : (h.object < 0 ? EMPTY : h.patch);
+                hits[k] = out;
+            }
+            if (int32_t* objects = J.objects) objects[k] = h.object;
+            nOverflow += overflow;
+        }
+    }
+    if (counts)
+    {
+        addWave(counts + 0, nPrimary);
+        addWave(counts + 1, nShadow);
+        addWave(counts + 2, nSecondary);
+        addWave(counts + 3, nOverflow);
+    }
+}
+
 } // namespace cgs
 
 // ---------------------------------------------------------------------------
@@ -1215,8 +1321,9 @@ hipError_t persistentGrid(PersistKernel k, uint32_t& blocksPerCU)
     case PersistKernel::TraceAny: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kTraceAny<Clip>, 256, 0); break;
     case PersistKernel::ClipBatch: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kClipBatch, 256, 0); break;
     case PersistKernel::Render: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kRender, 256, 0); break;
-    case PersistKernel::Batch: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kTraceBatch, 256, 0); break;
+    case PersistKernel::Batch: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kTraceBatch<Clip, false>, 256, 0); break;
     case PersistKernel::Pick: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kPick<false>, 256, 0); break;
+    case PersistKernel::Shade: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kShade<true>, 256, 0); break;
     }
     blocksPerCU = n > 0 ? (uint32_t)n : 1u;
     return e;
@@ -1272,11 +1379,27 @@ hipError_t launchTraceAny(const SceneDev& sc, const PersistArgs& pa, const cgs_r
 }
 
 hipError_t launchTraceBatch(const BatchJob* jobs, uint32_t njobs, uint32_t total, uint32_t chunk, float2* stack,
-                            uint32_t lanes, unsigned int* counter, uint32_t grid, unsigned long long* overflow, hipStream_t s)
+                            uint32_t lanes, unsigned int* counter, uint32_t grid, unsigned long long* overflow, hipStream_t s,
+                            const TraceMode& m)
 {
     if (total == 0) return hipSuccess;
     Persist ps {stack, lanes, counter};
-    hipLaunchKernelGGL(kTraceBatch, dim3(grid), dim3(256), 0, s, jobs, njobs, total, chunk, ps, overflow);
+    const TraceParams tp = traceParams(m);
+#define CGS_LAUNCH_BATCH(M, P) \
+    hipLaunchKernelGGL((kTraceBatch<M, P>), dim3(grid), dim3(256), 0, s, jobs, njobs, total, chunk, ps, overflow, tp)
+    if (m.method == CGS_METHOD_SUBDIVISION)
+    {
+        if (m.polish) CGS_LAUNCH_BATCH(Subdiv, true);
+        else CGS_LAUNCH_BATCH(Subdiv, false);
+    }
+    else if (m.maxIters || m.tol != kClipTol)
+    {
+        if (m.polish) CGS_LAUNCH_BATCH(ClipCapped, true);
+        else CGS_LAUNCH_BATCH(ClipCapped, false);
+    }
+    else if (m.polish) CGS_LAUNCH_BATCH(Clip, true);
+    else CGS_LAUNCH_BATCH(Clip, false);
+#undef CGS_LAUNCH_BATCH
     return hipGetLastError();
 }
 
@@ -1311,6 +1434,20 @@ hipError_t launchSubpatch(const float* patches, const float* ranges, uint32_t n,
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(kSubpatch, dim3(blocks(n, 256)), dim3(256), 0, s, (const float4*)patches, (const float4*)ranges,
                        n, (float4*)out);
+    return hipGetLastError();
+}
+
+hipError_t launchShade(const SceneDev* one, const ShadeJob* jobs, uint32_t njobs, uint32_t total, uint32_t chunk,
+                       float2* stack, uint32_t lanes, unsigned int* counter, uint32_t grid, unsigned long long* counts,
+                       hipStream_t s)
+{
+    if (total == 0 || njobs == 0) return hipSuccess;
+    Persist ps {stack, lanes, counter};
+    if (one)
+        hipLaunchKernelGGL(kShade<true>, dim3(grid), dim3(256), 0, s, *one, jobs, njobs, total, chunk, ps, counts);
+    else
+        hipLaunchKernelGGL(kShade<false>, dim3(grid), dim3(256), 0, s, SceneDev {}, jobs, njobs, total, chunk, ps,
+                           counts);
     return hipGetLastError();
 }
 

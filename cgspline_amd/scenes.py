@@ -485,13 +485,31 @@ def from_dl(path, width=320, height=240, depth=2, fovy_deg=60.0) -> Workload:
                                                     f"{r.scene.npatch} patches, {r.scene.lights.shape[0]} lights")
 
 
+_libm = None
+
+
+def _tanf(x) -> np.float32:
+    """The C library's single-precision tangent, which the render's camera
+    setup calls on the host (std::tan of a float, RayTracer.cu:141-163):
+    it is not always the correctly rounded value (tan(pi/6 in float32)
+    rounds up, where the double tangent rounded to float32 rounds down)."""
+    global _libm
+    if _libm is None:
+        import ctypes
+        import ctypes.util
+        _libm = ctypes.CDLL(ctypes.util.find_library("m") or "libm.so.6")
+        _libm.tanf.restype = ctypes.c_float
+        _libm.tanf.argtypes = [ctypes.c_float]
+    return np.float32(_libm.tanf(float(np.float32(x))))
+
+
 def camera_rays(cam: Camera, rows=None) -> np.ndarray:
     """World rays (n, 8) float32 (o xyz, tmax, d xyz, 0) through pixel centres,
     in float32 with the reference's camera formula (RayTracer.cu:59-71,
     141-163)."""
     f32 = np.float32
     w, h = cam.width, cam.height
-    half_h = f32(math.tan(f32(0.5) * f32(cam.fovy)))
+    half_h = _tanf(f32(0.5) * f32(cam.fovy))
     half_dy = f32(half_h / f32(h))
     half_dx = half_dy
     tl_x = f32(-half_dx * f32(w - 1))

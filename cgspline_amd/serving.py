@@ -44,6 +44,22 @@ class Server:
         self.live[rid.value] = rec
         return rid.value
 
+    def submit_shaded(self, scene: int, rays: np.ndarray, eps: float = 1e-4, depth: int = 0) -> int:
+        """A shaded request (cgs_server_submit_shaded): the record gets
+        "color" (n,3), "pixels" (n,), "hits" (n,4) and "objects" (n,)."""
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        n = rays.shape[0]
+        rec = {"rays": rays, "any": False, "shaded": True, "color": np.empty((n, 3), np.float32),
+               "pixels": np.empty(n, np.uint32), "hits": np.empty((n, 4), np.float32),
+               "objects": np.empty(n, np.int32)}
+        op = _abi.RenderOptions(eps, depth, 0)
+        rid = C.c_uint64()
+        check(lib().cgs_server_submit_shaded(self.h, scene, rays.ctypes.data, n, C.byref(op), rec["color"].ctypes.data,
+                                             rec["pixels"].ctypes.data, rec["hits"].ctypes.data,
+                                             rec["objects"].ctypes.data, C.byref(rid)), "cgs_server_submit_shaded")
+        self.live[rid.value] = rec
+        return rid.value
+
     def poll(self, rid: int):
         """None while pending, else (record, latency in seconds)."""
         lat = C.c_uint64()

@@ -171,6 +171,32 @@ typedef struct cgs_trace_options
     uint32_t reserved[3];
 } cgs_trace_options;
 
+/* Accuracy limits of cgs_batch_autotune_accuracy, against the reference's
+ * clipping (polished when allow_polish is set, so the reference is then the
+ * FP64-refined root): the largest relative t difference and the largest
+ * (u,v) difference over rays hit on the same patch, and the fraction of rays
+ * whose hit or miss, object, occlusion, or (beyond max_rel_t in t) patch
+ * differ. */
+typedef struct cgs_accuracy_limits
+{
+    float max_rel_t;
+    float max_uv;
+    float max_disagree;
+    uint32_t allow_polish;
+} cgs_accuracy_limits;
+
+/* One candidate of cgs_batch_autotune_accuracy: its options, the batched
+ * launch's time (ms) and its differences from the reference. */
+typedef struct cgs_autotune_entry
+{
+    cgs_trace_options options;
+    float ms;
+    float max_rel_t;
+    float max_uv;
+    float disagree;
+    uint32_t admissible;
+} cgs_autotune_entry;
+
 /* Per-launch ray counters written by cgs_render (device memory, 4 × u64):
  * primary, shadow, secondary, overflowed clip stacks. */
 typedef struct cgs_ray_counts
@@ -455,6 +481,24 @@ cgs_status cgs_render(cgs_ctx* ctx, cgs_scene* scene, const cgs_camera* camera,
 /* RayTracer::running() (rt/RayTracer.h:45-48): 1 while the last render runs. */
 int cgs_render_running(cgs_ctx* ctx);
 
+/* Shaded ray queries: each of n world rays is traced and shaded as the render
+ * kernel traces and shades a camera ray (the trace loop of RayTracer.cu:
+ * 194-294 with Context::trace / miss / closestHit, :385-544, non-Monte-Carlo;
+ * the reference starts it only at camera rays): closest hit up to ray.tmax,
+ * background on a miss, else ambient plus one shadow ray per light with the
+ * PBR terms, then the reflection bounce (to infinity) while
+ * options->recursion_depth remains and the attenuation exceeds 2^-8.
+ * Directions are used as given: unit directions give the render's shading,
+ * as its camera rays are unit.  options: eps and recursion_depth
+ * (frame_stride unused).  Device pointers: color (3 floats per ray,
+ * unclamped radiance), pixels (nullable: the frame's packed sRGB), hits and
+ * objects (nullable: the first segment's closest hit as cgs_trace_closest
+ * reports it; patch CGS_PATCH_OVERFLOW when any segment overflowed its clip
+ * stack); counts (nullable, accumulated): rays of each kind as cgs_render. */
+cgs_status cgs_shade(cgs_ctx* ctx, cgs_scene* scene, const cgs_ray* rays, uint32_t n, const cgs_render_options* options,
+                     float* color, uint32_t* pixels, cgs_hit* hits, int32_t* objects, cgs_ray_counts* counts,
+                     void* stream);
+
 /* Work-event totals of the renders on `ctx`'s device since the last reset,
  * for the work model (no counterpart in the reference): cull-tree node and
  * element boxes tested, BVH inner nodes visited, perfect-hash probes, leaf
@@ -482,6 +526,25 @@ cgs_status cgs_trace_batch(cgs_ctx* ctx, const cgs_batch_job* jobs, uint32_t njo
 cgs_status cgs_batch_create(cgs_ctx* ctx, const cgs_batch_job* jobs, uint32_t njobs, cgs_batch** out);
 cgs_status cgs_batch_launch(cgs_batch* batch, void* stream);
 cgs_status cgs_batch_destroy(cgs_batch* batch);
+/* Leaf method of the context's later batched calls and batches
+ * (cgs_trace_options' meaning; null: the reference's clipping).  Closest-hit
+ * jobs are polished when the options ask for it. */
+cgs_status cgs_batch_set_trace_options(cgs_ctx* ctx, const cgs_trace_options* options);
+cgs_status cgs_batch_get_trace_options(cgs_ctx* ctx, cgs_trace_options* options);
+/* Accuracy-gated tuning of the leaf method (no counterpart in the reference):
+ * runs the jobs with the reference's clipping (the reference results,
+ * polished when limits->allow_polish), then with each candidate: clipping
+ * with a pass cap of 8, 12, 16, 24 or 32, subdivision at depth 4, 5, 6, 8,
+ * 10 or 12, and the reference's clipping, each also polished when allowed;
+ * times each (the faster of two launches at the context's launch shape),
+ * compares its results with the reference's, and keeps the fastest candidate
+ * within the limits as the context's batched leaf method (the reference
+ * candidate always is).  report (up to report_cap entries, nullable) and
+ * report_count receive every candidate.  The jobs' outputs are overwritten
+ * with valid results of some candidate. */
+cgs_status cgs_batch_autotune_accuracy(cgs_ctx* ctx, const cgs_batch_job* jobs, uint32_t njobs,
+                                       const cgs_accuracy_limits* limits, cgs_trace_options* best,
+                                       cgs_autotune_entry* report, uint32_t report_cap, uint32_t* report_count);
 /* Launch shape of the context's later batched calls and batches. */
 cgs_status cgs_batch_set_config(cgs_ctx* ctx, const cgs_batch_config* config);
 cgs_status cgs_batch_get_config(cgs_ctx* ctx, cgs_batch_config* config);
@@ -514,6 +577,14 @@ cgs_status cgs_server_create(cgs_ctx* ctx, cgs_scene* const* scenes, uint32_t ns
  * one launch's 32-bit work counter (n plus one chunk per resident wave). */
 cgs_status cgs_server_submit(cgs_server* server, uint32_t scene, const cgs_ray* rays, uint32_t n, int32_t any,
                              cgs_hit* hits, int32_t* objects, uint32_t* occluded, uint64_t* id);
+/* Queues a shaded request: n host rays traced and shaded as cgs_shade does,
+ * with options' eps and recursion_depth, into host color (3 floats per ray)
+ * and, when not null, pixels, hits and objects.  Shaded requests join the
+ * same launches as the others (a second kernel of the launch).  Same
+ * failures as cgs_server_submit. */
+cgs_status cgs_server_submit_shaded(cgs_server* server, uint32_t scene, const cgs_ray* rays, uint32_t n,
+                                    const cgs_render_options* options, float* color, uint32_t* pixels, cgs_hit* hits,
+                                    int32_t* objects, uint64_t* id);
 /* 1 when request `id` is done (outputs written), 0 while pending, -1 when
  * its launch failed or the id is unknown or already collected; latency_ns
  * (nullable): submit to done.  A result of 1 or -1 collects the request: the

@@ -708,6 +708,43 @@ int orc_scene_render(const orc_scene* h, const orc_camera* c, float eps, int dep
     return 0;
 }
 
+// The render's trace loop from n given world rays (8 floats each, as
+// orc_scene_trace): color3 unclamped colours, pixel packed sRGB (nullable),
+// hits3 / patch / object the first segment's closest hit (nullable).
+int orc_scene_shade(const orc_scene* h, const float* rays, uint64_t n, float eps, int depth, float* color3,
+    uint32_t* pixel, float* hits3, uint32_t* patch, int32_t* object, uint64_t* raycounts3, int threads)
+{
+    RenderOptions op {eps, depth};
+    std::mutex mu;
+    RayCounts rc;
+    parallelFor(n, threads, [&](uint64_t a, uint64_t b) {
+        RayCounts r;
+        for (uint64_t k = a; k < b; k++)
+        {
+            const float* ry = rays + 8 * k;
+            const Ray ray {{ry[0], ry[1], ry[2]}, {ry[4], ry[5], ry[6]}, ry[3]};
+            FirstHit fh;
+            const f3 col = shadeRay(h->s, ray, op, nullptr, &r, &fh);
+            color3[3 * k] = col.x;
+            color3[3 * k + 1] = col.y;
+            color3[3 * k + 2] = col.z;
+            if (pixel) pixel[k] = packColor(col);
+            if (hits3)
+            {
+                hits3[3 * k] = fh.t;
+                hits3[3 * k + 1] = fh.u;
+                hits3[3 * k + 2] = fh.v;
+            }
+            if (patch) patch[k] = fh.patch;
+            if (object) object[k] = fh.object;
+        }
+        std::lock_guard lk(mu);
+        rc.primary += r.primary; rc.shadow += r.shadow; rc.secondary += r.secondary;
+    });
+    if (raycounts3) { raycounts3[0] = rc.primary; raycounts3[1] = rc.shadow; raycounts3[2] = rc.secondary; }
+    return 0;
+}
+
 // CPURayTracer restatement (the CPU baseline): renders rows [row0,row1) with
 // `threads` workers over 32×32 tiles (rows row0, row0+stride, ...).
 // Returns wall seconds; ray counts out.

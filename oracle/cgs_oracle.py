@@ -105,6 +105,7 @@ def lib():
         L.orc_set_clip_stack_cap.argtypes = [C.c_int]
         L.orc_scene_trace_ex.argtypes = [_P, _P, _U64, C.c_int, _P, _P, _P, _P, C.c_int, C.POINTER(TraceOpts)]
         L.orc_pixel_rays.argtypes = [C.POINTER(CameraDesc), _U32, _U32, _P, _U64, _P]
+        L.orc_scene_shade.argtypes = [_P, _P, _U64, _F32, C.c_int, _P, _P, _P, _P, _P, _P, C.c_int]
         _lib = L
     return _lib
 
@@ -395,6 +396,23 @@ class OracleScene:
         lib().orc_scene_render(self.h, C.byref(c), eps, depth, w, h, _ptr(color), _ptr(frame), C.byref(wk),
                                _ptr(rc), th, r0, r1)
         return color, frame, wk.asdict(), dict(primary=int(rc[0]), shadow=int(rc[1]), secondary=int(rc[2]))
+
+    def shade(self, rays, eps=1e-4, depth=0, threads=None):
+        """The render's trace loop from given world rays (n,8): returns
+        (color (n,3), packed pixels (n,), first-segment hits (n,3) t,u,v,
+        patch, object, ray counts)."""
+        rays = np.ascontiguousarray(rays, np.float32)
+        n = rays.shape[0]
+        color = np.zeros((n, 3), np.float32)
+        px = np.zeros(n, np.uint32)
+        hits = np.zeros((n, 3), np.float32)
+        patch = np.zeros(n, np.uint32)
+        obj = np.zeros(n, np.int32)
+        rc = np.zeros(3, np.uint64)
+        th = threads or min(16, os.cpu_count() or 1)
+        lib().orc_scene_shade(self.h, _ptr(rays), n, eps, depth, _ptr(color), _ptr(px), _ptr(hits), _ptr(patch),
+                              _ptr(obj), _ptr(rc), th)
+        return color, px, hits, patch, obj, dict(primary=int(rc[0]), shadow=int(rc[1]), secondary=int(rc[2]))
 
     def cpu_render(self, cam, eps=1e-4, depth=0, rows=None, threads=None, stride=1):
         """CPURayTracer restatement over rows r0, r0+stride, ... < r1; returns

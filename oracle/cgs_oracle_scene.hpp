@@ -487,10 +487,19 @@ struct RayCounts
     uint64_t primary = 0, shadow = 0, secondary = 0;
 };
 
+// The first segment's closest hit of a shaded ray (t, u, v, patch, object).
+struct FirstHit
+{
+    float t = 0, u = 0, v = 0;
+    uint32_t patch = EMPTY;
+    int object = -1;
+};
+
 // render kernel and Context::trace/miss/closestHit (RayTracer.cu:194-294,385-544, non-Monte-Carlo path): the GPU renderer's
-// iterative trace loop.  Writes the unclamped colour and the packed pixel.
-inline void renderPixel(const Scene& sc, const CameraSetup& cam, const RenderOptions& op,
-    uint32_t i, uint32_t j, f3& colorOut, uint32_t& pixel, Work* w, RayCounts* rc)
+// iterative trace loop from one ray (the render starts it at a camera ray with max = infinity).  Returns the
+// unclamped colour; `first` (nullable) receives the first segment's hit.
+inline f3 shadeRay(const Scene& sc, const Ray& ray, const RenderOptions& op, Work* w, RayCounts* rc,
+    FirstHit* firstHit = nullptr)
 {
     struct Payload
     {
@@ -499,18 +508,21 @@ inline void renderPixel(const Scene& sc, const CameraSetup& cam, const RenderOpt
         int depth;
     } pl;
     pl.color = splat(0);
-    pl.ray = {cam.position, pixelRayDirection(cam, (float)i, (float)j), std::numeric_limits<float>::infinity()};
+    pl.ray = ray;
     pl.attenuation = splat(1);
     pl.depth = op.recursionDepth;
     bool first = true;
     while (true)
     {
         if (rc) (first ? rc->primary : rc->secondary)++;
-        first = false;
         Hit hit;
         hit.t = pl.ray.max;
         uint32_t patch = EMPTY;
         int obj = sceneIntersect(sc, hit, pl.ray, &patch, w);
+        if (first && firstHit)
+            *firstHit = obj < 0 ? FirstHit {hit.t, 0, 0, EMPTY, -1}
+                                : FirstHit {hit.t, hit.coordinates.x, hit.coordinates.y, patch, obj};
+        first = false;
         if (obj < 0)
         {
             pl.color = pl.color + sc.background * pl.attenuation;
@@ -560,11 +572,26 @@ inline void renderPixel(const Scene& sc, const CameraSetup& cam, const RenderOpt
         pl.attenuation = a;
         pl.depth = pl.depth - 1;
     }
-    colorOut = pl.color;
-    f3 c = pl.color;
-    // __saturatef semantics: NaN -> 0
+    return pl.color;
+}
+
+// The packed frame pixel of a colour (RayTracer.cu:262-266): __saturatef
+// (NaN -> 0) on each channel, then pack_sRGB.
+inline uint32_t packColor(f3 c)
+{
     auto sat = [](float x) { return x > 0.0f ? (x < 1.0f ? x : 1.0f) : 0.0f; };
-    pixel = packSRGB(sat(c.x), sat(c.y), sat(c.z));
+    return packSRGB(sat(c.x), sat(c.y), sat(c.z));
+}
+
+// One pixel of the render kernel (RayTracer.cu:194-294): the camera ray
+// through pixel (i, j) with max = infinity, shaded by shadeRay.  Writes the
+// unclamped colour and the packed pixel.
+inline void renderPixel(const Scene& sc, const CameraSetup& cam, const RenderOptions& op,
+    uint32_t i, uint32_t j, f3& colorOut, uint32_t& pixel, Work* w, RayCounts* rc)
+{
+    const Ray ray {cam.position, pixelRayDirection(cam, (float)i, (float)j), std::numeric_limits<float>::infinity()};
+    colorOut = shadeRay(sc, ray, op, w, rc);
+    pixel = packColor(colorOut);
 }
 
 // ---------------------------------------------------------------------------

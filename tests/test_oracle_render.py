@@ -62,8 +62,10 @@ def test_python_camera_rays_equal_the_oracle(oracle):
     """scenes.camera_rays (the rays traced on the GPU) = the oracle's camera
     bit for bit at every pixel centre."""
     from cgspline_amd import scenes
-    for flip in (False, True):
-        cam = make_cam(flip, w=64, h=48)
+    # 60 degrees: the C library's tanf of pi/6 in float32 is not the
+    # correctly rounded tangent, and the camera must use the former
+    for flip, fovy in ((False, 40.0), (True, 40.0), (False, 60.0)):
+        cam = make_cam(flip, w=64, h=48, fovy_deg=fovy)
         rays = scenes.camera_rays(cam)
         jj, ii = np.meshgrid(np.arange(cam.height), np.arange(cam.width), indexing="ij")
         xy = np.stack([ii.ravel(), jj.ravel()], 1).astype(np.float32)
@@ -174,6 +176,47 @@ def test_shading_closed_form(oracle):
 
 def test_shading_closed_form_without_bounce(oracle):
     check_shading(oracle, depth=0)
+
+
+def test_shaded_rays_closed_form_and_render(oracle):
+    """shadeRay from arbitrary rays (the shaded queries' oracle): rays from
+    random points above and below the square in random directions, some with
+    a range that ends before the plane (background), equal the FP64 closed
+    form; from the render's own camera rays it equals the render bit for bit."""
+    from cgspline_amd import scenes
+    sc = flat_square_scene()
+    osc = oracle.OracleScene(sc)
+    r = rng(91)
+    n = 4000
+    o = np.column_stack([r.uniform(-1.5, 1.5, n), r.uniform(-1.5, 1.5, n), r.choice([-1, 1], n) * r.uniform(0.3, 2, n)])
+    tgt = np.column_stack([r.uniform(-1.2, 1.2, n), r.uniform(-1.2, 1.2, n), np.zeros(n)])
+    d = tgt - o
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, 0:3], rays[:, 3], rays[:, 4:7] = o, np.inf, d
+    t = -rays[:, 2].astype(np.float64) / rays[:, 6]
+    short = np.arange(n) % 5 == 0
+    rays[short, 3] = (0.5 * t[short]).astype(np.float32)
+    color, _, hits, patch, obj, counts = osc.shade(rays, eps=1e-4, depth=1)
+    for k in range(n):
+        want = np.array(BG) if short[k] else expected_colour(rays[k, 0:3], rays[k, 4:7], 1)
+        rel = np.abs(color[k] - want) / np.maximum(np.abs(want), 1e-3)
+        assert rel.max() < 2e-4, (k, color[k], want)
+    assert np.all(obj[short] == -1) and np.all(patch[short] == 0xFFFFFFFF)
+    assert np.all(hits[short, 0] == rays[short, 3])
+    hit = obj == 0
+    assert hit.sum() > n // 2
+    P = rays[hit, 0:3].astype(np.float64) + hits[hit, 0:1] * rays[hit, 4:7]
+    assert np.abs(P[:, 2]).max() < 1e-5
+    assert np.abs(hits[hit, 1] - (P[:, 0] + 1) / 2).max() < 1e-4 and np.abs(hits[hit, 2] - (P[:, 1] + 1) / 2).max() < 1e-4
+    assert counts["primary"] == n and counts["shadow"] > 0
+    # the render's camera rays, shaded one by one, give the render's colours
+    cam = scenes.make_camera((0.4, -1.8, 2.2), (0.0, 0.1, 0.0), (0.0, 0.0, 1.0), 60.0, 40, 30)
+    rcolor, rframe, _, rcounts = osc.render(cam, eps=1e-4, depth=1)
+    crays = scenes.camera_rays(cam)
+    scolor, spx, _, _, _, scounts = osc.shade(crays, eps=1e-4, depth=1)
+    assert np.array_equal(scolor.view(np.uint32), rcolor.reshape(-1, 3).view(np.uint32))
+    assert np.array_equal(spx, rframe.reshape(-1)) and scounts == rcounts
 
 
 # ---------------------------------------------------------------------------
