@@ -25,10 +25,13 @@ Prints one JSON line.  Besides the contract's fields:
 * `moving_camera`: an orbit of the same view, a small step per frame, each
   render taking its tiles in the order the previous frame measured.
 * `batched`: 64 resident scenes (1k–200k patches, 4k–2M rays), one
-  persistent batched launch against one trace call per scene on the same rays.
+  persistent batched launch against one trace call per scene on the same rays;
+  `accuracy_tuned`: the leaf method cgs_batch_autotune_accuracy keeps within
+  1e-4 of the FP64-polished reference, and its pass time.
 * `serving`: 8 resident scenes (one of them this 1M-patch scene) under
-  Poisson arrivals of 1k–4M-ray requests at 30/60/90 % of capacity: p50/p99
-  latency and sustained Mrays/s.
+  Poisson arrivals of 1k–4M-ray requests (40 % occlusion, 20 % shaded, the
+  rest closest hits) at 30/60/90 % of capacity: p50/p99 latency and sustained
+  Mrays/s.
 
 Each render records its tiles' times and the next render of the same frame
 size takes the tiles longest first; the warmup renders set that order up, and
@@ -529,12 +532,35 @@ def batched_leg(ctx, args):
     ms_graph = timed(lambda: graph.launch(sh))
     graph.destroy()
     best = min(ms_batch, ms_graph)
+    # the leaf method tuned under the accuracy limits (t and (u,v) within
+    # 1e-4 of the FP64-polished reference, at most 1e-4 of rays disagreeing)
+    t0 = time.time()
+    limits = dict(max_rel_t=1e-4, max_uv=1e-4, max_disagree=1e-4)
+    picked, report = ctx.batch_autotune_accuracy(jobs, allow_polish=True, **limits)
+    ms_acc = timed(lambda: ctx.trace_batch(jobs, stream=sh))
+    ctx.set_batch_trace_options()
+    ent = next(e for e in report if e["options"] == picked)
+    ref_ent = next(e for e in report if e["options"]["method"] == 0 and e["options"]["max_clip_iters"] == 0
+                   and e["options"]["polish"])
+    acc = {"limits": limits, "picked": picked, "ms_per_pass": round(ms_acc, 4),
+           "value": round(nray / ms_acc / 1e3, 3), "max_rel_t": ent["max_rel_t"], "max_uv": ent["max_uv"],
+           "disagree": ent["disagree"], "reference_polished_ms": round(ref_ent["ms"], 4),
+           "candidates": [{"options": {k: v for k, v in e["options"].items() if v}, "ms": round(e["ms"], 3),
+                           "max_rel_t": float(f"{e['max_rel_t']:.3g}"), "max_uv": float(f"{e['max_uv']:.3g}"),
+                           "disagree": float(f"{e['disagree']:.3g}"), "admissible": e["admissible"]}
+                          for e in report],
+           "tune_s": round(time.time() - t0, 1),
+           "note": "cgs_batch_autotune_accuracy over clip caps 8-32, subdivision depths 4-12 and the reference "
+                   "clipping, each with and without the FP64 polish; errors against the reference clipping with "
+                   "the polish (north_star's 1e-4 on t and (u,v) is met only by polished candidates)"}
+    log(f"batched, accuracy-tuned: {picked} {ms_acc:.3f} ms ({acc['tune_s']} s to tune)")
     res = {"value": round(nray / best / 1e3, 3), "unit": "Mrays/s", "scenes": len(items), "patches": npatch,
            "rays": nray, "ms_per_pass": round(best, 4), "per_job_value": round(nray / ms_per_job / 1e3, 3),
            "batched_over_per_job": round(ms_per_job / best, 3),
            "forms_ms": {"per_job": round(ms_per_job, 3), "batched": round(ms_batch, 3), "graph": round(ms_graph, 3)},
            "autotuned_shape": {"chunk": tuned[0], "blocks_per_cu": tuned[1]},
            "results_equal_per_job": bool(same),
+           "accuracy_tuned": acc,
            "note": "closest hits of half camera, half random rays; the batched results equal the per-job "
                    "results bit for bit (same per-ray algorithm), so the accuracy is equal"}
     log(f"batched: {res}")
@@ -563,7 +589,8 @@ def serving_leg(ctx, args, big_ds, big_wl):
             pool = pools[k]
             n = int(min(n, pool.shape[0]))
             start = int(r.integers(0, pool.shape[0] - n + 1))
-            reqs.append((k, pool[start:start + n], bool(r.random() < 0.4)))
+            u = r.random()
+            reqs.append((k, pool[start:start + n], True if u < 0.4 else ("shade" if u < 0.6 else False)))
         return reqs
 
     budget = 8 << 30
@@ -587,8 +614,10 @@ def serving_leg(ctx, args, big_ds, big_wl):
     res = {"capacity": round(capacity / 1e6, 3), "unit": "Mrays/s", "scenes": len(ds),
            "patches": sum(int(d.scene.npatch) for d in ds), "hbm_budget_bytes": budget, "slots": args.slots,
            "max_batch_rays": args.max_batch_rays, "request_rays": "log-uniform [1e3, 4e6]",
-           "occlusion_fraction": 0.4, "loads": loads,
-           "note": "latency is submit to results in the caller's host arrays (host copies and PCIe included)"}
+           "occlusion_fraction": 0.4, "shaded_fraction": 0.2, "loads": loads,
+           "note": "latency is submit to results in the caller's host arrays (host copies and PCIe included); "
+                   "shaded requests (closest hit, one shadow ray per light, reflection depth 1) count their "
+                   "request rays only"}
     log(f"serving: {res}")
     for d in ds[:-1]:
         d.destroy()

@@ -1641,12 +1641,13 @@ cgs_status readJobResults(const cgs_batch_job* jobs, uint32_t njobs, std::vector
 }
 
 // A candidate's results against the reference's: the largest relative t and
-// (u,v) differences over rays both hit on the same patch, and the rays whose
-// hit or miss, object, or occlusion differ, or whose patch differs with t
-// outside the limit (a hit on the neighbouring patch of a seam at the same
-// distance agrees; its (u,v) is on another patch and is not compared).
+// (u,v) differences over rays both hit on the same patch, and the fraction
+// of rays outside the limits: hit or miss, object or occlusion differ, the
+// patch differs with t beyond max_rel_t (a hit on the neighbouring patch of a
+// seam at the same distance agrees; its (u,v) is on another patch and is not
+// compared), or t or (u,v) beyond the limits on the same patch.
 void compareResults(const cgs_batch_job* jobs, uint32_t njobs, const std::vector<JobResults>& ref,
-                    const std::vector<JobResults>& got, float maxRelT, cgs_autotune_entry& e)
+                    const std::vector<JobResults>& got, float maxRelT, float maxUV, cgs_autotune_entry& e)
 {
     double relT = 0.0, uv = 0.0;
     uint64_t disagree = 0, total = 0;
@@ -1677,8 +1678,10 @@ void compareResults(const cgs_batch_job* jobs, uint32_t njobs, const std::vector
                 if (dt > maxRelT) disagree++;
                 continue;
             }
+            const double duv = std::max(std::fabs((double)b.u - a.u), std::fabs((double)b.v - a.v));
             relT = std::max(relT, dt);
-            uv = std::max(uv, std::max(std::fabs((double)b.u - a.u), std::fabs((double)b.v - a.v)));
+            uv = std::max(uv, duv);
+            if (dt > maxRelT || duv > maxUV) disagree++;
         }
     }
     e.max_rel_t = (float)relT;
@@ -1737,7 +1740,7 @@ cgs_status cgs_batch_autotune_accuracy(cgs_ctx* ctx, const cgs_batch_job* jobs, 
         if (st == CGS_OK && hipStreamSynchronize(s) != hipSuccess) st = fail(CGS_ERR_HIP, "cgs_batch_autotune_accuracy");
         if (st == CGS_OK) st = readJobResults(jobs, njobs, got);
         if (st != CGS_OK) break;
-        compareResults(jobs, njobs, ref, got, limits->max_rel_t, e);
+        compareResults(jobs, njobs, ref, got, limits->max_rel_t, limits->max_uv, e);
         float bestMs = std::numeric_limits<float>::infinity();
         for (int rep = 0; rep < 2 && st == CGS_OK; rep++)
         {
@@ -1752,8 +1755,7 @@ cgs_status cgs_batch_autotune_accuracy(cgs_ctx* ctx, const cgs_batch_job* jobs, 
             bestMs = std::min(bestMs, ms);
         }
         e.ms = bestMs;
-        e.admissible = e.max_rel_t <= limits->max_rel_t && e.max_uv <= limits->max_uv &&
-                               e.disagree <= limits->max_disagree ? 1u : 0u;
+        e.admissible = e.disagree <= limits->max_disagree ? 1u : 0u;
         entries.push_back(e);
         if (e.admissible && (pick < 0 || e.ms < entries[pick].ms)) pick = (int)entries.size() - 1;
     }

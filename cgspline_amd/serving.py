@@ -92,24 +92,29 @@ def request_sizes(rng, count, lo=1_000, hi=4_000_000):
     return np.exp(rng.uniform(math.log(lo), math.log(hi), count)).round().astype(np.int64)
 
 
-def run_load(server: Server, requests, rate_hz: float, seed: int = 0):
-    """Submits `requests` [(scene, rays, any)] at Poisson arrival times of the
-    given rate (in-process, open loop) and waits for all of them.  Returns the
-    latencies (s), the rays traced and the wall time from the first arrival
+def run_load(server: Server, requests, rate_hz: float, seed: int = 0, depth: int = 1):
+    """Submits `requests` [(scene, rays, kind)] at Poisson arrival times of
+    the given rate (in-process, open loop) and waits for all of them; kind is
+    False (closest hits), True (occlusion) or "shade" (shaded at `depth`).
+    Returns the latencies (s), the requests' rays (a shaded request's shadow
+    and reflection rays not counted) and the wall time from the first arrival
     to the last completion."""
     rng = np.random.default_rng(seed)
     gaps = rng.exponential(1.0 / rate_hz, len(requests)) if rate_hz > 0 else np.zeros(len(requests))
     start = time.perf_counter()
     due = start
     ids = []
-    for (scene, rays, any_hit), gap in zip(requests, gaps):
+    for (scene, rays, kind), gap in zip(requests, gaps):
         due += gap
         while True:
             now = time.perf_counter()
             if now >= due:
                 break
             time.sleep(min(due - now, 5e-4))
-        ids.append(server.submit(scene, rays, any_hit))
+        if kind == "shade":
+            ids.append(server.submit_shaded(scene, rays, depth=depth))
+        else:
+            ids.append(server.submit(scene, rays, bool(kind)))
     lats = [server.wait(i)[1] for i in ids]
     wall = time.perf_counter() - start
     return np.array(lats), int(sum(r[1].shape[0] for r in requests)), wall

@@ -173,10 +173,12 @@ typedef struct cgs_trace_options
 
 /* Accuracy limits of cgs_batch_autotune_accuracy, against the reference's
  * clipping (polished when allow_polish is set, so the reference is then the
- * FP64-refined root): the largest relative t difference and the largest
- * (u,v) difference over rays hit on the same patch, and the fraction of rays
- * whose hit or miss, object, occlusion, or (beyond max_rel_t in t) patch
- * differ. */
+ * FP64-refined root): a ray is outside the limits when its hit or miss,
+ * object or occlusion differ, when its patch differs with t beyond max_rel_t
+ * relative (a seam hit at the same distance agrees), or when on the same
+ * patch t differs beyond max_rel_t relative or u or v beyond max_uv.  A
+ * candidate is admissible when at most the fraction max_disagree of the
+ * rays is outside (0: every ray within the limits). */
 typedef struct cgs_accuracy_limits
 {
     float max_rel_t;
@@ -186,7 +188,9 @@ typedef struct cgs_accuracy_limits
 } cgs_accuracy_limits;
 
 /* One candidate of cgs_batch_autotune_accuracy: its options, the batched
- * launch's time (ms) and its differences from the reference. */
+ * launch's time (ms), the largest relative t and (u,v) differences over rays
+ * hit on the same patch as the reference, and the fraction of rays outside
+ * the limits. */
 typedef struct cgs_autotune_entry
 {
     cgs_trace_options options;
