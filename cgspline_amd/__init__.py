@@ -277,6 +277,38 @@ class Context:
         check(lib().cgs_stream_sync(self.h, None))
         return out.to_host(np.uint32, keys.shape)
 
+    def phf_build_wide(self, keys: np.ndarray, seed: int = 42):
+        """GPU build of the wide family over 64-bit keys (cgs_phf_build_wide);
+        returns (status, params dict, displacement)."""
+        keys = np.ascontiguousarray(keys, np.uint64)
+        n = keys.shape[0]
+        dk = DeviceArray.from_host(self, keys)
+        m = lib().cgs_phf_displacement_size(n)
+        dd = DeviceArray(self, 4 * m)
+        h = _abi.PHF()
+        st = lib().cgs_phf_build_wide(self.h, dk.dptr, n, seed, dd.dptr, C.byref(h), None)
+        check(lib().cgs_stream_sync(self.h, None))
+        params = dict(A=h.A, B=h.B, mask=h.mask, table_size=h.table_size, attempts=h.attempts)
+        return st, params, dd.to_host(np.uint32, (m,))
+
+    def phf_hash_wide(self, params: dict, displacement: np.ndarray, keys: np.ndarray) -> np.ndarray:
+        keys = np.ascontiguousarray(keys, np.uint64)
+        h = _abi.PHF(params["A"], params["B"], params["mask"], params["table_size"], 0, displacement.shape[0])
+        dk, dd = DeviceArray.from_host(self, keys), DeviceArray.from_host(self, displacement.astype(np.uint32))
+        out = DeviceArray(self, 4 * keys.shape[0])
+        check(lib().cgs_phf_hash_wide(self.h, C.byref(h), dd.dptr, dk.dptr, keys.shape[0], out.dptr, None))
+        check(lib().cgs_stream_sync(self.h, None))
+        return out.to_host(np.uint32, keys.shape)
+
+    def phf_fill_table_wide(self, params: dict, displacement: np.ndarray, keys: np.ndarray) -> np.ndarray:
+        keys = np.ascontiguousarray(keys, np.uint64)
+        h = _abi.PHF(params["A"], params["B"], params["mask"], params["table_size"], 0, displacement.shape[0])
+        dk, dd = DeviceArray.from_host(self, keys), DeviceArray.from_host(self, displacement.astype(np.uint32))
+        out = DeviceArray(self, 4 * params["table_size"])
+        check(lib().cgs_phf_fill_table_wide(self.h, C.byref(h), dd.dptr, dk.dptr, keys.shape[0], out.dptr, None))
+        check(lib().cgs_stream_sync(self.h, None))
+        return out.to_host(np.uint32, (params["table_size"],))
+
     # --- scenes -------------------------------------------------------------------
     def scene(self, scene: "scenes.Scene", hash_retries: int = 0) -> "DeviceScene":
         """Makes `scene` resident.  Like the reference's BVH::buildHashTable, a
@@ -464,6 +496,13 @@ class DeviceScene:
         check(lib().cgs_scene_create(ctx.h, C.byref(d), C.byref(p)), "cgs_scene_create")
         self.h = p.value
 
+    def bvh_wide(self, surface: int) -> bool:
+        """Whether that BVH (-1: the top level) is hashed with the wide family."""
+        r = lib().cgs_scene_bvh_wide(self.h, surface)
+        if r < 0:
+            raise ValueError(f"no BVH {surface}")
+        return r == 1
+
     @property
     def culled(self) -> bool:
         """Whether the queries find objects through the cull tree (cgs_scene_culled)."""
@@ -494,7 +533,7 @@ class DeviceScene:
         check(lib().cgs_scene_bvh_copy(self.h, surface, nodes.ctypes.data, keys.ctypes.data, idx.ctypes.data,
                                        D.ctypes.data, T.ctypes.data))
         return dict(nodes=nodes, keys=keys, indices=idx, A=h.A, B=h.B, mask=h.mask, table_size=h.table_size,
-                    attempts=h.attempts, displacement=D, table=T)
+                    attempts=h.attempts, displacement=D, table=T, wide=self.bvh_wide(surface))
 
     # device-pointer entry points (for resident buffers)
     def trace_closest_dev(self, rays, n, hits, objects=None, stream=None, **options):

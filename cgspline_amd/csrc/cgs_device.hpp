@@ -707,6 +707,7 @@ struct BVHRef
     FastMod T;
     uint32_t nodeCount;                  // size of `nodes`, bounds table lookups
     uint32_t finiteBoxes;                // 1: every child box of an inner node is finite
+    uint32_t wide;                       // 1: the wide hash family (exact arithmetic)
 };
 
 struct NodeBoxes
@@ -838,8 +839,13 @@ CGS_DEV NodeBoxes loadNodeCached(const BVHRef& b, const NodeCache& nc, uint32_t 
     return LOW ? loadNodeLow(b.nodes, node) : loadNode(b.nodes, node);
 }
 
+// PerfectHashFunction::hash (PerfectHashFunction.h:58-63): 32-bit
+// arithmetic, or for a BVH hashed with the wide family the same function in
+// exact arithmetic, (A·(k mod T) + B + D) mod T.
 CGS_DEV uint32_t hashKey(const BVHRef& b, uint32_t key)
 {
+    if (b.wide)
+        return (uint32_t)(((uint64_t)b.A * fastmod(key, b.T) + b.B + b.displ[key & b.mask]) % (uint64_t)b.T.d);
     uint32_t x = b.A * key + b.B + b.displ[key & b.mask];
     return fastmod(x, b.T);
 }

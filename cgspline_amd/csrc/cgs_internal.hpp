@@ -29,7 +29,7 @@ struct ShadeJob;
 struct PhfJob
 {
     uint32_t keyOff, n, M, mask;
-    uint32_t scratchOff;   // per-job scratch words: 13·M + 2·tcap
+    uint32_t scratchOff;   // per-job scratch words: phfScratchWords(M, tcap, key words)
     uint32_t tcap;         // capacity for the growing table: n + 16·step
     uint32_t displOff;     // output displacement (M words)
     uint32_t tableOff;     // output table (tcap words)
@@ -44,7 +44,12 @@ struct PhfResult
     uint32_t A, B, T, attempts, status;
 };
 
-constexpr uint32_t phfScratchWords(uint32_t M, uint32_t tcap) { return 13u * M + 2u * tcap; }
+// Scratch words of one build: 9·M words of bucket state, 4·M keys of
+// keyWords words each, 2·tcap words of slot state.
+constexpr uint32_t phfScratchWords(uint32_t M, uint32_t tcap, uint32_t keyWords = 1)
+{
+    return (9u + 4u * keyWords) * M + 2u * tcap;
+}
 
 // Displacement-table size of n keys: bit_floor(max(2, (n-1) >> 1))
 // (PerfectHashFunction.cu:234-248).
@@ -76,11 +81,19 @@ static_assert(sizeof(PhfResult) <= 4 * (kStaticPhfScratchWord - kStaticPhfResult
 hipError_t phfStaticInit(int device);
 uint32_t* phfStaticScratch(int device);
 
+// Perfect-hash builds of 32-bit keys, with the reference's family or (wide)
+// the same function in exact arithmetic; and of 64-bit keys (wide family).
 hipError_t launchPhfBuild(const PhfJob* jobs, uint32_t njobs, const uint32_t* keys, const uint32_t* ab, uint32_t* scratch,
-                          uint32_t* displ, uint32_t* tables, PhfResult* results, hipStream_t s);
+                          uint32_t* displ, uint32_t* tables, PhfResult* results, hipStream_t s, bool wide = false);
+hipError_t launchPhfBuildWide64(const PhfJob* jobs, uint32_t njobs, const uint64_t* keys, const uint32_t* ab,
+                                uint32_t* scratch, uint32_t* displ, uint32_t* tables, PhfResult* results, hipStream_t s);
 hipError_t launchPhfHash(const cgs_phf& h, const uint32_t* D, const uint32_t* keys, uint32_t n, uint32_t* out, hipStream_t s);
+hipError_t launchPhfHashWide64(const cgs_phf& h, const uint32_t* D, const uint64_t* keys, uint32_t n, uint32_t* out,
+                               hipStream_t s);
 hipError_t launchPhfFill(const cgs_phf& h, const uint32_t* D, const uint32_t* keys, uint32_t n, uint32_t base,
                          uint32_t* table, hipStream_t s);
+hipError_t launchPhfFillWide64(const cgs_phf& h, const uint32_t* D, const uint64_t* keys, uint32_t n, uint32_t* table,
+                               hipStream_t s);
 hipError_t launchFill(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
 
 // A cull tree over the leaf boxes of a BVH (cgs_cull.cpp): nodes 8 floats,

@@ -461,7 +461,7 @@ struct PhfBatch
 };
 
 cgs_status runPhfBatch(PhfBatch& b, const uint32_t* dKeys, uint32_t* dDispl, uint32_t* dTables,
-                       std::vector<PhfResult>& results, hipStream_t s)
+                       std::vector<PhfResult>& results, hipStream_t s, bool wide = false)
 {
     DeviceBuffer jobs, ab, scratch, res;
     HIP_TRY(jobs.alloc(b.jobs.size() * sizeof(PhfJob)));
@@ -471,7 +471,7 @@ cgs_status runPhfBatch(PhfBatch& b, const uint32_t* dKeys, uint32_t* dDispl, uin
     HIP_TRY(hipMemcpyAsync(jobs.p, b.jobs.data(), b.jobs.size() * sizeof(PhfJob), hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(ab.p, b.ab.data(), b.ab.size() * 4, hipMemcpyHostToDevice, s));
     HIP_TRY(launchPhfBuild(jobs.as<PhfJob>(), (uint32_t)b.jobs.size(), dKeys, ab.as<uint32_t>(), scratch.as<uint32_t>(),
-                           dDispl, dTables, res.as<PhfResult>(), s));
+                           dDispl, dTables, res.as<PhfResult>(), s, wide));
     results.resize(b.jobs.size());
     HIP_TRY(hipMemcpyAsync(results.data(), res.p, b.jobs.size() * sizeof(PhfResult), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
@@ -746,6 +746,66 @@ cgs_status cgs_phf_build_static(cgs_ctx* ctx, const uint32_t* keys, uint32_t n, 
     return CGS_OK;
 }
 
+cgs_status cgs_phf_build_wide(cgs_ctx* ctx, const uint64_t* keys, uint32_t n, uint32_t seed, uint32_t* displacement,
+                              cgs_phf* out, void* stream)
+{
+    if (!ctx || !out || !displacement || (n && !keys)) return fail(CGS_ERR_INVALID, "cgs_phf_build_wide: null argument");
+    if (n == 0) return fail(CGS_ERR_INVALID, "cgs_phf_build_wide: empty key set");
+    if (n >= (1u << 30)) return fail(CGS_ERR_INVALID, "cgs_phf_build_wide: more than 2^30 keys");
+    HIP_TRY(hipSetDevice(ctx->device));
+    hipStream_t s = (hipStream_t)stream;
+    PhfJob j {};
+    j.n = n;
+    j.M = displSize(n);
+    j.mask = j.M - 1;
+    j.step = growthStep(n);
+    j.tcap = n + 16 * j.step;
+    const std::vector<uint32_t> ab = drawAB(n, seed);
+    DeviceBuffer job, dab, scratch, tables, res;
+    HIP_TRY(job.alloc(sizeof j));
+    HIP_TRY(dab.alloc(ab.size() * 4));
+    HIP_TRY(scratch.alloc((size_t)phfScratchWords(j.M, j.tcap, 2) * 4));
+    HIP_TRY(tables.alloc((size_t)j.tcap * 4));
+    HIP_TRY(res.alloc(sizeof(PhfResult)));
+    HIP_TRY(hipMemcpyAsync(job.p, &j, sizeof j, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(dab.p, ab.data(), ab.size() * 4, hipMemcpyHostToDevice, s));
+    HIP_TRY(launchPhfBuildWide64(job.as<PhfJob>(), 1, keys, dab.as<uint32_t>(), scratch.as<uint32_t>(), displacement,
+                                 tables.as<uint32_t>(), res.as<PhfResult>(), s));
+    PhfResult r {};
+    HIP_TRY(hipMemcpyAsync(&r, res.p, sizeof r, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    out->A = r.A;
+    out->B = r.B;
+    out->mask = j.mask;
+    out->table_size = r.T;
+    out->attempts = r.attempts;
+    out->displacement_size = j.M;
+    if (r.status != CGS_OK)
+        return fail((cgs_status)r.status, r.status == CGS_ERR_HASH_DUPLICATE ? "PerfectHashFunction: DUPLICATE FOUND!"
+                                                                            : "PerfectHashFunction: bucket overflow");
+    return CGS_OK;
+}
+
+cgs_status cgs_phf_fill_table_wide(cgs_ctx* ctx, const cgs_phf* h, const uint32_t* displacement, const uint64_t* keys,
+                                   uint32_t n, uint32_t* table, void* stream)
+{
+    if (!ctx || !h || !displacement || !table || (n && !keys) || h->table_size == 0)
+        return fail(CGS_ERR_INVALID, "cgs_phf_fill_table_wide: bad argument");
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(launchPhfFillWide64(*h, displacement, keys, n, table, (hipStream_t)stream));
+    return CGS_OK;
+}
+
+cgs_status cgs_phf_hash_wide(cgs_ctx* ctx, const cgs_phf* h, const uint32_t* displacement, const uint64_t* keys,
+                             uint32_t n, uint32_t* out, void* stream)
+{
+    if (!ctx || !h || !displacement || !out || (n && !keys) || h->table_size == 0)
+        return fail(CGS_ERR_INVALID, "cgs_phf_hash_wide: bad argument");
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(launchPhfHashWide64(*h, displacement, keys, n, out, (hipStream_t)stream));
+    return CGS_OK;
+}
+
 cgs_status cgs_phf_fill_table(cgs_ctx* ctx, const cgs_phf* h, const uint32_t* displacement, const uint32_t* keys,
                               uint32_t n, uint32_t* table, void* stream)
 {
@@ -1013,6 +1073,38 @@ cgs_status cgs_scene_create(cgs_ctx* ctx, const cgs_scene_desc* d, cgs_scene** o
     std::vector<PhfResult> res;
     cgs_status ps = runPhfBatch(batch, sc->keys.as<uint32_t>(), sc->displ.as<uint32_t>(), sc->tables.as<uint32_t>(), res, s);
     if (ps != CGS_OK) return ps;
+    // A BVH whose reference-family hash has a duplicate (every attempt fails
+    // from 2^21 heap keys on) is hashed again with the wide family, into the
+    // same displacement and table ranges; a bucket overflow fails both.
+    std::vector<uint8_t> wide(S + 1, 0);
+    {
+        PhfBatch again;
+        std::vector<uint32_t> which;
+        for (uint32_t k = 0; k <= S; k++)
+            if (res[k].status == CGS_ERR_HASH_DUPLICATE)
+            {
+                PhfJob j = batch.jobs[k];
+                j.scratchOff = (uint32_t)again.scratchWords;
+                again.scratchWords += phfScratchWords(j.M, j.tcap);
+                j.abOff = (uint32_t)(again.ab.size() / 2);
+                again.ab.insert(again.ab.end(), batch.ab.begin() + 2 * batch.jobs[k].abOff,
+                                batch.ab.begin() + 2 * (batch.jobs[k].abOff + 16));
+                again.jobs.push_back(j);
+                which.push_back(k);
+            }
+        if (!which.empty())
+        {
+            std::vector<PhfResult> res2;
+            ps = runPhfBatch(again, sc->keys.as<uint32_t>(), sc->displ.as<uint32_t>(), sc->tables.as<uint32_t>(), res2, s,
+                             true);
+            if (ps != CGS_OK) return ps;
+            for (size_t q = 0; q < which.size(); q++)
+            {
+                res[which[q]] = res2[q];
+                wide[which[q]] = 1;
+            }
+        }
+    }
     sc->dev.resize(S + 1);
     sc->phf.resize(S + 1);
     for (uint32_t k = 0; k <= S; k++)
@@ -1033,6 +1125,7 @@ cgs_status cgs_scene_create(cgs_ctx* ctx, const cgs_scene_desc* d, cgs_scene** o
         b.T = r.T;
         b.patchBase = k < S ? d->surf_pbase[k] : 0;
         b.Tm = fastmodM(r.T);
+        b.wide = wide[k];
         b.finiteBoxes = 1;
         for (const cgs_bvh_node& n : sc->bvh[k].nodes)
             if (n.left != CGS_EMPTY)
@@ -1069,6 +1162,12 @@ cgs_status cgs_scene_bvh_info(cgs_scene* sc, int32_t surface, uint32_t* node_cou
 }
 
 int cgs_scene_culled(cgs_scene* sc) { return sc ? (sc->cull ? 1 : 0) : -1; }
+
+int cgs_scene_bvh_wide(cgs_scene* sc, int32_t surface)
+{
+    if (!sc || surface < -1 || surface >= (int32_t)sc->nsurf) return -1;
+    return sc->dev[surface < 0 ? sc->nsurf : (uint32_t)surface].wide ? 1 : 0;
+}
 
 } // extern "C"
 

@@ -3,7 +3,8 @@
 // Deterministic perfect-hash build for BVH node keys on gfx950
 // (rt/PerfectHashFunction.{h,cu}, rt/BVH.cpp:24-38).
 //
-// Hash family h(k) = (A·k + B + D[k & mask]) mod T in 32-bit arithmetic,
+// Hash family h(k) = (A·k + B + D[k & mask]) mod T in 32-bit arithmetic (or,
+// as an extension, the same function in exact arithmetic: slotOf below),
 // buckets of at most 4 keys, (A,B) drawn by the host's std::mt19937 exactly as
 // the reference draws them, up to 16 functions with the table growing by
 // max(4, 0.05·n) after each failed one, at most 32 displacement attempts per
@@ -34,12 +35,30 @@ constexpr uint32_t kPending = 0, kPlaced = 1, kFailed = 2;
 constexpr int kCap = 4, kMaxAttempts = 32, kMaxFunctions = 16;
 constexpr int kPhfThreads = 256;
 
-__device__ __forceinline__ uint32_t slotOf(uint32_t A, uint32_t B, uint32_t key, uint32_t d, uint32_t T)
+// The function's slot for a key and displacement: the reference's 32-bit
+// arithmetic (PerfectHashFunction.h:58-63), or with WIDE the same function in
+// exact arithmetic on keys of up to 64 bits (no counterpart in the
+// reference): (A·(k mod T) + B + d) mod T = (A·k + B + d) mod T without the
+// 32-bit wrap of A·k, which breaks the function's near-bijection on dense
+// heap keys (every reference attempt fails at 2^21 keys).  T < 2^31, so the
+// 64-bit sum cannot wrap.
+template<bool WIDE, class KEY>
+__device__ __forceinline__ uint32_t slotOf(uint32_t A, uint32_t B, KEY key, uint32_t d, uint32_t T)
 {
-    return (A * key + B + d) % T;
+    if constexpr (WIDE)
+        return (uint32_t)(((uint64_t)A * (uint64_t)(key % T) + B + d) % T);
+    else
+        return (A * (uint32_t)key + B + d) % T;
 }
 
-__global__ __launch_bounds__(kPhfThreads) void kPhfBuild(const PhfJob* __restrict__ jobs, const uint32_t* __restrict__ keys,
+template<class KEY>
+__device__ __forceinline__ uint32_t bucketOf(KEY key, uint32_t mask)
+{
+    return (uint32_t)(key & mask);
+}
+
+template<class KEY, bool WIDE>
+__global__ __launch_bounds__(kPhfThreads) void kPhfBuild(const PhfJob* __restrict__ jobs, const KEY* __restrict__ keys,
                                                          const uint32_t* __restrict__ ab, uint32_t* __restrict__ scratch,
                                                          uint32_t* __restrict__ displ, uint32_t* __restrict__ tables,
                                                          PhfResult* __restrict__ results)
@@ -48,8 +67,8 @@ __global__ __launch_bounds__(kPhfThreads) void kPhfBuild(const PhfJob* __restric
     const uint32_t tid = threadIdx.x;
     const uint32_t n = J.n, M = J.M, mask = J.mask;
     uint32_t* bsize = scratch + J.scratchOff;      // M
-    uint32_t* bkeys = bsize + M;                   // 4M
-    uint32_t* list = bkeys + kCap * M;             // M   (class lists, ascending bucket id)
+    KEY* bkeys = (KEY*)(bsize + M);                // 4M keys (scratch offsets and M are even)
+    uint32_t* list = (uint32_t*)(bkeys + kCap * M); // M   (class lists, ascending bucket id)
     uint32_t* att = list + M;                      // M
     uint32_t* state = att + M;                     // M
     uint32_t* valid = state + M;                   // M
@@ -74,8 +93,8 @@ __global__ __launch_bounds__(kPhfThreads) void kPhfBuild(const PhfJob* __restric
     // 1. buckets (init_and_bucketsort, PerfectHashFunction.cu:27-55)
     for (uint32_t i = tid; i < n; i += kPhfThreads)
     {
-        uint32_t k = keys[J.keyOff + i];
-        uint32_t j = k & mask;
+        const KEY k = keys[J.keyOff + i];
+        uint32_t j = bucketOf(k, mask);
         uint32_t s = atomicAdd(bsize + j, 1u);
         if (s < (uint32_t)kCap) bkeys[kCap * j + s] = k;
         else sFlag = 1;
@@ -169,7 +188,7 @@ __global__ __launch_bounds__(kPhfThreads) void kPhfBuild(const PhfJob* __restric
                     bool v = true;
                     for (int k = 0; k < s; k++)
                     {
-                        sl[k] = slotOf(A, B, bkeys[kCap * b + k], d, T);
+                        sl[k] = slotOf<WIDE>(A, B, bkeys[kCap * b + k], d, T);
                         if (occ[sl[k]]) v = false;
                         for (int m = 0; m < k; m++)
                             if (sl[m] == sl[k]) v = false;
@@ -240,11 +259,10 @@ __global__ __launch_bounds__(kPhfThreads) void kPhfBuild(const PhfJob* __restric
     for (uint32_t i = tid; i < T; i += kPhfThreads) table[i] = EMPTY;
     if (tid == 0) sFlag = 0;
     __syncthreads();
-    const FastMod fm {0xFFFFFFFFFFFFFFFFull / T + 1, T};
     for (uint32_t i = tid; i < n; i += kPhfThreads)
     {
-        uint32_t k = keys[J.keyOff + i];
-        uint32_t h = fastmod(A * k + B + D[k & mask], fm);
+        const KEY k = keys[J.keyOff + i];
+        uint32_t h = slotOf<WIDE>(A, B, k, D[bucketOf(k, mask)], T);
         uint32_t old = atomicCAS(table + h, EMPTY, J.nodeBase + i);
         if (old != EMPTY) sFlag = 1;
     }
@@ -254,22 +272,30 @@ __global__ __launch_bounds__(kPhfThreads) void kPhfBuild(const PhfJob* __restric
 }
 
 // PerfectHashFunction::hash for a batch of keys.
+template<class KEY, bool WIDE>
 __global__ void kPhfHash(uint32_t A, uint32_t B, uint32_t mask, uint64_t Tm, uint32_t T, const uint32_t* __restrict__ D,
-                         const uint32_t* __restrict__ keys, uint32_t n, uint32_t* __restrict__ out)
+                         const KEY* __restrict__ keys, uint32_t n, uint32_t* __restrict__ out)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    uint32_t k = keys[i];
-    out[i] = fastmod(A * k + B + D[k & mask], FastMod {Tm, T});
+    const KEY k = keys[i];
+    if constexpr (WIDE)
+        out[i] = slotOf<true>(A, B, k, D[bucketOf(k, mask)], T);
+    else
+        out[i] = fastmod(A * k + B + D[k & mask], FastMod {Tm, T});
 }
 
+template<class KEY, bool WIDE>
 __global__ void kPhfFill(uint32_t A, uint32_t B, uint32_t mask, uint64_t Tm, uint32_t T, const uint32_t* __restrict__ D,
-                         const uint32_t* __restrict__ keys, uint32_t n, uint32_t base, uint32_t* __restrict__ table)
+                         const KEY* __restrict__ keys, uint32_t n, uint32_t base, uint32_t* __restrict__ table)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    uint32_t k = keys[i];
-    table[fastmod(A * k + B + D[k & mask], FastMod {Tm, T})] = base + i;
+    const KEY k = keys[i];
+    if constexpr (WIDE)
+        table[slotOf<true>(A, B, k, D[bucketOf(k, mask)], T)] = base + i;
+    else
+        table[fastmod(A * k + B + D[k & mask], FastMod {Tm, T})] = base + i;
 }
 
 __global__ void kFill(uint32_t* __restrict__ p, uint32_t v, size_t n)
@@ -312,31 +338,80 @@ uint32_t* phfStaticScratch(int device)
 }
 
 hipError_t launchPhfBuild(const PhfJob* jobs, uint32_t njobs, const uint32_t* keys, const uint32_t* ab, uint32_t* scratch,
-                          uint32_t* displ, uint32_t* tables, PhfResult* results, hipStream_t s)
+                          uint32_t* displ, uint32_t* tables, PhfResult* results, hipStream_t s, bool wide)
 {
     if (njobs == 0) return hipSuccess;
-    hipLaunchKernelGGL(kPhfBuild, dim3(njobs), dim3(kPhfThreads), 0, s, jobs, keys, ab, scratch, displ, tables, results);
+    if (wide)
+        hipLaunchKernelGGL((kPhfBuild<uint32_t, true>), dim3(njobs), dim3(kPhfThreads), 0, s, jobs, keys, ab, scratch,
+                           displ, tables, results);
+    else
+        hipLaunchKernelGGL((kPhfBuild<uint32_t, false>), dim3(njobs), dim3(kPhfThreads), 0, s, jobs, keys, ab, scratch,
+                           displ, tables, results);
     return hipGetLastError();
 }
 
-hipError_t launchPhfHash(const cgs_phf& h, const uint32_t* D, const uint32_t* keys, uint32_t n, uint32_t* out, hipStream_t s)
+hipError_t launchPhfBuildWide64(const PhfJob* jobs, uint32_t njobs, const uint64_t* keys, const uint32_t* ab,
+                                uint32_t* scratch, uint32_t* displ, uint32_t* tables, PhfResult* results, hipStream_t s)
+{
+    if (njobs == 0) return hipSuccess;
+    hipLaunchKernelGGL((kPhfBuild<uint64_t, true>), dim3(njobs), dim3(kPhfThreads), 0, s, jobs, keys, ab, scratch, displ,
+                       tables, results);
+    return hipGetLastError();
+}
+
+template<class KEY>
+static hipError_t phfHash(const cgs_phf& h, const uint32_t* D, const KEY* keys, uint32_t n, uint32_t* out, hipStream_t s,
+                          bool wide)
 {
     if (n == 0) return hipSuccess;
     uint64_t Tm = 0xFFFFFFFFFFFFFFFFull / h.table_size + 1;
-    hipLaunchKernelGGL(kPhfHash, dim3((n + 255) / 256), dim3(256), 0, s, h.A, h.B, h.mask, Tm, h.table_size, D, keys, n, out);
+    if (wide)
+        hipLaunchKernelGGL((kPhfHash<KEY, true>), dim3((n + 255) / 256), dim3(256), 0, s, h.A, h.B, h.mask, Tm,
+                           h.table_size, D, keys, n, out);
+    else
+        hipLaunchKernelGGL((kPhfHash<KEY, false>), dim3((n + 255) / 256), dim3(256), 0, s, h.A, h.B, h.mask, Tm,
+                           h.table_size, D, keys, n, out);
     return hipGetLastError();
 }
 
-hipError_t launchPhfFill(const cgs_phf& h, const uint32_t* D, const uint32_t* keys, uint32_t n, uint32_t base,
-                         uint32_t* table, hipStream_t s)
+template<class KEY>
+static hipError_t phfFill(const cgs_phf& h, const uint32_t* D, const KEY* keys, uint32_t n, uint32_t base,
+                          uint32_t* table, hipStream_t s, bool wide)
 {
     size_t T = h.table_size;
     hipLaunchKernelGGL(kFill, dim3((unsigned)((T + 255) / 256)), dim3(256), 0, s, table, EMPTY, T);
     if (n == 0) return hipGetLastError();
     uint64_t Tm = 0xFFFFFFFFFFFFFFFFull / h.table_size + 1;
-    hipLaunchKernelGGL(kPhfFill, dim3((n + 255) / 256), dim3(256), 0, s, h.A, h.B, h.mask, Tm, h.table_size, D, keys, n,
-                       base, table);
+    if (wide)
+        hipLaunchKernelGGL((kPhfFill<KEY, true>), dim3((n + 255) / 256), dim3(256), 0, s, h.A, h.B, h.mask, Tm,
+                           h.table_size, D, keys, n, base, table);
+    else
+        hipLaunchKernelGGL((kPhfFill<KEY, false>), dim3((n + 255) / 256), dim3(256), 0, s, h.A, h.B, h.mask, Tm,
+                           h.table_size, D, keys, n, base, table);
     return hipGetLastError();
+}
+
+hipError_t launchPhfHash(const cgs_phf& h, const uint32_t* D, const uint32_t* keys, uint32_t n, uint32_t* out, hipStream_t s)
+{
+    return phfHash(h, D, keys, n, out, s, false);
+}
+
+hipError_t launchPhfHashWide64(const cgs_phf& h, const uint32_t* D, const uint64_t* keys, uint32_t n, uint32_t* out,
+                               hipStream_t s)
+{
+    return phfHash(h, D, keys, n, out, s, true);
+}
+
+hipError_t launchPhfFill(const cgs_phf& h, const uint32_t* D, const uint32_t* keys, uint32_t n, uint32_t base,
+                         uint32_t* table, hipStream_t s)
+{
+    return phfFill(h, D, keys, n, base, table, s, false);
+}
+
+hipError_t launchPhfFillWide64(const cgs_phf& h, const uint32_t* D, const uint64_t* keys, uint32_t n, uint32_t* table,
+                               hipStream_t s)
+{
+    return phfFill(h, D, keys, n, 0u, table, s, true);
 }
 
 hipError_t launchFill(uint32_t* p, uint32_t v, size_t n, hipStream_t s)

@@ -34,7 +34,7 @@ struct DevBVH
     uint32_t patchBase;   // first patch of the surface (surfaces only)
     uint64_t Tm;          // fastmod multiplier for T
     uint32_t finiteBoxes; // 1: every child box of an inner node is finite
-    uint32_t pad;
+    uint32_t wide;        // 1: hashed with the wide family (exact arithmetic)
 };
 static_assert(sizeof(DevBVH) == 48, "DevBVH layout");
 
@@ -126,6 +126,7 @@ CGS_DEV BVHRef bvhRef(const SceneDev& sc, const DevBVH& d)
     b.T = {d.Tm, d.T};
     b.nodeCount = sc.nodeCount;
     b.finiteBoxes = d.finiteBoxes;
+    b.wide = d.wide;
     return b;
 }
 

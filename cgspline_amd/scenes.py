@@ -329,6 +329,31 @@ def displaced_grids(seed=BASE_SEED + 3, grids=4096, width=2048, height=2048,
                                             "primary + shadow + 1 reflection bounce")
 
 
+def displaced_grids_single(seed=BASE_SEED + 3, grids=4096, width=2048, height=2048,
+                           degenerate=0.10, box=25.0) -> Workload:
+    """The displaced grids of displaced_grids (the same seeded generator) as
+    one surface in world space: one BVH over all `grids`×256 patches under
+    one identity object, with the first grid's material.  At 4096 grids its
+    BVH has 2,097,151 nodes, which the reference's hash family cannot index;
+    the scene hashes it with the wide family."""
+    wl = displaced_grids(seed, grids, width, height, degenerate, box)
+    sc = wl.scene
+    parts, idx, nv = [], [], 0
+    for k in range(sc.nobj):
+        v0, v1 = int(sc.surf_vbase[k]), int(sc.surf_vbase[k + 1])
+        p0, p1 = int(sc.surf_pbase[k]), int(sc.surf_pbase[k + 1])
+        M = sc.obj_local2world[k].reshape(4, 4).T.astype(np.float64)     # column-major
+        w = sc.vertices[v0:v1].astype(np.float64) @ M.T
+        parts.append(np.column_stack([w[:, :3] / w[:, 3:4], np.ones(v1 - v0)]).astype(np.float32))
+        idx.append(sc.indices[p0:p1] + nv)
+        nv += v1 - v0
+    one = assemble([(np.concatenate(parts), np.concatenate(idx))],
+                   [(0, IDENTITY16, IDENTITY16, sc.obj_material[0])], sc.lights, name="displaced_grids_single")
+    return Workload(one, wl.camera, depth=wl.depth,
+                    desc=f"{grids} displaced grids ({grids * 256} patches) in one BVH, GGX, "
+                         "primary + shadow + 1 reflection bounce")
+
+
 def grazing_heightfield(seed=BASE_SEED + 5, n=128, width=128, height=128) -> Workload:
     """One n×n-patch height field (16,384 patches at n=128: a 14-level BVH)
     seen at a grazing angle, so that rays find both children of a node hit at
@@ -537,4 +562,5 @@ WORKLOADS = {
     "heightfield_c1": heightfield_c1,
     "closed_surfaces": closed_surfaces,
     "displaced_grids": displaced_grids,
+    "displaced_grids_single": displaced_grids_single,
 }

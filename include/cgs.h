@@ -408,6 +408,23 @@ cgs_status cgs_phf_fill_table(cgs_ctx* ctx, const cgs_phf* h, const uint32_t* di
 cgs_status cgs_phf_hash(cgs_ctx* ctx, const cgs_phf* h, const uint32_t* displacement,
                         const uint32_t* keys, uint32_t n, uint32_t* out, void* stream);
 
+/* The wide family (no counterpart in the reference, which widens its
+ * PerfectHashFunction.h:58-63 family): the same function, sizes, (A,B) draws
+ * and build, evaluated in exact integer arithmetic on keys of up to 64 bits,
+ * h(k) = (A·(k mod T) + B + D[k & mask]) mod T.  The reference's 32-bit
+ * A·k wraps, which breaks the function's near-bijection on dense heap keys:
+ * every reference attempt fails at 2^21 keys (one BVH over 2^20 patches),
+ * where the wide family succeeds.  Buckets are the reference's (k & mask, at
+ * most 4 keys), so key sets whose low bits spread like heap keys hash;
+ * others may overflow a bucket (CGS_ERR_BUCKET_OVERFLOW).  keys: n device
+ * u64; n below 2^30. */
+cgs_status cgs_phf_build_wide(cgs_ctx* ctx, const uint64_t* keys, uint32_t n, uint32_t seed,
+                              uint32_t* displacement, cgs_phf* out, void* stream);
+cgs_status cgs_phf_fill_table_wide(cgs_ctx* ctx, const cgs_phf* h, const uint32_t* displacement,
+                                   const uint64_t* keys, uint32_t n, uint32_t* table, void* stream);
+cgs_status cgs_phf_hash_wide(cgs_ctx* ctx, const cgs_phf* h, const uint32_t* displacement,
+                             const uint64_t* keys, uint32_t n, uint32_t* out, void* stream);
+
 /* ---------------------------------------------------------------------- */
 /* Scenes (rt::Scene + MainWindow::convertScene)                            */
 /* ---------------------------------------------------------------------- */
@@ -437,6 +454,12 @@ cgs_status cgs_scene_bvh_copy(cgs_scene* scene, int32_t surface, cgs_bvh_node* n
  * object transform is affine with a consistent inverse; CGS_CULL=0 in the
  * environment at scene creation turns it off. */
 int cgs_scene_culled(cgs_scene* scene);
+/* 1 when that BVH (surface >= 0, or -1: the top level) is hashed with the
+ * wide family, 0 with the reference's, -1 for a bad argument.  A scene
+ * hashes a BVH with the wide family when the reference family's build ends
+ * with a duplicate (from 2^21 heap keys: a surface of 2^20 patches); the
+ * reference would go on with that broken table. */
+int cgs_scene_bvh_wide(cgs_scene* scene, int32_t surface);
 
 /* Scenes are read-only once created: any context on the scene's device may
  * trace or render it, and several contexts may do so at the same time.

@@ -435,6 +435,26 @@ int orc_phf_build(const uint32_t* keys, uint64_t n, uint32_t seed, uint32_t* par
     return r;
 }
 
+// The wide family over 64-bit keys (no counterpart in the reference).
+int orc_phf_build_wide(const uint64_t* keys, uint64_t n, uint32_t seed, uint32_t* params5, uint32_t* D)
+{
+    PHF h;
+    int r = phfBuild<uint64_t>(h, std::span<const uint64_t>(keys, n), seed, true);
+    params5[0] = h.A; params5[1] = h.B; params5[2] = h.mask; params5[3] = h.tableSize; params5[4] = (uint32_t)h.attempts;
+    if (D && !h.D.empty()) std::memcpy(D, h.D.data(), 4 * h.D.size());
+    return r;
+}
+
+// h(key) of the wide family (params: A, B, mask, tableSize) for n keys.
+void orc_phf_hash_wide(const uint32_t* params4, const uint32_t* D, const uint64_t* keys, uint64_t n, uint32_t* out)
+{
+    PHF h;
+    h.A = params4[0]; h.B = params4[1]; h.mask = params4[2]; h.tableSize = params4[3];
+    h.D.assign(D, D + (size_t)h.mask + 1);
+    h.wide = true;
+    for (uint64_t i = 0; i < n; i++) out[i] = h.hash(keys[i]);
+}
+
 // --- scenes --------------------------------------------------------------------
 // Scene description (host arrays), identical in meaning to the product's
 // cgs_scene_desc: vertices (x,y,z,w) of all surfaces concatenated, patch
@@ -531,6 +551,12 @@ int orc_scene_bvh_sizes(const orc_scene* h, int surface, uint64_t* nnodes, uint6
     *ndispl = p.D.size();
     *ntable = p.tableSize;
     return 0;
+}
+
+// 1 when that BVH is hashed with the wide family.
+int orc_scene_bvh_wide(const orc_scene* h, int surface)
+{
+    return (surface < 0 ? h->s.topHash : h->s.surfaces.at(surface).hash).wide ? 1 : 0;
 }
 
 int orc_scene_bvh_copy(const orc_scene* h, int surface, uint32_t* nodes16, uint32_t* keys, uint32_t* indices,

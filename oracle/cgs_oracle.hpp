@@ -1060,6 +1060,12 @@ inline std::vector<Element> patchElements(const float* vertices4, const uint32_t
 //    advances a; a bucket reaching 32 attempts fails the (A,B) attempt.
 // ---------------------------------------------------------------------------
 
+// The wide family (no counterpart in the reference) is the same function
+// evaluated in exact integer arithmetic on keys of up to 64 bits:
+// h(k) = (A·(k mod T) + B + D[k & mask]) mod T, equal to (A·k + B + D) mod T
+// without the reference's 32-bit wrap of A·k, which breaks the function's
+// near-bijection on dense heap keys once A·k passes 2^32 (at 2^21 keys every
+// reference attempt fails).  Sizes, draws and the build are the reference's.
 struct PHF
 {
     uint32_t A = 31, B = 19;
@@ -1067,14 +1073,24 @@ struct PHF
     uint32_t tableSize = 0;
     std::vector<uint32_t> D; // displacement, size mask+1
     int attempts = 0;        // (A,B) pairs tried
-    uint32_t hash(uint32_t key) const { return (A * key + B + D[key & mask]) % tableSize; }
+    bool wide = false;       // the wide family
+    uint32_t hash(uint64_t key) const { return slot(key, D[(uint32_t)(key & mask)], tableSize); }
+    // PerfectHashFunction.h:58-63 (32-bit arithmetic), or the wide family
+    uint32_t slot(uint64_t key, uint32_t d, uint32_t T) const
+    {
+        if (wide) return (uint32_t)(((uint64_t)A * (key % T) + B + d) % T);
+        return (A * (uint32_t)key + B + d) % T;
+    }
 };
 
 // Returns 0 on success, 1 when the final function is not injective
 // (reference: "DUPLICATE FOUND!", PerfectHashFunction.cu:486-502), 2 when a
 // bucket holds more than 4 keys (reference: __trap in init_and_bucketsort).
-inline int phfBuild(PHF& h, std::span<const uint32_t> keys, uint32_t seed = 42)
+// wide: build the wide family (keys of up to 64 bits).
+template<class Key = uint32_t>
+inline int phfBuild(PHF& h, std::span<const Key> keys, uint32_t seed = 42, bool wide = false)
 {
+    h.wide = wide;
     constexpr uint32_t kCap = 4, kMaxAttempts = 32, kMaxFunctions = 16;
     const size_t n = keys.size();
     int s = int(n - 1) >> 1;
@@ -1088,10 +1104,11 @@ inline int phfBuild(PHF& h, std::span<const uint32_t> keys, uint32_t seed = 42)
     std::uniform_int_distribution<uint32_t> dist(0, (uint32_t)n);
 
     // buckets (input order inside each bucket)
-    std::vector<uint32_t> bsize(displSize, 0), bkeys(size_t(displSize) * kCap, 0);
+    std::vector<uint32_t> bsize(displSize, 0);
+    std::vector<Key> bkeys(size_t(displSize) * kCap, 0);
     for (size_t i = 0; i < n; i++)
     {
-        uint32_t j = keys[i] & h.mask;
+        uint32_t j = (uint32_t)(keys[i] & h.mask);
         uint32_t k = bsize[j]++;
         if (k >= kCap) return 2;
         bkeys[size_t(j) * kCap + k] = keys[i];
@@ -1137,8 +1154,8 @@ inline int phfBuild(PHF& h, std::span<const uint32_t> keys, uint32_t seed = 42)
                     bool ok = true;
                     for (int k = 0; k < sz; k++)
                     {
-                        uint32_t key = bkeys[size_t(b) * kCap + k];
-                        uint32_t sl = (h.A * key + h.B + d) % T;
+                        const Key key = bkeys[size_t(b) * kCap + k];
+                        uint32_t sl = h.slot(key, d, T);
                         slots[size_t(g) * kCap + k] = sl;
                         if (occ[sl]) ok = false;
                         for (int m = 0; m < k; m++)
@@ -1188,7 +1205,7 @@ inline int phfBuild(PHF& h, std::span<const uint32_t> keys, uint32_t seed = 42)
     if (h.tableSize != T) h.tableSize = T;
     std::vector<uint8_t> seen(h.tableSize, 0);
     bool dup = false;
-    for (uint32_t k : keys)
+    for (Key k : keys)
     {
         uint32_t v = h.hash(k);
         if (seen[v]) dup = true;
@@ -1198,7 +1215,8 @@ inline int phfBuild(PHF& h, std::span<const uint32_t> keys, uint32_t seed = 42)
 }
 
 // rt/BVH.cpp:24-38 — table[hash(key_i)] = i, EMPTY elsewhere.
-inline std::vector<uint32_t> hashTable(const PHF& h, std::span<const uint32_t> keys)
+template<class Key = uint32_t>
+inline std::vector<uint32_t> hashTable(const PHF& h, std::span<const Key> keys)
 {
     std::vector<uint32_t> t(h.tableSize, EMPTY);
     for (size_t i = 0; i < keys.size(); i++) t[h.hash(keys[i])] = (uint32_t)i;

@@ -91,6 +91,9 @@ def lib():
         L.orc_phf_displacement_size.argtypes = [_U64]
         L.orc_phf_displacement_size.restype = _U32
         L.orc_phf_build.argtypes = [_P, _U64, _U32, _P, _P]
+        L.orc_phf_build_wide.argtypes = [_P, _U64, _U32, _P, _P]
+        L.orc_phf_hash_wide.argtypes = [_P, _P, _P, _U64, _P]
+        L.orc_scene_bvh_wide.argtypes = [_P, C.c_int]
         L.orc_scene_create.argtypes = [C.POINTER(SceneDesc), C.POINTER(C.c_int)]
         L.orc_scene_create.restype = _P
         L.orc_scene_destroy.argtypes = [_P]
@@ -296,6 +299,24 @@ def phf_build(keys, seed=42):
     return rc, dict(A=int(p[0]), B=int(p[1]), mask=int(p[2]), table_size=int(p[3]), attempts=int(p[4])), D
 
 
+def phf_build_wide(keys, seed=42):
+    """The wide family (exact arithmetic, 64-bit keys): (status, params, D)."""
+    keys = np.ascontiguousarray(keys, np.uint64)
+    p = np.zeros(5, np.uint32)
+    D = np.zeros(lib().orc_phf_displacement_size(keys.shape[0]), np.uint32)
+    rc = lib().orc_phf_build_wide(_ptr(keys), keys.shape[0], seed, _ptr(p), _ptr(D))
+    return rc, dict(A=int(p[0]), B=int(p[1]), mask=int(p[2]), table_size=int(p[3]), attempts=int(p[4])), D
+
+
+def phf_hash_wide(params, D, keys):
+    keys = np.ascontiguousarray(keys, np.uint64)
+    p = np.array([params["A"], params["B"], params["mask"], params["table_size"]], np.uint32)
+    D = np.ascontiguousarray(D, np.uint32)
+    out = np.zeros(keys.shape[0], np.uint32)
+    lib().orc_phf_hash_wide(_ptr(p), _ptr(D), _ptr(keys), keys.shape[0], _ptr(out))
+    return out
+
+
 # ---------------------------------------------------------------------------
 # scenes
 # ---------------------------------------------------------------------------
@@ -334,7 +355,8 @@ class OracleScene:
         T = np.zeros(nt.value, np.uint32)
         lib().orc_scene_bvh_copy(self.h, surface, _ptr(nodes), _ptr(keys), _ptr(idx), _ptr(p), _ptr(D), _ptr(T))
         return dict(nodes=nodes, keys=keys, indices=idx, A=int(p[0]), B=int(p[1]), mask=int(p[2]),
-                    table_size=int(p[3]), attempts=int(p[4]), displacement=D, table=T)
+                    table_size=int(p[3]), attempts=int(p[4]), displacement=D, table=T,
+                    wide=bool(lib().orc_scene_bvh_wide(self.h, surface)))
 
     def trace(self, rays, any_hit=False, threads=None, stack=False, method=0, clip_tol=0.0, subdiv_depth=0,
               max_clip_iters=0, polish=False):

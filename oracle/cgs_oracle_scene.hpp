@@ -109,16 +109,19 @@ inline int buildScene(Scene& s, uint32_t seed = 42)
         e.centroid = e.bounds.center();
     }
     s.top = bvhBuild(el, 1);
-    for (auto& surf : s.surfaces)
-    {
-        int r = phfBuild(surf.hash, surf.bvh.keys, seed);
+    // A BVH whose reference-family build fails (a duplicate: from 2^21 heap
+    // keys on) is hashed with the wide family instead; a bucket overflow
+    // fails both.
+    auto hashOne = [&](PHF& h, const std::vector<uint32_t>& keys, std::vector<uint32_t>& table) -> int {
+        int r = phfBuild<uint32_t>(h, keys, seed);
+        if (r == 1) r = phfBuild<uint32_t>(h, keys, seed, true);
         if (r) return r;
-        surf.table = hashTable(surf.hash, surf.bvh.keys);
-    }
-    int r = phfBuild(s.topHash, s.top.keys, seed);
-    if (r) return r;
-    s.topTable = hashTable(s.topHash, s.top.keys);
-    return 0;
+        table = hashTable<uint32_t>(h, keys);
+        return 0;
+    };
+    for (auto& surf : s.surfaces)
+        if (int r = hashOne(surf.hash, surf.bvh.keys, surf.table)) return r;
+    return hashOne(s.topHash, s.top.keys, s.topTable);
 }
 
 struct TraceHit

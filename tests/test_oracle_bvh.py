@@ -334,3 +334,72 @@ def test_mesh_elements_and_bvh_match_oracle(oracle):
     assert runs.max() <= 17
     with pytest.raises(cgs.CgsError, match="out of range"):
         cgs.Context.mesh_elements(V[:3], np.array([[0, 1, 3]], np.uint32))
+
+
+# ---------------------------------------------------------------------------
+# The wide family: the reference's function in exact arithmetic, 64-bit keys
+# ---------------------------------------------------------------------------
+
+def wide_hash_bigint(params, D, keys):
+    """h(k) = (A·k + B + D[k & mask]) mod T with Python integers (no wrap)."""
+    A, B, mask, T = params["A"], params["B"], params["mask"], params["table_size"]
+    return np.array([(A * int(k) + B + int(D[int(k) & mask])) % T for k in keys], np.uint64)
+
+
+def check_wide(oracle, keys, seed=42):
+    st, p, D = oracle.phf_build_wide(keys, seed)
+    assert st == 0, st
+    n = keys.shape[0]
+    assert p["mask"] + 1 == oracle.lib().orc_phf_displacement_size(n)
+    step = max(4, int(np.float32(0.05) * np.float32(n)))
+    assert p["table_size"] >= n and (p["table_size"] - n) % step == 0
+    assert p["table_size"] == n + (p["attempts"] - 1) * step
+    h = oracle.phf_hash_wide(p, D, keys)
+    assert h.max() < p["table_size"]
+    assert np.unique(h).shape[0] == n, "not injective"
+    # the function is the reference's family evaluated without wrapping
+    sub = keys[:: max(1, n // 20000)]
+    assert np.array_equal(oracle.phf_hash_wide(p, D, sub).astype(np.uint64), wide_hash_bigint(p, D, sub))
+    return st, p, D
+
+
+def test_wide_family_hashes_two_million_heap_keys(oracle):
+    """One BVH over 2^20 patches (2,097,151 heap keys), where every reference
+    attempt fails, hashes with the wide family; so does 2^21 patches."""
+    for npatch in (1 << 20, 1 << 21):
+        _, p, _ = check_wide(oracle, heap_keys(npatch))
+        assert p["attempts"] <= 2
+
+
+def test_wide_family_equals_the_reference_where_nothing_wraps(oracle):
+    """For key sets small enough that A·k + B + d stays below 2^32, the wide
+    family's build is the reference's build: the same (A,B), table size and
+    displacements."""
+    for npatch in (5, 100, 3000, 1 << 14):
+        keys = heap_keys(npatch)
+        st, p, D = oracle.phf_build(keys)
+        stw, pw, Dw = check_wide(oracle, keys.astype(np.uint64))
+        assert st == stw == 0
+        assert p == pw and np.array_equal(D, Dw)
+
+
+def test_wide_family_64_bit_keys(oracle):
+    """Heap keys lifted above 2^32 (the same low bits, so the same buckets):
+    the build succeeds and equals the big-integer formula."""
+    keys = heap_keys(1 << 16).astype(np.uint64) + (np.uint64(0x5A5) << np.uint64(40))
+    check_wide(oracle, keys)
+    # keys differing only above the bucket bits share buckets: five of them overflow one
+    bad = np.array([7 + (k << 33) for k in range(5)], np.uint64)
+    st, _, _ = oracle.phf_build_wide(bad)
+    assert st == 2
+
+
+def test_wide_family_pin_catches_a_wrap():
+    """The big-integer pin fails for a function that wraps A·k at 32 bits,
+    the reference's arithmetic, on keys past 2^32 / A."""
+    p = dict(A=2_000_003, B=17, mask=7, table_size=1_000_003)
+    D = np.arange(8, dtype=np.uint32)
+    keys = np.arange(3_000_000, 3_000_100, dtype=np.uint64)
+    wrapped = np.array([((p["A"] * int(k)) % (1 << 32) + p["B"] + int(D[int(k) & 7])) % p["table_size"]
+                        for k in keys], np.uint64)
+    assert not np.array_equal(wrapped, wide_hash_bigint(p, D, keys))
