@@ -496,6 +496,13 @@ class DeviceScene:
         check(lib().cgs_scene_create(ctx.h, C.byref(d), C.byref(p)), "cgs_scene_create")
         self.h = p.value
 
+    def bvh_culled(self, surface: int) -> bool:
+        """Whether that BVH (-1: the top level) has a cull tree."""
+        r = lib().cgs_scene_bvh_culled(self.h, surface)
+        if r < 0:
+            raise ValueError(f"no BVH {surface}")
+        return r == 1
+
     def bvh_wide(self, surface: int) -> bool:
         """Whether that BVH (-1: the top level) is hashed with the wide family."""
         r = lib().cgs_scene_bvh_wide(self.h, surface)

@@ -35,6 +35,7 @@ EXPORTS = [
     "cgs_dl_read", "cgs_dl_parse", "cgs_dl_free",
     "cgs_phf_displacement_size", "cgs_phf_build", "cgs_phf_build_static", "cgs_phf_fill_table", "cgs_phf_hash",
     "cgs_phf_build_wide", "cgs_phf_fill_table_wide", "cgs_phf_hash_wide", "cgs_scene_bvh_wide",
+    "cgs_scene_bvh_culled",
     "cgs_scene_create", "cgs_scene_destroy", "cgs_scene_bvh_info", "cgs_scene_bvh_copy", "cgs_scene_culled",
     "cgs_trace_closest", "cgs_trace_any", "cgs_trace_closest_ex", "cgs_trace_any_ex", "cgs_render",
     "cgs_render_running", "cgs_shade", "cgs_work_counts",
@@ -172,6 +173,7 @@ def lib():
             "cgs_phf_fill_table_wide": ([_P, C.POINTER(PHF), _P, _P, _U32, _P, _P], C.c_int),
             "cgs_phf_hash_wide": ([_P, C.POINTER(PHF), _P, _P, _U32, _P, _P], C.c_int),
             "cgs_scene_bvh_wide": ([_P, _I32], C.c_int),
+            "cgs_scene_bvh_culled": ([_P, _I32], C.c_int),
             "cgs_scene_create": ([_P, C.POINTER(SceneDesc), C.POINTER(_P)], C.c_int),
             "cgs_scene_destroy": ([_P], C.c_int),
             "cgs_scene_bvh_info": ([_P, _I32, C.POINTER(_U32), C.POINTER(_U32), C.POINTER(PHF)], C.c_int),

@@ -1,8 +1,9 @@
 // SPDX-License-Identifier: MIT
 //
 // Cull trees: a surface-area-heuristic BVH over the leaf boxes of a scene's
-// top-level BVH, which the ray queries use to find the objects a ray can
-// reach without walking the reference's median-split tree (whose split axis
+// top-level BVH (or of a large surface's patch BVH), which the ray queries
+// use to find the objects (patches) a ray can reach without walking the
+// reference's median-split tree (whose split axis
 // is always y, rt/BVH.cpp:46-52, so its boxes are thin y-slabs spanning the
 // whole scene and a ray through the scene crosses most of them).  The tree
 // only chooses which objects to test: every object test still uses the
@@ -271,6 +272,26 @@ bool buildCullTree(const cgs_bvh_node* refNodes, size_t nnodes, const uint32_t* 
     }
     out.nodeCount = (uint32_t)nodes.size();
     return true;
+}
+
+uint32_t appendCullTree(CullTreeHost& all, const CullTreeHost& t)
+{
+    const uint32_t nodeBase = all.nodeCount, elemBase = (uint32_t)(all.elems.size() / 8);
+    const size_t n0 = all.nodes.size(), l0 = all.links.size();
+    all.nodes.insert(all.nodes.end(), t.nodes.begin(), t.nodes.end());
+    all.links.insert(all.links.end(), t.links.begin(), t.links.end());
+    all.elems.insert(all.elems.end(), t.elems.begin(), t.elems.end());
+    for (size_t i = 0; i < t.nodeCount; i++)
+    {
+        uint32_t first;
+        std::memcpy(&first, &all.nodes[n0 + 8 * i + 3], 4);
+        first += elemBase;
+        std::memcpy(&all.nodes[n0 + 8 * i + 3], &first, 4);
+    }
+    for (size_t i = l0; i < all.links.size(); i++)
+        if (all.links[i] != CGS_EMPTY) all.links[i] += nodeBase;
+    all.nodeCount += t.nodeCount;
+    return nodeBase;
 }
 
 } // namespace cgs

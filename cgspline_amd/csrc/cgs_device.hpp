@@ -842,10 +842,20 @@ CGS_DEV NodeBoxes loadNodeCached(const BVHRef& b, const NodeCache& nc, uint32_t 
 // PerfectHashFunction::hash (PerfectHashFunction.h:58-63): 32-bit
 // arithmetic, or for a BVH hashed with the wide family the same function in
 // exact arithmetic, (A·(k mod T) + B + D) mod T.
+template<bool BIG = true>
 CGS_DEV uint32_t hashKey(const BVHRef& b, uint32_t key)
 {
-    if (b.wide)
-        return (uint32_t)(((uint64_t)b.A * fastmod(key, b.T) + b.B + b.displ[key & b.mask]) % (uint64_t)b.T.d);
+    if (BIG && b.wide)
+    {
+        // x mod T through a double quotient: x < 2^63 and T < 2^31, so the
+        // quotient is off by at most one and one correction each way is exact
+        const uint64_t x = (uint64_t)b.A * fastmod(key, b.T) + b.B + b.displ[key & b.mask];
+        const double q = __builtin_floor((double)x / (double)b.T.d);
+        int64_t r = (int64_t)x - (int64_t)q * (int64_t)b.T.d;
+        r += r < 0 ? (int64_t)b.T.d : 0;
+        r -= r >= (int64_t)b.T.d ? (int64_t)b.T.d : 0;
+        return (uint32_t)r;
+    }
     uint32_t x = b.A * key + b.B + b.displ[key & b.mask];
     return fastmod(x, b.T);
 }
@@ -901,6 +911,7 @@ struct TravState
 // ray misses.  Returns false at the end of the traversal (nothing postponed),
 // or on a table entry past the node array (a corrupt table ends the ray
 // instead of faulting).
+template<bool BIG = true>
 CGS_DEV bool ascend(const BVHRef& b, const NodeCache& nc, const SiblingRing& ring, TravState& st)
 {
     if (!st.postponed) return false;
@@ -911,7 +922,7 @@ CGS_DEV bool ascend(const BVHRef& b, const NodeCache& nc, const SiblingRing& rin
     if (st.rs.pop(ring, st.node)) return true;
     if (st.key < nc.keys) return true;    // staged in LDS: read by key, no lookup
     CGS_COUNT(kWorkProbe);
-    st.node = b.table[hashKey(b, st.key)];
+    st.node = b.table[hashKey<BIG>(b, st.key)];
     return st.node < b.nodeCount;
 }
 
@@ -922,7 +933,7 @@ CGS_DEV bool ascend(const BVHRef& b, const NodeCache& nc, const SiblingRing& rin
 // ascends: the per-lane order of node visits and leaf tests is the
 // reference's, while the loop itself stays small enough to keep its
 // registers, and lanes that reach leaves together test them together.
-template<int OCT = -1, bool LOW = false>
+template<int OCT = -1, bool LOW = false, bool BIG = true>
 CGS_DEV uint32_t nextLeaf(const BVHRef& b, const NodeCache& nc, const SiblingRing& ring, V3 o, V3 Dinv,
                           const float& limit, TravState& st)
 {
@@ -944,7 +955,7 @@ CGS_DEV uint32_t nextLeaf(const BVHRef& b, const NodeCache& nc, const SiblingRin
             st.node = goLeft ? n.left : n.right;
             continue;
         }
-        if (!ascend(b, nc, ring, st)) return EMPTY;
+        if (!ascend<BIG>(b, nc, ring, st)) return EMPTY;
     }
 }
 
@@ -968,21 +979,21 @@ CGS_DEV int waveOctant(V3 o, V3 Dinv, uint32_t finiteBoxes)
 
 // nextLeaf with the near/far planes fixed at compile time when the wave
 // shares a direction octant (a wave-uniform branch), per lane otherwise.
-template<bool LOW>
+template<bool LOW, bool BIG = true>
 CGS_DEV uint32_t nextLeafOct(int oct, const BVHRef& b, const NodeCache& nc, const SiblingRing& ring, V3 o, V3 Dinv,
                              const float& limit, TravState& st)
 {
     switch (oct)
     {
-    case 0: return nextLeaf<0, LOW>(b, nc, ring, o, Dinv, limit, st);
-    case 1: return nextLeaf<1, LOW>(b, nc, ring, o, Dinv, limit, st);
-    case 2: return nextLeaf<2, LOW>(b, nc, ring, o, Dinv, limit, st);
-    case 3: return nextLeaf<3, LOW>(b, nc, ring, o, Dinv, limit, st);
-    case 4: return nextLeaf<4, LOW>(b, nc, ring, o, Dinv, limit, st);
-    case 5: return nextLeaf<5, LOW>(b, nc, ring, o, Dinv, limit, st);
-    case 6: return nextLeaf<6, LOW>(b, nc, ring, o, Dinv, limit, st);
-    case 7: return nextLeaf<7, LOW>(b, nc, ring, o, Dinv, limit, st);
-    default: return nextLeaf<-1, LOW>(b, nc, ring, o, Dinv, limit, st);
+    case 0: return nextLeaf<0, LOW, BIG>(b, nc, ring, o, Dinv, limit, st);
+    case 1: return nextLeaf<1, LOW, BIG>(b, nc, ring, o, Dinv, limit, st);
+    case 2: return nextLeaf<2, LOW, BIG>(b, nc, ring, o, Dinv, limit, st);
+    case 3: return nextLeaf<3, LOW, BIG>(b, nc, ring, o, Dinv, limit, st);
+    case 4: return nextLeaf<4, LOW, BIG>(b, nc, ring, o, Dinv, limit, st);
+    case 5: return nextLeaf<5, LOW, BIG>(b, nc, ring, o, Dinv, limit, st);
+    case 6: return nextLeaf<6, LOW, BIG>(b, nc, ring, o, Dinv, limit, st);
+    case 7: return nextLeaf<7, LOW, BIG>(b, nc, ring, o, Dinv, limit, st);
+    default: return nextLeaf<-1, LOW, BIG>(b, nc, ring, o, Dinv, limit, st);
     }
 }
 
@@ -992,7 +1003,7 @@ CGS_DEV uint32_t nextLeafOct(int oct, const BVHRef& b, const NodeCache& nc, cons
 // OCTANTS: specialise the node loop on the wave's direction octant (worth its
 // code size where a wave's rays are coherent: the top level, in world space).
 // LOW: the BVH's nodes lie in the first 4 GiB of the node array (loadNodeLow).
-template<bool OCTANTS = false, bool LOW = false, class Leaf>
+template<bool OCTANTS = false, bool LOW = false, bool BIG = true, class Leaf>
 CGS_DEV bool hashIntersect(const BVHRef& b, V3 o, V3 dir, const float& limit, Leaf&& leaf,
                            const NodeCache& nc = NodeCache {nullptr, 0}, const SiblingRing& ring = {nullptr, 0})
 {
@@ -1001,8 +1012,8 @@ CGS_DEV bool hashIntersect(const BVHRef& b, V3 o, V3 dir, const float& limit, Le
     const int oct = OCTANTS ? waveOctant(o, Dinv, b.finiteBoxes) : -1;
     bool result = false;
     uint32_t i;
-    while ((i = (OCTANTS ? nextLeafOct<LOW>(oct, b, nc, ring, o, Dinv, limit, st)
-                         : nextLeaf<-1, LOW>(b, nc, ring, o, Dinv, limit, st))) != EMPTY)
+    while ((i = (OCTANTS ? nextLeafOct<LOW, BIG>(oct, b, nc, ring, o, Dinv, limit, st)
+                         : nextLeaf<-1, LOW, BIG>(b, nc, ring, o, Dinv, limit, st))) != EMPTY)
     {
         uint32_t e;
         while ((e = b.indices[i]) != EMPTY)
@@ -1011,13 +1022,13 @@ CGS_DEV bool hashIntersect(const BVHRef& b, V3 o, V3 dir, const float& limit, Le
             result |= leaf(e);
             i++;
         }
-        if (!ascend(b, nc, ring, st)) break;
+        if (!ascend<BIG>(b, nc, ring, st)) break;
     }
     return result;
 }
 
 // Stackless any-hit traversal against `tmax` (rt/BVH.h:268-350).
-template<bool OCTANTS = false, bool LOW = false, class Leaf>
+template<bool OCTANTS = false, bool LOW = false, bool BIG = true, class Leaf>
 CGS_DEV bool hashIntersectAny(const BVHRef& b, V3 o, V3 dir, float tmax, Leaf&& leaf,
                               const NodeCache& nc = NodeCache {nullptr, 0}, const SiblingRing& ring = {nullptr, 0})
 {
@@ -1025,8 +1036,8 @@ CGS_DEV bool hashIntersectAny(const BVHRef& b, V3 o, V3 dir, float tmax, Leaf&& 
     TravState st {b.root, 1u, 0u, RingState {}};
     const int oct = OCTANTS ? waveOctant(o, Dinv, b.finiteBoxes) : -1;
     uint32_t i;
-    while ((i = (OCTANTS ? nextLeafOct<LOW>(oct, b, nc, ring, o, Dinv, tmax, st)
-                         : nextLeaf<-1, LOW>(b, nc, ring, o, Dinv, tmax, st))) != EMPTY)
+    while ((i = (OCTANTS ? nextLeafOct<LOW, BIG>(oct, b, nc, ring, o, Dinv, tmax, st)
+                         : nextLeaf<-1, LOW, BIG>(b, nc, ring, o, Dinv, tmax, st))) != EMPTY)
     {
         uint32_t e;
         while ((e = b.indices[i]) != EMPTY)
@@ -1035,7 +1046,7 @@ CGS_DEV bool hashIntersectAny(const BVHRef& b, V3 o, V3 dir, float tmax, Leaf&& 
             if (leaf(e)) return true;
             i++;
         }
-        if (!ascend(b, nc, ring, st)) break;
+        if (!ascend<BIG>(b, nc, ring, st)) break;
     }
     return false;
 }

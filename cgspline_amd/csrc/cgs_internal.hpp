@@ -107,6 +107,11 @@ struct CullTreeHost
     uint32_t nodeCount = 0;
 };
 bool buildCullTree(const cgs_bvh_node* refNodes, size_t nnodes, const uint32_t* indices, CullTreeHost& out);
+// Appends tree t to the combined arrays `all` (node links and element starts
+// rebased); returns t's root node in `all`.
+uint32_t appendCullTree(CullTreeHost& all, const CullTreeHost& t);
+// Surfaces of at least this many patches get a cull tree of their own.
+constexpr uint32_t kSurfaceCullMinPatches = 4096;
 
 // Persistent kernels: the grid is the resident workgroups; every lane owns a
 // slice of the region-stack buffer and waves pull work from a counter.
@@ -150,9 +155,10 @@ hipError_t launchTraceAny(const SceneDev& sc, const PersistArgs& pa, const cgs_r
 // Batched queries with leaf method `m` (closest-hit jobs polished when it
 // asks); the counter must be zero at launch (cleared with a fill kernel, so
 // the pair can be captured in a graph).
+// big: some job's scene has a wide-hashed BVH or a surface cull tree.
 hipError_t launchTraceBatch(const BatchJob* jobs, uint32_t njobs, uint32_t total, uint32_t chunk, float2* stack,
                             uint32_t lanes, unsigned int* counter, uint32_t grid, unsigned long long* overflow, hipStream_t s,
-                            const TraceMode& m = TraceMode {});
+                            const TraceMode& m, bool big);
 hipError_t launchClipBatch(const PersistArgs& pa, const cgs_ray* rays, const float* patches, uint32_t n, float tol,
                            cgs_hit* hits, hipStream_t s);
 hipError_t launchSubdivBatch(const cgs_ray* rays, const float* patches, uint32_t n, float threshold, cgs_hit* hits,

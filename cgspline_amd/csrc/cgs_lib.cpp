@@ -514,12 +514,14 @@ struct cgs_scene
         s.objects = objects.as<DevObject>();
         s.lights = lights.as<DevLight>();
         s.topByKey = topByKey.as<uint32_t>();
-        s.cullNodes = cull ? cullNodes.as<float4>() : nullptr;
-        s.cullLinks = cull ? cullLinks.as<uint2>() : nullptr;
-        s.cullElems = cull ? cullElems.as<float4>() : nullptr;
+        s.cullNodes = cullNodes.as<float4>();
+        s.cullLinks = cullLinks.as<uint2>();
+        s.cullElems = cullElems.as<float4>();
         s.nlight = nlight;
         s.nobj = nobj;
         s.nodeCount = nodeCount;
+        s.big = 0;
+        for (const DevBVH& b : dev) s.big |= b.wide | (b.cullRoot != CGS_EMPTY && &b != &dev.back() ? 1u : 0u);
         s.top = dev.back();
         for (int i = 0; i < 3; i++)
         {
@@ -836,10 +838,15 @@ cgs_status cgs_phf_hash(cgs_ctx* ctx, const cgs_phf* h, const uint32_t* displace
 // size plus distance from the origin, in world units.
 // CGS_CULL=0 in the environment turns the tree off (a diagnostic: the
 // results are the same).
+static bool cullEnabled()
+{
+    const char* e = std::getenv("CGS_CULL");
+    return !(e && e[0] == '0');
+}
+
 static bool cullAllowed(const cgs_scene_desc* d)
 {
-    if (const char* e = std::getenv("CGS_CULL"))
-        if (e[0] == '0') return false;
+    if (!cullEnabled()) return false;
     if (d->nobj < 2) return false;
     for (uint64_t v = 0; v < d->surf_vbase[d->nsurf]; v++)
         if (d->vertices[4 * v + 3] != 1.0f) return false;
@@ -965,10 +972,24 @@ cgs_status cgs_scene_create(cgs_ctx* ctx, const cgs_scene_desc* d, cgs_scene** o
         if (s != CGS_OK) return s;
     }
 
-    // 2b. the cull tree over the top-level leaf boxes, where the scene allows one
-    CullTreeHost cullTree;
-    sc->cull = cullAllowed(d) && buildCullTree(sc->bvh[S].nodes.data(), sc->bvh[S].nodes.size(),
-                                              sc->bvh[S].indices.data(), cullTree);
+    // 2b. cull trees: over the top-level leaf boxes where the scene allows
+    // one, and over the leaf boxes of each surface of at least
+    // kSurfaceCullMinPatches patches whose control points all have weight 1
+    // (a hit then lies in its patch's box); all in one set of arrays
+    CullTreeHost cullTree, one;
+    std::vector<uint32_t> cullRoot(S + 1, CGS_EMPTY);
+    if (cullAllowed(d) && buildCullTree(sc->bvh[S].nodes.data(), sc->bvh[S].nodes.size(), sc->bvh[S].indices.data(), one))
+        cullRoot[S] = appendCullTree(cullTree, one);
+    sc->cull = cullRoot[S] != CGS_EMPTY;
+    if (cullEnabled())
+        for (uint32_t k = 0; k < S; k++)
+        {
+            if (d->surf_pbase[k + 1] - d->surf_pbase[k] < kSurfaceCullMinPatches) continue;
+            bool unit = true;
+            for (uint64_t v = d->surf_vbase[k]; v < d->surf_vbase[k + 1] && unit; v++) unit = d->vertices[4 * v + 3] == 1.0f;
+            if (unit && buildCullTree(sc->bvh[k].nodes.data(), sc->bvh[k].nodes.size(), sc->bvh[k].indices.data(), one))
+                cullRoot[k] = appendCullTree(cullTree, one);
+        }
 
     // 3. layout: absolute node offsets, leaf slots and patch bases.  The
     // top-level BVH (index S) comes first in the node array: the kernels
@@ -1054,7 +1075,7 @@ cgs_status cgs_scene_create(cgs_ctx* ctx, const cgs_scene_desc* d, cgs_scene** o
     if (d->nlight)
         HIP_TRY(hipMemcpyAsync(sc->lights.p, d->lights, d->nlight * sizeof(DevLight), hipMemcpyHostToDevice, s));
     sc->nlight = d->nlight;
-    if (sc->cull)
+    if (cullTree.nodeCount)
     {
         HIP_TRY(sc->cullNodes.alloc(cullTree.nodes.size() * 4));
         HIP_TRY(sc->cullLinks.alloc(cullTree.links.size() * 4));
@@ -1126,6 +1147,7 @@ cgs_status cgs_scene_create(cgs_ctx* ctx, const cgs_scene_desc* d, cgs_scene** o
         b.patchBase = k < S ? d->surf_pbase[k] : 0;
         b.Tm = fastmodM(r.T);
         b.wide = wide[k];
+        b.cullRoot = cullRoot[k];
         b.finiteBoxes = 1;
         for (const cgs_bvh_node& n : sc->bvh[k].nodes)
             if (n.left != CGS_EMPTY)
@@ -1162,6 +1184,12 @@ cgs_status cgs_scene_bvh_info(cgs_scene* sc, int32_t surface, uint32_t* node_cou
 }
 
 int cgs_scene_culled(cgs_scene* sc) { return sc ? (sc->cull ? 1 : 0) : -1; }
+
+int cgs_scene_bvh_culled(cgs_scene* sc, int32_t surface)
+{
+    if (!sc || surface < -1 || surface >= (int32_t)sc->nsurf) return -1;
+    return sc->dev[surface < 0 ? sc->nsurf : (uint32_t)surface].cullRoot != CGS_EMPTY ? 1 : 0;
+}
 
 int cgs_scene_bvh_wide(cgs_scene* sc, int32_t surface)
 {
@@ -1499,6 +1527,14 @@ static cgs_status validChunk(uint32_t chunk)
     return CGS_OK;
 }
 
+// Whether some job's scene needs the kernels' wide-hash and surface-cull paths.
+static bool anyBig(const std::vector<BatchJob>& table)
+{
+    for (const BatchJob& b : table)
+        if (b.sc.big) return true;
+    return false;
+}
+
 // One batched launch through the context's own buffers (waits for the
 // previous one to finish before reusing them).
 static cgs_status traceBatch(cgs_ctx* ctx, const cgs_batch_job* jobs, uint32_t njobs, uint32_t chunk, uint32_t bpc,
@@ -1524,7 +1560,7 @@ static cgs_status traceBatch(cgs_ctx* ctx, const cgs_batch_job* jobs, uint32_t n
     HIP_TRY(hipStreamSynchronize(s)); // the host table is released on return
     HIP_TRY(launchTraceBatch(ctx->batchTable.as<BatchJob>(), (uint32_t)table.size(), total, chunk, pa.stack,
                              std::min<uint32_t>(pa.lanes, (uint32_t)ctx->stackLanes), pa.counter, grid,
-                             ctx->dOverflow, s, mode));
+                             ctx->dOverflow, s, mode, anyBig(table)));
     HIP_TRY(hipEventRecord(ctx->batchDone, s));
     ctx->batchLaunched = true;
     return persistEnd(ctx, s);
@@ -1582,7 +1618,7 @@ cgs_status cgs_batch_create(cgs_ctx* ctx, const cgs_batch_job* jobs, uint32_t nj
     if (e == hipSuccess)
         e = launchTraceBatch(b->table.as<BatchJob>(), (uint32_t)table.size(), total, ctx->batchChunk,
                              b->stack.as<float2>(), lanes, b->counter.as<unsigned int>(), grid,
-                             b->overflow.as<unsigned long long>(), cs, ctx->batchMode);
+                             b->overflow.as<unsigned long long>(), cs, ctx->batchMode, anyBig(table));
     hipGraph_t g = nullptr;
     hipError_t e2 = hipStreamEndCapture(cs, &g);
     if (e == hipSuccess) e = e2;
@@ -2145,9 +2181,11 @@ struct cgs_server
             if ((e = hipMemcpyAsync(sl.table.p, jt, nTrace * sizeof(BatchJob), hipMemcpyHostToDevice, sl.stream)) !=
                 hipSuccess)
                 return e;
+            bool big = false;
+            for (size_t k = 0; k < nTrace; k++) big |= jt[k].sc.big != 0;
             if ((e = launchTraceBatch(sl.table.as<BatchJob>(), (uint32_t)nTrace, traceRays, chunk, sl.stack.as<float2>(),
                                       lanes, sl.counter.as<unsigned int>(), grid, sl.overflow.as<unsigned long long>(),
-                                      sl.stream)) != hipSuccess)
+                                      sl.stream, TraceMode {}, big)) != hipSuccess)
                 return e;
         }
         if (nShade)

@@ -35,8 +35,10 @@ struct DevBVH
     uint64_t Tm;          // fastmod multiplier for T
     uint32_t finiteBoxes; // 1: every child box of an inner node is finite
     uint32_t wide;        // 1: hashed with the wide family (exact arithmetic)
+    uint32_t cullRoot;    // root of its cull tree in the scene's cull arrays, or EMPTY
+    uint32_t pad[3];
 };
-static_assert(sizeof(DevBVH) == 48, "DevBVH layout");
+static_assert(sizeof(DevBVH) == 64, "DevBVH layout");
 
 struct DevObject
 {
@@ -71,15 +73,16 @@ struct SceneDev
     const DevObject* __restrict__ objects;
     const DevLight* __restrict__ lights;
     const uint32_t* __restrict__ topByKey;   // kTopKeyCap entries
-    // cull tree over the top-level leaf boxes (cgs_cull.cpp), or null where
-    // the scene does not allow one (the queries then use the reference's
-    // traversal alone)
+    // cull trees (cgs_cull.cpp): over the top-level leaf boxes where the
+    // scene allows one (top.cullRoot), and over the leaf boxes of each large
+    // surface (its cullRoot); null when there is none
     const float4* __restrict__ cullNodes;
     const uint2* __restrict__ cullLinks;
     const float4* __restrict__ cullElems;
     uint32_t nlight;
     uint32_t nobj;
     uint32_t nodeCount;
+    uint32_t big;         // 1: some BVH is hashed with the wide family or some surface has a cull tree
     DevBVH top;
     float background[3];
     float ambient[3];

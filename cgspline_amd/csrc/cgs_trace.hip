@@ -273,88 +273,10 @@ CGS_DEV Rings laneRings(uint32_t* lds)
                   {CGS_RING_BOT ? lane + CGS_RING_TOP * kRingStride : nullptr, CGS_RING_BOT}};
 }
 
-// rt::intersect(BezierSurface, hit, ray) (Primitive.cu:222-248) for object
-// i, from RayTracer.cu:305-333: the object-local ray, the per-object hashed
-// traversal from t = ray.max·|M d|, and the leaf test at each patch.  On a
-// hit, t is the world distance (local t / |M d|).
-template<int METHOD>
-CGS_DEV bool objectClosest(const SceneDev& sc, const Rings& rg, uint32_t i, V3 o, V3 d, float tmax,
-                           const ClipStack& stk, bool& overflow, const TraceParams& tp, float& t, float& u, float& v,
-                           uint32_t& patch)
-{
-    const DevObject& ob = sc.objects[i];
-    if (ob.surface == EMPTY) return false;   // no geometry (empty box: only a NaN ray gets here)
-    CGS_COUNT(kWorkObject);
-    LocalRay lr = toLocal(ob, o, d, tmax);
-    const DevBVH surf = sc.surfaces[ob.surface];
-    const BVHRef b = bvhRef(sc, surf);
-    const ClipRay cr = makeClipRay(lr.o, versor3(lr.d));
-    float lt = lr.max, lu = 0.0f, lv = 0.0f;
-    uint32_t lp = EMPTY;
-    bool hit = hashIntersect<CGS_BOT_OCTANTS>(b, lr.o, lr.d, lt, [&](uint32_t pid) -> bool {
-        const float4* S = sc.patches + 16 * (size_t)(surf.patchBase + pid);
-        bool f;
-        // the default keeps the direct call: routing it through leafTest
-        // costs the render and trace kernels hundreds of spilled VGPRs
-        if constexpr (METHOD == Clip)
-            f = clipPatch<false>(S, cr, kClipTol, lt, lu, lv, stk, overflow);
-        else
-            f = leafTest<METHOD, false>(S, cr, tp, lt, lu, lv, stk, overflow);
-        if (f) lp = pid;
-        return f;
-    }, NodeCache {nullptr, 0}, rg.bot);
-    if (!hit) return false;
-    t = lt * lr.dinv;
-    u = lu;
-    v = lv;
-    patch = lp;
-    return true;
-}
-
-// rt::intersect(BezierSurface, ray) (Primitive.cu:250-268) for object i.
-template<int METHOD>
-CGS_DEV bool objectAny(const SceneDev& sc, const Rings& rg, uint32_t i, V3 o, V3 d, float tmax, const ClipStack& stk,
-                       bool& overflow, const TraceParams& tp)
-{
-    const DevObject& ob = sc.objects[i];
-    if (ob.surface == EMPTY) return false;   // no geometry (empty box: only a NaN ray gets here)
-    CGS_COUNT(kWorkObject);
-    LocalRay lr = toLocal(ob, o, d, tmax);
-    const DevBVH surf = sc.surfaces[ob.surface];
-    const BVHRef b = bvhRef(sc, surf);
-    const ClipRay cr = makeClipRay(lr.o, versor3(lr.d));
-    return hashIntersectAny<CGS_BOT_OCTANTS>(b, lr.o, lr.d, lr.max, [&](uint32_t pid) -> bool {
-        const float4* S = sc.patches + 16 * (size_t)(surf.patchBase + pid);
-        float best = lr.max, bu, bv;
-        if constexpr (METHOD == Clip)
-            return clipPatch<true>(S, cr, kClipTol, best, bu, bv, stk, overflow);
-        else
-            return leafTest<METHOD, true>(S, cr, tp, best, bu, bv, stk, overflow);
-    }, NodeCache {nullptr, 0}, rg.bot);
-}
-
 // ---------------------------------------------------------------------------
-// Object enumeration through the cull tree (cgs_cull.cpp)
+// Cull-tree enumeration (cgs_cull.cpp): the elements (objects or patches)
+// whose reference leaf box a ray passes, in the tree's near-to-far order
 // ---------------------------------------------------------------------------
-//
-// The reference's top-level traversal reaches object X only if the ray
-// passes X's leaf box with entry L0 below the hit distance at that moment
-// (rt/BVH.h:147-266), and every ancestor box contains X's box, so it passes
-// there whenever it passes X's (the slab test is monotone in the box).
-// Each object's own result does not depend on the visit order (its traversal
-// starts from ray.max·|M d|, RayTracer.cu:311-314).  So:
-//  - any hit: the reference's answer is whether some object whose leaf box
-//    the ray passes with L0 < ray.max has a hit, in any order;
-//  - closest: enumerating every such object whose L0 is not far beyond the
-//    nearest hit found so far (a hit lies inside its object's box, up to
-//    rounding, for which the slack below leaves a wide margin) finds the
-//    nearest hit W.  The reference reaches W when L0_W < t_W (its limit is
-//    never below t_W), and then reports W unless another object has a hit at
-//    exactly t_W, where its visit order decides.
-// A lane falls back to the reference's traversal for a tie, for L0_W >= t_W,
-// for a clip-stack overflow on the way (so that overflows are reported as in
-// the reference order), and for rays on which the slab test could meet a
-// NaN (a zero or non-finite direction component, a non-finite origin).
 
 // The cull tree's node and element records, read per lane.
 struct CullIter
@@ -413,7 +335,200 @@ CGS_DEV int dirOctant(V3 Dinv)
     return (__builtin_signbit(Dinv.x) ? 1 : 0) | (__builtin_signbit(Dinv.y) ? 2 : 0) | (__builtin_signbit(Dinv.z) ? 4 : 0);
 }
 
+// The patches of a large surface through its cull tree: the argument of the
+// objects' enumeration (the comment above closestCulled) one level down.  A
+// patch's clip accepts candidates by the running distance only (onHit's
+// t < best), so a patch's own nearest hit does not depend on the distance
+// its test starts from; every patch whose leaf box the local ray passes near
+// or before the nearest hit so far is clipped from the local ray's max, and
+// the nearest wins.  Returns false, its outputs untouched, where the
+// reference's visit order could decide: a tie, L0_W >= t_W, a region-stack
+// overflow, a ray with a zero or non-finite component.  Clipping only (the
+// subdivision leaf keeps the reference's traversal).
 template<int METHOD>
+CGS_DEV bool surfaceClosestCulled(const SceneDev& sc, const DevBVH& surf, const LocalRay& lr, const ClipRay& cr,
+                                  const ClipStack& stk, const TraceParams& tp, float& lt, float& lu, float& lv,
+                                  uint32_t& lp, bool& hit)
+{
+    const V3 Dinv {1.0f / lr.d.x, 1.0f / lr.d.y, 1.0f / lr.d.z};
+    if (!cullable(lr.o, Dinv)) return false;
+    const int oct = dirOctant(Dinv);
+    const float mag = fmax_(__builtin_fabsf(lr.o.x), fmax_(__builtin_fabsf(lr.o.y), __builtin_fabsf(lr.o.z)));
+    float bt = lr.max, bu = 0.0f, bv = 0.0f, L0W = 0.0f, L0;
+    uint32_t bp = EMPTY, pid;
+    bool tie = false, overflow = false;
+    CullIter it {surf.cullRoot, 0u, 0u};
+    while ((pid = cullNext(sc, oct, lr.o, Dinv, cullLimit(bt, mag), lr.max, it, L0)) != EMPTY)
+    {
+        const float4* S = sc.patches + 16 * (size_t)(surf.patchBase + pid);
+        float t = lr.max, u = 0.0f, v = 0.0f;
+        bool f;
+        if constexpr (METHOD == Clip)
+            f = clipPatch<false>(S, cr, kClipTol, t, u, v, stk, overflow);
+        else
+            f = leafTest<METHOD, false>(S, cr, tp, t, u, v, stk, overflow);
+        if (overflow) return false;
+        if (!f) continue;
+        if (t < bt)
+        {
+            bt = t;
+            bu = u;
+            bv = v;
+            bp = pid;
+            L0W = L0;
+            tie = false;
+        }
+        else if (t == bt)
+            tie = true;
+    }
+    if (tie || (bp != EMPTY && !(L0W < bt))) return false;
+    hit = bp != EMPTY;
+    if (hit)
+    {
+        lt = bt;
+        lu = bu;
+        lv = bv;
+        lp = bp;
+    }
+    return true;
+}
+
+// rt::intersect(BezierSurface, ray) through the surface's cull tree: any
+// patch whose leaf box the ray passes below the local max and that has a
+// hit, in any order (the reference's any-hit walk never narrows its limit).
+// Returns false where an overflow or the ray's components call for the
+// reference's traversal.
+template<int METHOD>
+CGS_DEV bool surfaceAnyCulled(const SceneDev& sc, const DevBVH& surf, const LocalRay& lr, const ClipRay& cr,
+                              const ClipStack& stk, const TraceParams& tp, bool& occluded)
+{
+    const V3 Dinv {1.0f / lr.d.x, 1.0f / lr.d.y, 1.0f / lr.d.z};
+    if (!cullable(lr.o, Dinv)) return false;
+    const int oct = dirOctant(Dinv);
+    float L0;
+    uint32_t pid;
+    bool overflow = false;
+    CullIter it {surf.cullRoot, 0u, 0u};
+    while ((pid = cullNext(sc, oct, lr.o, Dinv, lr.max, lr.max, it, L0)) != EMPTY)
+    {
+        const float4* S = sc.patches + 16 * (size_t)(surf.patchBase + pid);
+        float best = lr.max, bu, bv;
+        bool f;
+        if constexpr (METHOD == Clip)
+            f = clipPatch<true>(S, cr, kClipTol, best, bu, bv, stk, overflow);
+        else
+            f = leafTest<METHOD, true>(S, cr, tp, best, bu, bv, stk, overflow);
+        if (overflow) return false;
+        if (f)
+        {
+            occluded = true;
+            return true;
+        }
+    }
+    occluded = false;
+    return true;
+}
+
+// rt::intersect(BezierSurface, hit, ray) (Primitive.cu:222-248) for object
+// i, from RayTracer.cu:305-333: the object-local ray, the per-object hashed
+// traversal from t = ray.max·|M d|, and the leaf test at each patch.  On a
+// hit, t is the world distance (local t / |M d|).
+template<int METHOD, bool BIG = true>
+CGS_DEV bool objectClosest(const SceneDev& sc, const Rings& rg, uint32_t i, V3 o, V3 d, float tmax,
+                           const ClipStack& stk, bool& overflow, const TraceParams& tp, float& t, float& u, float& v,
+                           uint32_t& patch)
+{
+    const DevObject& ob = sc.objects[i];
+    if (ob.surface == EMPTY) return false;   // no geometry (empty box: only a NaN ray gets here)
+    CGS_COUNT(kWorkObject);
+    LocalRay lr = toLocal(ob, o, d, tmax);
+    const DevBVH surf = sc.surfaces[ob.surface];
+    const BVHRef b = bvhRef(sc, surf);
+    const ClipRay cr = makeClipRay(lr.o, versor3(lr.d));
+    float lt = lr.max, lu = 0.0f, lv = 0.0f;
+    uint32_t lp = EMPTY;
+    bool hit = false;
+    if constexpr (BIG && METHOD != Subdiv)
+        if (surf.cullRoot != EMPTY && surfaceClosestCulled<METHOD>(sc, surf, lr, cr, stk, tp, lt, lu, lv, lp, hit))
+        {
+            if (!hit) return false;
+            t = lt * lr.dinv;
+            u = lu;
+            v = lv;
+            patch = lp;
+            return true;
+        }
+    hit = hashIntersect<CGS_BOT_OCTANTS, false, BIG>(b, lr.o, lr.d, lt, [&](uint32_t pid) -> bool {
+        const float4* S = sc.patches + 16 * (size_t)(surf.patchBase + pid);
+        bool f;
+        // the default keeps the direct call: routing it through leafTest
+        // costs the render and trace kernels hundreds of spilled VGPRs
+        if constexpr (METHOD == Clip)
+            f = clipPatch<false>(S, cr, kClipTol, lt, lu, lv, stk, overflow);
+        else
+            f = leafTest<METHOD, false>(S, cr, tp, lt, lu, lv, stk, overflow);
+        if (f) lp = pid;
+        return f;
+    }, NodeCache {nullptr, 0}, rg.bot);
+    if (!hit) return false;
+    t = lt * lr.dinv;
+    u = lu;
+    v = lv;
+    patch = lp;
+    return true;
+}
+
+// rt::intersect(BezierSurface, ray) (Primitive.cu:250-268) for object i.
+template<int METHOD, bool BIG = true>
+CGS_DEV bool objectAny(const SceneDev& sc, const Rings& rg, uint32_t i, V3 o, V3 d, float tmax, const ClipStack& stk,
+                       bool& overflow, const TraceParams& tp)
+{
+    const DevObject& ob = sc.objects[i];
+    if (ob.surface == EMPTY) return false;   // no geometry (empty box: only a NaN ray gets here)
+    CGS_COUNT(kWorkObject);
+    LocalRay lr = toLocal(ob, o, d, tmax);
+    const DevBVH surf = sc.surfaces[ob.surface];
+    const BVHRef b = bvhRef(sc, surf);
+    const ClipRay cr = makeClipRay(lr.o, versor3(lr.d));
+    if constexpr (BIG && METHOD != Subdiv)
+    {
+        bool occluded;
+        if (surf.cullRoot != EMPTY && surfaceAnyCulled<METHOD>(sc, surf, lr, cr, stk, tp, occluded)) return occluded;
+    }
+    return hashIntersectAny<CGS_BOT_OCTANTS, false, BIG>(b, lr.o, lr.d, lr.max, [&](uint32_t pid) -> bool {
+        const float4* S = sc.patches + 16 * (size_t)(surf.patchBase + pid);
+        float best = lr.max, bu, bv;
+        if constexpr (METHOD == Clip)
+            return clipPatch<true>(S, cr, kClipTol, best, bu, bv, stk, overflow);
+        else
+            return leafTest<METHOD, true>(S, cr, tp, best, bu, bv, stk, overflow);
+    }, NodeCache {nullptr, 0}, rg.bot);
+}
+
+// ---------------------------------------------------------------------------
+// Object enumeration through the cull tree (cgs_cull.cpp)
+// ---------------------------------------------------------------------------
+//
+// The reference's top-level traversal reaches object X only if the ray
+// passes X's leaf box with entry L0 below the hit distance at that moment
+// (rt/BVH.h:147-266), and every ancestor box contains X's box, so it passes
+// there whenever it passes X's (the slab test is monotone in the box).
+// Each object's own result does not depend on the visit order (its traversal
+// starts from ray.max·|M d|, RayTracer.cu:311-314).  So:
+//  - any hit: the reference's answer is whether some object whose leaf box
+//    the ray passes with L0 < ray.max has a hit, in any order;
+//  - closest: enumerating every such object whose L0 is not far beyond the
+//    nearest hit found so far (a hit lies inside its object's box, up to
+//    rounding, for which the slack below leaves a wide margin) finds the
+//    nearest hit W.  The reference reaches W when L0_W < t_W (its limit is
+//    never below t_W), and then reports W unless another object has a hit at
+//    exactly t_W, where its visit order decides.
+// A lane falls back to the reference's traversal for a tie, for L0_W >= t_W,
+// for a clip-stack overflow on the way (so that overflows are reported as in
+// the reference order), and for rays on which the slab test could meet a
+// NaN (a zero or non-finite direction component, a non-finite origin).
+
+template<int METHOD, bool BIG = true>
 CGS_DEV ClosestHit closestCulled(const SceneDev& sc, const Rings& rg, V3 o, V3 d, V3 Dinv, float tmax,
                                  const ClipStack& stk, const TraceParams& tp, bool& exact)
 {
@@ -422,13 +537,13 @@ CGS_DEV ClosestHit closestCulled(const SceneDev& sc, const Rings& rg, V3 o, V3 d
     const float mag = fmax_(__builtin_fabsf(o.x), fmax_(__builtin_fabsf(o.y), __builtin_fabsf(o.z)));
     float L0W = 0.0f, L0;
     bool tie = false, overflow = false;
-    CullIter it {0u, 0u, 0u};
+    CullIter it {sc.top.cullRoot, 0u, 0u};
     uint32_t i;
     while ((i = cullNext(sc, oct, o, Dinv, cullLimit(h.t, mag), tmax, it, L0)) != EMPTY)
     {
         float t, u, v;
         uint32_t p;
-        if (!objectClosest<METHOD>(sc, rg, i, o, d, tmax, stk, overflow, tp, t, u, v, p)) continue;
+        if (!objectClosest<METHOD, BIG>(sc, rg, i, o, d, tmax, stk, overflow, tp, t, u, v, p)) continue;
         if (t < h.t)
         {
             h = {t, u, v, p, (int32_t)i};
@@ -442,18 +557,18 @@ CGS_DEV ClosestHit closestCulled(const SceneDev& sc, const Rings& rg, V3 o, V3 d
     return h;
 }
 
-template<int METHOD>
+template<int METHOD, bool BIG = true>
 CGS_DEV bool anyCulled(const SceneDev& sc, const Rings& rg, V3 o, V3 d, V3 Dinv, float tmax, const ClipStack& stk,
                        const TraceParams& tp, bool& exact)
 {
     const int oct = dirOctant(Dinv);
     bool overflow = false;
-    CullIter it {0u, 0u, 0u};
+    CullIter it {sc.top.cullRoot, 0u, 0u};
     float L0;
     uint32_t i;
     while ((i = cullNext(sc, oct, o, Dinv, tmax, tmax, it, L0)) != EMPTY)
     {
-        const bool hit = objectAny<METHOD>(sc, rg, i, o, d, tmax, stk, overflow, tp);
+        const bool hit = objectAny<METHOD, BIG>(sc, rg, i, o, d, tmax, stk, overflow, tp);
         if (overflow) break;
         if (hit) return true;
     }
@@ -464,26 +579,26 @@ CGS_DEV bool anyCulled(const SceneDev& sc, const Rings& rg, V3 o, V3 d, V3 Dinv,
 // Context::intersect(hit, ray) (RayTracer.cu:296-350): the nearest object
 // hit, through the cull tree where it applies, else (or where its answer
 // could depend on the visit order) by the reference's top-level traversal.
-template<int METHOD = Clip>
+template<int METHOD = Clip, bool BIG = true>
 CGS_DEV ClosestHit sceneClosest(const SceneDev& sc, const NodeCache& nc, const Rings& rg, V3 o, V3 d, float tmax,
                                 const ClipStack& stk, bool& overflow, const TraceParams& tp = {kClipTol, 0x1p-4f, 0})
 {
-    if (sc.cullNodes)
+    if (sc.top.cullRoot != EMPTY)
     {
         const V3 Dinv {1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
         if (cullable(o, Dinv))
         {
             bool exact = false;
-            const ClosestHit h = closestCulled<METHOD>(sc, rg, o, d, Dinv, tmax, stk, tp, exact);
+            const ClosestHit h = closestCulled<METHOD, BIG>(sc, rg, o, d, Dinv, tmax, stk, tp, exact);
             if (!exact) return h;
         }
     }
     ClosestHit h {tmax, 0.0f, 0.0f, EMPTY, -1};
     const BVHRef top = bvhRef(sc, sc.top);
-    hashIntersect<true, true>(top, o, d, h.t, [&](uint32_t i) -> bool {
+    hashIntersect<true, true, BIG>(top, o, d, h.t, [&](uint32_t i) -> bool {
         float t, u, v;
         uint32_t p;
-        if (objectClosest<METHOD>(sc, rg, i, o, d, tmax, stk, overflow, tp, t, u, v, p) && t < h.t)
+        if (objectClosest<METHOD, BIG>(sc, rg, i, o, d, tmax, stk, overflow, tp, t, u, v, p) && t < h.t)
         {
             h = {t, u, v, p, (int32_t)i};
             return true;
@@ -495,23 +610,23 @@ CGS_DEV ClosestHit sceneClosest(const SceneDev& sc, const NodeCache& nc, const R
 
 // Context::intersect(ray) (RayTracer.cu:352-383), through the cull tree
 // where it applies.
-template<int METHOD = Clip>
+template<int METHOD = Clip, bool BIG = true>
 CGS_DEV bool sceneAny(const SceneDev& sc, const NodeCache& nc, const Rings& rg, V3 o, V3 d, float tmax,
                       const ClipStack& stk, bool& overflow, const TraceParams& tp = {kClipTol, 0x1p-4f, 0})
 {
-    if (sc.cullNodes)
+    if (sc.top.cullRoot != EMPTY)
     {
         const V3 Dinv {1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
         if (cullable(o, Dinv))
         {
             bool exact = false;
-            if (anyCulled<METHOD>(sc, rg, o, d, Dinv, tmax, stk, tp, exact)) return true;
+            if (anyCulled<METHOD, BIG>(sc, rg, o, d, Dinv, tmax, stk, tp, exact)) return true;
             if (!exact) return false;
         }
     }
     const BVHRef top = bvhRef(sc, sc.top);
-    return hashIntersectAny<true, true>(top, o, d, tmax, [&](uint32_t i) -> bool {
-        return objectAny<METHOD>(sc, rg, i, o, d, tmax, stk, overflow, tp);
+    return hashIntersectAny<true, true, BIG>(top, o, d, tmax, [&](uint32_t i) -> bool {
+        return objectAny<METHOD, BIG>(sc, rg, i, o, d, tmax, stk, overflow, tp);
     }, nc, rg.top);
 }
 
@@ -658,7 +773,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CGS_TRACE_W
 // without a launch per scene.  Each lane finds its job by binary search over
 // the jobs' first ray numbers and runs the same per-ray query as
 // kTraceClosest / kTraceAny, with the same leaf methods and polish.
-template<int METHOD, bool POLISH>
+template<int METHOD, bool POLISH, bool BIG>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CGS_TRACE_WAVES))) void kTraceBatch(
     const BatchJob* __restrict__ jobs, uint32_t njobs, uint32_t total, uint32_t chunk, Persist ps,
     unsigned long long* __restrict__ overflowCount, TraceParams tp)
@@ -693,13 +808,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CGS_TRACE_W
             bool overflow = false;
             if (J.any)
             {
-                const bool occ = sceneAny<METHOD>(J.sc, none, rg, {ray.ox, ray.oy, ray.oz}, {ray.dx, ray.dy, ray.dz},
+                const bool occ = sceneAny<METHOD, BIG>(J.sc, none, rg, {ray.ox, ray.oy, ray.oz}, {ray.dx, ray.dy, ray.dz},
                                                   ray.tmax, stk, overflow, tp);
                 J.occluded[k] = overflow ? 2u : (occ ? 1u : 0u);
             }
             else
             {
-                ClosestHit h = sceneClosest<METHOD>(J.sc, none, rg, {ray.ox, ray.oy, ray.oz}, {ray.dx, ray.dy, ray.dz},
+                ClosestHit h = sceneClosest<METHOD, BIG>(J.sc, none, rg, {ray.ox, ray.oy, ray.oz}, {ray.dx, ray.dy, ray.dz},
                                                     ray.tmax, stk, overflow, tp);
                 if constexpr (POLISH)
                     if (h.object >= 0 && !overflow)
@@ -1077,7 +1192,7 @@ CGS_DEV float saturate(float x) { return x > 0.0f ? (x < 1.0f ? x : 1.0f) : 0.0f
 // infinity.  eps is read from device memory per use (the traversal loops need
 // the registers a kernel argument would hold for the whole launch).  With
 // RECORD, the first segment's hit is returned in `first`.
-template<bool RECORD>
+template<bool RECORD, bool BIG = true>
 CGS_DEV V3 rayRadiance(const SceneDev& sc, const NodeCache& nc, const Rings& rg, V3 ro, V3 rd, float rmax, int depth,
                        const float* eps, const ClipStack& stk, bool& overflow, uint32_t& nShadow,
                        uint32_t& nSecondary, ClosestHit& first)
@@ -1088,7 +1203,7 @@ CGS_DEV V3 rayRadiance(const SceneDev& sc, const NodeCache& nc, const Rings& rg,
     while (true)
     {
         if (!primary) nSecondary++;
-        ClosestHit h = sceneClosest(sc, nc, rg, ro, rd, rmax, stk, overflow);
+        ClosestHit h = sceneClosest<Clip, BIG>(sc, nc, rg, ro, rd, rmax, stk, overflow);
         if (RECORD && primary) first = h;
         primary = false;
         if (h.object < 0)
@@ -1125,7 +1240,7 @@ CGS_DEV V3 rayRadiance(const SceneDev& sc, const NodeCache& nc, const Rings& rg,
             const float dotNL = dot3(N, L);
             if (dotNL < 1e-14f) continue;
             nShadow++;
-            if (sceneAny(sc, nc, rg, add3(Pt, mul3(*eps, L)), L, d, stk, overflow)) continue;
+            if (sceneAny<Clip, BIG>(sc, nc, rg, add3(Pt, mul3(*eps, L)), L, d, stk, overflow)) continue;
             V3 H = versor3(add3(L, V));
             float dotHN = dot3(H, N);
             float dotHL = dot3(H, L);
@@ -1155,6 +1270,10 @@ CGS_DEV V3 rayRadiance(const SceneDev& sc, const NodeCache& nc, const Rings& rg,
 // keeps VGPR + AGPR use within 512 / CGS_RENDER_WAVES.  On the 1M-patch frame
 // three waves per SIMD (168 registers) measured 39.5 ms, two waves 44.5 ms
 // and four waves 46 ms.
+// BIG: the scene has a BVH hashed with the wide family or a surface with a
+// cull tree (kernels for other scenes leave those paths out and keep their
+// registers).
+template<bool BIG>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CGS_RENDER_WAVES))) void kRender(SceneDev sc, Persist ps, const RenderArgs* ra)
 {
     __shared__ float4 topLds[4 * CGS_TOP_LDS_KEYS];
@@ -1191,7 +1310,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CGS_RENDER_
         float4 P = make_float4(cam.tlx + cam.hdx * (2.0f * (float)i), cam.tly - cam.hdy * (2.0f * (float)j), -1.0f, 1.0f);
         const V3 rd = versor3(sub3(project4(m4mul(C, P)), camPos));
         ClosestHit unused;
-        const V3 acc = rayRadiance<false>(sc, nc, rg, camPos, rd, __builtin_inff(), op.depth, &ra->op.eps, stk, overflow,
+        const V3 acc = rayRadiance<false, BIG>(sc, nc, rg, camPos, rd, __builtin_inff(), op.depth, &ra->op.eps, stk, overflow,
                                           nShadow, nSecondary, unused);
         const uint32_t q = j * ra->op.width + i;
         if (float* color = ra->color)
@@ -1320,8 +1439,8 @@ hipError_t persistentGrid(PersistKernel k, uint32_t& blocksPerCU)
     case PersistKernel::TraceClosest: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kTraceClosest<Clip, false>, 256, 0); break;
     case PersistKernel::TraceAny: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kTraceAny<Clip>, 256, 0); break;
     case PersistKernel::ClipBatch: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kClipBatch, 256, 0); break;
-    case PersistKernel::Render: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kRender, 256, 0); break;
-    case PersistKernel::Batch: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kTraceBatch<Clip, false>, 256, 0); break;
+    case PersistKernel::Render: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kRender<true>, 256, 0); break;
+    case PersistKernel::Batch: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kTraceBatch<Clip, false, true>, 256, 0); break;
     case PersistKernel::Pick: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kPick<false>, 256, 0); break;
     case PersistKernel::Shade: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kShade<true>, 256, 0); break;
     }
@@ -1380,13 +1499,20 @@ hipError_t launchTraceAny(const SceneDev& sc, const PersistArgs& pa, const cgs_r
 
 hipError_t launchTraceBatch(const BatchJob* jobs, uint32_t njobs, uint32_t total, uint32_t chunk, float2* stack,
                             uint32_t lanes, unsigned int* counter, uint32_t grid, unsigned long long* overflow, hipStream_t s,
-                            const TraceMode& m)
+                            const TraceMode& m, bool big)
 {
     if (total == 0) return hipSuccess;
     Persist ps {stack, lanes, counter};
     const TraceParams tp = traceParams(m);
+    // the default method has a kernel without the wide-hash and surface-cull paths
+    if (!big && m.method != CGS_METHOD_SUBDIVISION && !m.maxIters && m.tol == kClipTol && !m.polish)
+    {
+        hipLaunchKernelGGL((kTraceBatch<Clip, false, false>), dim3(grid), dim3(256), 0, s, jobs, njobs, total, chunk, ps,
+                           overflow, tp);
+        return hipGetLastError();
+    }
 #define CGS_LAUNCH_BATCH(M, P) \
-    hipLaunchKernelGGL((kTraceBatch<M, P>), dim3(grid), dim3(256), 0, s, jobs, njobs, total, chunk, ps, overflow, tp)
+    hipLaunchKernelGGL((kTraceBatch<M, P, true>), dim3(grid), dim3(256), 0, s, jobs, njobs, total, chunk, ps, overflow, tp)
     if (m.method == CGS_METHOD_SUBDIVISION)
     {
         if (m.polish) CGS_LAUNCH_BATCH(Subdiv, true);
@@ -1486,7 +1612,10 @@ hipError_t launchRender(const SceneDev& sc, const PersistArgs& pa, const float* 
     hipError_t e = hipMemcpyAsync(argsDev, &ra, sizeof ra, hipMemcpyHostToDevice, s);
     if (e != hipSuccess) return e;
     Persist ps {pa.stack, pa.lanes, pa.counter};
-    hipLaunchKernelGGL(kRender, dim3(pa.grid), dim3(256), 0, s, sc, ps, (const RenderArgs*)argsDev);
+    if (sc.big)
+        hipLaunchKernelGGL(kRender<true>, dim3(pa.grid), dim3(256), 0, s, sc, ps, (const RenderArgs*)argsDev);
+    else
+        hipLaunchKernelGGL(kRender<false>, dim3(pa.grid), dim3(256), 0, s, sc, ps, (const RenderArgs*)argsDev);
     return hipGetLastError();
 }
 

@@ -454,6 +454,14 @@ cgs_status cgs_scene_bvh_copy(cgs_scene* scene, int32_t surface, cgs_bvh_node* n
  * object transform is affine with a consistent inverse; CGS_CULL=0 in the
  * environment at scene creation turns it off. */
 int cgs_scene_culled(cgs_scene* scene);
+/* 1 when that BVH (surface >= 0, or -1: the top level, as cgs_scene_culled)
+ * has a cull tree, 0 when not, -1 for a bad argument.  A surface of at least
+ * 4096 patches whose control points all have weight 1 gets one (unless
+ * CGS_CULL=0): closest and any hits by clipping find its patches through
+ * the tree, and fall back to the reference's traversal of the surface where
+ * the visit order could decide (ties, a hit before its patch's box entry, an
+ * overflow, a ray with a zero component).  The results are the same. */
+int cgs_scene_bvh_culled(cgs_scene* scene, int32_t surface);
 /* 1 when that BVH (surface >= 0, or -1: the top level) is hashed with the
  * wide family, 0 with the reference's, -1 for a bad argument.  A scene
  * hashes a BVH with the wide family when the reference family's build ends
