@@ -529,6 +529,12 @@ class DeviceScene:
         except Exception:
             pass
 
+    def bvh_info(self, surface: int) -> tuple[int, int, dict]:
+        """(node count, index count, hash parameters) of that BVH (-1: the top level)."""
+        nc, ic, h = C.c_uint32(), C.c_uint32(), _abi.PHF()
+        check(lib().cgs_scene_bvh_info(self.h, surface, C.byref(nc), C.byref(ic), C.byref(h)))
+        return nc.value, ic.value, dict(A=h.A, B=h.B, mask=h.mask, table_size=h.table_size, attempts=h.attempts)
+
     def bvh(self, surface: int) -> dict:
         nc, ic, h = C.c_uint32(), C.c_uint32(), _abi.PHF()
         check(lib().cgs_scene_bvh_info(self.h, surface, C.byref(nc), C.byref(ic), C.byref(h)))

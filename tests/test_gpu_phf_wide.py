@@ -73,3 +73,28 @@ def test_single_bvh_scene_hashes_wide_and_matches_oracle(ctx, oracle, single_bvh
     oocc, _ = osc.trace(rays, any_hit=True)
     assert np.array_equal(occ == 1, oocc)
     ds.destroy()
+
+
+def test_single_bvh_render_matches_oracle(ctx, oracle, single_bvh):
+    """A 40×30 render of the single-BVH scene (shadow rays and a reflection
+    bounce, through the surface's cull tree) equals the oracle's."""
+    from cgspline_amd import scenes
+    wl = single_bvh
+    cam = scenes.make_camera(*_view(wl.camera), 40, 30)
+    ds = ctx.scene(wl.scene)
+    assert ds.bvh_culled(0)
+    say("single-BVH render 40x30")
+    color, frame, counts = ds.render(cam, eps=wl.eps, depth=wl.depth)
+    ocolor, oframe, _, orc = oracle.OracleScene(wl.scene).render(cam, eps=wl.eps, depth=wl.depth)
+    assert counts["overflow"] == 0
+    assert (counts["primary"], counts["shadow"], counts["secondary"]) == (orc["primary"], orc["shadow"], orc["secondary"])
+    assert_bits_equal(color.reshape(-1, 3), ocolor.reshape(-1, 3), "color")
+    assert orc["shadow"] > 0
+    ds.destroy()
+
+
+def _view(cam):
+    import math
+    M = cam.to_world.reshape(4, 4).astype(np.float64)
+    pos = M[3, :3]
+    return pos, pos - M[2, :3], M[1, :3], math.degrees(cam.fovy)
