@@ -43,6 +43,66 @@ def load_be(source) -> tuple[np.ndarray, np.ndarray]:
 
 
 @dataclasses.dataclass
+class PatchTable:
+    """A surface's patch table as Surface::load_se fills it (cgs_load_se):
+    points (n, 4); node indices; basis matrices; per patch, in the table's
+    group order, (offset, matrix offset or -1, node count, type)."""
+
+    points: np.ndarray
+    indices: np.ndarray
+    matrices: np.ndarray
+    patches: np.ndarray       # (npatches, 4) int64: offset, matrix_offset, size, type
+
+    def _c(self):
+        keep = (np.ascontiguousarray(self.points, np.float32), np.ascontiguousarray(self.indices, np.uint32),
+                np.ascontiguousarray(self.matrices, np.float32),
+                np.ascontiguousarray(self.patches.astype(np.int64).astype(np.uint32)))
+        t = _abi.PatchTable(keep[0].ctypes.data_as(C.POINTER(C.c_float)), keep[0].shape[0],
+                            keep[1].ctypes.data_as(C.POINTER(C.c_uint32)), keep[1].shape[0],
+                            keep[2].ctypes.data_as(C.POINTER(C.c_float)), keep[2].shape[0],
+                            keep[3].ctypes.data_as(C.POINTER(_abi.PatchRef)), self.patches.shape[0])
+        return t, keep
+
+    def elements(self) -> np.ndarray:
+        """cgs_patch_table_elements: (npatches, 10) float32 view of the PatchBVH
+        element boxes (bmin, bmax, centroid, index bits)."""
+        t, keep = self._c()
+        out = np.zeros((self.patches.shape[0], 10), np.float32)
+        check(lib().cgs_patch_table_elements(C.byref(t), out.ctypes.data), "cgs_patch_table_elements")
+        return out
+
+    def patch_indices(self) -> np.ndarray:
+        """cgs_patch_table_patches: (npatches, 16) the indices the leaf test clips."""
+        t, keep = self._c()
+        out = np.zeros((self.patches.shape[0], 16), np.uint32)
+        check(lib().cgs_patch_table_patches(C.byref(t), out.ctypes.data), "cgs_patch_table_patches")
+        return out
+
+
+def load_se(source) -> PatchTable:
+    """Surface::load_se (Surface.cpp:214-348) through cgs_load_se: `source`
+    is the file's text as bytes, or a path."""
+    if isinstance(source, (bytes, bytearray, memoryview)):
+        text = bytes(source)
+    else:
+        with open(source, "rb") as f:
+            text = f.read()
+    tp = C.POINTER(_abi.PatchTable)()
+    check(lib().cgs_load_se(text, len(text), C.byref(tp)), "cgs_load_se")
+    try:
+        t = tp.contents
+        arr = lambda ptr, n, dt: np.ctypeslib.as_array(ptr, (max(n, 1),))[:n].astype(dt).copy()
+        pts = arr(t.points, 4 * t.npoints, np.float32).reshape(-1, 4)
+        idx = arr(t.indices, t.nindices, np.uint32)
+        mat = arr(t.matrices, t.nmatrices, np.float32)
+        refs = np.array([(t.patches[k].offset, t.patches[k].matrix_offset, t.patches[k].size, t.patches[k].type)
+                         for k in range(t.npatches)], np.int64).reshape(-1, 4)
+    finally:
+        lib().cgs_patch_table_free(tp)
+    return PatchTable(pts, idx, mat, refs)
+
+
+@dataclasses.dataclass
 class DlScene:
     """A scene read from a `.dl` file (cgs_dl_read): the scene, the file's
     view (camera→world, column-major, and position) and the names of its
@@ -432,13 +492,24 @@ class PatchBVH:
     """cg::PatchBVH over one surface's Bézier patches (cgs_patchbvh_create):
     closest and any hits of rays in the surface's frame."""
 
-    def __init__(self, ctx: Context, vertices: np.ndarray, indices: np.ndarray):
+    def __init__(self, ctx: Context, vertices: np.ndarray, indices: np.ndarray, elements: np.ndarray = None):
         v = np.ascontiguousarray(vertices, np.float32).reshape(-1, 4)
         i = np.ascontiguousarray(indices, np.uint32).reshape(-1, 16)
         p = C.c_void_p()
-        check(lib().cgs_patchbvh_create(ctx.h, v.ctypes.data, v.shape[0], i.ctypes.data, i.shape[0], C.byref(p)),
-              "cgs_patchbvh_create")
+        if elements is None:
+            check(lib().cgs_patchbvh_create(ctx.h, v.ctypes.data, v.shape[0], i.ctypes.data, i.shape[0], C.byref(p)),
+                  "cgs_patchbvh_create")
+        else:
+            e = np.ascontiguousarray(elements, np.float32).reshape(-1, 10)
+            check(lib().cgs_patchbvh_create_elements(ctx.h, v.ctypes.data, v.shape[0], i.ctypes.data, i.shape[0],
+                                                     e.ctypes.data, C.byref(p)), "cgs_patchbvh_create_elements")
         self.ctx, self.h = ctx, p.value
+
+    @classmethod
+    def from_table(cls, ctx: Context, table: PatchTable) -> "PatchBVH":
+        """cg::PatchBVH(const Surface*) of a `.se` surface: boxes from the
+        table's basis matrices, leaf tests on each patch's 16 indices."""
+        return cls(ctx, table.points, table.patch_indices(), table.elements())
 
     def intersect(self, rays: np.ndarray):
         """Closest hits of host rays (n, 8): (t,u,v (n,3), patch (n,))."""

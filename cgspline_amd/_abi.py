@@ -35,7 +35,8 @@ EXPORTS = [
     "cgs_dl_read", "cgs_dl_parse", "cgs_dl_free",
     "cgs_phf_displacement_size", "cgs_phf_build", "cgs_phf_build_static", "cgs_phf_fill_table", "cgs_phf_hash",
     "cgs_phf_build_wide", "cgs_phf_fill_table_wide", "cgs_phf_hash_wide", "cgs_scene_bvh_wide",
-    "cgs_scene_bvh_culled",
+    "cgs_scene_bvh_culled", "cgs_load_se", "cgs_patch_table_free", "cgs_patch_table_elements",
+    "cgs_patch_table_patches", "cgs_patchbvh_create_elements",
     "cgs_scene_create", "cgs_scene_destroy", "cgs_scene_bvh_info", "cgs_scene_bvh_copy", "cgs_scene_culled",
     "cgs_trace_closest", "cgs_trace_any", "cgs_trace_closest_ex", "cgs_trace_any_ex", "cgs_render",
     "cgs_render_running", "cgs_shade", "cgs_work_counts",
@@ -116,6 +117,16 @@ class TraceOptions(C.Structure):
                 ("polish", _U32), ("reserved", _U32 * 3)]
 
 
+class PatchRef(C.Structure):
+    _fields_ = [("offset", _U32), ("matrix_offset", _I32), ("size", _U32), ("type", _U32)]
+
+
+class PatchTable(C.Structure):
+    _fields_ = [("points", C.POINTER(_F32)), ("npoints", _U32), ("indices", C.POINTER(_U32)), ("nindices", _U32),
+                ("matrices", C.POINTER(_F32)), ("nmatrices", _U32), ("patches", C.POINTER(PatchRef)),
+                ("npatches", _U32)]
+
+
 class AccuracyLimits(C.Structure):
     _fields_ = [("max_rel_t", _F32), ("max_uv", _F32), ("max_disagree", _F32), ("allow_polish", _U32)]
 
@@ -174,6 +185,11 @@ def lib():
             "cgs_phf_hash_wide": ([_P, C.POINTER(PHF), _P, _P, _U32, _P, _P], C.c_int),
             "cgs_scene_bvh_wide": ([_P, _I32], C.c_int),
             "cgs_scene_bvh_culled": ([_P, _I32], C.c_int),
+            "cgs_load_se": ([_P, _SZ, C.POINTER(C.POINTER(PatchTable))], C.c_int),
+            "cgs_patch_table_free": ([C.POINTER(PatchTable)], None),
+            "cgs_patch_table_elements": ([C.POINTER(PatchTable), _P], C.c_int),
+            "cgs_patch_table_patches": ([C.POINTER(PatchTable), _P], C.c_int),
+            "cgs_patchbvh_create_elements": ([_P, _P, _U32, _P, _U32, _P, C.POINTER(_P)], C.c_int),
             "cgs_scene_create": ([_P, C.POINTER(SceneDesc), C.POINTER(_P)], C.c_int),
             "cgs_scene_destroy": ([_P], C.c_int),
             "cgs_scene_bvh_info": ([_P, _I32, C.POINTER(_U32), C.POINTER(_U32), C.POINTER(PHF)], C.c_int),

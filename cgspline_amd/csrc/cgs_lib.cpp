@@ -897,7 +897,11 @@ static bool cullAllowed(const cgs_scene_desc* d)
     return true;
 }
 
-cgs_status cgs_scene_create(cgs_ctx* ctx, const cgs_scene_desc* d, cgs_scene** out)
+// cgs_scene_create, with surface 0's BVH built over the given element boxes
+// when `surface0` is not null (a PatchBVH over a patch table, whose boxes
+// come from basis-transformed points: that surface gets no cull tree, whose
+// exactness needs every hit inside its patch's box).
+static cgs_status sceneCreate(cgs_ctx* ctx, const cgs_scene_desc* d, cgs_scene** out, const cgs_bvh_element* surface0)
 {
     if (!ctx || !d || !out) return fail(CGS_ERR_INVALID, "cgs_scene_create: null argument");
     if (d->nsurf == 0 || d->nobj == 0 || !d->vertices || !d->indices || !d->surf_vbase || !d->surf_pbase ||
@@ -934,7 +938,10 @@ cgs_status cgs_scene_create(cgs_ctx* ctx, const cgs_scene_desc* d, cgs_scene** o
     parallelFor(S, [&](size_t k) {
         const uint32_t p0 = d->surf_pbase[k], p1 = d->surf_pbase[k + 1];
         std::vector<cgs_bvh_element> el(p1 - p0);
-        patchElements(d->vertices + 4 * (size_t)d->surf_vbase[k], d->indices + 16 * (size_t)p0, p1 - p0, el.data());
+        if (k == 0 && surface0)
+            std::memcpy(el.data(), surface0, el.size() * sizeof(cgs_bvh_element));
+        else
+            patchElements(d->vertices + 4 * (size_t)d->surf_vbase[k], d->indices + 16 * (size_t)p0, p1 - p0, el.data());
         st[k] = buildBVH(el, 1, sc->bvh[k]);
     });
     for (auto s : st)
@@ -984,7 +991,7 @@ cgs_status cgs_scene_create(cgs_ctx* ctx, const cgs_scene_desc* d, cgs_scene** o
     if (cullEnabled())
         for (uint32_t k = 0; k < S; k++)
         {
-            if (d->surf_pbase[k + 1] - d->surf_pbase[k] < kSurfaceCullMinPatches) continue;
+            if (d->surf_pbase[k + 1] - d->surf_pbase[k] < kSurfaceCullMinPatches || (k == 0 && surface0)) continue;
             bool unit = true;
             for (uint64_t v = d->surf_vbase[k]; v < d->surf_vbase[k + 1] && unit; v++) unit = d->vertices[4 * v + 3] == 1.0f;
             if (unit && buildCullTree(sc->bvh[k].nodes.data(), sc->bvh[k].nodes.size(), sc->bvh[k].indices.data(), one))
@@ -1164,6 +1171,11 @@ cgs_status cgs_scene_create(cgs_ctx* ctx, const cgs_scene_desc* d, cgs_scene** o
     return CGS_OK;
 }
 
+cgs_status cgs_scene_create(cgs_ctx* ctx, const cgs_scene_desc* d, cgs_scene** out)
+{
+    return sceneCreate(ctx, d, out, nullptr);
+}
+
 cgs_status cgs_scene_destroy(cgs_scene* scene)
 {
     if (!scene) return CGS_OK;
@@ -1209,8 +1221,8 @@ struct cgs_patchbvh
 
 extern "C" {
 
-cgs_status cgs_patchbvh_create(cgs_ctx* ctx, const float* vertices, uint32_t nv, const uint32_t* indices, uint32_t np,
-                               cgs_patchbvh** out)
+static cgs_status patchbvhCreate(cgs_ctx* ctx, const float* vertices, uint32_t nv, const uint32_t* indices, uint32_t np,
+                                 const cgs_bvh_element* elements, cgs_patchbvh** out)
 {
     if (!ctx || !out || !vertices || !indices || nv == 0 || np == 0)
         return fail(CGS_ERR_INVALID, "cgs_patchbvh_create: empty or null surface");
@@ -1230,11 +1242,27 @@ cgs_status cgs_patchbvh_create(cgs_ctx* ctx, const float* vertices, uint32_t nv,
     d.nobj = 1;
     d.hash_seed = 42;
     auto pb = std::make_unique<cgs_patchbvh>();
-    cgs_status st = cgs_scene_create(ctx, &d, &pb->scene);
+    cgs_status st = sceneCreate(ctx, &d, &pb->scene, elements);
     if (st != CGS_OK) return st;
     pb->device = ctx->device;
     *out = pb.release();
     return CGS_OK;
+}
+
+cgs_status cgs_patchbvh_create(cgs_ctx* ctx, const float* vertices, uint32_t nv, const uint32_t* indices, uint32_t np,
+                               cgs_patchbvh** out)
+{
+    return patchbvhCreate(ctx, vertices, nv, indices, np, nullptr, out);
+}
+
+cgs_status cgs_patchbvh_create_elements(cgs_ctx* ctx, const float* vertices, uint32_t nv, const uint32_t* indices,
+                                        uint32_t np, const cgs_bvh_element* elements, cgs_patchbvh** out)
+{
+    if (!elements) return fail(CGS_ERR_INVALID, "cgs_patchbvh_create_elements: null elements");
+    // the leaf lists hold the element indices, which address the patches
+    for (uint32_t i = 0; i < np; i++)
+        if (elements[i].index >= np) return fail(CGS_ERR_INVALID, "cgs_patchbvh_create_elements: element index out of range");
+    return patchbvhCreate(ctx, vertices, nv, indices, np, elements, out);
 }
 
 cgs_status cgs_patchbvh_destroy(cgs_patchbvh* pb)

@@ -306,6 +306,12 @@ cgs_status cgs_patch_subpatch(cgs_ctx* ctx, const float* patches, const float* r
 typedef struct cgs_patchbvh cgs_patchbvh;
 cgs_status cgs_patchbvh_create(cgs_ctx* ctx, const float* vertices, uint32_t nv, const uint32_t* indices, uint32_t np,
                                cgs_patchbvh** out);
+/* cg::PatchBVH over a patch table (PatchBVH(const Surface*) of a `.se`
+ * surface): the tree over the given element boxes (cgs_patch_table_elements)
+ * and the leaf test on the 16 control points of each patch
+ * (cgs_patch_table_patches, np × 16 indices into the nv points). */
+cgs_status cgs_patchbvh_create_elements(cgs_ctx* ctx, const float* vertices, uint32_t nv, const uint32_t* indices,
+                                        uint32_t np, const cgs_bvh_element* elements, cgs_patchbvh** out);
 cgs_status cgs_patchbvh_destroy(cgs_patchbvh* bvh);
 /* BVHBase::intersect(ray, hit) with PatchBVH::intersectLeaf (PatchBVH.cpp:104-121):
  * the closest patch hit of each ray (a Ray3f: its direction normalized once),
@@ -353,6 +359,55 @@ cgs_status cgs_mesh_elements(const float* vertices, uint32_t nvertices, const ui
  * reference lets it through). */
 cgs_status cgs_load_be(const char* text, size_t length, float** vertices, uint32_t* npoints,
                        uint32_t** indices, uint32_t* npatches);
+
+/* A surface's patch table (PatchTable, Surface.h:45-95) as Surface::load_se
+ * fills it (Surface.cpp:214-348, PatchTable::set :36-148): points (x, y, z,
+ * w), node indices and basis matrices of every element, grouped by type,
+ * node count and matrix presence in order of first appearance; patches[i]
+ * is the i-th patch in that order (PatchTable::forEachPatch / find). */
+#define CGS_PATCH_BEZIER 0u
+#define CGS_PATCH_BSPLINE 1u
+#define CGS_PATCH_GREGORY 2u
+typedef struct cgs_patch_ref
+{
+    uint32_t offset;         /* first node index in `indices` */
+    int32_t matrix_offset;   /* first matrix entry, or -1: no matrix */
+    uint32_t size;           /* node count */
+    uint32_t type;           /* CGS_PATCH_* */
+} cgs_patch_ref;
+typedef struct cgs_patch_table
+{
+    float* points;           /* npoints × 4 */
+    uint32_t npoints;
+    uint32_t* indices;
+    uint32_t nindices;
+    float* matrices;         /* per patch with one: (16 or 20) × size, column-major */
+    uint32_t nmatrices;
+    cgs_patch_ref* patches;
+    uint32_t npatches;
+} cgs_patch_table;
+/* Surface::load_se (Surface.cpp:214-348): parses a `.se` text (nodes with
+ * ids and (x, y, z, w); elements of type 1 = B-spline, 2 = Gregory, degree
+ * 3, with node indices, a discarded value per node, and a basis matrix of 16
+ * (boundary B-spline) or 20 (Gregory) rows, transposed on reading; '#'
+ * comments).  Fails with CGS_ERR_INVALID and the reference's messages; an
+ * index equal to the node count is rejected (the reference lets it through),
+ * and a truncated text fails where the reference would read undefined
+ * values.  Release with cgs_patch_table_free. */
+cgs_status cgs_load_se(const char* text, size_t length, cgs_patch_table** out);
+void cgs_patch_table_free(cgs_patch_table* table);
+/* The element boxes cg::PatchBVH gives a table's patches (PatchBVH.cpp:40-80):
+ * a patch with a basis matrix is boxed by its 16 or 20 basis-transformed
+ * points, projected by w; one without by its first 16 points.  Centroid:
+ * the box centre.  A patch whose leaf test would read past the index table
+ * (the reference reads undefined memory there) gets the empty box: no ray
+ * reaches it. */
+cgs_status cgs_patch_table_elements(const cgs_patch_table* table, cgs_bvh_element* elements);
+/* The 16 control-point indices PatchBVH::intersectLeaf clips for each patch
+ * (PatchBVH.cpp:84-121): the 16 from its offset, whatever its node count or
+ * type (a patch of fewer nodes reads on into the next one's); 16 zeros for a
+ * patch whose 16 would run past the index table (its box is empty). */
+cgs_status cgs_patch_table_patches(const cgs_patch_table* table, uint32_t* indices16);
 
 /* A scene read from the reference's scene description language (`.dl`;
  * reader/DescriptionLanguage.cpp, reader/SceneReader.cpp), flattened the way

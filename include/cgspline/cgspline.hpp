@@ -79,9 +79,34 @@ public:
         return s;
     }
 
+    // Surface::load_se (Surface.cpp:214-348): reads a `.se` element surface.
+    // Its patches keep the reference's table order; indices() holds the 16
+    // indices PatchBVH's leaf test reads for each (cgs_patch_table_patches)
+    // and elements() the boxes PatchBVH gives them (cgs_patch_table_elements).
+    // Malformed text throws std::runtime_error with the reference's messages.
+    static std::shared_ptr<Surface> load_se(std::istream& input)
+    {
+        const std::string text {std::istreambuf_iterator<char>(input), std::istreambuf_iterator<char>()};
+        cgs_patch_table* t = nullptr;
+        if (cgs_load_se(text.data(), text.size(), &t) != CGS_OK) throw std::runtime_error(cgs_last_error());
+        std::vector<uint32_t> idx(16 * (size_t)t->npatches);
+        std::vector<cgs_bvh_element> el(t->npatches);
+        const bool ok = cgs_patch_table_patches(t, idx.data()) == CGS_OK && cgs_patch_table_elements(t, el.data()) == CGS_OK;
+        auto s = std::make_shared<Surface>(std::vector<float>(t->points, t->points + 4 * (size_t)t->npoints), std::move(idx));
+        cgs_patch_table_free(t);
+        if (!ok) throw std::runtime_error(cgs_last_error());
+        s->_elements = std::move(el);
+        return s;
+    }
+
+    // The PatchBVH element boxes of a `.se` surface; empty for Bézier patches,
+    // which PatchBVH boxes by their control points.
+    std::span<const cgs_bvh_element> elements() const { return _elements; }
+
 private:
     std::vector<float> _points;
     std::vector<uint32_t> _indices;
+    std::vector<cgs_bvh_element> _elements;
 };
 
 namespace rt
@@ -686,9 +711,15 @@ public:
     {
         const auto pts = s->points();
         const auto idx = s->indices();
-        rt::check(cgs_patchbvh_create(dev.handle(), pts.data(), (uint32_t)(pts.size() / 4), idx.data(),
-                                      (uint32_t)(idx.size() / 16), &_bvh),
-                  "PatchBVH");
+        const auto el = s->elements();
+        if (el.empty())
+            rt::check(cgs_patchbvh_create(dev.handle(), pts.data(), (uint32_t)(pts.size() / 4), idx.data(),
+                                          (uint32_t)(idx.size() / 16), &_bvh),
+                      "PatchBVH");
+        else
+            rt::check(cgs_patchbvh_create_elements(dev.handle(), pts.data(), (uint32_t)(pts.size() / 4), idx.data(),
+                                                   (uint32_t)(idx.size() / 16), el.data(), &_bvh),
+                      "PatchBVH");
     }
     ~PatchBVH()
     {

@@ -490,7 +490,18 @@ static M4<float> mat16(const float* m)
     return M;
 }
 
-orc_scene* orc_scene_create(const orc_scene_desc* d, int* status)
+static orc_scene* sceneCreate(const orc_scene_desc* d, int* status, const float* elements10);
+
+orc_scene* orc_scene_create(const orc_scene_desc* d, int* status) { return sceneCreate(d, status, nullptr); }
+
+// The scene with surface 0's BVH over the given elements (10 floats each:
+// min xyz, max xyz, centroid xyz, index bits), one per patch of surface 0.
+orc_scene* orc_scene_create_elements(const orc_scene_desc* d, const float* elements10, int* status)
+{
+    return sceneCreate(d, status, elements10);
+}
+
+static orc_scene* sceneCreate(const orc_scene_desc* d, int* status, const float* elements10)
 {
     auto h = std::make_unique<orc_scene>();
     Scene& s = h->s;
@@ -526,7 +537,20 @@ orc_scene* orc_scene_create(const orc_scene_desc* d, int* status)
         }
         s.background = {d->background[0], d->background[1], d->background[2]};
         s.ambient = {d->ambient[0], d->ambient[1], d->ambient[2]};
-        int r = buildScene(s, d->seed);
+        std::vector<Element> el;
+        if (elements10)
+        {
+            el.resize(d->surf_pbase[1] - d->surf_pbase[0]);
+            for (size_t i = 0; i < el.size(); i++)
+            {
+                const float* q = elements10 + 10 * i;
+                el[i].bounds.mn = {q[0], q[1], q[2]};
+                el[i].bounds.mx = {q[3], q[4], q[5]};
+                el[i].centroid = {q[6], q[7], q[8]};
+                std::memcpy(&el[i].index, q + 9, 4);
+            }
+        }
+        int r = buildScene(s, d->seed, elements10 ? &el : nullptr);
         *status = r;
         if (r) return nullptr;
     }
@@ -550,6 +574,52 @@ int orc_scene_bvh_sizes(const orc_scene* h, int surface, uint64_t* nnodes, uint6
     *nindices = b.indices.size();
     *ndispl = p.D.size();
     *ntable = p.tableSize;
+    return 0;
+}
+
+// cg::PatchBVH's queries (PatchBVH.cpp:84-121 under BVHBase::intersect) on
+// surface 0 of a scene: the ray is a Ray3f (its direction normalized once);
+// closest: the hit distance starts at ray.max; any: each leaf clips from
+// distance infinity.  hits3 (t,u,v) / patch, or occluded (1/0).
+int orc_scene_pick(const orc_scene* h, const float* rays, uint64_t n, int any, float* hits3, uint32_t* patch,
+    uint32_t* occluded, int threads)
+{
+    const Surface& s = h->s.surfaces.at(0);
+    parallelFor(n, threads, [&](uint64_t a, uint64_t b) {
+        for (uint64_t r = a; r < b; r++)
+        {
+            const float* ry = rays + 8 * r;
+            const Ray ray {{ry[0], ry[1], ry[2]}, versor(V3<float> {ry[4], ry[5], ry[6]}), ry[3]};
+            if (any)
+            {
+                occluded[r] = hashIntersectAny(s.view(), ray, [&](uint32_t idx) -> bool {
+                    PatchHit<float> ph {std::numeric_limits<float>::infinity(), {0, 0}};
+                    f4 S[16];
+                    s.patch(idx, S);
+                    return doBezierClipping<float>(ph, ray.origin, ray.direction, S, 0x1p-12f, nullptr);
+                }, nullptr) ? 1u : 0u;
+                continue;
+            }
+            Hit hit;
+            hit.t = ray.max;
+            uint32_t pid = EMPTY;
+            hashIntersect(s.view(), hit, ray, [&](Hit& hh, uint32_t idx) -> bool {
+                PatchHit<float> ph {hh.t, {hh.coordinates.x, hh.coordinates.y}};
+                f4 S[16];
+                s.patch(idx, S);
+                if (!doBezierClipping<float>(ph, ray.origin, ray.direction, S, 0x1p-12f, nullptr)) return false;
+                hh.index = idx;
+                hh.t = ph.distance;
+                hh.coordinates = {ph.p.x, ph.p.y, 0.0f};
+                pid = idx;
+                return true;
+            }, nullptr);
+            hits3[3 * r] = hit.t;
+            hits3[3 * r + 1] = pid == EMPTY ? 0.0f : hit.coordinates.x;
+            hits3[3 * r + 2] = pid == EMPTY ? 0.0f : hit.coordinates.y;
+            patch[r] = pid;
+        }
+    });
     return 0;
 }
 

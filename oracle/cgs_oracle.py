@@ -94,6 +94,9 @@ def lib():
         L.orc_phf_build_wide.argtypes = [_P, _U64, _U32, _P, _P]
         L.orc_phf_hash_wide.argtypes = [_P, _P, _P, _U64, _P]
         L.orc_scene_bvh_wide.argtypes = [_P, C.c_int]
+        L.orc_scene_create_elements.argtypes = [C.POINTER(SceneDesc), _P, C.POINTER(C.c_int)]
+        L.orc_scene_create_elements.restype = _P
+        L.orc_scene_pick.argtypes = [_P, _P, _U64, C.c_int, _P, _P, _P, C.c_int]
         L.orc_scene_create.argtypes = [C.POINTER(SceneDesc), C.POINTER(C.c_int)]
         L.orc_scene_create.restype = _P
         L.orc_scene_destroy.argtypes = [_P]
@@ -299,6 +302,124 @@ def phf_build(keys, seed=42):
     return rc, dict(A=int(p[0]), B=int(p[1]), mask=int(p[2]), table_size=int(p[3]), attempts=int(p[4])), D
 
 
+# ---------------------------------------------------------------------------
+# `.se` surfaces (Surface::load_se, Surface.cpp:214-348), the patch table
+# (PatchTable::set, :36-148) and cg::PatchBVH's boxes (PatchBVH.cpp:40-80):
+# an independent Python restatement, token by token
+# ---------------------------------------------------------------------------
+
+class SeError(ValueError):
+    pass
+
+
+def _se_tokens(text):
+    """Whitespace-separated tokens, '#' comments removed (load_se's discard()
+    runs before every item that may follow a comment; the reference's files
+    only hold comments there)."""
+    out = []
+    for line in text.splitlines():
+        k = line.find("#")
+        out.extend((line if k < 0 else line[:k]).split())
+    return out
+
+
+def load_se(text):
+    """Returns (points (n,4) f32, indices u32, matrices f32, patches (m,4):
+    offset, matrix offset or -1, node count, type 1 B-spline / 2 Gregory)."""
+    tok = _se_tokens(text.decode() if isinstance(text, (bytes, bytearray)) else text)
+    pos = 0
+
+    def take(kind, msg):
+        nonlocal pos
+        if pos >= len(tok):
+            raise SeError(msg)
+        t = tok[pos]
+        pos += 1
+        try:
+            return int(t) if kind == "i" else np.float32(float(t))
+        except ValueError:
+            raise SeError(msg)
+
+    n = take("i", "could not read no. points / nodes")
+    pts = np.zeros((n, 4), np.float32)
+    for i in range(n):
+        if take("i", "unexpected point id") != i:
+            raise SeError("unexpected point id")
+        for c in range(4):
+            pts[i, c] = take("f", "unexpected point id")
+    m = take("i", "could not read no. elements")
+    elements = []
+    for _ in range(m):
+        _id, typ, nodes, degree, boundary = (take("i", "read failed during element header") for _ in range(5))
+        if degree != 3:
+            raise SeError("degree value unsupported")
+        if typ == 1:
+            lines = 16 if boundary else 0
+        elif typ == 2:
+            lines = 20
+        else:
+            raise SeError("unknown patch type")
+        idx = [take("i", "Invalid/corrupted file. Indices out of range") for _ in range(nodes)]
+        if any(x >= n for x in idx):
+            raise SeError("Invalid/corrupted file. Indices out of range")
+        for _ in range(nodes):
+            take("i", "read failed during element data")
+        mat = np.zeros(lines * nodes, np.float32)
+        for j in range(lines):
+            for k in range(nodes):
+                mat[j + k * lines] = take("f", "read failed during element data")
+        elements.append((typ, idx, mat))
+    groups = []            # (type, nodes, has matrix) in order of first appearance
+    members = {}
+    for e, (typ, idx, mat) in enumerate(elements):
+        key = (typ, len(idx), mat.size > 0)
+        if key not in members:
+            groups.append(key)
+            members[key] = []
+        if mat.size and mat.size != (20 if typ == 2 else 16) * len(idx):
+            raise SeError("invalid element matrix size")
+        members[key].append(e)
+    indices, matrices, refs = [], [], []
+    for key in groups:
+        for e in members[key]:
+            typ, idx, mat = elements[e]
+            refs.append((len(indices), len(matrices) if mat.size else -1, len(idx), typ))
+            indices.extend(idx)
+            matrices.extend(mat.tolist())
+    return (pts, np.array(indices, np.uint32), np.array(matrices, np.float32),
+            np.array(refs, np.int64).reshape(-1, 4))
+
+
+def se_elements(points, indices, matrices, patches):
+    """cg::PatchBVH's element boxes (PatchBVH.cpp:40-80), FP32 in the
+    reference's order; the empty box for a patch whose 16-index leaf read runs
+    past the table.  Returns (m, 10) float32 (min, max, centroid, index bits)."""
+    f32 = np.float32
+    out = np.zeros((patches.shape[0], 10), np.float32)
+    for i, (off, moff, size, typ) in enumerate(patches):
+        n = 20 if typ == 2 else 16
+        if moff >= 0:
+            pts = []
+            for r in range(n):
+                s = None
+                for col in range(size):
+                    w = matrices[moff + r + col * n]
+                    term = w * points[indices[off + col]]
+                    s = term if s is None else (s + term).astype(f32)
+                pts.append(s[:3] / s[3])
+            pts = np.array(pts, np.float32)
+        else:
+            pts = points[indices[off:off + n], :3]
+        mn, mx = pts.min(axis=0), pts.max(axis=0)
+        if off + 16 > indices.shape[0]:
+            mn = np.full(3, np.finfo(np.float32).max, np.float32)
+            mx = -mn
+        out[i, 0:3], out[i, 3:6] = mn, mx
+        out[i, 6:9] = f32(0.5) * (mn + mx)
+        out[i, 9] = np.array([i], np.uint32).view(np.float32)[0]
+    return out
+
+
 def phf_build_wide(keys, seed=42):
     """The wide family (exact arithmetic, 64-bit keys): (status, params, D)."""
     keys = np.ascontiguousarray(keys, np.uint64)
@@ -324,7 +445,9 @@ def phf_hash_wide(params, D, keys):
 class OracleScene:
     """The oracle's copy of a scene: BVHs, hashes and tables built on the CPU."""
 
-    def __init__(self, scene):
+    def __init__(self, scene, surface0_elements=None):
+        """surface0_elements: (npatch, 10) float32 BVH elements of surface 0
+        (cg::PatchBVH's boxes of a patch table) instead of its patch boxes."""
         self._keep = [np.ascontiguousarray(a) for a in (
             scene.vertices, scene.indices, scene.surf_vbase, scene.surf_pbase, scene.obj_surface,
             scene.obj_local2world, scene.obj_world2local, scene.obj_material, scene.lights)]
@@ -334,7 +457,12 @@ class OracleScene:
                       _ptr(k[8]), k[8].shape[0], (_F32 * 3)(*scene.background), (_F32 * 3)(*scene.ambient),
                       scene.hash_seed)
         st = C.c_int(0)
-        self.h = lib().orc_scene_create(C.byref(d), C.byref(st))
+        if surface0_elements is None:
+            self.h = lib().orc_scene_create(C.byref(d), C.byref(st))
+        else:
+            e = np.ascontiguousarray(surface0_elements, np.float32).reshape(-1, 10)
+            self._keep.append(e)
+            self.h = lib().orc_scene_create_elements(C.byref(d), _ptr(e), C.byref(st))
         self.status = st.value
         if not self.h:
             raise RuntimeError(f"oracle scene build failed (status {st.value})")
@@ -380,6 +508,18 @@ class OracleScene:
         if any_hit:
             return obj.astype(bool), w.asdict()
         return hits, patch, obj, w.asdict()
+
+    def pick(self, rays, any_hit=False, threads=None):
+        """cg::PatchBVH's queries on surface 0 (orc_scene_pick): closest
+        (t,u,v (n,3), patch (n,)), or occlusion (n,) bool."""
+        rays = np.ascontiguousarray(rays, np.float32)
+        n = rays.shape[0]
+        hits = np.zeros((n, 3), np.float32)
+        patch = np.zeros(n, np.uint32)
+        occ = np.zeros(n, np.uint32)
+        th = threads or min(16, os.cpu_count() or 1)
+        lib().orc_scene_pick(self.h, _ptr(rays), n, 1 if any_hit else 0, _ptr(hits), _ptr(patch), _ptr(occ), th)
+        return occ.astype(bool) if any_hit else (hits, patch)
 
     def record_patch_tests(self, rays, cap=1 << 22):
         """Every patch test of the closest-hit traces of `rays`, in call order:

@@ -82,10 +82,16 @@ struct Scene
 // Builds every BVH and hash table (MainWindow.cpp:363-492 convertScene order:
 // per-surface BVHs, top-level BVH, then the hash tables).  Returns 0 or the
 // first failing hash build's code.
-inline int buildScene(Scene& s, uint32_t seed = 42)
+// surface0 (optional): surface 0's BVH elements, as cg::PatchBVH gives a
+// patch table's patches their boxes (PatchBVH.cpp:40-80).
+inline int buildScene(Scene& s, uint32_t seed = 42, const std::vector<Element>* surface0 = nullptr)
 {
-    for (auto& surf : s.surfaces)
-        surf.bvh = bvhBuild(patchElements(&surf.vertices[0].x, surf.indices.data(), surf.patchCount()), 1);
+    for (size_t k = 0; k < s.surfaces.size(); k++)
+    {
+        auto& surf = s.surfaces[k];
+        surf.bvh = k == 0 && surface0 ? bvhBuild(*surface0, 1)
+                                      : bvhBuild(patchElements(&surf.vertices[0].x, surf.indices.data(), surf.patchCount()), 1);
+    }
     // rt/Scene.cu:39-53 — objects bounded by their world-space control points.
     // An object without a surface (EMPTY: the reference's Mesh and Sphere,
     // whose bounds() are empty, Primitive.cu:15-23,47-56) keeps the empty box:

@@ -7,6 +7,7 @@
 //   facade_check hash   <in> <out>   PerfectHashFunction + BVH::buildHashTable
 //   facade_check render <in> <out>   Scene::buildBVH + RayTracer::render
 //   facade_check load_be <in> <out>  Surface::load_be on a `.be` model
+//   facade_check load_se <in> <out>  Surface::load_se on a `.se` surface
 //   facade_check errors              error behaviour without a device
 #include <cstdio>
 #include <cstring>
@@ -205,6 +206,23 @@ int runLoadBe(const char* inPath, const char* outPath)
     return 0;
 }
 
+// Surface::load_se on a `.se` surface: counts, points, the 16 leaf indices
+// per patch and the PatchBVH element boxes.
+int runLoadSe(const char* inPath, const char* outPath)
+{
+    std::ifstream in(inPath, std::ios::binary);
+    if (!in) throw std::runtime_error(std::string("cannot open ") + inPath);
+    const std::shared_ptr<Surface> s = Surface::load_se(in);
+    Writer w(outPath);
+    w.put((uint32_t)(s->points().size() / 4));
+    w.put(s->count());
+    w.vec(s->points());
+    w.vec(s->indices());
+    const auto el = s->elements();
+    w.vec(std::span<const float>((const float*)el.data(), el.size() * 10));
+    return 0;
+}
+
 // The per-call functions and PatchBVH the way a picking caller uses them:
 // a .be model, rays (n × 8 floats: origin, tMax, direction, pad); for each
 // ray: PatchBVH::intersect(ray, hit) (found, distance, u, v, patch),
@@ -306,6 +324,7 @@ int main(int argc, char** argv)
         if (mode == "hash" && argc == 4) return runHash(argv[2], argv[3]);
         if (mode == "render" && argc == 4) return runRender(argv[2], argv[3]);
         if (mode == "load_be" && argc == 4) return runLoadBe(argv[2], argv[3]);
+        if (mode == "load_se" && argc == 4) return runLoadSe(argv[2], argv[3]);
         if (mode == "errors") return runErrors();
         if (mode == "pick" && argc == 5) return runPick(argv[2], argv[3], argv[4]);
         std::fprintf(stderr, "usage: facade_check bvh|hash|render|load_be <in> <out> | pick <model> <rays> <out> | errors\n");

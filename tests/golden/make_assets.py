@@ -4,7 +4,9 @@ inside the scene files still resolve:
 
 * assets/bezier/{teapot,teacup,teaspoon,longwave}: the `.be` patch tables;
 * assets/scenes/*.dl: the scene description files that load them
-  (Teaset01, Teaset02, teaset, Test32 … Test152).
+  (Teaset01, Teaset02, teaset, Test32 … Test152);
+* assets/surfaces/*.se: the B-spline and Gregory element surfaces
+  (quad16, tube, ball0, dice16, cube16).
 
 These are data files (control points, scene descriptions), not source.
 assets.json records each file's byte size and SHA-256, and for the models the
@@ -22,6 +24,7 @@ import os
 SRC = "/root/reference/assets"
 HERE = os.path.dirname(os.path.abspath(__file__))
 MODELS = ("teapot", "teacup", "teaspoon", "longwave")
+SURFACES = ("quad16.se", "tube.se", "ball0.se", "dice16.se", "cube16.se")
 SCENES = ("Teaset01.dl", "Teaset02.dl", "teaset.dl", "Test32.dl", "Test56.dl", "Test80.dl", "Test104.dl",
           "Test128.dl", "Test152.dl")
 
@@ -54,6 +57,9 @@ def main():
     for n in SCENES:
         data = copy(f"scenes/{n}")
         meta[f"scenes/{n}"] = {"bytes": len(data), "sha256": hashlib.sha256(data).hexdigest()}
+    for n in SURFACES:
+        data = copy(f"surfaces/{n}")
+        meta[f"surfaces/{n}"] = {"bytes": len(data), "sha256": hashlib.sha256(data).hexdigest()}
     with open(os.path.join(HERE, "assets", "assets.json"), "w") as f:
         json.dump(meta, f, indent=1, sort_keys=True)
         f.write("\n")

@@ -65,6 +65,32 @@ def test_facade_load_be_equals_oracle(tmp_path, oracle, name):
     assert np.array_equal(v.view(np.uint32), ov.view(np.uint32)) and np.array_equal(ix, oi)
 
 
+@pytest.mark.parametrize("name", ["tube", "ball0", "dice16"])
+def test_facade_load_se_equals_oracle(tmp_path, oracle, name):
+    """Surface::load_se through the facade: points, the leaf test's 16 indices
+    per patch (16 zeros past the table's end) and PatchBVH's element boxes,
+    against the oracle's restatement."""
+    src = os.path.join(conftest.ROOT, "tests", "golden", "assets", "surfaces", name + ".se")
+    rc, out = run("load_se", src, tmp_path / "s.bin")
+    assert rc == 0, out
+    raw = (tmp_path / "s.bin").read_bytes()
+    nv, npch = np.frombuffer(raw[:8], np.uint32)
+    o = 8
+    v = np.frombuffer(raw[o:o + 16 * nv], np.float32).reshape(-1, 4)
+    o += 16 * nv
+    ix = np.frombuffer(raw[o:o + 64 * npch], np.uint32).reshape(-1, 16)
+    o += 64 * npch
+    el = np.frombuffer(raw[o:], np.float32).reshape(-1, 10)
+    with open(src, "rb") as f:
+        pts, idx, mat, refs = oracle.load_se(f.read())
+    want = np.zeros((refs.shape[0], 16), np.uint32)
+    for k, (off, _, _, _) in enumerate(refs):
+        if off + 16 <= idx.shape[0]:
+            want[k] = idx[off:off + 16]
+    assert np.array_equal(v.view(np.uint32), pts.view(np.uint32)) and np.array_equal(ix, want)
+    assert np.array_equal(el.view(np.uint32), oracle.se_elements(pts, idx, mat, refs).view(np.uint32))
+
+
 @pytest.mark.skipif(os.path.exists("/dev/kfd") and os.access("/dev/kfd", os.R_OK), reason="a device is present")
 def test_facade_error_behaviour_without_device():
     rc, out = run("errors")

@@ -129,3 +129,38 @@ def test_patchbvh_any(ctx, oracle):
     assert np.array_equal(g == 1, want)
     assert want.sum() > 600 and (~want).sum() > 100
     pb.destroy()
+
+
+SE_ASSETS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "assets", "surfaces")
+
+
+@pytest.mark.parametrize("name, min_hits", [("quad16", 0), ("tube", 1000), ("ball0", 1000), ("dice16", 2000),
+                                             ("cube16", 0)])
+def test_patchbvh_over_se_tables_matches_oracle(ctx, oracle, name, min_hits):
+    """cg::PatchBVH of the reference's `.se` surfaces: the tree over the boxes
+    of the basis-transformed points, the leaf test on each patch's 16 indices
+    (PatchBVH.cpp:40-121).  Closest and any picks equal the oracle's
+    PatchBVH queries on the same tree, bit for bit.  The flat quad and the
+    cube's faces have boxes of zero thickness, which the reference's strict
+    slab test never enters: no ray hits them, there as here."""
+    import cgspline_amd as cgs
+    from cgspline_amd import scenes
+    t = cgs.load_se(os.path.join(SE_ASSETS, name + ".se"))
+    el, pi = t.elements(), t.patch_indices()
+    pb = cgs.PatchBVH.from_table(ctx, t)
+    rays = _pick_rays(t.points, 3000, 75)
+    rays[1::3, 3] = 2.0
+    say("PatchBVH over", name, pi.shape[0], "patches,", rays.shape[0], "rays")
+    g_tuv, g_p = pb.intersect(rays)
+    sc = scenes.assemble([(t.points, pi)], [(0, scenes.IDENTITY16, scenes.IDENTITY16,
+                                             np.full(8, 0.5, np.float32))], np.zeros((0, 12), np.float32))
+    osc = oracle.OracleScene(sc, surface0_elements=el)
+    o_tuv, o_p = osc.pick(rays)
+    assert np.array_equal(g_p, o_p), np.flatnonzero(g_p != o_p)[:5]
+    hit = o_p != 0xFFFFFFFF
+    assert_bits_equal(g_tuv, o_tuv, "t,u,v")
+    occ = pb.intersect_any(rays)
+    o_occ = osc.pick(rays, any_hit=True)
+    assert not np.any(occ == 2) and np.array_equal(occ == 1, o_occ)
+    assert hit.sum() >= min_hits and (min_hits > 0 or hit.sum() == 0)
+    pb.destroy()
