@@ -647,6 +647,16 @@ class DeviceScene:
         return lib().cgs_render(self.ctx.h, self.h, C.byref(cam), C.byref(op), camera.width, camera.height,
                                 ptr(frame), ptr(color), ptr(counts), stream)
 
+    def render_sampled_dev(self, camera: "scenes.Camera", frame, color=None, counts=None, samples=8, seed=42, eps=1e-4,
+                           depth=0, stream=None, stride=0):
+        """cgs_render_sampled into device buffers: the Monte Carlo render with
+        `samples` camera rays per pixel, draws from `seed`."""
+        cam = _abi.Camera((C.c_float * 16)(*camera.to_world), (C.c_float * 3)(*camera.position), camera.fovy,
+                          int(camera.flip_y))
+        op = _abi.RenderOptions(eps, depth, stride)
+        return lib().cgs_render_sampled(self.ctx.h, self.h, C.byref(cam), C.byref(op), samples, seed, camera.width,
+                                        camera.height, ptr(frame), ptr(color), ptr(counts), stream)
+
     # host convenience wrappers
     def trace_closest(self, rays: np.ndarray, **options):
         """Closest hits of host rays (n, 8): returns (t,u,v (n,3), patch (n,), object (n,)).
@@ -699,13 +709,18 @@ class DeviceScene:
                 h.view(np.uint32)[:, 3].copy(), do.to_host(np.int32, (n,)),
                 dict(primary=int(cnt[0]), shadow=int(cnt[1]), secondary=int(cnt[2]), overflow=int(cnt[3])))
 
-    def render(self, camera: "scenes.Camera", eps=1e-4, depth=0):
-        """Returns (color (h,w,3) f32, frame (h,w) u32, counts dict)."""
+    def render(self, camera: "scenes.Camera", eps=1e-4, depth=0, samples=None, seed=42):
+        """Returns (color (h,w,3) f32, frame (h,w) u32, counts dict).
+        samples: None for the plain render, n for the sampled (Monte Carlo)
+        render with n camera rays per pixel and draws from `seed`."""
         w, h = camera.width, camera.height
         df = DeviceArray(self.ctx, 4 * w * h)
         dc = DeviceArray(self.ctx, 12 * w * h)
         dn = DeviceArray.from_host(self.ctx, np.zeros(4, np.uint64))
-        check(self.render_dev(camera, df, dc, dn, eps, depth), "cgs_render")
+        if samples is None:
+            check(self.render_dev(camera, df, dc, dn, eps, depth), "cgs_render")
+        else:
+            check(self.render_sampled_dev(camera, df, dc, dn, samples, seed, eps, depth), "cgs_render_sampled")
         self.ctx.sync()
         cnt = dn.to_host(np.uint64, (4,))
         return (dc.to_host(np.float32, (h, w, 3)), df.to_host(np.uint32, (h, w)),

@@ -39,7 +39,7 @@ EXPORTS = [
     "cgs_patch_table_patches", "cgs_patchbvh_create_elements",
     "cgs_scene_create", "cgs_scene_destroy", "cgs_scene_bvh_info", "cgs_scene_bvh_copy", "cgs_scene_culled",
     "cgs_trace_closest", "cgs_trace_any", "cgs_trace_closest_ex", "cgs_trace_any_ex", "cgs_render",
-    "cgs_render_running", "cgs_shade", "cgs_work_counts",
+    "cgs_render_running", "cgs_render_sampled", "cgs_shade", "cgs_work_counts",
     "cgs_trace_batch", "cgs_batch_create", "cgs_batch_launch", "cgs_batch_destroy", "cgs_batch_set_config",
     "cgs_batch_get_config", "cgs_batch_autotune", "cgs_batch_set_trace_options", "cgs_batch_get_trace_options",
     "cgs_batch_autotune_accuracy",
@@ -202,6 +202,8 @@ def lib():
             "cgs_render": ([_P, _P, C.POINTER(Camera), C.POINTER(RenderOptions), _U32, _U32, _P, _P, _P, _P],
                            C.c_int),
             "cgs_render_running": ([_P], C.c_int),
+            "cgs_render_sampled": ([_P, _P, C.POINTER(Camera), C.POINTER(RenderOptions), _U32, _U32, _U32, _U32, _P,
+                                    _P, _P, _P], C.c_int),
             "cgs_shade": ([_P, _P, _P, _U32, C.POINTER(RenderOptions), _P, _P, _P, _P, _P, _P], C.c_int),
             "cgs_work_counts": ([_P, _P, _U32, C.c_int], C.c_int),
             "cgs_trace_batch": ([_P, C.POINTER(BatchJob), _U32, _P], C.c_int),

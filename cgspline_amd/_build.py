@@ -14,7 +14,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libcgs.so")
-SOURCES = ["cgs_lib.cpp", "cgs_trace.hip", "cgs_phf.hip", "cgs_models.cpp", "cgs_dl.cpp", "cgs_cull.cpp"]
+SOURCES = ["cgs_lib.cpp", "cgs_trace.hip", "cgs_render_sampled.hip", "cgs_phf.hip", "cgs_models.cpp", "cgs_dl.cpp",
+           "cgs_cull.cpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++20", "-fPIC",

@@ -180,10 +180,16 @@ hipError_t launchShade(const SceneDev* one, const ShadeJob* jobs, uint32_t njobs
 constexpr size_t kRenderArgsBytes = 256;
 // `order` (rank -> tile) sets the order in which the waves take the tiles
 // (null: row by row); `tileTicks`, when set, receives each tile's time.
+// samples > 0: the sampled (Monte Carlo) render with that many camera rays
+// per pixel and draws from `seed`.
 hipError_t launchRender(const SceneDev& sc, const PersistArgs& pa, const float* toWorld16, const float* pos3, float tlx,
                         float tly, float hdx, float hdy, float eps, int depth, uint32_t w, uint32_t h, uint32_t* frame,
                         float* color, unsigned long long* counts, uint32_t stride, const uint32_t* order,
-                        uint32_t* tileTicks, void* argsDev, hipStream_t s);
+                        uint32_t* tileTicks, void* argsDev, hipStream_t s, uint32_t samples = 0, uint32_t seed = 0);
+// The sampled render's kernel (its own translation unit, cgs_render_sampled.hip:
+// in the plain render kernels' code object it raised their register spills
+// ninefold); argsDev as filled by launchRender.
+hipError_t launchRenderSampled(const SceneDev& sc, const PersistArgs& pa, const void* argsDev, hipStream_t s);
 // The render's work-event totals (a CGS_WORK_COUNTS=1 build; zeros
 // otherwise), kWorkEvents of them (cgs_device.hpp); cleared when `reset`.
 hipError_t workCounts(unsigned long long* out, bool reset);

@@ -1388,14 +1388,19 @@ cgs_status cgs_trace_any(cgs_ctx* ctx, cgs_scene* sc, const cgs_ray* rays, uint3
     return cgs_trace_any_ex(ctx, sc, rays, n, occluded, nullptr, stream);
 }
 
-cgs_status cgs_render(cgs_ctx* ctx, cgs_scene* sc, const cgs_camera* cam, const cgs_render_options* op, uint32_t w,
-                      uint32_t h, uint32_t* frame, float* color, cgs_ray_counts* counts, void* stream)
+namespace
 {
-    if (!ctx || !sc || !cam || !op || !frame) return fail(CGS_ERR_INVALID, "cgs_render: null argument");
-    if (sc->device != ctx->device) return fail(CGS_ERR_INVALID, "cgs_render: scene on another device");
-    if ((uint64_t)w * h >= (1ull << 32)) return fail(CGS_ERR_INVALID, "cgs_render: frame too large");
+// cgs_render and cgs_render_sampled: samples = 0 is the plain render.
+cgs_status renderFrame(const char* name, cgs_ctx* ctx, cgs_scene* sc, const cgs_camera* cam, const cgs_render_options* op,
+                       uint32_t samples, uint32_t seed, uint32_t w, uint32_t h, uint32_t* frame, float* color,
+                       cgs_ray_counts* counts, void* stream)
+{
+    const std::string fn(name);
+    if (!ctx || !sc || !cam || !op || !frame) return fail(CGS_ERR_INVALID, fn + ": null argument");
+    if (sc->device != ctx->device) return fail(CGS_ERR_INVALID, fn + ": scene on another device");
+    if ((uint64_t)w * h >= (1ull << 32)) return fail(CGS_ERR_INVALID, fn + ": frame too large");
     const uint32_t stride = op->frame_stride ? op->frame_stride : w;
-    if (stride < w) return fail(CGS_ERR_INVALID, "cgs_render: frame stride smaller than the width");
+    if (stride < w) return fail(CGS_ERR_INVALID, fn + ": frame stride smaller than the width");
     HIP_TRY(hipSetDevice(ctx->device));
     std::lock_guard lk(ctx->mu);
     if (ctx->renderLaunched)
@@ -1433,7 +1438,8 @@ cgs_status cgs_render(cgs_ctx* ctx, cgs_scene* sc, const cgs_camera* cam, const 
     const bool ordered = ts.ready && ts.w == w && ts.h == h;
     HIP_TRY(launchRender(sc->view(), pa, cam->to_world, cam->position, tlx, tly, hdx, hdy, op->eps, op->recursion_depth,
                          w, h, frame, color, (unsigned long long*)counts, stride,
-                         ordered ? ts.order.as<uint32_t>() : nullptr, ts.ticks.as<uint32_t>(), ctx->dRenderArgs, s));
+                         ordered ? ts.order.as<uint32_t>() : nullptr, ts.ticks.as<uint32_t>(), ctx->dRenderArgs, s,
+                         samples, seed));
     HIP_TRY(launchTileOrder(ts.ticks.as<uint32_t>(), nTiles, ts.order.as<uint32_t>(), s));
     ts.ready = true;
     ts.w = w;
@@ -1441,6 +1447,22 @@ cgs_status cgs_render(cgs_ctx* ctx, cgs_scene* sc, const cgs_camera* cam, const 
     HIP_TRY(hipEventRecord(ctx->renderDone, s));
     ctx->renderLaunched = true;
     return persistEnd(ctx, s);
+}
+} // namespace
+
+cgs_status cgs_render(cgs_ctx* ctx, cgs_scene* sc, const cgs_camera* cam, const cgs_render_options* op, uint32_t w,
+                      uint32_t h, uint32_t* frame, float* color, cgs_ray_counts* counts, void* stream)
+{
+    return renderFrame("cgs_render", ctx, sc, cam, op, 0, 0, w, h, frame, color, counts, stream);
+}
+
+cgs_status cgs_render_sampled(cgs_ctx* ctx, cgs_scene* sc, const cgs_camera* cam, const cgs_render_options* op,
+                              uint32_t samples, uint32_t seed, uint32_t w, uint32_t h, uint32_t* frame, float* color,
+                              cgs_ray_counts* counts, void* stream)
+{
+    if (samples == 0 || samples > CGS_MAX_SAMPLES)
+        return fail(CGS_ERR_INVALID, "cgs_render_sampled: samples must lie in 1.." + std::to_string(CGS_MAX_SAMPLES));
+    return renderFrame("cgs_render_sampled", ctx, sc, cam, op, samples, seed, w, h, frame, color, counts, stream);
 }
 
 cgs_status cgs_shade(cgs_ctx* ctx, cgs_scene* sc, const cgs_ray* rays, uint32_t n, const cgs_render_options* op,

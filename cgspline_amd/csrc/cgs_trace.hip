@@ -13,6 +13,7 @@
 
 #include "cgs_internal.hpp"
 #include "cgs_scene_dev.hpp"
+#include "cgs_sample.hpp"
 
 namespace cgs
 {
@@ -630,7 +631,11 @@ CGS_DEV bool sceneAny(const SceneDev& sc, const NodeCache& nc, const Rings& rg, 
     }, nc, rg.top);
 }
 
-// Device totals of the work events (CGS_WORK_COUNTS builds).
+// Device totals of the work events (CGS_WORK_COUNTS builds).  The sampled
+// render's translation unit keeps a copy of its own, which nothing reads.
+#if CGS_SAMPLED_TU
+static
+#endif
 __device__ unsigned long long g_workCounts[kWorkEvents];
 
 // Clears the workgroup's event slots (the caller then passes a barrier
@@ -838,6 +843,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CGS_TRACE_W
     addWave(overflowCount, nOverflow);
 }
 
+#if !CGS_SAMPLED_TU
 // Batched doBezierClipping: ray r against patch r (Spline_BezierClipping.cu:46-114).
 __global__ __launch_bounds__(256) void kClipBatch(Persist ps, const cgs_ray* __restrict__ rays,
                                                   const float4* __restrict__ patches, uint32_t n, float tol,
@@ -1019,6 +1025,7 @@ __global__ __launch_bounds__(256) void kSubpatch(const float4* __restrict__ patc
 #pragma unroll
     for (int i = 0; i < 16; i++) out[16 * (size_t)r + i] = p[i];
 }
+#endif // !CGS_SAMPLED_TU
 
 // ---------------------------------------------------------------------------
 // cg::PatchBVH (PatchBVH.h:13-28, PatchBVH.cpp:84-121): rays against the
@@ -1091,6 +1098,8 @@ struct RenderDev
     int depth;
     uint32_t width, height;
     uint32_t stride;      // frame row pitch in pixels (>= width)
+    uint32_t samples;     // camera rays per pixel of the sampled render (0: the plain render)
+    uint32_t seed;        // the sampled render's random seed
 };
 
 // A render's camera, options and outputs, read by kRender from device memory
@@ -1184,20 +1193,32 @@ CGS_DEV uint32_t packSRGB(float r, float g, float b)
 
 CGS_DEV float saturate(float x) { return x > 0.0f ? (x < 1.0f ? x : 1.0f) : 0.0f; }
 
+// One pixel's random stream in the sampled render: the seed, the pixel's cell
+// of the 256×256 grid and the draws taken so far.
+struct SampleState
+{
+    uint32_t seed, stream, draw;
+};
+
 // The radiance carried back along one ray: the render kernel's iterative
 // trace loop with Context::trace / miss / closestHit (RayTracer.cu:194-294,
-// 385-544, non-Monte-Carlo): closest hit, ambient + one shadow ray per light
-// with the PBR terms, then the reflection bounce while depth remains and the
-// attenuation exceeds 2^-8.  The first segment ends at rmax, bounces at
-// infinity.  eps is read from device memory per use (the traversal loops need
-// the registers a kernel argument would hold for the whole launch).  With
-// RECORD, the first segment's hit is returned in `first`.
-template<bool RECORD, bool BIG = true>
+// 385-544): closest hit, ambient + one shadow ray per light with the PBR
+// terms, then the reflection bounce while depth remains and the attenuation
+// exceeds 2^-8.  The first segment ends at rmax, bounces at infinity.  eps is
+// read from device memory per use (the traversal loops need the registers a
+// kernel argument would hold for the whole launch).  With RECORD, the first
+// segment's hit is returned in `first`.
+// SAMPLED: the Monte Carlo branches (RayTracer.cu:403-407,473-522): no ambient
+// term, and the bounce mirrors about a GGX microfacet normal drawn from `rs`
+// (microfacet(roughness², x, y), PBR.h:159-176) with weight mix(diffuse BRDF,
+// F(M·V) G(N·R, N·V) (M·V) / (N·V N·M), metalness); the result adds to `acc`,
+// the colour carried over from the pixel's earlier samples.
+template<bool RECORD, bool BIG = true, bool SAMPLED = false>
 CGS_DEV V3 rayRadiance(const SceneDev& sc, const NodeCache& nc, const Rings& rg, V3 ro, V3 rd, float rmax, int depth,
                        const float* eps, const ClipStack& stk, bool& overflow, uint32_t& nShadow,
-                       uint32_t& nSecondary, ClosestHit& first)
+                       uint32_t& nSecondary, ClosestHit& first, V3 acc = {0.0f, 0.0f, 0.0f},
+                       SampleState* rs = nullptr)
 {
-    V3 acc {0.0f, 0.0f, 0.0f};
     V3 att {1.0f, 1.0f, 1.0f};
     bool primary = true;
     while (true)
@@ -1217,7 +1238,7 @@ CGS_DEV V3 rayRadiance(const SceneDev& sc, const NodeCache& nc, const Rings& rg,
         const V3 spec {m1.x, m1.y, m1.z};
         const float invPi = 0x1.45f306p-2f; // std::numbers::inv_pi_v<float>
         const V3 diffuse = mul3(1.0f - metal, mul3(invPi, {m0.x, m0.y, m0.z}));
-        V3 col = mulv3({sc.ambient[0], sc.ambient[1], sc.ambient[2]}, diffuse);
+        V3 col = SAMPLED ? V3 {0.0f, 0.0f, 0.0f} : mulv3({sc.ambient[0], sc.ambient[1], sc.ambient[2]}, diffuse);
         V3 V = neg3(rd);
         const DevBVH surf = sc.surfaces[ob.surface];
         V3 N = versor3(normalMat(sc.patches + 16 * (size_t)(surf.patchBase + h.patch), h.u, h.v));
@@ -1250,11 +1271,38 @@ CGS_DEV V3 rayRadiance(const SceneDev& sc, const NodeCache& nc, const Rings& rg,
         }
         acc = add3(acc, mulv3(mul3(0x1.921fb6p+1f, col), att));
         if (depth <= 0) break;
-        V3 refl = schlick(spec, dotNV);
-        V3 R = sub3(neg3(V), mul3(2.0f * -dotNV, N));
-        float g = Gsmith(dotNV, dotNV, rough);
-        float rr = 1.0f - rough;
-        V3 brdf = mul3(rr * rr, mul3(dotNV, add3(diffuse, mul3(metal, mul3(g, refl)))));
+        V3 R, brdf;
+        if constexpr (SAMPLED)
+        {
+            float x, y;
+            drawPair(rs->seed, rs->stream, rs->draw, x, y);
+            rs->draw += 2;
+            const float alpha = rough * rough;
+            const float thetaM = atan2Quadrant(alpha * sqrtf(x), sqrtf(1.0f - x));
+            const float phiM = 0x1.921fb6p+2f * y;   // 2π y
+            const V3 ts = versor3(cross3(N, V));
+            const V3 tp = versor3(cross3(N, ts));
+            float sinT, cosT, sinP, cosP;
+            sinCosAngle(thetaM, sinT, cosT);
+            sinCosAngle(phiM, sinP, cosP);
+            const V3 M = versor3(add3(add3(mul3(sinT * cosP, ts), mul3(sinT * sinP, tp)), mul3(cosT, N)));
+            const float dotMV = dot3(M, V);
+            const V3 refl = schlick(spec, dotMV);
+            R = sub3(neg3(V), mul3(2.0f * -dotMV, M));
+            const float dotMN = dot3(M, N);
+            const float dotNR = dot3(N, R);
+            if (dotNR < 1e-14f) break;
+            const float g = Gsmith(dotNR, dotNV, rough) * dotMV / (dotNV * dotMN);
+            brdf = add3(mul3(1.0f - metal, mul3(invPi, {m0.x, m0.y, m0.z})), mul3(metal, mul3(g, refl)));
+        }
+        else
+        {
+            V3 refl = schlick(spec, dotNV);
+            R = sub3(neg3(V), mul3(2.0f * -dotNV, N));
+            float g = Gsmith(dotNV, dotNV, rough);
+            float rr = 1.0f - rough;
+            brdf = mul3(rr * rr, mul3(dotNV, add3(diffuse, mul3(metal, mul3(g, refl)))));
+        }
         V3 a = mulv3(att, brdf);
         if (max3(a) <= 0x1p-8f) break;
         ro = add3(Pt, mul3(*eps, R));
@@ -1273,7 +1321,10 @@ CGS_DEV V3 rayRadiance(const SceneDev& sc, const NodeCache& nc, const Rings& rg,
 // BIG: the scene has a BVH hashed with the wide family or a surface with a
 // cull tree (kernels for other scenes leave those paths out and keep their
 // registers).
-template<bool BIG>
+// SAMPLED: the Monte Carlo render (RayTracer.cu:212-246): op.samples camera
+// rays per pixel through (i + s/n, j + vdc(s)), one running colour over them,
+// scaled by 1/n; the pixel's stream is its cell of a 256×256 grid.
+template<bool BIG, bool SAMPLED = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CGS_RENDER_WAVES))) void kRender(SceneDev sc, Persist ps, const RenderArgs* ra)
 {
     __shared__ float4 topLds[4 * CGS_TOP_LDS_KEYS];
@@ -1305,13 +1356,38 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CGS_RENDER_
         const uint32_t i = (tile % tilesX) * 8 + (lane & 7), j = (tile / tilesX) * 8 + (lane >> 3);
         if (i >= op.width || j >= op.height) continue;
         bool overflow = false;
-        nPrimary++;
-        // pixelRayDirection (RayTracer.cu:59-71)
-        float4 P = make_float4(cam.tlx + cam.hdx * (2.0f * (float)i), cam.tly - cam.hdy * (2.0f * (float)j), -1.0f, 1.0f);
-        const V3 rd = versor3(sub3(project4(m4mul(C, P)), camPos));
         ClosestHit unused;
-        const V3 acc = rayRadiance<false, BIG>(sc, nc, rg, camPos, rd, __builtin_inff(), op.depth, &ra->op.eps, stk, overflow,
+        V3 acc;
+        if constexpr (SAMPLED)
+        {
+            SampleState rs {op.seed, (i & 255u) + 256u * (j & 255u), 0u};
+            const uint32_t n = op.samples;
+            const float nInv = 1.0f / (float)n;
+            acc = {0.0f, 0.0f, 0.0f};
+            for (uint32_t smp = 0; smp < n; smp++)
+            {
+                nPrimary++;
+                const CamDev cs = ra->cam;
+                M4 Cs;
+                Cs.c[0] = cs.c[0]; Cs.c[1] = cs.c[1]; Cs.c[2] = cs.c[2]; Cs.c[3] = cs.c[3];
+                const V3 pos {cs.px, cs.py, cs.pz};
+                const float x = (float)i + (float)smp * nInv, y = (float)j + vdc(smp);
+                float4 P = make_float4(cs.tlx + cs.hdx * (2.0f * x), cs.tly - cs.hdy * (2.0f * y), -1.0f, 1.0f);
+                const V3 rd = versor3(sub3(project4(m4mul(Cs, P)), pos));
+                acc = rayRadiance<false, BIG, true>(sc, nc, rg, pos, rd, __builtin_inff(), ra->op.depth, &ra->op.eps, stk,
+                                                    overflow, nShadow, nSecondary, unused, acc, &rs);
+            }
+            acc = mul3(nInv, acc);
+        }
+        else
+        {
+            nPrimary++;
+            // pixelRayDirection (RayTracer.cu:59-71)
+            float4 P = make_float4(cam.tlx + cam.hdx * (2.0f * (float)i), cam.tly - cam.hdy * (2.0f * (float)j), -1.0f, 1.0f);
+            const V3 rd = versor3(sub3(project4(m4mul(C, P)), camPos));
+            acc = rayRadiance<false, BIG>(sc, nc, rg, camPos, rd, __builtin_inff(), op.depth, &ra->op.eps, stk, overflow,
                                           nShadow, nSecondary, unused);
+        }
         const uint32_t q = j * ra->op.width + i;
         if (float* color = ra->color)
         {
@@ -1421,6 +1497,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(CGS_RENDER_
 
 } // namespace cgs
 
+#if CGS_SAMPLED_TU
+namespace cgs
+{
+hipError_t launchRenderSampled(const SceneDev& sc, const PersistArgs& pa, const void* argsDev, hipStream_t s)
+{
+    Persist ps {pa.stack, pa.lanes, pa.counter};
+    hipLaunchKernelGGL((kRender<true, true>), dim3(pa.grid), dim3(256), 0, s, sc, ps, (const RenderArgs*)argsDev);
+    return hipGetLastError();
+}
+} // namespace cgs
+#else
 // ---------------------------------------------------------------------------
 // Launchers (called by the host library)
 // ---------------------------------------------------------------------------
@@ -1592,7 +1679,7 @@ hipError_t launchPick(const SceneDev& sc, const PersistArgs& pa, const cgs_ray* 
 hipError_t launchRender(const SceneDev& sc, const PersistArgs& pa, const float* toWorld16, const float* pos3, float tlx,
                         float tly, float hdx, float hdy, float eps, int depth, uint32_t w, uint32_t h, uint32_t* frame,
                         float* color, unsigned long long* counts, uint32_t stride, const uint32_t* order,
-                        uint32_t* tileTicks, void* argsDev, hipStream_t s)
+                        uint32_t* tileTicks, void* argsDev, hipStream_t s, uint32_t samples, uint32_t seed)
 {
     CamDev cam;
     for (int c = 0; c < 4; c++)
@@ -1605,13 +1692,14 @@ hipError_t launchRender(const SceneDev& sc, const PersistArgs& pa, const float* 
     cam.hdx = hdx;
     cam.hdy = hdy;
     if ((uint64_t)w * h == 0) return hipSuccess;
-    const RenderArgs ra {cam, RenderDev {eps, depth, w, h, stride}, frame, color, counts, order, tileTicks};
+    const RenderArgs ra {cam, RenderDev {eps, depth, w, h, stride, samples, seed}, frame, color, counts, order, tileTicks};
     static_assert(sizeof(RenderArgs) <= kRenderArgsBytes, "render argument block");
     // `ra` is pageable (the stack): HIP performs such a copy synchronously,
     // so it has been read when the call returns.
     hipError_t e = hipMemcpyAsync(argsDev, &ra, sizeof ra, hipMemcpyHostToDevice, s);
     if (e != hipSuccess) return e;
     Persist ps {pa.stack, pa.lanes, pa.counter};
+    if (samples > 0) return launchRenderSampled(sc, pa, argsDev, s);
     if (sc.big)
         hipLaunchKernelGGL(kRender<true>, dim3(pa.grid), dim3(256), 0, s, sc, ps, (const RenderArgs*)argsDev);
     else
@@ -1673,3 +1761,4 @@ hipError_t workCounts(unsigned long long* out, bool reset)
 }
 
 } // namespace cgs
+#endif // CGS_SAMPLED_TU

@@ -571,6 +571,24 @@ cgs_status cgs_render(cgs_ctx* ctx, cgs_scene* scene, const cgs_camera* camera,
 /* RayTracer::running() (rt/RayTracer.h:45-48): 1 while the last render runs. */
 int cgs_render_running(cgs_ctx* ctx);
 
+/* The sampled (Monte Carlo) render: RayTracer::render built with
+ * USE_MONTECARLO_SAMPLING (rt/StaticConfig.h:5; RayTracer.cu:212-246 and the
+ * sampled branches of closestHit, RayTracer.cu:398-407,473-522), with
+ * RayTracer::Options::samples (rt/RayTracer.h:28, default 8).  Each pixel
+ * traces `samples` camera rays through (i + s/n, j + vdc(s)) and averages
+ * them; a hit has no ambient term and its reflection mirrors about a GGX
+ * microfacet normal drawn from the pixel's random stream (the cell
+ * (i mod 256, j mod 256) of a 256×256 grid, as the reference's).  The draws
+ * are Philox4x32-10 (Salmon et al., SC'11) under `seed` (the reference
+ * seeds 42): the same for the same seed, scene, camera and options, but not
+ * cuRAND's stream, so frames equal the reference's in distribution only.
+ * Arguments and errors as cgs_render; samples outside 1..CGS_MAX_SAMPLES is
+ * CGS_ERR_INVALID. */
+#define CGS_MAX_SAMPLES 65536u
+cgs_status cgs_render_sampled(cgs_ctx* ctx, cgs_scene* scene, const cgs_camera* camera,
+                              const cgs_render_options* options, uint32_t samples, uint32_t seed, uint32_t width,
+                              uint32_t height, uint32_t* frame, float* color, cgs_ray_counts* counts, void* stream);
+
 /* Shaded ray queries: each of n world rays is traced and shaded as the render
  * kernel traces and shades a camera ray (the trace loop of RayTracer.cu:
  * 194-294 with Context::trace / miss / closestHit, :385-544, non-Monte-Carlo;

@@ -493,7 +493,10 @@ struct Frame
 };
 
 // ---------------------------------------------------------------------------
-// rt::RayTracer (rt/RayTracer.h:13-72), non-Monte-Carlo path
+// rt::RayTracer (rt/RayTracer.h:13-72).  The reference chooses the Monte Carlo
+// render at compile time (USE_MONTECARLO_SAMPLING, rt/StaticConfig.h:5, off);
+// here Options::monteCarlo chooses it per tracer (cgs_render_sampled, with
+// Options::samples rays per pixel and draws from Options::seed).
 // ---------------------------------------------------------------------------
 class RayTracer
 {
@@ -502,10 +505,13 @@ public:
 
     struct Options
     {
+        int samples = 8;   // camera rays per pixel of the Monte Carlo render
         float eps = 1e-4f;
         bool flipYAxis = false;
         int recursionDepth = 6;
         int device = 0;
+        bool monteCarlo = false;
+        uint32_t seed = 42;
     };
 
     RayTracer() : RayTracer(Options {}) {}
@@ -521,8 +527,12 @@ public:
         c.fovy = camera->fieldOfView;
         c.flip_y = options.flipYAxis ? 1 : 0;
         cgs_render_options o {options.eps, options.recursionDepth, frame->stride};
-        cgs_status s = cgs_render(_dev.handle(), scene->handle(), &c, &o, frame->width, frame->height, frame->data,
-                                  nullptr, deviceCounts, stream);
+        cgs_status s = options.monteCarlo
+                           ? cgs_render_sampled(_dev.handle(), scene->handle(), &c, &o, (uint32_t)options.samples,
+                                                options.seed, frame->width, frame->height, frame->data, nullptr,
+                                                deviceCounts, stream)
+                           : cgs_render(_dev.handle(), scene->handle(), &c, &o, frame->width, frame->height,
+                                        frame->data, nullptr, deviceCounts, stream);
         if (s == CGS_ERR_BUSY)
         {
             std::fprintf(stderr, "CUDA ray tracer not ready: wait the previous work to finish or create a new instance\n");

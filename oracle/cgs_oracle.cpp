@@ -841,6 +841,91 @@ int orc_scene_shade(const orc_scene* h, const float* rays, uint64_t n, float eps
     return 0;
 }
 
+// The sampled render (USE_MONTECARLO_SAMPLING): rows [row0, row1) as
+// orc_scene_render, `samples` camera rays per pixel, draws from `seed`.
+int orc_scene_render_sampled(const orc_scene* h, const orc_camera* c, float eps, int depth, uint32_t samples,
+    uint32_t seed, uint32_t w, uint32_t hgt, float* color, uint32_t* frame, uint64_t* raycounts3, int threads,
+    uint32_t row0, uint32_t row1)
+{
+    if (samples == 0) return 1;
+    Camera cam {mat16(c->toWorld), {c->position[0], c->position[1], c->position[2]}, c->fovy, c->flipY != 0};
+    CameraSetup cs = setupCamera(cam, w, hgt);
+    RenderOptions op {eps, depth};
+    row1 = std::min(row1, hgt);
+    std::mutex mu;
+    RayCounts rc;
+    uint64_t npx = (uint64_t)w * (row1 - row0);
+    parallelFor(npx, threads, [&](uint64_t a, uint64_t b) {
+        RayCounts r;
+        for (uint64_t k = a; k < b; k++)
+        {
+            uint32_t i = (uint32_t)(k % w), j = row0 + (uint32_t)(k / w);
+            f3 col;
+            uint32_t px;
+            renderPixelSampled(h->s, cs, op, samples, seed, i, j, col, px, nullptr, &r);
+            size_t q = (size_t)j * w + i;
+            if (color) { color[3 * q] = col.x; color[3 * q + 1] = col.y; color[3 * q + 2] = col.z; }
+            if (frame) frame[q] = px;
+        }
+        std::lock_guard lk(mu);
+        rc.primary += r.primary; rc.shadow += r.shadow; rc.secondary += r.secondary;
+    });
+    if (raycounts3) { raycounts3[0] = rc.primary; raycounts3[1] = rc.shadow; raycounts3[2] = rc.secondary; }
+    return 0;
+}
+
+// The sampling pieces, for their pins: Philox4x32-10 blocks (4 counter words
+// and 2 key words per block in, 4 words out), draws as floats, vdc, the
+// angle functions, and one sampled bounce (N, V: 3 floats each; material:
+// diffuse3, specular3, metalness, roughness; out: R3, brdf3, 1 or 0).
+void orc_philox(const uint32_t* ctr4, const uint32_t* key2, uint64_t n, uint32_t* out4)
+{
+    for (uint64_t k = 0; k < n; k++) philox4x32_10(ctr4 + 4 * k, key2 + 2 * k, out4 + 4 * k);
+}
+void orc_sample_words(uint32_t seed, uint32_t stream, uint32_t first, uint64_t n, uint32_t* words, float* units)
+{
+    for (uint64_t k = 0; k < n; k++)
+    {
+        const uint32_t wv = sampleWord(seed, stream, first + (uint32_t)k);
+        if (words) words[k] = wv;
+        if (units) units[k] = unitFloat(wv);
+    }
+}
+void orc_vdc(const uint32_t* i, uint64_t n, float* out)
+{
+    for (uint64_t k = 0; k < n; k++) out[k] = vdc(i[k]);
+}
+void orc_sincos(const float* a, uint64_t n, float* s, float* c)
+{
+    for (uint64_t k = 0; k < n; k++) sinCosAngle(a[k], s[k], c[k]);
+}
+void orc_atan2q(const float* y, const float* x, uint64_t n, float* out)
+{
+    for (uint64_t k = 0; k < n; k++) out[k] = atan2Quadrant(y[k], x[k]);
+}
+void orc_sampled_bounce(const float* N3, const float* V3, const float* material8, const float* xy, uint64_t n,
+    float* out7)
+{
+    for (uint64_t k = 0; k < n; k++)
+    {
+        const float* Np = N3 + 3 * k;
+        const float* Vp = V3 + 3 * k;
+        const float* mp = material8 + 8 * k;
+        Material m;
+        m.diffuse = {mp[0], mp[1], mp[2]};
+        m.specular = {mp[3], mp[4], mp[5]};
+        m.metalness = mp[6];
+        m.roughness = mp[7];
+        const f3 N {Np[0], Np[1], Np[2]}, V {Vp[0], Vp[1], Vp[2]};
+        f3 R {0, 0, 0}, brdf {0, 0, 0};
+        const bool ok = sampledBounce(N, V, N.dot(V), m, xy[2 * k], xy[2 * k + 1], R, brdf);
+        float* o = out7 + 7 * k;
+        o[0] = R.x; o[1] = R.y; o[2] = R.z;
+        o[3] = brdf.x; o[4] = brdf.y; o[5] = brdf.z;
+        o[6] = ok ? 1.0f : 0.0f;
+    }
+}
+
 // CPURayTracer restatement (the CPU baseline): renders rows [row0,row1) with
 // `threads` workers over 32×32 tiles (rows row0, row0+stride, ...).
 // Returns wall seconds; ray counts out.

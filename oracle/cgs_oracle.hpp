@@ -34,6 +34,9 @@
 //   7 scene: an object's local t not rescaled by 1 / |M d|
 //   8 polish: the residual S - o + t d instead of S - o - t d
 //   9 polish: the u derivative without the quotient rule's -p w_u term
+//  10 sampled render: sample s at (i + s/n, j + s/n) instead of (i + s/n, j + vdc(s))
+//  11 sampled bounce: the microfacet distribution's alpha = roughness instead of roughness^2
+//  12 sampled bounce: the weight G without the factor (M.V) / (N.V N.M)
 //
 #pragma once
 

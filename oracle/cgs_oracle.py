@@ -112,6 +112,14 @@ def lib():
         L.orc_scene_trace_ex.argtypes = [_P, _P, _U64, C.c_int, _P, _P, _P, _P, C.c_int, C.POINTER(TraceOpts)]
         L.orc_pixel_rays.argtypes = [C.POINTER(CameraDesc), _U32, _U32, _P, _U64, _P]
         L.orc_scene_shade.argtypes = [_P, _P, _U64, _F32, C.c_int, _P, _P, _P, _P, _P, _P, C.c_int]
+        L.orc_scene_render_sampled.argtypes = [_P, C.POINTER(CameraDesc), _F32, C.c_int, _U32, _U32, _U32, _U32,
+                                               _P, _P, _P, C.c_int, _U32, _U32]
+        L.orc_philox.argtypes = [_P, _P, _U64, _P]
+        L.orc_sample_words.argtypes = [_U32, _U32, _U32, _U64, _P, _P]
+        L.orc_vdc.argtypes = [_P, _U64, _P]
+        L.orc_sincos.argtypes = [_P, _U64, _P, _P]
+        L.orc_atan2q.argtypes = [_P, _P, _U64, _P]
+        L.orc_sampled_bounce.argtypes = [_P, _P, _P, _P, _U64, _P]
         _lib = L
     return _lib
 
@@ -420,6 +428,58 @@ def se_elements(points, indices, matrices, patches):
     return out
 
 
+def philox(ctr, key):
+    """Philox4x32-10 blocks: ctr (n,4) u32, key (n,2) u32 -> (n,4) u32."""
+    ctr = np.ascontiguousarray(ctr, np.uint32).reshape(-1, 4)
+    key = np.ascontiguousarray(key, np.uint32).reshape(-1, 2)
+    out = np.zeros_like(ctr)
+    lib().orc_philox(_ptr(ctr), _ptr(key), ctr.shape[0], _ptr(out))
+    return out
+
+
+def sample_words(seed, stream, first, n):
+    """Draws first .. first+n-1 of a stream: (words u32, floats in (0,1))."""
+    words = np.zeros(n, np.uint32)
+    units = np.zeros(n, np.float32)
+    lib().orc_sample_words(seed, stream, first, n, _ptr(words), _ptr(units))
+    return words, units
+
+
+def vdc(i):
+    i = np.ascontiguousarray(i, np.uint32)
+    out = np.zeros(i.shape, np.float32)
+    lib().orc_vdc(_ptr(i), i.size, _ptr(out))
+    return out
+
+
+def sincos(a):
+    a = np.ascontiguousarray(a, np.float32)
+    s, c = np.zeros_like(a), np.zeros_like(a)
+    lib().orc_sincos(_ptr(a), a.size, _ptr(s), _ptr(c))
+    return s, c
+
+
+def atan2q(y, x):
+    y = np.ascontiguousarray(y, np.float32)
+    x = np.ascontiguousarray(x, np.float32)
+    out = np.zeros_like(y)
+    lib().orc_atan2q(_ptr(y), _ptr(x), y.size, _ptr(out))
+    return out
+
+
+def sampled_bounce(N, V, material, xy):
+    """One sampled reflection per row: N, V (n,3), material (n,8) diffuse3,
+    specular3, metalness, roughness; xy (n,2) draws -> (R (n,3), brdf (n,3),
+    continued (n,) bool)."""
+    N = np.ascontiguousarray(N, np.float32).reshape(-1, 3)
+    V = np.ascontiguousarray(V, np.float32).reshape(-1, 3)
+    material = np.ascontiguousarray(material, np.float32).reshape(-1, 8)
+    xy = np.ascontiguousarray(xy, np.float32).reshape(-1, 2)
+    out = np.zeros((N.shape[0], 7), np.float32)
+    lib().orc_sampled_bounce(_ptr(N), _ptr(V), _ptr(material), _ptr(xy), N.shape[0], _ptr(out))
+    return out[:, 0:3], out[:, 3:6], out[:, 6] == 1
+
+
 def phf_build_wide(keys, seed=42):
     """The wide family (exact arithmetic, 64-bit keys): (status, params, D)."""
     keys = np.ascontiguousarray(keys, np.uint64)
@@ -558,6 +618,22 @@ class OracleScene:
         lib().orc_scene_render(self.h, C.byref(c), eps, depth, w, h, _ptr(color), _ptr(frame), C.byref(wk),
                                _ptr(rc), th, r0, r1)
         return color, frame, wk.asdict(), dict(primary=int(rc[0]), shadow=int(rc[1]), secondary=int(rc[2]))
+
+    def render_sampled(self, cam, samples=8, seed=42, eps=1e-4, depth=0, rows=None, threads=None):
+        """The sampled (Monte Carlo) render: returns (color (h,w,3), frame
+        (h,w) u32, ray counts); rows outside `rows` stay zero."""
+        w, h = cam.width, cam.height
+        r0, r1 = (0, h) if rows is None else rows
+        color = np.zeros((h, w, 3), np.float32)
+        frame = np.zeros((h, w), np.uint32)
+        rc = np.zeros(3, np.uint64)
+        th = threads or min(16, os.cpu_count() or 1)
+        c = self._cam(cam)
+        st = lib().orc_scene_render_sampled(self.h, C.byref(c), eps, depth, samples, seed, w, h, _ptr(color),
+                                            _ptr(frame), _ptr(rc), th, r0, r1)
+        if st:
+            raise ValueError("orc_scene_render_sampled: samples must be positive")
+        return color, frame, dict(primary=int(rc[0]), shadow=int(rc[1]), secondary=int(rc[2]))
 
     def shade(self, rays, eps=1e-4, depth=0, threads=None):
         """The render's trace loop from given world rays (n,8): returns

@@ -16,6 +16,7 @@
 #include <thread>
 
 #include "cgs_oracle.hpp"
+#include "cgs_oracle_sample.hpp"
 
 namespace orc
 {
@@ -504,6 +505,47 @@ struct FirstHit
     int object = -1;
 };
 
+// The direct light of Context::closestHit (RayTracer.cu:398-466): from
+// `color`, for each light that is in range, faces the normal and whose shadow
+// ray reaches it, the PBR diffuse + specular terms.
+inline f3 directLight(const Scene& sc, const Material& m, f3 diffuse, f3 N, f3 V, float dotNV, f3 P, f3 color,
+    const RenderOptions& op, Work* w, RayCounts* rc)
+{
+    for (const Light& light : sc.lights)
+    {
+        float d;
+        f3 L;
+        if (!lightVector(d, L, P, light)) continue;
+        const float dotNL = N.dot(L);
+        if (dotNL < 1e-14f) continue;
+        Ray r1 {P + op.eps * L, L, d};
+        if (rc) rc->shadow++;
+        if (sceneIntersectAny(sc, r1, w)) continue;
+        f3 H = versor(L + V);
+        float dotHN = H.dot(N);
+        float dotHL = H.dot(L);
+        f3 I = lightColor(d, light);
+        color = color + (dotNL * I) * (diffuse + m.metalness * brdfSpecular(dotHL, dotHN, dotNV, dotNL, m.roughness, m.specular));
+    }
+    return color;
+}
+
+// The shading normal of closestHit (RayTracer.cu:398-431): the patch normal
+// to world space by the transposed world→local matrix, normalized, turned to
+// face V; dotNV = N·V >= 0.
+inline f3 shadingNormal(const Scene& sc, const Object& o, const Hit& hit, f3 V, float& dotNV)
+{
+    f3 N = surfaceNormal(sc.surfaces[o.surface], hit.index, hit.coordinates.x, hit.coordinates.y);
+    N = versor(M3<float>::from4(o.world2local).transposed() * N);
+    dotNV = N.dot(V);
+    if (dotNV < 0)
+    {
+        N = -N;
+        dotNV = -dotNV;
+    }
+    return N;
+}
+
 // render kernel and Context::trace/miss/closestHit (RayTracer.cu:194-294,385-544, non-Monte-Carlo path): the GPU renderer's
 // iterative trace loop from one ray (the render starts it at a camera ray with max = infinity).  Returns the
 // unclamped colour; `first` (nullable) receives the first segment's hit.
@@ -541,33 +583,11 @@ inline f3 shadeRay(const Scene& sc, const Ray& ray, const RenderOptions& op, Wor
         const Object& o = sc.objects[obj];
         const Material& m = o.material;
         const f3 diffuse = (1 - m.metalness) * brdfDiffuse(m);
-        f3 color = sc.ambient * diffuse;
         f3 V = -pl.ray.direction;
-        f3 N = surfaceNormal(sc.surfaces[o.surface], hit.index, hit.coordinates.x, hit.coordinates.y);
-        N = versor(M3<float>::from4(o.world2local).transposed() * N);
-        float dotNV = N.dot(V);
-        if (dotNV < 0)
-        {
-            N = -N;
-            dotNV = -dotNV;
-        }
+        float dotNV;
+        const f3 N = shadingNormal(sc, o, hit, V, dotNV);
         f3 P = pl.ray.origin + hit.t * pl.ray.direction;
-        for (const Light& light : sc.lights)
-        {
-            float d;
-            f3 L;
-            if (!lightVector(d, L, P, light)) continue;
-            const float dotNL = N.dot(L);
-            if (dotNL < 1e-14f) continue;
-            Ray r1 {P + op.eps * L, L, d};
-            if (rc) rc->shadow++;
-            if (sceneIntersectAny(sc, r1, w)) continue;
-            f3 H = versor(L + V);
-            float dotHN = H.dot(N);
-            float dotHL = H.dot(L);
-            f3 I = lightColor(d, light);
-            color = color + (dotNL * I) * (diffuse + m.metalness * brdfSpecular(dotHL, dotHN, dotNV, dotNL, m.roughness, m.specular));
-        }
+        const f3 color = directLight(sc, m, diffuse, N, V, dotNV, P, sc.ambient * diffuse, op, w, rc);
         pl.color = pl.color + ((CGS_ORACLE_PLANT == 5 ? 1.0f : std::numbers::pi_v<float>) * color) * pl.attenuation;
         if (pl.depth <= 0) break;
         f3 reflectance = schlick(m.specular, dotNV);
@@ -600,6 +620,117 @@ inline void renderPixel(const Scene& sc, const CameraSetup& cam, const RenderOpt
 {
     const Ray ray {cam.position, pixelRayDirection(cam, (float)i, (float)j), std::numeric_limits<float>::infinity()};
     colorOut = shadeRay(sc, ray, op, w, rc);
+    pixel = packColor(colorOut);
+}
+
+// ---------------------------------------------------------------------------
+// Monte Carlo render (USE_MONTECARLO_SAMPLING: RayTracer.cu:212-246 and the
+// sampled branches of closestHit, 403-407 and 473-522).  The draws, sample
+// positions and angle functions are this build's (cgs_oracle_sample.hpp).
+// ---------------------------------------------------------------------------
+
+// One pixel's random stream: the seed, the reference's grid cell and the
+// number of draws taken so far (the reference's copied curand state).
+struct SampleStream
+{
+    uint32_t seed, stream, draw;
+};
+
+// The sampled reflection of closestHit (RayTracer.cu:398-522): a microfacet
+// normal M from (x, y) by microfacet(roughness², x, y) (PBR.h:159-176) in the
+// frame t_s = N×V, t_p = N×t_s, N; R = V mirrored about M; the path ends where
+// N·R < 1e-14.  brdf = mix(diffuse BRDF, F(M·V) G(N·R, N·V) (M·V) / (N·V N·M),
+// metalness) (PBR.h:27-31).
+inline bool sampledBounce(f3 N, f3 V, float dotNV, const Material& m, float x, float y, f3& R, f3& brdf)
+{
+    const float alpha = CGS_ORACLE_PLANT == 11 ? m.roughness : m.roughness * m.roughness;
+    const float thetaM = atan2Quadrant(alpha * std::sqrt(x), std::sqrt(1 - x));
+    const float phiM = 2 * std::numbers::pi_v<float> * y;
+    const f3 ts = versor(N.cross(V));
+    const f3 tp = versor(N.cross(ts));
+    float sinT, cosT, sinP, cosP;
+    sinCosAngle(thetaM, sinT, cosT);
+    sinCosAngle(phiM, sinP, cosP);
+    f3 M = (sinT * cosP) * ts + (sinT * sinP) * tp + cosT * N;
+    M = versor(M);
+    const float dotMV = M.dot(V);
+    const f3 reflectance = schlick(m.specular, dotMV);
+    R = reflect(-V, M, -dotMV);
+    const float dotMN = M.dot(N);
+    const float dotNR = N.dot(R);
+    if (dotNR < 1e-14f) return false;
+    const float G = Gsmith(dotNR, dotNV, m.roughness);
+    const float g = CGS_ORACLE_PLANT == 12 ? G : G * dotMV / (dotNV * dotMN);
+    brdf = (1 - m.metalness) * brdfDiffuse(m) + m.metalness * (g * reflectance);
+    return true;
+}
+
+// The sampled trace loop from one ray (RayTracer.cu:232-246 with the sampled
+// closestHit): the payload colour `color` carries over from earlier samples
+// and takes each hit's direct light (no ambient term) and the background at a
+// miss, times the attenuation; while depth remains, a bounce takes two draws.
+inline void shadeRaySampled(const Scene& sc, const Ray& ray, const RenderOptions& op, SampleStream& rs, f3& color,
+    Work* w, RayCounts* rc)
+{
+    Ray r = ray;
+    f3 att = splat(1);
+    int depth = op.recursionDepth;
+    bool first = true;
+    while (true)
+    {
+        if (rc) (first ? rc->primary : rc->secondary)++;
+        first = false;
+        Hit hit;
+        hit.t = r.max;
+        uint32_t patch = EMPTY;
+        const int obj = sceneIntersect(sc, hit, r, &patch, w);
+        if (obj < 0)
+        {
+            color = color + sc.background * att;
+            break;
+        }
+        const Object& o = sc.objects[obj];
+        const Material& m = o.material;
+        const f3 diffuse = (1 - m.metalness) * brdfDiffuse(m);
+        const f3 V = -r.direction;
+        float dotNV;
+        const f3 N = shadingNormal(sc, o, hit, V, dotNV);
+        const f3 P = r.origin + hit.t * r.direction;
+        const f3 direct = directLight(sc, m, diffuse, N, V, dotNV, P, splat(0), op, w, rc);
+        color = color + (std::numbers::pi_v<float> * direct) * att;
+        if (depth <= 0) break;
+        const uint32_t m0 = rs.draw;
+        rs.draw += 2;
+        const float x = unitFloat(sampleWord(rs.seed, rs.stream, m0));
+        const float y = unitFloat(sampleWord(rs.seed, rs.stream, m0 + 1));
+        f3 R, brdf;
+        if (!sampledBounce(N, V, dotNV, m, x, y, R, brdf)) break;
+        const f3 a = att * brdf;
+        if (a.max() <= 0x1p-8f) break;
+        r = {P + op.eps * R, R, std::numeric_limits<float>::infinity()};
+        att = a;
+        depth = depth - 1;
+    }
+}
+
+// One pixel of the sampled render (RayTracer.cu:212-246): `samples` camera
+// rays through (i + s/n, j + vdc(s)), one running colour, scaled by 1/n; the
+// stream is the pixel's cell of the 256×256 grid (RayTracer.cu:213-218).
+inline void renderPixelSampled(const Scene& sc, const CameraSetup& cam, const RenderOptions& op, uint32_t samples,
+    uint32_t seed, uint32_t i, uint32_t j, f3& colorOut, uint32_t& pixel, Work* w, RayCounts* rc)
+{
+    SampleStream rs {seed, (i & 255u) + 256u * (j & 255u), 0u};
+    const float nInv = 1.0f / (float)samples;
+    f3 color = splat(0);
+    for (uint32_t s = 0; s < samples; s++)
+    {
+        const float s0 = (float)s * nInv;
+        const float s1 = CGS_ORACLE_PLANT == 10 ? s0 : vdc(s);
+        const Ray ray {cam.position, pixelRayDirection(cam, (float)i + s0, (float)j + s1),
+                       std::numeric_limits<float>::infinity()};
+        shadeRaySampled(sc, ray, op, rs, color, w, rc);
+    }
+    colorOut = nInv * color;
     pixel = packColor(colorOut);
 }
 

@@ -190,6 +190,16 @@ int runRender(const char* inPath, const char* outPath)
     t2.device().destroyStream(s2);
     w.vec(std::span<const uint32_t>(p1.toHost()));
     w.vec(std::span<const uint32_t>(p2.toHost()));
+
+    // The Monte Carlo render (the reference's USE_MONTECARLO_SAMPLING build)
+    // with two samples per pixel and the default seed.
+    rt::RayTracer mc({.samples = 2, .eps = eps, .flipYAxis = false, .recursionDepth = depth, .device = 0,
+                      .monteCarlo = true});
+    rt::DeviceArray<uint32_t> p3(mc.device(), (size_t)width * height);
+    rt::Frame f3 {p3.data(), width, height};
+    mc.render(&f3, &cam, &shared);
+    mc.device().synchronize();
+    w.vec(std::span<const uint32_t>(p3.toHost()));
     return 0;
 }
 

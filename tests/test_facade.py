@@ -150,12 +150,15 @@ def test_facade_render_matches_oracle(tmp_path, oracle, c1):
     rc, out = run("render", tmp_path / "in", tmp_path / "out")
     assert rc == 0, out
     # the tracer's own frame, then the frames of two RayTracers rendering one
-    # independently built scene at the same time on two streams
-    frames = np.fromfile(tmp_path / "out", np.uint32).reshape(3, cam.height, cam.width)
+    # independently built scene at the same time on two streams, then a Monte
+    # Carlo tracer's frame (2 samples, seed 42)
+    frames = np.fromfile(tmp_path / "out", np.uint32).reshape(4, cam.height, cam.width)
     _, oframe, _, _ = oracle.OracleScene(sc).render(cam, eps=wl.eps, depth=wl.depth)
     ch = lambda fr: np.stack([(fr >> s) & 0xFF for s in (0, 8, 16, 24)], -1).astype(np.int32)
     assert np.abs(ch(frames[0]) - ch(oframe)).max() <= 1
     assert np.array_equal(frames[1], frames[0]) and np.array_equal(frames[2], frames[0])
+    _, sframe, _ = oracle.OracleScene(sc).render_sampled(cam, samples=2, seed=42, eps=wl.eps, depth=wl.depth)
+    assert np.abs(ch(frames[3]) - ch(sframe)).max() <= 1 and not np.array_equal(frames[3], frames[0])
 
 
 @pytest.mark.gpu
