@@ -30,6 +30,9 @@ Prints one JSON line.  Besides the contract's fields:
   1e-4 of the FP64-polished reference, and its pass time.
 * `single_bvh`: the same 1M patches as one BVH (hashed with the wide family,
   patches found through a cull tree over its leaf boxes), same frame.
+* `sampled`: the Monte Carlo render of the same frame, 4 camera rays per
+  pixel with GGX-sampled reflections, and the oracle's sampled render on 8
+  rows beside it.
 * `serving`: 8 resident scenes (one of them this 1M-patch scene) under
   Poisson arrivals of 1k–4M-ray requests (40 % occlusion, 20 % shaded, the
   rest closest hits) at 30/60/90 % of capacity: p50/p99 latency and sustained
@@ -167,6 +170,9 @@ def main():
     ap.add_argument("--width", type=int, default=1920, help="--scene-file: frame width")
     ap.add_argument("--height", type=int, default=1080, help="--scene-file: frame height")
     ap.add_argument("--depth", type=int, default=6, help="--scene-file: recursion depth (reference default 6)")
+    ap.add_argument("--samples", type=int, default=0,
+                    help="time the sampled (Monte Carlo) render with this many camera rays per pixel instead of the "
+                         "plain render (no roofline, CPU baseline or extra legs)")
     ap.add_argument("--no-extras", dest="extras", action="store_false",
                     help="plain mode: skip the moving-camera, batched and serving sub-objects")
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
@@ -201,6 +207,10 @@ def main():
         wl = scenes.WORKLOADS[args.workload]()
         metric = "Mrays/s at 1M-patch scene (plain launch)"
         data = "synthetic (seeded displaced bicubic grids)"
+    if args.samples > 0:
+        metric = metric.replace("(plain launch)", f"(sampled render, {args.samples} samples per pixel)")
+        args.no_work_count = args.no_cpu_baseline = True
+        args.extras = False
     cam = wl.camera
     log(f"workload {args.workload}: {wl.scene.npatch} patches, {wl.scene.nobj} objects, "
         f"{cam.width}x{cam.height}, depth {wl.depth} (generated in {time.time() - t0:.1f}s)")
@@ -218,7 +228,11 @@ def main():
     counts = torch.zeros(4, dtype=torch.int64, device=dev)
 
     def step():
-        st = ds.render_dev(cam, frame, None, counts, eps=wl.eps, depth=wl.depth, stream=sh)
+        if args.samples > 0:
+            st = ds.render_sampled_dev(cam, frame, None, counts, samples=args.samples, seed=42, eps=wl.eps,
+                                       depth=wl.depth, stream=sh)
+        else:
+            st = ds.render_dev(cam, frame, None, counts, eps=wl.eps, depth=wl.depth, stream=sh)
         cgs.check(st, "cgs_render")
 
     # The scene's first render at this size takes the tiles row by row; every
@@ -300,6 +314,7 @@ def main():
             "width": cam.width,
             "height": cam.height,
             "recursion_depth": wl.depth,
+            **({"samples_per_pixel": args.samples, "seed": 42} if args.samples > 0 else {}),
             "rays_per_step": rays_per_step,
             "primary_shadow_secondary_per_step": [int(c[0]) // args.steps, int(c[1]) // args.steps,
                                                   int(c[2]) // args.steps],
@@ -437,9 +452,59 @@ def main():
         out["batched"] = batched_leg(ctx, args)
         out["serving"] = serving_leg(ctx, args, ds, wl)
         out["single_bvh"] = single_bvh_leg(ctx, args, frame, counts, stream)
+        out["sampled"] = sampled_leg(ds, wl, frame, counts, stream, args, osc, threads)
         if "batched" in out and out.get("value"):
             out["batched"]["batched_over_plain_render"] = round(out["batched"]["value"] / value, 3)
     print(json.dumps(out), flush=True)
+
+
+def sampled_leg(ds, wl, frame, counts, stream, args, osc, threads, samples=4):
+    """The Monte Carlo render (the reference's USE_MONTECARLO_SAMPLING build)
+    of the headline frame: `samples` camera rays per pixel, GGX-sampled
+    reflections, same scene, camera and depth; rays counted as the plain
+    render counts them.  Beside it, the oracle's sampled render on a few rows
+    of the same frame (the CPU restatement of the GPU path)."""
+    import torch
+    import cgspline_amd as cgs
+    sh = stream.cuda_stream
+
+    def render():
+        cgs.check(ds.render_sampled_dev(wl.camera, frame, None, counts, samples=samples, seed=42, eps=wl.eps,
+                                        depth=wl.depth, stream=sh), "cgs_render_sampled")
+
+    for _ in range(2):
+        render()
+        torch.cuda.synchronize()
+    counts.zero_()
+    torch.cuda.synchronize()
+    steps = max(3, min(args.steps, 6))
+    total = 0.0
+    for _ in range(steps):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        render()
+        b.record(stream)
+        b.synchronize()
+        total += a.elapsed_time(b)
+    ms = total / steps
+    cnt = counts.cpu().numpy().astype(np.int64)
+    rays = int(cnt[0] + cnt[1] + cnt[2]) // steps
+    res = {"value": round(rays / ms / 1e3, 3), "unit": "Mrays/s", "samples_per_pixel": samples, "seed": 42,
+           "ms_per_step": round(ms, 4), "rays_per_step": rays,
+           "primary_shadow_secondary_per_step": [int(c) // steps for c in cnt[:3]], "overflow": int(cnt[3])}
+    if osc is not None:
+        h = wl.camera.height
+        rows = (h // 2 - 4, h // 2 + 4)
+        t0 = time.perf_counter()
+        _, _, orc = osc.render_sampled(wl.camera, samples=samples, seed=42, eps=wl.eps, depth=wl.depth, rows=rows,
+                                       threads=threads)
+        sec = time.perf_counter() - t0
+        orays = orc["primary"] + orc["shadow"] + orc["secondary"]
+        res["cpu_oracle"] = {"value": round(orays / sec / 1e6, 3), "unit": "Mrays/s", "cores": threads,
+                             "sample": f"rows {rows[0]}-{rows[1] - 1} of the frame, {orays} rays, {sec:.2f}s",
+                             "kind": "port"}
+    log(f"sampled render: {res}")
+    return res
 
 
 def single_bvh_leg(ctx, args, frame, counts, stream):

@@ -79,6 +79,20 @@ def test_sampled_displaced_grid_rows_equal_oracle(ctx, oracle, c3_small):
     _compare(ctx, oracle, wl.scene, wl.camera, 2, 42, wl.eps, wl.depth, rows=(100, 116))
 
 
+def test_sampled_surface_cull_tree_equals_oracle(ctx, oracle, c3_small):
+    """A scene whose one surface has a cull tree (the displaced grids' 24,576
+    patches as one surface): the sampled kernel that carries the wide-hash and
+    surface-cull paths."""
+    from test_gpu_cull import one_surface
+    wl = c3_small
+    sc = one_surface(wl.scene)
+    ds = ctx.scene(sc)
+    assert ds.bvh_culled(0)
+    ds.destroy()
+    say("sampled render: one surface with a cull tree, rows 96-111, 2 samples")
+    _compare(ctx, oracle, sc, wl.camera, 2, 42, wl.eps, wl.depth, rows=(96, 112))
+
+
 def test_sampled_render_is_seeded(ctx, oracle, c1):
     from cgspline_amd import scenes
     cam = scenes.make_camera(*_view(c1.camera), 64, 48)
