@@ -81,7 +81,10 @@ def errors_against(ref, got, items, max_rel_t, max_uv):
         hita, hitb = pa < 0xFFFFFFFE, pb < 0xFFFFFFFE
         bad = (hita != hitb) | ((hita & hitb) & (oa != ob))
         both = hita & hitb & ~bad
-        dt = np.abs(tb[:, 0].astype(np.float64) - ta[:, 0]) / np.maximum(np.abs(ta[:, 0].astype(np.float64)), 1e-30)
+        # misses keep t = inf: compare distances only where both hit
+        t64a = np.where(both, ta[:, 0].astype(np.float64), 1.0)
+        t64b = np.where(both, tb[:, 0].astype(np.float64), 1.0)
+        dt = np.abs(t64b - t64a) / np.maximum(np.abs(t64a), 1e-30)
         seam = both & (pa != pb)
         bad |= seam & (dt > max_rel_t)
         same_p = both & (pa == pb)
