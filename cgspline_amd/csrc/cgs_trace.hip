@@ -394,6 +394,19 @@ CGS_DEV bool surfaceClosestCulled(const SceneDev& sc, const DevBVH& surf, const 
     return true;
 }
 
+// A shadow ray's own surface patch, clipped last.  An any-hit answer is the
+// OR of the patch tests the traversal reaches, in whatever order, so a query
+// may pass over one patch, note that the traversal reached it (`seen`), and
+// clip it only if no other patch is hit (originAny).  A shadow ray that
+// grazes its origin patch clips it for up to thousands of passes; one that
+// another patch occludes then never does.
+struct OriginSkip
+{
+    int32_t object;
+    uint32_t patch;
+    bool seen;
+};
+
 // rt::intersect(BezierSurface, ray) through the surface's cull tree: any
 // patch whose leaf box the ray passes below the local max and that has a
 // hit, in any order (the reference's any-hit walk never narrows its limit).
@@ -401,7 +414,8 @@ CGS_DEV bool surfaceClosestCulled(const SceneDev& sc, const DevBVH& surf, const 
 // reference's traversal.
 template<int METHOD>
 CGS_DEV bool surfaceAnyCulled(const SceneDev& sc, const DevBVH& surf, const LocalRay& lr, const ClipRay& cr,
-                              const ClipStack& stk, const TraceParams& tp, bool& occluded)
+                              const ClipStack& stk, const TraceParams& tp, bool& occluded, uint32_t skip = EMPTY,
+                              OriginSkip* origin = nullptr)
 {
     const V3 Dinv {1.0f / lr.d.x, 1.0f / lr.d.y, 1.0f / lr.d.z};
     if (!cullable(lr.o, Dinv)) return false;
@@ -412,6 +426,11 @@ CGS_DEV bool surfaceAnyCulled(const SceneDev& sc, const DevBVH& surf, const Loca
     CullIter it {surf.cullRoot, 0u, 0u};
     while ((pid = cullNext(sc, oct, lr.o, Dinv, lr.max, lr.max, it, L0)) != EMPTY)
     {
+        if (pid == skip)
+        {
+            origin->seen = true;
+            continue;
+        }
         const float4* S = sc.patches + 16 * (size_t)(surf.patchBase + pid);
         float best = lr.max, bu, bv;
         bool f;
@@ -482,7 +501,7 @@ CGS_DEV bool objectClosest(const SceneDev& sc, const Rings& rg, uint32_t i, V3 o
 // rt::intersect(BezierSurface, ray) (Primitive.cu:250-268) for object i.
 template<int METHOD, bool BIG = true>
 CGS_DEV bool objectAny(const SceneDev& sc, const Rings& rg, uint32_t i, V3 o, V3 d, float tmax, const ClipStack& stk,
-                       bool& overflow, const TraceParams& tp)
+                       bool& overflow, const TraceParams& tp, OriginSkip* origin = nullptr)
 {
     const DevObject& ob = sc.objects[i];
     if (ob.surface == EMPTY) return false;   // no geometry (empty box: only a NaN ray gets here)
@@ -491,12 +510,19 @@ CGS_DEV bool objectAny(const SceneDev& sc, const Rings& rg, uint32_t i, V3 o, V3
     const DevBVH surf = sc.surfaces[ob.surface];
     const BVHRef b = bvhRef(sc, surf);
     const ClipRay cr = makeClipRay(lr.o, versor3(lr.d));
+    const uint32_t skip = origin && (int32_t)i == origin->object ? origin->patch : EMPTY;
     if constexpr (BIG && METHOD != Subdiv)
     {
         bool occluded;
-        if (surf.cullRoot != EMPTY && surfaceAnyCulled<METHOD>(sc, surf, lr, cr, stk, tp, occluded)) return occluded;
+        if (surf.cullRoot != EMPTY && surfaceAnyCulled<METHOD>(sc, surf, lr, cr, stk, tp, occluded, skip, origin))
+            return occluded;
     }
     return hashIntersectAny<CGS_BOT_OCTANTS, false, BIG>(b, lr.o, lr.d, lr.max, [&](uint32_t pid) -> bool {
+        if (pid == skip)
+        {
+            origin->seen = true;
+            return false;
+        }
         const float4* S = sc.patches + 16 * (size_t)(surf.patchBase + pid);
         float best = lr.max, bu, bv;
         if constexpr (METHOD == Clip)
@@ -560,7 +586,7 @@ CGS_DEV ClosestHit closestCulled(const SceneDev& sc, const Rings& rg, V3 o, V3 d
 
 template<int METHOD, bool BIG = true>
 CGS_DEV bool anyCulled(const SceneDev& sc, const Rings& rg, V3 o, V3 d, V3 Dinv, float tmax, const ClipStack& stk,
-                       const TraceParams& tp, bool& exact)
+                       const TraceParams& tp, bool& exact, OriginSkip* origin = nullptr)
 {
     const int oct = dirOctant(Dinv);
     bool overflow = false;
@@ -569,7 +595,7 @@ CGS_DEV bool anyCulled(const SceneDev& sc, const Rings& rg, V3 o, V3 d, V3 Dinv,
     uint32_t i;
     while ((i = cullNext(sc, oct, o, Dinv, tmax, tmax, it, L0)) != EMPTY)
     {
-        const bool hit = objectAny<METHOD, BIG>(sc, rg, i, o, d, tmax, stk, overflow, tp);
+        const bool hit = objectAny<METHOD, BIG>(sc, rg, i, o, d, tmax, stk, overflow, tp, origin);
         if (overflow) break;
         if (hit) return true;
     }
@@ -610,10 +636,12 @@ CGS_DEV ClosestHit sceneClosest(const SceneDev& sc, const NodeCache& nc, const R
 }
 
 // Context::intersect(ray) (RayTracer.cu:352-383), through the cull tree
-// where it applies.
+// where it applies.  With `origin`, that patch is passed over and
+// origin->seen tells whether the traversal reached it (OriginSkip).
 template<int METHOD = Clip, bool BIG = true>
 CGS_DEV bool sceneAny(const SceneDev& sc, const NodeCache& nc, const Rings& rg, V3 o, V3 d, float tmax,
-                      const ClipStack& stk, bool& overflow, const TraceParams& tp = {kClipTol, 0x1p-4f, 0})
+                      const ClipStack& stk, bool& overflow, const TraceParams& tp = {kClipTol, 0x1p-4f, 0},
+                      OriginSkip* origin = nullptr)
 {
     if (sc.top.cullRoot != EMPTY)
     {
@@ -621,14 +649,29 @@ CGS_DEV bool sceneAny(const SceneDev& sc, const NodeCache& nc, const Rings& rg, 
         if (cullable(o, Dinv))
         {
             bool exact = false;
-            if (anyCulled<METHOD, BIG>(sc, rg, o, d, Dinv, tmax, stk, tp, exact)) return true;
+            if (anyCulled<METHOD, BIG>(sc, rg, o, d, Dinv, tmax, stk, tp, exact, origin)) return true;
             if (!exact) return false;
+            if (origin) origin->seen = false;   // the traversal below decides
         }
     }
     const BVHRef top = bvhRef(sc, sc.top);
     return hashIntersectAny<true, true, BIG>(top, o, d, tmax, [&](uint32_t i) -> bool {
-        return objectAny<METHOD, BIG>(sc, rg, i, o, d, tmax, stk, overflow, tp);
+        return objectAny<METHOD, BIG>(sc, rg, i, o, d, tmax, stk, overflow, tp, origin);
     }, nc, rg.top);
+}
+
+// The test of a shadow ray's origin patch that sceneAny passed over: object
+// i's local ray and patch test, as objectAny's leaf test.
+CGS_DEV bool originAny(const SceneDev& sc, const OriginSkip& origin, V3 o, V3 d, float tmax, const ClipStack& stk,
+                       bool& overflow)
+{
+    const DevObject& ob = sc.objects[origin.object];
+    const LocalRay lr = toLocal(ob, o, d, tmax);
+    const DevBVH surf = sc.surfaces[ob.surface];
+    const ClipRay cr = makeClipRay(lr.o, versor3(lr.d));
+    const float4* S = sc.patches + 16 * (size_t)(surf.patchBase + origin.patch);
+    float best = lr.max, bu, bv;
+    return clipPatch<true>(S, cr, kClipTol, best, bu, bv, stk, overflow);
 }
 
 // Device totals of the work events (CGS_WORK_COUNTS builds).  The sampled
@@ -1261,7 +1304,11 @@ CGS_DEV V3 rayRadiance(const SceneDev& sc, const NodeCache& nc, const Rings& rg,
             const float dotNL = dot3(N, L);
             if (dotNL < 1e-14f) continue;
             nShadow++;
-            if (sceneAny<Clip, BIG>(sc, nc, rg, add3(Pt, mul3(*eps, L)), L, d, stk, overflow)) continue;
+            // the hit's own patch last (OriginSkip): the answer is the same
+            OriginSkip origin {h.object, h.patch, false};
+            const V3 so = add3(Pt, mul3(*eps, L));
+            if (sceneAny<Clip, BIG>(sc, nc, rg, so, L, d, stk, overflow, {kClipTol, 0x1p-4f, 0}, &origin)) continue;
+            if (origin.seen && originAny(sc, origin, so, L, d, stk, overflow)) continue;
             V3 H = versor3(add3(L, V));
             float dotHN = dot3(H, N);
             float dotHL = dot3(H, L);
