@@ -742,7 +742,7 @@ CGS_DEV uint32_t nextChunk(unsigned int* counter)
 }
 
 #ifndef CGS_RENDER_WAVES
-#define CGS_RENDER_WAVES 3
+#define CGS_RENDER_WAVES 2
 #endif
 #ifndef CGS_TRACE_WAVES
 #define CGS_TRACE_WAVES 3
@@ -1362,9 +1362,12 @@ CGS_DEV V3 rayRadiance(const SceneDev& sc, const NodeCache& nc, const Rings& rg,
 }
 
 // Occupancy target of the render megakernel (waves per SIMD): the compiler
-// keeps VGPR + AGPR use within 512 / CGS_RENDER_WAVES.  On the 1M-patch frame
-// three waves per SIMD (168 registers) measured 39.5 ms, two waves 44.5 ms
-// and four waves 46 ms.
+// keeps VGPR + AGPR use within 512 / CGS_RENDER_WAVES.  With shadow rays
+// clipping their origin patch last, two waves per SIMD (256 registers)
+// render the 1M-patch frame in 9.0 ms against 9.4 ms with three (one wave:
+// the same as two) and Test152 in 114 ms against 129 ms; the sampled render
+// is the same either way.  (Before that change, when waves waited on
+// grazing origin clips, three waves measured best.)
 // BIG: the scene has a BVH hashed with the wide family or a surface with a
 // cull tree (kernels for other scenes leave those paths out and keep their
 // registers).
