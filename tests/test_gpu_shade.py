@@ -171,3 +171,50 @@ def test_served_shaded_requests_equal_shade_alone(ctx, oracle, teaset, c2_small)
     want = oracle.OracleScene([teaset.scene, c2_small.scene][s]).shade(rays, eps=1e-4, depth=depth)
     color = ds[s].shade(rays, eps=1e-4, depth=depth)[0]
     assert_bits_equal(color, want[0], "served vs oracle")
+
+
+def trough_scene():
+    """One bicubic patch bent into a trough along y (x, z from the curve
+    (-1, 1), (-1, -1), (1, -1), (1, 1)), lit from beyond its right rim: the
+    shadow rays of the floor and left wall pass through the right wall, the
+    same patch they start on."""
+    from cgspline_amd import scenes
+    xz = [(-1.0, 1.0), (-1.0, -1.0), (1.0, -1.0), (1.0, 1.0)]
+    v = np.zeros((16, 4), np.float32)
+    for j in range(4):
+        for i in range(4):
+            v[4 * j + i] = (xz[i][0], -1.0 + 2.0 * j / 3.0, xz[i][1], 1.0)
+    light = scenes.point_light((3.0, 0.2, 2.0), (6.0, 6.0, 6.0))
+    mat = np.array([0.7, 0.6, 0.5, 0.04, 0.04, 0.04, 0.2, 0.4], np.float32)
+    return scenes.assemble([(v, np.arange(16, dtype=np.uint32)[None])],
+                           [(0, scenes.IDENTITY16, scenes.IDENTITY16, mat)], [light], name="trough")
+
+
+def test_self_shadowed_patch_renders_as_the_oracle(ctx, oracle):
+    """Shadow rays occluded by nothing but the patch they start on: the
+    render clips a shadow ray's own patch after the rest of the scene, and
+    with one patch in the scene that clip alone decides.  The oracle (its
+    traversal order) shows such rays exist; the frame and counts equal it."""
+    from cgspline_amd import scenes
+    sc = trough_scene()
+    cam = scenes.make_camera((0.3, -2.6, 3.2), (0.0, 0.0, -0.3), (0.0, 0.0, 1.0), 45.0, 96, 72)
+    osc = oracle.OracleScene(sc)
+    rays = scenes.camera_rays(cam)
+    tuv, patch, obj, _ = osc.trace(rays)
+    hit = obj >= 0
+    P = rays[hit, 0:3].astype(np.float64) + tuv[hit, 0:1] * rays[hit, 4:7]
+    L = np.array([3.0, 0.2, 2.0]) - P
+    d = np.linalg.norm(L, axis=1)
+    L /= d[:, None]
+    sh = np.zeros((P.shape[0], 8), np.float32)
+    sh[:, 0:3], sh[:, 3], sh[:, 4:7] = P + 1e-4 * L, d, L
+    occ, _ = osc.trace(sh, any_hit=True)
+    say("trough:", int(hit.sum()), "hits,", int(occ.sum()), "self-occluded shadow rays")
+    assert hit.sum() > 1000 and 100 < occ.sum() < hit.sum()
+    ds = ctx.scene(sc)
+    color, frame, counts = ds.render(cam, eps=1e-4, depth=1)
+    ocolor, oframe, _, orc = osc.render(cam, eps=1e-4, depth=1)
+    assert counts["overflow"] == 0
+    assert (counts["primary"], counts["shadow"], counts["secondary"]) == (orc["primary"], orc["shadow"], orc["secondary"])
+    assert_bits_equal(color.reshape(-1, 3), ocolor.reshape(-1, 3), "color")
+    ds.destroy()
