@@ -2,8 +2,9 @@
 the oracle's sampled render, bit for bit in the colours and the ray counts:
 the flat square of the closed-form pins, the height field, the closed
 surfaces, the reference's teaset and Test152 scenes at recursion depth 6,
-and rows of the displaced-grid frame; the same seed gives the same frame,
-another seed another; the sample count is checked."""
+and rows of the displaced-grid frame, at depth 0 too; padded frames; the
+same seed gives the same frame, another seed another; the sample count is
+checked."""
 from __future__ import annotations
 
 import os
@@ -91,6 +92,36 @@ def test_sampled_surface_cull_tree_equals_oracle(ctx, oracle, c3_small):
     ds.destroy()
     say("sampled render: one surface with a cull tree, rows 96-111, 2 samples")
     _compare(ctx, oracle, sc, wl.camera, 2, 42, wl.eps, wl.depth, rows=(96, 112))
+
+
+def test_sampled_flat_square_without_bounce_equals_oracle(ctx, oracle):
+    """Depth 0: each sample is its hit's direct light without the ambient
+    term, or the background; no draws are taken."""
+    from cgspline_amd import scenes
+    from test_oracle_render import flat_square_scene
+    cam = scenes.make_camera((0.4, -1.8, 2.2), (0.0, 0.1, 0.0), (0.0, 0.0, 1.0), 50.0, 48, 40)
+    say("sampled render: flat square 48x40, 5 samples, depth 0")
+    _, orc = _compare(ctx, oracle, flat_square_scene(), cam, 5, 7, 1e-4, 0)
+    assert orc["secondary"] == 0 and orc["primary"] == 48 * 40 * 5
+
+
+def test_sampled_render_into_padded_frame(ctx, c1):
+    """A frame of rows padded to a stride: the rows equal the dense sampled
+    render's, the padding is left as it was."""
+    import cgspline_amd as cgs
+    from cgspline_amd import scenes
+    cam = scenes.make_camera(*_view(c1.camera), 64, 40)
+    w, h, stride = cam.width, cam.height, cam.width + 13
+    ds = ctx.scene(c1.scene)
+    _, dense, _ = ds.render(cam, eps=c1.eps, depth=2, samples=3, seed=5)
+    df = cgs.DeviceArray.from_host(ctx, np.full(stride * h, 0x5A5A5A5A, np.uint32))
+    cgs.check(ds.render_sampled_dev(cam, df, samples=3, seed=5, eps=c1.eps, depth=2, stride=stride),
+              "cgs_render_sampled")
+    ctx.sync()
+    out = df.to_host(np.uint32, (h, stride))
+    assert np.array_equal(out[:, :w], dense.reshape(h, w))
+    assert np.all(out[:, w:] == 0x5A5A5A5A)
+    ds.destroy()
 
 
 def test_sampled_render_is_seeded(ctx, oracle, c1):
