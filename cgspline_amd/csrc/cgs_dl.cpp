@@ -757,7 +757,8 @@ private:
     }
 
     // SceneReader.cpp:215-242 and Surface::load (Surface.cpp:350-373): a
-    // name ending in ".se" is a `.se` surface, anything else a `.be` model.
+    // name ending in ".se" is a `.se` surface (cgs_load_se), anything else a
+    // `.be` model.
     Value createSurface(const List& a)
     {
         if (a.empty()) fail("Surface: expected a dictionary");
@@ -771,13 +772,33 @@ private:
         const std::string path = rel.size() && rel[0] == '/' ? rel : (_dir.empty() ? rel : _dir + "/" + rel);
         std::string ext = rel.size() >= 3 ? rel.substr(rel.size() - 3) : "";
         for (char& c : ext) c = char(std::tolower((unsigned char)c));
-        if (ext == ".se") fail("Surface: `.se` surfaces are not supported ('" + rel + "')");
         FILE* fp = std::fopen(path.c_str(), "rb");
         if (!fp) fail("file '" + rel + "' does not exist");
         std::string text;
         char buf[65536];
         for (size_t got; (got = std::fread(buf, 1, sizeof buf, fp)) > 0;) text.append(buf, got);
         std::fclose(fp);
+        if (ext == ".se")
+        {
+            // MainWindow::convertScene (MainWindow.cpp:464-483) copies the
+            // surface's points and its whole grouped index list into a
+            // BezierSurface, which reads indexCount / 16 patches of 16
+            // consecutive indices (Primitive.h:110): the elements' own node
+            // counts and basis matrices play no part in the render.
+            cgs_patch_table* t = nullptr;
+            if (cgs_load_se(text.data(), text.size(), &t) != CGS_OK) fail("'" + rel + "': " + cgs_last_error());
+            SurfaceAsset s;
+            s.name = name;
+            s.vertices.assign(t->points, t->points + 4 * size_t(t->npoints));
+            const size_t whole = size_t(t->nindices) / 16 * 16;
+            s.indices.assign(t->indices, t->indices + whole);
+            cgs_patch_table_free(t);
+            if (whole == 0) fail("'" + rel + "': fewer than 16 node indices, no patch");
+            surfaces.push_back(std::move(s));
+            const Value r = Value::ofAsset(Value::SurfaceK, int(surfaces.size()) - 1);
+            insertAsset(name, r);
+            return r;
+        }
         float* v = nullptr;
         uint32_t* ix = nullptr;
         uint32_t nv = 0, np = 0;

@@ -118,3 +118,32 @@ def test_test152_benchmark_frame_matches_oracle(ctx, oracle):
             (orc["primary"], orc["shadow"], orc["secondary"])
         assert_bits_equal(color.reshape(-1, 3), ocolor.reshape(-1, 3), f"color, pass {k}")
         assert np.abs(ch(frame) - ch(oframe)).max() <= 1
+
+
+@pytest.mark.parametrize("name", ["tube", "dice16"])
+def test_se_surface_scene_render_matches_oracle(ctx, oracle, name):
+    """A `.se` surface read through a scene file, as the reference's renderer
+    receives it (its grouped index list as patches of 16 consecutive
+    indices), lit by one point light: BVH, hits and frame equal the
+    oracle's."""
+    import cgspline_amd as cgs
+    from cgspline_amd import scenes
+    se_dir = os.path.join(ASSETS, "surfaces")
+    text = f'Surface {{ name:"s" file:"{name}.se" }}\nScene {{ objects:[ {{ surface:{{ model:Get("s") }} }} ] }}'
+    d = cgs.read_dl(text.encode(), name="se.dl", working_dir=se_dir)
+    v, ix = d.scene.vertices, d.scene.indices
+    c = v[:, :3].astype(np.float64).mean(axis=0)
+    r = float(np.linalg.norm(v[:, :3] - c, axis=1).max())
+    mat = np.array([0.7, 0.6, 0.5, 0.04, 0.04, 0.04, 0.3, 0.4], np.float32)
+    sc = scenes.assemble([(v, ix)], [(0, scenes.IDENTITY16, scenes.IDENTITY16, mat)],
+                         [scenes.point_light(c + 3 * r * np.array([0.3, -0.5, 0.8]), (8.0, 8.0, 8.0))], name=name)
+    cam = scenes.make_camera(c + 2.5 * r * np.array([0.4, -0.8, 0.45]), c, (0.0, 0.0, 1.0), 50.0, 96, 72)
+    say("se surface scene render", name, ix.shape[0], "patches")
+    ds = ctx.scene(sc)
+    color, frame, counts = ds.render(cam, eps=1e-4, depth=2)
+    ocolor, oframe, _, orc = oracle.OracleScene(sc).render(cam, eps=1e-4, depth=2)
+    assert counts["overflow"] == 0
+    assert (counts["primary"], counts["shadow"], counts["secondary"]) == (orc["primary"], orc["shadow"], orc["secondary"])
+    assert_bits_equal(color.reshape(-1, 3), ocolor.reshape(-1, 3), "color")
+    assert orc["shadow"] > 0
+    ds.destroy()

@@ -168,7 +168,7 @@ def test_lights():
     ('Material { refractive_index:0 }', "'refractive_index' (index of refraction) should be greater than zero"),
     ('Surface { name:"x" }', "surface data or source not provided"),
     ('Surface { file:"nope" }', "file 'nope' does not exist"),
-    ('Surface { file:"x.se" }', "`.se` surfaces are not supported"),
+    ('Surface { file:"x.se" }', "file 'x.se' does not exist"),
     ('Mesh { file:"box.obj" }', "not yet implemented"),
     ('Mesh { vertices:[0 0 0 1 0 0] normals:[0 0 1] }', "`normals` size must match `vertices` size"),
     ('Mesh { vertices:[0 0 0 1 0 0 0 1 0] triangles:[0 1 2.0] }', "triangle indices must be integers"),
@@ -260,3 +260,26 @@ def test_read_missing_file():
     import cgspline_amd as cgs
     with pytest.raises(cgs.CgsError, match="cannot open"):
         cgs.read_dl(os.path.join(ASSETS, "scenes", "missing.dl"))
+
+
+SE_DIR = os.path.join(ASSETS, "surfaces")
+
+
+@pytest.mark.parametrize("name", ["tube", "ball0", "dice16"])
+def test_se_surface_as_the_renderer_reads_it(oracle, name):
+    """A `.se` surface named in a scene file (Surface::load by extension,
+    Surface.cpp:350-373) reaches the renderer as MainWindow::convertScene
+    hands it over (MainWindow.cpp:464-483): its points, and its whole grouped
+    index list read as patches of 16 consecutive indices, indexCount / 16 of
+    them (Primitive.h:110)."""
+    import cgspline_amd as cgs
+    text = f'Surface {{ name:"s" file:"{name}.se" }}\nScene {{ objects:[ {{ surface:{{ model:Get("s") }} }} ] }}'
+    d = cgs.read_dl(text.encode(), name="se.dl", working_dir=SE_DIR)
+    with open(os.path.join(SE_DIR, name + ".se"), "rb") as f:
+        pts, idx, _, _ = oracle.load_se(f.read())
+    sc = d.scene
+    assert sc.nsurf == 1 and sc.nobj == 1
+    assert np.array_equal(sc.vertices.view(np.uint32), pts.view(np.uint32))
+    whole = idx.shape[0] // 16 * 16
+    assert np.array_equal(sc.indices.reshape(-1), idx[:whole]) and sc.indices.shape[0] == idx.shape[0] // 16
+    assert np.all(sc.indices < pts.shape[0])
