@@ -780,7 +780,7 @@ private:
         std::fclose(fp);
         if (ext == ".se")
         {
-            // MainWindow::convertScene (MainWindow.cpp:464-483) copies the
+            // MainWindow::convertScene (MainWindow.cpp:363-492) copies the
             // surface's points and its whole grouped index list into a
             // BezierSurface, which reads indexCount / 16 patches of 16
             // consecutive indices (Primitive.h:110): the elements' own node
